@@ -1,0 +1,154 @@
+"""Generates hankel_tables.h: the coefficient tables of the device H0^(1)/H1^(1) of real argument.
+
+    python bem2d_amd/csrc/gen_hankel_tables.py      (needs mpmath; the header is committed)
+
+Representation (x > 0):
+  x < 2      ascending series in t = x^2/4 (A&S 9.1.10, 9.1.11 with n = 0, 1):
+               J0 = S_J0(t),  J1 = (x/2) S_J1(t),
+               Y0 = (2/pi) ln(x/2) J0 + S_Y0(t),
+               Y1 = (2/pi) ln(x/2) J1 - 2/(pi x) + (x/2) S_Y1(t)
+  x >= 2     modulus/phase form of A&S 9.2.5-9.2.6:
+               H_nu(x) = sqrt(2/(pi x)) (P_nu(x) + i Q_nu(x)) exp(i (x - (2 nu + 1) pi/4)),
+             with P_nu and x*Q_nu fitted per piece [a, b) as polynomials of degree DEG in
+             z = (y - ym)/yh in [-1, 1], y = (a/x)^2 (the last piece reaches x = infinity, y -> 0).
+The fits are Chebyshev interpolants computed at 40 digits from mpmath's hankel1 and converted
+to the monomial basis in z; the script reports the float64 Horner error of every table.
+"""
+import os
+
+import mpmath as mp
+import numpy as np
+
+mp.mp.dps = 40
+HERE = os.path.dirname(os.path.abspath(__file__))
+PIECES = [2.0, 2.6, 3.5, 5.0, 8.0]  # piece p covers [PIECES[p], PIECES[p+1]), the last one [8, inf)
+DEG = 12
+NSER = 14  # series terms k = 0..NSER-1
+
+
+def PQ(nu, x):
+    x = mp.mpf(x)
+    h = mp.hankel1(nu, x)
+    v = h * mp.exp(-1j * (x - (2 * nu + 1) * mp.pi / 4)) * mp.sqrt(mp.pi * x / 2)
+    return v.real, v.imag
+
+
+def cheb_to_monomial(c):
+    """Chebyshev coefficients c_k (sum c_k T_k(z)) -> monomial coefficients in z (ascending)."""
+    n = len(c)
+    T = [[mp.mpf(1)] + [mp.mpf(0)] * (n - 1), [mp.mpf(0), mp.mpf(1)] + [mp.mpf(0)] * (n - 2)]
+    for k in range(2, n):
+        t = [mp.mpf(0)] * n
+        for i in range(n - 1):
+            t[i + 1] += 2 * T[k - 1][i]
+        for i in range(n):
+            t[i] -= T[k - 2][i]
+        T.append(t)
+    m = [mp.mpf(0)] * n
+    for k in range(n):
+        for i in range(n):
+            m[i] += c[k] * T[k][i]
+    return m
+
+
+def fit_piece(f, ylo, yhi, deg):
+    ym, yh = (ylo + yhi) / 2, (yhi - ylo) / 2
+    nodes = [mp.cos(mp.pi * (j + mp.mpf("0.5")) / (deg + 1)) for j in range(deg + 1)]
+    vals = [f(ym + yh * z) for z in nodes]
+    c = []
+    for k in range(deg + 1):
+        s = mp.fsum(vals[j] * mp.cos(mp.pi * k * (j + mp.mpf("0.5")) / (deg + 1))
+                    for j in range(deg + 1))
+        c.append(2 * s / (deg + 1))
+    c[0] /= 2
+    return cheb_to_monomial(c), ym, yh
+
+
+def horner(coefs, z):
+    r = np.float64(coefs[-1])
+    for cc in coefs[-2::-1]:
+        r = r * z + np.float64(cc)
+    return r
+
+
+def main():
+    lines = []
+    emit = lines.append
+    emit("// Generated by gen_hankel_tables.py -- do not edit.  Coefficients of the device")
+    emit("// H0^(1), H1^(1) of real argument; see the generator for the representation.")
+    emit("#pragma once")
+    emit("")
+    emit("#define BEM2D_HANKEL_NPIECE %d" % len(PIECES))
+    emit("#define BEM2D_HANKEL_DEG %d" % DEG)
+    emit("#define BEM2D_HANKEL_NSER %d" % NSER)
+    emit("")
+    # ---- series
+    g = mp.euler
+    H = [mp.mpf(0)]
+    for k in range(1, NSER + 2):
+        H.append(H[-1] + mp.mpf(1) / k)
+    sj0 = [(-1) ** k / mp.factorial(k) ** 2 for k in range(NSER)]
+    sj1 = [(-1) ** k / (mp.factorial(k) * mp.factorial(k + 1)) for k in range(NSER)]
+    sy0 = [(2 / mp.pi) * (-1) ** k * (g - H[k]) / mp.factorial(k) ** 2 for k in range(NSER)]
+    sy1 = [-(1 / mp.pi) * (-1) ** k * (H[k] + H[k + 1] - 2 * g) /
+           (mp.factorial(k) * mp.factorial(k + 1)) for k in range(NSER)]
+    for name, arr in (("SER_J0", sj0), ("SER_J1", sj1), ("SER_Y0", sy0), ("SER_Y1", sy1)):
+        emit("static constexpr double BEM2D_%s[BEM2D_HANKEL_NSER] = {" % name)
+        emit("    " + ", ".join(mp.nstr(v, 20, min_fixed=1, max_fixed=0) for v in arr) + "};")
+    # ---- pieces
+    tabs = {"P0": [], "Q0": [], "P1": [], "Q1": []}
+    zmap = []
+    worst = 0.0
+    for p, a in enumerate(PIECES):
+        a = mp.mpf(a)
+        b = mp.mpf(PIECES[p + 1]) if p + 1 < len(PIECES) else None
+        ylo = (a / b) ** 2 if b is not None else mp.mpf(0)
+        for nu in (0, 1):
+            def fP(y, nu=nu):
+                if y == 0:
+                    return mp.mpf(1)
+                return PQ(nu, a / mp.sqrt(y))[0]
+
+            def fQ(y, nu=nu):
+                if y == 0:
+                    return (4 * nu * nu - 1) / mp.mpf(8)
+                x = a / mp.sqrt(y)
+                return PQ(nu, x)[1] * x
+            for fname, f in (("P%d" % nu, fP), ("Q%d" % nu, fQ)):
+                coefs, ym, yh = fit_piece(f, ylo, mp.mpf(1), DEG)
+                tabs[fname].append([float(c) for c in coefs])
+                # float64 Horner error check
+                err = 0.0
+                for y in np.linspace(float(ylo), 1.0, 257)[1:] if b is None else \
+                        np.linspace(float(ylo), 1.0, 257):
+                    z = (np.float64(y) - np.float64(ym)) / np.float64(yh)
+                    approx = horner([float(c) for c in coefs], z)
+                    exact = f(mp.mpf(y))
+                    scale = 1 if fname[0] == "P" else a / mp.sqrt(mp.mpf(y))  # Q = fit / x
+                    err = max(err, float(abs((approx - exact) / scale)))
+                worst = max(worst, err)
+                print("piece %d [%s, %s) %s: max |err| = %.3g" % (p, a, b, fname, err))
+        zmap.append((float(a) ** 2, float(ym), float(yh)))
+    emit("// piece p: y = PIECE_A2[p] / x^2, z = (y - PIECE_YM[p]) * PIECE_YHINV[p]")
+    emit("static constexpr double BEM2D_PIECE_X[BEM2D_HANKEL_NPIECE] = {%s};" %
+         ", ".join(repr(v) for v in PIECES))
+    emit("static constexpr double BEM2D_PIECE_A2[BEM2D_HANKEL_NPIECE] = {%s};" %
+         ", ".join(repr(z[0]) for z in zmap))
+    emit("static constexpr double BEM2D_PIECE_YM[BEM2D_HANKEL_NPIECE] = {%s};" %
+         ", ".join(repr(z[1]) for z in zmap))
+    emit("static constexpr double BEM2D_PIECE_YHINV[BEM2D_HANKEL_NPIECE] = {%s};" %
+         ", ".join(repr(1.0 / z[2]) for z in zmap))
+    for fname in ("P0", "Q0", "P1", "Q1"):
+        emit("static constexpr double BEM2D_FIT_%s[BEM2D_HANKEL_NPIECE][BEM2D_HANKEL_DEG + 1] = {"
+             % fname)
+        for row in tabs[fname]:
+            emit("    {" + ", ".join(repr(v) for v in row) + "},")
+        emit("};")
+    emit("// worst float64 Horner error of the fitted tables (absolute, on P and Q): %.3g" % worst)
+    with open(os.path.join(HERE, "hankel_tables.h"), "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+    print("worst", worst)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,142 @@
+"""ctypes front end of the CPU oracle (oracle/liboracle.so).  TEST INFRASTRUCTURE ONLY.
+
+Only tests/, the smoke entry and bench.py's cpu_baseline leg import this module; the product
+package never does.  Every function here forwards to the C restatement in bem2d_oracle.c, which
+cites the reference file:line it follows.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+R_GAUSS2D, R_AG1, R_AG2, R_AG3 = 0, 1, 2, 3
+K_NONE, K_SL, K_DL, K_CDL, K_COMBINED, K_ID = 0, 1, 2, 3, 4, 5
+F_PLANE, F_NORMAL_PLANE, F_COMBINED_PLANE = 0, 1, 2
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (run make -C oracle)")
+        L = ctypes.CDLL(path)
+        P = ctypes.c_void_p
+        i64 = ctypes.c_int64
+        d = ctypes.c_double
+        i32 = ctypes.c_int
+        L.oracle_gauss.argtypes = [i32, P, P]
+        L.oracle_rule_size.argtypes = [i32, i32, i32]
+        L.oracle_rule.argtypes = [i32, i32, i32, d, P, P, P]
+        L.oracle_hankel_many.argtypes = [P, i64, P]
+        L.oracle_hankel_many.restype = None
+        L.oracle_assemble.argtypes = [i64, P, P, P, P, P, i32, i32, d, i32, d, d, d, d, P, i64,
+                                      P, i64]
+        L.oracle_rhs.argtypes = [i64, P, P, i32, i32, d, d, P, i32, d, P, i64]
+        L.oracle_lu_solve.argtypes = [i64, i64, P, P]
+        L.oracle_eval_sl.argtypes = [i64, P, P, i32, d, P, P, i64, P]
+        L.oracle_farfield.argtypes = [i64, P, P, i32, d, P, i32, P, i64, P]
+        L.oracle_elem_eval.argtypes = [i64, P, P, i64, d, P]
+        L.oracle_elem_eval.restype = None
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def gauss(n):
+    x = np.zeros(n)
+    w = np.zeros(n)
+    rc = lib().oracle_gauss(n, _p(x), _p(w))
+    if rc:
+        raise ValueError("oracle_gauss failed")
+    return x, w
+
+
+def rule(which, N, L=3, sigma=0.15):
+    m = lib().oracle_rule_size(which, N, L)
+    x, y, w = np.zeros(m), np.zeros(m), np.zeros(m)
+    c = lib().oracle_rule(which, N, L, sigma, _p(x), _p(y), _p(w))
+    assert c == m
+    return x, y, w
+
+
+def hankel(x):
+    """Returns (J0, Y0, J1, Y1) arrays of real x > 0."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros((x.size, 4))
+    lib().oracle_hankel_many(_p(x), x.size, _p(out))
+    return out[:, 0], out[:, 1], out[:, 2], out[:, 3]
+
+
+def assemble(geom, N, L, sigma, kind, k, eta, mass_scale=0.0, kernel_scale=1.0, cols=None):
+    """Column-major N x ncols complex128 block of mass_scale*M + kernel_scale*K."""
+    n = geom.n
+    if cols is None:
+        ncols = n
+        cptr = None
+    else:
+        cols = np.ascontiguousarray(cols, dtype=np.int64)
+        ncols = cols.size
+        cptr = _p(cols)
+    out = np.zeros((ncols, n), dtype=np.complex128)  # row c = column cols[c]
+    rc = lib().oracle_assemble(n, _p(geom.type), _p(geom.param), _p(geom.index), _p(geom.next),
+                               _p(geom.prev), N, L, sigma, kind, k, eta, mass_scale, kernel_scale,
+                               cptr, ncols, _p(out), n)
+    if rc:
+        raise ValueError("oracle_assemble: bad quadrature option")
+    return out.T  # (n, ncols) view, column-major storage
+
+
+def rhs(geom, N, kind, k, eta, dirs, scale=1.0):
+    dirs = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1, 2)
+    nrhs = dirs.shape[0]
+    out = np.zeros((nrhs, geom.n), dtype=np.complex128)
+    lib().oracle_rhs(geom.n, _p(geom.type), _p(geom.param), N, kind, k, eta, _p(dirs), nrhs,
+                     scale, _p(out), geom.n)
+    return out.T
+
+
+def lu_solve(A, B):
+    """X = A^{-1} B with zgetrf/zgetrs semantics; raises on a singular matrix (LapackError)."""
+    n = A.shape[0]
+    Af = np.asfortranarray(A, dtype=np.complex128)
+    Bf = np.array(B, dtype=np.complex128, order="F", copy=True)
+    if Bf.ndim == 1:
+        Bf = Bf.reshape(n, 1, order="F")
+    info = lib().oracle_lu_solve(n, Bf.shape[1], _p(Af), _p(Bf))
+    if info:
+        raise np.linalg.LinAlgError("singular matrix: U(%d,%d) = 0" % (info, info))
+    return Bf
+
+
+def eval_sl(geom, N, k, coef, pts):
+    coef = np.ascontiguousarray(coef, dtype=np.complex128)
+    pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 2)
+    out = np.zeros(pts.shape[0], dtype=np.complex128)
+    lib().oracle_eval_sl(geom.n, _p(geom.type), _p(geom.param), N, k, _p(coef), _p(pts),
+                         pts.shape[0], _p(out))
+    return out
+
+
+def farfield(geom, N, k, coef, theta):
+    coef = np.asfortranarray(coef, dtype=np.complex128)
+    if coef.ndim == 1:
+        coef = coef.reshape(-1, 1, order="F")
+    theta = np.ascontiguousarray(theta, dtype=np.float64)
+    nrhs = coef.shape[1]
+    out = np.zeros((nrhs, theta.size), dtype=np.complex128)
+    lib().oracle_farfield(geom.n, _p(geom.type), _p(geom.param), N, k, _p(coef), nrhs, _p(theta),
+                          theta.size, _p(out))
+    return out.T
+
+
+def elem_eval(geom, e, t):
+    out = np.zeros(6)
+    lib().oracle_elem_eval(geom.n, _p(geom.type), _p(geom.param), e, t, _p(out))
+    return out
