@@ -1,0 +1,245 @@
+"""ctypes binding of the C-ABI in include/bem2d_hip.h (libbem2d_hip.so, built in-tree).
+
+This is plumbing for the tests, the smoke entry and bench.py: device buffers are torch tensors
+(complex128, column-major views), passed to the library as raw pointers.  Loading fails loudly
+when the HIP library has not been built -- there is no fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbem2d_hip.so")
+
+OK, ERR_ARG, ERR_SINGULAR, ERR_NOMEM, ERR_DEVICE, ERR_UNSUPPORTED, ERR_NOT_CONVERGED = range(7)
+ELEM_FLAT, ELEM_CIRCLE, ELEM_TREFOIL, ELEM_KITE, ELEM_ELLIPSE, ELEM_ELLIPSE_ARC, ELEM_INV_ELLIPSE = \
+    range(7)
+KERNEL_NONE, KERNEL_SL, KERNEL_DL, KERNEL_CDL, KERNEL_COMBINED, KERNEL_ID = range(6)
+FIELD_PLANE, FIELD_NORMAL_PLANE, FIELD_COMBINED_PLANE = range(3)
+PROF_ASSEMBLE_REGULAR, PROF_ASSEMBLE_SPECIAL, PROF_LU = range(3)
+
+# Exported symbols, with (restype, argtypes).  Kept in the order of include/bem2d_hip.h.
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_D = ctypes.c_double
+_I = ctypes.c_int
+
+
+class Geometry(ctypes.Structure):
+    _fields_ = [("n", _I64), ("type", _P), ("param", _P), ("index", _P), ("next", _P),
+                ("prev", _P)]
+
+
+class QuadOption(ctypes.Structure):
+    _fields_ = [("L", _I), ("N", _I), ("sigma", _D)]
+
+
+class IRReport(ctypes.Structure):
+    """Outcome of the mixed-precision solve (bem2d_ir_report in include/bem2d_hip.h)."""
+    _fields_ = [("iterations", _I), ("fallback_fp64", _I), ("backward_error", _D),
+                ("residual_norm", _D), ("matrix_norm", _D)]
+
+
+SIGNATURES = {
+    "bem2d_ctx_create": (_I, [ctypes.POINTER(_P), _I]),
+    "bem2d_ctx_destroy": (None, [_P]),
+    "bem2d_ctx_set_stream": (_I, [_P, _P]),
+    "bem2d_ctx_stream": (_P, [_P]),
+    "bem2d_synchronize": (_I, [_P]),
+    "bem2d_last_error": (ctypes.c_char_p, []),
+    "bem2d_device_alloc": (_I, [_P, _I64, ctypes.POINTER(_P)]),
+    "bem2d_device_free": (_I, [_P, _P]),
+    "bem2d_copy_to_device": (_I, [_P, _P, _P, _I64]),
+    "bem2d_copy_to_host": (_I, [_P, _P, _P, _I64]),
+    "bem2d_mesh_create": (_I, [_P, ctypes.POINTER(Geometry), QuadOption, ctypes.POINTER(_P)]),
+    "bem2d_mesh_destroy": (None, [_P]),
+    "bem2d_mesh_size": (_I64, [_P]),
+    "bem2d_mesh_special_count": (_I64, [_P]),
+    "bem2d_assemble": (_I, [_P, _P, _I, _D, _D, _D, _D, _P, _I64]),
+    "bem2d_rhs": (_I, [_P, _P, _I, _D, _D, _P, _I, _D, _P, _I64]),
+    "bem2d_solve_fp64": (_I, [_P, _I64, _I64, _P, _I64, _P, _I64, _I]),
+    "bem2d_eval_single_layer": (_I, [_P, _P, _D, _P, _P, _I64, _P]),
+    "bem2d_farfield": (_I, [_P, _P, _D, _P, _I64, _I, _P, _I64, _P]),
+    "bem2d_profile_enable": (_I, [_P, _I]),
+    "bem2d_profile_reset": (_I, [_P]),
+    "bem2d_profile_read": (_I, [_P, _I, ctypes.POINTER(_D), ctypes.POINTER(_I64)]),
+    "bem2d_hankel01": (_I, [_P, _P, _I64, _P]),
+    "bem2d_hankel01_host": (_I, [_P, _I64, _P]),
+}
+
+_LIB = None
+
+
+class Bem2dError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("bem2d status %d: %s" % (code, msg))
+        self.code = code
+
+
+def lib():
+    """The loaded HIP library; raises if it is not built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("%s is not built: run `make -C bem2d_amd` (or "
+                               "__graft_entry__.build())" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def check(code):
+    if code != OK:
+        raise Bem2dError(code, lib().bem2d_last_error().decode(errors="replace"))
+
+
+def _np_ptr(a):
+    return a.ctypes.data_as(_P)
+
+
+def _t_ptr(t):
+    return _P(t.data_ptr())
+
+
+def hankel01_host(x):
+    """Host build of the device Hankel routine: returns (H0, H1) complex arrays."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros((x.size, 4))
+    check(lib().bem2d_hankel01_host(_np_ptr(x), x.size, _np_ptr(out)))
+    return out[:, 0] + 1j * out[:, 1], out[:, 2] + 1j * out[:, 3]
+
+
+class Context:
+    """A bem2d_ctx on one GPU, issuing work on torch's current stream of that device."""
+
+    def __init__(self, device=0, use_torch_stream=True):
+        import torch
+        self.device = device
+        h = _P()
+        check(lib().bem2d_ctx_create(ctypes.byref(h), device))
+        self.h = h
+        if use_torch_stream:
+            s = torch.cuda.current_stream(device)
+            check(lib().bem2d_ctx_set_stream(self.h, _P(s.cuda_stream)))
+
+    def sync(self):
+        check(lib().bem2d_synchronize(self.h))
+
+    def close(self):
+        if self.h:
+            lib().bem2d_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def profile(self, on=True):
+        check(lib().bem2d_profile_enable(self.h, 1 if on else 0))
+        check(lib().bem2d_profile_reset(self.h))
+
+    def profile_read(self, kernel_id):
+        """(total ms, launches) of an instrumented kernel since the last reset."""
+        ms = _D()
+        cnt = _I64()
+        check(lib().bem2d_profile_read(self.h, kernel_id, ctypes.byref(ms), ctypes.byref(cnt)))
+        return ms.value, cnt.value
+
+    # ---- operations ------------------------------------------------------------------------
+    def mesh(self, geom, N=16, L=3, sigma=0.15):
+        return Mesh(self, geom, N, L, sigma)
+
+    def hankel01(self, x):
+        """Device Hankel: x a float64 cuda tensor; returns (n, 4) float64 tensor."""
+        import torch
+        out = torch.empty((x.numel(), 4), dtype=torch.float64, device=x.device)
+        check(lib().bem2d_hankel01(self.h, _t_ptr(x), x.numel(), _t_ptr(out)))
+        return out
+
+    def assemble(self, mesh, kind, k, eta=0.0, mass_scale=0.0, kernel_scale=1.0, out=None):
+        """Returns the n x n complex128 matrix as a column-major (transposed-storage) view."""
+        import torch
+        n = mesh.n
+        if out is None:
+            store = torch.empty((n, n), dtype=torch.complex128, device="cuda:%d" % self.device)
+            out = store.t()
+        assert out.stride(0) == 1 and out.shape == (n, n)
+        check(lib().bem2d_assemble(self.h, mesh.h, kind, k, eta, mass_scale, kernel_scale,
+                                   _t_ptr(out), out.stride(1)))
+        return out
+
+    def rhs(self, mesh, kind, k, eta, dirs, scale=1.0, out=None):
+        import torch
+        dirs = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1, 2)
+        nrhs = dirs.shape[0]
+        n = mesh.n
+        if out is None:
+            out = torch.empty((nrhs, n), dtype=torch.complex128,
+                              device="cuda:%d" % self.device).t()
+        check(lib().bem2d_rhs(self.h, mesh.h, kind, k, eta, _np_ptr(dirs), nrhs, scale,
+                              _t_ptr(out), out.stride(1)))
+        return out
+
+    def solve_fp64(self, A, B, preserve_A=True):
+        """Overwrites B (n x nrhs column-major complex128) with A^{-1} B."""
+        n = A.shape[0]
+        nrhs = B.shape[1]
+        check(lib().bem2d_solve_fp64(self.h, n, nrhs, _t_ptr(A), A.stride(1), _t_ptr(B),
+                                     B.stride(1), 1 if preserve_A else 0))
+        return B
+
+    def eval_single_layer(self, mesh, k, coef, pts):
+        import torch
+        pts = pts.contiguous()
+        out = torch.empty(pts.shape[0], dtype=torch.complex128, device=pts.device)
+        check(lib().bem2d_eval_single_layer(self.h, mesh.h, k, _t_ptr(coef.contiguous()),
+                                            _t_ptr(pts), pts.shape[0], _t_ptr(out)))
+        return out
+
+    def farfield(self, mesh, k, coef, theta):
+        """coef: n x nrhs column-major; returns nang x nrhs column-major."""
+        import torch
+        nrhs = coef.shape[1]
+        nang = theta.numel()
+        out = torch.empty((nrhs, nang), dtype=torch.complex128, device=theta.device).t()
+        check(lib().bem2d_farfield(self.h, mesh.h, k, _t_ptr(coef), coef.stride(1), nrhs,
+                                   _t_ptr(theta.contiguous()), nang, _t_ptr(out)))
+        return out
+
+
+class Mesh:
+    """A bem2d_mesh: device copy of a Geometry plus its quadrature tables."""
+
+    def __init__(self, ctx, geom, N=16, L=3, sigma=0.15):
+        self.ctx = ctx
+        self.geom = geom
+        self.n = geom.n
+        g = Geometry(geom.n, _np_ptr(geom.type), _np_ptr(geom.param), _np_ptr(geom.index),
+                     _np_ptr(geom.next), _np_ptr(geom.prev))
+        h = _P()
+        check(lib().bem2d_mesh_create(ctx.h, ctypes.byref(g), QuadOption(L, N, sigma),
+                                      ctypes.byref(h)))
+        self.h = h
+        self.N, self.L, self.sigma = N, L, sigma
+
+    @property
+    def special_count(self):
+        return lib().bem2d_mesh_special_count(self.h)
+
+    def close(self):
+        if self.h:
+            lib().bem2d_mesh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

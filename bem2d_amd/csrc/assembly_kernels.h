@@ -1,0 +1,334 @@
+// Device kernels of the FP64 hot path: Gauss-node tables, regular-pair and singular-pair Galerkin
+// assembly, plane-wave right-hand sides, single-layer potential and far field.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "geometry.h"
+#include "hankel.h"
+
+namespace bem2d_dev {
+
+// Node tables of a mesh: five planes of nq*n doubles, plane[a*n + e] for Gauss node a of element e.
+enum { NODE_X = 0, NODE_Y = 1, NODE_NX = 2, NODE_NY = 3, NODE_SW = 4, NODE_PLANES = 5 };
+
+// Kernel functors of lib/bem2d_kernel.cpp:15-112 and IdKernel (lib/bem2d_kernel.h:91-98) for real k.
+// (x, y) = test and trial points, n1 / n2 = test and trial normals.  k4 = k/4, eta4 = eta/4.
+template <int KIND>
+__host__ __device__ __forceinline__ void kernel_eval(double k, double k4, double eta4, double xx,
+                                                     double xy, double yx, double yy, double n1x,
+                                                     double n1y, double n2x, double n2y,
+                                                     double& kr, double& ki)
+{
+    if (KIND == BEM2D_KERNEL_ID) {
+        kr = 1.0;
+        ki = 0.0;
+        return;
+    }
+    if (KIND == BEM2D_KERNEL_NONE) {
+        kr = 0.0;
+        ki = 0.0;
+        return;
+    }
+    const double wx = xx - yx, wy = xy - yy; // x - y
+    const double d = sqrt(wx * wx + wy * wy);
+    double h0r, h0i, h1r, h1i;
+    hankel01(k * d, h0r, h0i, h1r, h1i);
+    if (KIND == BEM2D_KERNEL_SL) { // (i/4) H0
+        kr = -0.25 * h0i;
+        ki = 0.25 * h0r;
+        return;
+    }
+    // -(ik/4) c H1 with c = n.w/d
+    double c;
+    if (KIND == BEM2D_KERNEL_DL)
+        c = -(n2x * wx + n2y * wy) / d; // n_y.(y - x)/d
+    else
+        c = (n1x * wx + n1y * wy) / d; // n_x.(x - y)/d
+    kr = k4 * c * h1i;
+    ki = -k4 * c * h1r;
+    if (KIND == BEM2D_KERNEL_COMBINED) { // CDL - i eta (i/4) H0 = CDL + (eta/4) H0
+        kr += eta4 * h0r;
+        ki += eta4 * h0i;
+    }
+}
+
+// Gauss-node tables: for element e and node a, position, unit normal and |Gamma'| w_a.
+__global__ void prep_nodes_kernel(int64_t n, int nq, const int32_t* __restrict__ type,
+                                  const double* __restrict__ param, const double* __restrict__ gx,
+                                  const double* __restrict__ gw, double* __restrict__ nodes)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * nq) return;
+    const int64_t e = t % n;
+    const int a = (int)(t / n);
+    double p[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) p[q] = param[8 * e + q];
+    double mx, my, dx, dy, nx, ny, len;
+    elem_eval(type[e], p, gx[a], mx, my, dx, dy);
+    normal_len(dx, dy, nx, ny, len);
+    const int64_t plane = n * nq;
+    nodes[NODE_X * plane + t] = mx;
+    nodes[NODE_Y * plane + t] = my;
+    nodes[NODE_NX * plane + t] = nx;
+    nodes[NODE_NY * plane + t] = ny;
+    nodes[NODE_SW * plane + t] = len * gw[a];
+}
+
+constexpr int kRowsPerBlock = 64;   // one wavefront of rows (test elements)
+constexpr int kWavesPerBlock = 4;
+constexpr int kColsPerWave = 4;     // trial elements handled in turn by each wavefront
+constexpr int kColsPerBlock = kWavesPerBlock * kColsPerWave;
+
+// Regular pairs (Integrate with Gauss2D, lib/bem2d_quadrature.h:37-65, :283-286).
+// Block = 64 rows x 16 columns; lane = row i (coalesced column-major stores), the trial element's
+// node data sit in LDS and are read as wave-uniform broadcasts.  Coincident / adjacent pairs are
+// skipped here and written by assemble_special_kernel.  NQ = nq at compile time (0: runtime nq).
+template <int KIND, int NQ>
+__global__ __launch_bounds__(256) void assemble_regular_kernel(
+    int64_t n, int nq_rt, const double* __restrict__ nodes, const int64_t* __restrict__ index,
+    const int64_t* __restrict__ next, const int64_t* __restrict__ prev, double k, double eta,
+    double kscale, double2* __restrict__ A, int64_t lda)
+{
+    extern __shared__ double sh[]; // [kColsPerBlock][nq][5]
+    const int nq = NQ ? NQ : nq_rt;
+    constexpr int kUnroll = 1;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int64_t i = (int64_t)blockIdx.x * kRowsPerBlock + lane;
+    const int64_t j0 = (int64_t)blockIdx.y * kColsPerBlock;
+    const int64_t plane = n * nq;
+
+    for (int t = threadIdx.x; t < kColsPerBlock * nq; t += blockDim.x) {
+        const int c = t / nq, b = t - c * nq;
+        const int64_t j = j0 + c;
+        if (j < n) {
+            const int64_t o = (int64_t)b * n + j;
+#pragma unroll
+            for (int f = 0; f < NODE_PLANES; ++f) sh[(c * nq + b) * 5 + f] = nodes[f * plane + o];
+        }
+    }
+    __syncthreads();
+    if (i >= n) return;
+
+    const double k4 = 0.25 * k, eta4 = 0.25 * eta;
+    const int64_t idx_i = index[i], nx_i = next[i], pv_i = prev[i];
+    for (int cc = 0; cc < kColsPerWave; ++cc) {
+        const int c = wave * kColsPerWave + cc;
+        const int64_t j = j0 + c;
+        if (j >= n) break;
+        const int64_t idx_j = index[j];
+        if (idx_j == idx_i || nx_i == idx_j || pv_i == idx_j) continue;
+        const double* sc = sh + c * nq * 5;
+        double ar = 0.0, ai = 0.0;
+        for (int a = 0; a < nq; ++a) {
+            const int64_t o = (int64_t)a * n + i;
+            const double xx = nodes[NODE_X * plane + o], xy = nodes[NODE_Y * plane + o];
+            const double n1x = nodes[NODE_NX * plane + o], n1y = nodes[NODE_NY * plane + o];
+            const double swa = nodes[NODE_SW * plane + o];
+            double br = 0.0, bi = 0.0;
+#pragma unroll kUnroll
+            for (int b = 0; b < nq; ++b) {
+                const double* y = sc + b * 5;
+                double kr, ki;
+                kernel_eval<KIND>(k, k4, eta4, xx, xy, y[0], y[1], n1x, n1y, y[2], y[3], kr, ki);
+                br = fma(kr, y[4], br);
+                bi = fma(ki, y[4], bi);
+            }
+            ar = fma(br, swa, ar);
+            ai = fma(bi, swa, ai);
+        }
+        A[i + lda * j] = make_double2(kscale * ar, kscale * ai);
+    }
+}
+
+// Coincident / adjacent pairs: one block of 256 threads per entry (i, j, rule).
+// rule 1: IntegrateIdent with AdaptedGauss1 (lib/bem2d_quadrature.h:67-109), plus the EvalIdent
+//         mass entry (lib/bem2d_quadrature.cpp:108-123) scaled by mscale;
+// rule 3: next(i) == index(j), Integrate with AdaptedGauss3; rule 2: prev, AdaptedGauss2.
+template <int KIND>
+__global__ __launch_bounds__(256) void assemble_special_kernel(
+    const int64_t* __restrict__ special, const int32_t* __restrict__ type,
+    const double* __restrict__ param, const double* __restrict__ rules, int m1, int m2, int m3,
+    const double* __restrict__ gx, const double* __restrict__ gw, int nq, double k, double eta,
+    double kscale, double mscale, double2* __restrict__ A, int64_t lda)
+{
+    __shared__ double red[2][256];
+    const int64_t e = blockIdx.x;
+    const int64_t i = special[3 * e], j = special[3 * e + 1];
+    const int rule = (int)special[3 * e + 2];
+    const int ti = type[i], tj = type[j];
+    double pi_[8], pj[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        pi_[q] = param[8 * i + q];
+        pj[q] = param[8 * j + q];
+    }
+    const double k4 = 0.25 * k, eta4 = 0.25 * eta;
+    double ar = 0.0, ai = 0.0;
+    if (KIND != BEM2D_KERNEL_NONE && kscale != 0.0) {
+        const double* rx;
+        int m;
+        if (rule == 1) { rx = rules; m = m1; }
+        else if (rule == 3) { rx = rules + 3 * (int64_t)(m1 + m2); m = m3; }
+        else { rx = rules + 3 * (int64_t)m1; m = m2; }
+        const double* ry = rx + m;
+        const double* rw = ry + m;
+        for (int q = threadIdx.x; q < m; q += blockDim.x) {
+            const double xq = rx[q], yq = ry[q], wq = rw[q];
+            double mx1, my1, dx1, dy1, mx2, my2, dx2, dy2, n1x, n1y, s1, n2x, n2y, s2, kr, ki;
+            if (rule == 1) {
+                // Duffy split of the coincident pair: (x, (1-y)x) and (1-x, 1-x+yx), Jacobian x
+                elem_eval(ti, pi_, xq, mx1, my1, dx1, dy1);
+                elem_eval(tj, pj, (1 - yq) * xq, mx2, my2, dx2, dy2);
+                normal_len(dx1, dy1, n1x, n1y, s1);
+                normal_len(dx2, dy2, n2x, n2y, s2);
+                kernel_eval<KIND>(k, k4, eta4, mx1, my1, mx2, my2, n1x, n1y, n2x, n2y, kr, ki);
+                ar += xq * kr * s1 * s2 * wq;
+                ai += xq * ki * s1 * s2 * wq;
+                elem_eval(ti, pi_, 1 - xq, mx1, my1, dx1, dy1);
+                elem_eval(tj, pj, 1 - xq + yq * xq, mx2, my2, dx2, dy2);
+                normal_len(dx1, dy1, n1x, n1y, s1);
+                normal_len(dx2, dy2, n2x, n2y, s2);
+                kernel_eval<KIND>(k, k4, eta4, mx1, my1, mx2, my2, n1x, n1y, n2x, n2y, kr, ki);
+                ar += xq * kr * s1 * s2 * wq;
+                ai += xq * ki * s1 * s2 * wq;
+            } else {
+                elem_eval(ti, pi_, xq, mx1, my1, dx1, dy1);
+                elem_eval(tj, pj, yq, mx2, my2, dx2, dy2);
+                normal_len(dx1, dy1, n1x, n1y, s1);
+                normal_len(dx2, dy2, n2x, n2y, s2);
+                kernel_eval<KIND>(k, k4, eta4, mx1, my1, mx2, my2, n1x, n1y, n2x, n2y, kr, ki);
+                ar += kr * s1 * s2 * wq;
+                ai += ki * s1 * s2 * wq;
+            }
+        }
+    }
+    red[0][threadIdx.x] = ar;
+    red[1][threadIdx.x] = ai;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + s];
+            red[1][threadIdx.x] += red[1][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double vr = kscale * red[0][0], vi = kscale * red[1][0];
+        if (rule == 1 && mscale != 0.0) {
+            double mass = 0.0; // sum_t |Gamma_j'(x_t)| w_t, Gauss1D(N)
+            for (int t = 0; t < nq; ++t) {
+                double mx, my, dx, dy;
+                elem_eval(tj, pj, gx[t], mx, my, dx, dy);
+                mass += sqrt(dx * dx + dy * dy) * gw[t];
+            }
+            vr += mscale * mass;
+        }
+        A[i + lda * j] = make_double2(vr, vi);
+    }
+}
+
+// DotProdBasFuns (lib/bem2d_quadrature.h:111-156) for the plane-wave field functors
+// (lib/bem2d_fun.h:21-24, 45-49, 71-75): thread per (row i, right-hand side r).
+__global__ void rhs_kernel(int64_t n, int nq, const double* __restrict__ nodes, int kind, double k,
+                           double eta, const double* __restrict__ dirs, double scale,
+                           double2* __restrict__ B, int64_t ldb)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (i >= n) return;
+    const double ddx = dirs[2 * r], ddy = dirs[2 * r + 1];
+    const int64_t plane = n * nq;
+    double ar = 0.0, ai = 0.0;
+    for (int a = 0; a < nq; ++a) {
+        const int64_t o = (int64_t)a * n + i;
+        const double px = nodes[NODE_X * plane + o], py = nodes[NODE_Y * plane + o];
+        const double sw = nodes[NODE_SW * plane + o];
+        const double ph = k * (ddx * px + ddy * py);
+        const double c = cos(ph), s = sin(ph);
+        double fr, fi;
+        if (kind == BEM2D_FIELD_PLANE) {
+            fr = c;
+            fi = s;
+        } else {
+            const double nd = nodes[NODE_NX * plane + o] * ddx + nodes[NODE_NY * plane + o] * ddy;
+            const double g = (kind == BEM2D_FIELD_COMBINED_PLANE) ? k * nd - eta : k * nd;
+            fr = -g * s; // i g e^{i ph}
+            fi = g * c;
+        }
+        ar = fma(fr, sw, ar);
+        ai = fma(fi, sw, ai);
+    }
+    B[i + ldb * r] = make_double2(scale * ar, scale * ai);
+}
+
+// EvalKernel<SingleLayer> at points (lib/bem2d_quadrature.h:184-228): thread per point.
+__global__ void eval_sl_kernel(int64_t n, int nq, const double* __restrict__ nodes, double k,
+                               const double2* __restrict__ coef, const double* __restrict__ pts,
+                               int64_t npts, double2* __restrict__ out)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npts) return;
+    const double px = pts[2 * p], py = pts[2 * p + 1];
+    const int64_t plane = n * nq;
+    double vr = 0.0, vi = 0.0;
+    for (int64_t j = 0; j < n; ++j) {
+        double sr = 0.0, si = 0.0;
+        for (int a = 0; a < nq; ++a) {
+            const int64_t o = (int64_t)a * n + j;
+            double kr, ki;
+            kernel_eval<BEM2D_KERNEL_SL>(k, 0.25 * k, 0.0, px, py, nodes[NODE_X * plane + o],
+                                         nodes[NODE_Y * plane + o], 0, 0, 0, 0, kr, ki);
+            const double sw = nodes[NODE_SW * plane + o];
+            sr = fma(kr, sw, sr);
+            si = fma(ki, sw, si);
+        }
+        const double2 c = coef[j];
+        vr += c.x * sr - c.y * si;
+        vi += c.x * si + c.y * sr;
+    }
+    out[p] = make_double2(vr, vi);
+}
+
+// Far-field signature F[a + nang*j] = sum_t exp(-ik xhat_a.y_t) |Gamma_j'| w_t: thread per (a, j).
+__global__ void farfield_signature_kernel(int64_t n, int nq, const double* __restrict__ nodes,
+                                          double k, const double* __restrict__ theta, int64_t nang,
+                                          double2* __restrict__ F)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * nang) return;
+    const int64_t a = t % nang, j = t / nang;
+    const double cx = cos(theta[a]), cy = sin(theta[a]);
+    const int64_t plane = n * nq;
+    double sr = 0.0, si = 0.0;
+    for (int q = 0; q < nq; ++q) {
+        const int64_t o = (int64_t)q * n + j;
+        const double ph = -k * (cx * nodes[NODE_X * plane + o] + cy * nodes[NODE_Y * plane + o]);
+        const double sw = nodes[NODE_SW * plane + o];
+        sr = fma(cos(ph), sw, sr);
+        si = fma(sin(ph), sw, si);
+    }
+    F[t] = make_double2(sr, si);
+}
+
+// out[a + nang*r] = pref * sum_j F[a + nang*j] coef[j + ldc*r]: thread per (a, r).
+__global__ void farfield_combine_kernel(int64_t n, const double2* __restrict__ F, int64_t nang,
+                                        const double2* __restrict__ coef, int64_t ldc, double pr,
+                                        double pi_, double2* __restrict__ out)
+{
+    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (a >= nang) return;
+    double vr = 0.0, vi = 0.0;
+    for (int64_t j = 0; j < n; ++j) {
+        const double2 f = F[a + nang * j];
+        const double2 c = coef[j + ldc * r];
+        vr += f.x * c.x - f.y * c.y;
+        vi += f.x * c.y + f.y * c.x;
+    }
+    out[a + nang * r] = make_double2(pr * vr - pi_ * vi, pr * vi + pi_ * vr);
+}
+
+} // namespace bem2d_dev

@@ -1,0 +1,655 @@
+// C-ABI of the MI355X hot path (declared in include/bem2d_hip.h): contexts, meshes, FP64 assembly,
+// right-hand sides, the plain FP64 solve, the single-layer potential and the far field.
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/bem2d_hip.h"
+#include "assembly_kernels.h"
+#include "internal.h"
+
+using namespace bem2d_dev;
+
+// ------------------------------------------------------------------------------------------------
+// errors
+// ------------------------------------------------------------------------------------------------
+static thread_local std::string g_last_error;
+
+int bem2d_internal_fail(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+extern "C" const char* bem2d_last_error(void) { return g_last_error.c_str(); }
+
+// ------------------------------------------------------------------------------------------------
+// context
+// ------------------------------------------------------------------------------------------------
+extern "C" int bem2d_ctx_create(bem2d_ctx** out, int device)
+{
+    if (!out) return FAIL(BEM2D_ERR_ARG, "bem2d_ctx_create: null output");
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev)
+        return FAIL(BEM2D_ERR_ARG, "bem2d_ctx_create: device %d of %d", device, ndev);
+    HIPCHK(hipSetDevice(device));
+    bem2d_ctx* c = new bem2d_ctx();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return FAIL(BEM2D_ERR_DEVICE, "hipStreamCreate failed");
+    }
+    c->stream = c->own_stream;
+    if (rocblas_create_handle(&c->blas) != rocblas_status_success) {
+        (void)hipStreamDestroy(c->own_stream);
+        delete c;
+        return FAIL(BEM2D_ERR_DEVICE, "rocblas_create_handle failed");
+    }
+    rocblas_set_stream(c->blas, c->stream);
+    *out = c;
+    return BEM2D_OK;
+}
+
+extern "C" void bem2d_ctx_destroy(bem2d_ctx* c)
+{
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (auto& p : c->prof)
+        for (hipEvent_t e : p.pending) (void)hipEventDestroy(e);
+    for (auto& s : c->scratch)
+        if (s.ptr) (void)hipFree(s.ptr);
+    bem2d_internal_mixed_release(c);
+    rocblas_destroy_handle(c->blas);
+    (void)hipStreamDestroy(c->own_stream);
+    delete c;
+}
+
+extern "C" int bem2d_ctx_set_stream(bem2d_ctx* c, void* stream)
+{
+    if (!c) return FAIL(BEM2D_ERR_ARG, "null context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->stream = stream ? (hipStream_t)stream : c->own_stream;
+    rocblas_set_stream(c->blas, c->stream);
+    return BEM2D_OK;
+}
+
+extern "C" void* bem2d_ctx_stream(bem2d_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+extern "C" int bem2d_synchronize(bem2d_ctx* c)
+{
+    if (!c) return FAIL(BEM2D_ERR_ARG, "null context");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return BEM2D_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
+// kernel timing
+// ------------------------------------------------------------------------------------------------
+void bem2d_internal_prof_begin(bem2d_ctx* c, int id)
+{
+    if (!c->profiling) return;
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    (void)hipEventRecord(e, c->stream);
+    c->prof[id].pending.push_back(e);
+}
+
+void bem2d_internal_prof_end(bem2d_ctx* c, int id)
+{
+    if (!c->profiling || c->prof[id].pending.size() % 2 == 0) return;
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    (void)hipEventRecord(e, c->stream);
+    c->prof[id].pending.push_back(e);
+}
+
+extern "C" int bem2d_profile_enable(bem2d_ctx* c, int on)
+{
+    if (!c) return FAIL(BEM2D_ERR_ARG, "null context");
+    c->profiling = on != 0;
+    return BEM2D_OK;
+}
+
+extern "C" int bem2d_profile_reset(bem2d_ctx* c)
+{
+    if (!c) return FAIL(BEM2D_ERR_ARG, "null context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (auto& p : c->prof) {
+        for (hipEvent_t e : p.pending) (void)hipEventDestroy(e);
+        p.pending.clear();
+        p.total_ms = 0.0;
+        p.launches = 0;
+    }
+    return BEM2D_OK;
+}
+
+extern "C" int bem2d_profile_read(bem2d_ctx* c, int id, double* total_ms, int64_t* launches)
+{
+    if (!c || id < 0 || id >= BEM2D_PROF_COUNT) return FAIL(BEM2D_ERR_ARG, "bem2d_profile_read: bad argument");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    auto& p = c->prof[id];
+    const size_t pairs = p.pending.size() / 2;
+    for (size_t q = 0; q < pairs; ++q) {
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, p.pending[2 * q], p.pending[2 * q + 1]));
+        p.total_ms += ms;
+        p.launches += 1;
+        (void)hipEventDestroy(p.pending[2 * q]);
+        (void)hipEventDestroy(p.pending[2 * q + 1]);
+    }
+    p.pending.erase(p.pending.begin(), p.pending.begin() + 2 * pairs);
+    if (total_ms) *total_ms = p.total_ms;
+    if (launches) *launches = p.launches;
+    return BEM2D_OK;
+}
+
+void* bem2d_internal_scratch(bem2d_ctx* c, int slot, size_t bytes)
+{
+    auto& s = c->scratch[slot];
+    if (s.bytes >= bytes && s.ptr) return s.ptr;
+    if (s.ptr) {
+        (void)hipStreamSynchronize(c->stream); // earlier work may still read the old buffer
+        (void)hipFree(s.ptr);
+        s.ptr = nullptr;
+        s.bytes = 0;
+    }
+    size_t want = bytes < 256 ? 256 : bytes;
+    if (hipMalloc(&s.ptr, want) != hipSuccess) {
+        s.ptr = nullptr;
+        return nullptr;
+    }
+    s.bytes = want;
+    return s.ptr;
+}
+
+extern "C" int bem2d_device_alloc(bem2d_ctx* c, int64_t bytes, void** out)
+{
+    if (!c || !out || bytes < 0) return FAIL(BEM2D_ERR_ARG, "bem2d_device_alloc: bad argument");
+    HIPCHK(hipSetDevice(c->device));
+    if (hipMalloc(out, bytes > 0 ? (size_t)bytes : 1) != hipSuccess)
+        return FAIL(BEM2D_ERR_NOMEM, "hipMalloc(%lld) failed", (long long)bytes);
+    return BEM2D_OK;
+}
+
+extern "C" int bem2d_device_free(bem2d_ctx* c, void* p)
+{
+    if (!c) return FAIL(BEM2D_ERR_ARG, "null context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (p) HIPCHK(hipFree(p));
+    return BEM2D_OK;
+}
+
+extern "C" int bem2d_copy_to_device(bem2d_ctx* c, void* dst, const void* src, int64_t bytes)
+{
+    if (!c || bytes < 0) return FAIL(BEM2D_ERR_ARG, "bem2d_copy_to_device: bad argument");
+    if (bytes == 0) return BEM2D_OK;
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return BEM2D_OK;
+}
+
+extern "C" int bem2d_copy_to_host(bem2d_ctx* c, void* dst, const void* src, int64_t bytes)
+{
+    if (!c || bytes < 0) return FAIL(BEM2D_ERR_ARG, "bem2d_copy_to_host: bad argument");
+    if (bytes == 0) return BEM2D_OK;
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return BEM2D_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
+// quadrature rules on the host
+// ------------------------------------------------------------------------------------------------
+// Gauss-Legendre nodes (ascending) and weights on [0, 1]: Newton's method on P_n in long double.
+// The reference obtains the same rule by Golub-Welsch (lib/bem2d_quadrature.cpp:10-56); both give
+// the rule to within an ulp.
+void bem2d_internal_gauss01(int n, std::vector<double>& x, std::vector<double>& w)
+{
+    x.assign(n, 0.0);
+    w.assign(n, 0.0);
+    if (n == 1) {
+        x[0] = 0.5;
+        w[0] = 1.0;
+        return;
+    }
+    const long double pi = 3.141592653589793238462643383279502884L;
+    for (int i = 0; i < n; ++i) {
+        long double z = cosl(pi * (i + 0.75L) / (n + 0.5L)); // i-th largest root
+        long double dp = 1.0L;
+        for (int it = 0; it < 100; ++it) {
+            long double p0 = 1.0L, p1 = z;
+            for (int m = 2; m <= n; ++m) {
+                long double p2 = ((2 * m - 1) * z * p1 - (m - 1) * p0) / m;
+                p0 = p1;
+                p1 = p2;
+            }
+            dp = n * (z * p1 - p0) / (z * z - 1.0L);
+            long double dz = p1 / dp;
+            z -= dz;
+            if (fabsl(dz) < 1e-19L) break;
+        }
+        {
+            long double p0 = 1.0L, p1 = z;
+            for (int m = 2; m <= n; ++m) {
+                long double p2 = ((2 * m - 1) * z * p1 - (m - 1) * p0) / m;
+                p0 = p1;
+                p1 = p2;
+            }
+            dp = n * (z * p1 - p0) / (z * z - 1.0L);
+        }
+        const int o = n - 1 - i; // ascending order
+        x[o] = (double)((1.0L + z) / 2.0L);
+        w[o] = (double)(1.0L / ((1.0L - z * z) * dp * dp)); // (2/((1-z^2)P'^2)) / 2
+    }
+}
+
+namespace {
+
+struct Rule {
+    std::vector<double> x, y, w;
+};
+
+// Tensor rule of Gauss2D(N) rescaled onto [a,b] x [c,d] and appended (lib/bem2d_quadpoints.cpp:
+// 166-204 and MapPoints2d, lib/bem2d_quadrature.cpp:68-76).
+void append_box(Rule& r, const std::vector<double>& g, const std::vector<double>& gw, double a,
+                double b, double c, double d)
+{
+    const int N = (int)g.size();
+    for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j) {
+            r.x.push_back(a + (b - a) * g[i]);
+            r.y.push_back(c + (d - c) * g[j]);
+            r.w.push_back((b - a) * (d - c) * (gw[i] * gw[j]));
+        }
+}
+
+// AdaptedGauss1: L x L graded boxes on [sigma^L, 1]^2 with breakpoints sigma^L, ..., sigma, 1
+// (lib/bem2d_quadpoints.cpp:210-235).  The strips below sigma^L are not covered, as in the reference.
+Rule adapted1(const std::vector<double>& g, const std::vector<double>& gw, int L, double sigma)
+{
+    Rule r;
+    std::vector<double> br;
+    for (int i = 0; i < L; ++i) br.push_back(std::pow(sigma, (double)(L - i)));
+    br.push_back(1.0);
+    for (int i = 0; i < L; ++i)
+        for (int j = 0; j < L; ++j) append_box(r, g, gw, br[i], br[i + 1], br[j], br[j + 1]);
+    return r;
+}
+
+// AdaptedGauss2: three boxes per level graded toward the corner (0, 1); AdaptedGauss3: toward (1, 0)
+// (lib/bem2d_quadpoints.cpp:240-318).
+Rule adapted23(const std::vector<double>& g, const std::vector<double>& gw, int L, double sigma,
+               bool toward_10)
+{
+    Rule r;
+    if (!toward_10) {
+        double a = sigma, b = 1.0;
+        for (int l = 0; l < L; ++l) {
+            append_box(r, g, gw, 0, a, 1 - b, 1 - a);
+            append_box(r, g, gw, a, b, 1 - b, 1 - a);
+            append_box(r, g, gw, a, b, 1 - a, 1);
+            a *= sigma;
+            b *= sigma;
+        }
+    } else {
+        double a = 1.0, b = sigma;
+        for (int l = 0; l < L; ++l) {
+            append_box(r, g, gw, 1 - a, 1 - b, 0, b);
+            append_box(r, g, gw, 1 - a, 1 - b, b, a);
+            append_box(r, g, gw, 1 - b, 1, b, a);
+            a *= sigma;
+            b *= sigma;
+        }
+    }
+    return r;
+}
+
+} // namespace
+
+// ------------------------------------------------------------------------------------------------
+// mesh
+// ------------------------------------------------------------------------------------------------
+extern "C" int bem2d_mesh_create(bem2d_ctx* c, const bem2d_geometry* g, bem2d_quad_option q,
+                                 bem2d_mesh** out)
+{
+    if (!c || !g || !out) return FAIL(BEM2D_ERR_ARG, "bem2d_mesh_create: null argument");
+    if (g->n < 1) return FAIL(BEM2D_ERR_ARG, "bem2d_mesh_create: empty geometry");
+    if (q.N < 1 || q.N > kMaxNq)
+        return FAIL(BEM2D_ERR_ARG, "QuadOption.N = %d outside [1, %d]", q.N, kMaxNq);
+    if (q.L < 0 || q.L > 12) return FAIL(BEM2D_ERR_ARG, "QuadOption.L = %d outside [0, 12]", q.L);
+    if (!(q.sigma > 0.0 && q.sigma < 1.0))
+        return FAIL(BEM2D_ERR_ARG, "QuadOption.sigma = %g outside (0, 1)", q.sigma);
+    if (!g->type || !g->param || !g->index || !g->next || !g->prev)
+        return FAIL(BEM2D_ERR_ARG, "bem2d_mesh_create: null geometry array");
+    const int64_t n = g->n;
+    for (int64_t e = 0; e < n; ++e)
+        if (g->type[e] < BEM2D_ELEM_FLAT || g->type[e] > BEM2D_ELEM_INV_ELLIPSE)
+            return FAIL(BEM2D_ERR_UNSUPPORTED, "element %lld has unknown type %d", (long long)e,
+                        g->type[e]);
+    HIPCHK(hipSetDevice(c->device));
+
+    std::vector<double> gx, gw;
+    bem2d_internal_gauss01(q.N, gx, gw);
+    Rule r1 = adapted1(gx, gw, q.L, q.sigma);
+    Rule r2 = adapted23(gx, gw, q.L, q.sigma, false);
+    Rule r3 = adapted23(gx, gw, q.L, q.sigma, true);
+
+    // Special (coincident / adjacent) entries with the precedence of lib/bem2d_quadrature.h:271-287:
+    // identical index -> rule 1; else next(i) == index(j) -> rule 3; else prev(i) == index(j) -> 2.
+    std::unordered_map<int64_t, std::vector<int64_t>> pos;
+    pos.reserve((size_t)n * 2);
+    for (int64_t e = 0; e < n; ++e) pos[g->index[e]].push_back(e);
+    std::vector<int64_t> special;
+    std::vector<int64_t> seen;
+    for (int64_t i = 0; i < n; ++i) {
+        seen.clear();
+        auto add = [&](int64_t idx, int rule) {
+            auto it = pos.find(idx);
+            if (it == pos.end()) return;
+            for (int64_t j : it->second) {
+                bool dup = false;
+                for (int64_t s : seen) dup |= (s == j);
+                if (dup) continue;
+                seen.push_back(j);
+                special.push_back(i);
+                special.push_back(j);
+                special.push_back(rule);
+            }
+        };
+        add(g->index[i], 1);
+        add(g->next[i], 3);
+        add(g->prev[i], 2);
+    }
+
+    bem2d_mesh* m = new bem2d_mesh();
+    m->ctx = c;
+    m->n = n;
+    m->opts = q;
+    m->nq = q.N;
+    m->m1 = (int)r1.w.size();
+    m->m2 = (int)r2.w.size();
+    m->m3 = (int)r3.w.size();
+    m->nspecial = (int64_t)special.size() / 3;
+    m->h_type.assign(g->type, g->type + n);
+    m->h_param.assign(g->param, g->param + 8 * n);
+
+    std::vector<double> rules;
+    for (const Rule* r : {&r1, &r2, &r3}) {
+        rules.insert(rules.end(), r->x.begin(), r->x.end());
+        rules.insert(rules.end(), r->y.begin(), r->y.end());
+        rules.insert(rules.end(), r->w.begin(), r->w.end());
+    }
+    std::vector<double> g1d(gx);
+    g1d.insert(g1d.end(), gw.begin(), gw.end());
+
+    auto cleanup = [&](int code, const char* what) {
+        bem2d_mesh_destroy(m);
+        return FAIL(code, "bem2d_mesh_create: %s", what);
+    };
+    size_t bytes_nodes = sizeof(double) * NODE_PLANES * (size_t)n * q.N;
+    if (hipMalloc(&m->d_type, sizeof(int32_t) * n) != hipSuccess ||
+        hipMalloc(&m->d_param, sizeof(double) * 8 * n) != hipSuccess ||
+        hipMalloc(&m->d_adj, sizeof(int64_t) * 3 * n) != hipSuccess ||
+        hipMalloc(&m->d_nodes, bytes_nodes) != hipSuccess ||
+        hipMalloc(&m->d_rules, sizeof(double) * (rules.size() + 1)) != hipSuccess ||
+        hipMalloc(&m->d_g1d, sizeof(double) * g1d.size()) != hipSuccess ||
+        hipMalloc(&m->d_special, sizeof(int64_t) * (special.size() + 3)) != hipSuccess)
+        return cleanup(BEM2D_ERR_NOMEM, "device allocation failed");
+    std::vector<int64_t> adj(3 * n);
+    std::memcpy(adj.data(), g->index, sizeof(int64_t) * n);
+    std::memcpy(adj.data() + n, g->next, sizeof(int64_t) * n);
+    std::memcpy(adj.data() + 2 * n, g->prev, sizeof(int64_t) * n);
+    hipStream_t s = c->stream;
+    bool ok = hipMemcpyAsync(m->d_type, g->type, sizeof(int32_t) * n, hipMemcpyHostToDevice, s) ==
+                  hipSuccess &&
+              hipMemcpyAsync(m->d_param, g->param, sizeof(double) * 8 * n, hipMemcpyHostToDevice,
+                             s) == hipSuccess &&
+              hipMemcpyAsync(m->d_adj, adj.data(), sizeof(int64_t) * 3 * n, hipMemcpyHostToDevice,
+                             s) == hipSuccess &&
+              (rules.empty() || hipMemcpyAsync(m->d_rules, rules.data(), sizeof(double) * rules.size(),
+                                               hipMemcpyHostToDevice, s) == hipSuccess) &&
+              hipMemcpyAsync(m->d_g1d, g1d.data(), sizeof(double) * g1d.size(),
+                             hipMemcpyHostToDevice, s) == hipSuccess &&
+              (special.empty() ||
+               hipMemcpyAsync(m->d_special, special.data(), sizeof(int64_t) * special.size(),
+                              hipMemcpyHostToDevice, s) == hipSuccess);
+    if (!ok) return cleanup(BEM2D_ERR_DEVICE, "upload failed");
+    const int64_t total = n * q.N;
+    prep_nodes_kernel<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(
+        n, q.N, m->d_type, m->d_param, m->d_g1d, m->d_g1d + q.N, m->d_nodes);
+    if (hipGetLastError() != hipSuccess) return cleanup(BEM2D_ERR_DEVICE, "prep kernel launch");
+    // The host vectors above go out of scope on return: wait for the uploads.
+    if (hipStreamSynchronize(s) != hipSuccess) return cleanup(BEM2D_ERR_DEVICE, "prep kernel");
+    *out = m;
+    return BEM2D_OK;
+}
+
+extern "C" void bem2d_mesh_destroy(bem2d_mesh* m)
+{
+    if (!m) return;
+    (void)hipSetDevice(m->ctx->device);
+    (void)hipStreamSynchronize(m->ctx->stream);
+    (void)hipFree(m->d_type);
+    (void)hipFree(m->d_param);
+    (void)hipFree(m->d_adj);
+    (void)hipFree(m->d_nodes);
+    (void)hipFree(m->d_rules);
+    (void)hipFree(m->d_g1d);
+    (void)hipFree(m->d_special);
+    delete m;
+}
+
+extern "C" int64_t bem2d_mesh_size(const bem2d_mesh* m) { return m ? m->n : -1; }
+extern "C" int64_t bem2d_mesh_special_count(const bem2d_mesh* m) { return m ? m->nspecial : -1; }
+
+// ------------------------------------------------------------------------------------------------
+// assembly
+// ------------------------------------------------------------------------------------------------
+template <int KIND>
+static int launch_assembly(bem2d_ctx* c, const bem2d_mesh* m, double k, double eta, double mscale,
+                           double kscale, double2* A, int64_t lda)
+{
+    hipStream_t s = c->stream;
+    const int64_t n = m->n;
+    const int nq = m->nq;
+    const int64_t* idx = m->d_adj;
+    const int64_t* nxt = m->d_adj + n;
+    const int64_t* prv = m->d_adj + 2 * n;
+    dim3 grid((unsigned)((n + kRowsPerBlock - 1) / kRowsPerBlock),
+              (unsigned)((n + kColsPerBlock - 1) / kColsPerBlock));
+    size_t shm = sizeof(double) * kColsPerBlock * nq * NODE_PLANES;
+    bem2d_internal_prof_begin(c, BEM2D_PROF_ASSEMBLE_REGULAR);
+    if (nq == 16)
+        assemble_regular_kernel<KIND, 16><<<grid, 256, shm, s>>>(n, nq, m->d_nodes, idx, nxt, prv,
+                                                                 k, eta, kscale, A, lda);
+    else
+        assemble_regular_kernel<KIND, 0><<<grid, 256, shm, s>>>(n, nq, m->d_nodes, idx, nxt, prv,
+                                                                k, eta, kscale, A, lda);
+    bem2d_internal_prof_end(c, BEM2D_PROF_ASSEMBLE_REGULAR);
+    if (m->nspecial > 0) {
+        bem2d_internal_prof_begin(c, BEM2D_PROF_ASSEMBLE_SPECIAL);
+        assemble_special_kernel<KIND><<<(unsigned)m->nspecial, 256, 0, s>>>(
+            m->d_special, m->d_type, m->d_param, m->d_rules, m->m1, m->m2, m->m3, m->d_g1d,
+            m->d_g1d + nq, nq, k, eta, kscale, mscale, A, lda);
+        bem2d_internal_prof_end(c, BEM2D_PROF_ASSEMBLE_SPECIAL);
+    }
+    return hipGetLastError() == hipSuccess ? BEM2D_OK : BEM2D_ERR_DEVICE;
+}
+
+extern "C" int bem2d_assemble(bem2d_ctx* c, const bem2d_mesh* m, int kind, double k, double eta,
+                              double mscale, double kscale, void* A, int64_t lda)
+{
+    if (!c || !m || !A) return FAIL(BEM2D_ERR_ARG, "bem2d_assemble: null argument");
+    if (lda < m->n) return FAIL(BEM2D_ERR_ARG, "bem2d_assemble: lda %lld < n %lld",
+                                (long long)lda, (long long)m->n);
+    if (!(k > 0.0) && kind != BEM2D_KERNEL_NONE && kind != BEM2D_KERNEL_ID)
+        return FAIL(BEM2D_ERR_ARG, "bem2d_assemble: wavenumber must be positive (got %g)", k);
+    HIPCHK(hipSetDevice(c->device));
+    double2* out = (double2*)A;
+    int rc;
+    switch (kind) {
+    case BEM2D_KERNEL_NONE: rc = launch_assembly<BEM2D_KERNEL_NONE>(c, m, k, eta, mscale, kscale, out, lda); break;
+    case BEM2D_KERNEL_SL: rc = launch_assembly<BEM2D_KERNEL_SL>(c, m, k, eta, mscale, kscale, out, lda); break;
+    case BEM2D_KERNEL_DL: rc = launch_assembly<BEM2D_KERNEL_DL>(c, m, k, eta, mscale, kscale, out, lda); break;
+    case BEM2D_KERNEL_CDL: rc = launch_assembly<BEM2D_KERNEL_CDL>(c, m, k, eta, mscale, kscale, out, lda); break;
+    case BEM2D_KERNEL_COMBINED: rc = launch_assembly<BEM2D_KERNEL_COMBINED>(c, m, k, eta, mscale, kscale, out, lda); break;
+    case BEM2D_KERNEL_ID: rc = launch_assembly<BEM2D_KERNEL_ID>(c, m, k, eta, mscale, kscale, out, lda); break;
+    default: return FAIL(BEM2D_ERR_UNSUPPORTED, "unknown kernel kind %d", kind);
+    }
+    if (rc) return FAIL(rc, "assembly kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
+    return BEM2D_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
+// right-hand sides, potential, far field
+// ------------------------------------------------------------------------------------------------
+extern "C" int bem2d_rhs(bem2d_ctx* c, const bem2d_mesh* m, int kind, double k, double eta,
+                         const double* dirs, int nrhs, double scale, void* B, int64_t ldb)
+{
+    if (!c || !m || !B || (!dirs && nrhs > 0)) return FAIL(BEM2D_ERR_ARG, "bem2d_rhs: null argument");
+    if (nrhs < 0 || nrhs > 65535 || ldb < m->n) return FAIL(BEM2D_ERR_ARG, "bem2d_rhs: bad sizes");
+    if (kind < BEM2D_FIELD_PLANE || kind > BEM2D_FIELD_COMBINED_PLANE)
+        return FAIL(BEM2D_ERR_UNSUPPORTED, "unknown field kind %d", kind);
+    if (nrhs == 0) return BEM2D_OK;
+    HIPCHK(hipSetDevice(c->device));
+    double* d_dirs = (double*)bem2d_internal_scratch(c, SCRATCH_SMALL, sizeof(double) * 2 * nrhs);
+    if (!d_dirs) return FAIL(BEM2D_ERR_NOMEM, "scratch allocation failed");
+    HIPCHK(hipMemcpyAsync(d_dirs, dirs, sizeof(double) * 2 * nrhs, hipMemcpyHostToDevice, c->stream));
+    dim3 grid((unsigned)((m->n + 255) / 256), (unsigned)nrhs);
+    rhs_kernel<<<grid, 256, 0, c->stream>>>(m->n, m->nq, m->d_nodes, kind, k, eta, d_dirs, scale,
+                                            (double2*)B, ldb);
+    HIPCHK(hipGetLastError());
+    // dirs was staged from pageable host memory: make the call safe to return before the kernel runs
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return BEM2D_OK;
+}
+
+extern "C" int bem2d_eval_single_layer(bem2d_ctx* c, const bem2d_mesh* m, double k,
+                                       const void* coef, const double* pts, int64_t npts, void* out)
+{
+    if (!c || !m || !coef || (!pts && npts > 0) || (!out && npts > 0))
+        return FAIL(BEM2D_ERR_ARG, "bem2d_eval_single_layer: null argument");
+    if (npts <= 0) return BEM2D_OK;
+    HIPCHK(hipSetDevice(c->device));
+    eval_sl_kernel<<<(unsigned)((npts + 127) / 128), 128, 0, c->stream>>>(
+        m->n, m->nq, m->d_nodes, k, (const double2*)coef, pts, npts, (double2*)out);
+    HIPCHK(hipGetLastError());
+    return BEM2D_OK;
+}
+
+extern "C" int bem2d_farfield(bem2d_ctx* c, const bem2d_mesh* m, double k, const void* coef,
+                              int64_t ldc, int nrhs, const double* theta, int64_t nang, void* out)
+{
+    if (!c || !m || !coef || !theta || !out) return FAIL(BEM2D_ERR_ARG, "bem2d_farfield: null argument");
+    if (ldc < m->n || nrhs < 1 || nrhs > 65535 || nang < 1 || !(k > 0))
+        return FAIL(BEM2D_ERR_ARG, "bem2d_farfield: bad sizes");
+    HIPCHK(hipSetDevice(c->device));
+    double2* F = (double2*)bem2d_internal_scratch(c, SCRATCH_FARFIELD, sizeof(double2) * nang * m->n);
+    if (!F) return FAIL(BEM2D_ERR_NOMEM, "scratch allocation failed");
+    const int64_t tot = nang * m->n;
+    farfield_signature_kernel<<<(unsigned)((tot + 255) / 256), 256, 0, c->stream>>>(
+        m->n, m->nq, m->d_nodes, k, theta, nang, F);
+    const double pi = 3.14159265358979323846;
+    const double mag = -1.0 / std::sqrt(8.0 * pi * k);
+    dim3 grid((unsigned)((nang + 127) / 128), (unsigned)nrhs);
+    farfield_combine_kernel<<<grid, 128, 0, c->stream>>>(m->n, F, nang, (const double2*)coef, ldc,
+                                                         mag * std::cos(pi / 4), mag * std::sin(pi / 4),
+                                                         (double2*)out);
+    HIPCHK(hipGetLastError());
+    return BEM2D_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
+// plain FP64 solve: rocSOLVER zgetrf + zgetrs
+// ------------------------------------------------------------------------------------------------
+__global__ void copy_matrix_kernel(int64_t n, const double2* __restrict__ src, int64_t lds,
+                                   double2* __restrict__ dst, int64_t ldd)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t j = blockIdx.y;
+    if (i < n) dst[i + ldd * j] = src[i + lds * j];
+}
+
+extern "C" int bem2d_solve_fp64(bem2d_ctx* c, int64_t n, int64_t nrhs, void* A, int64_t lda,
+                                void* B, int64_t ldb, int preserve_A)
+{
+    if (!c || !A || !B) return FAIL(BEM2D_ERR_ARG, "bem2d_solve_fp64: null argument");
+    if (n < 1 || nrhs < 1 || lda < n || ldb < n || n > 2000000000 || lda > 2000000000)
+        return FAIL(BEM2D_ERR_ARG, "bem2d_solve_fp64: bad sizes");
+    HIPCHK(hipSetDevice(c->device));
+    double2* F = (double2*)A;
+    int64_t ldf = lda;
+    if (preserve_A) {
+        F = (double2*)bem2d_internal_scratch(c, SCRATCH_FACTOR, sizeof(double2) * n * n);
+        if (!F) return FAIL(BEM2D_ERR_NOMEM, "scratch allocation for the factor copy failed");
+        ldf = n;
+        dim3 grid((unsigned)((n + 255) / 256), (unsigned)n);
+        copy_matrix_kernel<<<grid, 256, 0, c->stream>>>(n, (const double2*)A, lda, F, ldf);
+        HIPCHK(hipGetLastError());
+    }
+    int* piv = (int*)bem2d_internal_scratch(c, SCRATCH_PIVOT, sizeof(int) * (n + 1));
+    if (!piv) return FAIL(BEM2D_ERR_NOMEM, "scratch allocation failed");
+    int* info = piv + n;
+    bem2d_internal_prof_begin(c, BEM2D_PROF_LU);
+    rocblas_status st = rocsolver_zgetrf(c->blas, (rocblas_int)n, (rocblas_int)n,
+                                         (rocblas_double_complex*)F, (rocblas_int)ldf, piv, info);
+    bem2d_internal_prof_end(c, BEM2D_PROF_LU);
+    if (st != rocblas_status_success) return FAIL(BEM2D_ERR_DEVICE, "rocsolver_zgetrf status %d", (int)st);
+    int h_info = 0;
+    HIPCHK(hipMemcpyAsync(&h_info, info, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (h_info > 0) return FAIL(BEM2D_ERR_SINGULAR, "singular matrix: U(%d,%d) = 0", h_info, h_info);
+    st = rocsolver_zgetrs(c->blas, rocblas_operation_none, (rocblas_int)n, (rocblas_int)nrhs,
+                          (rocblas_double_complex*)F, (rocblas_int)ldf, piv,
+                          (rocblas_double_complex*)B, (rocblas_int)ldb);
+    if (st != rocblas_status_success) return FAIL(BEM2D_ERR_DEVICE, "rocsolver_zgetrs status %d", (int)st);
+    return BEM2D_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
+// special functions
+// ------------------------------------------------------------------------------------------------
+__global__ void hankel01_kernel(const double* __restrict__ x, int64_t n, double* __restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double a, b, c, d;
+    hankel01(x[i], a, b, c, d);
+    out[4 * i] = a;
+    out[4 * i + 1] = b;
+    out[4 * i + 2] = c;
+    out[4 * i + 3] = d;
+}
+
+extern "C" int bem2d_hankel01(bem2d_ctx* c, const double* x, int64_t n, double* out)
+{
+    if (!c || (n > 0 && (!x || !out)) || n < 0) return FAIL(BEM2D_ERR_ARG, "bem2d_hankel01: bad argument");
+    if (n == 0) return BEM2D_OK;
+    HIPCHK(hipSetDevice(c->device));
+    hankel01_kernel<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(x, n, out);
+    HIPCHK(hipGetLastError());
+    return BEM2D_OK;
+}
+
+extern "C" int bem2d_hankel01_host(const double* x, int64_t n, double* out)
+{
+    if (n < 0 || (n > 0 && (!x || !out))) return FAIL(BEM2D_ERR_ARG, "bem2d_hankel01_host: bad argument");
+    for (int64_t i = 0; i < n; ++i) hankel01(x[i], out[4 * i], out[4 * i + 1], out[4 * i + 2], out[4 * i + 3]);
+    return BEM2D_OK;
+}

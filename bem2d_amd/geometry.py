@@ -1,0 +1,118 @@
+"""Host description of P0 geometries in the C-ABI's element format, and mesh builders.
+
+A Geometry holds, per element (= matrix row/column), the element type, its 8 parameters and the
+index / next / prev adjacency (-1 = none) that selects the singular quadrature rules.  The builders
+mirror the reference's element constructions:
+
+  curve_mesh         AnalyticCurveElement(i/n, (i+1)/n, curve) with ring adjacency, as built by hand
+                     in the reference's drivers (normcomps/cracknormcond.cpp:67-75) -- uniform in
+                     the curve parameter, not by arc length
+  disk_chords        DiskShapePiecewiseConst(n, r)        lib/bem2d_shape.cpp:11-34
+  polygon            Polygon(points, n)                   lib/bem2d_shape.cpp:114-138
+  line               Line(p0, p1, n) with open ends        lib/bem2d_shape.cpp:225-236
+"""
+import math
+
+import numpy as np
+
+from .capi import (ELEM_CIRCLE, ELEM_ELLIPSE, ELEM_ELLIPSE_ARC, ELEM_FLAT, ELEM_INV_ELLIPSE,
+                   ELEM_KITE, ELEM_TREFOIL)
+
+CURVES = {"circle": ELEM_CIRCLE, "trefoil": ELEM_TREFOIL, "kite": ELEM_KITE,
+          "ellipse": ELEM_ELLIPSE, "ellipse_arc": ELEM_ELLIPSE_ARC,
+          "inv_ellipse": ELEM_INV_ELLIPSE}
+
+
+class Geometry:
+    def __init__(self, type_, param, index, next_, prev):
+        self.type = np.ascontiguousarray(type_, dtype=np.int32)
+        self.param = np.ascontiguousarray(param, dtype=np.float64).reshape(-1, 8)
+        self.index = np.ascontiguousarray(index, dtype=np.int64)
+        self.next = np.ascontiguousarray(next_, dtype=np.int64)
+        self.prev = np.ascontiguousarray(prev, dtype=np.int64)
+        self.n = int(self.type.size)
+        assert self.param.shape[0] == self.n == self.index.size == self.next.size == \
+            self.prev.size
+
+    def flat_endpoints(self):
+        """For flat elements: (p1, p2) arrays."""
+        p1 = self.param[:, 0:2]
+        return p1, p1 + self.param[:, 2:4]
+
+
+def _ring(n, closed=True):
+    idx = np.arange(n, dtype=np.int64)
+    nxt = (idx + 1) % n
+    prv = (idx - 1) % n
+    if not closed:
+        nxt[-1] = -1
+        prv[0] = -1
+    return idx, nxt, prv
+
+
+def curve_mesh(curve, n, params=(0.0, 0.0, 0.0, 0.0), closed=True):
+    """n AnalyticCurveElements at uniform curve parameter t_i = i/n."""
+    ty = CURVES[curve] if isinstance(curve, str) else int(curve)
+    param = np.zeros((n, 8))
+    i = np.arange(n, dtype=np.float64)
+    param[:, 0] = i / n
+    param[:, 1] = (i + 1) / n
+    param[:, 2:2 + len(params)] = params
+    return Geometry(np.full(n, ty), param, *_ring(n, closed))
+
+
+def disk_chords(n, radius=1.0):
+    """DiskShapePiecewiseConst: n flat chords of a circle (lib/bem2d_shape.cpp:11-34)."""
+    angle = 2 * math.pi / n
+    pts = [(radius * math.cos(i * angle), radius * math.sin(i * angle)) for i in range(n)]
+    param = np.zeros((n, 8))
+    for i in range(n):
+        p1 = pts[i]
+        p2 = pts[(i + 1) % n]
+        param[i, :4] = (p1[0], p1[1], p2[0] - p1[0], p2[1] - p1[1])
+    return Geometry(np.full(n, ELEM_FLAT), param, *_ring(n))
+
+
+def polygon(points, per_side):
+    """Polygon(points, n): n flat elements per side, closed ring (lib/bem2d_shape.cpp:114-138)."""
+    m = len(points)
+    pts = [tuple(map(float, p)) for p in points] + [tuple(map(float, points[0]))]
+    param = np.zeros((m * per_side, 8))
+    for s in range(m):
+        dx = pts[s + 1][0] - pts[s][0]
+        dy = pts[s + 1][1] - pts[s][1]
+        for j in range(per_side):
+            # p1 = p + (j/n) d, p2 = p + ((j+1)/n) d, stored as p1 and p2 - p1
+            p1 = (pts[s][0] + (1.0 * j) / per_side * dx, pts[s][1] + (1.0 * j) / per_side * dy)
+            p2 = (pts[s][0] + (1.0 * (j + 1)) / per_side * dx,
+                  pts[s][1] + (1.0 * (j + 1)) / per_side * dy)
+            param[s * per_side + j, :4] = (p1[0], p1[1], p2[0] - p1[0], p2[1] - p1[1])
+    return Geometry(np.full(m * per_side, ELEM_FLAT), param, *_ring(m * per_side))
+
+
+def line(p0, p1, n):
+    """Line(p0, p1, n): n flat elements, open ends (-1 sentinels) (lib/bem2d_shape.cpp:225-236)."""
+    d = (p1[0] - p0[0], p1[1] - p0[1])
+    param = np.zeros((n, 8))
+    for j in range(n):
+        a = (p0[0] + (1.0 * j) / n * d[0], p0[1] + (1.0 * j) / n * d[1])
+        b = (p0[0] + (1.0 * (j + 1)) / n * d[0], p0[1] + (1.0 * (j + 1)) / n * d[1])
+        param[j, :4] = (a[0], a[1], b[0] - a[0], b[1] - a[1])
+    return Geometry(np.full(n, ELEM_FLAT), param, *_ring(n, closed=False))
+
+
+# ---- the workloads the benchmarks and parity tests use ---------------------------------------
+
+def unit_circle(n):
+    return curve_mesh("circle", n)
+
+
+def kite(n):
+    """Kite x = cos t + 0.65 cos 2t - 0.65, y = 1.5 sin t (lib/bem2d_curvetypes.h:39-48)."""
+    return curve_mesh("kite", n)
+
+
+def open_arc(n, aperture=0.3, radius=1.0):
+    """Open circular arc of the given aperture centred on +x: EllipseArc(r, r, a/2, 2 pi - a/2)."""
+    return curve_mesh("ellipse_arc", n, (radius, radius, aperture / 2, 2 * math.pi - aperture / 2),
+                      closed=False)

@@ -1,0 +1,179 @@
+/*
+ * bem2d_hip.h -- C-ABI of the MI355X (gfx950) implementation of the bem2d hot path:
+ * dense Galerkin assembly of the Helmholtz single-layer, double-layer, adjoint double-layer and
+ * combined-field operators with piecewise-constant bases, the plane-wave right-hand sides, the
+ * dense complex solve behind SoundSoftScattering, and the potential / far-field evaluation.
+ *
+ * Plain C types only.  Every entry point returns a status (BEM2D_OK = 0); bem2d_last_error()
+ * returns the message of the calling thread's last failure.  The C++ drop-in headers under
+ * include/bem2d/ map the statuses back onto the reference's exception tags
+ * (lib/bem2d_exceptions.h:10-33).
+ *
+ * Ownership: the caller owns every buffer it passes.  Matrices, right-hand sides and solutions are
+ * device pointers (complex128 = interleaved re, im; column-major with the stated leading
+ * dimension) unless a parameter says otherwise.  A context owns a HIP stream (or borrows one via
+ * bem2d_ctx_set_stream), a rocSOLVER handle and scratch memory; a mesh owns its device copy of the
+ * geometry and the quadrature rules.  Calls on one context are issued in stream order; calls on one
+ * context from several host threads must be serialised by the caller.
+ */
+#ifndef BEM2D_HIP_H
+#define BEM2D_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------------------------------- */
+#define BEM2D_OK 0
+#define BEM2D_ERR_ARG 1           /* bad argument: ParameterException / SizeError / ArrayMismatch */
+#define BEM2D_ERR_SINGULAR 2      /* exactly singular U in the LU: LapackError */
+#define BEM2D_ERR_NOMEM 3         /* device allocation failed */
+#define BEM2D_ERR_DEVICE 4        /* HIP / rocSOLVER failure */
+#define BEM2D_ERR_UNSUPPORTED 5   /* element type, kernel or field kind not implemented */
+#define BEM2D_ERR_NOT_CONVERGED 6 /* mixed-precision refinement missed its tolerance */
+
+/* ---- element types (per element of a geometry) ------------------------------------------------
+ * FLAT:      ConstElement(p1, p2)        param = (p1.x, p1.y, d.x, d.y, 0, 0, 0, 0), d = p2 - p1
+ *            (lib/bem2d_element.h:72-94)
+ * others:    AnalyticCurveElement(tstart, tend, curve)  (lib/bem2d_element.h:96-116)
+ *            param = (tstart, tend, c0, c1, c2, c3, 0, 0) with the curve's constructor arguments
+ *            CIRCLE (none), TREFOIL (none), KITE (none), ELLIPSE (rho), ELLIPSE_ARC (a, b, theta1,
+ *            theta2), INV_ELLIPSE (alpha)            (lib/bem2d_curvetypes.h:13-99)            */
+#define BEM2D_ELEM_FLAT 0
+#define BEM2D_ELEM_CIRCLE 1
+#define BEM2D_ELEM_TREFOIL 2
+#define BEM2D_ELEM_KITE 3
+#define BEM2D_ELEM_ELLIPSE 4
+#define BEM2D_ELEM_ELLIPSE_ARC 5
+#define BEM2D_ELEM_INV_ELLIPSE 6
+
+/* ---- kernel kinds (lib/bem2d_kernel.h:12-98) -------------------------------------------------- */
+#define BEM2D_KERNEL_NONE 0     /* no kernel term: the matrix is mass_scale * M (EvalIdent) */
+#define BEM2D_KERNEL_SL 1       /* SingleLayer             (i/4) H0(kd) */
+#define BEM2D_KERNEL_DL 2       /* DoubleLayer             -(ik/4) H1(kd)/d n_y.(y-x) */
+#define BEM2D_KERNEL_CDL 3      /* ConjDoubleLayer         -(ik/4) H1(kd)/d n_x.(x-y) */
+#define BEM2D_KERNEL_COMBINED 4 /* CombinedSingleConjDouble  CDL - i eta SL */
+#define BEM2D_KERNEL_ID 5       /* IdKernel                1 */
+
+/* ---- field kinds (lib/bem2d_fun.h:14-93) ------------------------------------------------------ */
+#define BEM2D_FIELD_PLANE 0          /* PlaneWave          exp(ik d.x) */
+#define BEM2D_FIELD_NORMAL_PLANE 1   /* NormalPlaneWave    ik (n.d) exp(ik d.x) */
+#define BEM2D_FIELD_COMBINED_PLANE 2 /* CombinedPlaneWave  ik (n.d) exp(ik d.x) - i eta exp(ik d.x) */
+
+typedef struct bem2d_ctx bem2d_ctx;
+typedef struct bem2d_mesh bem2d_mesh;
+
+/* Host-side description of a P0 geometry: element i is matrix row/column i (the FlatMap order of
+ * lib/bem2d_geometry.h:48-57 with one PolBasis(0) per element).  index/next/prev mirror
+ * Element::index/next/prev (lib/bem2d_element.h:36-58); -1 means "no neighbour". */
+typedef struct {
+    int64_t n;
+    const int32_t* type;   /* [n] BEM2D_ELEM_* */
+    const double* param;   /* [8 n] */
+    const int64_t* index;  /* [n] */
+    const int64_t* next;   /* [n] */
+    const int64_t* prev;   /* [n] */
+} bem2d_geometry;
+
+/* QuadOption (lib/bem2d_quadrature.h:26-30): N Gauss points per element, L grading levels and
+ * grading ratio sigma of the singular rules. */
+typedef struct {
+    int L;
+    int N;
+    double sigma;
+} bem2d_quad_option;
+
+/* ---- context ----------------------------------------------------------------------------------- */
+int bem2d_ctx_create(bem2d_ctx** out, int device);
+void bem2d_ctx_destroy(bem2d_ctx* ctx);
+/* Borrow a caller stream (hipStream_t, NULL = the context's own stream). */
+int bem2d_ctx_set_stream(bem2d_ctx* ctx, void* stream);
+void* bem2d_ctx_stream(bem2d_ctx* ctx);
+int bem2d_synchronize(bem2d_ctx* ctx);
+const char* bem2d_last_error(void);
+
+/* Device memory helpers, so that host code above this layer needs no HIP headers. */
+int bem2d_device_alloc(bem2d_ctx* ctx, int64_t bytes, void** out);
+int bem2d_device_free(bem2d_ctx* ctx, void* p);
+int bem2d_copy_to_device(bem2d_ctx* ctx, void* dst, const void* src, int64_t bytes);
+int bem2d_copy_to_host(bem2d_ctx* ctx, void* dst, const void* src, int64_t bytes);
+
+/* ---- mesh: device copy of the geometry, the Gauss-node tables and the singular rules ---------- */
+/* Replaces Geometry::FlatMap + the rule construction at the top of DiscreteKernel
+ * (lib/bem2d_quadrature.h:235-242): Gauss2D(N), AdaptedGauss1/2/3(N, L, sigma). */
+int bem2d_mesh_create(bem2d_ctx* ctx, const bem2d_geometry* geom, bem2d_quad_option opts,
+                      bem2d_mesh** out);
+void bem2d_mesh_destroy(bem2d_mesh* mesh);
+int64_t bem2d_mesh_size(const bem2d_mesh* mesh);
+/* Number of coincident / adjacent (singular-rule) entries of the mesh. */
+int64_t bem2d_mesh_special_count(const bem2d_mesh* mesh);
+
+/* ---- assembly ---------------------------------------------------------------------------------- */
+/* A[i + lda*j] = mass_scale * M_ij + kernel_scale * K_ij for all 0 <= i, j < n, where
+ *   K = DiscreteKernel(geom, opts, kernel)   lib/bem2d_quadrature.h:231-294 (rule selection
+ *       identical -> AdaptedGauss1 with the Duffy split of IntegrateIdent (:67-109), next ->
+ *       AdaptedGauss3, prev -> AdaptedGauss2, otherwise Gauss2D; Integrate :37-65)
+ *   M = EvalIdent(geom, opts)                lib/bem2d_quadrature.cpp:79-127
+ * SoundSoftScattering::DiscretizeMatrix (lib/bem2d_soundsoftscattering.h:95-103) is
+ * kernel = COMBINED, eta = the field's eta, mass_scale = 1, kernel_scale = 2.
+ * k is a real wavenumber (freqtype with im == 0); A is a device pointer. */
+int bem2d_assemble(bem2d_ctx* ctx, const bem2d_mesh* mesh, int kernel_kind, double k, double eta,
+                   double mass_scale, double kernel_scale, void* A, int64_t lda);
+
+/* ---- right-hand sides -------------------------------------------------------------------------- */
+/* B[i + ldb*r] = scale * DotProdBasFuns(geom, opts, field(dirs[r]))_i   (lib/bem2d_quadrature.h:
+ * 111-156) for r < nrhs; dirs is a HOST array of nrhs (x, y) directions (the field's direction,
+ * used as given).  DiscretizeRhs (lib/bem2d_soundsoftscattering.h:105-110) is scale = 2. */
+int bem2d_rhs(bem2d_ctx* ctx, const bem2d_mesh* mesh, int field_kind, double k, double eta,
+              const double* dirs, int nrhs, double scale, void* B, int64_t ldb);
+
+/* ---- plain FP64 solve -------------------------------------------------------------------------- */
+/* SolveSystem (lib/bem2d_mathroutines.cpp:323-351): LU with partial pivoting of A, then the
+ * solve for nrhs right-hand sides; B is overwritten with the solution.  With preserve_A != 0, A
+ * is left unchanged (the reference factors a copy); otherwise A is overwritten by its factors.
+ * Returns BEM2D_ERR_SINGULAR when U has an exactly zero pivot (zgetrf info > 0). */
+int bem2d_solve_fp64(bem2d_ctx* ctx, int64_t n, int64_t nrhs, void* A, int64_t lda, void* B,
+                     int64_t ldb, int preserve_A);
+
+/* ---- potential and far field ------------------------------------------------------------------- */
+/* out[p] = sum_j coef[j] * int_{elem j} (i/4) H0(k |P_p - y|) ds(y) with Gauss1D(N):
+ * EvalKernel<SingleLayer> (lib/bem2d_quadrature.h:184-228) as used by EvalSol
+ * (lib/bem2d_soundsoftscattering.h:149-168; EvalSol = u_inc - out).  coef, pts (2 doubles per
+ * point) and out are device pointers. */
+int bem2d_eval_single_layer(bem2d_ctx* ctx, const bem2d_mesh* mesh, double k, const void* coef,
+                            const double* pts, int64_t npts, void* out);
+
+/* Far field of u = u_inc - S phi (not in the reference; derived from EvalSol's representation):
+ * out[a + nang*r] = -(exp(i pi/4)/sqrt(8 pi k)) sum_j coef[j + ldc*r] sum_t
+ *                   exp(-i k (cos th_a, sin th_a).y_t) |Gamma_j'(t)| w_t.
+ * coef, theta and out are device pointers. */
+int bem2d_farfield(bem2d_ctx* ctx, const bem2d_mesh* mesh, double k, const void* coef,
+                   int64_t ldc, int nrhs, const double* theta, int64_t nang, void* out);
+
+/* ---- kernel timing ------------------------------------------------------------------------------
+ * With profiling on, the context records HIP events around each launch of the instrumented kernels
+ * on the stream they run on.  bem2d_profile_read synchronises and returns the summed duration (ms)
+ * and the number of launches since the last reset.  Kernel ids: */
+#define BEM2D_PROF_ASSEMBLE_REGULAR 0 /* regular-pair assembly kernel */
+#define BEM2D_PROF_ASSEMBLE_SPECIAL 1 /* coincident / adjacent assembly kernel */
+#define BEM2D_PROF_LU 2               /* LU factorisation (FP64 rocSOLVER or the FP32 MFMA LU) */
+#define BEM2D_PROF_COUNT 3
+int bem2d_profile_enable(bem2d_ctx* ctx, int on);
+int bem2d_profile_reset(bem2d_ctx* ctx);
+int bem2d_profile_read(bem2d_ctx* ctx, int kernel_id, double* total_ms, int64_t* launches);
+
+/* ---- special functions ------------------------------------------------------------------------- */
+/* BesselH0(double), BesselH1(double) (lib/bem2d_mathroutines.h:19-20, .cpp:20-42) for n real
+ * arguments x > 0: out[4 i .. 4 i + 3] = Re H0, Im H0, Re H1, Im H1 of x[i].
+ * bem2d_hankel01 runs on the device (x, out device pointers); bem2d_hankel01_host runs the same
+ * FP64 routine on the host (host pointers) and needs no GPU. */
+int bem2d_hankel01(bem2d_ctx* ctx, const double* x, int64_t n, double* out);
+int bem2d_hankel01_host(const double* x, int64_t n, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BEM2D_HIP_H */

@@ -1,0 +1,199 @@
+"""GPU parity of the HIP path (through the C-ABI) against the CPU oracle and the golden vectors.
+
+Tolerances: the plain FP64 path must reproduce the oracle's matrices, right-hand sides and
+solutions to a relative Frobenius error of 1e-12 (the bar of the plain method); the device
+Hankel routine is held to 1e-15 relative to |H| against mpmath.
+"""
+import numpy as np
+import pytest
+import torch
+
+from bem2d_amd import capi
+from bem2d_amd import geometry as G
+from conftest import analytic_farfield, golden, rel_fro
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def _np(t):
+    return t.cpu().numpy()
+
+
+def test_device_hankel_matches_mpmath(ctx):
+    g = golden("hankel_mpmath.npz")
+    x = torch.tensor(g["x"], device="cuda")
+    out = _np(ctx.hankel01(x))
+    torch.cuda.synchronize()
+    v = g["j0y0j1y1"]
+    e0 = np.hypot(out[:, 0] - v[:, 0], out[:, 1] - v[:, 1]) / np.hypot(v[:, 0], v[:, 1])
+    e1 = np.hypot(out[:, 2] - v[:, 2], out[:, 3] - v[:, 3]) / np.hypot(v[:, 2], v[:, 3])
+    assert e0.max() < 1e-15 and e1.max() < 1e-15, (e0.max(), e1.max())
+
+
+CASES = [
+    # geometry, (N, L, sigma), kind, k, eta, mass_scale, kernel_scale
+    ("circle200_config1", lambda: G.unit_circle(200), (16, 3, 0.15), capi.KERNEL_COMBINED, 10.0,
+     10.0, 1.0, 2.0),
+    ("circle40_sl", lambda: G.unit_circle(40), (16, 3, 0.15), capi.KERNEL_SL, 3.0, 0.0, 0.0, 1.0),
+    ("circle40_dl", lambda: G.unit_circle(40), (16, 3, 0.15), capi.KERNEL_DL, 3.0, 0.0, 0.0, 1.0),
+    ("circle40_cdl", lambda: G.unit_circle(40), (16, 3, 0.15), capi.KERNEL_CDL, 3.0, 0.0, 0.0,
+     1.0),
+    ("circle40_id", lambda: G.unit_circle(40), (16, 3, 0.15), capi.KERNEL_ID, 3.0, 0.0, 0.0, 1.0),
+    ("circle40_mass", lambda: G.unit_circle(40), (16, 3, 0.15), capi.KERNEL_NONE, 3.0, 0.0, 1.0,
+     0.0),
+    ("polygon_q5_default", lambda: G.polygon([(0, 0), (1.5, 0.2), (0.4, 1.1), (-0.3, 0.6)], 9),
+     (5, 3, 0.15), capi.KERNEL_COMBINED, 12.0, 12.0, 1.0, 2.0),
+    ("chords_q30", lambda: G.disk_chords(48, 1.3), (30, 7, 0.15), capi.KERNEL_COMBINED, 6.0, 6.0,
+     1.0, 2.0),
+    ("open_arc", lambda: G.open_arc(80), (16, 3, 0.15), capi.KERNEL_COMBINED, 25.0, 25.0, 1.0,
+     2.0),
+    ("kite_dl", lambda: G.kite(150), (16, 3, 0.15), capi.KERNEL_DL, 40.0, 0.0, 0.0, 1.0),
+    ("ellipse_sl", lambda: G.curve_mesh("ellipse", 70, (1.7,)), (8, 4, 0.2), capi.KERNEL_SL, 9.0,
+     0.0, 0.0, 1.0),
+    ("ellipse_arc_full", lambda: G.curve_mesh("ellipse_arc", 90, (1.4, 0.6, 0.0, 2 * np.pi)),
+     (16, 3, 0.15), capi.KERNEL_COMBINED, 30.0, 30.0, 1.0, 2.0),
+    ("trefoil_cdl", lambda: G.curve_mesh("trefoil", 64), (12, 3, 0.15), capi.KERNEL_CDL, 7.0,
+     0.0, 0.0, 1.0),
+    ("inv_ellipse_sl", lambda: G.curve_mesh("inv_ellipse", 48, (0.3,)), (10, 3, 0.15),
+     capi.KERNEL_SL, 5.0, 0.0, 0.0, 1.0),
+    ("line_open", lambda: G.line((0.0, 0.0), (0.0, 1.0), 60), (16, 3, 0.15), capi.KERNEL_SL, 20.0,
+     0.0, 1.0, 1.0),
+    ("two_elements", lambda: G.unit_circle(2), (16, 3, 0.15), capi.KERNEL_COMBINED, 2.0, 2.0, 1.0,
+     2.0),
+    ("single_element_L0", lambda: G.line((0.0, 0.0), (1.0, 0.0), 1), (4, 0, 0.15),
+     capi.KERNEL_SL, 2.0, 0.0, 1.0, 1.0),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_assembly_matches_oracle(ctx, oracle, case):
+    name, build, (N, L, sigma), kind, k, eta, ms, ks = case
+    print("case", name, flush=True)
+    geom = build()
+    mesh = ctx.mesh(geom, N, L, sigma)
+    A = _np(ctx.assemble(mesh, kind, k, eta, ms, ks))
+    ref = oracle.assemble(geom, N, L, sigma, kind, k, eta, ms, ks)
+    if np.linalg.norm(ref) == 0:
+        assert np.linalg.norm(A) == 0
+    else:
+        assert rel_fro(A, ref) < TOL, (name, rel_fro(A, ref))
+    assert np.all(np.isfinite(A))
+
+
+def test_entries_match_mpmath_fixture(ctx):
+    f = golden("entries_mpmath.npz")
+    for name in ("circle4", "kite5", "triangle6", "arc4open", "circle3_q16"):
+        geom = G.Geometry(f[name + "_type"], f[name + "_param"], f[name + "_index"],
+                          f[name + "_next"], f[name + "_prev"])
+        N, L, sigma = f[name + "_quad"]
+        k, eta = f[name + "_keta"]
+        mesh = ctx.mesh(geom, int(N), int(L), float(sigma))
+        for key in f.files:
+            if not key.startswith(name + "_kind"):
+                continue
+            kind = int(key.rsplit("kind", 1)[1])
+            ms, ks = (1.0, 0.0) if kind == 0 else (0.0, 1.0)
+            A = _np(ctx.assemble(mesh, kind, float(k), float(eta), ms, ks))
+            ref = f[key]
+            assert np.abs(A - ref).max() / np.abs(ref).max() < 2e-14, (name, kind)
+
+
+@pytest.mark.parametrize("kind", [capi.FIELD_PLANE, capi.FIELD_NORMAL_PLANE,
+                                  capi.FIELD_COMBINED_PLANE])
+def test_rhs_matches_oracle(ctx, oracle, kind):
+    geom = G.kite(300)
+    mesh = ctx.mesh(geom, 16, 3, 0.15)
+    rng = np.random.default_rng(5)
+    ang = rng.uniform(0, 2 * np.pi, 7)
+    dirs = np.stack([np.cos(ang), np.sin(ang)], axis=1)
+    B = _np(ctx.rhs(mesh, kind, 30.0, 30.0, dirs, 2.0))
+    ref = oracle.rhs(geom, 16, kind, 30.0, 30.0, dirs, 2.0)
+    assert rel_fro(B, ref) < TOL
+
+
+def _config1(ctx, oracle):
+    k = 10.0
+    geom = G.unit_circle(200)
+    mesh = ctx.mesh(geom, 16, 3, 0.15)
+    A = ctx.assemble(mesh, capi.KERNEL_COMBINED, k, k, 1.0, 2.0)
+    B = ctx.rhs(mesh, capi.FIELD_COMBINED_PLANE, k, k, [[1.0, 0.0]], 2.0)
+    return k, geom, mesh, A, B
+
+
+def test_solve_matches_oracle_config1(ctx, oracle):
+    k, geom, mesh, A, B = _config1(ctx, oracle)
+    A_before = A.clone()
+    X = ctx.solve_fp64(A, B.clone(), preserve_A=True)
+    assert torch.equal(A, A_before)  # SolveSystem factors a copy
+    Ar = oracle.assemble(geom, 16, 3, 0.15, oracle.K_COMBINED, k, k, 1.0, 2.0)
+    br = oracle.rhs(geom, 16, oracle.F_COMBINED_PLANE, k, k, [1.0, 0.0], 2.0)
+    xr = oracle.lu_solve(Ar, br)
+    assert rel_fro(_np(X), xr) < TOL
+
+
+def test_farfield_config1(ctx, oracle):
+    k, geom, mesh, A, B = _config1(ctx, oracle)
+    X = ctx.solve_fp64(A, B.clone())
+    th = torch.tensor(2 * np.pi * np.arange(360) / 360, device="cuda")
+    ff = _np(ctx.farfield(mesh, k, X, th))[:, 0]
+    ser = analytic_farfield(k, _np(th))
+    assert np.abs(ff - ser).max() / np.abs(ser).max() < 2.5e-3  # discretisation error
+    ref = oracle.farfield(geom, 16, k, _np(X), _np(th))[:, 0]
+    assert rel_fro(ff, ref) < TOL
+
+
+def test_eval_single_layer_matches_oracle(ctx, oracle):
+    k, geom, mesh, A, B = _config1(ctx, oracle)
+    X = ctx.solve_fp64(A, B.clone())
+    rng = np.random.default_rng(2)
+    pts = np.concatenate([rng.uniform(-3, 3, (40, 2)), [[-1.5, 0.0], [1.2, 0.3]]])
+    pts = pts[np.hypot(pts[:, 0], pts[:, 1]) > 1.05]
+    v = _np(ctx.eval_single_layer(mesh, k, X[:, 0], torch.tensor(pts, device="cuda")))
+    ref = oracle.eval_sl(geom, 16, k, _np(X)[:, 0], pts)
+    assert rel_fro(v, ref) < TOL
+
+
+def test_singular_matrix_raises(ctx):
+    A = torch.ones((4, 4), dtype=torch.complex128, device="cuda")
+    B = torch.ones((4, 1), dtype=torch.complex128, device="cuda")
+    with pytest.raises(capi.Bem2dError) as e:
+        ctx.solve_fp64(A, B)
+    assert e.value.code == capi.ERR_SINGULAR
+
+
+def test_bad_arguments_raise(ctx):
+    geom = G.unit_circle(8)
+    with pytest.raises(capi.Bem2dError) as e:
+        ctx.mesh(geom, 0, 3, 0.15)
+    assert e.value.code == capi.ERR_ARG
+    bad = G.unit_circle(8)
+    bad.type[3] = 42
+    with pytest.raises(capi.Bem2dError) as e:
+        ctx.mesh(bad, 16, 3, 0.15)
+    assert e.value.code == capi.ERR_UNSUPPORTED
+
+
+def test_kite_config2_full_size(ctx, oracle):
+    """Config 2 at its full size (kite, N = 4000, k = 100): sampled columns against the oracle,
+    the complete matrix through size-independent properties, and the FP64 solve's backward error."""
+    n, k = 4000, 100.0
+    geom = G.kite(n)
+    mesh = ctx.mesh(geom, 16, 3, 0.15)
+    A = ctx.assemble(mesh, capi.KERNEL_COMBINED, k, k, 1.0, 2.0)
+    cols = np.array([0, 1, 777, 1999, 2000, 3998, 3999])
+    ref = oracle.assemble(geom, 16, 3, 0.15, oracle.K_COMBINED, k, k, 1.0, 2.0, cols=cols)
+    got = _np(A[:, torch.tensor(cols, device="cuda")])
+    assert rel_fro(got, ref) < TOL
+    assert torch.isfinite(torch.view_as_real(A)).all()
+    # linearity in the scales: A(1, 2) = M + 2 K
+    K = ctx.assemble(mesh, capi.KERNEL_COMBINED, k, k, 0.0, 1.0)
+    M = ctx.assemble(mesh, capi.KERNEL_NONE, k, k, 1.0, 0.0)
+    assert (torch.linalg.norm(A - (M + 2 * K)) / torch.linalg.norm(A)).item() < 1e-15
+    rng = np.random.default_rng(0)
+    th = 2 * np.pi * rng.uniform()
+    B = ctx.rhs(mesh, capi.FIELD_COMBINED_PLANE, k, k, [[np.cos(th), np.sin(th)]], 2.0)
+    X = ctx.solve_fp64(A, B.clone())
+    r = torch.linalg.norm(B - A @ X) / (torch.linalg.norm(A) * torch.linalg.norm(X) +
+                                         torch.linalg.norm(B))
+    assert r.item() < 1e-15
