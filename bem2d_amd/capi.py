@@ -45,6 +45,7 @@ SIGNATURES = {
     "bem2d_ctx_create": (_I, [ctypes.POINTER(_P), _I]),
     "bem2d_ctx_destroy": (None, [_P]),
     "bem2d_ctx_set_stream": (_I, [_P, _P]),
+    "bem2d_ctx_reset_stream": (_I, [_P]),
     "bem2d_ctx_stream": (_P, [_P]),
     "bem2d_synchronize": (_I, [_P]),
     "bem2d_last_error": (ctypes.c_char_p, []),
@@ -64,6 +65,8 @@ SIGNATURES = {
     "bem2d_profile_enable": (_I, [_P, _I]),
     "bem2d_profile_reset": (_I, [_P]),
     "bem2d_profile_read": (_I, [_P, _I, ctypes.POINTER(_D), ctypes.POINTER(_I64)]),
+    "bem2d_gauss_legendre": (_I, [_I, _P, _P]),
+    "bem2d_quad_rule": (_I, [_I, _I, _I, _D, _P, _P, _P]),
     "bem2d_hankel01": (_I, [_P, _P, _I64, _P]),
     "bem2d_hankel01_host": (_I, [_P, _I64, _P]),
 }
@@ -112,6 +115,17 @@ def hankel01_host(x):
     out = np.zeros((x.size, 4))
     check(lib().bem2d_hankel01_host(_np_ptr(x), x.size, _np_ptr(out)))
     return out[:, 0] + 1j * out[:, 1], out[:, 2] + 1j * out[:, 3]
+
+
+def quad_rule(which, N, L=3, sigma=0.15):
+    """Host rule of the product: Gauss2D (0), AdaptedGauss1 (1), AdaptedGauss2 (2), AdaptedGauss3
+    (3) -- the tables the device assembly uses."""
+    m = lib().bem2d_quad_rule(which, N, L, sigma, None, None, None)
+    if m < 0:
+        check(ERR_ARG)
+    x, y, w = np.zeros(m), np.zeros(m), np.zeros(m)
+    lib().bem2d_quad_rule(which, N, L, sigma, _np_ptr(x), _np_ptr(y), _np_ptr(w))
+    return x, y, w
 
 
 class Context:

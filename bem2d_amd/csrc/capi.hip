@@ -83,7 +83,16 @@ extern "C" int bem2d_ctx_set_stream(bem2d_ctx* c, void* stream)
 {
     if (!c) return FAIL(BEM2D_ERR_ARG, "null context");
     HIPCHK(hipStreamSynchronize(c->stream));
-    c->stream = stream ? (hipStream_t)stream : c->own_stream;
+    c->stream = (hipStream_t)stream;
+    rocblas_set_stream(c->blas, c->stream);
+    return BEM2D_OK;
+}
+
+extern "C" int bem2d_ctx_reset_stream(bem2d_ctx* c)
+{
+    if (!c) return FAIL(BEM2D_ERR_ARG, "null context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->stream = c->own_stream;
     rocblas_set_stream(c->blas, c->stream);
     return BEM2D_OK;
 }
@@ -321,6 +330,37 @@ Rule adapted23(const std::vector<double>& g, const std::vector<double>& gw, int 
 }
 
 } // namespace
+
+extern "C" int bem2d_gauss_legendre(int n, double* x, double* w)
+{
+    if (n < 1 || !x || !w) return FAIL(BEM2D_ERR_ARG, "bem2d_gauss_legendre: bad argument");
+    std::vector<double> gx, gw;
+    bem2d_internal_gauss01(n, gx, gw);
+    std::memcpy(x, gx.data(), sizeof(double) * n);
+    std::memcpy(w, gw.data(), sizeof(double) * n);
+    return BEM2D_OK;
+}
+
+extern "C" int bem2d_quad_rule(int which, int N, int L, double sigma, double* x, double* y, double* w)
+{
+    if (N < 1 || L < 0 || !(sigma > 0 && sigma < 1) || which < 0 || which > 3) {
+        FAIL(BEM2D_ERR_ARG, "bem2d_quad_rule: bad argument");
+        return -1;
+    }
+    std::vector<double> gx, gw;
+    bem2d_internal_gauss01(N, gx, gw);
+    Rule r;
+    if (which == 0) append_box(r, gx, gw, 0.0, 1.0, 0.0, 1.0);
+    else if (which == 1) r = adapted1(gx, gw, L, sigma);
+    else r = adapted23(gx, gw, L, sigma, which == 3);
+    const int m = (int)r.w.size();
+    if (x && y && w) {
+        std::memcpy(x, r.x.data(), sizeof(double) * m);
+        std::memcpy(y, r.y.data(), sizeof(double) * m);
+        std::memcpy(w, r.w.data(), sizeof(double) * m);
+    }
+    return m;
+}
 
 // ------------------------------------------------------------------------------------------------
 // mesh

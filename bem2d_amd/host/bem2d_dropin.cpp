@@ -1,0 +1,306 @@
+// Implementation of the C++ drop-in API (include/bem2d/) over the C-ABI (include/bem2d_hip.h).
+// Host code only: flattens geometries into the C-ABI's element format, moves matrices between the
+// reference's host Matrix and device buffers, and maps C-ABI statuses onto the reference's
+// exception tags.
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+
+#include "../../include/bem2d/bem2d.h"
+#include "../../include/bem2d_hip.h"
+
+namespace bem2d {
+
+// ---- status mapping --------------------------------------------------------------------------
+static void raise(int rc)
+{
+    if (rc == BEM2D_OK) return;
+    if (std::getenv("BEM2D_VERBOSE")) std::fprintf(stderr, "bem2d: %s\n", bem2d_last_error());
+    switch (rc) {
+    case BEM2D_ERR_ARG:
+    case BEM2D_ERR_UNSUPPORTED: throw ParameterException();
+    case BEM2D_ERR_SINGULAR: throw LapackError();
+    default: throw DeviceError();
+    }
+}
+
+// ---- special functions (lib/bem2d_mathroutines.cpp:20-75) -------------------------------------
+complex BesselH0(double x)
+{
+    double v[4];
+    raise(bem2d_hankel01_host(&x, 1, v));
+    return complex(v[0], v[1]);
+}
+complex BesselH1(double x)
+{
+    double v[4];
+    raise(bem2d_hankel01_host(&x, 1, v));
+    return complex(v[2], v[3]);
+}
+complex BesselH0(complex) { throw ParameterException(); }
+complex BesselH1(complex) { throw ParameterException(); }
+
+// ---- Curve::Length ---------------------------------------------------------------------------
+double Curve::Length(double t1, double t2) const
+{
+    const int panels = 64, q = 16;
+    double x[q], w[q];
+    bem2d_gauss_legendre(q, x, w);
+    double h = (t2 - t1) / panels, s = 0;
+    for (int p = 0; p < panels; p++)
+        for (int i = 0; i < q; i++) s += w[i] * length(Deriv(t1 + (p + x[i]) * h));
+    return s * h;
+}
+
+void Gauss(dvector& x, dvector& w, int N)
+{
+    if (N < 1) throw ParameterException();
+    x.resize(N);
+    w.resize(N);
+    raise(bem2d_gauss_legendre(N, x.data(), w.data()));
+}
+
+// ---- dense algebra (lib/bem2d_mathroutines.cpp:143-206) ---------------------------------------
+Matrix operator+(const Matrix& lhs, const Matrix& rhs)
+{
+    if (lhs.dim[0] != rhs.dim[0] || lhs.dim[1] != rhs.dim[1]) throw ArrayMismatch();
+    Matrix r(rhs);
+    for (std::size_t i = 0; i < r.data->size(); i++) (*r.data)[i] += (*lhs.data)[i];
+    return r;
+}
+Matrix operator-(const Matrix& lhs, const Matrix& rhs)
+{
+    if (lhs.dim[0] != rhs.dim[0] || lhs.dim[1] != rhs.dim[1]) throw ArrayMismatch();
+    Matrix r(lhs);
+    for (std::size_t i = 0; i < r.data->size(); i++) (*r.data)[i] -= (*rhs.data)[i];
+    return r;
+}
+Matrix operator*(const Matrix& lhs, const complex& alpha)
+{
+    Matrix r(lhs);
+    for (auto& v : *r.data) v *= alpha;
+    return r;
+}
+Matrix operator*(const complex& alpha, const Matrix& rhs) { return rhs * alpha; }
+Matrix operator*(const Matrix& lhs, const double& alpha)
+{
+    Matrix r(lhs);
+    for (auto& v : *r.data) v *= alpha;
+    return r;
+}
+Matrix operator*(const double& alpha, const Matrix& rhs) { return rhs * alpha; }
+Matrix operator*(const Matrix& lhs, const Matrix& rhs)
+{
+    if (lhs.dim[1] != rhs.dim[0]) throw ArrayMismatch();
+    const int m = lhs.dim[0], n = rhs.dim[1], kk = lhs.dim[1];
+    Matrix r(m, n);
+    for (int j = 0; j < n; j++)
+        for (int p = 0; p < kk; p++) {
+            const complex b = (*rhs.data)[p + (std::size_t)kk * j];
+            for (int i = 0; i < m; i++)
+                (*r.data)[i + (std::size_t)m * j] += (*lhs.data)[i + (std::size_t)m * p] * b;
+        }
+    return r;
+}
+
+namespace device {
+
+// ---- process-wide device context ---------------------------------------------------------------
+static bem2d_ctx* context()
+{
+    static std::once_flag once;
+    static bem2d_ctx* ctx = nullptr;
+    static int rc = BEM2D_OK;
+    std::call_once(once, [] {
+        const char* d = std::getenv("BEM2D_DEVICE");
+        rc = bem2d_ctx_create(&ctx, d ? std::atoi(d) : 0);
+    });
+    raise(rc);
+    return ctx;
+}
+
+// Device buffer owned for the duration of one call.
+struct DevBuf {
+    void* p = nullptr;
+    explicit DevBuf(std::size_t bytes) { raise(bem2d_device_alloc(context(), (int64_t)bytes, &p)); }
+    ~DevBuf() { bem2d_device_free(context(), p); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+};
+
+// Flattened geometry + its bem2d_mesh.
+struct MeshHandle {
+    bem2d_mesh* mesh = nullptr;
+    int64_t n = 0;
+    ~MeshHandle()
+    {
+        if (mesh) bem2d_mesh_destroy(mesh);
+    }
+};
+
+static void flatten(const Geometry& geom, QuadOption opts, MeshHandle& h)
+{
+    auto flat = geom.FlatMap();
+    const std::size_t n = flat->size();
+    if (n == 0) throw ParameterException();
+    std::vector<int32_t> type(n);
+    std::vector<double> param(8 * n);
+    std::vector<int64_t> index(n), next(n), prev(n);
+    for (std::size_t i = 0; i < n; i++) {
+        const pElement& e = (*flat)[i].first;
+        const pBasis& b = (*flat)[i].second;
+        // The device path is P0: one PolBasis(0) per element, FlatMap order = element order.
+        if (!b || b->pol_degree() != 0 || (i > 0 && (*flat)[i - 1].first == e))
+            throw ParameterException();
+        int t = -1;
+        if (!e->device_desc(t, &param[8 * i])) throw ParameterException();
+        type[i] = t;
+        index[i] = (int64_t)e->index();
+        next[i] = e->next() == (std::size_t)-1 ? -1 : (int64_t)e->next();
+        prev[i] = e->prev() == (std::size_t)-1 ? -1 : (int64_t)e->prev();
+    }
+    bem2d_geometry g{(int64_t)n, type.data(), param.data(), index.data(), next.data(), prev.data()};
+    bem2d_quad_option q{opts.L, opts.N, opts.sigma};
+    raise(bem2d_mesh_create(context(), &g, q, &h.mesh));
+    h.n = (int64_t)n;
+}
+
+static pMatrix assemble(const Geometry& geom, QuadOption opts, int kind, freqtype k, double eta,
+                        double ms, double ks)
+{
+    if (k.im != 0) throw ParameterException();
+    MeshHandle h;
+    flatten(geom, opts, h);
+    const int64_t n = h.n;
+    pMatrix out(new Matrix((int)n));
+    DevBuf A(sizeof(complex) * n * n);
+    raise(bem2d_assemble(context(), h.mesh, kind, k.re, eta, ms, ks, A.p, n));
+    raise(bem2d_copy_to_host(context(), out->data->data(), A.p, sizeof(complex) * n * n));
+    return out;
+}
+
+pMatrix AssembleCombined(const Geometry& geom, QuadOption opts, freqtype k, double eta,
+                         double mass_scale, double kernel_scale)
+{
+    return assemble(geom, opts, BEM2D_KERNEL_COMBINED, k, eta, mass_scale, kernel_scale);
+}
+
+static pMatrix rhs(const Geometry& geom, QuadOption opts, int kind, freqtype k, double eta,
+                   Point dir)
+{
+    if (k.im != 0) throw ParameterException();
+    MeshHandle h;
+    flatten(geom, opts, h);
+    const int64_t n = h.n;
+    pMatrix out(new Matrix((int)n, 1));
+    DevBuf B(sizeof(complex) * n);
+    double d[2] = {dir.x, dir.y};
+    raise(bem2d_rhs(context(), h.mesh, kind, k.re, eta, d, 1, 1.0, B.p, n));
+    raise(bem2d_copy_to_host(context(), out->data->data(), B.p, sizeof(complex) * n));
+    return out;
+}
+
+} // namespace device
+
+pMatrix DiscreteKernel(const Geometry& geom, QuadOption opts, SingleLayer g)
+{
+    return device::assemble(geom, opts, BEM2D_KERNEL_SL, g.k(), 0.0, 0.0, 1.0);
+}
+pMatrix DiscreteKernel(const Geometry& geom, QuadOption opts, DoubleLayer g)
+{
+    return device::assemble(geom, opts, BEM2D_KERNEL_DL, g.k(), 0.0, 0.0, 1.0);
+}
+pMatrix DiscreteKernel(const Geometry& geom, QuadOption opts, ConjDoubleLayer g)
+{
+    return device::assemble(geom, opts, BEM2D_KERNEL_CDL, g.k(), 0.0, 0.0, 1.0);
+}
+pMatrix DiscreteKernel(const Geometry& geom, QuadOption opts, CombinedSingleConjDouble g)
+{
+    return device::assemble(geom, opts, BEM2D_KERNEL_COMBINED, g.getk(), g.eta(), 0.0, 1.0);
+}
+pMatrix DiscreteKernel(const Geometry& geom, QuadOption opts, IdKernel)
+{
+    return device::assemble(geom, opts, BEM2D_KERNEL_ID, freqtype{1.0, 0.0}, 0.0, 0.0, 1.0);
+}
+pMatrix EvalIdent(const Geometry& geom, QuadOption opts)
+{
+    return device::assemble(geom, opts, BEM2D_KERNEL_NONE, freqtype{1.0, 0.0}, 0.0, 1.0, 0.0);
+}
+
+pMatrix DotProdBasFuns(const Geometry& geom, QuadOption ops, PlaneWave fun)
+{
+    return device::rhs(geom, ops, BEM2D_FIELD_PLANE, fun.k(), 0.0, fun.direction());
+}
+pMatrix DotProdBasFuns(const Geometry& geom, QuadOption ops, NormalPlaneWave fun)
+{
+    return device::rhs(geom, ops, BEM2D_FIELD_NORMAL_PLANE, fun.k(), 0.0, fun.direction());
+}
+pMatrix DotProdBasFuns(const Geometry& geom, QuadOption ops, CombinedPlaneWave fun)
+{
+    return device::rhs(geom, ops, BEM2D_FIELD_COMBINED_PLANE, fun.k(), fun.eta(), fun.direction());
+}
+
+void EvalKernel(const Geometry::flat_basis_map& basfuns, const std::vector<Point>& points,
+                cvector& vals, Matrix& alpha, SingleLayer g, const Gauss1D& g1d)
+{
+    if (points.size() != vals.size()) throw SizeError();
+    if (g.k().im != 0) throw ParameterException();
+    std::vector<pElement> elems;
+    for (auto& fb : basfuns) elems.push_back(fb.first);
+    Geometry geom(elems);
+    geom.AddBasis(basfuns.empty() ? pBasis() : basfuns[0].second);
+    QuadOption opts{1, g1d.Size(), 0.5};
+    device::MeshHandle h;
+    device::flatten(geom, opts, h);
+    const int64_t n = h.n, np = (int64_t)points.size();
+    if ((int64_t)alpha.data->size() < n) throw SizeError();
+    if (np == 0) return;
+    device::DevBuf c(sizeof(complex) * n), P(sizeof(double) * 2 * np), out(sizeof(complex) * np);
+    std::vector<double> pts(2 * np);
+    for (int64_t i = 0; i < np; i++) {
+        pts[2 * i] = points[i].x;
+        pts[2 * i + 1] = points[i].y;
+    }
+    raise(bem2d_copy_to_device(device::context(), c.p, alpha.data->data(), sizeof(complex) * n));
+    raise(bem2d_copy_to_device(device::context(), P.p, pts.data(), sizeof(double) * 2 * np));
+    raise(bem2d_eval_single_layer(device::context(), h.mesh, g.k().re, c.p, (const double*)P.p, np,
+                                  out.p));
+    cvector v(np);
+    raise(bem2d_copy_to_host(device::context(), v.data(), out.p, sizeof(complex) * np));
+    for (int64_t i = 0; i < np; i++) vals[i] += v[i];
+}
+
+pMatrix FarField(const Geometry& geom, QuadOption opts, freqtype k, Matrix& alpha,
+                 const dvector& theta)
+{
+    if (k.im != 0) throw ParameterException();
+    device::MeshHandle h;
+    device::flatten(geom, opts, h);
+    const int64_t n = h.n, na = (int64_t)theta.size();
+    const int nrhs = alpha.dim[1];
+    if (alpha.dim[0] != n || na == 0) throw SizeError();
+    pMatrix out(new Matrix((int)na, nrhs));
+    device::DevBuf c(sizeof(complex) * n * nrhs), th(sizeof(double) * na),
+        o(sizeof(complex) * na * nrhs);
+    raise(bem2d_copy_to_device(device::context(), c.p, alpha.data->data(), sizeof(complex) * n * nrhs));
+    raise(bem2d_copy_to_device(device::context(), th.p, theta.data(), sizeof(double) * na));
+    raise(bem2d_farfield(device::context(), h.mesh, k.re, c.p, n, nrhs, (const double*)th.p, na, o.p));
+    raise(bem2d_copy_to_host(device::context(), out->data->data(), o.p, sizeof(complex) * na * nrhs));
+    return out;
+}
+
+pMatrix SolveSystem(Matrix& m, Matrix& rhs)
+{
+    const int64_t n = m.dim[0], nrhs = rhs.dim[1];
+    if (m.dim[1] != n || rhs.dim[0] != n) throw ArrayMismatch();
+    pMatrix res(new Matrix(rhs));
+    device::DevBuf A(sizeof(complex) * n * n), B(sizeof(complex) * n * nrhs);
+    raise(bem2d_copy_to_device(device::context(), A.p, m.data->data(), sizeof(complex) * n * n));
+    raise(bem2d_copy_to_device(device::context(), B.p, res->data->data(), sizeof(complex) * n * nrhs));
+    raise(bem2d_solve_fp64(device::context(), n, nrhs, A.p, n, B.p, n, 0));
+    raise(bem2d_copy_to_host(device::context(), res->data->data(), B.p, sizeof(complex) * n * nrhs));
+    return res;
+}
+
+} // namespace bem2d

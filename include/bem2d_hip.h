@@ -88,8 +88,10 @@ typedef struct {
 /* ---- context ----------------------------------------------------------------------------------- */
 int bem2d_ctx_create(bem2d_ctx** out, int device);
 void bem2d_ctx_destroy(bem2d_ctx* ctx);
-/* Borrow a caller stream (hipStream_t, NULL = the context's own stream). */
+/* Issue all further work on a caller stream (hipStream_t; NULL is the null/default stream).
+ * bem2d_ctx_reset_stream returns to the context's own non-blocking stream. */
 int bem2d_ctx_set_stream(bem2d_ctx* ctx, void* stream);
+int bem2d_ctx_reset_stream(bem2d_ctx* ctx);
 void* bem2d_ctx_stream(bem2d_ctx* ctx);
 int bem2d_synchronize(bem2d_ctx* ctx);
 const char* bem2d_last_error(void);
@@ -163,6 +165,15 @@ int bem2d_farfield(bem2d_ctx* ctx, const bem2d_mesh* mesh, double k, const void*
 int bem2d_profile_enable(bem2d_ctx* ctx, int on);
 int bem2d_profile_reset(bem2d_ctx* ctx);
 int bem2d_profile_read(bem2d_ctx* ctx, int kernel_id, double* total_ms, int64_t* launches);
+
+/* ---- quadrature rules (host) ---------------------------------------------------------------------
+ * bem2d_gauss_legendre: n-point Gauss-Legendre rule on [0, 1], nodes ascending (replaces Gauss(),
+ * lib/bem2d_quadrature.cpp:10-56).  bem2d_quad_rule: node list of Gauss2D (which = 0),
+ * AdaptedGauss1 (1), AdaptedGauss2 (2) or AdaptedGauss3 (3) in the reference's node order
+ * (lib/bem2d_quadpoints.cpp:166-318); returns the node count, or -1 on bad arguments.  With null
+ * x/y/w only the count is returned. */
+int bem2d_gauss_legendre(int n, double* x, double* w);
+int bem2d_quad_rule(int which, int N, int L, double sigma, double* x, double* y, double* w);
 
 /* ---- special functions ------------------------------------------------------------------------- */
 /* BesselH0(double), BesselH1(double) (lib/bem2d_mathroutines.h:19-20, .cpp:20-42) for n real

@@ -1,0 +1,53 @@
+"""The C++ drop-in API (include/bem2d/, libbem2d.so) used the way the reference's drivers use
+bem2d: bem2d_amd/host/scattering_driver runs SoundSoftScattering<PlaneWave, CombinedPlaneWave> on
+the GPU; its matrix, solution, EvalSol value and far field are checked against the oracle."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from bem2d_amd import geometry as G
+from conftest import ROOT, analytic_farfield, rel_fro
+
+pytestmark = pytest.mark.gpu
+DRIVER = os.path.join(ROOT, "bem2d_amd", "host", "scattering_driver")
+
+
+def _run(shape, n, k, N=16):
+    if not os.path.exists(DRIVER):
+        pytest.fail("scattering_driver is not built")
+    out = subprocess.run([DRIVER, shape, str(n), str(k), str(N)], capture_output=True, text=True,
+                         timeout=120)
+    assert out.returncode == 0, out.stderr
+    d = json.loads(out.stdout)
+    return {key: (np.array(v)[:, 0] + 1j * np.array(v)[:, 1]) if isinstance(v, list) else v
+            for key, v in d.items()}
+
+
+@pytest.mark.parametrize("shape,n,k,N", [("circle", 200, 10.0, 16), ("kite", 300, 25.0, 16),
+                                         ("polygon", 90, 8.0, 5)])
+def test_driver_matches_oracle(oracle, ctx, shape, n, k, N):
+    d = _run(shape, n, k, N)
+    if shape == "circle":
+        geom = G.unit_circle(n)
+    elif shape == "kite":
+        geom = G.kite(n)
+    else:
+        geom = G.polygon([(0, 0), (1.5, 0.2), (0.4, 1.1)], n // 3)
+    A = oracle.assemble(geom, N, 3, 0.15, oracle.K_COMBINED, k, k, 1.0, 2.0)
+    b = oracle.rhs(geom, N, oracle.F_COMBINED_PLANE, k, k, [1.0, 0.0], 2.0)
+    x = oracle.lu_solve(A, b)[:, 0]
+    assert rel_fro(d["diag"], np.diag(A)) < 1e-12
+    assert rel_fro(d["col0"], A[:, 0]) < 1e-12
+    assert rel_fro(d["solution"], x) < 1e-12
+    th = 2 * np.pi * np.arange(360) / 360
+    ff = oracle.farfield(geom, N, k, x, th)[:, 0]
+    assert rel_fro(d["farfield"], ff) < 1e-12
+    u_inc = np.exp(1j * k * -1.5)
+    u = u_inc - oracle.eval_sl(geom, N, k, x, np.array([[-1.5, 0.0]]))
+    assert abs(d["u_at_point"][0] - u[0]) < 1e-12 * abs(u[0])
+    if shape == "circle":
+        ser = analytic_farfield(k, th)
+        assert np.abs(d["farfield"] - ser).max() / np.abs(ser).max() < 2.5e-3
