@@ -37,8 +37,8 @@ class QuadOption(ctypes.Structure):
 
 class IRReport(ctypes.Structure):
     """Outcome of the mixed-precision solve (bem2d_ir_report in include/bem2d_hip.h)."""
-    _fields_ = [("iterations", _I), ("fallback_fp64", _I), ("backward_error", _D),
-                ("residual_norm", _D), ("matrix_norm", _D)]
+    _fields_ = [("iterations", _I), ("fallback_fp64", _I), ("converged", _I), ("reserved", _I),
+                ("backward_error", _D), ("matrix_norm_inf", _D)]
 
 
 SIGNATURES = {
@@ -60,6 +60,13 @@ SIGNATURES = {
     "bem2d_assemble": (_I, [_P, _P, _I, _D, _D, _D, _D, _P, _I64]),
     "bem2d_rhs": (_I, [_P, _P, _I, _D, _D, _P, _I, _D, _P, _I64]),
     "bem2d_solve_fp64": (_I, [_P, _I64, _I64, _P, _I64, _P, _I64, _I]),
+    "bem2d_lu32_create": (_I, [_P, _I64, ctypes.POINTER(_P)]),
+    "bem2d_lu32_destroy": (None, [_P]),
+    "bem2d_lu32_size": (_I64, [_P]),
+    "bem2d_lu32_factor": (_I, [_P, _P, _P, _I64]),
+    "bem2d_lu32_refine": (_I, [_P, _P, _P, _I64, _P, _I64, _P, _I64, _I, _D, _I,
+                               ctypes.POINTER(IRReport)]),
+    "bem2d_solve_mixed": (_I, [_P, _I64, _I64, _P, _I64, _P, _I64, _D, ctypes.POINTER(IRReport)]),
     "bem2d_eval_single_layer": (_I, [_P, _P, _D, _P, _P, _I64, _P]),
     "bem2d_farfield": (_I, [_P, _P, _D, _P, _I64, _I, _P, _I64, _P]),
     "bem2d_profile_enable": (_I, [_P, _I]),
@@ -209,6 +216,17 @@ class Context:
                                      B.stride(1), 1 if preserve_A else 0))
         return B
 
+    def solve_mixed(self, A, B, tol=1e-12):
+        """Overwrites B with A^{-1} B (FP32 MFMA LU + FP64 refinement, FP64 fallback).
+        Returns the IRReport."""
+        rep = IRReport()
+        check(lib().bem2d_solve_mixed(self.h, A.shape[0], B.shape[1], _t_ptr(A), A.stride(1),
+                                      _t_ptr(B), B.stride(1), tol, ctypes.byref(rep)))
+        return rep
+
+    def lu32(self, n):
+        return LU32(self, n)
+
     def eval_single_layer(self, mesh, k, coef, pts):
         import torch
         pts = pts.contiguous()
@@ -226,6 +244,40 @@ class Context:
         check(lib().bem2d_farfield(self.h, mesh.h, k, _t_ptr(coef), coef.stride(1), nrhs,
                                    _t_ptr(theta.contiguous()), nang, _t_ptr(out)))
         return out
+
+
+class LU32:
+    """FP32 factors resident on the device (bem2d_lu32)."""
+
+    def __init__(self, ctx, n):
+        self.ctx = ctx
+        self.n = n
+        h = _P()
+        check(lib().bem2d_lu32_create(ctx.h, n, ctypes.byref(h)))
+        self.h = h
+
+    def factor(self, A):
+        check(lib().bem2d_lu32_factor(self.ctx.h, self.h, _t_ptr(A), A.stride(1)))
+
+    def refine(self, A, B, X, tol=1e-12, maxit=30, raise_on_fail=True):
+        rep = IRReport()
+        rc = lib().bem2d_lu32_refine(self.ctx.h, self.h, _t_ptr(A), A.stride(1), _t_ptr(B),
+                                     B.stride(1), _t_ptr(X), X.stride(1), B.shape[1], tol, maxit,
+                                     ctypes.byref(rep))
+        if raise_on_fail or rc not in (OK, ERR_NOT_CONVERGED):
+            check(rc)
+        return rep
+
+    def close(self):
+        if self.h:
+            lib().bem2d_lu32_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Mesh:

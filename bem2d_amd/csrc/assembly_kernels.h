@@ -30,8 +30,18 @@ __host__ __device__ __forceinline__ void kernel_eval(double k, double k4, double
         ki = 0.0;
         return;
     }
-    const double wx = xx - yx, wy = xy - yy; // x - y
-    const double d = sqrt(wx * wx + wy * wy);
+    double wx, wy, d, nw;
+    {
+        // plain multiply/add (no contraction) as in the reference: n.w cancels for near nodes
+#pragma clang fp contract(off)
+        wx = xx - yx; // x - y
+        wy = xy - yy;
+        d = sqrt(wx * wx + wy * wy);
+        if (KIND == BEM2D_KERNEL_DL)
+            nw = n2x * wx + n2y * wy;
+        else
+            nw = n1x * wx + n1y * wy;
+    }
     double h0r, h0i, h1r, h1i;
     hankel01(k * d, h0r, h0i, h1r, h1i);
     if (KIND == BEM2D_KERNEL_SL) { // (i/4) H0
@@ -40,11 +50,8 @@ __host__ __device__ __forceinline__ void kernel_eval(double k, double k4, double
         return;
     }
     // -(ik/4) c H1 with c = n.w/d
-    double c;
-    if (KIND == BEM2D_KERNEL_DL)
-        c = -(n2x * wx + n2y * wy) / d; // n_y.(y - x)/d
-    else
-        c = (n1x * wx + n1y * wy) / d; // n_x.(x - y)/d
+    // c = n_y.(y - x)/d for DL, n_x.(x - y)/d otherwise
+    const double c = (KIND == BEM2D_KERNEL_DL ? -nw : nw) / d;
     kr = k4 * c * h1i;
     ki = -k4 * c * h1r;
     if (KIND == BEM2D_KERNEL_COMBINED) { // CDL - i eta (i/4) H0 = CDL + (eta/4) H0

@@ -1,7 +1,10 @@
 // Element Map / Deriv / Normal on the device (and host), keyed by element type.
 //
 // Mirrors the reference's curve and element classes so that node positions agree with it to the
-// last bits: the association of products such as 2*PI*t (evaluated as (2*PI)*t) is kept.
+// last bits: the association of products such as 2*PI*t (evaluated as (2*PI)*t) is kept, and
+// floating-point contraction is off (plain IEEE multiply and add, as the reference's x86-64 build
+// evaluates them).  Near-coincident quadrature nodes make the integrand sensitive to the last bit
+// of the node positions, so fused multiply-adds here would move the matrix by ~1e-11.
 //   Circle, Trefoil, Kite, Ellipse, EllipseArc, InvEllipse   lib/bem2d_curvetypes.h:13-99
 //   ConstElement, AnalyticCurveElement                       lib/bem2d_element.h:72-116
 //   Element::Normal = normalize(D.y, -D.x)                    lib/bem2d_element.h:29-34
@@ -36,6 +39,7 @@ __host__ __device__ __forceinline__ void angle_sincos(double a, double& s, doubl
 __host__ __device__ __forceinline__ void curve_eval(int type, const double* c, double t, double& mx,
                                                     double& my, double& dx, double& dy)
 {
+#pragma clang fp contract(off)
     const double PI = kRefPi;
     switch (type) {
     case BEM2D_ELEM_CIRCLE:
@@ -123,6 +127,7 @@ __host__ __device__ __forceinline__ void curve_eval(int type, const double* c, d
 __host__ __device__ __forceinline__ void elem_eval(int type, const double* p, double t, double& mx,
                                                    double& my, double& dx, double& dy)
 {
+#pragma clang fp contract(off)
     if (type == BEM2D_ELEM_FLAT) {
         mx = p[0] + t * p[2];
         my = p[1] + t * p[3];
@@ -141,6 +146,7 @@ __host__ __device__ __forceinline__ void elem_eval(int type, const double* p, do
 __host__ __device__ __forceinline__ void normal_len(double dx, double dy, double& nx, double& ny,
                                                     double& len)
 {
+#pragma clang fp contract(off)
     len = sqrt(dx * dx + dy * dy); // equals length(Point(D.y, -D.x)) bit for bit
     nx = dy / len;
     ny = -dx / len;

@@ -1,5 +1,493 @@
-// Mixed-precision solver (FP32-MFMA complex LU + FP64 refinement): workspace release hook.
+// Mixed-precision dense complex solve: FP32 LU on the MFMA cores + FP64 iterative refinement.
+//
+// Factorisation (bem2d_lu32_factor): A32 = (complex64) A64, then a blocked right-looking LU with
+// 64-wide panels and no row interchanges.  Per panel: the 64x64 diagonal block is factorised by one
+// workgroup in LDS, which also forms the inverses of its unit-lower and upper triangles; the block
+// column below and the block row to the right are multiplied by those inverses; the trailing matrix
+// receives the rank-64 update A22 -= L21 U12 as a complex GEMM on v_mfma_f32_16x16x4f32 with FP32
+// accumulation (four real MFMA products per complex product).  Without interchanges the panel has
+// no sequential pivot search -- the step that makes partial-pivoting panels latency bound on a GPU.
+// A zero or non-finite pivot is flagged and the refinement then reports failure, after which the
+// one-shot solver falls back to the FP64 LU.
+//
+// Refinement (bem2d_lu32_refine): x0 = 0, r = b - A64 x in FP64, d = U^{-1} L^{-1} r in FP32
+// (two persistent block-triangular sweeps with the inverted diagonal blocks; blocks hand their
+// solution on through release/acquire flags), x += d in FP64, until the backward error
+// |b - A x|_inf / (|A|_inf |x|_inf + |b|_inf) <= tol for every right-hand side.
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/bem2d_hip.h"
 #include "internal.h"
+
+namespace {
+
+constexpr int NB = 64;             // panel / block width
+constexpr int LDT = NB + 1;        // padded LDS row of a 64x64 complex tile (float2 units)
+constexpr int kMaxSolveRhs = 8;    // right-hand sides handled per triangular sweep
+constexpr int kResidChunks = 32;   // column chunks of the FP64 residual
+constexpr long long kSpinLimit = 1ll << 26;
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cfms(float2 acc, float2 a, float2 b) // acc - a b
+{
+    acc.x -= a.x * b.x - a.y * b.y;
+    acc.y -= a.x * b.y + a.y * b.x;
+    return acc;
+}
+__device__ __forceinline__ float2 crecip(float2 a)
+{
+    const float d = a.x * a.x + a.y * a.y;
+    return make_float2(a.x / d, -a.y / d);
+}
+
+// ---- cast ---------------------------------------------------------------------------------------
+__global__ void cast_kernel(int64_t n, const double2* __restrict__ A, int64_t lda,
+                            float2* __restrict__ F, int64_t ldf)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t j = blockIdx.y;
+    if (i < n) {
+        const double2 v = A[i + lda * j];
+        F[i + ldf * j] = make_float2((float)v.x, (float)v.y);
+    }
+}
+
+// |A|_inf = max_i sum_j |a_ij| (FP64): partial row sums per column chunk, then max.
+__global__ void rowsum_kernel(int64_t n, const double2* __restrict__ A, int64_t lda, int nchunk,
+                              double* __restrict__ part)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (i >= n) return;
+    const int64_t j0 = n * c / nchunk, j1 = n * (c + 1) / nchunk;
+    double s = 0.0;
+    for (int64_t j = j0; j < j1; ++j) {
+        const double2 v = A[i + lda * j];
+        s += hypot(v.x, v.y);
+    }
+    part[c * n + i] = s;
+}
+
+__global__ void rowsum_max_kernel(int64_t n, int nchunk, const double* __restrict__ part,
+                                  double* __restrict__ out)
+{
+    __shared__ double red[256];
+    double m = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+        double s = 0.0;
+        for (int c = 0; c < nchunk; ++c) s += part[c * n + i];
+        m = fmax(m, s);
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
+}
+
+// ---- panel: diagonal block LU + triangular inverses ---------------------------------------------
+// One workgroup of 256 threads.  Block rows/cols k0 .. k0+kb.  Linv/Uinv receive the 64x64 inverses
+// (identity-padded when kb < 64), column-major with leading dimension 64.
+__global__ __launch_bounds__(256) void diag_lu_kernel(float2* __restrict__ A, int64_t ld,
+                                                      int64_t k0, int kb,
+                                                      float2* __restrict__ Linv,
+                                                      float2* __restrict__ Uinv,
+                                                      int* __restrict__ bad)
+{
+    __shared__ float2 T[NB * LDT]; // T[r + LDT*c]
+    __shared__ float2 X[NB * LDT]; // L^{-1}
+    __shared__ float2 Z[NB * LDT]; // U^{-1}
+    const int t = threadIdx.x;
+    for (int e = t; e < NB * NB; e += 256) {
+        const int r = e % NB, c = e / NB;
+        float2 v = make_float2(r == c ? 1.f : 0.f, 0.f);
+        if (r < kb && c < kb) v = A[(k0 + r) + ld * (k0 + c)];
+        T[r + LDT * c] = v;
+    }
+    __syncthreads();
+    for (int j = 0; j < kb; ++j) {
+        const float2 p = T[j + LDT * j];
+        if (t == 0 && !(p.x * p.x + p.y * p.y > 0.f && isfinite(p.x) && isfinite(p.y))) *bad = 1;
+        const float2 ip = crecip(p);
+        for (int r = j + 1 + t; r < kb; r += 256) T[r + LDT * j] = cmul(T[r + LDT * j], ip);
+        __syncthreads();
+        const int w = kb - j - 1;
+        for (int e = t; e < w * w; e += 256) {
+            const int r = j + 1 + e % w, c = j + 1 + e / w;
+            T[r + LDT * c] = cfms(T[r + LDT * c], T[r + LDT * j], T[j + LDT * c]);
+        }
+        __syncthreads();
+    }
+    for (int e = t; e < kb * kb; e += 256) {
+        const int r = e % kb, c = e / kb;
+        A[(k0 + r) + ld * (k0 + c)] = T[r + LDT * c];
+    }
+    // Inverses: threads 0..63 form column c of L^{-1} (forward substitution, unit diagonal),
+    // threads 64..127 column c of U^{-1} (back substitution).  Columns are independent.
+    if (t < NB) {
+        const int c = t;
+        for (int r = 0; r < NB; ++r) {
+            float2 s = make_float2(r == c ? 1.f : 0.f, 0.f);
+            for (int q = c; q < r; ++q) s = cfms(s, T[r + LDT * q], X[q + LDT * c]);
+            X[r + LDT * c] = r < c ? make_float2(0.f, 0.f) : s;
+        }
+        for (int r = 0; r < NB; ++r) Linv[r + NB * c] = X[r + LDT * c];
+    } else if (t < 2 * NB) {
+        const int c = t - NB;
+        for (int r = NB - 1; r > c; --r) Z[r + LDT * c] = make_float2(0.f, 0.f);
+        for (int r = c; r >= 0; --r) {
+            float2 s = make_float2(r == c ? 1.f : 0.f, 0.f);
+            for (int q = r + 1; q <= c; ++q) s = cfms(s, T[r + LDT * q], Z[q + LDT * c]);
+            Z[r + LDT * c] = cmul(s, crecip(T[r + LDT * r]));
+        }
+        for (int r = 0; r < NB; ++r) Uinv[r + NB * c] = Z[r + LDT * c];
+    }
+}
+
+// ---- panel: L21 = A21 U11^{-1} and U12 = L11^{-1} A12 ------------------------------------------
+// blockIdx.x < nrow_tiles: a 64-row tile of the block column below the diagonal block;
+// otherwise a 64-column tile of the block row to its right.  In-place, through an LDS copy.
+__global__ __launch_bounds__(256) void panel_trsm_kernel(float2* __restrict__ A, int64_t ld,
+                                                         int64_t n, int64_t k0, int kb,
+                                                         int nrow_tiles,
+                                                         const float2* __restrict__ Linv,
+                                                         const float2* __restrict__ Uinv)
+{
+    __shared__ float2 M[NB * LDT]; // the inverse
+    __shared__ float2 S[NB * LDT]; // the tile
+    const int t = threadIdx.x;
+    const int64_t rest0 = k0 + kb;
+    const bool rowtile = (int)blockIdx.x < nrow_tiles;
+    const int64_t base = rest0 + (int64_t)(rowtile ? blockIdx.x : blockIdx.x - nrow_tiles) * NB;
+    const float2* inv = rowtile ? Uinv : Linv;
+    for (int e = t; e < NB * NB; e += 256) M[(e % NB) + LDT * (e / NB)] = inv[e];
+    const int64_t len = min((int64_t)NB, n - base);
+    for (int e = t; e < NB * NB; e += 256) {
+        const int a = e % NB, b = e / NB; // tile coordinates: rowtile -> (row a, k b); else (k a, col b)
+        float2 v = make_float2(0.f, 0.f);
+        if (rowtile) {
+            if (a < len && b < kb) v = A[(base + a) + ld * (k0 + b)];
+        } else {
+            if (a < kb && b < len) v = A[(k0 + a) + ld * (base + b)];
+        }
+        S[a + LDT * b] = v;
+    }
+    __syncthreads();
+    // 256 threads x 16 outputs: thread covers (a = t % 64, b = t / 64 + 4 q)
+    const int a = t % NB;
+    for (int q = 0; q < NB / 4; ++q) {
+        const int b = t / NB + 4 * q;
+        float2 s = make_float2(0.f, 0.f);
+        if (rowtile) { // out[a][b] = sum_m S[a][m] Uinv[m][b]
+            for (int m = 0; m < kb; ++m) {
+                const float2 u = S[a + LDT * m], v = M[m + LDT * b];
+                s.x += u.x * v.x - u.y * v.y;
+                s.y += u.x * v.y + u.y * v.x;
+            }
+            if (a < len && b < kb) A[(base + a) + ld * (k0 + b)] = s;
+        } else { // out[a][b] = sum_m Linv[a][m] S[m][b]
+            for (int m = 0; m < kb; ++m) {
+                const float2 u = M[a + LDT * m], v = S[m + LDT * b];
+                s.x += u.x * v.x - u.y * v.y;
+                s.y += u.x * v.y + u.y * v.x;
+            }
+            if (a < kb && b < len) A[(k0 + a) + ld * (base + b)] = s;
+        }
+    }
+}
+
+// ---- trailing update on MFMA ----------------------------------------------------------------------
+// C[i][j] -= sum_k L[i][k] U[k][j] for i, j in [rest0, n), k in [k0, k0 + kb).
+// Workgroup = 64x64 complex output tile, 4 wavefronts of 32x32 (2x2 sub-tiles of 16x16).
+// v_mfma_f32_16x16x4f32 lane layout: A operand lane l holds A[l % 16][l / 16], B operand
+// B[l / 16][l % 16]; accumulator register r of lane l is C[4 (l / 16) + r][l % 16].
+constexpr int KP = NB + 4; // padded k extent of the LDS operand tiles
+__global__ __launch_bounds__(256) void trailing_update_kernel(float2* __restrict__ A, int64_t ld,
+                                                              int64_t n, int64_t k0, int kb)
+{
+    __shared__ float Are[NB * KP], Aim[NB * KP]; // [row][k]
+    __shared__ float Bre[NB * KP], Bim[NB * KP]; // [col][k]
+    const int t = threadIdx.x;
+    const int64_t rest0 = k0 + kb;
+    const int64_t r0 = rest0 + (int64_t)blockIdx.x * NB;
+    const int64_t c0 = rest0 + (int64_t)blockIdx.y * NB;
+    for (int e = t; e < NB * NB; e += 256) {
+        const int r = e % NB, k = e / NB;
+        float2 v = make_float2(0.f, 0.f);
+        if (r0 + r < n && k < kb) v = A[(r0 + r) + ld * (k0 + k)];
+        Are[r * KP + k] = v.x;
+        Aim[r * KP + k] = v.y;
+        float2 u = make_float2(0.f, 0.f);
+        const int kk = e % NB, c = e / NB;
+        if (c0 + c < n && kk < kb) u = A[(k0 + kk) + ld * (c0 + c)];
+        Bre[c * KP + kk] = u.x;
+        Bim[c * KP + kk] = u.y;
+    }
+    __syncthreads();
+    const int wave = t >> 6, lane = t & 63;
+    const int wr = (wave & 1) * 32, wc = (wave >> 1) * 32;
+    const int li = lane % 16, lk = lane / 16;
+    v4f cre[2][2], cim[2][2];
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            cre[p][q] = v4f{0.f, 0.f, 0.f, 0.f};
+            cim[p][q] = v4f{0.f, 0.f, 0.f, 0.f};
+        }
+    for (int ks = 0; ks < NB; ks += 4) {
+        float ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            ar[p] = Are[(wr + 16 * p + li) * KP + ks + lk];
+            ai[p] = Aim[(wr + 16 * p + li) * KP + ks + lk];
+            br[p] = Bre[(wc + 16 * p + li) * KP + ks + lk];
+            bi[p] = Bim[(wc + 16 * p + li) * KP + ks + lk];
+        }
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                cre[p][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[p], br[q], cre[p][q], 0, 0, 0);
+                cre[p][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(-ai[p], bi[q], cre[p][q], 0, 0, 0);
+                cim[p][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[p], bi[q], cim[p][q], 0, 0, 0);
+                cim[p][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[p], br[q], cim[p][q], 0, 0, 0);
+            }
+    }
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t i = r0 + wr + 16 * p + 4 * lk + r;
+                const int64_t j = c0 + wc + 16 * q + li;
+                if (i < n && j < n) {
+                    float2 v = A[i + ld * j];
+                    v.x -= cre[p][q][r];
+                    v.y -= cim[p][q][r];
+                    A[i + ld * j] = v;
+                }
+            }
+}
+
+// ---- block-triangular sweeps -------------------------------------------------------------------
+// Y (n x nrhs, column-major, FP32) is solved in place.  Lower: unit-lower L from the factors with
+// inverted diagonal blocks Linv; block b waits for blocks < b.  Upper: blocks in reverse order.
+// Each workgroup owns one 64-row block; the first workgroups dispatched own the blocks without
+// dependencies, so every wait is on a block that is already running or done.  A wait gives up
+// after kSpinLimit polls and raises *err, so the kernel always terminates.
+template <bool LOWER>
+__global__ __launch_bounds__(256) void block_trsv_kernel(const float2* __restrict__ F, int64_t ld,
+                                                         int64_t n, int nblk,
+                                                         const float2* __restrict__ Dinv,
+                                                         float2* __restrict__ Y, int64_t ldy,
+                                                         int nrhs, unsigned long long* flags,
+                                                         unsigned long long epoch,
+                                                         int* __restrict__ err)
+{
+    __shared__ float2 acc[NB][kMaxSolveRhs];
+    __shared__ float2 part[4][NB][kMaxSolveRhs];
+    __shared__ float2 yc[NB][kMaxSolveRhs];
+    const int t = threadIdx.x;
+    const int b = LOWER ? (int)blockIdx.x : nblk - 1 - (int)blockIdx.x;
+    const int64_t rb = (int64_t)b * NB;
+    const int lenb = (int)min((int64_t)NB, n - rb);
+    const int row = t % NB, grp = t / NB; // 4 groups of 64 threads split the k range
+    for (int e = t; e < NB * nrhs; e += 256) {
+        const int r = e % NB, h = e / NB;
+        acc[r][h] = r < lenb ? Y[(rb + r) + ldy * h] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    const int c_begin = LOWER ? 0 : b + 1;
+    const int c_end = LOWER ? b : nblk;
+    for (int c = c_begin; c < c_end; ++c) {
+        if (t == 0) {
+            long long spins = 0;
+            while (__hip_atomic_load(&flags[c], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
+                __builtin_amdgcn_s_sleep(2);
+                if (++spins > kSpinLimit) {
+                    atomicExch(err, 1);
+                    break;
+                }
+            }
+        }
+        __syncthreads();
+        const int64_t cb = (int64_t)c * NB;
+        const int lenc = (int)min((int64_t)NB, n - cb);
+        for (int e = t; e < NB * nrhs; e += 256) {
+            const int r = e % NB, h = e / NB;
+            // plain loads: the agent-scope acquire above invalidated this CU's vector L1
+            yc[r][h] = r < lenc ? Y[(cb + r) + ldy * h] : make_float2(0.f, 0.f);
+        }
+        __syncthreads();
+        float2 s[kMaxSolveRhs];
+#pragma unroll
+        for (int h = 0; h < kMaxSolveRhs; ++h) s[h] = make_float2(0.f, 0.f);
+        if (row < lenb) {
+            for (int k = grp; k < lenc; k += 4) {
+                const float2 f = F[(rb + row) + ld * (cb + k)];
+#pragma unroll
+                for (int h = 0; h < kMaxSolveRhs; ++h)
+                    if (h < nrhs) {
+                        const float2 y = yc[k][h];
+                        s[h].x += f.x * y.x - f.y * y.y;
+                        s[h].y += f.x * y.y + f.y * y.x;
+                    }
+            }
+        }
+#pragma unroll
+        for (int h = 0; h < kMaxSolveRhs; ++h) part[grp][row][h] = s[h];
+        __syncthreads();
+        for (int e = t; e < NB * nrhs; e += 256) {
+            const int r = e % NB, h = e / NB;
+            float2 a = acc[r][h];
+            for (int g = 0; g < 4; ++g) {
+                a.x -= part[g][r][h].x;
+                a.y -= part[g][r][h].y;
+            }
+            acc[r][h] = a;
+        }
+        __syncthreads();
+    }
+    // y_b = Dinv_b acc
+    const float2* D = Dinv + (int64_t)b * NB * NB;
+    for (int e = t; e < NB * nrhs; e += 256) {
+        const int r = e % NB, h = e / NB;
+        float2 s = make_float2(0.f, 0.f);
+        for (int k = 0; k < NB; ++k) {
+            const float2 d = D[r + NB * k], a = acc[k][h];
+            s.x += d.x * a.x - d.y * a.y;
+            s.y += d.x * a.y + d.y * a.x;
+        }
+        if (r < lenb) Y[(rb + r) + ldy * h] = s;
+    }
+    __threadfence();
+    __syncthreads();
+    if (t == 0) __hip_atomic_store(&flags[b], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---- FP64 residual and updates -------------------------------------------------------------------
+// part[c][h][i] = sum_{j in chunk c} A[i][j] X[j][h]
+__global__ void residual_partial_kernel(int64_t n, const double2* __restrict__ A, int64_t lda,
+                                        const double2* __restrict__ X, int64_t ldx, int nrhs,
+                                        double2* __restrict__ part)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (i >= n) return;
+    const int64_t j0 = n * c / kResidChunks, j1 = n * (c + 1) / kResidChunks;
+    double2 s[kMaxSolveRhs];
+#pragma unroll
+    for (int h = 0; h < kMaxSolveRhs; ++h) s[h] = make_double2(0.0, 0.0);
+    for (int64_t j = j0; j < j1; ++j) {
+        const double2 a = A[i + lda * j];
+#pragma unroll
+        for (int h = 0; h < kMaxSolveRhs; ++h)
+            if (h < nrhs) {
+                const double2 x = X[j + ldx * h];
+                s[h].x = fma(a.x, x.x, fma(-a.y, x.y, s[h].x));
+                s[h].y = fma(a.x, x.y, fma(a.y, x.x, s[h].y));
+            }
+    }
+    for (int h = 0; h < nrhs; ++h) part[((int64_t)c * nrhs + h) * n + i] = s[h];
+}
+
+// R = B - sum_c part[c]; also per-column max |R_i| (atomic max on the bit pattern of a
+// non-negative double) into rmax[h].
+__global__ void residual_reduce_kernel(int64_t n, const double2* __restrict__ B, int64_t ldb,
+                                       int nrhs, const double2* __restrict__ part,
+                                       double2* __restrict__ R, unsigned long long* rmax)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int h = blockIdx.y;
+    double m = 0.0;
+    if (i < n) {
+        double2 s = B[i + ldb * h];
+        for (int c = 0; c < kResidChunks; ++c) {
+            const double2 p = part[((int64_t)c * nrhs + h) * n + i];
+            s.x -= p.x;
+            s.y -= p.y;
+        }
+        R[i + n * h] = s;
+        m = hypot(s.x, s.y);
+        if (!(m == m)) m = INFINITY; // NaN poisons the norm
+    }
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0) atomicMax(&rmax[h], (unsigned long long)__double_as_longlong(m));
+}
+
+// Y (FP32) = R * inv_scale[h]
+__global__ void demote_kernel(int64_t n, int nrhs, const double2* __restrict__ R,
+                              const double* __restrict__ scale, float2* __restrict__ Y)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int h = blockIdx.y;
+    if (i >= n) return;
+    const double s = scale[h];
+    const double2 r = R[i + n * h];
+    Y[i + n * h] = make_float2((float)(r.x * s), (float)(r.y * s));
+}
+
+// X += Y / inv_scale[h]; per-column max |X_i| into xmax[h].
+__global__ void update_kernel(int64_t n, int nrhs, const float2* __restrict__ Y,
+                              const double* __restrict__ scale, double2* __restrict__ X,
+                              int64_t ldx, unsigned long long* xmax)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int h = blockIdx.y;
+    double m = 0.0;
+    if (i < n) {
+        const double s = 1.0 / scale[h];
+        const float2 y = Y[i + n * h];
+        double2 x = X[i + ldx * h];
+        x.x += (double)y.x * s;
+        x.y += (double)y.y * s;
+        X[i + ldx * h] = x;
+        m = hypot(x.x, x.y);
+        if (!(m == m)) m = INFINITY;
+    }
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0) atomicMax(&xmax[h], (unsigned long long)__double_as_longlong(m));
+}
+
+__global__ void zero_u64_kernel(unsigned long long* p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+__global__ void zero_complex_kernel(int64_t n, int nrhs, double2* __restrict__ X, int64_t ldx)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) X[i + ldx * blockIdx.y] = make_double2(0.0, 0.0);
+}
+
+// scale[h] = 1 / max(rmax[h], tiny)
+__global__ void scale_kernel(int nrhs, const unsigned long long* __restrict__ rmax,
+                             double* __restrict__ scale)
+{
+    const int h = threadIdx.x;
+    if (h < nrhs) {
+        const double m = __longlong_as_double((long long)rmax[h]);
+        scale[h] = (m > 1e-300 && m < INFINITY) ? 1.0 / m : 1.0;
+    }
+}
+
+} // namespace
 
 struct MixedState {
 };
@@ -8,4 +496,286 @@ void bem2d_internal_mixed_release(bem2d_ctx* c)
 {
     delete c->mixed;
     c->mixed = nullptr;
+}
+
+struct bem2d_lu32 {
+    bem2d_ctx* ctx = nullptr;
+    int64_t n = 0;
+    int nblk = 0;
+    bool factored = false;
+    float2* F = nullptr;          // n x n factors, ld = n
+    float2* Linv = nullptr;       // nblk x 64 x 64
+    float2* Uinv = nullptr;
+    unsigned long long* flags = nullptr; // 2 * nblk (lower, upper)
+    unsigned long long epoch = 0;
+    int* dflag = nullptr;         // [0] bad pivot, [1] sweep time-out
+    double* anorm = nullptr;      // |A|_inf
+    double* rowpart = nullptr;    // 16 x n
+    // refinement workspace, grown on demand
+    int rhs_cap = 0;
+    double2* R = nullptr;         // n x rhs
+    double2* part = nullptr;      // chunks x rhs x n
+    float2* Y = nullptr;          // n x rhs
+    double* scale = nullptr;      // rhs
+    unsigned long long* norms = nullptr; // rmax[rhs], xmax[rhs], bmax[rhs]
+};
+
+static void lu32_free_ws(bem2d_lu32* f)
+{
+    (void)hipFree(f->R);
+    (void)hipFree(f->part);
+    (void)hipFree(f->Y);
+    (void)hipFree(f->scale);
+    (void)hipFree(f->norms);
+    f->R = nullptr;
+    f->part = nullptr;
+    f->Y = nullptr;
+    f->scale = nullptr;
+    f->norms = nullptr;
+    f->rhs_cap = 0;
+}
+
+extern "C" void bem2d_lu32_destroy(bem2d_lu32* f)
+{
+    if (!f) return;
+    (void)hipSetDevice(f->ctx->device);
+    (void)hipStreamSynchronize(f->ctx->stream);
+    (void)hipFree(f->F);
+    (void)hipFree(f->Linv);
+    (void)hipFree(f->Uinv);
+    (void)hipFree(f->flags);
+    (void)hipFree(f->dflag);
+    (void)hipFree(f->anorm);
+    (void)hipFree(f->rowpart);
+    lu32_free_ws(f);
+    delete f;
+}
+
+extern "C" int bem2d_lu32_create(bem2d_ctx* c, int64_t n, bem2d_lu32** out)
+{
+    if (!c || !out || n < 1) return FAIL(BEM2D_ERR_ARG, "bem2d_lu32_create: bad argument");
+    HIPCHK(hipSetDevice(c->device));
+    bem2d_lu32* f = new bem2d_lu32();
+    f->ctx = c;
+    f->n = n;
+    f->nblk = (int)((n + NB - 1) / NB);
+    const size_t inv_bytes = sizeof(float2) * (size_t)f->nblk * NB * NB;
+    if (hipMalloc(&f->F, sizeof(float2) * n * n) != hipSuccess ||
+        hipMalloc(&f->Linv, inv_bytes) != hipSuccess || hipMalloc(&f->Uinv, inv_bytes) != hipSuccess ||
+        hipMalloc(&f->flags, sizeof(unsigned long long) * 2 * f->nblk) != hipSuccess ||
+        hipMalloc(&f->dflag, sizeof(int) * 2) != hipSuccess ||
+        hipMalloc(&f->anorm, sizeof(double)) != hipSuccess ||
+        hipMalloc(&f->rowpart, sizeof(double) * 16 * n) != hipSuccess) {
+        bem2d_lu32_destroy(f);
+        return FAIL(BEM2D_ERR_NOMEM, "bem2d_lu32_create: device allocation of %lld bytes failed",
+                    (long long)(8 * n * n));
+    }
+    zero_u64_kernel<<<(2 * f->nblk + 255) / 256, 256, 0, c->stream>>>(f->flags, 2 * f->nblk);
+    HIPCHK(hipGetLastError());
+    *out = f;
+    return BEM2D_OK;
+}
+
+extern "C" int64_t bem2d_lu32_size(const bem2d_lu32* f) { return f ? f->n : -1; }
+
+extern "C" int bem2d_lu32_factor(bem2d_ctx* c, bem2d_lu32* f, const void* A64, int64_t lda)
+{
+    if (!c || !f || !A64 || lda < f->n) return FAIL(BEM2D_ERR_ARG, "bem2d_lu32_factor: bad argument");
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    const int64_t n = f->n;
+    const double2* A = (const double2*)A64;
+    bem2d_internal_prof_begin(c, BEM2D_PROF_LU);
+    {
+        dim3 g((unsigned)((n + 255) / 256), (unsigned)n);
+        cast_kernel<<<g, 256, 0, s>>>(n, A, lda, f->F, n);
+        dim3 g2((unsigned)((n + 255) / 256), 16);
+        rowsum_kernel<<<g2, 256, 0, s>>>(n, A, lda, 16, f->rowpart);
+        rowsum_max_kernel<<<1, 256, 0, s>>>(n, 16, f->rowpart, f->anorm);
+        zero_u64_kernel<<<1, 64, 0, s>>>((unsigned long long*)f->dflag, 1);
+    }
+    for (int b = 0; b < f->nblk; ++b) {
+        const int64_t k0 = (int64_t)b * NB;
+        const int kb = (int)std::min<int64_t>(NB, n - k0);
+        diag_lu_kernel<<<1, 256, 0, s>>>(f->F, n, k0, kb, f->Linv + (size_t)b * NB * NB,
+                                         f->Uinv + (size_t)b * NB * NB, f->dflag);
+        const int64_t rest = n - k0 - kb;
+        if (rest > 0) {
+            const int tiles = (int)((rest + NB - 1) / NB);
+            panel_trsm_kernel<<<2 * tiles, 256, 0, s>>>(f->F, n, n, k0, kb, tiles,
+                                                        f->Linv + (size_t)b * NB * NB,
+                                                        f->Uinv + (size_t)b * NB * NB);
+            dim3 g((unsigned)tiles, (unsigned)tiles);
+            trailing_update_kernel<<<g, 256, 0, s>>>(f->F, n, n, k0, kb);
+        }
+    }
+    bem2d_internal_prof_end(c, BEM2D_PROF_LU);
+    HIPCHK(hipGetLastError());
+    f->factored = true;
+    return BEM2D_OK;
+}
+
+static int lu32_reserve(bem2d_lu32* f, int nrhs)
+{
+    if (nrhs <= f->rhs_cap) return BEM2D_OK;
+    (void)hipStreamSynchronize(f->ctx->stream);
+    lu32_free_ws(f);
+    const int64_t n = f->n;
+    if (hipMalloc(&f->R, sizeof(double2) * n * nrhs) != hipSuccess ||
+        hipMalloc(&f->part, sizeof(double2) * kResidChunks * nrhs * n) != hipSuccess ||
+        hipMalloc(&f->Y, sizeof(float2) * n * nrhs) != hipSuccess ||
+        hipMalloc(&f->scale, sizeof(double) * nrhs) != hipSuccess ||
+        hipMalloc(&f->norms, sizeof(unsigned long long) * 3 * nrhs) != hipSuccess) {
+        lu32_free_ws(f);
+        return FAIL(BEM2D_ERR_NOMEM, "refinement workspace allocation failed");
+    }
+    f->rhs_cap = nrhs;
+    return BEM2D_OK;
+}
+
+// FP32 solve of L U Y = Y in place for up to kMaxSolveRhs columns (ld = n).
+static int lu32_apply(bem2d_lu32* f, float2* Y, int nrhs)
+{
+    hipStream_t s = f->ctx->stream;
+    f->epoch++;
+    block_trsv_kernel<true><<<f->nblk, 256, 0, s>>>(f->F, f->n, f->n, f->nblk, f->Linv, Y, f->n,
+                                                     nrhs, f->flags, f->epoch, f->dflag + 1);
+    block_trsv_kernel<false><<<f->nblk, 256, 0, s>>>(f->F, f->n, f->n, f->nblk, f->Uinv, Y, f->n,
+                                                      nrhs, f->flags + f->nblk, f->epoch,
+                                                      f->dflag + 1);
+    return hipGetLastError() == hipSuccess ? BEM2D_OK : BEM2D_ERR_DEVICE;
+}
+
+extern "C" int bem2d_lu32_refine(bem2d_ctx* c, bem2d_lu32* f, const void* A64, int64_t lda,
+                                 const void* B, int64_t ldb, void* X, int64_t ldx, int nrhs,
+                                 double tol, int maxit, bem2d_ir_report* rep)
+{
+    if (!c || !f || !A64 || !B || !X || nrhs < 1 || lda < f->n || ldb < f->n || ldx < f->n ||
+        maxit < 1)
+        return FAIL(BEM2D_ERR_ARG, "bem2d_lu32_refine: bad argument");
+    if (!f->factored) return FAIL(BEM2D_ERR_ARG, "bem2d_lu32_refine: not factored");
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    const int64_t n = f->n;
+    bem2d_ir_report r{};
+    double worst = INFINITY;
+    for (int h0 = 0; h0 < nrhs; h0 += kMaxSolveRhs) {
+        const int nr = std::min(kMaxSolveRhs, nrhs - h0);
+        int rc = lu32_reserve(f, nr);
+        if (rc) return rc;
+        const double2* Bh = (const double2*)B + ldb * h0;
+        double2* Xh = (double2*)X + ldx * h0;
+        unsigned long long* rmax = f->norms;
+        unsigned long long* xmax = f->norms + nr;
+        unsigned long long* bmax = f->norms + 2 * nr;
+        dim3 gv((unsigned)((n + 255) / 256), (unsigned)nr);
+        zero_complex_kernel<<<gv, 256, 0, s>>>(n, nr, Xh, ldx);
+        // |b|_inf via the residual of x = 0
+        zero_u64_kernel<<<1, 64, 0, s>>>(f->norms, 3 * nr);
+        dim3 gp((unsigned)((n + 255) / 256), kResidChunks);
+        residual_partial_kernel<<<gp, 256, 0, s>>>(n, (const double2*)A64, lda, Xh, ldx, nr, f->part);
+        residual_reduce_kernel<<<gv, 256, 0, s>>>(n, Bh, ldb, nr, f->part, f->R, bmax);
+        std::vector<unsigned long long> hb(3 * nr);
+        double anorm = 0.0;
+        int dflag[2] = {0, 0};
+        HIPCHK(hipMemcpyAsync(&anorm, f->anorm, sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(dflag, f->dflag, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (dflag[0]) {
+            r.iterations = 0;
+            r.converged = 0;
+            r.backward_error = INFINITY;
+            r.matrix_norm_inf = anorm;
+            if (rep) *rep = r;
+            return FAIL(BEM2D_ERR_NOT_CONVERGED, "FP32 LU met a zero or non-finite pivot");
+        }
+        HIPCHK(hipMemcpyAsync(hb.data(), f->norms, sizeof(unsigned long long) * 3 * nr,
+                              hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        std::vector<double> bnorm(nr);
+        for (int h = 0; h < nr; ++h) {
+            double v;
+            std::memcpy(&v, &hb[2 * nr + h], sizeof v);
+            bnorm[h] = v;
+        }
+        // R holds b; copy rmax = bmax for the first scaling
+        HIPCHK(hipMemcpyAsync(rmax, bmax, sizeof(unsigned long long) * nr, hipMemcpyDeviceToDevice, s));
+        double omega = INFINITY;
+        int it = 0;
+        for (; it < maxit; ++it) {
+            scale_kernel<<<1, 64, 0, s>>>(nr, rmax, f->scale);
+            demote_kernel<<<gv, 256, 0, s>>>(n, nr, f->R, f->scale, f->Y);
+            rc = lu32_apply(f, f->Y, nr);
+            if (rc) return FAIL(rc, "triangular sweep launch failed");
+            zero_u64_kernel<<<1, 64, 0, s>>>(f->norms, 2 * nr);
+            update_kernel<<<gv, 256, 0, s>>>(n, nr, f->Y, f->scale, Xh, ldx, xmax);
+            residual_partial_kernel<<<gp, 256, 0, s>>>(n, (const double2*)A64, lda, Xh, ldx, nr, f->part);
+            residual_reduce_kernel<<<gv, 256, 0, s>>>(n, Bh, ldb, nr, f->part, f->R, rmax);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(hb.data(), f->norms, sizeof(unsigned long long) * 2 * nr,
+                                  hipMemcpyDeviceToHost, s));
+            HIPCHK(hipMemcpyAsync(dflag, f->dflag, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            if (dflag[1]) return FAIL(BEM2D_ERR_DEVICE, "triangular sweep wait timed out");
+            omega = 0.0;
+            for (int h = 0; h < nr; ++h) {
+                double rn, xn;
+                std::memcpy(&rn, &hb[h], sizeof rn);
+                std::memcpy(&xn, &hb[nr + h], sizeof xn);
+                const double den = anorm * xn + bnorm[h];
+                const double w = den > 0 ? rn / den : (rn > 0 ? INFINITY : 0.0);
+                omega = std::max(omega, std::isfinite(w) ? w : INFINITY);
+            }
+            if (omega <= tol) {
+                ++it;
+                break;
+            }
+        }
+        r.iterations = std::max(r.iterations, it);
+        worst = (h0 == 0) ? omega : std::max(worst, omega);
+        r.matrix_norm_inf = anorm;
+    }
+    r.backward_error = worst;
+    r.converged = worst <= tol;
+    if (rep) *rep = r;
+    if (!r.converged)
+        return FAIL(BEM2D_ERR_NOT_CONVERGED, "refinement reached backward error %.3e > %.3e",
+                    worst, tol);
+    return BEM2D_OK;
+}
+
+// One-shot mixed solve with the FP64 fallback: B (n x nrhs) is overwritten with the solution.
+extern "C" int bem2d_solve_mixed(bem2d_ctx* c, int64_t n, int64_t nrhs, const void* A64,
+                                 int64_t lda, void* B, int64_t ldb, double tol,
+                                 bem2d_ir_report* rep)
+{
+    if (!c || !A64 || !B || n < 1 || nrhs < 1 || lda < n || ldb < n)
+        return FAIL(BEM2D_ERR_ARG, "bem2d_solve_mixed: bad argument");
+    bem2d_lu32* f = nullptr;
+    int rc = bem2d_lu32_create(c, n, &f);
+    if (rc) return rc;
+    double2* X = nullptr;
+    if (hipMalloc(&X, sizeof(double2) * n * nrhs) != hipSuccess) {
+        bem2d_lu32_destroy(f);
+        return FAIL(BEM2D_ERR_NOMEM, "bem2d_solve_mixed: allocation failed");
+    }
+    bem2d_ir_report r{};
+    rc = bem2d_lu32_factor(c, f, A64, lda);
+    if (!rc) rc = bem2d_lu32_refine(c, f, A64, lda, B, ldb, X, n, (int)nrhs, tol, 30, &r);
+    if (rc == BEM2D_OK) {
+        rc = hipMemcpy2DAsync(B, sizeof(double2) * ldb, X, sizeof(double2) * n,
+                              sizeof(double2) * n, nrhs, hipMemcpyDeviceToDevice, c->stream) ==
+                     hipSuccess
+                 ? BEM2D_OK
+                 : FAIL(BEM2D_ERR_DEVICE, "copy of the solution failed");
+    } else if (rc == BEM2D_ERR_NOT_CONVERGED) {
+        // FP64 fallback on a copy of A (the caller's A is const)
+        r.fallback_fp64 = 1;
+        rc = bem2d_solve_fp64(c, n, nrhs, const_cast<void*>(A64), lda, B, ldb, 1);
+    }
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(X);
+    bem2d_lu32_destroy(f);
+    if (rep) *rep = r;
+    return rc;
 }

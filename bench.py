@@ -112,6 +112,30 @@ def main():
     lu_avg = lu_ms / max(lu_n, 1)
     asm_ms = reg_avg + sp_avg
 
+    # ---- the chosen form: same assembly + FP32 MFMA LU with FP64 refinement -------------------
+    lu = ctx.lu32(n)
+    X = torch.empty_like(B)
+
+    def step_mixed():
+        ctx.assemble(mesh, capi.KERNEL_COMBINED, k, k, 1.0, 2.0, out=A)
+        ctx.rhs(mesh, capi.FIELD_COMBINED_PLANE, k, k, dirs, 2.0, out=B)
+        lu.factor(A)
+        return lu.refine(A, B, X, tol=1e-12, raise_on_fail=False)
+
+    for _ in range(args.warmup):
+        step_mixed()
+    torch.cuda.synchronize()
+    ctx.profile(True)
+    t0 = time.perf_counter()
+    reports = [step_mixed() for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    dt_mixed = time.perf_counter() - t0
+    lu32_ms, lu32_n = ctx.profile_read(capi.PROF_LU)
+    ctx.profile(False)
+    mixed_ok = all(r.converged for r in reports)
+    mixed_bw = max(r.backward_error for r in reports)
+    mixed_it = max(r.iterations for r in reports)
+
     # one more assembly for the checks and the CPU baseline (outside the timed region)
     ctx.assemble(mesh, capi.KERNEL_COMBINED, k, k, 1.0, 2.0, out=A)
     ctx.rhs(mesh, capi.FIELD_COMBINED_PLANE, k, k, dirs, 2.0, out=B)
@@ -122,16 +146,23 @@ def main():
     torch.cuda.synchronize()
     bw = (torch.linalg.norm(B - A @ X) / (torch.linalg.norm(A) * torch.linalg.norm(X) +
                                           torch.linalg.norm(B))).item()
+    Xm = torch.empty_like(B)
+    lu.factor(A)
+    lu.refine(A, B, Xm, tol=1e-12, raise_on_fail=False)
+    mixed_vs_fp64 = (torch.linalg.norm(Xm - X) / torch.linalg.norm(X)).item()
+    lu.close()
 
     achieved = FLOP_PER_PAIRPOINT * P_reg / (reg_avg * 1e-3) / 1e12
+    plain_sps = args.steps / dt
+    mixed_sps = args.steps / dt_mixed
     out = {
         "metric": METRIC,
-        "value": args.steps / dt,
+        "value": mixed_sps if mixed_ok else plain_sps,
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * dt / args.steps,
+        "ms_per_step": 1e3 * (dt_mixed if mixed_ok else dt) / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -139,16 +170,25 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "kite N=%d k=%g combined-field (eta=k), P0, 16-pt Gauss, L=3 sigma=0.15, "
-                        "1 plane-wave RHS; plain FP64 path: assembly A=M+2K + RHS + FP64 LU "
-                        "solve (rocSOLVER zgetrf/zgetrs)" % (n, k),
-            "n": n, "k": k, "nrhs": 1, "path": "plain-fp64",
+                        "1 plane-wave RHS at a seeded random angle; one step = FP64 assembly "
+                        "A=M+2K + RHS + solve; value = mixed path (FP32 MFMA LU + FP64 refinement "
+                        "to backward error 1e-12)" % (n, k),
+            "n": n, "k": k, "nrhs": 1, "path": "mixed" if mixed_ok else "plain-fp64",
         },
+        "solves_per_s_mixed": mixed_sps,
+        "solves_per_s_plain_fp64": plain_sps,
+        "mixed_over_plain": mixed_sps / plain_sps,
+        "mixed_converged": mixed_ok,
+        "mixed_backward_error": mixed_bw,
+        "mixed_refinement_steps": mixed_it,
+        "mixed_vs_fp64_solution_rel_err": mixed_vs_fp64,
+        "lu32_ms": lu32_ms / max(lu32_n, 1),
         "gentries_per_s": n * n / (asm_ms * 1e-3) / 1e9,
         "gpairpts_per_s": P / (asm_ms * 1e-3) / 1e9,
         "assembly_ms": asm_ms,
         "assembly_regular_ms": reg_avg,
         "assembly_special_ms": sp_avg,
-        "lu_ms": lu_avg,
+        "lu_fp64_ms": lu_avg,
         "pair_points": P,
         "backward_error_fp64": bw,
         "roofline": {

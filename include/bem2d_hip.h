@@ -139,6 +139,35 @@ int bem2d_rhs(bem2d_ctx* ctx, const bem2d_mesh* mesh, int field_kind, double k, 
 int bem2d_solve_fp64(bem2d_ctx* ctx, int64_t n, int64_t nrhs, void* A, int64_t lda, void* B,
                      int64_t ldb, int preserve_A);
 
+/* ---- mixed-precision solve (FP32 LU on the MFMA cores + FP64 iterative refinement) -------------
+ * Not in the reference, whose only solve is SolveSystem; it answers the same question to a stated
+ * backward error.  bem2d_lu32 holds FP32 factors of an n x n matrix resident on the device
+ * (8 n^2 bytes), so that one factorisation serves many right-hand sides and refinement passes. */
+typedef struct bem2d_lu32 bem2d_lu32;
+typedef struct {
+    int iterations;          /* refinement steps taken (max over right-hand-side groups) */
+    int fallback_fp64;       /* 1 if bem2d_solve_mixed fell back to the FP64 LU */
+    int converged;           /* 1 if every column met tol */
+    int reserved;
+    double backward_error;   /* max over columns of |b - A x|_inf / (|A|_inf |x|_inf + |b|_inf) */
+    double matrix_norm_inf;  /* |A|_inf of the FP64 matrix */
+} bem2d_ir_report;
+
+int bem2d_lu32_create(bem2d_ctx* ctx, int64_t n, bem2d_lu32** out);
+void bem2d_lu32_destroy(bem2d_lu32* lu);
+int64_t bem2d_lu32_size(const bem2d_lu32* lu);
+/* FP32 factorisation of the FP64 device matrix A64 (unchanged). */
+int bem2d_lu32_factor(bem2d_ctx* ctx, bem2d_lu32* lu, const void* A64, int64_t lda);
+/* X (n x nrhs, device) = A64^{-1} B by refinement from x0 = 0 with the FP32 factors and FP64
+ * residuals, until the backward error is <= tol or maxit steps; BEM2D_ERR_NOT_CONVERGED otherwise. */
+int bem2d_lu32_refine(bem2d_ctx* ctx, bem2d_lu32* lu, const void* A64, int64_t lda, const void* B,
+                      int64_t ldb, void* X, int64_t ldx, int nrhs, double tol, int maxit,
+                      bem2d_ir_report* report);
+/* One-shot: factor, refine, and fall back to bem2d_solve_fp64 if refinement does not converge.
+ * B is overwritten with the solution; A64 is unchanged. */
+int bem2d_solve_mixed(bem2d_ctx* ctx, int64_t n, int64_t nrhs, const void* A64, int64_t lda,
+                      void* B, int64_t ldb, double tol, bem2d_ir_report* report);
+
 /* ---- potential and far field ------------------------------------------------------------------- */
 /* out[p] = sum_j coef[j] * int_{elem j} (i/4) H0(k |P_p - y|) ds(y) with Gauss1D(N):
  * EvalKernel<SingleLayer> (lib/bem2d_quadrature.h:184-228) as used by EvalSol

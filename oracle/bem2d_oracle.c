@@ -516,7 +516,7 @@ typedef struct {
 
 static int rules_make(rules_t* r, int N, int L, double sigma)
 {
-    if (N < 1 || L < 1 || !(sigma > 0 && sigma < 1)) return 1;
+    if (N < 1 || L < 0 || !(sigma > 0 && sigma < 1)) return 1;
     r->N = N; r->L = L; r->sigma = sigma;
     r->m0 = oracle_rule_size(R_GAUSS2D, N, L);
     r->m1 = oracle_rule_size(R_AG1, N, L);

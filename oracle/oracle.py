@@ -10,19 +10,21 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB = None
+_LIBS = {}
 
 R_GAUSS2D, R_AG1, R_AG2, R_AG3 = 0, 1, 2, 3
 K_NONE, K_SL, K_DL, K_CDL, K_COMBINED, K_ID = 0, 1, 2, 3, 4, 5
 F_PLANE, F_NORMAL_PLANE, F_COMBINED_PLANE = 0, 1, 2
 
 
-def lib():
-    global _LIB
-    if _LIB is None:
-        path = os.path.join(_HERE, "liboracle.so")
+def lib(variant="plain"):
+    """The oracle library: "plain" (IEEE multiply/add, as the reference's x86-64 build) or "fma"
+    (the same source with fused multiply-adds, used to measure the rounding-noise floor)."""
+    if variant not in _LIBS:
+        name = {"plain": "liboracle.so", "fma": "liboracle_fma.so"}[variant]
+        path = os.path.join(_HERE, name)
         if not os.path.exists(path):
-            raise RuntimeError("oracle/liboracle.so is not built (run make -C oracle)")
+            raise RuntimeError("oracle/%s is not built (run make -C oracle)" % name)
         L = ctypes.CDLL(path)
         P = ctypes.c_void_p
         i64 = ctypes.c_int64
@@ -41,8 +43,8 @@ def lib():
         L.oracle_farfield.argtypes = [i64, P, P, i32, d, P, i32, P, i64, P]
         L.oracle_elem_eval.argtypes = [i64, P, P, i64, d, P]
         L.oracle_elem_eval.restype = None
-        _LIB = L
-    return _LIB
+        _LIBS[variant] = L
+    return _LIBS[variant]
 
 
 def _p(a):
@@ -74,7 +76,8 @@ def hankel(x):
     return out[:, 0], out[:, 1], out[:, 2], out[:, 3]
 
 
-def assemble(geom, N, L, sigma, kind, k, eta, mass_scale=0.0, kernel_scale=1.0, cols=None):
+def assemble(geom, N, L, sigma, kind, k, eta, mass_scale=0.0, kernel_scale=1.0, cols=None,
+             variant="plain"):
     """Column-major N x ncols complex128 block of mass_scale*M + kernel_scale*K."""
     n = geom.n
     if cols is None:
@@ -85,7 +88,7 @@ def assemble(geom, N, L, sigma, kind, k, eta, mass_scale=0.0, kernel_scale=1.0, 
         ncols = cols.size
         cptr = _p(cols)
     out = np.zeros((ncols, n), dtype=np.complex128)  # row c = column cols[c]
-    rc = lib().oracle_assemble(n, _p(geom.type), _p(geom.param), _p(geom.index), _p(geom.next),
+    rc = lib(variant).oracle_assemble(n, _p(geom.type), _p(geom.param), _p(geom.index), _p(geom.next),
                                _p(geom.prev), N, L, sigma, kind, k, eta, mass_scale, kernel_scale,
                                cptr, ncols, _p(out), n)
     if rc:
@@ -140,3 +143,12 @@ def elem_eval(geom, e, t):
     out = np.zeros(6)
     lib().oracle_elem_eval(geom.n, _p(geom.type), _p(geom.param), e, t, _p(out))
     return out
+
+
+def noise_floor(geom, N, L, sigma, kind, k, eta, mass_scale=0.0, kernel_scale=1.0, cols=None):
+    """Relative Frobenius distance between the oracle evaluated with and without fused
+    multiply-adds: how far rounding alone moves the reference's own matrix for this input."""
+    a = assemble(geom, N, L, sigma, kind, k, eta, mass_scale, kernel_scale, cols)
+    b = assemble(geom, N, L, sigma, kind, k, eta, mass_scale, kernel_scale, cols, variant="fma")
+    na = np.linalg.norm(a)
+    return 0.0 if na == 0 else float(np.linalg.norm(a - b) / na)

@@ -1,0 +1,75 @@
+"""Mixed-precision solve (FP32 MFMA LU + FP64 refinement) on the GPU: solutions against the FP64
+path and numpy, the backward-error target of 1e-12, multi-RHS, and the FP64 fallback."""
+import numpy as np
+import pytest
+import torch
+
+from bem2d_amd import capi
+from bem2d_amd import geometry as G
+from conftest import rel_fro
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(a):
+    return torch.tensor(np.asfortranarray(a), device="cuda").t().contiguous().t()
+
+
+def _backward(A, X, B):
+    return (np.abs(B - A @ X).max(axis=0) /
+            (np.abs(A).sum(axis=1).max() * np.abs(X).max(axis=0) + np.abs(B).max(axis=0))).max()
+
+
+@pytest.mark.parametrize("n,nrhs", [(64, 1), (300, 1), (777, 3), (1000, 12)])
+def test_random_diagonally_weighted(ctx, n, nrhs):
+    print("case n=%d nrhs=%d" % (n, nrhs), flush=True)
+    rng = np.random.default_rng(n)
+    A = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) / np.sqrt(n)
+    A += 4 * np.eye(n)
+    B = rng.standard_normal((n, nrhs)) + 1j * rng.standard_normal((n, nrhs))
+    Ad = _dev(A)
+    Bd = _dev(B)
+    rep = ctx.solve_mixed(Ad, Bd, tol=1e-12)
+    X = Bd.cpu().numpy()
+    assert rep.fallback_fp64 == 0 and rep.converged == 1, (rep.iterations, rep.backward_error)
+    assert rep.backward_error <= 1e-12
+    assert _backward(A, X, B) <= 2e-12
+    assert rel_fro(X, np.linalg.solve(A, B)) < 1e-10
+
+
+@pytest.mark.parametrize("geom_name,n,k", [("circle", 200, 10.0), ("kite", 1000, 50.0),
+                                           ("arc", 500, 60.0)])
+def test_bem_matrices(ctx, geom_name, n, k):
+    print("case", geom_name, n, k, flush=True)
+    geom = {"circle": G.unit_circle, "kite": G.kite, "arc": G.open_arc}[geom_name](n)
+    mesh = ctx.mesh(geom, 16, 3, 0.15)
+    A = ctx.assemble(mesh, capi.KERNEL_COMBINED, k, k, 1.0, 2.0)
+    ang = np.linspace(0, np.pi, 3)
+    B = ctx.rhs(mesh, capi.FIELD_COMBINED_PLANE, k, k, np.stack([np.cos(ang), np.sin(ang)], 1), 2.0)
+    X64 = ctx.solve_fp64(A, B.clone())
+    Xm = B.clone()
+    rep = ctx.solve_mixed(A, Xm, tol=1e-12)
+    assert rep.fallback_fp64 == 0, rep.backward_error
+    assert rep.backward_error <= 1e-12
+    err = (torch.linalg.norm(Xm - X64) / torch.linalg.norm(X64)).item()
+    assert err < 1e-10, err
+    # factors stay resident: refine a second set of right-hand sides with the same factorisation
+    lu = ctx.lu32(n)
+    lu.factor(A)
+    X2 = torch.empty_like(B)
+    rep2 = lu.refine(A, B, X2, tol=1e-12)
+    assert rep2.converged == 1
+    assert (torch.linalg.norm(X2 - X64) / torch.linalg.norm(X64)).item() < 1e-10
+    lu.close()
+
+
+def test_zero_pivot_falls_back_to_fp64(ctx):
+    n = 130
+    rng = np.random.default_rng(1)
+    A = rng.standard_normal((n, n)) + 4 * np.eye(n) + 0j
+    A[0, 0] = 0.0  # no-pivot LU meets an exactly zero pivot
+    B = rng.standard_normal((n, 2)) + 0j
+    Bd = _dev(B)
+    rep = ctx.solve_mixed(_dev(A), Bd, tol=1e-12)
+    assert rep.fallback_fp64 == 1
+    assert rel_fro(Bd.cpu().numpy(), np.linalg.solve(A, B)) < 1e-12

@@ -1,8 +1,12 @@
 """GPU parity of the HIP path (through the C-ABI) against the CPU oracle and the golden vectors.
 
 Tolerances: the plain FP64 path must reproduce the oracle's matrices, right-hand sides and
-solutions to a relative Frobenius error of 1e-12 (the bar of the plain method); the device
-Hankel routine is held to 1e-15 relative to |H| against mpmath.
+solutions to a relative Frobenius error of 1e-12 (the bar of the plain method).  Where the
+reference computation is itself ill-conditioned at the rounding level -- near-coincident graded
+quadrature nodes, where the double-layer factor n.(x - y)/d^2 is rounding noise of the node
+positions -- the bar is max(1e-12, 2 x the oracle's own noise floor), the floor being measured live
+as the distance between the oracle built with and without fused multiply-adds.  The device Hankel
+routine is held to 1e-15 relative to |H| against mpmath.
 """
 import numpy as np
 import pytest
@@ -77,7 +81,10 @@ def test_assembly_matches_oracle(ctx, oracle, case):
     if np.linalg.norm(ref) == 0:
         assert np.linalg.norm(A) == 0
     else:
-        assert rel_fro(A, ref) < TOL, (name, rel_fro(A, ref))
+        floor = oracle.noise_floor(geom, N, L, sigma, kind, k, eta, ms, ks)
+        err = rel_fro(A, ref)
+        print("case %s: |A-A_oracle|/|A_oracle| = %.2e, oracle noise floor %.2e" % (name, err, floor))
+        assert err < max(TOL, 2 * floor), (name, err, floor)
     assert np.all(np.isfinite(A))
 
 
@@ -155,8 +162,8 @@ def test_eval_single_layer_matches_oracle(ctx, oracle):
 
 
 def test_singular_matrix_raises(ctx):
-    A = torch.ones((4, 4), dtype=torch.complex128, device="cuda")
-    B = torch.ones((4, 1), dtype=torch.complex128, device="cuda")
+    A = torch.ones((4, 4), dtype=torch.complex128, device="cuda").t()  # column-major view
+    B = torch.ones((1, 4), dtype=torch.complex128, device="cuda").t()
     with pytest.raises(capi.Bem2dError) as e:
         ctx.solve_fp64(A, B)
     assert e.value.code == capi.ERR_SINGULAR
@@ -183,8 +190,11 @@ def test_kite_config2_full_size(ctx, oracle):
     A = ctx.assemble(mesh, capi.KERNEL_COMBINED, k, k, 1.0, 2.0)
     cols = np.array([0, 1, 777, 1999, 2000, 3998, 3999])
     ref = oracle.assemble(geom, 16, 3, 0.15, oracle.K_COMBINED, k, k, 1.0, 2.0, cols=cols)
+    floor = oracle.noise_floor(geom, 16, 3, 0.15, oracle.K_COMBINED, k, k, 1.0, 2.0, cols=cols)
     got = _np(A[:, torch.tensor(cols, device="cuda")])
-    assert rel_fro(got, ref) < TOL
+    err = rel_fro(got, ref)
+    print("kite N=4000 sampled columns: err %.2e, oracle noise floor %.2e" % (err, floor))
+    assert err < max(TOL, 2 * floor), (err, floor)
     assert torch.isfinite(torch.view_as_real(A)).all()
     # linearity in the scales: A(1, 2) = M + 2 K
     K = ctx.assemble(mesh, capi.KERNEL_COMBINED, k, k, 0.0, 1.0)
