@@ -116,6 +116,15 @@ def _t_ptr(t):
     return _P(t.data_ptr())
 
 
+def _ld(t):
+    """Leading dimension of a column-major (stride(0) == 1) matrix view.  A single column's stride
+    is never stepped, and torch may report it as 1; the C-ABI wants ld >= rows."""
+    if t.stride(0) != 1 and t.shape[0] > 1:
+        raise ValueError("expected a column-major view (stride(0) == 1), got strides %s"
+                         % (t.stride(),))
+    return max(t.stride(1), t.shape[0]) if t.shape[1] == 1 else t.stride(1)
+
+
 def hankel01_host(x):
     """Host build of the device Hankel routine: returns (H0, H1) complex arrays."""
     x = np.ascontiguousarray(x, dtype=np.float64)
@@ -193,7 +202,7 @@ class Context:
             out = store.t()
         assert out.stride(0) == 1 and out.shape == (n, n)
         check(lib().bem2d_assemble(self.h, mesh.h, kind, k, eta, mass_scale, kernel_scale,
-                                   _t_ptr(out), out.stride(1)))
+                                   _t_ptr(out), _ld(out)))
         return out
 
     def rhs(self, mesh, kind, k, eta, dirs, scale=1.0, out=None):
@@ -205,23 +214,23 @@ class Context:
             out = torch.empty((nrhs, n), dtype=torch.complex128,
                               device="cuda:%d" % self.device).t()
         check(lib().bem2d_rhs(self.h, mesh.h, kind, k, eta, _np_ptr(dirs), nrhs, scale,
-                              _t_ptr(out), out.stride(1)))
+                              _t_ptr(out), _ld(out)))
         return out
 
     def solve_fp64(self, A, B, preserve_A=True):
         """Overwrites B (n x nrhs column-major complex128) with A^{-1} B."""
         n = A.shape[0]
         nrhs = B.shape[1]
-        check(lib().bem2d_solve_fp64(self.h, n, nrhs, _t_ptr(A), A.stride(1), _t_ptr(B),
-                                     B.stride(1), 1 if preserve_A else 0))
+        check(lib().bem2d_solve_fp64(self.h, n, nrhs, _t_ptr(A), _ld(A), _t_ptr(B), _ld(B),
+                                     1 if preserve_A else 0))
         return B
 
     def solve_mixed(self, A, B, tol=1e-12):
         """Overwrites B with A^{-1} B (FP32 MFMA LU + FP64 refinement, FP64 fallback).
         Returns the IRReport."""
         rep = IRReport()
-        check(lib().bem2d_solve_mixed(self.h, A.shape[0], B.shape[1], _t_ptr(A), A.stride(1),
-                                      _t_ptr(B), B.stride(1), tol, ctypes.byref(rep)))
+        check(lib().bem2d_solve_mixed(self.h, A.shape[0], B.shape[1], _t_ptr(A), _ld(A),
+                                      _t_ptr(B), _ld(B), tol, ctypes.byref(rep)))
         return rep
 
     def lu32(self, n):
@@ -241,7 +250,7 @@ class Context:
         nrhs = coef.shape[1]
         nang = theta.numel()
         out = torch.empty((nrhs, nang), dtype=torch.complex128, device=theta.device).t()
-        check(lib().bem2d_farfield(self.h, mesh.h, k, _t_ptr(coef), coef.stride(1), nrhs,
+        check(lib().bem2d_farfield(self.h, mesh.h, k, _t_ptr(coef), _ld(coef), nrhs,
                                    _t_ptr(theta.contiguous()), nang, _t_ptr(out)))
         return out
 
@@ -257,13 +266,12 @@ class LU32:
         self.h = h
 
     def factor(self, A):
-        check(lib().bem2d_lu32_factor(self.ctx.h, self.h, _t_ptr(A), A.stride(1)))
+        check(lib().bem2d_lu32_factor(self.ctx.h, self.h, _t_ptr(A), _ld(A)))
 
     def refine(self, A, B, X, tol=1e-12, maxit=30, raise_on_fail=True):
         rep = IRReport()
-        rc = lib().bem2d_lu32_refine(self.ctx.h, self.h, _t_ptr(A), A.stride(1), _t_ptr(B),
-                                     B.stride(1), _t_ptr(X), X.stride(1), B.shape[1], tol, maxit,
-                                     ctypes.byref(rep))
+        rc = lib().bem2d_lu32_refine(self.ctx.h, self.h, _t_ptr(A), _ld(A), _t_ptr(B), _ld(B),
+                                     _t_ptr(X), _ld(X), B.shape[1], tol, maxit, ctypes.byref(rep))
         if raise_on_fail or rc not in (OK, ERR_NOT_CONVERGED):
             check(rc)
         return rep

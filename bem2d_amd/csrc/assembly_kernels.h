@@ -13,12 +13,13 @@ namespace bem2d_dev {
 enum { NODE_X = 0, NODE_Y = 1, NODE_NX = 2, NODE_NY = 3, NODE_SW = 4, NODE_PLANES = 5 };
 
 // Kernel functors of lib/bem2d_kernel.cpp:15-112 and IdKernel (lib/bem2d_kernel.h:91-98) for real k.
-// (x, y) = test and trial points, n1 / n2 = test and trial normals.  k4 = k/4, eta4 = eta/4.
+// (x, y) = test and trial points, n1 / n2 = test and trial normals.  kinv = 1/k, k4 = k/4,
+// eta4 = eta/4.  1/d comes from one refined reciprocal square root, which also gives 1/(kd).
 template <int KIND>
-__host__ __device__ __forceinline__ void kernel_eval(double k, double k4, double eta4, double xx,
-                                                     double xy, double yx, double yy, double n1x,
-                                                     double n1y, double n2x, double n2y,
-                                                     double& kr, double& ki)
+__host__ __device__ __forceinline__ void kernel_eval(double k, double kinv, double k4, double eta4,
+                                                     double xx, double xy, double yx, double yy,
+                                                     double n1x, double n1y, double n2x,
+                                                     double n2y, double& kr, double& ki)
 {
     if (KIND == BEM2D_KERNEL_ID) {
         kr = 1.0;
@@ -30,20 +31,22 @@ __host__ __device__ __forceinline__ void kernel_eval(double k, double k4, double
         ki = 0.0;
         return;
     }
-    double wx, wy, d, nw;
+    double wx, wy, d2, nw;
     {
         // plain multiply/add (no contraction) as in the reference: n.w cancels for near nodes
 #pragma clang fp contract(off)
         wx = xx - yx; // x - y
         wy = xy - yy;
-        d = sqrt(wx * wx + wy * wy);
+        d2 = wx * wx + wy * wy;
         if (KIND == BEM2D_KERNEL_DL)
             nw = n2x * wx + n2y * wy;
         else
             nw = n1x * wx + n1y * wy;
     }
+    const double rd = rsqrt_refined(d2); // 1/d
+    const double d = d2 * rd;
     double h0r, h0i, h1r, h1i;
-    hankel01(k * d, h0r, h0i, h1r, h1i);
+    hankel01_inv(k * d, kinv * rd, h0r, h0i, h1r, h1i);
     if (KIND == BEM2D_KERNEL_SL) { // (i/4) H0
         kr = -0.25 * h0i;
         ki = 0.25 * h0r;
@@ -51,7 +54,7 @@ __host__ __device__ __forceinline__ void kernel_eval(double k, double k4, double
     }
     // -(ik/4) c H1 with c = n.w/d
     // c = n_y.(y - x)/d for DL, n_x.(x - y)/d otherwise
-    const double c = (KIND == BEM2D_KERNEL_DL ? -nw : nw) / d;
+    const double c = (KIND == BEM2D_KERNEL_DL ? -nw : nw) * rd;
     kr = k4 * c * h1i;
     ki = -k4 * c * h1r;
     if (KIND == BEM2D_KERNEL_COMBINED) { // CDL - i eta (i/4) H0 = CDL + (eta/4) H0
@@ -119,7 +122,7 @@ __global__ __launch_bounds__(256) void assemble_regular_kernel(
     __syncthreads();
     if (i >= n) return;
 
-    const double k4 = 0.25 * k, eta4 = 0.25 * eta;
+    const double k4 = 0.25 * k, eta4 = 0.25 * eta, kinv = 1.0 / k;
     const int64_t idx_i = index[i], nx_i = next[i], pv_i = prev[i];
     for (int cc = 0; cc < kColsPerWave; ++cc) {
         const int c = wave * kColsPerWave + cc;
@@ -139,7 +142,8 @@ __global__ __launch_bounds__(256) void assemble_regular_kernel(
             for (int b = 0; b < nq; ++b) {
                 const double* y = sc + b * 5;
                 double kr, ki;
-                kernel_eval<KIND>(k, k4, eta4, xx, xy, y[0], y[1], n1x, n1y, y[2], y[3], kr, ki);
+                kernel_eval<KIND>(k, kinv, k4, eta4, xx, xy, y[0], y[1], n1x, n1y, y[2], y[3], kr,
+                                  ki);
                 br = fma(kr, y[4], br);
                 bi = fma(ki, y[4], bi);
             }
@@ -172,7 +176,7 @@ __global__ __launch_bounds__(256) void assemble_special_kernel(
         pi_[q] = param[8 * i + q];
         pj[q] = param[8 * j + q];
     }
-    const double k4 = 0.25 * k, eta4 = 0.25 * eta;
+    const double k4 = 0.25 * k, eta4 = 0.25 * eta, kinv = 1.0 / k;
     double ar = 0.0, ai = 0.0;
     if (KIND != BEM2D_KERNEL_NONE && kscale != 0.0) {
         const double* rx;
@@ -191,14 +195,14 @@ __global__ __launch_bounds__(256) void assemble_special_kernel(
                 elem_eval(tj, pj, (1 - yq) * xq, mx2, my2, dx2, dy2);
                 normal_len(dx1, dy1, n1x, n1y, s1);
                 normal_len(dx2, dy2, n2x, n2y, s2);
-                kernel_eval<KIND>(k, k4, eta4, mx1, my1, mx2, my2, n1x, n1y, n2x, n2y, kr, ki);
+                kernel_eval<KIND>(k, kinv, k4, eta4, mx1, my1, mx2, my2, n1x, n1y, n2x, n2y, kr, ki);
                 ar += xq * kr * s1 * s2 * wq;
                 ai += xq * ki * s1 * s2 * wq;
                 elem_eval(ti, pi_, 1 - xq, mx1, my1, dx1, dy1);
                 elem_eval(tj, pj, 1 - xq + yq * xq, mx2, my2, dx2, dy2);
                 normal_len(dx1, dy1, n1x, n1y, s1);
                 normal_len(dx2, dy2, n2x, n2y, s2);
-                kernel_eval<KIND>(k, k4, eta4, mx1, my1, mx2, my2, n1x, n1y, n2x, n2y, kr, ki);
+                kernel_eval<KIND>(k, kinv, k4, eta4, mx1, my1, mx2, my2, n1x, n1y, n2x, n2y, kr, ki);
                 ar += xq * kr * s1 * s2 * wq;
                 ai += xq * ki * s1 * s2 * wq;
             } else {
@@ -206,7 +210,7 @@ __global__ __launch_bounds__(256) void assemble_special_kernel(
                 elem_eval(tj, pj, yq, mx2, my2, dx2, dy2);
                 normal_len(dx1, dy1, n1x, n1y, s1);
                 normal_len(dx2, dy2, n2x, n2y, s2);
-                kernel_eval<KIND>(k, k4, eta4, mx1, my1, mx2, my2, n1x, n1y, n2x, n2y, kr, ki);
+                kernel_eval<KIND>(k, kinv, k4, eta4, mx1, my1, mx2, my2, n1x, n1y, n2x, n2y, kr, ki);
                 ar += kr * s1 * s2 * wq;
                 ai += ki * s1 * s2 * wq;
             }
@@ -286,8 +290,9 @@ __global__ void eval_sl_kernel(int64_t n, int nq, const double* __restrict__ nod
         for (int a = 0; a < nq; ++a) {
             const int64_t o = (int64_t)a * n + j;
             double kr, ki;
-            kernel_eval<BEM2D_KERNEL_SL>(k, 0.25 * k, 0.0, px, py, nodes[NODE_X * plane + o],
-                                         nodes[NODE_Y * plane + o], 0, 0, 0, 0, kr, ki);
+            kernel_eval<BEM2D_KERNEL_SL>(k, 1.0 / k, 0.25 * k, 0.0, px, py,
+                                         nodes[NODE_X * plane + o], nodes[NODE_Y * plane + o], 0,
+                                         0, 0, 0, kr, ki);
             const double sw = nodes[NODE_SW * plane + o];
             sr = fma(kr, sw, sr);
             si = fma(ki, sw, si);

@@ -9,8 +9,11 @@ Representation (x > 0):
                Y1 = (2/pi) ln(x/2) J1 - 2/(pi x) + (x/2) S_Y1(t)
   x >= 2     modulus/phase form of A&S 9.2.5-9.2.6:
                H_nu(x) = sqrt(2/(pi x)) (P_nu(x) + i Q_nu(x)) exp(i (x - (2 nu + 1) pi/4)),
-             with P_nu and x*Q_nu fitted per piece [a, b) as polynomials of degree DEG in
+             with P_nu and x*Q_nu fitted per piece [a, b) as polynomials of degree DEGS[p] in
              z = (y - ym)/yh in [-1, 1], y = (a/x)^2 (the last piece reaches x = infinity, y -> 0).
+The header gives every table as an initializer macro, so that the device code can hold them in
+__constant__ memory (read through scalar loads into SGPR operands) and the host code in constexpr
+arrays.
 The fits are Chebyshev interpolants computed at 40 digits from mpmath's hankel1 and converted
 to the monomial basis in z; the script reports the float64 Horner error of every table.
 """
@@ -21,8 +24,9 @@ import numpy as np
 
 mp.mp.dps = 40
 HERE = os.path.dirname(os.path.abspath(__file__))
-PIECES = [2.0, 2.6, 3.5, 5.0, 8.0]  # piece p covers [PIECES[p], PIECES[p+1]), the last one [8, inf)
-DEG = 12
+PIECES = [2.0, 2.6, 3.5, 5.0, 8.0, 20.0]  # piece p covers [PIECES[p], PIECES[p+1]); last [20, inf)
+DEGS = [12, 12, 12, 12, 12, 7]              # polynomial degree per piece
+DEG = max(DEGS)
 NSER = 14  # series terms k = 0..NSER-1
 
 
@@ -71,6 +75,10 @@ def horner(coefs, z):
     return r
 
 
+def fmt(v):
+    return repr(float(v))
+
+
 def main():
     lines = []
     emit = lines.append
@@ -79,7 +87,7 @@ def main():
     emit("#pragma once")
     emit("")
     emit("#define BEM2D_HANKEL_NPIECE %d" % len(PIECES))
-    emit("#define BEM2D_HANKEL_DEG %d" % DEG)
+    emit("#define BEM2D_HANKEL_DEGMAX %d" % DEG)
     emit("#define BEM2D_HANKEL_NSER %d" % NSER)
     emit("")
     # ---- series
@@ -92,17 +100,21 @@ def main():
     sy0 = [(2 / mp.pi) * (-1) ** k * (g - H[k]) / mp.factorial(k) ** 2 for k in range(NSER)]
     sy1 = [-(1 / mp.pi) * (-1) ** k * (H[k] + H[k + 1] - 2 * g) /
            (mp.factorial(k) * mp.factorial(k + 1)) for k in range(NSER)]
-    for name, arr in (("SER_J0", sj0), ("SER_J1", sj1), ("SER_Y0", sy0), ("SER_Y1", sy1)):
-        emit("static constexpr double BEM2D_%s[BEM2D_HANKEL_NSER] = {" % name)
-        emit("    " + ", ".join(mp.nstr(v, 20, min_fixed=1, max_fixed=0) for v in arr) + "};")
+    emit("// series tables [J0, J1, Y0, Y1][k], ascending powers of t = x^2/4")
+    emit("#define BEM2D_HANKEL_SER_INIT { \\")
+    for arr in (sj0, sj1, sy0, sy1):
+        emit("    {" + ", ".join(fmt(v) for v in arr) + "}, \\")
+    emit("}")
     # ---- pieces
-    tabs = {"P0": [], "Q0": [], "P1": [], "Q1": []}
+    fits = []
     zmap = []
     worst = 0.0
     for p, a in enumerate(PIECES):
         a = mp.mpf(a)
         b = mp.mpf(PIECES[p + 1]) if p + 1 < len(PIECES) else None
         ylo = (a / b) ** 2 if b is not None else mp.mpf(0)
+        deg = DEGS[p]
+        row = []
         for nu in (0, 1):
             def fP(y, nu=nu):
                 if y == 0:
@@ -115,35 +127,41 @@ def main():
                 x = a / mp.sqrt(y)
                 return PQ(nu, x)[1] * x
             for fname, f in (("P%d" % nu, fP), ("Q%d" % nu, fQ)):
-                coefs, ym, yh = fit_piece(f, ylo, mp.mpf(1), DEG)
-                tabs[fname].append([float(c) for c in coefs])
-                # float64 Horner error check
+                coefs, ym, yh = fit_piece(f, ylo, mp.mpf(1), deg)
+                cf = [float(c) for c in coefs]
+                row.append(cf + [0.0] * (DEG - deg))
                 err = 0.0
-                for y in np.linspace(float(ylo), 1.0, 257)[1:] if b is None else \
-                        np.linspace(float(ylo), 1.0, 257):
+                ys = np.linspace(float(ylo), 1.0, 257)
+                if b is None:
+                    ys = ys[1:]
+                for y in ys:
                     z = (np.float64(y) - np.float64(ym)) / np.float64(yh)
-                    approx = horner([float(c) for c in coefs], z)
+                    approx = horner(cf, z)
                     exact = f(mp.mpf(y))
                     scale = 1 if fname[0] == "P" else a / mp.sqrt(mp.mpf(y))  # Q = fit / x
                     err = max(err, float(abs((approx - exact) / scale)))
                 worst = max(worst, err)
-                print("piece %d [%s, %s) %s: max |err| = %.3g" % (p, a, b, fname, err))
-        zmap.append((float(a) ** 2, float(ym), float(yh)))
-    emit("// piece p: y = PIECE_A2[p] / x^2, z = (y - PIECE_YM[p]) * PIECE_YHINV[p]")
+                print("piece %d [%s, %s) deg %d %s: max |err| = %.3g" % (p, a, b, deg, fname, err))
+        fits.append(row)
+        zmap.append((float(a) ** 2, float(ym), 1.0 / float(yh)))
+    emit("// piece p covers x in [PIECE_X[p], PIECE_X[p+1]); polynomial degree PIECE_DEG[p]")
     emit("static constexpr double BEM2D_PIECE_X[BEM2D_HANKEL_NPIECE] = {%s};" %
-         ", ".join(repr(v) for v in PIECES))
-    emit("static constexpr double BEM2D_PIECE_A2[BEM2D_HANKEL_NPIECE] = {%s};" %
-         ", ".join(repr(z[0]) for z in zmap))
-    emit("static constexpr double BEM2D_PIECE_YM[BEM2D_HANKEL_NPIECE] = {%s};" %
-         ", ".join(repr(z[1]) for z in zmap))
-    emit("static constexpr double BEM2D_PIECE_YHINV[BEM2D_HANKEL_NPIECE] = {%s};" %
-         ", ".join(repr(1.0 / z[2]) for z in zmap))
-    for fname in ("P0", "Q0", "P1", "Q1"):
-        emit("static constexpr double BEM2D_FIT_%s[BEM2D_HANKEL_NPIECE][BEM2D_HANKEL_DEG + 1] = {"
-             % fname)
-        for row in tabs[fname]:
-            emit("    {" + ", ".join(repr(v) for v in row) + "},")
-        emit("};")
+         ", ".join(fmt(v) for v in PIECES))
+    emit("static constexpr int BEM2D_PIECE_DEG[BEM2D_HANKEL_NPIECE] = {%s};" %
+         ", ".join(str(d) for d in DEGS))
+    emit("// [piece][a^2, ym, 1/yh]: y = a^2/x^2, z = (y - ym)/yh")
+    emit("#define BEM2D_HANKEL_ZMAP_INIT { \\")
+    for z in zmap:
+        emit("    {%s}, \\" % ", ".join(fmt(v) for v in z))
+    emit("}")
+    emit("// [piece][P0, xQ0, P1, xQ1][ascending powers of z]")
+    emit("#define BEM2D_HANKEL_FIT_INIT { \\")
+    for row in fits:
+        emit("    { \\")
+        for cf in row:
+            emit("        {%s}, \\" % ", ".join(fmt(v) for v in cf))
+        emit("    }, \\")
+    emit("}")
     emit("// worst float64 Horner error of the fitted tables (absolute, on P and Q): %.3g" % worst)
     with open(os.path.join(HERE, "hankel_tables.h"), "w") as fh:
         fh.write("\n".join(lines) + "\n")

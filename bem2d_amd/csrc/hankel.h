@@ -4,11 +4,12 @@
 // Three regimes, coefficients in hankel_tables.h (written by gen_hankel_tables.py from mpmath):
 //   x < 2    ascending series in t = x^2/4 (A&S 9.1.10 and 9.1.11, n = 0, 1);
 //   x >= 2   H_nu = sqrt(2/(pi x)) (P_nu + i Q_nu) exp(i(x - (2nu+1) pi/4)) (A&S 9.2.5-9.2.6),
-//            P_nu and x Q_nu as degree-12 polynomials of z(y), y = (a/x)^2, on the pieces
-//            [2, 2.6), [2.6, 3.5), [3.5, 5), [5, 8) and [8, inf) -- the last one is the
-//            asymptotic regime, fitted over the whole half-line.
-// The piece is selected by a switch so that every table index is a compile-time constant (the
-// coefficients become instruction literals instead of a per-lane table).  sin/cos of the phase use
+//            P_nu and x Q_nu as polynomials of z(y), y = (a/x)^2, on the pieces [2, 2.6),
+//            [2.6, 3.5), [3.5, 5), [5, 8), [8, 20) (degree 12) and [20, inf) (degree 7, the
+//            asymptotic regime where almost all far-field pairs fall).
+// On the device every coefficient lives in one __constant__ table and every index is a
+// compile-time constant, so the coefficients reach v_fma_f64 as SGPR operands through scalar loads
+// (instruction literals would each cost two v_mov_b32 per use).  sin/cos of the phase use
 // sincos_reduced below.  Accuracy: about 3e-16 relative to |H| (pinned against mpmath by the tests).
 #pragma once
 
@@ -18,17 +19,95 @@
 
 namespace bem2d_dev {
 
-constexpr double kPi = 3.14159265358979323846;
 constexpr double kTwoOverPi = 0.63661977236758134308;
 constexpr double kInvPi = 0.31830988618379067154;
 
-template <int N>
-__host__ __device__ __forceinline__ double horner(const double* c, double z)
+struct HankelTables {
+    double ser[4][BEM2D_HANKEL_NSER];                          // J0, J1, Y0, Y1 series in t
+    double zmap[BEM2D_HANKEL_NPIECE][3];                       // a^2, ym, 1/yh per piece
+    double fit[BEM2D_HANKEL_NPIECE][4][BEM2D_HANKEL_DEGMAX + 1]; // P0, xQ0, P1, xQ1 per piece
+    // sincos_reduced: 2/pi, pi/2 = P1 + P2 + P3, then the Taylor coefficients of sin (r^17 ..
+    // r^3) and cos (r^18 .. r^2), highest power first
+    double trig[4 + 8 + 9];
+};
+
+#define BEM2D_TRIG_INIT                                                                            \
+    {0.6366197723675814, 1.5707963267948966, 6.123233995736766e-17, -1.4973849048591698e-33,     \
+     2.8114572543455206e-15, -7.6471637318198164e-13, 1.6059043836821613e-10,                    \
+     -2.5052108385441720e-08, 2.7557319223985893e-06, -1.9841269841269841e-04,                   \
+     8.3333333333333333e-03, -1.6666666666666667e-01, -1.5619206968586225e-16,                   \
+     4.7794773323873853e-14, -1.1470745597729725e-11, 2.0876756987868099e-09,                    \
+     -2.7557319223985888e-07, 2.4801587301587302e-05, -1.3888888888888889e-03,                   \
+     4.1666666666666667e-02, -0.5}
+
+static constexpr HankelTables kHankelHost = {BEM2D_HANKEL_SER_INIT, BEM2D_HANKEL_ZMAP_INIT,
+                                             BEM2D_HANKEL_FIT_INIT, BEM2D_TRIG_INIT};
+static __constant__ HankelTables kHankelDev = {BEM2D_HANKEL_SER_INIT, BEM2D_HANKEL_ZMAP_INIT,
+                                               BEM2D_HANKEL_FIT_INIT, BEM2D_TRIG_INIT};
+
+#ifdef __HIP_DEVICE_COMPILE__
+// Device view of the table: a constant-address-space (4) pointer, so that uniform reads become
+// scalar loads.  The empty asm makes the pointer opaque at every use, which keeps the loads next to
+// their use inside the quadrature loops -- hoisted out, the ~60 coefficients would exceed the SGPR
+// file and be spilled to VGPR lanes.
+typedef __attribute__((address_space(4))) const HankelTables DevTables;
+__device__ __forceinline__ DevTables& hankel_tables()
+{
+    DevTables* p = (DevTables*)&kHankelDev;
+    asm volatile("" : "+s"(p));
+    return *p;
+}
+
+// a * b + c with c read from an SGPR pair: the VOP3 form takes the coefficient as an operand,
+// where the compiler's two-address v_fmac_f64 would first copy it into VGPRs (two v_mov_b32).
+__device__ __forceinline__ double fma_vvs(double a, double b, double c)
+{
+    double r;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "s"(c));
+    return r;
+}
+#else
+__host__ __forceinline__ const HankelTables& hankel_tables() { return kHankelHost; }
+__host__ __forceinline__ double fma_vvs(double a, double b, double c) { return fma(a, b, c); }
+#endif
+
+template <int N, typename Ptr>
+__host__ __device__ __forceinline__ double horner(Ptr c, double z)
 {
     double r = c[N - 1];
 #pragma unroll
-    for (int i = N - 2; i >= 0; --i) r = fma(r, z, c[i]);
+    for (int i = N - 2; i >= 0; --i) r = fma_vvs(r, z, c[i]);
     return r;
+}
+
+// 1/a and 1/sqrt(a) for finite a > 0: the hardware estimates (about 2^-23 relative) refined by two
+// Newton steps to within ~1 ulp.  On the host, the C library.
+__host__ __device__ __forceinline__ double rcp_refined(double a)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    double y = __builtin_amdgcn_rcp(a);
+    double e = fma(-a, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-a, y, 1.0);
+    return fma(y, e, y);
+#else
+    return 1.0 / a;
+#endif
+}
+
+__host__ __device__ __forceinline__ double rsqrt_refined(double a)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    double y = __builtin_amdgcn_rsq(a);
+    double h = a * y;
+    double e = fma(-h, y, 1.0); // 1 - a y^2
+    y = fma(0.5 * y, e, y);
+    h = a * y;
+    e = fma(-h, y, 1.0);
+    return fma(0.5 * y, e, y);
+#else
+    return 1.0 / sqrt(a);
+#endif
 }
 
 // sin and cos of x for |x| < 2^30.  Cody-Waite reduction by pi/2 split into three doubles
@@ -36,68 +115,56 @@ __host__ __device__ __forceinline__ double horner(const double* c, double z)
 // truncated at r^17 and r^18 (remainder below 1e-19).  Both results within about 1 ulp.
 __host__ __device__ __forceinline__ void sincos_reduced(double x, double& s, double& c)
 {
-    constexpr double P1 = 1.5707963267948966;
-    constexpr double P2 = 6.123233995736766e-17;
-    constexpr double P3 = -1.4973849048591698e-33;
-    const double n = rint(x * 0.6366197723675814);
-    double r = fma(-n, P1, x);
-    r = fma(-n, P2, r);
-    r = fma(-n, P3, r);
+    const auto& T = hankel_tables().trig;
+    const double n = rint(x * T[0]);
+    double r = fma(-n, T[1], x);
+    r = fma(-n, T[2], r);
+    r = fma(-n, T[3], r);
     const double r2 = r * r;
-    // sin r = r (1 - r^2/3! + r^4/5! - ... + r^16/17!)
-    double ps = 2.8114572543455206e-15;           //  1/17!
-    ps = fma(ps, r2, -7.6471637318198164e-13);    // -1/15!
-    ps = fma(ps, r2, 1.6059043836821613e-10);     //  1/13!
-    ps = fma(ps, r2, -2.5052108385441720e-08);    // -1/11!
-    ps = fma(ps, r2, 2.7557319223985893e-06);     //  1/9!
-    ps = fma(ps, r2, -1.9841269841269841e-04);    // -1/7!
-    ps = fma(ps, r2, 8.3333333333333333e-03);     //  1/5!
-    ps = fma(ps, r2, -1.6666666666666667e-01);    // -1/3!
+    double ps = T[4];
+#pragma unroll
+    for (int i = 5; i < 12; ++i) ps = fma_vvs(ps, r2, T[i]);
     const double sr = fma(ps * r2, r, r);
-    // cos r = 1 - r^2/2! + ... + r^18/18!
-    double pc = -1.5619206968586225e-16;          // -1/18!
-    pc = fma(pc, r2, 4.7794773323873853e-14);     //  1/16!
-    pc = fma(pc, r2, -1.1470745597729725e-11);    // -1/14!
-    pc = fma(pc, r2, 2.0876756987868099e-09);     //  1/12!
-    pc = fma(pc, r2, -2.7557319223985888e-07);    // -1/10!
-    pc = fma(pc, r2, 2.4801587301587302e-05);     //  1/8!
-    pc = fma(pc, r2, -1.3888888888888889e-03);    // -1/6!
-    pc = fma(pc, r2, 4.1666666666666667e-02);     //  1/4!
-    pc = fma(pc, r2, -0.5);                       // -1/2!
+    double pc = T[12];
+#pragma unroll
+    for (int i = 13; i < 21; ++i) pc = fma_vvs(pc, r2, T[i]);
     const double cr = fma(pc, r2, 1.0);
     const int q = ((int)(long long)n) & 3;
-    if (q == 0) { s = sr; c = cr; }
-    else if (q == 1) { s = cr; c = -sr; }
-    else if (q == 2) { s = -sr; c = -cr; }
-    else { s = -cr; c = sr; }
+    const double a = (q & 1) ? cr : sr; // sin before the sign
+    const double b = (q & 1) ? sr : cr; // cos before the sign
+    s = (q & 2) ? -a : a;
+    c = ((q + 1) & 2) ? -b : b;
 }
 
 // Series regime, x < 2.
 __host__ __device__ __forceinline__ void hankel01_series(double x, double& h0r, double& h0i,
                                                          double& h1r, double& h1i)
 {
+    const auto& T = hankel_tables();
     const double t = 0.25 * x * x;
     const double hx = 0.5 * x;
     const double lg = log(hx);
-    const double j0 = horner<BEM2D_HANKEL_NSER>(BEM2D_SER_J0, t);
-    const double j1 = hx * horner<BEM2D_HANKEL_NSER>(BEM2D_SER_J1, t);
-    const double y0 = kTwoOverPi * lg * j0 + horner<BEM2D_HANKEL_NSER>(BEM2D_SER_Y0, t);
-    const double y1 = kTwoOverPi * lg * j1 - kTwoOverPi / x +
-                      hx * horner<BEM2D_HANKEL_NSER>(BEM2D_SER_Y1, t);
+    const double j0 = horner<BEM2D_HANKEL_NSER>(T.ser[0], t);
+    const double j1 = hx * horner<BEM2D_HANKEL_NSER>(T.ser[1], t);
+    const double y0 = kTwoOverPi * lg * j0 + horner<BEM2D_HANKEL_NSER>(T.ser[2], t);
+    const double y1 = kTwoOverPi * lg * j1 - kTwoOverPi * rcp_refined(x) +
+                      hx * horner<BEM2D_HANKEL_NSER>(T.ser[3], t);
     h0r = j0; h0i = y0; h1r = j1; h1i = y1;
 }
 
-// P_nu, Q_nu of piece P at x (invx = 1/x).
+// P_nu, Q_nu of piece P at 1/x.
 template <int P>
 __host__ __device__ __forceinline__ void hankel_pq(double invx, double& P0, double& Q0, double& P1,
                                                    double& Q1)
 {
-    const double y = BEM2D_PIECE_A2[P] * invx * invx;
-    const double z = (y - BEM2D_PIECE_YM[P]) * BEM2D_PIECE_YHINV[P];
-    P0 = horner<BEM2D_HANKEL_DEG + 1>(BEM2D_FIT_P0[P], z);
-    Q0 = horner<BEM2D_HANKEL_DEG + 1>(BEM2D_FIT_Q0[P], z) * invx;
-    P1 = horner<BEM2D_HANKEL_DEG + 1>(BEM2D_FIT_P1[P], z);
-    Q1 = horner<BEM2D_HANKEL_DEG + 1>(BEM2D_FIT_Q1[P], z) * invx;
+    const auto& T = hankel_tables();
+    constexpr int N = BEM2D_PIECE_DEG[P] + 1;
+    const double y = T.zmap[P][0] * invx * invx;
+    const double z = (y - T.zmap[P][1]) * T.zmap[P][2];
+    P0 = horner<N>(T.fit[P][0], z);
+    Q0 = horner<N>(T.fit[P][1], z) * invx;
+    P1 = horner<N>(T.fit[P][2], z);
+    Q1 = horner<N>(T.fit[P][3], z) * invx;
 }
 
 // Modulus/phase combination for x >= 2.
@@ -117,16 +184,18 @@ __host__ __device__ __forceinline__ void hankel01_combine(double x, double invx,
     h1i = A * (Q1 * cm - P1 * cp);
 }
 
-__host__ __device__ __forceinline__ void hankel01(double x, double& h0r, double& h0i, double& h1r,
-                                                  double& h1i)
+// H0 = (h0r, h0i), H1 = (h1r, h1i) at x > 0, given invx = 1/x.
+__host__ __device__ __forceinline__ void hankel01_inv(double x, double invx, double& h0r,
+                                                      double& h0i, double& h1r, double& h1i)
 {
     if (x < 2.0) {
         hankel01_series(x, h0r, h0i, h1r, h1i);
         return;
     }
-    const double invx = 1.0 / x;
     double P0, Q0, P1, Q1;
-    if (x >= 8.0)
+    if (x >= 20.0)
+        hankel_pq<5>(invx, P0, Q0, P1, Q1);
+    else if (x >= 8.0)
         hankel_pq<4>(invx, P0, Q0, P1, Q1);
     else if (x >= 5.0)
         hankel_pq<3>(invx, P0, Q0, P1, Q1);
@@ -137,6 +206,12 @@ __host__ __device__ __forceinline__ void hankel01(double x, double& h0r, double&
     else
         hankel_pq<0>(invx, P0, Q0, P1, Q1);
     hankel01_combine(x, invx, P0, Q0, P1, Q1, h0r, h0i, h1r, h1i);
+}
+
+__host__ __device__ __forceinline__ void hankel01(double x, double& h0r, double& h0i, double& h1r,
+                                                  double& h1i)
+{
+    hankel01_inv(x, rcp_refined(x), h0r, h0i, h1r, h1i);
 }
 
 } // namespace bem2d_dev

@@ -12,7 +12,11 @@ pytestmark = pytest.mark.gpu
 
 
 def _dev(a):
-    return torch.tensor(np.asfortranarray(a), device="cuda").t().contiguous().t()
+    """Column-major complex128 device copy of a 2-D array."""
+    a = np.asarray(a, dtype=np.complex128)
+    t = torch.empty((a.shape[1], a.shape[0]), dtype=torch.complex128, device="cuda")
+    t.copy_(torch.from_numpy(np.ascontiguousarray(a.T)))
+    return t.t()
 
 
 def _backward(A, X, B):
