@@ -13,7 +13,12 @@ Representation (x > 0):
              z = (y - ym)/yh in [-1, 1], y = (a/x)^2 (the last piece reaches x = infinity, y -> 0).
 The header gives every table as an initializer macro, so that the device code can hold them in
 __constant__ memory (read through scalar loads into SGPR operands) and the host code in constexpr
-arrays.
+arrays.  Two more tables serve the far-field fast path of the assembly:
+  FAR   the last piece [PIECES[-1], inf) again, as plain polynomials in w = 1/x^2 (the affine map
+        to z folded into the coefficients);
+  TRIG  sin r = r + r^3 S(r^2), cos r = 1 + r^2 C(r^2) on |r| <= pi/4, S of degree 5 and C of
+        degree 6, interpolated at Chebyshev nodes in r^2 (near-minimax; 1.4e-17 relative and
+        1.2e-19 absolute), plus 2/pi and pi/2 split into three doubles for the reduction.
 The fits are Chebyshev interpolants computed at 40 digits from mpmath's hankel1 and converted
 to the monomial basis in z; the script reports the float64 Horner error of every table.
 """
@@ -24,8 +29,8 @@ import numpy as np
 
 mp.mp.dps = 40
 HERE = os.path.dirname(os.path.abspath(__file__))
-PIECES = [2.0, 2.6, 3.5, 5.0, 8.0, 20.0]  # piece p covers [PIECES[p], PIECES[p+1]); last [20, inf)
-DEGS = [12, 12, 12, 12, 12, 7]              # polynomial degree per piece
+PIECES = [2.0, 2.6, 3.5, 5.0, 8.0, 20.0, 40.0]  # piece p covers [PIECES[p], PIECES[p+1]); last [40, inf)
+DEGS = [12, 12, 12, 12, 12, 7, 4]             # polynomial degree per piece
 DEG = max(DEGS)
 NSER = 14  # series terms k = 0..NSER-1
 
@@ -162,6 +167,70 @@ def main():
             emit("        {%s}, \\" % ", ".join(fmt(v) for v in cf))
         emit("    }, \\")
     emit("}")
+    # ---- far piece as polynomials in w = 1/x^2: z = (a^2 w - ym)/yh ------------------------------
+    a = mp.mpf(PIECES[-1])
+    degf = DEGS[-1]
+    far = []
+    for nu in (0, 1):
+        def fP(y, nu=nu):
+            return mp.mpf(1) if y == 0 else PQ(nu, a / mp.sqrt(y))[0]
+
+        def fQ(y, nu=nu):
+            if y == 0:
+                return (4 * nu * nu - 1) / mp.mpf(8)
+            x = a / mp.sqrt(y)
+            return PQ(nu, x)[1] * x
+        for f in (fP, fQ):
+            cz, ym, yh = fit_piece(f, mp.mpf(0), mp.mpf(1), degf)
+            # p(z) with z = (a^2 w - ym)/yh  ->  coefficients in w
+            cw = [mp.mpf(0)] * (degf + 1)
+            lin = [-ym / yh, a * a / yh]  # z = lin[0] + lin[1] w
+            powz = [mp.mpf(1)] + [mp.mpf(0)] * degf
+            for kk in range(degf + 1):
+                for i in range(degf + 1):
+                    cw[i] += cz[kk] * powz[i]
+                nxt = [mp.mpf(0)] * (degf + 1)
+                for i in range(degf):
+                    nxt[i] += powz[i] * lin[0]
+                    nxt[i + 1] += powz[i] * lin[1]
+                powz = nxt
+            far.append([float(c) for c in cw])
+            err = 0.0
+            for x in np.linspace(float(a), 4000.0, 400):
+                approx = horner([float(c) for c in cw], 1.0 / (x * x))
+                exact = f(a * a / mp.mpf(x) ** 2)
+                scale = 1 if f is fP else x
+                err = max(err, float(abs((approx - exact) / scale)))
+            worst = max(worst, err)
+            print("far piece w-form: max |err| = %.3g" % err)
+    emit("// far piece [%s, inf): [P0, xQ0, P1, xQ1][ascending powers of w = 1/x^2]" % PIECES[-1])
+    emit("#define BEM2D_HANKEL_FAR_DEG %d" % degf)
+    emit("#define BEM2D_HANKEL_FAR_INIT { \\")
+    for cf in far:
+        emit("    {%s}, \\" % ", ".join(fmt(v) for v in cf))
+    emit("}")
+    # ---- trig kernels -----------------------------------------------------------------------------
+    R2 = (mp.pi / 4) ** 2
+
+    def interp(fn, deg):
+        nodes = [R2 / 2 + R2 / 2 * mp.cos(mp.pi * (j + mp.mpf("0.5")) / (deg + 1))
+                 for j in range(deg + 1)]
+        A = mp.matrix([[u ** kk for kk in range(deg + 1)] for u in nodes])
+        yv = mp.matrix([fn(u) for u in nodes])
+        c = mp.lu_solve(A, yv)
+        return [c[kk] for kk in range(deg + 1)]
+    Sc = interp(lambda u: (mp.sin(mp.sqrt(u)) - mp.sqrt(u)) / (u * mp.sqrt(u)), 5)
+    Cc = interp(lambda u: (mp.cos(mp.sqrt(u)) - 1) / u, 6)
+    pio2 = mp.pi / 2
+    P1 = float(pio2)
+    P2 = float(pio2 - mp.mpf(P1))
+    P3 = float(pio2 - mp.mpf(P1) - mp.mpf(P2))
+    trig = [float(2 / mp.pi), P1, P2, P3] + [float(c) for c in reversed(Sc)] + \
+        [float(c) for c in reversed(Cc)]
+    emit("// [2/pi, P1, P2, P3, S5..S0, C6..C0]: sin r = r + r^3 S(r^2), cos r = 1 + r^2 C(r^2)")
+    emit("#define BEM2D_TRIG_NS 6")
+    emit("#define BEM2D_TRIG_NC 7")
+    emit("#define BEM2D_TRIG_INIT {%s}" % ", ".join(fmt(v) for v in trig))
     emit("// worst float64 Horner error of the fitted tables (absolute, on P and Q): %.3g" % worst)
     with open(os.path.join(HERE, "hankel_tables.h"), "w") as fh:
         fh.write("\n".join(lines) + "\n")

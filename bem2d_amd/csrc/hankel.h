@@ -5,8 +5,8 @@
 //   x < 2    ascending series in t = x^2/4 (A&S 9.1.10 and 9.1.11, n = 0, 1);
 //   x >= 2   H_nu = sqrt(2/(pi x)) (P_nu + i Q_nu) exp(i(x - (2nu+1) pi/4)) (A&S 9.2.5-9.2.6),
 //            P_nu and x Q_nu as polynomials of z(y), y = (a/x)^2, on the pieces [2, 2.6),
-//            [2.6, 3.5), [3.5, 5), [5, 8), [8, 20) (degree 12) and [20, inf) (degree 7, the
-//            asymptotic regime where almost all far-field pairs fall).
+//            [2.6, 3.5), [3.5, 5), [5, 8), [8, 20) (degree 12), [20, 40) (degree 7) and
+//            [40, inf) (degree 4, the asymptotic regime where most far-field pairs fall).
 // On the device every coefficient lives in one __constant__ table and every index is a
 // compile-time constant, so the coefficients reach v_fma_f64 as SGPR operands through scalar loads
 // (instruction literals would each cost two v_mov_b32 per use).  sin/cos of the phase use
@@ -19,6 +19,7 @@
 
 namespace bem2d_dev {
 
+constexpr double kPi = 3.14159265358979323846;
 constexpr double kTwoOverPi = 0.63661977236758134308;
 constexpr double kInvPi = 0.31830988618379067154;
 
@@ -167,21 +168,51 @@ __host__ __device__ __forceinline__ void hankel_pq(double invx, double& P0, doub
     Q1 = horner<N>(T.fit[P][3], z) * invx;
 }
 
-// Modulus/phase combination for x >= 2.
+// sin and cos of x - pi/4 for 0 <= x < 2^30: the reduction of sincos_reduced applied to
+// x - (n + 1/2) pi/2 with n = rint(2x/pi - 1/2), so the pi/4 shift costs nothing.
+__host__ __device__ __forceinline__ void sincos_minus_quarter(double x, double& s, double& c)
+{
+    const auto& T = hankel_tables().trig;
+    const double n = rint(fma(x, T[0], -0.5));
+    const double m = n + 0.5;
+    double r = fma(-m, T[1], x);
+    r = fma(-m, T[2], r);
+    r = fma(-m, T[3], r);
+    const double r2 = r * r;
+    double ps = T[4];
+#pragma unroll
+    for (int i = 5; i < 12; ++i) ps = fma_vvs(ps, r2, T[i]);
+    const double sr = fma(ps * r2, r, r);
+    double pc = T[12];
+#pragma unroll
+    for (int i = 13; i < 21; ++i) pc = fma_vvs(pc, r2, T[i]);
+    const double cr = fma(pc, r2, 1.0);
+    const int q = ((int)(long long)n) & 3;
+    const double a = (q & 1) ? cr : sr;
+    const double b = (q & 1) ? sr : cr;
+    s = (q & 2) ? -a : a;
+    c = ((q + 1) & 2) ? -b : b;
+}
+
+// Modulus/phase combination for x >= 2:
+//   H0 = sqrt(2/(pi x)) (P0 + i Q0) (C + i S),  H1 = sqrt(2/(pi x)) (P1 + i Q1) (S - i C),
+// with C + i S = exp(i (x - pi/4)).
 __host__ __device__ __forceinline__ void hankel01_combine(double x, double invx, double P0,
                                                           double Q0, double P1, double Q1,
                                                           double& h0r, double& h0i, double& h1r,
                                                           double& h1i)
 {
-    double s, c;
-    sincos_reduced(x, s, c);
-    const double A = sqrt(kInvPi * invx); // sqrt(2/(pi x)) / sqrt(2)
-    const double cp = c + s;             // sqrt(2) cos(x - pi/4)
-    const double cm = s - c;             // sqrt(2) sin(x - pi/4)
-    h0r = A * (P0 * cp - Q0 * cm);
-    h0i = A * (P0 * cm + Q0 * cp);
-    h1r = A * (P1 * cm + Q1 * cp);
-    h1i = A * (Q1 * cm - P1 * cp);
+    double S, C;
+    sincos_minus_quarter(x, S, C);
+#ifdef __HIP_DEVICE_COMPILE__
+    const double A = rsqrt_refined(0.5 * kPi * x); // sqrt(2/(pi x))
+#else
+    const double A = sqrt(2.0 * kInvPi * invx);
+#endif
+    h0r = A * (P0 * C - Q0 * S);
+    h0i = A * (P0 * S + Q0 * C);
+    h1r = A * (P1 * S + Q1 * C);
+    h1i = A * (Q1 * S - P1 * C);
 }
 
 // H0 = (h0r, h0i), H1 = (h1r, h1i) at x > 0, given invx = 1/x.
@@ -193,7 +224,9 @@ __host__ __device__ __forceinline__ void hankel01_inv(double x, double invx, dou
         return;
     }
     double P0, Q0, P1, Q1;
-    if (x >= 20.0)
+    if (x >= 40.0)
+        hankel_pq<6>(invx, P0, Q0, P1, Q1);
+    else if (x >= 20.0)
         hankel_pq<5>(invx, P0, Q0, P1, Q1);
     else if (x >= 8.0)
         hankel_pq<4>(invx, P0, Q0, P1, Q1);

@@ -2,10 +2,10 @@
 //
 // Factorisation (bem2d_lu32_factor): A32 = (complex64) A64, then a blocked right-looking LU with
 // 64-wide panels and no row interchanges.  Per panel: the 64x64 diagonal block is factorised by one
-// workgroup in LDS, which also forms the inverses of its unit-lower and upper triangles; the block
-// column below and the block row to the right are multiplied by those inverses; the trailing matrix
-// receives the rank-64 update A22 -= L21 U12 as a complex GEMM on v_mfma_f32_16x16x4f32 with FP32
-// accumulation (four real MFMA products per complex product).  Without interchanges the panel has
+// workgroup with the tile in registers, which also forms the inverses of its unit-lower and upper
+// triangles; the block column below and the block row to the right are multiplied by those
+// inverses; the trailing matrix receives the rank-64 update A22 -= L21 U12 as a complex GEMM on
+// v_mfma_f32_16x16x4f32 with FP32 accumulation (four real MFMA products per complex product).  Without interchanges the panel has
 // no sequential pivot search -- the step that makes partial-pivoting panels latency bound on a GPU.
 // A zero or non-finite pivot is flagged and the refinement then reports failure, after which the
 // one-shot solver falls back to the FP64 LU.
@@ -30,7 +30,7 @@ namespace {
 constexpr int NB = 64;             // panel / block width
 constexpr int LDT = NB + 1;        // padded LDS row of a 64x64 complex tile (float2 units)
 constexpr int kMaxSolveRhs = 8;    // right-hand sides handled per triangular sweep
-constexpr int kResidChunks = 32;   // column chunks of the FP64 residual
+constexpr int kResidChunks = 64;   // column chunks of the FP64 residual
 constexpr long long kSpinLimit = 1ll << 26;
 
 typedef float v4f __attribute__((ext_vector_type(4)));
@@ -101,59 +101,185 @@ __global__ void rowsum_max_kernel(int64_t n, int nchunk, const double* __restric
 // ---- panel: diagonal block LU + triangular inverses ---------------------------------------------
 // One workgroup of 256 threads.  Block rows/cols k0 .. k0+kb.  Linv/Uinv receive the 64x64 inverses
 // (identity-padded when kb < 64), column-major with leading dimension 64.
+// Thread t owns the 4x4 sub-block rows 4 (t % 16) .. +3, cols 4 (t / 16) .. +3 of the tile, in
+// registers.  Each elimination step publishes the pivot row and column through ping-pong LDS
+// buffers, so a step costs one barrier; the register updates are branch-free (selects instead of
+// per-element conditions, which would each become an exec-mask branch).
+__device__ __forceinline__ float2 fast_crecip(float2 a)
+{
+    const float d = __builtin_amdgcn_rcpf(a.x * a.x + a.y * a.y);
+    return make_float2(a.x * d, -a.y * d);
+}
+
+// Thread t = 64 w + l owns rows 16 w + 4 (l % 4) .. +3 and cols 4 (l / 4) .. +3: each wavefront
+// holds 16 contiguous rows, so at every step the wavefronts whose rows are all inactive skip the
+// update with a uniform branch (about half the work of the triangular sweeps).  The rows of U^{-1}
+// are scaled by 1/U[r][r] once at the end instead of at every step.
 __global__ __launch_bounds__(256) void diag_lu_kernel(float2* __restrict__ A, int64_t ld,
                                                       int64_t k0, int kb,
                                                       float2* __restrict__ Linv,
                                                       float2* __restrict__ Uinv,
                                                       int* __restrict__ bad)
 {
-    __shared__ float2 T[NB * LDT]; // T[r + LDT*c]
-    __shared__ float2 X[NB * LDT]; // L^{-1}
-    __shared__ float2 Z[NB * LDT]; // U^{-1}
+    __shared__ float2 rowbuf[2][NB], colbuf[2][NB];   // factorisation: pivot row / column
+    __shared__ float2 xrow[2][NB], lcol[2][NB];       // L^{-1} sweep: row j of X, column j of L
+    __shared__ float2 zrow[2][NB], ucol[2][NB];       // U^{-1} sweep: row j of Z~, column j of U
+    __shared__ float2 udiag_inv[NB];
     const int t = threadIdx.x;
-    for (int e = t; e < NB * NB; e += 256) {
-        const int r = e % NB, c = e / NB;
-        float2 v = make_float2(r == c ? 1.f : 0.f, 0.f);
-        if (r < kb && c < kb) v = A[(k0 + r) + ld * (k0 + c)];
-        T[r + LDT * c] = v;
-    }
-    __syncthreads();
-    for (int j = 0; j < kb; ++j) {
-        const float2 p = T[j + LDT * j];
-        if (t == 0 && !(p.x * p.x + p.y * p.y > 0.f && isfinite(p.x) && isfinite(p.y))) *bad = 1;
-        const float2 ip = crecip(p);
-        for (int r = j + 1 + t; r < kb; r += 256) T[r + LDT * j] = cmul(T[r + LDT * j], ip);
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int l = t & 63;
+    const int r0 = 16 * w + 4 * (l & 3), c0 = 4 * (l >> 2);
+    const int wr0 = 16 * w, wr1 = 16 * w + 15; // rows of this wavefront
+    const float2 zero = make_float2(0.f, 0.f);
+    float2 T[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int r = r0 + a, c = c0 + b;
+            float2 v = make_float2(r == c ? 1.f : 0.f, 0.f);
+            if (r < kb && c < kb) v = A[(k0 + r) + ld * (k0 + c)];
+            T[a][b] = v;
+        }
+    // ---- LU without interchanges: T = L U in place.  Row j / column j of the current matrix are
+    // published into ping-pong buffers (stale entries are masked by the readers); only compile-time
+    // indices into the register tiles (a runtime index would put them in scratch). ----
+    auto publish_lu = [&](int j) {
+        const int buf = j & 1;
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+            if (r0 + a == j) {
+#pragma unroll
+                for (int b = 0; b < 4; ++b) rowbuf[buf][c0 + b] = T[a][b];
+            }
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+            if (c0 + b == j) {
+#pragma unroll
+                for (int a = 0; a < 4; ++a) colbuf[buf][r0 + a] = T[a][b];
+            }
+    };
+    publish_lu(0);
+    for (int j = 0; j < NB; ++j) {
         __syncthreads();
-        const int w = kb - j - 1;
-        for (int e = t; e < w * w; e += 256) {
-            const int r = j + 1 + e % w, c = j + 1 + e / w;
-            T[r + LDT * c] = cfms(T[r + LDT * c], T[r + LDT * j], T[j + LDT * c]);
+        const int buf = j & 1;
+        if (wr1 > j) {
+            const float2 p = rowbuf[buf][j];
+            if (t == 0 && j < kb &&
+                !(p.x * p.x + p.y * p.y > 0.f && isfinite(p.x) && isfinite(p.y)))
+                *bad = 1;
+            const float2 ip = fast_crecip(p);
+            float2 m[4], rb[4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const float2 cv = colbuf[buf][r0 + a];
+                m[a] = (r0 + a > j) ? cmul(cv, ip) : zero;
+            }
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const float2 rv = rowbuf[buf][c0 + b];
+                rb[b] = (c0 + b > j) ? rv : zero;
+            }
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const float2 u = cfms(T[a][b], m[a], rb[b]);
+                    T[a][b] = (c0 + b == j && r0 + a > j) ? m[a] : u;
+                }
         }
+        if (j + 1 < NB) publish_lu(j + 1);
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int r = r0 + a, c = c0 + b;
+            if (r < kb && c < kb) A[(k0 + r) + ld * (k0 + c)] = T[a][b];
+            if (r == c) udiag_inv[r] = fast_crecip(T[a][b]);
+        }
+    // ---- inverses by row operations, both sweeps in the same 64 steps ----
+    // X = L^{-1}: X = I; for j ascending, rows r > j: X[r][:] -= L[r][j] X[j][:].
+    // Z~: Z~ = I; for j descending, rows r < j: Z~[r][:] -= (U[r][j] / U[j][j]) Z~[j][:];
+    // then U^{-1}[r][:] = Z~[r][:] / U[r][r].
+    float2 X[4][4], Z[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            X[a][b] = make_float2(r0 + a == c0 + b ? 1.f : 0.f, 0.f);
+            Z[a][b] = X[a][b];
+        }
+    auto publish_inv = [&](int s) {
+        const int buf = s & 1;
+        const int jl = s, ju = NB - 1 - s;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            if (r0 + a == jl) {
+#pragma unroll
+                for (int b = 0; b < 4; ++b) xrow[buf][c0 + b] = X[a][b];
+            }
+            if (r0 + a == ju) {
+#pragma unroll
+                for (int b = 0; b < 4; ++b) zrow[buf][c0 + b] = Z[a][b];
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            if (c0 + b == jl) {
+#pragma unroll
+                for (int a = 0; a < 4; ++a) lcol[buf][r0 + a] = T[a][b];
+            }
+            if (c0 + b == ju) {
+#pragma unroll
+                for (int a = 0; a < 4; ++a) ucol[buf][r0 + a] = T[a][b];
+            }
+        }
+    };
+    publish_inv(0); // (the barrier at the top of the loop also orders the udiag_inv writes)
+    for (int s2 = 0; s2 < NB; ++s2) {
         __syncthreads();
-    }
-    for (int e = t; e < kb * kb; e += 256) {
-        const int r = e % kb, c = e / kb;
-        A[(k0 + r) + ld * (k0 + c)] = T[r + LDT * c];
-    }
-    // Inverses: threads 0..63 form column c of L^{-1} (forward substitution, unit diagonal),
-    // threads 64..127 column c of U^{-1} (back substitution).  Columns are independent.
-    if (t < NB) {
-        const int c = t;
-        for (int r = 0; r < NB; ++r) {
-            float2 s = make_float2(r == c ? 1.f : 0.f, 0.f);
-            for (int q = c; q < r; ++q) s = cfms(s, T[r + LDT * q], X[q + LDT * c]);
-            X[r + LDT * c] = r < c ? make_float2(0.f, 0.f) : s;
+        const int buf = s2 & 1;
+        const int jl = s2, ju = NB - 1 - s2;
+        if (wr1 > jl) {
+            float2 lx[4], xr[4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const float2 lv = lcol[buf][r0 + a];
+                lx[a] = r0 + a > jl ? lv : zero;
+            }
+#pragma unroll
+            for (int b = 0; b < 4; ++b) xr[b] = xrow[buf][c0 + b];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) X[a][b] = cfms(X[a][b], lx[a], xr[b]);
         }
-        for (int r = 0; r < NB; ++r) Linv[r + NB * c] = X[r + LDT * c];
-    } else if (t < 2 * NB) {
-        const int c = t - NB;
-        for (int r = NB - 1; r > c; --r) Z[r + LDT * c] = make_float2(0.f, 0.f);
-        for (int r = c; r >= 0; --r) {
-            float2 s = make_float2(r == c ? 1.f : 0.f, 0.f);
-            for (int q = r + 1; q <= c; ++q) s = cfms(s, T[r + LDT * q], Z[q + LDT * c]);
-            Z[r + LDT * c] = cmul(s, crecip(T[r + LDT * r]));
+        if (wr0 < ju) {
+            const float2 iu = udiag_inv[ju];
+            float2 uz[4], zr[4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const float2 uv = ucol[buf][r0 + a];
+                uz[a] = r0 + a < ju ? cmul(uv, iu) : zero;
+            }
+#pragma unroll
+            for (int b = 0; b < 4; ++b) zr[b] = zrow[buf][c0 + b];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) Z[a][b] = cfms(Z[a][b], uz[a], zr[b]);
         }
-        for (int r = 0; r < NB; ++r) Uinv[r + NB * c] = Z[r + LDT * c];
+        if (s2 + 1 < NB) publish_inv(s2 + 1);
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        const float2 iu = udiag_inv[r0 + a];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            Linv[(r0 + a) + NB * (c0 + b)] = X[a][b];
+            Uinv[(r0 + a) + NB * (c0 + b)] = cmul(Z[a][b], iu);
+        }
     }
 }
 
@@ -290,6 +416,9 @@ __global__ __launch_bounds__(256) void trailing_update_kernel(float2* __restrict
 // Each workgroup owns one 64-row block; the first workgroups dispatched own the blocks without
 // dependencies, so every wait is on a block that is already running or done.  A wait gives up
 // after kSpinLimit polls and raises *err, so the kernel always terminates.
+// Thread (row = t % 64, grp = t / 64) owns 16 consecutive k of each 64-wide column block: the
+// matrix block is loaded into registers before the flag wait (it does not depend on it), and the
+// partial sums stay in registers until the end, so a dependency costs one wait and one barrier.
 template <bool LOWER>
 __global__ __launch_bounds__(256) void block_trsv_kernel(const float2* __restrict__ F, int64_t ld,
                                                          int64_t n, int nblk,
@@ -299,26 +428,37 @@ __global__ __launch_bounds__(256) void block_trsv_kernel(const float2* __restric
                                                          unsigned long long epoch,
                                                          int* __restrict__ err)
 {
-    __shared__ float2 acc[NB][kMaxSolveRhs];
     __shared__ float2 part[4][NB][kMaxSolveRhs];
-    __shared__ float2 yc[NB][kMaxSolveRhs];
+    __shared__ float2 accs[NB][kMaxSolveRhs];
     const int t = threadIdx.x;
     const int b = LOWER ? (int)blockIdx.x : nblk - 1 - (int)blockIdx.x;
     const int64_t rb = (int64_t)b * NB;
     const int lenb = (int)min((int64_t)NB, n - rb);
-    const int row = t % NB, grp = t / NB; // 4 groups of 64 threads split the k range
-    for (int e = t; e < NB * nrhs; e += 256) {
-        const int r = e % NB, h = e / NB;
-        acc[r][h] = r < lenb ? Y[(rb + r) + ldy * h] : make_float2(0.f, 0.f);
-    }
-    __syncthreads();
-    const int c_begin = LOWER ? 0 : b + 1;
-    const int c_end = LOWER ? b : nblk;
-    for (int c = c_begin; c < c_end; ++c) {
+    const int row = t % NB, grp = t / NB, k0 = 16 * grp;
+    const float2 zero = make_float2(0.f, 0.f);
+    float2 s[kMaxSolveRhs];
+#pragma unroll
+    for (int h = 0; h < kMaxSolveRhs; ++h) s[h] = zero;
+    // the diagonal-block inverse, this thread's 16 columns of row `row`
+    float2 dv[16];
+    const float2* D = Dinv + (int64_t)b * NB * NB;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) dv[q] = D[row + NB * (k0 + q)];
+    // Dependencies are consumed in the order they complete: ascending for the lower sweep,
+    // descending for the upper sweep (block b + 1 is the last of b's dependencies to finish).
+    const int ndep = LOWER ? b : nblk - 1 - b;
+    for (int q = 0; q < ndep; ++q) {
+        const int c = LOWER ? q : nblk - 1 - q;
+        const int64_t cb = (int64_t)c * NB;
+        const int lenc = (int)min((int64_t)NB, n - cb);
+        float2 fv[16];
+#pragma unroll
+        for (int m = 0; m < 16; ++m)
+            fv[m] = (row < lenb && k0 + m < lenc) ? F[(rb + row) + ld * (cb + k0 + m)] : zero;
         if (t == 0) {
             long long spins = 0;
             while (__hip_atomic_load(&flags[c], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
-                __builtin_amdgcn_s_sleep(2);
+                __builtin_amdgcn_s_sleep(1);
                 if (++spins > kSpinLimit) {
                     atomicExch(err, 1);
                     break;
@@ -326,54 +466,63 @@ __global__ __launch_bounds__(256) void block_trsv_kernel(const float2* __restric
             }
         }
         __syncthreads();
-        const int64_t cb = (int64_t)c * NB;
-        const int lenc = (int)min((int64_t)NB, n - cb);
-        for (int e = t; e < NB * nrhs; e += 256) {
-            const int r = e % NB, h = e / NB;
-            // plain loads: the agent-scope acquire above invalidated this CU's vector L1
-            yc[r][h] = r < lenc ? Y[(cb + r) + ldy * h] : make_float2(0.f, 0.f);
-        }
-        __syncthreads();
-        float2 s[kMaxSolveRhs];
+        // plain loads: the agent-scope acquire above invalidated this CU's vector L1
 #pragma unroll
-        for (int h = 0; h < kMaxSolveRhs; ++h) s[h] = make_float2(0.f, 0.f);
-        if (row < lenb) {
-            for (int k = grp; k < lenc; k += 4) {
-                const float2 f = F[(rb + row) + ld * (cb + k)];
+        for (int m = 0; m < 16; ++m) {
+            if (k0 + m < lenc) {
 #pragma unroll
                 for (int h = 0; h < kMaxSolveRhs; ++h)
                     if (h < nrhs) {
-                        const float2 y = yc[k][h];
-                        s[h].x += f.x * y.x - f.y * y.y;
-                        s[h].y += f.x * y.y + f.y * y.x;
+                        const float2 y = Y[(cb + k0 + m) + ldy * h];
+                        s[h].x += fv[m].x * y.x - fv[m].y * y.y;
+                        s[h].y += fv[m].x * y.y + fv[m].y * y.x;
                     }
             }
         }
-#pragma unroll
-        for (int h = 0; h < kMaxSolveRhs; ++h) part[grp][row][h] = s[h];
-        __syncthreads();
-        for (int e = t; e < NB * nrhs; e += 256) {
-            const int r = e % NB, h = e / NB;
-            float2 a = acc[r][h];
-            for (int g = 0; g < 4; ++g) {
-                a.x -= part[g][r][h].x;
-                a.y -= part[g][r][h].y;
-            }
-            acc[r][h] = a;
-        }
-        __syncthreads();
     }
-    // y_b = Dinv_b acc
-    const float2* D = Dinv + (int64_t)b * NB * NB;
+    // acc = y_b - sum_c F_bc y_c  (rows of this block)
+#pragma unroll
+    for (int h = 0; h < kMaxSolveRhs; ++h) part[grp][row][h] = s[h];
+    __syncthreads();
     for (int e = t; e < NB * nrhs; e += 256) {
         const int r = e % NB, h = e / NB;
-        float2 s = make_float2(0.f, 0.f);
-        for (int k = 0; k < NB; ++k) {
-            const float2 d = D[r + NB * k], a = acc[k][h];
-            s.x += d.x * a.x - d.y * a.y;
-            s.y += d.x * a.y + d.y * a.x;
+        float2 a = r < lenb ? Y[(rb + r) + ldy * h] : zero;
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            a.x -= part[g][r][h].x;
+            a.y -= part[g][r][h].y;
         }
-        if (r < lenb) Y[(rb + r) + ldy * h] = s;
+        accs[r][h] = a;
+    }
+    __syncthreads();
+    // y_b = Dinv_b acc: partial over this thread's 16 k, then summed over the 4 groups
+#pragma unroll
+    for (int h = 0; h < kMaxSolveRhs; ++h) {
+        float2 v = zero;
+        if (h < nrhs) {
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const float2 a = accs[k0 + q][h];
+                v.x += dv[q].x * a.x - dv[q].y * a.y;
+                v.y += dv[q].x * a.y + dv[q].y * a.x;
+            }
+        }
+        s[h] = v;
+    }
+    __syncthreads(); // part[] is reused
+#pragma unroll
+    for (int h = 0; h < kMaxSolveRhs; ++h) part[grp][row][h] = s[h];
+    __syncthreads();
+    if (grp == 0 && row < lenb) {
+        for (int h = 0; h < nrhs; ++h) {
+            float2 v = part[0][row][h];
+#pragma unroll
+            for (int g = 1; g < 4; ++g) {
+                v.x += part[g][row][h].x;
+                v.y += part[g][row][h].y;
+            }
+            Y[(rb + row) + ldy * h] = v;
+        }
     }
     __threadfence();
     __syncthreads();
@@ -393,6 +542,7 @@ __global__ void residual_partial_kernel(int64_t n, const double2* __restrict__ A
     double2 s[kMaxSolveRhs];
 #pragma unroll
     for (int h = 0; h < kMaxSolveRhs; ++h) s[h] = make_double2(0.0, 0.0);
+#pragma unroll 4
     for (int64_t j = j0; j < j1; ++j) {
         const double2 a = A[i + lda * j];
 #pragma unroll
@@ -403,7 +553,9 @@ __global__ void residual_partial_kernel(int64_t n, const double2* __restrict__ A
                 s[h].y = fma(a.x, x.y, fma(a.y, x.x, s[h].y));
             }
     }
-    for (int h = 0; h < nrhs; ++h) part[((int64_t)c * nrhs + h) * n + i] = s[h];
+#pragma unroll
+    for (int h = 0; h < kMaxSolveRhs; ++h)
+        if (h < nrhs) part[((int64_t)c * nrhs + h) * n + i] = s[h];
 }
 
 // R = B - sum_c part[c]; also per-column max |R_i| (atomic max on the bit pattern of a
