@@ -86,32 +86,214 @@ __global__ void prep_nodes_kernel(int64_t n, int nq, const int32_t* __restrict__
     nodes[NODE_SW * plane + t] = len * gw[a];
 }
 
-constexpr int kRowsPerBlock = 64;   // one wavefront of rows (test elements)
-constexpr int kWavesPerBlock = 4;
-constexpr int kColsPerWave = 4;     // trial elements handled in turn by each wavefront
-constexpr int kColsPerBlock = kWavesPerBlock * kColsPerWave;
+// Element bounding circles, three planes of n after the node planes: centre of the Gauss nodes
+// and the largest node distance from it (rounded up).  For any nodes x of e and y of f,
+// |x - y| >= |c_e - c_f| - r_e - r_f.
+enum { BOUND_CX = 0, BOUND_CY = 1, BOUND_R = 2, BOUND_PLANES = 3 };
 
-// Regular pairs (Integrate with Gauss2D, lib/bem2d_quadrature.h:37-65, :283-286).
-// Block = 64 rows x 16 columns; lane = row i (coalesced column-major stores), the trial element's
-// node data sit in LDS and are read as wave-uniform broadcasts.  Coincident / adjacent pairs are
-// skipped here and written by assemble_special_kernel.  NQ = nq at compile time (0: runtime nq).
+__global__ void prep_bounds_kernel(int64_t n, int nq, const double* __restrict__ nodes,
+                                   double* __restrict__ bounds)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const int64_t plane = n * nq;
+    double sx = 0.0, sy = 0.0;
+    for (int a = 0; a < nq; ++a) {
+        sx += nodes[NODE_X * plane + (int64_t)a * n + e];
+        sy += nodes[NODE_Y * plane + (int64_t)a * n + e];
+    }
+    const double cx = sx / nq, cy = sy / nq;
+    double r2 = 0.0;
+    for (int a = 0; a < nq; ++a) {
+        const double dx = nodes[NODE_X * plane + (int64_t)a * n + e] - cx;
+        const double dy = nodes[NODE_Y * plane + (int64_t)a * n + e] - cy;
+        r2 = fmax(r2, dx * dx + dy * dy);
+    }
+    bounds[BOUND_CX * n + e] = cx;
+    bounds[BOUND_CY * n + e] = cy;
+    bounds[BOUND_R * n + e] = sqrt(r2) * (1.0 + 1e-12) + 1e-300;
+}
+
+// Kernel values of one regular node pair for both orders of its element pair: (kr, ki) with the
+// test node x on element i and the trial node y on element j, (mr, mi) with test node y on j and
+// trial node x on i.  The two share |x - y| and with it H0 and H1; only the normal factor differs,
+// and the negated difference y - x is exact, so each value is the one kernel_eval gives for its
+// order.  FAR: k|x - y| >= kHankelFarX is known and the asymptotic piece is used without a test.
+template <int KIND, bool FAR>
+__device__ __forceinline__ void kernel_pair(double k, double kinv, double k4, double eta4,
+                                            double xx, double xy, double yx, double yy,
+                                            double n1x, double n1y, double n2x, double n2y,
+                                            double& kr, double& ki, double& mr, double& mi)
+{
+    if (KIND == BEM2D_KERNEL_ID || KIND == BEM2D_KERNEL_NONE) {
+        kr = mr = (KIND == BEM2D_KERNEL_ID) ? 1.0 : 0.0;
+        ki = mi = 0.0;
+        return;
+    }
+    double wx, wy, d2, nw1, nw2;
+    {
+#pragma clang fp contract(off)
+        wx = xx - yx; // x - y
+        wy = xy - yy;
+        d2 = wx * wx + wy * wy;
+        nw1 = n1x * wx + n1y * wy; // n_x.(x - y)
+        nw2 = n2x * wx + n2y * wy; // n_y.(x - y)
+    }
+    const double rd = rsqrt_refined(d2);
+    const double d = d2 * rd;
+    const double x = k * d, invx = kinv * rd;
+    if (FAR) {
+        // H0 = A (u0r + i u0i), H1 = A (u1r + i u1i) as in hankel01_combine, with the amplitude
+        // A = sqrt(2/(pi x)) folded into the scalar factors of the kernel
+        double P0, Q0, P1, Q1, S, C;
+        hankel_pq_far(invx, P0, Q0, P1, Q1);
+        sincos_minus_quarter(x, S, C);
+        const double A = rsqrt_refined(0.5 * kPi * x);
+        const double u0r = P0 * C - Q0 * S, u0i = P0 * S + Q0 * C;
+        const double u1r = P1 * S + Q1 * C, u1i = Q1 * S - P1 * C;
+        if (KIND == BEM2D_KERNEL_SL) {
+            const double a4 = 0.25 * A;
+            kr = mr = -a4 * u0i;
+            ki = mi = a4 * u0r;
+            return;
+        }
+        const double ka = k4 * A * rd;
+        const double c = ka * (KIND == BEM2D_KERNEL_DL ? -nw2 : nw1);
+        const double cm = ka * (KIND == BEM2D_KERNEL_DL ? nw1 : -nw2);
+        double gr = 0.0, gi = 0.0;
+        if (KIND == BEM2D_KERNEL_COMBINED) {
+            const double ea = eta4 * A;
+            gr = ea * u0r;
+            gi = ea * u0i;
+        }
+        kr = fma(c, u1i, gr);
+        ki = fma(-c, u1r, gi);
+        mr = fma(cm, u1i, gr);
+        mi = fma(-cm, u1r, gi);
+        return;
+    }
+    double h0r, h0i, h1r, h1i;
+    hankel01_inv(x, invx, h0r, h0i, h1r, h1i);
+    if (KIND == BEM2D_KERNEL_SL) {
+        kr = mr = -0.25 * h0i;
+        ki = mi = 0.25 * h0r;
+        return;
+    }
+    // -(ik/4) c H1: DL c = n_y.(y - x)/d, CDL c = n_x.(x - y)/d; in the swapped order the test
+    // and trial normals trade places and x - y changes sign.
+    const double c = (KIND == BEM2D_KERNEL_DL ? -nw2 : nw1) * rd;
+    const double cm = (KIND == BEM2D_KERNEL_DL ? nw1 : -nw2) * rd;
+    kr = k4 * c * h1i;
+    ki = -k4 * c * h1r;
+    mr = k4 * cm * h1i;
+    mi = -k4 * cm * h1r;
+    if (KIND == BEM2D_KERNEL_COMBINED) {
+        const double gr = eta4 * h0r, gi = eta4 * h0i;
+        kr += gr;
+        ki += gi;
+        mr += gr;
+        mi += gi;
+    }
+}
+
+// Tile shape of the regular assembly: one wavefront of 64 rows per trial column gives the
+// dispatcher the finest work units (columns differ in cost by up to 3x: near pairs take the
+// piecewise path); measured on MI355X, kite N = 4000: 4 columns per wave 12.8 ms, 2: 10.6, 1: 9.5.
+#ifndef BEM2D_ASM_WAVES
+#define BEM2D_ASM_WAVES 1
+#endif
+#ifndef BEM2D_ASM_COLS
+#define BEM2D_ASM_COLS 1
+#endif
+constexpr int kRowsPerBlock = 64;   // one wavefront of rows (test elements)
+constexpr int kWavesPerBlock = BEM2D_ASM_WAVES;
+constexpr int kColsPerWave = BEM2D_ASM_COLS; // trial elements handled in turn by each wavefront
+constexpr int kColsPerBlock = kWavesPerBlock * kColsPerWave;
+constexpr int kColBlocksPerRowBlock = kRowsPerBlock / kColsPerBlock;
+static_assert(kRowsPerBlock % kColsPerBlock == 0 && kColBlocksPerRowBlock % 2 == 0,
+              "a row block must hold an even number of column blocks");
+
+// Tiles of the upper triangle: row block p (64 rows) pairs with column blocks q >= m p, where
+// m = kColBlocksPerRowBlock.  Row block p holds Q - m p tiles, so tile t lies in the largest p with
+// tiles_before(p) = p Q - (m/2) p (p - 1) <= t.
+__host__ __device__ __forceinline__ int64_t tiles_before(int64_t p, int64_t Q)
+{
+    return p * Q - (int64_t)(kColBlocksPerRowBlock / 2) * p * (p - 1);
+}
+__host__ __device__ __forceinline__ int64_t upper_tile_count(int64_t n)
+{
+    const int64_t P = (n + kRowsPerBlock - 1) / kRowsPerBlock;
+    return tiles_before(P, (n + kColsPerBlock - 1) / kColsPerBlock);
+}
+
+// Node-pair sums of one element pair (i, j): direct (ar, ai) = sum_a w_a sum_b w_b K(x_a, y_b) for
+// entry (i, j), in the order of the reference's Integrate; mirror (mr, mi) for entry (j, i).
+template <int KIND, int NQ, bool FAR>
+__device__ __forceinline__ void pair_sums(int nq, int64_t n, int64_t i, const double* __restrict__ nodes,
+                                          const double* sc, double k, double kinv, double k4,
+                                          double eta4, double& ar, double& ai, double& mr,
+                                          double& mi)
+{
+    const int64_t plane = n * nq;
+    ar = ai = mr = mi = 0.0;
+    for (int a = 0; a < nq; ++a) {
+        const int64_t o = (int64_t)a * n + i;
+        const double xx = nodes[NODE_X * plane + o], xy = nodes[NODE_Y * plane + o];
+        const double n1x = nodes[NODE_NX * plane + o], n1y = nodes[NODE_NY * plane + o];
+        const double swa = nodes[NODE_SW * plane + o];
+        double br = 0.0, bi = 0.0, cr = 0.0, ci = 0.0;
+        for (int b = 0; b < nq; ++b) {
+            const double* y = sc + b * 5;
+            double kr, ki, kmr, kmi;
+            kernel_pair<KIND, FAR>(k, kinv, k4, eta4, xx, xy, y[0], y[1], n1x, n1y, y[2], y[3],
+                                   kr, ki, kmr, kmi);
+            br = fma(kr, y[4], br);
+            bi = fma(ki, y[4], bi);
+            cr = fma(kmr, y[4], cr);
+            ci = fma(kmi, y[4], ci);
+        }
+        ar = fma(br, swa, ar);
+        ai = fma(bi, swa, ai);
+        mr = fma(cr, swa, mr);
+        mi = fma(ci, swa, mi);
+    }
+}
+
+// Regular pairs (Integrate with Gauss2D, lib/bem2d_quadrature.h:37-65, :283-286), both orders of
+// each element pair from one evaluation of the Hankel functions: a block takes tile t of the upper
+// triangle (64 rows i x kColsPerBlock columns j, j > i) and writes A(i, j) (lane = row, coalesced)
+// and A(j, i).
+// The trial element's node data sit in LDS and are read as wave-uniform broadcasts.  A wavefront
+// whose 64 pairs of one column are all at k|x - y| >= kHankelFarX by the bounding circles takes the
+// branch-free asymptotic loop.  Coincident / adjacent pairs are skipped here and written by
+// assemble_special_kernel.  NQ = nq at compile time (0: runtime nq).
 template <int KIND, int NQ>
-__global__ __launch_bounds__(256) void assemble_regular_kernel(
-    int64_t n, int nq_rt, const double* __restrict__ nodes, const int64_t* __restrict__ index,
-    const int64_t* __restrict__ next, const int64_t* __restrict__ prev, double k, double eta,
-    double kscale, double2* __restrict__ A, int64_t lda)
+__global__ __launch_bounds__(64 * kWavesPerBlock) void assemble_regular_kernel(
+    int64_t n, int nq_rt, const double* __restrict__ nodes, const double* __restrict__ bounds,
+    const int64_t* __restrict__ index, const int64_t* __restrict__ next,
+    const int64_t* __restrict__ prev, double k, double eta, double kscale,
+    double2* __restrict__ A, int64_t lda)
 {
     extern __shared__ double sh[]; // [kColsPerBlock][nq][5]
     const int nq = NQ ? NQ : nq_rt;
-    constexpr int kUnroll = 1;
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int64_t i = (int64_t)blockIdx.x * kRowsPerBlock + lane;
-    const int64_t j0 = (int64_t)blockIdx.y * kColsPerBlock;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t Q = (n + kColsPerBlock - 1) / kColsPerBlock;
+    const int64_t P = (n + kRowsPerBlock - 1) / kRowsPerBlock;
+    const int64_t t = blockIdx.x;
+    constexpr double m = kColBlocksPerRowBlock;
+    const double qq = (double)Q + 0.5 * m; // smaller root of (m/2) p^2 - (Q + m/2) p + t = 0
+    int64_t p = (int64_t)((qq - sqrt(fmax(qq * qq - 2.0 * m * (double)t, 0.0))) / m);
+    p = p < 0 ? 0 : (p >= P ? P - 1 : p);
+    while (p > 0 && tiles_before(p, Q) > t) --p;
+    while (p + 1 < P && tiles_before(p + 1, Q) <= t) ++p;
+    const int64_t q = kColBlocksPerRowBlock * p + (t - tiles_before(p, Q));
+    const int64_t i = p * kRowsPerBlock + lane;
+    const int64_t j0 = q * kColsPerBlock;
     const int64_t plane = n * nq;
 
-    for (int t = threadIdx.x; t < kColsPerBlock * nq; t += blockDim.x) {
-        const int c = t / nq, b = t - c * nq;
+    for (int u = threadIdx.x; u < kColsPerBlock * nq; u += blockDim.x) {
+        const int c = u / nq, b = u - c * nq;
         const int64_t j = j0 + c;
         if (j < n) {
             const int64_t o = (int64_t)b * n + j;
@@ -124,33 +306,33 @@ __global__ __launch_bounds__(256) void assemble_regular_kernel(
 
     const double k4 = 0.25 * k, eta4 = 0.25 * eta, kinv = 1.0 / k;
     const int64_t idx_i = index[i], nx_i = next[i], pv_i = prev[i];
+    const double cx_i = bounds[BOUND_CX * n + i], cy_i = bounds[BOUND_CY * n + i];
+    const double r_i = bounds[BOUND_R * n + i];
     for (int cc = 0; cc < kColsPerWave; ++cc) {
         const int c = wave * kColsPerWave + cc;
         const int64_t j = j0 + c;
         if (j >= n) break;
-        const int64_t idx_j = index[j];
-        if (idx_j == idx_i || nx_i == idx_j || pv_i == idx_j) continue;
-        const double* sc = sh + c * nq * 5;
-        double ar = 0.0, ai = 0.0;
-        for (int a = 0; a < nq; ++a) {
-            const int64_t o = (int64_t)a * n + i;
-            const double xx = nodes[NODE_X * plane + o], xy = nodes[NODE_Y * plane + o];
-            const double n1x = nodes[NODE_NX * plane + o], n1y = nodes[NODE_NY * plane + o];
-            const double swa = nodes[NODE_SW * plane + o];
-            double br = 0.0, bi = 0.0;
-#pragma unroll kUnroll
-            for (int b = 0; b < nq; ++b) {
-                const double* y = sc + b * 5;
-                double kr, ki;
-                kernel_eval<KIND>(k, kinv, k4, eta4, xx, xy, y[0], y[1], n1x, n1y, y[2], y[3], kr,
-                                  ki);
-                br = fma(kr, y[4], br);
-                bi = fma(ki, y[4], bi);
-            }
-            ar = fma(br, swa, ar);
-            ai = fma(bi, swa, ai);
+        const int64_t idx_j = index[j], nx_j = next[j], pv_j = prev[j];
+        const bool direct = j > i && !(idx_j == idx_i || nx_i == idx_j || pv_i == idx_j);
+        const bool mirror = j > i && !(idx_i == idx_j || nx_j == idx_i || pv_j == idx_i);
+        const bool any = direct || mirror;
+        bool far = true;
+        if (any) {
+            const double dx = cx_i - bounds[BOUND_CX * n + j];
+            const double dy = cy_i - bounds[BOUND_CY * n + j];
+            const double gap = sqrt(dx * dx + dy * dy) - r_i - bounds[BOUND_R * n + j];
+            far = k * gap >= kHankelFarX * (1.0 + 1e-12);
         }
-        A[i + lda * j] = make_double2(kscale * ar, kscale * ai);
+        const bool all_far = __all(far);
+        if (!any) continue;
+        const double* sc = sh + c * nq * 5;
+        double ar, ai, mr, mi;
+        if (all_far)
+            pair_sums<KIND, NQ, true>(nq, n, i, nodes, sc, k, kinv, k4, eta4, ar, ai, mr, mi);
+        else
+            pair_sums<KIND, NQ, false>(nq, n, i, nodes, sc, k, kinv, k4, eta4, ar, ai, mr, mi);
+        if (direct) A[i + lda * j] = make_double2(kscale * ar, kscale * ai);
+        if (mirror) A[j + lda * i] = make_double2(kscale * mr, kscale * mi);
     }
 }
 

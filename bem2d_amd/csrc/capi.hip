@@ -442,7 +442,7 @@ extern "C" int bem2d_mesh_create(bem2d_ctx* c, const bem2d_geometry* g, bem2d_qu
         bem2d_mesh_destroy(m);
         return FAIL(code, "bem2d_mesh_create: %s", what);
     };
-    size_t bytes_nodes = sizeof(double) * NODE_PLANES * (size_t)n * q.N;
+    size_t bytes_nodes = sizeof(double) * (NODE_PLANES * (size_t)n * q.N + BOUND_PLANES * (size_t)n);
     if (hipMalloc(&m->d_type, sizeof(int32_t) * n) != hipSuccess ||
         hipMalloc(&m->d_param, sizeof(double) * 8 * n) != hipSuccess ||
         hipMalloc(&m->d_adj, sizeof(int64_t) * 3 * n) != hipSuccess ||
@@ -473,6 +473,8 @@ extern "C" int bem2d_mesh_create(bem2d_ctx* c, const bem2d_geometry* g, bem2d_qu
     const int64_t total = n * q.N;
     prep_nodes_kernel<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(
         n, q.N, m->d_type, m->d_param, m->d_g1d, m->d_g1d + q.N, m->d_nodes);
+    prep_bounds_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, q.N, m->d_nodes,
+                                                                   m->bounds());
     if (hipGetLastError() != hipSuccess) return cleanup(BEM2D_ERR_DEVICE, "prep kernel launch");
     // The host vectors above go out of scope on return: wait for the uploads.
     if (hipStreamSynchronize(s) != hipSuccess) return cleanup(BEM2D_ERR_DEVICE, "prep kernel");
@@ -511,16 +513,18 @@ static int launch_assembly(bem2d_ctx* c, const bem2d_mesh* m, double k, double e
     const int64_t* idx = m->d_adj;
     const int64_t* nxt = m->d_adj + n;
     const int64_t* prv = m->d_adj + 2 * n;
-    dim3 grid((unsigned)((n + kRowsPerBlock - 1) / kRowsPerBlock),
-              (unsigned)((n + kColsPerBlock - 1) / kColsPerBlock));
+    const int64_t tiles = upper_tile_count(n);
+    if (tiles > (int64_t)INT32_MAX)
+        return FAIL(BEM2D_ERR_UNSUPPORTED, "bem2d_assemble: n = %lld too large", (long long)n);
+    dim3 grid((unsigned)tiles);
     size_t shm = sizeof(double) * kColsPerBlock * nq * NODE_PLANES;
     bem2d_internal_prof_begin(c, BEM2D_PROF_ASSEMBLE_REGULAR);
     if (nq == 16)
-        assemble_regular_kernel<KIND, 16><<<grid, 256, shm, s>>>(n, nq, m->d_nodes, idx, nxt, prv,
-                                                                 k, eta, kscale, A, lda);
+        assemble_regular_kernel<KIND, 16><<<grid, 64 * kWavesPerBlock, shm, s>>>(
+            n, nq, m->d_nodes, m->bounds(), idx, nxt, prv, k, eta, kscale, A, lda);
     else
-        assemble_regular_kernel<KIND, 0><<<grid, 256, shm, s>>>(n, nq, m->d_nodes, idx, nxt, prv,
-                                                                k, eta, kscale, A, lda);
+        assemble_regular_kernel<KIND, 0><<<grid, 64 * kWavesPerBlock, shm, s>>>(
+            n, nq, m->d_nodes, m->bounds(), idx, nxt, prv, k, eta, kscale, A, lda);
     bem2d_internal_prof_end(c, BEM2D_PROF_ASSEMBLE_REGULAR);
     if (m->nspecial > 0) {
         bem2d_internal_prof_begin(c, BEM2D_PROF_ASSEMBLE_SPECIAL);

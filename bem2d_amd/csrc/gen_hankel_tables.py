@@ -221,11 +221,13 @@ def main():
         return [c[kk] for kk in range(deg + 1)]
     Sc = interp(lambda u: (mp.sin(mp.sqrt(u)) - mp.sqrt(u)) / (u * mp.sqrt(u)), 5)
     Cc = interp(lambda u: (mp.cos(mp.sqrt(u)) - 1) / u, 6)
-    pio2 = mp.pi / 2
-    P1 = float(pio2)
-    P2 = float(pio2 - mp.mpf(P1))
-    P3 = float(pio2 - mp.mpf(P1) - mp.mpf(P2))
-    trig = [float(2 / mp.pi), P1, P2, P3] + [float(c) for c in reversed(Sc)] + \
+    with mp.workdps(80):  # P3 is 2^-110 below pi/2: the split needs more than 40 digits
+        pio2 = mp.pi / 2
+        P1 = float(pio2)
+        P2 = float(pio2 - mp.mpf(P1))
+        P3 = float(pio2 - mp.mpf(P1) - mp.mpf(P2))
+        twoopi = float(2 / mp.pi)
+    trig = [twoopi, P1, P2, P3] + [float(c) for c in reversed(Sc)] + \
         [float(c) for c in reversed(Cc)]
     emit("// [2/pi, P1, P2, P3, S5..S0, C6..C0]: sin r = r + r^3 S(r^2), cos r = 1 + r^2 C(r^2)")
     emit("#define BEM2D_TRIG_NS 6")

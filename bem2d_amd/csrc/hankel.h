@@ -9,8 +9,8 @@
 //            [40, inf) (degree 4, the asymptotic regime where most far-field pairs fall).
 // On the device every coefficient lives in one __constant__ table and every index is a
 // compile-time constant, so the coefficients reach v_fma_f64 as SGPR operands through scalar loads
-// (instruction literals would each cost two v_mov_b32 per use).  sin/cos of the phase use
-// sincos_reduced below.  Accuracy: about 3e-16 relative to |H| (pinned against mpmath by the tests).
+// (instruction literals would each cost two v_mov_b32 per use).  The last piece is evaluated in
+// w = 1/x^2 directly (hankel_pq_far).  sin/cos of the phase use sincos_minus_quarter below.  Accuracy: about 3e-16 relative to |H| (pinned against mpmath by the tests).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -22,29 +22,24 @@ namespace bem2d_dev {
 constexpr double kPi = 3.14159265358979323846;
 constexpr double kTwoOverPi = 0.63661977236758134308;
 constexpr double kInvPi = 0.31830988618379067154;
+constexpr double kHankelFarX = BEM2D_PIECE_X[BEM2D_HANKEL_NPIECE - 1]; // start of the last piece
 
 struct HankelTables {
     double ser[4][BEM2D_HANKEL_NSER];                          // J0, J1, Y0, Y1 series in t
     double zmap[BEM2D_HANKEL_NPIECE][3];                       // a^2, ym, 1/yh per piece
     double fit[BEM2D_HANKEL_NPIECE][4][BEM2D_HANKEL_DEGMAX + 1]; // P0, xQ0, P1, xQ1 per piece
-    // sincos_reduced: 2/pi, pi/2 = P1 + P2 + P3, then the Taylor coefficients of sin (r^17 ..
-    // r^3) and cos (r^18 .. r^2), highest power first
-    double trig[4 + 8 + 9];
+    double far[4][BEM2D_HANKEL_FAR_DEG + 1];                   // last piece, in w = 1/x^2
+    // 2/pi, pi/2 = P1 + P2 + P3, then S (degree 5) and C (degree 6) of
+    // sin r = r + r^3 S(r^2), cos r = 1 + r^2 C(r^2), highest power first
+    double trig[4 + BEM2D_TRIG_NS + BEM2D_TRIG_NC];
 };
 
-#define BEM2D_TRIG_INIT                                                                            \
-    {0.6366197723675814, 1.5707963267948966, 6.123233995736766e-17, -1.4973849048591698e-33,     \
-     2.8114572543455206e-15, -7.6471637318198164e-13, 1.6059043836821613e-10,                    \
-     -2.5052108385441720e-08, 2.7557319223985893e-06, -1.9841269841269841e-04,                   \
-     8.3333333333333333e-03, -1.6666666666666667e-01, -1.5619206968586225e-16,                   \
-     4.7794773323873853e-14, -1.1470745597729725e-11, 2.0876756987868099e-09,                    \
-     -2.7557319223985888e-07, 2.4801587301587302e-05, -1.3888888888888889e-03,                   \
-     4.1666666666666667e-02, -0.5}
-
 static constexpr HankelTables kHankelHost = {BEM2D_HANKEL_SER_INIT, BEM2D_HANKEL_ZMAP_INIT,
-                                             BEM2D_HANKEL_FIT_INIT, BEM2D_TRIG_INIT};
+                                             BEM2D_HANKEL_FIT_INIT, BEM2D_HANKEL_FAR_INIT,
+                                             BEM2D_TRIG_INIT};
 static __constant__ HankelTables kHankelDev = {BEM2D_HANKEL_SER_INIT, BEM2D_HANKEL_ZMAP_INIT,
-                                               BEM2D_HANKEL_FIT_INIT, BEM2D_TRIG_INIT};
+                                               BEM2D_HANKEL_FIT_INIT, BEM2D_HANKEL_FAR_INIT,
+                                               BEM2D_TRIG_INIT};
 
 #ifdef __HIP_DEVICE_COMPILE__
 // Device view of the table: a constant-address-space (4) pointer, so that uniform reads become
@@ -67,17 +62,29 @@ __device__ __forceinline__ double fma_vvs(double a, double b, double c)
     asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "s"(c));
     return r;
 }
+
+// a + c with c read from an SGPR pair, kept apart from the product that feeds it (a contracted
+// fma would need two scalar operands, one more than a VALU instruction reads).
+__device__ __forceinline__ double add_vs(double a, double c)
+{
+    double r;
+    asm("v_add_f64 %0, %1, %2" : "=v"(r) : "v"(a), "s"(c));
+    return r;
+}
 #else
 __host__ __forceinline__ const HankelTables& hankel_tables() { return kHankelHost; }
 __host__ __forceinline__ double fma_vvs(double a, double b, double c) { return fma(a, b, c); }
+__host__ __forceinline__ double add_vs(double a, double c) { return a + c; }
 #endif
 
+// Horner's rule from the top, its first step as a multiply and an add so that no coefficient has
+// to be copied into vector registers.
 template <int N, typename Ptr>
 __host__ __device__ __forceinline__ double horner(Ptr c, double z)
 {
-    double r = c[N - 1];
+    double r = add_vs(z * c[N - 1], c[N - 2]);
 #pragma unroll
-    for (int i = N - 2; i >= 0; --i) r = fma_vvs(r, z, c[i]);
+    for (int i = N - 3; i >= 0; --i) r = fma_vvs(r, z, c[i]);
     return r;
 }
 
@@ -111,9 +118,30 @@ __host__ __device__ __forceinline__ double rsqrt_refined(double a)
 #endif
 }
 
-// sin and cos of x for |x| < 2^30.  Cody-Waite reduction by pi/2 split into three doubles
-// (pi/2 = P1 + P2 + P3 to 160 bits), then the Taylor series of sin and cos on |r| <= pi/4,
-// truncated at r^17 and r^18 (remainder below 1e-19).  Both results within about 1 ulp.
+// sin r and cos r on |r| <= pi/4 from the near-minimax polynomials of the table (interpolated at
+// Chebyshev nodes in r^2: 1.4e-17 relative on (sin r - r)/r^3, 1.2e-19 absolute on (cos r - 1)/r^2),
+// then the quadrant q of the reduction applied.
+__host__ __device__ __forceinline__ void sincos_kernel(double r, int q, double& s, double& c)
+{
+    const auto& T = hankel_tables().trig;
+    constexpr int S0 = 4, C0 = 4 + BEM2D_TRIG_NS;
+    const double r2 = r * r;
+    double ps = add_vs(r2 * T[S0], T[S0 + 1]);
+#pragma unroll
+    for (int i = S0 + 2; i < C0; ++i) ps = fma_vvs(ps, r2, T[i]);
+    const double sr = fma(ps * r2, r, r);
+    double pc = add_vs(r2 * T[C0], T[C0 + 1]);
+#pragma unroll
+    for (int i = C0 + 2; i < C0 + BEM2D_TRIG_NC; ++i) pc = fma_vvs(pc, r2, T[i]);
+    const double cr = fma(pc, r2, 1.0);
+    const double a = (q & 1) ? cr : sr; // sin before the sign
+    const double b = (q & 1) ? sr : cr; // cos before the sign
+    s = (q & 2) ? -a : a;
+    c = ((q + 1) & 2) ? -b : b;
+}
+
+// sin and cos of x for |x| < 2^30: Cody-Waite reduction by pi/2 split into three doubles
+// (pi/2 = P1 + P2 + P3 to 160 bits), then sincos_kernel.  Both results within about 1 ulp.
 __host__ __device__ __forceinline__ void sincos_reduced(double x, double& s, double& c)
 {
     const auto& T = hankel_tables().trig;
@@ -121,20 +149,7 @@ __host__ __device__ __forceinline__ void sincos_reduced(double x, double& s, dou
     double r = fma(-n, T[1], x);
     r = fma(-n, T[2], r);
     r = fma(-n, T[3], r);
-    const double r2 = r * r;
-    double ps = T[4];
-#pragma unroll
-    for (int i = 5; i < 12; ++i) ps = fma_vvs(ps, r2, T[i]);
-    const double sr = fma(ps * r2, r, r);
-    double pc = T[12];
-#pragma unroll
-    for (int i = 13; i < 21; ++i) pc = fma_vvs(pc, r2, T[i]);
-    const double cr = fma(pc, r2, 1.0);
-    const int q = ((int)(long long)n) & 3;
-    const double a = (q & 1) ? cr : sr; // sin before the sign
-    const double b = (q & 1) ? sr : cr; // cos before the sign
-    s = (q & 2) ? -a : a;
-    c = ((q + 1) & 2) ? -b : b;
+    sincos_kernel(r, ((int)n) & 3, s, c);
 }
 
 // Series regime, x < 2.
@@ -178,20 +193,21 @@ __host__ __device__ __forceinline__ void sincos_minus_quarter(double x, double& 
     double r = fma(-m, T[1], x);
     r = fma(-m, T[2], r);
     r = fma(-m, T[3], r);
-    const double r2 = r * r;
-    double ps = T[4];
-#pragma unroll
-    for (int i = 5; i < 12; ++i) ps = fma_vvs(ps, r2, T[i]);
-    const double sr = fma(ps * r2, r, r);
-    double pc = T[12];
-#pragma unroll
-    for (int i = 13; i < 21; ++i) pc = fma_vvs(pc, r2, T[i]);
-    const double cr = fma(pc, r2, 1.0);
-    const int q = ((int)(long long)n) & 3;
-    const double a = (q & 1) ? cr : sr;
-    const double b = (q & 1) ? sr : cr;
-    s = (q & 2) ? -a : a;
-    c = ((q + 1) & 2) ? -b : b;
+    sincos_kernel(r, ((int)n) & 3, s, c);
+}
+
+// P_nu, Q_nu for x >= BEM2D_PIECE_X[last] as polynomials in w = 1/x^2 (the last piece's map to z
+// folded into the coefficients).
+__host__ __device__ __forceinline__ void hankel_pq_far(double invx, double& P0, double& Q0,
+                                                       double& P1, double& Q1)
+{
+    const auto& T = hankel_tables();
+    constexpr int N = BEM2D_HANKEL_FAR_DEG + 1;
+    const double w = invx * invx;
+    P0 = horner<N>(T.far[0], w);
+    Q0 = horner<N>(T.far[1], w) * invx;
+    P1 = horner<N>(T.far[2], w);
+    Q1 = horner<N>(T.far[3], w) * invx;
 }
 
 // Modulus/phase combination for x >= 2:
@@ -224,8 +240,8 @@ __host__ __device__ __forceinline__ void hankel01_inv(double x, double invx, dou
         return;
     }
     double P0, Q0, P1, Q1;
-    if (x >= 40.0)
-        hankel_pq<6>(invx, P0, Q0, P1, Q1);
+    if (x >= kHankelFarX)
+        hankel_pq_far(invx, P0, Q0, P1, Q1);
     else if (x >= 20.0)
         hankel_pq<5>(invx, P0, Q0, P1, Q1);
     else if (x >= 8.0)

@@ -59,10 +59,12 @@ struct bem2d_mesh {
     int32_t* d_type = nullptr;
     double* d_param = nullptr;
     int64_t* d_adj = nullptr;            // index | next | prev, n each
-    double* d_nodes = nullptr;           // 5 planes of nq*n (x, y, nx, ny, |Gamma'| w)
+    double* d_nodes = nullptr;           // 5 planes of nq*n (x, y, nx, ny, |Gamma'| w), then
+                                         // 3 planes of n (bounding circle cx, cy, r)
     double* d_rules = nullptr;           // AG1 x|y|w, AG2 x|y|w, AG3 x|y|w
     double* d_g1d = nullptr;             // Gauss1D x | w
     int64_t* d_special = nullptr;        // (i, j, rule) triples
+    double* bounds() const { return d_nodes + 5 * (size_t)n * nq; }
 };
 
 int bem2d_internal_fail(int code, const char* fmt, ...);
