@@ -18,6 +18,7 @@
 #include <rocblas/rocblas.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -98,254 +99,306 @@ __global__ void rowsum_max_kernel(int64_t n, int nchunk, const double* __restric
     if (threadIdx.x == 0) *out = red[0];
 }
 
-// ---- panel: diagonal block LU + triangular inverses ---------------------------------------------
-// One workgroup of 256 threads.  Block rows/cols k0 .. k0+kb.  Linv/Uinv receive the 64x64 inverses
-// (identity-padded when kb < 64), column-major with leading dimension 64.
-// Thread t owns the 4x4 sub-block rows 4 (t % 16) .. +3, cols 4 (t / 16) .. +3 of the tile, in
-// registers.  Each elimination step publishes the pivot row and column through ping-pong LDS
-// buffers, so a step costs one barrier; the register updates are branch-free (selects instead of
-// per-element conditions, which would each become an exec-mask branch).
+// ---- diagonal block: LU without interchanges + inverses of both triangles -----------------------
+// 256 threads.  Thread t = 64 w + l holds T[a][b] = tile(R(a), C(b)) with R(a) = 16 a + (l & 15) and
+// C(b) = 16 b + 4 w + (l >> 4): each wavefront holds a 16 x 4 piece of each of the 4 x 4 sub-blocks
+// of 16, so at step j the sub-blocks above or left of row/column j drop out on every SIMD alike and
+// the work per step shrinks with the active region.
+//
+// Step j eliminates column j (pivot row j = U[j][:], multipliers L[r][j] = A[r][j] / U[j][j]) and,
+// in the same step, advances two forward sweeps that need only row j of U and column j of L:
+//   X = L^{-1}:           X = I;  rows r > j:  X[r][:] -= L[r][j] X[j][:];
+//   Y = (U^T)^{-1}:       Y = I;  Y[j][:] /= U[j][j];  rows r > j:  Y[r][:] -= U[j][r] Y[j][:];
+// so U^{-1} = Y^T comes out of the same 64 steps.  Row and column j+1 of the updated matrix and rows
+// j+1 of X and Y (Y unscaled) are published through ping-pong LDS buffers: one barrier per step.
+// Linv receives X and UinvT receives Y = (U^{-1})^T (column-major, ld 64; identity-padded when
+// kb < 64).
 __device__ __forceinline__ float2 fast_crecip(float2 a)
 {
     const float d = __builtin_amdgcn_rcpf(a.x * a.x + a.y * a.y);
     return make_float2(a.x * d, -a.y * d);
 }
 
-// Thread t = 64 w + l owns rows 16 w + 4 (l % 4) .. +3 and cols 4 (l / 4) .. +3: each wavefront
-// holds 16 contiguous rows, so at every step the wavefronts whose rows are all inactive skip the
-// update with a uniform branch (about half the work of the triangular sweeps).  The rows of U^{-1}
-// are scaled by 1/U[r][r] once at the end instead of at every step.
-__global__ __launch_bounds__(256) void diag_lu_kernel(float2* __restrict__ A, int64_t ld,
-                                                      int64_t k0, int kb,
-                                                      float2* __restrict__ Linv,
-                                                      float2* __restrict__ Uinv,
-                                                      int* __restrict__ bad)
+struct DiagLds {
+    float2 row[2][NB], col[2][NB];   // pivot row / column of the current matrix
+    float2 xrow[2][NB], yrow[2][NB]; // row j of X and of Y (before its scaling by 1/U[j][j])
+};
+
+__device__ __forceinline__ int diag_R(int a, int l) { return 16 * a + (l & 15); }
+__device__ __forceinline__ int diag_C(int b, int w, int l) { return 16 * b + 4 * w + (l >> 4); }
+
+template <int V> using ic = std::integral_constant<int, V>;
+
+// Lanes holding row j: (l & 15) == (j & 15), sub-block row j >> 4.  Lanes holding column j:
+// wavefront (j >> 2) & 3, (l >> 4) == (j & 3), sub-block column j >> 4.
+__device__ __forceinline__ bool holds_row(int j, int l) { return (l & 15) == (j & 15); }
+__device__ __forceinline__ bool holds_col(int j, int w, int l)
 {
-    __shared__ float2 rowbuf[2][NB], colbuf[2][NB];   // factorisation: pivot row / column
-    __shared__ float2 xrow[2][NB], lcol[2][NB];       // L^{-1} sweep: row j of X, column j of L
-    __shared__ float2 zrow[2][NB], ucol[2][NB];       // U^{-1} sweep: row j of Z~, column j of U
-    __shared__ float2 udiag_inv[NB];
+    return w == ((j >> 2) & 3) && (l >> 4) == (j & 3);
+}
+
+// The 64 steps run as four groups of 16, one per sub-block index js = j >> 4: within a group every
+// sub-block index is a compile-time constant, so the register tiles are never indexed at run time
+// (that would place them in scratch memory) and the inactive sub-blocks are dropped at compile time.
+__device__ __forceinline__ void diag_factor(float2* __restrict__ A, int64_t ld, int64_t k0, int kb,
+                                            float2* __restrict__ Linv,
+                                            float2* __restrict__ UinvT, int* __restrict__ bad,
+                                            DiagLds& S)
+{
     const int t = threadIdx.x;
     const int w = __builtin_amdgcn_readfirstlane(t >> 6);
     const int l = t & 63;
-    const int r0 = 16 * w + 4 * (l & 3), c0 = 4 * (l >> 2);
-    const int wr0 = 16 * w, wr1 = 16 * w + 15; // rows of this wavefront
     const float2 zero = make_float2(0.f, 0.f);
-    float2 T[4][4];
+    float2 T[4][4], X[4][4], Y[4][4];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
-            const int r = r0 + a, c = c0 + b;
-            float2 v = make_float2(r == c ? 1.f : 0.f, 0.f);
-            if (r < kb && c < kb) v = A[(k0 + r) + ld * (k0 + c)];
-            T[a][b] = v;
+            const int r = diag_R(a, l), c = diag_C(b, w, l);
+            const float2 e = make_float2(r == c ? 1.f : 0.f, 0.f);
+            T[a][b] = (r < kb && c < kb) ? A[(k0 + r) + ld * (k0 + c)] : e;
+            X[a][b] = e;
+            Y[a][b] = e;
         }
-    // ---- LU without interchanges: T = L U in place.  Row j / column j of the current matrix are
-    // published into ping-pong buffers (stale entries are masked by the readers); only compile-time
-    // indices into the register tiles (a runtime index would put them in scratch). ----
-    auto publish_lu = [&](int j) {
-        const int buf = j & 1;
+    // row / column j (sub-block index JS = j >> 4) into buffer buf
+    auto publish = [&](auto JS_, int j, int buf) __attribute__((always_inline)) {
+        constexpr int js = decltype(JS_)::value;
+        if (holds_row(j, l)) {
 #pragma unroll
-        for (int a = 0; a < 4; ++a)
-            if (r0 + a == j) {
-#pragma unroll
-                for (int b = 0; b < 4; ++b) rowbuf[buf][c0 + b] = T[a][b];
+            for (int b = 0; b < 4; ++b) {
+                const int c = diag_C(b, w, l);
+                S.row[buf][c] = T[js][b];
+                if (b <= js) {
+                    S.xrow[buf][c] = X[js][b];
+                    S.yrow[buf][c] = Y[js][b];
+                }
             }
+        }
+        if (holds_col(j, w, l)) {
 #pragma unroll
-        for (int b = 0; b < 4; ++b)
-            if (c0 + b == j) {
-#pragma unroll
-                for (int a = 0; a < 4; ++a) colbuf[buf][r0 + a] = T[a][b];
-            }
+            for (int a = 0; a < 4; ++a) S.col[buf][diag_R(a, l)] = T[a][js];
+        }
     };
-    publish_lu(0);
-    for (int j = 0; j < NB; ++j) {
-        __syncthreads();
-        const int buf = j & 1;
-        if (wr1 > j) {
-            const float2 p = rowbuf[buf][j];
+    auto group = [&](auto JS_) __attribute__((always_inline)) {
+        constexpr int js = decltype(JS_)::value;
+#pragma unroll 1
+        for (int jj = 0; jj < 16; ++jj) {
+            const int j = 16 * js + jj;
+            __syncthreads();
+            const int buf = j & 1;
+            const float2 p = S.row[buf][j];
             if (t == 0 && j < kb &&
                 !(p.x * p.x + p.y * p.y > 0.f && isfinite(p.x) && isfinite(p.y)))
                 *bad = 1;
             const float2 ip = fast_crecip(p);
-            float2 m[4], rb[4];
+            // m[a] = L[R(a)][j], g[a] = U[j][R(a)] for rows below j; u[b] = U[j][C(b)] right of j
+            float2 m[4], g[4], u[4], xr[4], yr[4];
 #pragma unroll
-            for (int a = 0; a < 4; ++a) {
-                const float2 cv = colbuf[buf][r0 + a];
-                m[a] = (r0 + a > j) ? cmul(cv, ip) : zero;
+            for (int a = js; a < 4; ++a) {
+                const int r = diag_R(a, l);
+                const bool below = a > js || r > j;
+                const float2 cv = S.col[buf][r];
+                const float2 rv = S.row[buf][r];
+                m[a] = below ? cmul(cv, ip) : zero;
+                g[a] = below ? rv : zero;
             }
 #pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const float2 rv = rowbuf[buf][c0 + b];
-                rb[b] = (c0 + b > j) ? rv : zero;
+            for (int b = js; b < 4; ++b) {
+                const int c = diag_C(b, w, l);
+                const float2 rv = S.row[buf][c];
+                u[b] = (b > js || c > j) ? rv : zero;
             }
 #pragma unroll
-            for (int a = 0; a < 4; ++a)
+            for (int b = 0; b <= js; ++b) {
+                const int c = diag_C(b, w, l);
+                xr[b] = S.xrow[buf][c];
+                yr[b] = cmul(S.yrow[buf][c], ip);
+            }
+            if (holds_row(j, l)) {
 #pragma unroll
-                for (int b = 0; b < 4; ++b) {
-                    const float2 u = cfms(T[a][b], m[a], rb[b]);
-                    T[a][b] = (c0 + b == j && r0 + a > j) ? m[a] : u;
+                for (int b = 0; b <= js; ++b) Y[js][b] = yr[b];
+            }
+#pragma unroll
+            for (int a = js; a < 4; ++a) {
+#pragma unroll
+                for (int b = js; b < 4; ++b) T[a][b] = cfms(T[a][b], m[a], u[b]);
+#pragma unroll
+                for (int b = 0; b <= js; ++b) {
+                    X[a][b] = cfms(X[a][b], m[a], xr[b]);
+                    Y[a][b] = cfms(Y[a][b], g[a], yr[b]);
                 }
-        }
-        if (j + 1 < NB) publish_lu(j + 1);
-    }
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const int r = r0 + a, c = c0 + b;
-            if (r < kb && c < kb) A[(k0 + r) + ld * (k0 + c)] = T[a][b];
-            if (r == c) udiag_inv[r] = fast_crecip(T[a][b]);
-        }
-    // ---- inverses by row operations, both sweeps in the same 64 steps ----
-    // X = L^{-1}: X = I; for j ascending, rows r > j: X[r][:] -= L[r][j] X[j][:].
-    // Z~: Z~ = I; for j descending, rows r < j: Z~[r][:] -= (U[r][j] / U[j][j]) Z~[j][:];
-    // then U^{-1}[r][:] = Z~[r][:] / U[r][r].
-    float2 X[4][4], Z[4][4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            X[a][b] = make_float2(r0 + a == c0 + b ? 1.f : 0.f, 0.f);
-            Z[a][b] = X[a][b];
-        }
-    auto publish_inv = [&](int s) {
-        const int buf = s & 1;
-        const int jl = s, ju = NB - 1 - s;
-#pragma unroll
-        for (int a = 0; a < 4; ++a) {
-            if (r0 + a == jl) {
-#pragma unroll
-                for (int b = 0; b < 4; ++b) xrow[buf][c0 + b] = X[a][b];
             }
-            if (r0 + a == ju) {
+            // column j takes the multipliers (u is zero there, so the update left it as it was)
+            if (holds_col(j, w, l)) {
 #pragma unroll
-                for (int b = 0; b < 4; ++b) zrow[buf][c0 + b] = Z[a][b];
+                for (int a = js; a < 4; ++a)
+                    if (a > js || diag_R(a, l) > j) T[a][js] = m[a];
             }
-        }
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            if (c0 + b == jl) {
-#pragma unroll
-                for (int a = 0; a < 4; ++a) lcol[buf][r0 + a] = T[a][b];
-            }
-            if (c0 + b == ju) {
-#pragma unroll
-                for (int a = 0; a < 4; ++a) ucol[buf][r0 + a] = T[a][b];
-            }
+            if (jj < 15)
+                publish(ic<js>(), j + 1, buf ^ 1);
+            else if (js < 3)
+                publish(ic<(js < 3 ? js + 1 : 3)>(), j + 1, buf ^ 1);
         }
     };
-    publish_inv(0); // (the barrier at the top of the loop also orders the udiag_inv writes)
-    for (int s2 = 0; s2 < NB; ++s2) {
-        __syncthreads();
-        const int buf = s2 & 1;
-        const int jl = s2, ju = NB - 1 - s2;
-        if (wr1 > jl) {
-            float2 lx[4], xr[4];
+    publish(ic<0>(), 0, 0);
+    group(ic<0>());
+    group(ic<1>());
+    group(ic<2>());
+    group(ic<3>());
 #pragma unroll
-            for (int a = 0; a < 4; ++a) {
-                const float2 lv = lcol[buf][r0 + a];
-                lx[a] = r0 + a > jl ? lv : zero;
-            }
-#pragma unroll
-            for (int b = 0; b < 4; ++b) xr[b] = xrow[buf][c0 + b];
-#pragma unroll
-            for (int a = 0; a < 4; ++a)
-#pragma unroll
-                for (int b = 0; b < 4; ++b) X[a][b] = cfms(X[a][b], lx[a], xr[b]);
-        }
-        if (wr0 < ju) {
-            const float2 iu = udiag_inv[ju];
-            float2 uz[4], zr[4];
-#pragma unroll
-            for (int a = 0; a < 4; ++a) {
-                const float2 uv = ucol[buf][r0 + a];
-                uz[a] = r0 + a < ju ? cmul(uv, iu) : zero;
-            }
-#pragma unroll
-            for (int b = 0; b < 4; ++b) zr[b] = zrow[buf][c0 + b];
-#pragma unroll
-            for (int a = 0; a < 4; ++a)
-#pragma unroll
-                for (int b = 0; b < 4; ++b) Z[a][b] = cfms(Z[a][b], uz[a], zr[b]);
-        }
-        if (s2 + 1 < NB) publish_inv(s2 + 1);
-    }
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-        const float2 iu = udiag_inv[r0 + a];
+    for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
-            Linv[(r0 + a) + NB * (c0 + b)] = X[a][b];
-            Uinv[(r0 + a) + NB * (c0 + b)] = cmul(Z[a][b], iu);
+            const int r = diag_R(a, l), c = diag_C(b, w, l);
+            if (r < kb && c < kb) A[(k0 + r) + ld * (k0 + c)] = T[a][b];
+            Linv[r + NB * c] = X[a][b];
+            UinvT[r + NB * c] = Y[a][b];
         }
+}
+
+__global__ __launch_bounds__(256) void diag_lu_kernel(float2* __restrict__ A, int64_t ld,
+                                                      int64_t k0, int kb,
+                                                      float2* __restrict__ Linv,
+                                                      float2* __restrict__ UinvT,
+                                                      int* __restrict__ bad)
+{
+    __shared__ DiagLds S;
+    diag_factor(A, ld, k0, kb, Linv, UinvT, bad, S);
+}
+
+// ---- 64 x 64 x 64 complex products on MFMA -----------------------------------------------------
+// acc += Aop Bop with Aop in LDS as [row][k] and Bop as [col][k] (pitch KP, real and imaginary
+// planes apart).  Wavefront w computes the 32 x 32 quarter at rows 32 (w & 1), cols 32 (w >> 1)
+// as 2 x 2 tiles of 16 x 16; each complex product is four real v_mfma_f32_16x16x4f32.
+// v_mfma_f32_16x16x4f32 lane layout: A operand lane l holds A[l % 16][l / 16], B operand
+// B[l / 16][l % 16]; accumulator register r of lane l is C[4 (l / 16) + r][l % 16].
+constexpr int KP = NB + 4; // padded k extent of the LDS operand tiles
+constexpr int kTileLdsFloats = 4 * NB * KP;
+
+struct TileAcc {
+    v4f re[2][2], im[2][2];
+};
+
+__device__ __forceinline__ void cgemm_64(const float* __restrict__ Are,
+                                         const float* __restrict__ Aim,
+                                         const float* __restrict__ Bre,
+                                         const float* __restrict__ Bim, TileAcc& c)
+{
+    const int t = threadIdx.x;
+    const int wave = t >> 6, lane = t & 63;
+    const int wr = (wave & 1) * 32, wc = (wave >> 1) * 32;
+    const int li = lane % 16, lk = lane / 16;
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            c.re[p][q] = v4f{0.f, 0.f, 0.f, 0.f};
+            c.im[p][q] = v4f{0.f, 0.f, 0.f, 0.f};
+        }
+    for (int ks = 0; ks < NB; ks += 4) {
+        float ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            ar[p] = Are[(wr + 16 * p + li) * KP + ks + lk];
+            ai[p] = Aim[(wr + 16 * p + li) * KP + ks + lk];
+            br[p] = Bre[(wc + 16 * p + li) * KP + ks + lk];
+            bi[p] = Bim[(wc + 16 * p + li) * KP + ks + lk];
+        }
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                c.re[p][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[p], br[q], c.re[p][q], 0, 0, 0);
+                c.re[p][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(-ai[p], bi[q], c.re[p][q], 0, 0, 0);
+                c.im[p][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[p], bi[q], c.im[p][q], 0, 0, 0);
+                c.im[p][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[p], br[q], c.im[p][q], 0, 0, 0);
+            }
     }
 }
 
-// ---- panel: L21 = A21 U11^{-1} and U12 = L11^{-1} A12 ------------------------------------------
+// Tile-local coordinates of accumulator element (p, q, r) of this thread.
+__device__ __forceinline__ int acc_row(int p, int r)
+{
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    return (wave & 1) * 32 + 16 * p + 4 * (lane / 16) + r;
+}
+__device__ __forceinline__ int acc_col(int q)
+{
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    return (wave >> 1) * 32 + 16 * q + lane % 16;
+}
+
+// ---- panel: L21 = A21 U11^{-1} and U12 = L11^{-1} A12 on MFMA ---------------------------------
 // blockIdx.x < nrow_tiles: a 64-row tile of the block column below the diagonal block;
-// otherwise a 64-column tile of the block row to its right.  In-place, through an LDS copy.
+// otherwise a 64-column tile of the block row to its right.  In place, through LDS.
 __global__ __launch_bounds__(256) void panel_trsm_kernel(float2* __restrict__ A, int64_t ld,
                                                          int64_t n, int64_t k0, int kb,
                                                          int nrow_tiles,
                                                          const float2* __restrict__ Linv,
-                                                         const float2* __restrict__ Uinv)
+                                                         const float2* __restrict__ UinvT)
 {
-    __shared__ float2 M[NB * LDT]; // the inverse
-    __shared__ float2 S[NB * LDT]; // the tile
+    __shared__ __align__(16) float lds[kTileLdsFloats];
+    float* Are = lds;
+    float* Aim = lds + NB * KP;
+    float* Bre = lds + 2 * NB * KP;
+    float* Bim = lds + 3 * NB * KP;
     const int t = threadIdx.x;
     const int64_t rest0 = k0 + kb;
     const bool rowtile = (int)blockIdx.x < nrow_tiles;
     const int64_t base = rest0 + (int64_t)(rowtile ? blockIdx.x : blockIdx.x - nrow_tiles) * NB;
-    const float2* inv = rowtile ? Uinv : Linv;
-    for (int e = t; e < NB * NB; e += 256) M[(e % NB) + LDT * (e / NB)] = inv[e];
-    const int64_t len = min((int64_t)NB, n - base);
+    const int len = (int)min((int64_t)NB, n - base);
     for (int e = t; e < NB * NB; e += 256) {
-        const int a = e % NB, b = e / NB; // tile coordinates: rowtile -> (row a, k b); else (k a, col b)
-        float2 v = make_float2(0.f, 0.f);
-        if (rowtile) {
-            if (a < len && b < kb) v = A[(base + a) + ld * (k0 + b)];
-        } else {
-            if (a < kb && b < len) v = A[(k0 + a) + ld * (base + b)];
+        const int x = e % NB, y = e / NB;
+        float2 a, b;
+        if (rowtile) { // out = S U^{-1}: Aop[x][y] = S[x][y], Bop[x][y] = U^{-1}[y][x] = UinvT[x][y]
+            a = (x < len && y < kb) ? A[(base + x) + ld * (k0 + y)] : make_float2(0.f, 0.f);
+            b = UinvT[x + NB * y];
+            Bre[x * KP + y] = b.x;
+            Bim[x * KP + y] = b.y;
+        } else {       // out = L^{-1} S: Aop[x][y] = Linv[x][y], Bop[y][x] = S[x][y]
+            a = Linv[x + NB * y];
+            b = (x < kb && y < len) ? A[(k0 + x) + ld * (base + y)] : make_float2(0.f, 0.f);
+            Bre[y * KP + x] = b.x;
+            Bim[y * KP + x] = b.y;
         }
-        S[a + LDT * b] = v;
+        Are[x * KP + y] = a.x;
+        Aim[x * KP + y] = a.y;
     }
     __syncthreads();
-    // 256 threads x 16 outputs: thread covers (a = t % 64, b = t / 64 + 4 q)
-    const int a = t % NB;
-    for (int q = 0; q < NB / 4; ++q) {
-        const int b = t / NB + 4 * q;
-        float2 s = make_float2(0.f, 0.f);
-        if (rowtile) { // out[a][b] = sum_m S[a][m] Uinv[m][b]
-            for (int m = 0; m < kb; ++m) {
-                const float2 u = S[a + LDT * m], v = M[m + LDT * b];
-                s.x += u.x * v.x - u.y * v.y;
-                s.y += u.x * v.y + u.y * v.x;
+    TileAcc c;
+    cgemm_64(Are, Aim, Bre, Bim, c);
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int x = acc_row(p, r), y = acc_col(q);
+                const float2 v = make_float2(c.re[p][q][r], c.im[p][q][r]);
+                if (rowtile) {
+                    if (x < len && y < kb) A[(base + x) + ld * (k0 + y)] = v;
+                } else {
+                    if (x < kb && y < len) A[(k0 + x) + ld * (base + y)] = v;
+                }
             }
-            if (a < len && b < kb) A[(base + a) + ld * (k0 + b)] = s;
-        } else { // out[a][b] = sum_m Linv[a][m] S[m][b]
-            for (int m = 0; m < kb; ++m) {
-                const float2 u = M[a + LDT * m], v = S[m + LDT * b];
-                s.x += u.x * v.x - u.y * v.y;
-                s.y += u.x * v.y + u.y * v.x;
-            }
-            if (a < kb && b < len) A[(k0 + a) + ld * (base + b)] = s;
-        }
-    }
 }
 
-// ---- trailing update on MFMA ----------------------------------------------------------------------
-// C[i][j] -= sum_k L[i][k] U[k][j] for i, j in [rest0, n), k in [k0, k0 + kb).
-// Workgroup = 64x64 complex output tile, 4 wavefronts of 32x32 (2x2 sub-tiles of 16x16).
-// v_mfma_f32_16x16x4f32 lane layout: A operand lane l holds A[l % 16][l / 16], B operand
-// B[l / 16][l % 16]; accumulator register r of lane l is C[4 (l / 16) + r][l % 16].
-constexpr int KP = NB + 4; // padded k extent of the LDS operand tiles
+// ---- trailing update on MFMA, fused with the next diagonal block ---------------------------------
+// C[i][j] -= sum_k L[i][k] U[k][j] for i, j in [rest0, n), k in [k0, k0 + kb); workgroup = one
+// 64 x 64 output tile.  Workgroup (0, 0) owns the next diagonal block: after its update it goes on
+// to factorise that block (when Linv_next is given), overlapping the rest of the update.
 __global__ __launch_bounds__(256) void trailing_update_kernel(float2* __restrict__ A, int64_t ld,
-                                                              int64_t n, int64_t k0, int kb)
+                                                              int64_t n, int64_t k0, int kb,
+                                                              float2* __restrict__ Linv_next,
+                                                              float2* __restrict__ UinvT_next,
+                                                              int* __restrict__ bad)
 {
-    __shared__ float Are[NB * KP], Aim[NB * KP]; // [row][k]
-    __shared__ float Bre[NB * KP], Bim[NB * KP]; // [col][k]
+    __shared__ __align__(16) float lds[kTileLdsFloats];
+    static_assert(sizeof(DiagLds) <= sizeof(float) * kTileLdsFloats, "LDS reuse");
+    float* Are = lds;
+    float* Aim = lds + NB * KP;
+    float* Bre = lds + 2 * NB * KP;
+    float* Bim = lds + 3 * NB * KP;
     const int t = threadIdx.x;
     const int64_t rest0 = k0 + kb;
     const int64_t r0 = rest0 + (int64_t)blockIdx.x * NB;
@@ -363,56 +416,34 @@ __global__ __launch_bounds__(256) void trailing_update_kernel(float2* __restrict
         Bim[c * KP + kk] = u.y;
     }
     __syncthreads();
-    const int wave = t >> 6, lane = t & 63;
-    const int wr = (wave & 1) * 32, wc = (wave >> 1) * 32;
-    const int li = lane % 16, lk = lane / 16;
-    v4f cre[2][2], cim[2][2];
-#pragma unroll
-    for (int p = 0; p < 2; ++p)
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            cre[p][q] = v4f{0.f, 0.f, 0.f, 0.f};
-            cim[p][q] = v4f{0.f, 0.f, 0.f, 0.f};
-        }
-    for (int ks = 0; ks < NB; ks += 4) {
-        float ar[2], ai[2], br[2], bi[2];
-#pragma unroll
-        for (int p = 0; p < 2; ++p) {
-            ar[p] = Are[(wr + 16 * p + li) * KP + ks + lk];
-            ai[p] = Aim[(wr + 16 * p + li) * KP + ks + lk];
-            br[p] = Bre[(wc + 16 * p + li) * KP + ks + lk];
-            bi[p] = Bim[(wc + 16 * p + li) * KP + ks + lk];
-        }
-#pragma unroll
-        for (int p = 0; p < 2; ++p)
-#pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                cre[p][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[p], br[q], cre[p][q], 0, 0, 0);
-                cre[p][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(-ai[p], bi[q], cre[p][q], 0, 0, 0);
-                cim[p][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[p], bi[q], cim[p][q], 0, 0, 0);
-                cim[p][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[p], br[q], cim[p][q], 0, 0, 0);
-            }
-    }
+    TileAcc acc;
+    cgemm_64(Are, Aim, Bre, Bim, acc);
 #pragma unroll
     for (int p = 0; p < 2; ++p)
 #pragma unroll
         for (int q = 0; q < 2; ++q)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int64_t i = r0 + wr + 16 * p + 4 * lk + r;
-                const int64_t j = c0 + wc + 16 * q + li;
+                const int64_t i = r0 + acc_row(p, r);
+                const int64_t j = c0 + acc_col(q);
                 if (i < n && j < n) {
                     float2 v = A[i + ld * j];
-                    v.x -= cre[p][q][r];
-                    v.y -= cim[p][q][r];
+                    v.x -= acc.re[p][q][r];
+                    v.y -= acc.im[p][q][r];
                     A[i + ld * j] = v;
                 }
             }
+    if (Linv_next && blockIdx.x == 0 && blockIdx.y == 0) {
+        __syncthreads(); // the tile's stores are visible to the workgroup; the LDS is free
+        diag_factor(A, ld, rest0, (int)min((int64_t)NB, n - rest0), Linv_next, UinvT_next, bad,
+                    *reinterpret_cast<DiagLds*>(lds));
+    }
 }
 
 // ---- block-triangular sweeps -------------------------------------------------------------------
 // Y (n x nrhs, column-major, FP32) is solved in place.  Lower: unit-lower L from the factors with
-// inverted diagonal blocks Linv; block b waits for blocks < b.  Upper: blocks in reverse order.
+// inverted diagonal blocks Linv; block b waits for blocks < b.  Upper: blocks in reverse order, with
+// the transposed inverses UinvT.
 // Each workgroup owns one 64-row block; the first workgroups dispatched own the blocks without
 // dependencies, so every wait is on a block that is already running or done.  A wait gives up
 // after kSpinLimit polls and raises *err, so the kernel always terminates.
@@ -439,11 +470,12 @@ __global__ __launch_bounds__(256) void block_trsv_kernel(const float2* __restric
     float2 s[kMaxSolveRhs];
 #pragma unroll
     for (int h = 0; h < kMaxSolveRhs; ++h) s[h] = zero;
-    // the diagonal-block inverse, this thread's 16 columns of row `row`
+    // the diagonal-block inverse, this thread's 16 columns of row `row` (the upper sweep's
+    // inverses are stored transposed)
     float2 dv[16];
     const float2* D = Dinv + (int64_t)b * NB * NB;
 #pragma unroll
-    for (int q = 0; q < 16; ++q) dv[q] = D[row + NB * (k0 + q)];
+    for (int q = 0; q < 16; ++q) dv[q] = LOWER ? D[row + NB * (k0 + q)] : D[(k0 + q) + NB * row];
     // Dependencies are consumed in the order they complete: ascending for the lower sweep,
     // descending for the upper sweep (block b + 1 is the last of b's dependencies to finish).
     const int ndep = LOWER ? b : nblk - 1 - b;
@@ -657,7 +689,7 @@ struct bem2d_lu32 {
     bool factored = false;
     float2* F = nullptr;          // n x n factors, ld = n
     float2* Linv = nullptr;       // nblk x 64 x 64
-    float2* Uinv = nullptr;
+    float2* UinvT = nullptr;      // nblk x (U_bb^{-1})^T
     unsigned long long* flags = nullptr; // 2 * nblk (lower, upper)
     unsigned long long epoch = 0;
     int* dflag = nullptr;         // [0] bad pivot, [1] sweep time-out
@@ -694,7 +726,7 @@ extern "C" void bem2d_lu32_destroy(bem2d_lu32* f)
     (void)hipStreamSynchronize(f->ctx->stream);
     (void)hipFree(f->F);
     (void)hipFree(f->Linv);
-    (void)hipFree(f->Uinv);
+    (void)hipFree(f->UinvT);
     (void)hipFree(f->flags);
     (void)hipFree(f->dflag);
     (void)hipFree(f->anorm);
@@ -713,7 +745,7 @@ extern "C" int bem2d_lu32_create(bem2d_ctx* c, int64_t n, bem2d_lu32** out)
     f->nblk = (int)((n + NB - 1) / NB);
     const size_t inv_bytes = sizeof(float2) * (size_t)f->nblk * NB * NB;
     if (hipMalloc(&f->F, sizeof(float2) * n * n) != hipSuccess ||
-        hipMalloc(&f->Linv, inv_bytes) != hipSuccess || hipMalloc(&f->Uinv, inv_bytes) != hipSuccess ||
+        hipMalloc(&f->Linv, inv_bytes) != hipSuccess || hipMalloc(&f->UinvT, inv_bytes) != hipSuccess ||
         hipMalloc(&f->flags, sizeof(unsigned long long) * 2 * f->nblk) != hipSuccess ||
         hipMalloc(&f->dflag, sizeof(int) * 2) != hipSuccess ||
         hipMalloc(&f->anorm, sizeof(double)) != hipSuccess ||
@@ -746,19 +778,24 @@ extern "C" int bem2d_lu32_factor(bem2d_ctx* c, bem2d_lu32* f, const void* A64, i
         rowsum_max_kernel<<<1, 256, 0, s>>>(n, 16, f->rowpart, f->anorm);
         zero_u64_kernel<<<1, 64, 0, s>>>((unsigned long long*)f->dflag, 1);
     }
+    // Step b: panel solves of block b, then the trailing update, whose workgroup (0, 0) also
+    // factorises diagonal block b + 1 as soon as its own tile is updated.
+    diag_lu_kernel<<<1, 256, 0, s>>>(f->F, n, 0, (int)std::min<int64_t>(NB, n), f->Linv, f->UinvT,
+                                     f->dflag);
     for (int b = 0; b < f->nblk; ++b) {
         const int64_t k0 = (int64_t)b * NB;
         const int kb = (int)std::min<int64_t>(NB, n - k0);
-        diag_lu_kernel<<<1, 256, 0, s>>>(f->F, n, k0, kb, f->Linv + (size_t)b * NB * NB,
-                                         f->Uinv + (size_t)b * NB * NB, f->dflag);
         const int64_t rest = n - k0 - kb;
         if (rest > 0) {
             const int tiles = (int)((rest + NB - 1) / NB);
             panel_trsm_kernel<<<2 * tiles, 256, 0, s>>>(f->F, n, n, k0, kb, tiles,
                                                         f->Linv + (size_t)b * NB * NB,
-                                                        f->Uinv + (size_t)b * NB * NB);
+                                                        f->UinvT + (size_t)b * NB * NB);
             dim3 g((unsigned)tiles, (unsigned)tiles);
-            trailing_update_kernel<<<g, 256, 0, s>>>(f->F, n, n, k0, kb);
+            trailing_update_kernel<<<g, 256, 0, s>>>(f->F, n, n, k0, kb,
+                                                     f->Linv + (size_t)(b + 1) * NB * NB,
+                                                     f->UinvT + (size_t)(b + 1) * NB * NB,
+                                                     f->dflag);
         }
     }
     bem2d_internal_prof_end(c, BEM2D_PROF_LU);
@@ -792,7 +829,7 @@ static int lu32_apply(bem2d_lu32* f, float2* Y, int nrhs)
     f->epoch++;
     block_trsv_kernel<true><<<f->nblk, 256, 0, s>>>(f->F, f->n, f->n, f->nblk, f->Linv, Y, f->n,
                                                      nrhs, f->flags, f->epoch, f->dflag + 1);
-    block_trsv_kernel<false><<<f->nblk, 256, 0, s>>>(f->F, f->n, f->n, f->nblk, f->Uinv, Y, f->n,
+    block_trsv_kernel<false><<<f->nblk, 256, 0, s>>>(f->F, f->n, f->n, f->nblk, f->UinvT, Y, f->n,
                                                       nrhs, f->flags + f->nblk, f->epoch,
                                                       f->dflag + 1);
     return hipGetLastError() == hipSuccess ? BEM2D_OK : BEM2D_ERR_DEVICE;
