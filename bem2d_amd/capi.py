@@ -78,6 +78,41 @@ SIGNATURES = {
     "bem2d_hankel01_host": (_I, [_P, _I64, _P]),
 }
 
+# ---- serving engine ------------------------------------------------------------------------------
+PRECISION_MIXED, PRECISION_FP64 = 0, 1
+
+
+class Request(ctypes.Structure):
+    """bem2d_request: one scattering problem for the serving engine (host arrays)."""
+    _fields_ = [("geom", Geometry), ("opts", QuadOption), ("k", _D), ("eta", _D), ("nrhs", _I),
+                ("precision", _I), ("dirs", _P), ("nang", _I64), ("theta", _P), ("tol", _D)]
+
+
+class ResultInfo(ctypes.Structure):
+    _fields_ = [("status", _I), ("precision", _I), ("n", _I64), ("nrhs", _I), ("iterations", _I),
+                ("backward_error", _D), ("t_arrival", _D), ("t_admit", _D), ("t_done", _D)]
+
+
+class EngineStats(ctypes.Structure):
+    _fields_ = [(name, _I64) for name in (
+        "submitted", "admitted", "completed", "failed", "assembly_launches", "lu_step_launches",
+        "refine_step_launches", "arena_bytes", "arena_in_use", "queued", "active")]
+
+
+SIGNATURES.update({
+    "bem2d_engine_footprint": (_I64, [_I64, _I, _I64, _I, QuadOption]),
+    "bem2d_engine_create": (_I, [_P, _I64, _I, ctypes.POINTER(_P)]),
+    "bem2d_engine_destroy": (None, [_P]),
+    "bem2d_engine_now": (_D, [_P]),
+    "bem2d_engine_submit": (_I, [_P, ctypes.POINTER(Request), _D, ctypes.POINTER(_I64)]),
+    "bem2d_engine_step": (_I, [_P, ctypes.POINTER(_I)]),
+    "bem2d_engine_poll": (_I, [_P, _I64, ctypes.POINTER(_I)]),
+    "bem2d_engine_result": (_I, [_P, _I64, _P, _P, ctypes.POINTER(ResultInfo)]),
+    "bem2d_engine_get_stats": (_I, [_P, ctypes.POINTER(EngineStats)]),
+    "bem2d_engine_run_trace": (_I, [_P, ctypes.POINTER(Request), _P, _I, _P]),
+})
+
+
 _LIB = None
 
 
@@ -310,6 +345,97 @@ class Mesh:
     def close(self):
         if self.h:
             lib().bem2d_mesh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Engine:
+    """The serving engine (bem2d_engine): continuous batching of scattering requests on one GPU.
+
+    Requests are dicts or objects with geom (Geometry), k, eta, dirs ((nrhs, 2)), theta (angles of
+    the far field), precision (PRECISION_MIXED / PRECISION_FP64), tol, N, L, sigma.  Host arrays
+    are copied by the engine at submission."""
+
+    def __init__(self, ctx, hbm_bytes, max_active=64):
+        self.ctx = ctx
+        h = _P()
+        check(lib().bem2d_engine_create(ctx.h, int(hbm_bytes), int(max_active), ctypes.byref(h)))
+        self.h = h
+        self._meta = {}
+
+    @staticmethod
+    def footprint(n, nrhs, nang, precision=PRECISION_MIXED, N=16, L=3, sigma=0.15):
+        return lib().bem2d_engine_footprint(n, nrhs, nang, precision, QuadOption(L, N, sigma))
+
+    def now(self):
+        return lib().bem2d_engine_now(self.h)
+
+    @staticmethod
+    def _request(req, keep):
+        geom = req["geom"]
+        dirs = np.ascontiguousarray(req["dirs"], dtype=np.float64).reshape(-1, 2)
+        theta = np.ascontiguousarray(req.get("theta", np.zeros(0)), dtype=np.float64)
+        keep.append((geom, dirs, theta))
+        g = Geometry(geom.n, _np_ptr(geom.type), _np_ptr(geom.param), _np_ptr(geom.index),
+                     _np_ptr(geom.next), _np_ptr(geom.prev))
+        return Request(g, QuadOption(req.get("L", 3), req.get("N", 16), req.get("sigma", 0.15)),
+                       float(req["k"]), float(req.get("eta", req["k"])), dirs.shape[0],
+                       int(req.get("precision", PRECISION_MIXED)), _np_ptr(dirs), theta.size,
+                       _np_ptr(theta) if theta.size else None, float(req.get("tol", 1e-12)))
+
+    def submit(self, req, t_arrival=-1.0):
+        keep = []
+        r = self._request(req, keep)
+        t = _I64()
+        check(lib().bem2d_engine_submit(self.h, ctypes.byref(r), t_arrival, ctypes.byref(t)))
+        self._meta[t.value] = (req["geom"].n, r.nrhs, r.nang)
+        return t.value
+
+    def step(self):
+        u = _I()
+        check(lib().bem2d_engine_step(self.h, ctypes.byref(u)))
+        return u.value
+
+    def done(self, ticket):
+        d = _I()
+        check(lib().bem2d_engine_poll(self.h, ticket, ctypes.byref(d)))
+        return bool(d.value)
+
+    def result(self, ticket, raise_on_fail=True):
+        """(x (n, nrhs), far field (nang, nrhs), info) of a finished request; forgets it."""
+        n, nrhs, nang = self._meta.pop(ticket)
+        x = np.zeros((nrhs, n), dtype=np.complex128)
+        ff = np.zeros((nrhs, max(nang, 1)), dtype=np.complex128)
+        info = ResultInfo()
+        rc = lib().bem2d_engine_result(self.h, ticket, _np_ptr(x), _np_ptr(ff), ctypes.byref(info))
+        if raise_on_fail:
+            check(rc)
+        return x.T, ff[:, :nang].T, info
+
+    def run_trace(self, reqs, arrival):
+        """Replays reqs with arrival times (s, relative to the call); returns the tickets."""
+        keep = []
+        arr = (Request * len(reqs))(*[self._request(q, keep) for q in reqs])
+        at = np.ascontiguousarray(arrival, dtype=np.float64)
+        tickets = np.zeros(len(reqs), dtype=np.int64)
+        check(lib().bem2d_engine_run_trace(self.h, arr, _np_ptr(at), len(reqs), _np_ptr(tickets)))
+        for q, r, t in zip(reqs, arr, tickets):
+            self._meta[int(t)] = (q["geom"].n, r.nrhs, r.nang)
+        return [int(t) for t in tickets]
+
+    def stats(self):
+        s = EngineStats()
+        check(lib().bem2d_engine_get_stats(self.h, ctypes.byref(s)))
+        return {name: getattr(s, name) for name, _ in EngineStats._fields_}
+
+    def close(self):
+        if self.h:
+            lib().bem2d_engine_destroy(self.h)
             self.h = None
 
     def __del__(self):

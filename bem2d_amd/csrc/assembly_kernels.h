@@ -114,6 +114,41 @@ __global__ void prep_bounds_kernel(int64_t n, int nq, const double* __restrict__
     bounds[BOUND_R * n + e] = sqrt(r2) * (1.0 + 1e-12) + 1e-300;
 }
 
+// ---- batched launches --------------------------------------------------------------------------
+// An assembly launch serves up to kMaxBatch problems (meshes) of one kernel kind and one nq: the
+// descriptors travel by value in the kernel arguments, and workgroup b belongs to the problem d with
+// start[d] <= b < start[d + 1].  A single problem is a batch of one.
+constexpr int kMaxBatch = 16;
+
+struct AsmProblem {
+    int64_t n;
+    int nq, m1, m2, m3;           // Gauss points; node counts of AdaptedGauss1/2/3
+    const double* nodes;          // NODE_PLANES planes of nq*n, then BOUND_PLANES planes of n
+    const int64_t* adj;           // index | next | prev, n each
+    const int64_t* special;       // (i, j, rule) triples
+    int64_t nspecial;
+    const int32_t* type;
+    const double* param;
+    const double* rules;          // AG1 x|y|w, AG2 x|y|w, AG3 x|y|w
+    const double* g1d;            // Gauss1D x | w
+    double k, eta, kscale, mscale;
+    double2* A;
+    int64_t lda;
+};
+
+struct AsmBatch {
+    int count;
+    int64_t start[kMaxBatch + 1]; // first workgroup of each problem; start[count] = grid size
+    AsmProblem p[kMaxBatch];
+};
+
+__device__ __forceinline__ int batch_find(const int64_t* start, int count, int64_t b)
+{
+    int d = 0;
+    while (d + 1 < count && b >= start[d + 1]) ++d;
+    return d;
+}
+
 // Kernel values of one regular node pair for both orders of its element pair: (kr, ki) with the
 // test node x on element i and the trial node y on element j, (mr, mi) with test node y on j and
 // trial node x on i.  The two share |x - y| and with it H0 and H1; only the normal factor differs,
@@ -268,19 +303,27 @@ __device__ __forceinline__ void pair_sums(int nq, int64_t n, int64_t i, const do
 // branch-free asymptotic loop.  Coincident / adjacent pairs are skipped here and written by
 // assemble_special_kernel.  NQ = nq at compile time (0: runtime nq).
 template <int KIND, int NQ>
-__global__ __launch_bounds__(64 * kWavesPerBlock) void assemble_regular_kernel(
-    int64_t n, int nq_rt, const double* __restrict__ nodes, const double* __restrict__ bounds,
-    const int64_t* __restrict__ index, const int64_t* __restrict__ next,
-    const int64_t* __restrict__ prev, double k, double eta, double kscale,
-    double2* __restrict__ A, int64_t lda)
+__global__ __launch_bounds__(64 * kWavesPerBlock) void assemble_regular_kernel(const AsmBatch batch)
 {
+    const int di = batch_find(batch.start, batch.count, blockIdx.x);
+    const AsmProblem& pb = batch.p[di];
+    const int64_t n = pb.n;
+    const int nq_rt = pb.nq;
+    const double* __restrict__ nodes = pb.nodes;
+    const double* __restrict__ bounds = pb.nodes + (int64_t)NODE_PLANES * n * pb.nq;
+    const int64_t* __restrict__ index = pb.adj;
+    const int64_t* __restrict__ next = pb.adj + n;
+    const int64_t* __restrict__ prev = pb.adj + 2 * n;
+    const double k = pb.k, eta = pb.eta, kscale = pb.kscale;
+    double2* __restrict__ A = pb.A;
+    const int64_t lda = pb.lda;
     extern __shared__ double sh[]; // [kColsPerBlock][nq][5]
     const int nq = NQ ? NQ : nq_rt;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t Q = (n + kColsPerBlock - 1) / kColsPerBlock;
     const int64_t P = (n + kRowsPerBlock - 1) / kRowsPerBlock;
-    const int64_t t = blockIdx.x;
+    const int64_t t = (int64_t)blockIdx.x - batch.start[di];
     constexpr double m = kColBlocksPerRowBlock;
     const double qq = (double)Q + 0.5 * m; // smaller root of (m/2) p^2 - (Q + m/2) p + t = 0
     int64_t p = (int64_t)((qq - sqrt(fmax(qq * qq - 2.0 * m * (double)t, 0.0))) / m);
@@ -341,14 +384,22 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void assemble_regular_kernel(
 //         mass entry (lib/bem2d_quadrature.cpp:108-123) scaled by mscale;
 // rule 3: next(i) == index(j), Integrate with AdaptedGauss3; rule 2: prev, AdaptedGauss2.
 template <int KIND>
-__global__ __launch_bounds__(256) void assemble_special_kernel(
-    const int64_t* __restrict__ special, const int32_t* __restrict__ type,
-    const double* __restrict__ param, const double* __restrict__ rules, int m1, int m2, int m3,
-    const double* __restrict__ gx, const double* __restrict__ gw, int nq, double k, double eta,
-    double kscale, double mscale, double2* __restrict__ A, int64_t lda)
+__global__ __launch_bounds__(256) void assemble_special_kernel(const AsmBatch batch)
 {
+    const int di = batch_find(batch.start, batch.count, blockIdx.x);
+    const AsmProblem& pb = batch.p[di];
+    const int64_t* __restrict__ special = pb.special;
+    const int32_t* __restrict__ type = pb.type;
+    const double* __restrict__ param = pb.param;
+    const double* __restrict__ rules = pb.rules;
+    const int m1 = pb.m1, m2 = pb.m2, m3 = pb.m3, nq = pb.nq;
+    const double* __restrict__ gx = pb.g1d;
+    const double* __restrict__ gw = pb.g1d + pb.nq;
+    const double k = pb.k, eta = pb.eta, kscale = pb.kscale, mscale = pb.mscale;
+    double2* __restrict__ A = pb.A;
+    const int64_t lda = pb.lda;
     __shared__ double red[2][256];
-    const int64_t e = blockIdx.x;
+    const int64_t e = (int64_t)blockIdx.x - batch.start[di];
     const int64_t i = special[3 * e], j = special[3 * e + 1];
     const int rule = (int)special[3 * e + 2];
     const int ti = type[i], tj = type[j];

@@ -365,8 +365,27 @@ extern "C" int bem2d_quad_rule(int which, int N, int L, double sigma, double* x,
 // ------------------------------------------------------------------------------------------------
 // mesh
 // ------------------------------------------------------------------------------------------------
+// Device bytes of a mesh of n elements (an upper bound: at most 3 singular entries per row).
+size_t bem2d_internal_mesh_bytes(int64_t n, bem2d_quad_option q)
+{
+    const int64_t N = q.N, L = q.L;
+    const int64_t nodes = N * N * L * L + 6 * N * N * L;
+    return sizeof(int32_t) * n + sizeof(double) * 8 * n + sizeof(int64_t) * 3 * n +
+           sizeof(double) * (NODE_PLANES * n * N + BOUND_PLANES * n) + sizeof(double) * (3 * nodes + 1) +
+           sizeof(double) * 2 * N + sizeof(int64_t) * (9 * n + 3) + 8 * 256;
+}
+
 extern "C" int bem2d_mesh_create(bem2d_ctx* c, const bem2d_geometry* g, bem2d_quad_option q,
                                  bem2d_mesh** out)
+{
+    if (!c) return FAIL(BEM2D_ERR_ARG, "bem2d_mesh_create: null argument");
+    return bem2d_internal_mesh_create(c, g, q, nullptr, c->stream, out);
+}
+
+// Mesh whose device arrays come from arena (null: hipMalloc), built on stream s (synchronised
+// before returning: the host staging vectors are local).
+int bem2d_internal_mesh_create(bem2d_ctx* c, const bem2d_geometry* g, bem2d_quad_option q,
+                               DevArena* arena, hipStream_t s, bem2d_mesh** out)
 {
     if (!c || !g || !out) return FAIL(BEM2D_ERR_ARG, "bem2d_mesh_create: null argument");
     if (g->n < 1) return FAIL(BEM2D_ERR_ARG, "bem2d_mesh_create: empty geometry");
@@ -419,6 +438,7 @@ extern "C" int bem2d_mesh_create(bem2d_ctx* c, const bem2d_geometry* g, bem2d_qu
 
     bem2d_mesh* m = new bem2d_mesh();
     m->ctx = c;
+    m->arena = arena;
     m->n = n;
     m->opts = q;
     m->nq = q.N;
@@ -443,19 +463,20 @@ extern "C" int bem2d_mesh_create(bem2d_ctx* c, const bem2d_geometry* g, bem2d_qu
         return FAIL(code, "bem2d_mesh_create: %s", what);
     };
     size_t bytes_nodes = sizeof(double) * (NODE_PLANES * (size_t)n * q.N + BOUND_PLANES * (size_t)n);
-    if (hipMalloc(&m->d_type, sizeof(int32_t) * n) != hipSuccess ||
-        hipMalloc(&m->d_param, sizeof(double) * 8 * n) != hipSuccess ||
-        hipMalloc(&m->d_adj, sizeof(int64_t) * 3 * n) != hipSuccess ||
-        hipMalloc(&m->d_nodes, bytes_nodes) != hipSuccess ||
-        hipMalloc(&m->d_rules, sizeof(double) * (rules.size() + 1)) != hipSuccess ||
-        hipMalloc(&m->d_g1d, sizeof(double) * g1d.size()) != hipSuccess ||
-        hipMalloc(&m->d_special, sizeof(int64_t) * (special.size() + 3)) != hipSuccess)
+    m->d_type = (int32_t*)bem2d_internal_dalloc(arena, sizeof(int32_t) * n);
+    m->d_param = (double*)bem2d_internal_dalloc(arena, sizeof(double) * 8 * n);
+    m->d_adj = (int64_t*)bem2d_internal_dalloc(arena, sizeof(int64_t) * 3 * n);
+    m->d_nodes = (double*)bem2d_internal_dalloc(arena, bytes_nodes);
+    m->d_rules = (double*)bem2d_internal_dalloc(arena, sizeof(double) * (rules.size() + 1));
+    m->d_g1d = (double*)bem2d_internal_dalloc(arena, sizeof(double) * g1d.size());
+    m->d_special = (int64_t*)bem2d_internal_dalloc(arena, sizeof(int64_t) * (special.size() + 3));
+    if (!m->d_type || !m->d_param || !m->d_adj || !m->d_nodes || !m->d_rules || !m->d_g1d ||
+        !m->d_special)
         return cleanup(BEM2D_ERR_NOMEM, "device allocation failed");
     std::vector<int64_t> adj(3 * n);
     std::memcpy(adj.data(), g->index, sizeof(int64_t) * n);
     std::memcpy(adj.data() + n, g->next, sizeof(int64_t) * n);
     std::memcpy(adj.data() + 2 * n, g->prev, sizeof(int64_t) * n);
-    hipStream_t s = c->stream;
     bool ok = hipMemcpyAsync(m->d_type, g->type, sizeof(int32_t) * n, hipMemcpyHostToDevice, s) ==
                   hipSuccess &&
               hipMemcpyAsync(m->d_param, g->param, sizeof(double) * 8 * n, hipMemcpyHostToDevice,
@@ -486,14 +507,15 @@ extern "C" void bem2d_mesh_destroy(bem2d_mesh* m)
 {
     if (!m) return;
     (void)hipSetDevice(m->ctx->device);
-    (void)hipStreamSynchronize(m->ctx->stream);
-    (void)hipFree(m->d_type);
-    (void)hipFree(m->d_param);
-    (void)hipFree(m->d_adj);
-    (void)hipFree(m->d_nodes);
-    (void)hipFree(m->d_rules);
-    (void)hipFree(m->d_g1d);
-    (void)hipFree(m->d_special);
+    if (!m->arena) (void)hipStreamSynchronize(m->ctx->stream);
+    DevArena* a = m->arena;
+    bem2d_internal_dfree(a, m->d_type);
+    bem2d_internal_dfree(a, m->d_param);
+    bem2d_internal_dfree(a, m->d_adj);
+    bem2d_internal_dfree(a, m->d_nodes);
+    bem2d_internal_dfree(a, m->d_rules);
+    bem2d_internal_dfree(a, m->d_g1d);
+    bem2d_internal_dfree(a, m->d_special);
     delete m;
 }
 
@@ -503,37 +525,107 @@ extern "C" int64_t bem2d_mesh_special_count(const bem2d_mesh* m) { return m ? m-
 // ------------------------------------------------------------------------------------------------
 // assembly
 // ------------------------------------------------------------------------------------------------
-template <int KIND>
-static int launch_assembly(bem2d_ctx* c, const bem2d_mesh* m, double k, double eta, double mscale,
-                           double kscale, double2* A, int64_t lda)
+static AsmProblem asm_problem(const bem2d_mesh* m, double k, double eta, double mscale,
+                              double kscale, double2* A, int64_t lda)
 {
-    hipStream_t s = c->stream;
-    const int64_t n = m->n;
-    const int nq = m->nq;
-    const int64_t* idx = m->d_adj;
-    const int64_t* nxt = m->d_adj + n;
-    const int64_t* prv = m->d_adj + 2 * n;
-    const int64_t tiles = upper_tile_count(n);
-    if (tiles > (int64_t)INT32_MAX)
-        return FAIL(BEM2D_ERR_UNSUPPORTED, "bem2d_assemble: n = %lld too large", (long long)n);
-    dim3 grid((unsigned)tiles);
-    size_t shm = sizeof(double) * kColsPerBlock * nq * NODE_PLANES;
-    bem2d_internal_prof_begin(c, BEM2D_PROF_ASSEMBLE_REGULAR);
-    if (nq == 16)
-        assemble_regular_kernel<KIND, 16><<<grid, 64 * kWavesPerBlock, shm, s>>>(
-            n, nq, m->d_nodes, m->bounds(), idx, nxt, prv, k, eta, kscale, A, lda);
-    else
-        assemble_regular_kernel<KIND, 0><<<grid, 64 * kWavesPerBlock, shm, s>>>(
-            n, nq, m->d_nodes, m->bounds(), idx, nxt, prv, k, eta, kscale, A, lda);
-    bem2d_internal_prof_end(c, BEM2D_PROF_ASSEMBLE_REGULAR);
-    if (m->nspecial > 0) {
-        bem2d_internal_prof_begin(c, BEM2D_PROF_ASSEMBLE_SPECIAL);
-        assemble_special_kernel<KIND><<<(unsigned)m->nspecial, 256, 0, s>>>(
-            m->d_special, m->d_type, m->d_param, m->d_rules, m->m1, m->m2, m->m3, m->d_g1d,
-            m->d_g1d + nq, nq, k, eta, kscale, mscale, A, lda);
-        bem2d_internal_prof_end(c, BEM2D_PROF_ASSEMBLE_SPECIAL);
+    AsmProblem p{};
+    p.n = m->n;
+    p.nq = m->nq;
+    p.m1 = m->m1;
+    p.m2 = m->m2;
+    p.m3 = m->m3;
+    p.nodes = m->d_nodes;
+    p.adj = m->d_adj;
+    p.special = m->d_special;
+    p.nspecial = m->nspecial;
+    p.type = m->d_type;
+    p.param = m->d_param;
+    p.rules = m->d_rules;
+    p.g1d = m->d_g1d;
+    p.k = k;
+    p.eta = eta;
+    p.kscale = kscale;
+    p.mscale = mscale;
+    p.A = A;
+    p.lda = lda;
+    return p;
+}
+
+// Regular and singular-pair kernels for problems[0 .. count) (one kernel kind, one nq), in launches
+// of up to kMaxBatch problems each.
+template <int KIND>
+static int launch_assembly(bem2d_ctx* c, hipStream_t s, const AsmProblem* probs, int count)
+{
+    const int nq = probs[0].nq;
+    for (int c0 = 0; c0 < count; c0 += kMaxBatch) {
+        const int cnt = std::min(kMaxBatch, count - c0);
+        AsmBatch reg{}, sp{};
+        reg.count = sp.count = cnt;
+        for (int d = 0; d < cnt; ++d) {
+            const AsmProblem& p = probs[c0 + d];
+            if (p.nq != nq) return FAIL(BEM2D_ERR_ARG, "batched assembly mixes Gauss orders");
+            reg.p[d] = sp.p[d] = p;
+            reg.start[d + 1] = reg.start[d] + upper_tile_count(p.n);
+            sp.start[d + 1] = sp.start[d] + p.nspecial;
+        }
+        if (reg.start[cnt] > (int64_t)INT32_MAX || sp.start[cnt] > (int64_t)INT32_MAX)
+            return FAIL(BEM2D_ERR_UNSUPPORTED, "assembly grid too large");
+        const size_t shm = sizeof(double) * kColsPerBlock * nq * NODE_PLANES;
+        if (reg.start[cnt] > 0) {
+            bem2d_internal_prof_begin(c, BEM2D_PROF_ASSEMBLE_REGULAR);
+            if (nq == 16)
+                assemble_regular_kernel<KIND, 16>
+                    <<<(unsigned)reg.start[cnt], 64 * kWavesPerBlock, shm, s>>>(reg);
+            else
+                assemble_regular_kernel<KIND, 0>
+                    <<<(unsigned)reg.start[cnt], 64 * kWavesPerBlock, shm, s>>>(reg);
+            bem2d_internal_prof_end(c, BEM2D_PROF_ASSEMBLE_REGULAR);
+        }
+        if (sp.start[cnt] > 0) {
+            bem2d_internal_prof_begin(c, BEM2D_PROF_ASSEMBLE_SPECIAL);
+            assemble_special_kernel<KIND><<<(unsigned)sp.start[cnt], 256, 0, s>>>(sp);
+            bem2d_internal_prof_end(c, BEM2D_PROF_ASSEMBLE_SPECIAL);
+        }
     }
     return hipGetLastError() == hipSuccess ? BEM2D_OK : BEM2D_ERR_DEVICE;
+}
+
+int bem2d_internal_assemble_many(bem2d_ctx* c, hipStream_t s, int kind,
+                                 const bem2d_mesh* const* meshes, const double* k,
+                                 const double* eta, double mscale, double kscale,
+                                 void* const* A, const int64_t* lda, int count)
+{
+    if (count <= 0) return BEM2D_OK;
+    std::vector<AsmProblem> probs(count);
+    for (int d = 0; d < count; ++d) {
+        if (!meshes[d] || !A[d] || lda[d] < meshes[d]->n)
+            return FAIL(BEM2D_ERR_ARG, "batched assembly: bad problem %d", d);
+        if (!(k[d] > 0.0) && kind != BEM2D_KERNEL_NONE && kind != BEM2D_KERNEL_ID)
+            return FAIL(BEM2D_ERR_ARG, "wavenumber must be positive (got %g)", k[d]);
+        probs[d] = asm_problem(meshes[d], k[d], eta[d], mscale, kscale, (double2*)A[d], lda[d]);
+    }
+    // group by Gauss order (the kernels are specialised for nq = 16)
+    std::stable_sort(probs.begin(), probs.end(),
+                     [](const AsmProblem& a, const AsmProblem& b) { return a.nq < b.nq; });
+    int rc = BEM2D_OK;
+    for (int g0 = 0; g0 < count && rc == BEM2D_OK;) {
+        int g1 = g0;
+        while (g1 < count && probs[g1].nq == probs[g0].nq) ++g1;
+        const AsmProblem* ps = probs.data() + g0;
+        const int cnt = g1 - g0;
+        switch (kind) {
+        case BEM2D_KERNEL_NONE: rc = launch_assembly<BEM2D_KERNEL_NONE>(c, s, ps, cnt); break;
+        case BEM2D_KERNEL_SL: rc = launch_assembly<BEM2D_KERNEL_SL>(c, s, ps, cnt); break;
+        case BEM2D_KERNEL_DL: rc = launch_assembly<BEM2D_KERNEL_DL>(c, s, ps, cnt); break;
+        case BEM2D_KERNEL_CDL: rc = launch_assembly<BEM2D_KERNEL_CDL>(c, s, ps, cnt); break;
+        case BEM2D_KERNEL_COMBINED: rc = launch_assembly<BEM2D_KERNEL_COMBINED>(c, s, ps, cnt); break;
+        case BEM2D_KERNEL_ID: rc = launch_assembly<BEM2D_KERNEL_ID>(c, s, ps, cnt); break;
+        default: return FAIL(BEM2D_ERR_UNSUPPORTED, "unknown kernel kind %d", kind);
+        }
+        g0 = g1;
+    }
+    if (rc) return FAIL(rc, "assembly kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
+    return BEM2D_OK;
 }
 
 extern "C" int bem2d_assemble(bem2d_ctx* c, const bem2d_mesh* m, int kind, double k, double eta,
@@ -545,19 +637,10 @@ extern "C" int bem2d_assemble(bem2d_ctx* c, const bem2d_mesh* m, int kind, doubl
     if (!(k > 0.0) && kind != BEM2D_KERNEL_NONE && kind != BEM2D_KERNEL_ID)
         return FAIL(BEM2D_ERR_ARG, "bem2d_assemble: wavenumber must be positive (got %g)", k);
     HIPCHK(hipSetDevice(c->device));
-    double2* out = (double2*)A;
-    int rc;
-    switch (kind) {
-    case BEM2D_KERNEL_NONE: rc = launch_assembly<BEM2D_KERNEL_NONE>(c, m, k, eta, mscale, kscale, out, lda); break;
-    case BEM2D_KERNEL_SL: rc = launch_assembly<BEM2D_KERNEL_SL>(c, m, k, eta, mscale, kscale, out, lda); break;
-    case BEM2D_KERNEL_DL: rc = launch_assembly<BEM2D_KERNEL_DL>(c, m, k, eta, mscale, kscale, out, lda); break;
-    case BEM2D_KERNEL_CDL: rc = launch_assembly<BEM2D_KERNEL_CDL>(c, m, k, eta, mscale, kscale, out, lda); break;
-    case BEM2D_KERNEL_COMBINED: rc = launch_assembly<BEM2D_KERNEL_COMBINED>(c, m, k, eta, mscale, kscale, out, lda); break;
-    case BEM2D_KERNEL_ID: rc = launch_assembly<BEM2D_KERNEL_ID>(c, m, k, eta, mscale, kscale, out, lda); break;
-    default: return FAIL(BEM2D_ERR_UNSUPPORTED, "unknown kernel kind %d", kind);
-    }
-    if (rc) return FAIL(rc, "assembly kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
-    return BEM2D_OK;
+    if (kind < BEM2D_KERNEL_NONE || kind > BEM2D_KERNEL_ID)
+        return FAIL(BEM2D_ERR_UNSUPPORTED, "unknown kernel kind %d", kind);
+    return bem2d_internal_assemble_many(c, c->stream, kind, &m, &k, &eta, mscale, kscale, &A, &lda,
+                                        1);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -584,6 +667,34 @@ extern "C" int bem2d_rhs(bem2d_ctx* c, const bem2d_mesh* m, int kind, double k, 
     return BEM2D_OK;
 }
 
+// Right-hand sides with device-resident directions on stream s (no synchronisation).
+int bem2d_internal_rhs(const bem2d_mesh* m, int kind, double k, double eta, const double* d_dirs,
+                       int nrhs, double scale, void* B, int64_t ldb, hipStream_t s)
+{
+    if (nrhs <= 0) return BEM2D_OK;
+    dim3 grid((unsigned)((m->n + 255) / 256), (unsigned)nrhs);
+    rhs_kernel<<<grid, 256, 0, s>>>(m->n, m->nq, m->d_nodes, kind, k, eta, d_dirs, scale,
+                                    (double2*)B, ldb);
+    return hipGetLastError() == hipSuccess ? BEM2D_OK : FAIL(BEM2D_ERR_DEVICE, "rhs launch failed");
+}
+
+// Far field on stream s with a caller-provided signature buffer F (nang x n complex128).
+int bem2d_internal_farfield(const bem2d_mesh* m, double k, const void* coef, int64_t ldc,
+                            int nrhs, const double* theta, int64_t nang, void* out, void* F,
+                            hipStream_t s)
+{
+    const int64_t tot = nang * m->n;
+    farfield_signature_kernel<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(
+        m->n, m->nq, m->d_nodes, k, theta, nang, (double2*)F);
+    const double pi = 3.14159265358979323846;
+    const double mag = -1.0 / std::sqrt(8.0 * pi * k);
+    dim3 grid((unsigned)((nang + 127) / 128), (unsigned)nrhs);
+    farfield_combine_kernel<<<grid, 128, 0, s>>>(m->n, (const double2*)F, nang,
+                                                 (const double2*)coef, ldc, mag * std::cos(pi / 4),
+                                                 mag * std::sin(pi / 4), (double2*)out);
+    return hipGetLastError() == hipSuccess ? BEM2D_OK : FAIL(BEM2D_ERR_DEVICE, "far-field launch failed");
+}
+
 extern "C" int bem2d_eval_single_layer(bem2d_ctx* c, const bem2d_mesh* m, double k,
                                        const void* coef, const double* pts, int64_t npts, void* out)
 {
@@ -606,17 +717,7 @@ extern "C" int bem2d_farfield(bem2d_ctx* c, const bem2d_mesh* m, double k, const
     HIPCHK(hipSetDevice(c->device));
     double2* F = (double2*)bem2d_internal_scratch(c, SCRATCH_FARFIELD, sizeof(double2) * nang * m->n);
     if (!F) return FAIL(BEM2D_ERR_NOMEM, "scratch allocation failed");
-    const int64_t tot = nang * m->n;
-    farfield_signature_kernel<<<(unsigned)((tot + 255) / 256), 256, 0, c->stream>>>(
-        m->n, m->nq, m->d_nodes, k, theta, nang, F);
-    const double pi = 3.14159265358979323846;
-    const double mag = -1.0 / std::sqrt(8.0 * pi * k);
-    dim3 grid((unsigned)((nang + 127) / 128), (unsigned)nrhs);
-    farfield_combine_kernel<<<grid, 128, 0, c->stream>>>(m->n, F, nang, (const double2*)coef, ldc,
-                                                         mag * std::cos(pi / 4), mag * std::sin(pi / 4),
-                                                         (double2*)out);
-    HIPCHK(hipGetLastError());
-    return BEM2D_OK;
+    return bem2d_internal_farfield(m, k, coef, ldc, nrhs, theta, nang, out, F, c->stream);
 }
 
 // ------------------------------------------------------------------------------------------------

@@ -47,8 +47,11 @@ struct bem2d_ctx {
 void bem2d_internal_prof_begin(bem2d_ctx* c, int id);
 void bem2d_internal_prof_end(bem2d_ctx* c, int id);
 
+struct DevArena;
+
 struct bem2d_mesh {
     bem2d_ctx* ctx = nullptr;
+    DevArena* arena = nullptr;           // source of the device arrays (null: hipMalloc)
     int64_t n = 0;
     bem2d_quad_option opts{};
     int nq = 0;
@@ -67,10 +70,33 @@ struct bem2d_mesh {
     double* bounds() const { return d_nodes + 5 * (size_t)n * nq; }
 };
 
+// Device memory source for meshes, factorisations and refinement workspaces; a null arena means
+// hipMalloc / hipFree.
+struct DevArena {
+    virtual void* alloc(size_t bytes) = 0;
+    virtual void release(void* p) = 0;
+    virtual ~DevArena() {}
+};
+void* bem2d_internal_dalloc(DevArena* a, size_t bytes);
+void bem2d_internal_dfree(DevArena* a, void* p);
+
 int bem2d_internal_fail(int code, const char* fmt, ...);
 void* bem2d_internal_scratch(bem2d_ctx* c, int slot, size_t bytes);
 void bem2d_internal_mixed_release(bem2d_ctx* c);
 void bem2d_internal_gauss01(int n, std::vector<double>& x, std::vector<double>& w);
+size_t bem2d_internal_mesh_bytes(int64_t n, bem2d_quad_option q);
+int bem2d_internal_mesh_create(bem2d_ctx* c, const bem2d_geometry* g, bem2d_quad_option q,
+                               DevArena* arena, hipStream_t s, bem2d_mesh** out);
+int bem2d_internal_rhs(const bem2d_mesh* m, int kind, double k, double eta, const double* d_dirs,
+                       int nrhs, double scale, void* B, int64_t ldb, hipStream_t s);
+int bem2d_internal_farfield(const bem2d_mesh* m, double k, const void* coef, int64_t ldc,
+                            int nrhs, const double* theta, int64_t nang, void* out, void* F,
+                            hipStream_t s);
+// Assembly of count meshes in batched launches on stream s (see bem2d_assemble for the meaning).
+int bem2d_internal_assemble_many(bem2d_ctx* c, hipStream_t s, int kind,
+                                 const bem2d_mesh* const* meshes, const double* k,
+                                 const double* eta, double mscale, double kscale,
+                                 void* const* A, const int64_t* lda, int count);
 
 #define FAIL(code, ...) bem2d_internal_fail((code), __VA_ARGS__)
 #define HIPCHK(expr)                                                                              \
@@ -80,3 +106,44 @@ void bem2d_internal_gauss01(int n, std::vector<double>& x, std::vector<double>& 
             return bem2d_internal_fail(BEM2D_ERR_DEVICE, "%s failed: %s", #expr,                  \
                                        hipGetErrorString(e_));                                    \
     } while (0)
+
+// ---- mixed-precision factorisation and refinement, several problems per launch ------------------
+int bem2d_internal_lu32_create(bem2d_ctx* c, int64_t n, DevArena* arena, bem2d_lu32** out);
+size_t bem2d_internal_lu32_bytes(int64_t n);
+// cast + |A|_inf + diagonal block 0, then one panel step per call (each problem at its own step)
+int bem2d_internal_lu_begin(hipStream_t s, bem2d_lu32* const* lus, const void* const* A64,
+                            const int64_t* lda, int count);
+int bem2d_internal_lu_advance(hipStream_t s, bem2d_lu32* const* lus, int count);
+bool bem2d_internal_lu_done(const bem2d_lu32* f);
+int64_t bem2d_internal_lu_n(const bem2d_lu32* f);
+
+struct RefSpec { // x = A^{-1} b by refinement with the FP32 factors lu
+    bem2d_lu32* lu;
+    const void* A64;
+    int64_t lda;
+    const void* B;
+    int64_t ldb;
+    void* X;
+    int64_t ldx;
+    int nrhs;
+    double tol;      // backward-error target
+    int maxit;       // refinement steps at most
+    DevArena* ws;    // source of this problem's workspace (null: the job's arena)
+};
+struct RefJob;
+size_t bem2d_internal_ref_bytes(int64_t n, int nrhs);
+// pinned: optional host buffer of >= kRefNormSlots x (units) 64-bit words for the read-backs
+// (null: the job allocates its own)
+constexpr int kRefNormSlots = 32;
+int bem2d_internal_ref_units(int nrhs);
+int bem2d_internal_ref_create(bem2d_ctx* c, DevArena* arena, const RefSpec* specs, int count,
+                              unsigned long long* pinned, RefJob** out);
+void bem2d_internal_ref_destroy(RefJob* j);
+// enqueue the initial residual (first call) or one correction step, then an asynchronous read-back
+int bem2d_internal_ref_step(RefJob* j, hipStream_t s);
+bool bem2d_internal_ref_ready(RefJob* j);
+int bem2d_internal_ref_collect(RefJob* j, bool* all_done);
+bool bem2d_internal_ref_spec_active(const RefJob* j, int sidx);
+void bem2d_internal_ref_report(const RefJob* j, int sidx, bem2d_ir_report* r);
+bool bem2d_internal_ref_bad_pivot(const RefJob* j, int sidx);
+int bem2d_internal_refine_sync(hipStream_t s, RefJob* j);

@@ -1,19 +1,25 @@
 // Mixed-precision dense complex solve: FP32 LU on the MFMA cores + FP64 iterative refinement.
 //
 // Factorisation (bem2d_lu32_factor): A32 = (complex64) A64, then a blocked right-looking LU with
-// 64-wide panels and no row interchanges.  Per panel: the 64x64 diagonal block is factorised by one
-// workgroup with the tile in registers, which also forms the inverses of its unit-lower and upper
-// triangles; the block column below and the block row to the right are multiplied by those
-// inverses; the trailing matrix receives the rank-64 update A22 -= L21 U12 as a complex GEMM on
-// v_mfma_f32_16x16x4f32 with FP32 accumulation (four real MFMA products per complex product).  Without interchanges the panel has
-// no sequential pivot search -- the step that makes partial-pivoting panels latency bound on a GPU.
-// A zero or non-finite pivot is flagged and the refinement then reports failure, after which the
+// 64-wide panels and no row interchanges.  Step b: the block column below and the block row right
+// of diagonal block b are multiplied by the inverses of its triangles (panel_trsm_kernel, MFMA),
+// and the trailing matrix receives the rank-64 update A22 -= L21 U12 as a complex GEMM on
+// v_mfma_f32_16x16x4f32 with FP32 accumulation (four real MFMA products per complex product).  The
+// workgroup of that update which owns diagonal block b + 1 goes on to factorise it -- LU plus the
+// inverses of both triangles, by one workgroup with the tile in registers (diag_factor) -- while
+// the rest of the update runs, so the panel factorisation overlaps the GEMM.  Without interchanges
+// the panel has no sequential pivot search, the step that makes partial-pivoting panels latency
+// bound on a GPU.  A zero or non-finite pivot is flagged; refinement then reports failure and the
 // one-shot solver falls back to the FP64 LU.
 //
 // Refinement (bem2d_lu32_refine): x0 = 0, r = b - A64 x in FP64, d = U^{-1} L^{-1} r in FP32
 // (two persistent block-triangular sweeps with the inverted diagonal blocks; blocks hand their
 // solution on through release/acquire flags), x += d in FP64, until the backward error
 // |b - A x|_inf / (|A|_inf |x|_inf + |b|_inf) <= tol for every right-hand side.
+//
+// Every kernel takes a by-value batch of up to kMaxLuBatch problems, so that several problems of
+// different sizes (each at its own panel step, or its own refinement step) advance in one launch
+// per kernel; the single-problem entry points are batches of one.
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
 
@@ -29,12 +35,44 @@
 namespace {
 
 constexpr int NB = 64;             // panel / block width
-constexpr int LDT = NB + 1;        // padded LDS row of a 64x64 complex tile (float2 units)
 constexpr int kMaxSolveRhs = 8;    // right-hand sides handled per triangular sweep
 constexpr int kResidChunks = 64;   // column chunks of the FP64 residual
 constexpr long long kSpinLimit = 1ll << 26;
 
 typedef float v4f __attribute__((ext_vector_type(4)));
+
+// ---- batched launches ----------------------------------------------------------------------------
+// Every factorisation and refinement kernel serves up to kMaxLuBatch problems per launch; the
+// descriptors travel by value in the kernel arguments and workgroup b belongs to the problem d with
+// start[d] <= b < start[d + 1].  A single problem is a batch of one.
+constexpr int kMaxLuBatch = 16;
+
+struct LuProblem {
+    float2* F;              // n x n FP32 factors (ld = n)
+    int64_t n;
+    int64_t k0;             // first row / column of the current diagonal block
+    int kb;                 // its size
+    int tiles;              // 64-wide tiles of the trailing matrix
+    const float2* Linv;     // inverses of the current diagonal block (panel solves)
+    const float2* UinvT;
+    float2* Linv_next;      // inverses to produce for the next diagonal block, or null
+    float2* UinvT_next;
+    int* bad;               // raised on a zero or non-finite pivot
+};
+
+struct LuBatch {
+    int count;
+    int64_t start[kMaxLuBatch + 1];
+    LuProblem p[kMaxLuBatch];
+};
+
+__device__ __forceinline__ int find_problem(const int64_t* start, int count, int64_t b)
+{
+    int d = 0;
+    while (d + 1 < count && b >= start[d + 1]) ++d;
+    return d;
+}
+
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b)
 {
@@ -45,11 +83,6 @@ __device__ __forceinline__ float2 cfms(float2 acc, float2 a, float2 b) // acc - 
     acc.x -= a.x * b.x - a.y * b.y;
     acc.y -= a.x * b.y + a.y * b.x;
     return acc;
-}
-__device__ __forceinline__ float2 crecip(float2 a)
-{
-    const float d = a.x * a.x + a.y * a.y;
-    return make_float2(a.x / d, -a.y / d);
 }
 
 // ---- cast ---------------------------------------------------------------------------------------
@@ -256,14 +289,12 @@ __device__ __forceinline__ void diag_factor(float2* __restrict__ A, int64_t ld, 
         }
 }
 
-__global__ __launch_bounds__(256) void diag_lu_kernel(float2* __restrict__ A, int64_t ld,
-                                                      int64_t k0, int kb,
-                                                      float2* __restrict__ Linv,
-                                                      float2* __restrict__ UinvT,
-                                                      int* __restrict__ bad)
+// Diagonal block p.k0 .. p.k0 + p.kb of problem blockIdx.x, inverses into Linv_next / UinvT_next.
+__global__ __launch_bounds__(256) void diag_lu_kernel(const LuBatch batch)
 {
     __shared__ DiagLds S;
-    diag_factor(A, ld, k0, kb, Linv, UinvT, bad, S);
+    const LuProblem& p = batch.p[blockIdx.x];
+    diag_factor(p.F, p.n, p.k0, p.kb, p.Linv_next, p.UinvT_next, p.bad, S);
 }
 
 // ---- 64 x 64 x 64 complex products on MFMA -----------------------------------------------------
@@ -329,14 +360,18 @@ __device__ __forceinline__ int acc_col(int q)
 }
 
 // ---- panel: L21 = A21 U11^{-1} and U12 = L11^{-1} A12 on MFMA ---------------------------------
-// blockIdx.x < nrow_tiles: a 64-row tile of the block column below the diagonal block;
+// Workgroup bid < tiles of a problem: a 64-row tile of the block column below the diagonal block;
 // otherwise a 64-column tile of the block row to its right.  In place, through LDS.
-__global__ __launch_bounds__(256) void panel_trsm_kernel(float2* __restrict__ A, int64_t ld,
-                                                         int64_t n, int64_t k0, int kb,
-                                                         int nrow_tiles,
-                                                         const float2* __restrict__ Linv,
-                                                         const float2* __restrict__ UinvT)
+__global__ __launch_bounds__(256) void panel_trsm_kernel(const LuBatch batch)
 {
+    const int di = find_problem(batch.start, batch.count, blockIdx.x);
+    const LuProblem& pb = batch.p[di];
+    float2* __restrict__ A = pb.F;
+    const int64_t ld = pb.n, n = pb.n, k0 = pb.k0;
+    const int kb = pb.kb, nrow_tiles = pb.tiles;
+    const float2* __restrict__ Linv = pb.Linv;
+    const float2* __restrict__ UinvT = pb.UinvT;
+    const int bid = (int)((int64_t)blockIdx.x - batch.start[di]);
     __shared__ __align__(16) float lds[kTileLdsFloats];
     float* Are = lds;
     float* Aim = lds + NB * KP;
@@ -344,8 +379,8 @@ __global__ __launch_bounds__(256) void panel_trsm_kernel(float2* __restrict__ A,
     float* Bim = lds + 3 * NB * KP;
     const int t = threadIdx.x;
     const int64_t rest0 = k0 + kb;
-    const bool rowtile = (int)blockIdx.x < nrow_tiles;
-    const int64_t base = rest0 + (int64_t)(rowtile ? blockIdx.x : blockIdx.x - nrow_tiles) * NB;
+    const bool rowtile = bid < nrow_tiles;
+    const int64_t base = rest0 + (int64_t)(rowtile ? bid : bid - nrow_tiles) * NB;
     const int len = (int)min((int64_t)NB, n - base);
     for (int e = t; e < NB * NB; e += 256) {
         const int x = e % NB, y = e / NB;
@@ -387,12 +422,15 @@ __global__ __launch_bounds__(256) void panel_trsm_kernel(float2* __restrict__ A,
 // C[i][j] -= sum_k L[i][k] U[k][j] for i, j in [rest0, n), k in [k0, k0 + kb); workgroup = one
 // 64 x 64 output tile.  Workgroup (0, 0) owns the next diagonal block: after its update it goes on
 // to factorise that block (when Linv_next is given), overlapping the rest of the update.
-__global__ __launch_bounds__(256) void trailing_update_kernel(float2* __restrict__ A, int64_t ld,
-                                                              int64_t n, int64_t k0, int kb,
-                                                              float2* __restrict__ Linv_next,
-                                                              float2* __restrict__ UinvT_next,
-                                                              int* __restrict__ bad)
+__global__ __launch_bounds__(256) void trailing_update_kernel(const LuBatch batch)
 {
+    const int di = find_problem(batch.start, batch.count, blockIdx.x);
+    const LuProblem& pb = batch.p[di];
+    float2* __restrict__ A = pb.F;
+    const int64_t ld = pb.n, n = pb.n, k0 = pb.k0;
+    const int kb = pb.kb;
+    const int bid = (int)((int64_t)blockIdx.x - batch.start[di]);
+    const int bx = bid % pb.tiles, by = bid / pb.tiles;
     __shared__ __align__(16) float lds[kTileLdsFloats];
     static_assert(sizeof(DiagLds) <= sizeof(float) * kTileLdsFloats, "LDS reuse");
     float* Are = lds;
@@ -401,8 +439,8 @@ __global__ __launch_bounds__(256) void trailing_update_kernel(float2* __restrict
     float* Bim = lds + 3 * NB * KP;
     const int t = threadIdx.x;
     const int64_t rest0 = k0 + kb;
-    const int64_t r0 = rest0 + (int64_t)blockIdx.x * NB;
-    const int64_t c0 = rest0 + (int64_t)blockIdx.y * NB;
+    const int64_t r0 = rest0 + (int64_t)bx * NB;
+    const int64_t c0 = rest0 + (int64_t)by * NB;
     for (int e = t; e < NB * NB; e += 256) {
         const int r = e % NB, k = e / NB;
         float2 v = make_float2(0.f, 0.f);
@@ -433,36 +471,77 @@ __global__ __launch_bounds__(256) void trailing_update_kernel(float2* __restrict
                     A[i + ld * j] = v;
                 }
             }
-    if (Linv_next && blockIdx.x == 0 && blockIdx.y == 0) {
+    if (pb.Linv_next && bid == 0) {
         __syncthreads(); // the tile's stores are visible to the workgroup; the LDS is free
-        diag_factor(A, ld, rest0, (int)min((int64_t)NB, n - rest0), Linv_next, UinvT_next, bad,
-                    *reinterpret_cast<DiagLds*>(lds));
+        diag_factor(A, ld, rest0, (int)min((int64_t)NB, n - rest0), pb.Linv_next, pb.UinvT_next,
+                    pb.bad, *reinterpret_cast<DiagLds*>(lds));
     }
 }
+
+// ---- refinement units ------------------------------------------------------------------------------
+// A unit is one problem's group of up to kMaxSolveRhs right-hand sides with its own workspace.
+struct RefUnit {
+    int64_t n;
+    int nrhs;                      // <= kMaxSolveRhs
+    int nblk;                      // 64-row blocks
+    int chunks;                    // column chunks of the residual (<= kResidChunks)
+    const double2* A;              // FP64 matrix
+    int64_t lda;
+    const double2* B;
+    int64_t ldb;
+    double2* X;
+    int64_t ldx;
+    double2* R;                    // n x nrhs residual (ld n)
+    double2* part;                 // chunks x nrhs x n partial products
+    float2* Y;                     // n x nrhs FP32 correction (ld n)
+    double* scale;                 // nrhs
+    unsigned long long* norms;     // kNormSlots: rmax | xmax | bmax | |A|_inf | pivot flag | time-out
+    const float2* F;               // FP32 factors (ld n)
+    const float2* Linv;
+    const float2* UinvT;
+    unsigned long long* flags;     // lower nblk | upper nblk
+    const double* anorm;           // |A|_inf of the factorised matrix
+    int* luflags;                  // [0] bad pivot (factorisation), [1] sweep time-out
+};
+constexpr int kNormSlots = 4 * kMaxSolveRhs;
+static_assert(kNormSlots == kRefNormSlots, "read-back slots per unit");
+constexpr int kNormA = 3 * kMaxSolveRhs, kNormBad = kNormA + 1, kNormTimeout = kNormA + 2;
+
+struct RefBatch {
+    int count;
+    unsigned long long epoch;
+    int64_t start[kMaxLuBatch + 1];
+    RefUnit u[kMaxLuBatch];
+};
 
 // ---- block-triangular sweeps -------------------------------------------------------------------
 // Y (n x nrhs, column-major, FP32) is solved in place.  Lower: unit-lower L from the factors with
 // inverted diagonal blocks Linv; block b waits for blocks < b.  Upper: blocks in reverse order, with
 // the transposed inverses UinvT.
-// Each workgroup owns one 64-row block; the first workgroups dispatched own the blocks without
-// dependencies, so every wait is on a block that is already running or done.  A wait gives up
-// after kSpinLimit polls and raises *err, so the kernel always terminates.
+// Each workgroup owns one 64-row block of one unit; a unit's workgroups are consecutive and the
+// first ones dispatched own the blocks without dependencies, so every wait is on a block that is
+// already running or done.  A wait gives up after kSpinLimit polls and raises the unit's time-out
+// flag, so the kernel always terminates.
 // Thread (row = t % 64, grp = t / 64) owns 16 consecutive k of each 64-wide column block: the
 // matrix block is loaded into registers before the flag wait (it does not depend on it), and the
 // partial sums stay in registers until the end, so a dependency costs one wait and one barrier.
 template <bool LOWER>
-__global__ __launch_bounds__(256) void block_trsv_kernel(const float2* __restrict__ F, int64_t ld,
-                                                         int64_t n, int nblk,
-                                                         const float2* __restrict__ Dinv,
-                                                         float2* __restrict__ Y, int64_t ldy,
-                                                         int nrhs, unsigned long long* flags,
-                                                         unsigned long long epoch,
-                                                         int* __restrict__ err)
+__global__ __launch_bounds__(256) void block_trsv_kernel(const RefBatch batch)
 {
     __shared__ float2 part[4][NB][kMaxSolveRhs];
     __shared__ float2 accs[NB][kMaxSolveRhs];
+    const int di = find_problem(batch.start, batch.count, blockIdx.x);
+    const RefUnit& U = batch.u[di];
+    const float2* __restrict__ F = U.F;
+    const int64_t ld = U.n, n = U.n;
+    const int nblk = U.nblk, nrhs = U.nrhs;
+    float2* __restrict__ Y = U.Y;
+    const int64_t ldy = U.n;
+    unsigned long long* flags = U.flags + (LOWER ? 0 : nblk);
+    const unsigned long long epoch = batch.epoch;
+    const int local = (int)((int64_t)blockIdx.x - batch.start[di]);
     const int t = threadIdx.x;
-    const int b = LOWER ? (int)blockIdx.x : nblk - 1 - (int)blockIdx.x;
+    const int b = LOWER ? local : nblk - 1 - local;
     const int64_t rb = (int64_t)b * NB;
     const int lenb = (int)min((int64_t)NB, n - rb);
     const int row = t % NB, grp = t / NB, k0 = 16 * grp;
@@ -473,7 +552,7 @@ __global__ __launch_bounds__(256) void block_trsv_kernel(const float2* __restric
     // the diagonal-block inverse, this thread's 16 columns of row `row` (the upper sweep's
     // inverses are stored transposed)
     float2 dv[16];
-    const float2* D = Dinv + (int64_t)b * NB * NB;
+    const float2* D = (LOWER ? U.Linv : U.UinvT) + (int64_t)b * NB * NB;
 #pragma unroll
     for (int q = 0; q < 16; ++q) dv[q] = LOWER ? D[row + NB * (k0 + q)] : D[(k0 + q) + NB * row];
     // Dependencies are consumed in the order they complete: ascending for the lower sweep,
@@ -492,7 +571,7 @@ __global__ __launch_bounds__(256) void block_trsv_kernel(const float2* __restric
             while (__hip_atomic_load(&flags[c], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
                 __builtin_amdgcn_s_sleep(1);
                 if (++spins > kSpinLimit) {
-                    atomicExch(err, 1);
+                    atomicExch(&U.luflags[1], 1);
                     break;
                 }
             }
@@ -561,16 +640,108 @@ __global__ __launch_bounds__(256) void block_trsv_kernel(const float2* __restric
     if (t == 0) __hip_atomic_store(&flags[b], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// ---- FP64 residual and updates -------------------------------------------------------------------
-// part[c][h][i] = sum_{j in chunk c} A[i][j] X[j][h]
-__global__ void residual_partial_kernel(int64_t n, const double2* __restrict__ A, int64_t lda,
-                                        const double2* __restrict__ X, int64_t ldx, int nrhs,
-                                        double2* __restrict__ part)
+// ---- FP64 residual, scaling and updates ------------------------------------------------------------
+// Row-parallel kernels: workgroup of 256 rows of one unit (start[] counts ceil(n / 256) per unit),
+// blockIdx.y = right-hand side h or residual chunk c.
+
+__device__ __forceinline__ void atomic_max_nonneg(unsigned long long* p, double v)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // the bit patterns of non-negative doubles are ordered like the values
+    atomicMax(p, (unsigned long long)__double_as_longlong(v));
+}
+
+__device__ __forceinline__ double wave_max(double m)
+{
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+    return m;
+}
+
+// X = 0, R = B, bmax = rmax = max_i |B_i| (the residual of x0 = 0).
+__global__ void ref_init_kernel(const RefBatch batch)
+{
+    const int di = find_problem(batch.start, batch.count, blockIdx.x);
+    const RefUnit& U = batch.u[di];
+    const int64_t i = ((int64_t)blockIdx.x - batch.start[di]) * blockDim.x + threadIdx.x;
+    const int h = blockIdx.y;
+    if (h >= U.nrhs) return;
+    double m = 0.0;
+    if (i < U.n) {
+        const double2 b = U.B[i + U.ldb * h];
+        U.X[i + U.ldx * h] = make_double2(0.0, 0.0);
+        U.R[i + U.n * h] = b;
+        m = hypot(b.x, b.y);
+        if (!(m == m)) m = INFINITY; // NaN poisons the norm
+    }
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) {
+        atomic_max_nonneg(&U.norms[2 * kMaxSolveRhs + h], m);
+        atomic_max_nonneg(&U.norms[h], m);
+    }
+}
+
+// scale[h] = 1 / max(rmax[h], tiny); then rmax = xmax = 0 for the coming step.
+__global__ void ref_scale_kernel(const RefBatch batch)
+{
+    const RefUnit& U = batch.u[blockIdx.x];
+    const int h = threadIdx.x;
+    if (h < U.nrhs) {
+        const double m = __longlong_as_double((long long)U.norms[h]);
+        U.scale[h] = (m > 1e-300 && m < INFINITY) ? 1.0 / m : 1.0;
+        U.norms[h] = 0ull;
+        U.norms[kMaxSolveRhs + h] = 0ull;
+    }
+}
+
+// Y (FP32) = R * scale[h]
+__global__ void ref_demote_kernel(const RefBatch batch)
+{
+    const int di = find_problem(batch.start, batch.count, blockIdx.x);
+    const RefUnit& U = batch.u[di];
+    const int64_t i = ((int64_t)blockIdx.x - batch.start[di]) * blockDim.x + threadIdx.x;
+    const int h = blockIdx.y;
+    if (h >= U.nrhs || i >= U.n) return;
+    const double s = U.scale[h];
+    const double2 r = U.R[i + U.n * h];
+    U.Y[i + U.n * h] = make_float2((float)(r.x * s), (float)(r.y * s));
+}
+
+// X += Y / scale[h]; xmax[h] = max_i |X_i|.
+__global__ void ref_update_kernel(const RefBatch batch)
+{
+    const int di = find_problem(batch.start, batch.count, blockIdx.x);
+    const RefUnit& U = batch.u[di];
+    const int64_t i = ((int64_t)blockIdx.x - batch.start[di]) * blockDim.x + threadIdx.x;
+    const int h = blockIdx.y;
+    if (h >= U.nrhs) return;
+    double m = 0.0;
+    if (i < U.n) {
+        const double s = 1.0 / U.scale[h];
+        const float2 y = U.Y[i + U.n * h];
+        double2 x = U.X[i + U.ldx * h];
+        x.x += (double)y.x * s;
+        x.y += (double)y.y * s;
+        U.X[i + U.ldx * h] = x;
+        m = hypot(x.x, x.y);
+        if (!(m == m)) m = INFINITY;
+    }
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) atomic_max_nonneg(&U.norms[kMaxSolveRhs + h], m);
+}
+
+// part[c][h][i] = sum_{j in chunk c} A[i][j] X[j][h]
+__global__ void ref_residual_partial_kernel(const RefBatch batch)
+{
+    const int di = find_problem(batch.start, batch.count, blockIdx.x);
+    const RefUnit& U = batch.u[di];
+    const int64_t n = U.n;
+    const int64_t i = ((int64_t)blockIdx.x - batch.start[di]) * blockDim.x + threadIdx.x;
     const int c = blockIdx.y;
-    if (i >= n) return;
-    const int64_t j0 = n * c / kResidChunks, j1 = n * (c + 1) / kResidChunks;
+    const int nrhs = U.nrhs;
+    if (i >= n || c >= U.chunks) return;
+    const double2* __restrict__ A = U.A;
+    const double2* __restrict__ X = U.X;
+    const int64_t lda = U.lda, ldx = U.ldx;
+    const int64_t j0 = n * c / U.chunks, j1 = n * (c + 1) / U.chunks;
     double2 s[kMaxSolveRhs];
 #pragma unroll
     for (int h = 0; h < kMaxSolveRhs; ++h) s[h] = make_double2(0.0, 0.0);
@@ -587,92 +758,57 @@ __global__ void residual_partial_kernel(int64_t n, const double2* __restrict__ A
     }
 #pragma unroll
     for (int h = 0; h < kMaxSolveRhs; ++h)
-        if (h < nrhs) part[((int64_t)c * nrhs + h) * n + i] = s[h];
+        if (h < nrhs) U.part[((int64_t)c * nrhs + h) * n + i] = s[h];
 }
 
-// R = B - sum_c part[c]; also per-column max |R_i| (atomic max on the bit pattern of a
-// non-negative double) into rmax[h].
-__global__ void residual_reduce_kernel(int64_t n, const double2* __restrict__ B, int64_t ldb,
-                                       int nrhs, const double2* __restrict__ part,
-                                       double2* __restrict__ R, unsigned long long* rmax)
+// R = B - sum_c part[c]; rmax[h] = max_i |R_i|.
+__global__ void ref_residual_reduce_kernel(const RefBatch batch)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int di = find_problem(batch.start, batch.count, blockIdx.x);
+    const RefUnit& U = batch.u[di];
+    const int64_t n = U.n;
+    const int64_t i = ((int64_t)blockIdx.x - batch.start[di]) * blockDim.x + threadIdx.x;
     const int h = blockIdx.y;
+    if (h >= U.nrhs) return;
     double m = 0.0;
     if (i < n) {
-        double2 s = B[i + ldb * h];
-        for (int c = 0; c < kResidChunks; ++c) {
-            const double2 p = part[((int64_t)c * nrhs + h) * n + i];
+        double2 s = U.B[i + U.ldb * h];
+        for (int c = 0; c < U.chunks; ++c) {
+            const double2 p = U.part[((int64_t)c * U.nrhs + h) * n + i];
             s.x -= p.x;
             s.y -= p.y;
         }
-        R[i + n * h] = s;
+        U.R[i + n * h] = s;
         m = hypot(s.x, s.y);
-        if (!(m == m)) m = INFINITY; // NaN poisons the norm
-    }
-    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
-    if ((threadIdx.x & 63) == 0) atomicMax(&rmax[h], (unsigned long long)__double_as_longlong(m));
-}
-
-// Y (FP32) = R * inv_scale[h]
-__global__ void demote_kernel(int64_t n, int nrhs, const double2* __restrict__ R,
-                              const double* __restrict__ scale, float2* __restrict__ Y)
-{
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int h = blockIdx.y;
-    if (i >= n) return;
-    const double s = scale[h];
-    const double2 r = R[i + n * h];
-    Y[i + n * h] = make_float2((float)(r.x * s), (float)(r.y * s));
-}
-
-// X += Y / inv_scale[h]; per-column max |X_i| into xmax[h].
-__global__ void update_kernel(int64_t n, int nrhs, const float2* __restrict__ Y,
-                              const double* __restrict__ scale, double2* __restrict__ X,
-                              int64_t ldx, unsigned long long* xmax)
-{
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int h = blockIdx.y;
-    double m = 0.0;
-    if (i < n) {
-        const double s = 1.0 / scale[h];
-        const float2 y = Y[i + n * h];
-        double2 x = X[i + ldx * h];
-        x.x += (double)y.x * s;
-        x.y += (double)y.y * s;
-        X[i + ldx * h] = x;
-        m = hypot(x.x, x.y);
         if (!(m == m)) m = INFINITY;
     }
-    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
-    if ((threadIdx.x & 63) == 0) atomicMax(&xmax[h], (unsigned long long)__double_as_longlong(m));
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) atomic_max_nonneg(&U.norms[h], m);
 }
 
-__global__ void zero_u64_kernel(unsigned long long* p, int n)
+// Copies |A|_inf and the factorisation's pivot / time-out flags into the norm slots, so that one
+// read-back carries everything the host needs.
+__global__ void ref_flags_kernel(const RefBatch batch)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0ull;
+    const RefUnit& U = batch.u[blockIdx.x];
+    if (threadIdx.x == 0) {
+        U.norms[kNormA] = (unsigned long long)__double_as_longlong(*U.anorm);
+        U.norms[kNormBad] = (unsigned long long)U.luflags[0];
+        U.norms[kNormTimeout] = (unsigned long long)U.luflags[1];
+    }
 }
 
-__global__ void zero_complex_kernel(int64_t n, int nrhs, double2* __restrict__ X, int64_t ldx)
+__global__ void zero_u64_kernel(unsigned long long* p, int64_t n)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) X[i + ldx * blockIdx.y] = make_double2(0.0, 0.0);
-}
-
-// scale[h] = 1 / max(rmax[h], tiny)
-__global__ void scale_kernel(int nrhs, const unsigned long long* __restrict__ rmax,
-                             double* __restrict__ scale)
-{
-    const int h = threadIdx.x;
-    if (h < nrhs) {
-        const double m = __longlong_as_double((long long)rmax[h]);
-        scale[h] = (m > 1e-300 && m < INFINITY) ? 1.0 / m : 1.0;
-    }
+    if (i < n) p[i] = 0ull;
 }
 
 } // namespace
 
+// ================================================================================================
+// host side
+// ================================================================================================
 struct MixedState {
 };
 
@@ -682,157 +818,499 @@ void bem2d_internal_mixed_release(bem2d_ctx* c)
     c->mixed = nullptr;
 }
 
+void* bem2d_internal_dalloc(DevArena* a, size_t bytes)
+{
+    if (bytes == 0) bytes = 256;
+    if (a) return a->alloc(bytes);
+    void* p = nullptr;
+    return hipMalloc(&p, bytes) == hipSuccess ? p : nullptr;
+}
+
+void bem2d_internal_dfree(DevArena* a, void* p)
+{
+    if (!p) return;
+    if (a)
+        a->release(p);
+    else
+        (void)hipFree(p);
+}
+
 struct bem2d_lu32 {
     bem2d_ctx* ctx = nullptr;
+    DevArena* arena = nullptr;
     int64_t n = 0;
     int nblk = 0;
+    int step = -1;                // next panel step; nblk - 1 when factorised; -1 before
     bool factored = false;
     float2* F = nullptr;          // n x n factors, ld = n
     float2* Linv = nullptr;       // nblk x 64 x 64
     float2* UinvT = nullptr;      // nblk x (U_bb^{-1})^T
-    unsigned long long* flags = nullptr; // 2 * nblk (lower, upper)
-    unsigned long long epoch = 0;
     int* dflag = nullptr;         // [0] bad pivot, [1] sweep time-out
     double* anorm = nullptr;      // |A|_inf
     double* rowpart = nullptr;    // 16 x n
-    // refinement workspace, grown on demand
-    int rhs_cap = 0;
-    double2* R = nullptr;         // n x rhs
-    double2* part = nullptr;      // chunks x rhs x n
-    float2* Y = nullptr;          // n x rhs
-    double* scale = nullptr;      // rhs
-    unsigned long long* norms = nullptr; // rmax[rhs], xmax[rhs], bmax[rhs]
 };
 
-static void lu32_free_ws(bem2d_lu32* f)
+size_t bem2d_internal_lu32_bytes(int64_t n)
 {
-    (void)hipFree(f->R);
-    (void)hipFree(f->part);
-    (void)hipFree(f->Y);
-    (void)hipFree(f->scale);
-    (void)hipFree(f->norms);
-    f->R = nullptr;
-    f->part = nullptr;
-    f->Y = nullptr;
-    f->scale = nullptr;
-    f->norms = nullptr;
-    f->rhs_cap = 0;
+    const int64_t nblk = (n + NB - 1) / NB;
+    return sizeof(float2) * (size_t)(n * n + 2 * nblk * NB * NB) + sizeof(double) * (16 * n + 1) +
+           8 * 256;
 }
 
 extern "C" void bem2d_lu32_destroy(bem2d_lu32* f)
 {
     if (!f) return;
     (void)hipSetDevice(f->ctx->device);
-    (void)hipStreamSynchronize(f->ctx->stream);
-    (void)hipFree(f->F);
-    (void)hipFree(f->Linv);
-    (void)hipFree(f->UinvT);
-    (void)hipFree(f->flags);
-    (void)hipFree(f->dflag);
-    (void)hipFree(f->anorm);
-    (void)hipFree(f->rowpart);
-    lu32_free_ws(f);
+    if (!f->arena) (void)hipStreamSynchronize(f->ctx->stream);
+    bem2d_internal_dfree(f->arena, f->F);
+    bem2d_internal_dfree(f->arena, f->Linv);
+    bem2d_internal_dfree(f->arena, f->UinvT);
+    bem2d_internal_dfree(f->arena, f->dflag);
+    bem2d_internal_dfree(f->arena, f->anorm);
+    bem2d_internal_dfree(f->arena, f->rowpart);
     delete f;
 }
 
-extern "C" int bem2d_lu32_create(bem2d_ctx* c, int64_t n, bem2d_lu32** out)
+int bem2d_internal_lu32_create(bem2d_ctx* c, int64_t n, DevArena* arena, bem2d_lu32** out)
 {
     if (!c || !out || n < 1) return FAIL(BEM2D_ERR_ARG, "bem2d_lu32_create: bad argument");
     HIPCHK(hipSetDevice(c->device));
     bem2d_lu32* f = new bem2d_lu32();
     f->ctx = c;
+    f->arena = arena;
     f->n = n;
     f->nblk = (int)((n + NB - 1) / NB);
     const size_t inv_bytes = sizeof(float2) * (size_t)f->nblk * NB * NB;
-    if (hipMalloc(&f->F, sizeof(float2) * n * n) != hipSuccess ||
-        hipMalloc(&f->Linv, inv_bytes) != hipSuccess || hipMalloc(&f->UinvT, inv_bytes) != hipSuccess ||
-        hipMalloc(&f->flags, sizeof(unsigned long long) * 2 * f->nblk) != hipSuccess ||
-        hipMalloc(&f->dflag, sizeof(int) * 2) != hipSuccess ||
-        hipMalloc(&f->anorm, sizeof(double)) != hipSuccess ||
-        hipMalloc(&f->rowpart, sizeof(double) * 16 * n) != hipSuccess) {
+    f->F = (float2*)bem2d_internal_dalloc(arena, sizeof(float2) * n * n);
+    f->Linv = (float2*)bem2d_internal_dalloc(arena, inv_bytes);
+    f->UinvT = (float2*)bem2d_internal_dalloc(arena, inv_bytes);
+    f->dflag = (int*)bem2d_internal_dalloc(arena, sizeof(int) * 2);
+    f->anorm = (double*)bem2d_internal_dalloc(arena, sizeof(double));
+    f->rowpart = (double*)bem2d_internal_dalloc(arena, sizeof(double) * 16 * n);
+    if (!f->F || !f->Linv || !f->UinvT || !f->dflag || !f->anorm || !f->rowpart) {
         bem2d_lu32_destroy(f);
         return FAIL(BEM2D_ERR_NOMEM, "bem2d_lu32_create: device allocation of %lld bytes failed",
                     (long long)(8 * n * n));
     }
-    zero_u64_kernel<<<(2 * f->nblk + 255) / 256, 256, 0, c->stream>>>(f->flags, 2 * f->nblk);
-    HIPCHK(hipGetLastError());
     *out = f;
     return BEM2D_OK;
 }
 
+extern "C" int bem2d_lu32_create(bem2d_ctx* c, int64_t n, bem2d_lu32** out)
+{
+    return bem2d_internal_lu32_create(c, n, nullptr, out);
+}
+
 extern "C" int64_t bem2d_lu32_size(const bem2d_lu32* f) { return f ? f->n : -1; }
+
+static LuProblem lu_problem(bem2d_lu32* f, int b)
+{
+    LuProblem p{};
+    p.F = f->F;
+    p.n = f->n;
+    p.k0 = (int64_t)b * NB;
+    p.kb = (int)std::min<int64_t>(NB, f->n - p.k0);
+    const int64_t rest = f->n - p.k0 - p.kb;
+    p.tiles = (int)((rest + NB - 1) / NB);
+    p.Linv = f->Linv + (size_t)b * NB * NB;
+    p.UinvT = f->UinvT + (size_t)b * NB * NB;
+    p.bad = f->dflag;
+    return p;
+}
+
+// Cast, |A|_inf and the factorisation of diagonal block 0 for count problems.
+int bem2d_internal_lu_begin(hipStream_t s, bem2d_lu32* const* lus, const void* const* A64,
+                            const int64_t* lda, int count)
+{
+    for (int d = 0; d < count; ++d) {
+        bem2d_lu32* f = lus[d];
+        const int64_t n = f->n;
+        const double2* A = (const double2*)A64[d];
+        for (int64_t j0 = 0; j0 < n; j0 += 65535) { // grid.y is limited to 65535
+            const int64_t nj = std::min<int64_t>(65535, n - j0);
+            dim3 g((unsigned)((n + 255) / 256), (unsigned)nj);
+            cast_kernel<<<g, 256, 0, s>>>(n, A + lda[d] * j0, lda[d], f->F + n * j0, n);
+        }
+        dim3 g2((unsigned)((n + 255) / 256), 16);
+        rowsum_kernel<<<g2, 256, 0, s>>>(n, A, lda[d], 16, f->rowpart);
+        rowsum_max_kernel<<<1, 256, 0, s>>>(n, 16, f->rowpart, f->anorm);
+        zero_u64_kernel<<<1, 64, 0, s>>>((unsigned long long*)f->dflag, 1);
+        f->step = 0;
+        f->factored = false;
+    }
+    for (int c0 = 0; c0 < count; c0 += kMaxLuBatch) {
+        LuBatch bt{};
+        bt.count = std::min(kMaxLuBatch, count - c0);
+        for (int d = 0; d < bt.count; ++d) {
+            bem2d_lu32* f = lus[c0 + d];
+            LuProblem p = lu_problem(f, 0);
+            p.Linv_next = f->Linv;
+            p.UinvT_next = f->UinvT;
+            bt.p[d] = p;
+            bt.start[d + 1] = d + 1;
+        }
+        diag_lu_kernel<<<bt.count, 256, 0, s>>>(bt);
+    }
+    if (hipGetLastError() != hipSuccess) return FAIL(BEM2D_ERR_DEVICE, "LU launch failed");
+    for (int d = 0; d < count; ++d)
+        if (lus[d]->nblk == 1) lus[d]->factored = true;
+    return BEM2D_OK;
+}
+
+// One panel step for each of count problems (each at its own step): the panel solves of block b,
+// then the trailing update, whose first workgroup also factorises diagonal block b + 1.
+int bem2d_internal_lu_advance(hipStream_t s, bem2d_lu32* const* lus, int count)
+{
+    std::vector<bem2d_lu32*> act;
+    for (int d = 0; d < count; ++d)
+        if (lus[d]->step >= 0 && lus[d]->step < lus[d]->nblk - 1) act.push_back(lus[d]);
+    for (size_t c0 = 0; c0 < act.size(); c0 += kMaxLuBatch) {
+        LuBatch tr{}, up{};
+        const int cnt = (int)std::min<size_t>(kMaxLuBatch, act.size() - c0);
+        tr.count = up.count = cnt;
+        for (int d = 0; d < cnt; ++d) {
+            bem2d_lu32* f = act[c0 + d];
+            const int b = f->step;
+            LuProblem p = lu_problem(f, b);
+            p.Linv_next = f->Linv + (size_t)(b + 1) * NB * NB;
+            p.UinvT_next = f->UinvT + (size_t)(b + 1) * NB * NB;
+            tr.p[d] = up.p[d] = p;
+            tr.start[d + 1] = tr.start[d] + 2 * p.tiles;
+            up.start[d + 1] = up.start[d] + (int64_t)p.tiles * p.tiles;
+        }
+        if (up.start[cnt] > (int64_t)INT32_MAX) return FAIL(BEM2D_ERR_UNSUPPORTED, "LU grid too large");
+        panel_trsm_kernel<<<(unsigned)tr.start[cnt], 256, 0, s>>>(tr);
+        trailing_update_kernel<<<(unsigned)up.start[cnt], 256, 0, s>>>(up);
+    }
+    if (hipGetLastError() != hipSuccess) return FAIL(BEM2D_ERR_DEVICE, "LU launch failed");
+    for (bem2d_lu32* f : act) {
+        f->step++;
+        if (f->step == f->nblk - 1) f->factored = true;
+    }
+    return BEM2D_OK;
+}
+
+bool bem2d_internal_lu_done(const bem2d_lu32* f) { return f->factored; }
+int64_t bem2d_internal_lu_n(const bem2d_lu32* f) { return f->n; }
 
 extern "C" int bem2d_lu32_factor(bem2d_ctx* c, bem2d_lu32* f, const void* A64, int64_t lda)
 {
     if (!c || !f || !A64 || lda < f->n) return FAIL(BEM2D_ERR_ARG, "bem2d_lu32_factor: bad argument");
     HIPCHK(hipSetDevice(c->device));
     hipStream_t s = c->stream;
-    const int64_t n = f->n;
-    const double2* A = (const double2*)A64;
     bem2d_internal_prof_begin(c, BEM2D_PROF_LU);
-    {
-        dim3 g((unsigned)((n + 255) / 256), (unsigned)n);
-        cast_kernel<<<g, 256, 0, s>>>(n, A, lda, f->F, n);
-        dim3 g2((unsigned)((n + 255) / 256), 16);
-        rowsum_kernel<<<g2, 256, 0, s>>>(n, A, lda, 16, f->rowpart);
-        rowsum_max_kernel<<<1, 256, 0, s>>>(n, 16, f->rowpart, f->anorm);
-        zero_u64_kernel<<<1, 64, 0, s>>>((unsigned long long*)f->dflag, 1);
-    }
-    // Step b: panel solves of block b, then the trailing update, whose workgroup (0, 0) also
-    // factorises diagonal block b + 1 as soon as its own tile is updated.
-    diag_lu_kernel<<<1, 256, 0, s>>>(f->F, n, 0, (int)std::min<int64_t>(NB, n), f->Linv, f->UinvT,
-                                     f->dflag);
-    for (int b = 0; b < f->nblk; ++b) {
-        const int64_t k0 = (int64_t)b * NB;
-        const int kb = (int)std::min<int64_t>(NB, n - k0);
-        const int64_t rest = n - k0 - kb;
-        if (rest > 0) {
-            const int tiles = (int)((rest + NB - 1) / NB);
-            panel_trsm_kernel<<<2 * tiles, 256, 0, s>>>(f->F, n, n, k0, kb, tiles,
-                                                        f->Linv + (size_t)b * NB * NB,
-                                                        f->UinvT + (size_t)b * NB * NB);
-            dim3 g((unsigned)tiles, (unsigned)tiles);
-            trailing_update_kernel<<<g, 256, 0, s>>>(f->F, n, n, k0, kb,
-                                                     f->Linv + (size_t)(b + 1) * NB * NB,
-                                                     f->UinvT + (size_t)(b + 1) * NB * NB,
-                                                     f->dflag);
+    int rc = bem2d_internal_lu_begin(s, &f, &A64, &lda, 1);
+    while (!rc && !f->factored) rc = bem2d_internal_lu_advance(s, &f, 1);
+    bem2d_internal_prof_end(c, BEM2D_PROF_LU);
+    return rc;
+}
+
+// ---- refinement jobs --------------------------------------------------------------------------
+// A job refines x = A^{-1} b for several problems at once: every problem's right-hand sides are
+// split into units of <= kMaxSolveRhs columns, and each step is one batched launch per kernel over
+// the units still running.  Progress is asynchronous: step() enqueues a step and an asynchronous
+// copy of every unit's norms into pinned host memory; ready() polls the step's event; collect()
+// reads the norms and retires the units that met the tolerance.
+struct RefJob {
+    bem2d_ctx* ctx = nullptr;
+    DevArena* arena = nullptr;
+    std::vector<RefUnit> units;
+    std::vector<int> unit_spec;           // spec index of each unit
+    std::vector<char> active;             // unit still refining
+    std::vector<int> iters;               // steps taken per unit
+    std::vector<double> omega;            // backward error per unit
+    std::vector<double> bnorm;            // |b_h|_inf per unit and column (kMaxSolveRhs each)
+    std::vector<double> tol;              // backward-error target per unit
+    std::vector<int> maxit;               // step limit per unit
+    bool own_pinned = true;
+    std::vector<void*> blocks;            // device allocations of the workspace
+    unsigned long long* dnorms = nullptr; // units x kNormSlots
+    unsigned long long* hnorms = nullptr; // pinned mirror
+    unsigned long long epoch = 0;
+    hipEvent_t ev = nullptr;
+    bool pending = false;
+    bool started = false;
+    bool last_init = false;               // the pending read-back is the initial residual
+    int nspec = 0;
+};
+
+static int ref_chunks(int64_t n)
+{
+    const int64_t rows = (n + 255) / 256;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(kResidChunks, (2048 + rows - 1) / rows));
+}
+
+size_t bem2d_internal_ref_bytes(int64_t n, int nrhs)
+{
+    const int64_t units = (nrhs + kMaxSolveRhs - 1) / kMaxSolveRhs;
+    const int64_t nblk = (n + NB - 1) / NB;
+    const size_t per = sizeof(double2) * n * kMaxSolveRhs * (1 + ref_chunks(n)) +
+                       sizeof(float2) * n * kMaxSolveRhs + sizeof(double) * kMaxSolveRhs +
+                       sizeof(unsigned long long) * (2 * nblk + kNormSlots) + 6 * 256;
+    return units * per;
+}
+
+void bem2d_internal_ref_destroy(RefJob* j)
+{
+    if (!j) return;
+    if (j->ev) (void)hipEventDestroy(j->ev);
+    if (j->hnorms && j->own_pinned) (void)hipHostFree(j->hnorms);
+    for (void* p : j->blocks) bem2d_internal_dfree(j->arena, p);
+    delete j;
+}
+
+int bem2d_internal_ref_units(int nrhs) { return (nrhs + kMaxSolveRhs - 1) / kMaxSolveRhs; }
+
+int bem2d_internal_ref_create(bem2d_ctx* c, DevArena* arena, const RefSpec* specs, int count,
+                              unsigned long long* pinned, RefJob** out)
+{
+    RefJob* j = new RefJob();
+    j->ctx = c;
+    j->arena = arena;
+    j->nspec = count;
+    auto fail = [&](int code, const char* what) {
+        bem2d_internal_ref_destroy(j);
+        return FAIL(code, "refinement: %s", what);
+    };
+    for (int sidx = 0; sidx < count; ++sidx) {
+        const RefSpec& sp = specs[sidx];
+        bem2d_lu32* f = sp.lu;
+        const int64_t n = f->n;
+        if (sp.lda < n || sp.ldb < n || sp.ldx < n || sp.nrhs < 1)
+            return fail(BEM2D_ERR_ARG, "bad leading dimension or column count");
+        for (int h0 = 0; h0 < sp.nrhs; h0 += kMaxSolveRhs) {
+            RefUnit u{};
+            u.n = n;
+            u.nrhs = std::min(kMaxSolveRhs, sp.nrhs - h0);
+            u.nblk = f->nblk;
+            u.chunks = ref_chunks(n);
+            u.A = (const double2*)sp.A64;
+            u.lda = sp.lda;
+            u.B = (const double2*)sp.B + sp.ldb * h0;
+            u.ldb = sp.ldb;
+            u.X = (double2*)sp.X + sp.ldx * h0;
+            u.ldx = sp.ldx;
+            u.F = f->F;
+            u.Linv = f->Linv;
+            u.UinvT = f->UinvT;
+            u.anorm = f->anorm;
+            u.luflags = f->dflag;
+            const size_t bytes = sizeof(double2) * n * kMaxSolveRhs * (1 + u.chunks) +
+                                 sizeof(float2) * n * kMaxSolveRhs +
+                                 sizeof(double) * kMaxSolveRhs +
+                                 sizeof(unsigned long long) * 2 * u.nblk + 4 * 256;
+            char* base = (char*)bem2d_internal_dalloc(sp.ws ? sp.ws : arena, bytes);
+            if (!base) return fail(BEM2D_ERR_NOMEM, "workspace allocation failed");
+            if (!sp.ws) j->blocks.push_back(base); // a spec's own arena is released by its owner
+            auto carve = [&](size_t b) {
+                char* p = base;
+                base += (b + 255) & ~(size_t)255;
+                return p;
+            };
+            u.R = (double2*)carve(sizeof(double2) * n * kMaxSolveRhs);
+            u.part = (double2*)carve(sizeof(double2) * n * kMaxSolveRhs * u.chunks);
+            u.Y = (float2*)carve(sizeof(float2) * n * kMaxSolveRhs);
+            u.scale = (double*)carve(sizeof(double) * kMaxSolveRhs);
+            u.flags = (unsigned long long*)carve(sizeof(unsigned long long) * 2 * u.nblk);
+            j->units.push_back(u);
+            j->unit_spec.push_back(sidx);
+            j->tol.push_back(sp.tol);
+            j->maxit.push_back(sp.maxit);
         }
     }
-    bem2d_internal_prof_end(c, BEM2D_PROF_LU);
-    HIPCHK(hipGetLastError());
-    f->factored = true;
-    return BEM2D_OK;
-}
-
-static int lu32_reserve(bem2d_lu32* f, int nrhs)
-{
-    if (nrhs <= f->rhs_cap) return BEM2D_OK;
-    (void)hipStreamSynchronize(f->ctx->stream);
-    lu32_free_ws(f);
-    const int64_t n = f->n;
-    if (hipMalloc(&f->R, sizeof(double2) * n * nrhs) != hipSuccess ||
-        hipMalloc(&f->part, sizeof(double2) * kResidChunks * nrhs * n) != hipSuccess ||
-        hipMalloc(&f->Y, sizeof(float2) * n * nrhs) != hipSuccess ||
-        hipMalloc(&f->scale, sizeof(double) * nrhs) != hipSuccess ||
-        hipMalloc(&f->norms, sizeof(unsigned long long) * 3 * nrhs) != hipSuccess) {
-        lu32_free_ws(f);
-        return FAIL(BEM2D_ERR_NOMEM, "refinement workspace allocation failed");
+    const size_t nu = j->units.size();
+    j->dnorms = (unsigned long long*)bem2d_internal_dalloc(arena, sizeof(unsigned long long) *
+                                                                      kNormSlots * nu);
+    if (!j->dnorms) return fail(BEM2D_ERR_NOMEM, "workspace allocation failed");
+    j->blocks.push_back(j->dnorms);
+    for (size_t u = 0; u < nu; ++u) j->units[u].norms = j->dnorms + kNormSlots * u;
+    if (pinned) {
+        j->hnorms = pinned;
+        j->own_pinned = false;
+    } else if (hipHostMalloc((void**)&j->hnorms, sizeof(unsigned long long) * kNormSlots * nu) !=
+               hipSuccess) {
+        j->hnorms = nullptr;
+        return fail(BEM2D_ERR_NOMEM, "pinned buffer allocation failed");
     }
-    f->rhs_cap = nrhs;
+    if (hipEventCreateWithFlags(&j->ev, hipEventDisableTiming) != hipSuccess)
+        return fail(BEM2D_ERR_NOMEM, "event creation failed");
+    j->active.assign(nu, 1);
+    j->iters.assign(nu, 0);
+    j->omega.assign(nu, INFINITY);
+    j->bnorm.assign(nu * kMaxSolveRhs, 0.0);
+    *out = j;
     return BEM2D_OK;
 }
 
-// FP32 solve of L U Y = Y in place for up to kMaxSolveRhs columns (ld = n).
-static int lu32_apply(bem2d_lu32* f, float2* Y, int nrhs)
+// Row-parallel grid over the listed units (start[] in workgroups of 256 rows).
+static RefBatch ref_batch(const RefJob* j, const std::vector<int>& ids, size_t c0, int per_unit)
 {
-    hipStream_t s = f->ctx->stream;
-    f->epoch++;
-    block_trsv_kernel<true><<<f->nblk, 256, 0, s>>>(f->F, f->n, f->n, f->nblk, f->Linv, Y, f->n,
-                                                     nrhs, f->flags, f->epoch, f->dflag + 1);
-    block_trsv_kernel<false><<<f->nblk, 256, 0, s>>>(f->F, f->n, f->n, f->nblk, f->UinvT, Y, f->n,
-                                                      nrhs, f->flags + f->nblk, f->epoch,
-                                                      f->dflag + 1);
-    return hipGetLastError() == hipSuccess ? BEM2D_OK : BEM2D_ERR_DEVICE;
+    RefBatch bt{};
+    bt.count = (int)std::min<size_t>(kMaxLuBatch, ids.size() - c0);
+    bt.epoch = j->epoch;
+    for (int d = 0; d < bt.count; ++d) {
+        const RefUnit& u = j->units[ids[c0 + d]];
+        bt.u[d] = u;
+        int64_t w = per_unit;
+        if (per_unit == 0) w = (u.n + 255) / 256;   // rows
+        if (per_unit == -1) w = u.nblk;              // sweep blocks
+        bt.start[d + 1] = bt.start[d] + w;
+    }
+    return bt;
+}
+
+// Enqueues one refinement step (or the initial residual when not yet started) for every active
+// unit, then the asynchronous copy of the norms and the job's event.
+int bem2d_internal_ref_step(RefJob* j, hipStream_t s)
+{
+    std::vector<int> ids;
+    for (size_t u = 0; u < j->units.size(); ++u)
+        if (j->active[u]) ids.push_back((int)u);
+    if (!j->started) {
+        const size_t nu = j->units.size();
+        zero_u64_kernel<<<(unsigned)((kNormSlots * nu + 255) / 256), 256, 0, s>>>(
+            j->dnorms, (int64_t)(kNormSlots * nu));
+        for (size_t u = 0; u < nu; ++u)
+            zero_u64_kernel<<<(unsigned)((2 * j->units[u].nblk + 255) / 256), 256, 0, s>>>(
+                j->units[u].flags, 2 * j->units[u].nblk);
+        for (size_t c0 = 0; c0 < ids.size(); c0 += kMaxLuBatch) {
+            RefBatch bt = ref_batch(j, ids, c0, 0);
+            ref_init_kernel<<<dim3((unsigned)bt.start[bt.count], kMaxSolveRhs), 256, 0, s>>>(bt);
+            RefBatch one = ref_batch(j, ids, c0, 1);
+            ref_flags_kernel<<<one.count, 64, 0, s>>>(one);
+        }
+        j->started = true;
+        j->last_init = true;
+    } else {
+        j->last_init = false;
+        j->epoch++;
+        for (size_t c0 = 0; c0 < ids.size(); c0 += kMaxLuBatch) {
+            RefBatch one = ref_batch(j, ids, c0, 1);
+            RefBatch rows = ref_batch(j, ids, c0, 0);
+            RefBatch sweep = ref_batch(j, ids, c0, -1);
+            RefBatch chunks = rows;
+            const unsigned rgrid = (unsigned)rows.start[rows.count];
+            ref_scale_kernel<<<one.count, 64, 0, s>>>(one);
+            ref_demote_kernel<<<dim3(rgrid, kMaxSolveRhs), 256, 0, s>>>(rows);
+            block_trsv_kernel<true><<<(unsigned)sweep.start[sweep.count], 256, 0, s>>>(sweep);
+            block_trsv_kernel<false><<<(unsigned)sweep.start[sweep.count], 256, 0, s>>>(sweep);
+            ref_update_kernel<<<dim3(rgrid, kMaxSolveRhs), 256, 0, s>>>(rows);
+            ref_residual_partial_kernel<<<dim3(rgrid, kResidChunks), 256, 0, s>>>(chunks);
+            ref_residual_reduce_kernel<<<dim3(rgrid, kMaxSolveRhs), 256, 0, s>>>(rows);
+            ref_flags_kernel<<<one.count, 64, 0, s>>>(one);
+        }
+    }
+    if (hipMemcpyAsync(j->hnorms, j->dnorms, sizeof(unsigned long long) * kNormSlots * j->units.size(),
+                       hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipEventRecord(j->ev, s) != hipSuccess || hipGetLastError() != hipSuccess)
+        return FAIL(BEM2D_ERR_DEVICE, "refinement launch failed");
+    j->pending = true;
+    return BEM2D_OK;
+}
+
+bool bem2d_internal_ref_ready(RefJob* j)
+{
+    return !j->pending || hipEventQuery(j->ev) == hipSuccess;
+}
+
+static double u64_as_double(unsigned long long v)
+{
+    double d;
+    std::memcpy(&d, &v, sizeof d);
+    return d;
+}
+
+// Reads the norms of the last step (waits for it) and retires the units that met tol (already
+// at x = 0 when b = 0), that failed (a bad pivot or a sweep time-out) or that used maxit steps.
+// Returns BEM2D_ERR_DEVICE on a sweep time-out.
+int bem2d_internal_ref_collect(RefJob* j, bool* all_done)
+{
+    if (j->pending) {
+        if (hipEventSynchronize(j->ev) != hipSuccess)
+            return FAIL(BEM2D_ERR_DEVICE, "refinement step failed");
+        j->pending = false;
+    }
+    int rc = BEM2D_OK;
+    bool done = true;
+    for (size_t u = 0; u < j->units.size(); ++u) {
+        if (!j->active[u]) continue;
+        const unsigned long long* nm = j->hnorms + kNormSlots * u;
+        const RefUnit& U = j->units[u];
+        if (nm[kNormTimeout] || nm[kNormBad]) {
+            j->active[u] = 0;
+            j->omega[u] = INFINITY;
+            if (nm[kNormTimeout]) rc = BEM2D_ERR_DEVICE;
+            continue;
+        }
+        if (j->last_init) {
+            for (int h = 0; h < U.nrhs; ++h)
+                j->bnorm[kMaxSolveRhs * u + h] = u64_as_double(nm[2 * kMaxSolveRhs + h]);
+        } else {
+            j->iters[u]++;
+        }
+        const double anorm = u64_as_double(nm[kNormA]);
+        double w = 0.0;
+        for (int h = 0; h < U.nrhs; ++h) {
+            const double rn = u64_as_double(nm[h]);
+            const double xn = j->last_init ? 0.0 : u64_as_double(nm[kMaxSolveRhs + h]);
+            const double den = anorm * xn + j->bnorm[kMaxSolveRhs * u + h];
+            const double wh = den > 0 ? rn / den : (rn > 0 ? INFINITY : 0.0);
+            w = std::max(w, std::isfinite(wh) ? wh : INFINITY);
+        }
+        j->omega[u] = w;
+        if (w <= j->tol[u] || j->iters[u] >= j->maxit[u]) j->active[u] = 0;
+        if (j->active[u]) done = false;
+    }
+    *all_done = done;
+    return rc;
+}
+
+bool bem2d_internal_ref_spec_active(const RefJob* j, int sidx)
+{
+    for (size_t u = 0; u < j->units.size(); ++u)
+        if (j->unit_spec[u] == sidx && j->active[u]) return true;
+    return false;
+}
+
+// Report of spec sidx: worst backward error and most steps over its units.
+void bem2d_internal_ref_report(const RefJob* j, int sidx, bem2d_ir_report* r)
+{
+    double tol = INFINITY;
+    double worst = 0.0;
+    int it = 0;
+    double anorm = 0.0;
+    bool any = false;
+    for (size_t u = 0; u < j->units.size(); ++u) {
+        if (j->unit_spec[u] != sidx) continue;
+        worst = any ? std::max(worst, j->omega[u]) : j->omega[u];
+        tol = std::min(tol, j->tol[u]);
+        it = std::max(it, j->iters[u]);
+        anorm = u64_as_double(j->hnorms[kNormSlots * u + kNormA]);
+        any = true;
+    }
+    r->iterations = it;
+    r->backward_error = worst;
+    r->converged = worst <= tol;
+    r->matrix_norm_inf = anorm;
+}
+
+bool bem2d_internal_ref_bad_pivot(const RefJob* j, int sidx)
+{
+    for (size_t u = 0; u < j->units.size(); ++u)
+        if (j->unit_spec[u] == sidx && j->hnorms[kNormSlots * u + kNormBad]) return true;
+    return false;
+}
+
+// Synchronous refinement of count problems (the engine drives the same steps asynchronously).
+int bem2d_internal_refine_sync(hipStream_t s, RefJob* j)
+{
+    bool done = false;
+    int rc = BEM2D_OK;
+    while (!rc && !done) {
+        rc = bem2d_internal_ref_step(j, s); // the initial residual first, then correction steps
+        if (!rc) rc = bem2d_internal_ref_collect(j, &done);
+    }
+    return rc;
 }
 
 extern "C" int bem2d_lu32_refine(bem2d_ctx* c, bem2d_lu32* f, const void* A64, int64_t lda,
@@ -844,92 +1322,29 @@ extern "C" int bem2d_lu32_refine(bem2d_ctx* c, bem2d_lu32* f, const void* A64, i
         return FAIL(BEM2D_ERR_ARG, "bem2d_lu32_refine: bad argument");
     if (!f->factored) return FAIL(BEM2D_ERR_ARG, "bem2d_lu32_refine: not factored");
     HIPCHK(hipSetDevice(c->device));
-    hipStream_t s = c->stream;
-    const int64_t n = f->n;
+    RefSpec sp{f, A64, lda, B, ldb, X, ldx, nrhs, tol, maxit, nullptr};
+    RefJob* j = nullptr;
+    int rc = bem2d_internal_ref_create(c, nullptr, &sp, 1, nullptr, &j);
+    if (rc) return rc;
+    rc = bem2d_internal_refine_sync(c->stream, j);
     bem2d_ir_report r{};
-    double worst = INFINITY;
-    for (int h0 = 0; h0 < nrhs; h0 += kMaxSolveRhs) {
-        const int nr = std::min(kMaxSolveRhs, nrhs - h0);
-        int rc = lu32_reserve(f, nr);
-        if (rc) return rc;
-        const double2* Bh = (const double2*)B + ldb * h0;
-        double2* Xh = (double2*)X + ldx * h0;
-        unsigned long long* rmax = f->norms;
-        unsigned long long* xmax = f->norms + nr;
-        unsigned long long* bmax = f->norms + 2 * nr;
-        dim3 gv((unsigned)((n + 255) / 256), (unsigned)nr);
-        zero_complex_kernel<<<gv, 256, 0, s>>>(n, nr, Xh, ldx);
-        // |b|_inf via the residual of x = 0
-        zero_u64_kernel<<<1, 64, 0, s>>>(f->norms, 3 * nr);
-        dim3 gp((unsigned)((n + 255) / 256), kResidChunks);
-        residual_partial_kernel<<<gp, 256, 0, s>>>(n, (const double2*)A64, lda, Xh, ldx, nr, f->part);
-        residual_reduce_kernel<<<gv, 256, 0, s>>>(n, Bh, ldb, nr, f->part, f->R, bmax);
-        std::vector<unsigned long long> hb(3 * nr);
-        double anorm = 0.0;
-        int dflag[2] = {0, 0};
-        HIPCHK(hipMemcpyAsync(&anorm, f->anorm, sizeof(double), hipMemcpyDeviceToHost, s));
-        HIPCHK(hipMemcpyAsync(dflag, f->dflag, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        if (dflag[0]) {
-            r.iterations = 0;
-            r.converged = 0;
-            r.backward_error = INFINITY;
-            r.matrix_norm_inf = anorm;
-            if (rep) *rep = r;
-            return FAIL(BEM2D_ERR_NOT_CONVERGED, "FP32 LU met a zero or non-finite pivot");
-        }
-        HIPCHK(hipMemcpyAsync(hb.data(), f->norms, sizeof(unsigned long long) * 3 * nr,
-                              hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        std::vector<double> bnorm(nr);
-        for (int h = 0; h < nr; ++h) {
-            double v;
-            std::memcpy(&v, &hb[2 * nr + h], sizeof v);
-            bnorm[h] = v;
-        }
-        // R holds b; copy rmax = bmax for the first scaling
-        HIPCHK(hipMemcpyAsync(rmax, bmax, sizeof(unsigned long long) * nr, hipMemcpyDeviceToDevice, s));
-        double omega = INFINITY;
-        int it = 0;
-        for (; it < maxit; ++it) {
-            scale_kernel<<<1, 64, 0, s>>>(nr, rmax, f->scale);
-            demote_kernel<<<gv, 256, 0, s>>>(n, nr, f->R, f->scale, f->Y);
-            rc = lu32_apply(f, f->Y, nr);
-            if (rc) return FAIL(rc, "triangular sweep launch failed");
-            zero_u64_kernel<<<1, 64, 0, s>>>(f->norms, 2 * nr);
-            update_kernel<<<gv, 256, 0, s>>>(n, nr, f->Y, f->scale, Xh, ldx, xmax);
-            residual_partial_kernel<<<gp, 256, 0, s>>>(n, (const double2*)A64, lda, Xh, ldx, nr, f->part);
-            residual_reduce_kernel<<<gv, 256, 0, s>>>(n, Bh, ldb, nr, f->part, f->R, rmax);
-            HIPCHK(hipGetLastError());
-            HIPCHK(hipMemcpyAsync(hb.data(), f->norms, sizeof(unsigned long long) * 2 * nr,
-                                  hipMemcpyDeviceToHost, s));
-            HIPCHK(hipMemcpyAsync(dflag, f->dflag, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
-            HIPCHK(hipStreamSynchronize(s));
-            if (dflag[1]) return FAIL(BEM2D_ERR_DEVICE, "triangular sweep wait timed out");
-            omega = 0.0;
-            for (int h = 0; h < nr; ++h) {
-                double rn, xn;
-                std::memcpy(&rn, &hb[h], sizeof rn);
-                std::memcpy(&xn, &hb[nr + h], sizeof xn);
-                const double den = anorm * xn + bnorm[h];
-                const double w = den > 0 ? rn / den : (rn > 0 ? INFINITY : 0.0);
-                omega = std::max(omega, std::isfinite(w) ? w : INFINITY);
-            }
-            if (omega <= tol) {
-                ++it;
-                break;
-            }
-        }
-        r.iterations = std::max(r.iterations, it);
-        worst = (h0 == 0) ? omega : std::max(worst, omega);
-        r.matrix_norm_inf = anorm;
-    }
-    r.backward_error = worst;
-    r.converged = worst <= tol;
+    bem2d_internal_ref_report(j, 0, &r);
+    const bool bad = bem2d_internal_ref_bad_pivot(j, 0);
+    bem2d_internal_ref_destroy(j);
     if (rep) *rep = r;
+    if (rc == BEM2D_ERR_DEVICE) return FAIL(BEM2D_ERR_DEVICE, "triangular sweep wait timed out");
+    if (bad) {
+        if (rep) {
+            rep->converged = 0;
+            rep->iterations = 0;
+            rep->backward_error = INFINITY;
+        }
+        return FAIL(BEM2D_ERR_NOT_CONVERGED, "FP32 LU met a zero or non-finite pivot");
+    }
+    if (rc) return rc;
     if (!r.converged)
         return FAIL(BEM2D_ERR_NOT_CONVERGED, "refinement reached backward error %.3e > %.3e",
-                    worst, tol);
+                    r.backward_error, tol);
     return BEM2D_OK;
 }
 

@@ -183,6 +183,73 @@ int bem2d_eval_single_layer(bem2d_ctx* ctx, const bem2d_mesh* mesh, double k, co
 int bem2d_farfield(bem2d_ctx* ctx, const bem2d_mesh* mesh, double k, const void* coef,
                    int64_t ldc, int nrhs, const double* theta, int64_t nang, void* out);
 
+/* ---- serving engine: continuous batching of scattering solves ----------------------------------
+ * Not in the reference, whose only driver is one SoundSoftScattering at a time
+ * (lib/bem2d_soundsoftscattering.h:74-126).  A request is one such problem -- DiscretizeMatrix
+ * (A = M + 2K, combined field), DiscretizeRhs (nrhs plane waves b = 2<f, phi>), Solve, and the far
+ * field at nang angles -- solved on the mixed path (FP32 MFMA LU + FP64 refinement to the
+ * request's backward-error tolerance) or the plain FP64 path (rocSOLVER).  The engine owns one
+ * preallocated HBM arena and admits queued requests first-in first-out as their footprint fits;
+ * each engine step advances every admitted request by one stage with one batched launch per
+ * kernel per stage, so requests of different sizes and arrival times share launches.  Results
+ * (x and u_inf) are copied to host memory; the device memory returns to the arena at once. */
+#define BEM2D_PRECISION_MIXED 0
+#define BEM2D_PRECISION_FP64 1
+
+typedef struct bem2d_engine bem2d_engine;
+
+typedef struct {
+    bem2d_geometry geom;     /* host arrays, copied at submission */
+    bem2d_quad_option opts;
+    double k;                /* real wavenumber > 0 */
+    double eta;              /* coupling of the combined-field operator and of CombinedPlaneWave */
+    int nrhs;                /* plane waves */
+    int precision;           /* BEM2D_PRECISION_* */
+    const double* dirs;      /* host, 2 nrhs: directions (x, y), used as given */
+    int64_t nang;            /* far-field angles (0: none) */
+    const double* theta;     /* host, nang */
+    double tol;              /* mixed path: backward-error target (<= 0: 1e-12) */
+} bem2d_request;
+
+typedef struct {
+    int status;              /* BEM2D_OK, or the failure of this request */
+    int precision;
+    int64_t n;
+    int nrhs;
+    int iterations;          /* refinement steps (mixed path) */
+    double backward_error;   /* mixed path: max over columns */
+    double t_arrival, t_admit, t_done; /* engine clock (s): latency = t_done - t_arrival */
+} bem2d_result_info;
+
+typedef struct {
+    int64_t submitted, admitted, completed, failed;
+    int64_t assembly_launches, lu_step_launches, refine_step_launches;
+    int64_t arena_bytes, arena_in_use, queued, active;
+} bem2d_engine_stats;
+
+/* Device bytes one request holds while admitted. */
+int64_t bem2d_engine_footprint(int64_t n, int nrhs, int64_t nang, int precision,
+                               bem2d_quad_option opts);
+int bem2d_engine_create(bem2d_ctx* ctx, int64_t hbm_bytes, int max_active, bem2d_engine** out);
+void bem2d_engine_destroy(bem2d_engine* eng);
+double bem2d_engine_now(const bem2d_engine* eng); /* engine clock, seconds since creation */
+/* Queues a request (t_arrival < 0: now); *ticket identifies it. */
+int bem2d_engine_submit(bem2d_engine* eng, const bem2d_request* req, double t_arrival,
+                        int64_t* ticket);
+/* One engine step (never blocks on the GPU except for mesh uploads); *unfinished = requests
+ * queued or in flight. */
+int bem2d_engine_step(bem2d_engine* eng, int* unfinished);
+int bem2d_engine_poll(bem2d_engine* eng, int64_t ticket, int* done);
+/* Copies a finished request's solution (n x nrhs complex128, column-major) and far field
+ * (nang x nrhs) into host memory (either may be NULL) and forgets the request. */
+int bem2d_engine_result(bem2d_engine* eng, int64_t ticket, void* x, void* farfield,
+                        bem2d_result_info* info);
+int bem2d_engine_get_stats(const bem2d_engine* eng, bem2d_engine_stats* out);
+/* Replays a trace: request q is submitted when the engine clock reaches (start + arrival[q]) s,
+ * and the engine steps until all are done; tickets[q] receives each ticket. */
+int bem2d_engine_run_trace(bem2d_engine* eng, const bem2d_request* reqs, const double* arrival,
+                           int count, int64_t* tickets);
+
 /* ---- kernel timing ------------------------------------------------------------------------------
  * With profiling on, the context records HIP events around each launch of the instrumented kernels
  * on the stream they run on.  bem2d_profile_read synchronises and returns the summed duration (ms)
