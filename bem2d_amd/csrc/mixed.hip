@@ -804,6 +804,67 @@ __global__ void zero_u64_kernel(unsigned long long* p, int64_t n)
     if (i < n) p[i] = 0ull;
 }
 
+// ---- many right-hand sides: column-parallel helpers of the library-GEMM refinement ------------------
+// m[h] = max_i |V[i + ld h]| (atomic max on the bit pattern of a non-negative double)
+__global__ void colmax_kernel(int64_t n, int nrhs, const double2* __restrict__ V, int64_t ld,
+                              unsigned long long* m)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int h = blockIdx.y;
+    double v = 0.0;
+    if (i < n && h < nrhs) {
+        const double2 x = V[i + ld * h];
+        v = hypot(x.x, x.y);
+        if (!(v == v)) v = INFINITY;
+    }
+    v = wave_max(v);
+    if ((threadIdx.x & 63) == 0 && h < nrhs) atomic_max_nonneg(&m[h], v);
+}
+
+// R = B, X = 0
+__global__ void wide_init_kernel(int64_t n, int nrhs, const double2* __restrict__ B, int64_t ldb,
+                                 double2* __restrict__ R, double2* __restrict__ X, int64_t ldx)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int h = blockIdx.y;
+    if (i >= n) return;
+    R[i + n * h] = B[i + ldb * h];
+    X[i + ldx * h] = make_double2(0.0, 0.0);
+}
+
+// Y = R / max(rmax, tiny) in FP32; s[h] records the scale.
+__global__ void wide_demote_kernel(int64_t n, int nrhs, const double2* __restrict__ R,
+                                   const unsigned long long* __restrict__ rmax,
+                                   double* __restrict__ scale, float2* __restrict__ Y)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int h = blockIdx.y;
+    const double m = __longlong_as_double((long long)rmax[h]);
+    const double sc = (m > 1e-300 && m < INFINITY) ? 1.0 / m : 1.0;
+    if (i == 0) scale[h] = sc;
+    if (i >= n) return;
+    const double2 r = R[i + n * h];
+    Y[i + n * h] = make_float2((float)(r.x * sc), (float)(r.y * sc));
+}
+
+// X += Y / scale; R = B (the residual is then formed in place by the GEMM)
+__global__ void wide_update_kernel(int64_t n, int nrhs, const float2* __restrict__ Y,
+                                   const double* __restrict__ scale, double2* __restrict__ X,
+                                   int64_t ldx, const double2* __restrict__ B, int64_t ldb,
+                                   double2* __restrict__ R)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int h = blockIdx.y;
+    if (i >= n) return;
+    const double s = 1.0 / scale[h];
+    const float2 y = Y[i + n * h];
+    double2 x = X[i + ldx * h];
+    x.x += (double)y.x * s;
+    x.y += (double)y.y * s;
+    X[i + ldx * h] = x;
+    R[i + n * h] = B[i + ldb * h];
+}
+
 } // namespace
 
 // ================================================================================================
@@ -1313,6 +1374,128 @@ int bem2d_internal_refine_sync(hipStream_t s, RefJob* j)
     return rc;
 }
 
+// Refinement of many right-hand sides at once (nrhs >= kWideRhs): the triangular solves are
+// rocBLAS ctrsm on the FP32 factors and the FP64 residual R = B - A X is one zgemm, so A64 and
+// the factors are read once per step for all columns instead of once per group of kMaxSolveRhs.
+constexpr int kWideRhs = 64;
+
+static int refine_wide(bem2d_ctx* c, bem2d_lu32* f, const double2* A, int64_t lda,
+                       const double2* B, int64_t ldb, double2* X, int64_t ldx, int nrhs,
+                       double tol, int maxit, bem2d_ir_report* rep)
+{
+    const int64_t n = f->n;
+    hipStream_t s = c->stream;
+    double2* R = nullptr;
+    float2* Y = nullptr;
+    double* scale = nullptr;
+    unsigned long long* nm = nullptr; // rmax | xmax | bmax
+    auto cleanup = [&]() {
+        (void)hipStreamSynchronize(s);
+        (void)hipFree(R);
+        (void)hipFree(Y);
+        (void)hipFree(scale);
+        (void)hipFree(nm);
+    };
+    if (hipMalloc(&R, sizeof(double2) * n * nrhs) != hipSuccess ||
+        hipMalloc(&Y, sizeof(float2) * n * nrhs) != hipSuccess ||
+        hipMalloc(&scale, sizeof(double) * nrhs) != hipSuccess ||
+        hipMalloc(&nm, sizeof(unsigned long long) * 3 * nrhs) != hipSuccess) {
+        cleanup();
+        return FAIL(BEM2D_ERR_NOMEM, "refinement workspace allocation failed");
+    }
+    std::vector<unsigned long long> h(3 * nrhs);
+    int dflag[2] = {0, 0};
+    double anorm = 0.0;
+    const dim3 g((unsigned)((n + 255) / 256), (unsigned)nrhs);
+    wide_init_kernel<<<g, 256, 0, s>>>(n, nrhs, B, ldb, R, X, ldx);
+    zero_u64_kernel<<<(unsigned)((3 * nrhs + 255) / 256), 256, 0, s>>>(nm, 3 * nrhs);
+    colmax_kernel<<<g, 256, 0, s>>>(n, nrhs, R, n, nm + 2 * nrhs);
+    (void)hipMemcpyAsync(&anorm, f->anorm, sizeof(double), hipMemcpyDeviceToHost, s);
+    (void)hipMemcpyAsync(dflag, f->dflag, sizeof(int) * 2, hipMemcpyDeviceToHost, s);
+    (void)hipMemcpyAsync(h.data(), nm, sizeof(unsigned long long) * 3 * nrhs,
+                         hipMemcpyDeviceToHost, s);
+    if (hipStreamSynchronize(s) != hipSuccess) {
+        cleanup();
+        return FAIL(BEM2D_ERR_DEVICE, "refinement start failed");
+    }
+    bem2d_ir_report r{};
+    r.matrix_norm_inf = anorm;
+    if (dflag[0]) {
+        cleanup();
+        r.backward_error = INFINITY;
+        if (rep) *rep = r;
+        return FAIL(BEM2D_ERR_NOT_CONVERGED, "FP32 LU met a zero or non-finite pivot");
+    }
+    std::vector<double> bnorm(nrhs);
+    for (int q = 0; q < nrhs; ++q) {
+        std::memcpy(&bnorm[q], &h[2 * nrhs + q], sizeof(double));
+    }
+    // rmax = bmax for the first scaling
+    (void)hipMemcpyAsync(nm, nm + 2 * nrhs, sizeof(unsigned long long) * nrhs,
+                         hipMemcpyDeviceToDevice, s);
+    const rocblas_float_complex one32 = {1.f, 0.f};
+    const rocblas_double_complex mone = {-1.0, 0.0}, one = {1.0, 0.0};
+    double omega = INFINITY;
+    int it = 0;
+    int rc = BEM2D_OK;
+    for (; it < maxit; ++it) {
+        wide_demote_kernel<<<g, 256, 0, s>>>(n, nrhs, R, nm, scale, Y);
+        rocblas_status st = rocblas_ctrsm(c->blas, rocblas_side_left, rocblas_fill_lower,
+                                          rocblas_operation_none, rocblas_diagonal_unit,
+                                          (rocblas_int)n, (rocblas_int)nrhs, &one32,
+                                          (const rocblas_float_complex*)f->F, (rocblas_int)n,
+                                          (rocblas_float_complex*)Y, (rocblas_int)n);
+        if (st == rocblas_status_success)
+            st = rocblas_ctrsm(c->blas, rocblas_side_left, rocblas_fill_upper,
+                               rocblas_operation_none, rocblas_diagonal_non_unit, (rocblas_int)n,
+                               (rocblas_int)nrhs, &one32, (const rocblas_float_complex*)f->F,
+                               (rocblas_int)n, (rocblas_float_complex*)Y, (rocblas_int)n);
+        wide_update_kernel<<<g, 256, 0, s>>>(n, nrhs, Y, scale, X, ldx, B, ldb, R);
+        if (st == rocblas_status_success)
+            st = rocblas_zgemm(c->blas, rocblas_operation_none, rocblas_operation_none,
+                               (rocblas_int)n, (rocblas_int)nrhs, (rocblas_int)n, &mone,
+                               (const rocblas_double_complex*)A, (rocblas_int)lda,
+                               (const rocblas_double_complex*)X, (rocblas_int)ldx, &one,
+                               (rocblas_double_complex*)R, (rocblas_int)n);
+        if (st != rocblas_status_success) {
+            rc = FAIL(BEM2D_ERR_DEVICE, "rocBLAS status %d in refinement", (int)st);
+            break;
+        }
+        zero_u64_kernel<<<(unsigned)((2 * nrhs + 255) / 256), 256, 0, s>>>(nm, 2 * nrhs);
+        colmax_kernel<<<g, 256, 0, s>>>(n, nrhs, R, n, nm);
+        colmax_kernel<<<g, 256, 0, s>>>(n, nrhs, X, ldx, nm + nrhs);
+        (void)hipMemcpyAsync(h.data(), nm, sizeof(unsigned long long) * 2 * nrhs,
+                             hipMemcpyDeviceToHost, s);
+        if (hipStreamSynchronize(s) != hipSuccess) {
+            rc = FAIL(BEM2D_ERR_DEVICE, "refinement step failed");
+            break;
+        }
+        omega = 0.0;
+        for (int q = 0; q < nrhs; ++q) {
+            double rn, xn;
+            std::memcpy(&rn, &h[q], sizeof rn);
+            std::memcpy(&xn, &h[nrhs + q], sizeof xn);
+            const double den = anorm * xn + bnorm[q];
+            const double w = den > 0 ? rn / den : (rn > 0 ? INFINITY : 0.0);
+            omega = std::max(omega, std::isfinite(w) ? w : INFINITY);
+        }
+        if (omega <= tol) {
+            ++it;
+            break;
+        }
+    }
+    cleanup();
+    r.iterations = it;
+    r.backward_error = omega;
+    r.converged = omega <= tol;
+    if (rep) *rep = r;
+    if (rc) return rc;
+    if (!r.converged)
+        return FAIL(BEM2D_ERR_NOT_CONVERGED, "refinement reached backward error %.3e > %.3e",
+                    omega, tol);
+    return BEM2D_OK;
+}
+
 extern "C" int bem2d_lu32_refine(bem2d_ctx* c, bem2d_lu32* f, const void* A64, int64_t lda,
                                  const void* B, int64_t ldb, void* X, int64_t ldx, int nrhs,
                                  double tol, int maxit, bem2d_ir_report* rep)
@@ -1322,6 +1505,10 @@ extern "C" int bem2d_lu32_refine(bem2d_ctx* c, bem2d_lu32* f, const void* A64, i
         return FAIL(BEM2D_ERR_ARG, "bem2d_lu32_refine: bad argument");
     if (!f->factored) return FAIL(BEM2D_ERR_ARG, "bem2d_lu32_refine: not factored");
     HIPCHK(hipSetDevice(c->device));
+    if (nrhs >= kWideRhs && nrhs <= 65535 && f->n <= 2000000000 && lda <= 2000000000 &&
+        ldx <= 2000000000)
+        return refine_wide(c, f, (const double2*)A64, lda, (const double2*)B, ldb, (double2*)X, ldx,
+                           nrhs, tol, maxit, rep);
     RefSpec sp{f, A64, lda, B, ldb, X, ldx, nrhs, tol, maxit, nullptr};
     RefJob* j = nullptr;
     int rc = bem2d_internal_ref_create(c, nullptr, &sp, 1, nullptr, &j);

@@ -77,3 +77,29 @@ def test_zero_pivot_falls_back_to_fp64(ctx):
     rep = ctx.solve_mixed(_dev(A), Bd, tol=1e-12)
     assert rep.fallback_fp64 == 1
     assert rel_fro(Bd.cpu().numpy(), np.linalg.solve(A, B)) < 1e-12
+
+
+@pytest.mark.parametrize("nrhs", [64, 360])
+def test_many_right_hand_sides(ctx, nrhs):
+    """The library-GEMM refinement path (ctrsm on the FP32 factors, zgemm residual) against the
+    FP64 solve, with the far field checked at 1e-10 as on the large configuration."""
+    print("case many rhs", nrhs, flush=True)
+    n, k = 700, 40.0
+    geom = G.open_arc(n)
+    mesh = ctx.mesh(geom, 16, 3, 0.15)
+    A = ctx.assemble(mesh, capi.KERNEL_COMBINED, k, k, 1.0, 2.0)
+    th = 2 * np.pi * np.arange(nrhs) / nrhs
+    B = ctx.rhs(mesh, capi.FIELD_COMBINED_PLANE, k, k, np.stack([np.cos(th), np.sin(th)], 1), 2.0)
+    X64 = ctx.solve_fp64(A, B.clone())
+    lu = ctx.lu32(n)
+    lu.factor(A)
+    X = torch.empty_like(B)
+    rep = lu.refine(A, B, X, tol=1e-12)
+    lu.close()
+    assert rep.converged == 1 and rep.backward_error <= 1e-12
+    assert (torch.linalg.norm(X - X64) / torch.linalg.norm(X64)).item() < 1e-10
+    ang = torch.tensor(2 * np.pi * np.arange(90) / 90, device="cuda")
+    f32 = ctx.farfield(mesh, k, X, ang)
+    f64 = ctx.farfield(mesh, k, X64, ang)
+    assert (torch.linalg.norm(f32 - f64) / torch.linalg.norm(f64)).item() < 1e-10
+    mesh.close()
