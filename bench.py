@@ -1,13 +1,19 @@
 """Benchmark of the bem2d hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-extras]
+    python bench.py --config batched|serving|large
 
-Workload (one step): sound-soft scattering by the kite x = cos t + 0.65 cos 2t - 0.65, y = 1.5 sin t
-at k = 100 with the combined-field operator (eta = k), N = 4000 piecewise-constant elements at
-uniform parameter spacing, 16-point Gauss with the singular rules at L = 3, sigma = 0.15, and one
-plane-wave right-hand side at a seeded random angle: Galerkin assembly A = M + 2K, the right-hand
-side b = 2<f, phi>, and the dense solve.  Inputs (geometry tables) are resident in HBM before the
-timed region starts.  Prints one JSON line.
+Default workload (one step): sound-soft scattering by the kite x = cos t + 0.65 cos 2t - 0.65,
+y = 1.5 sin t at k = 100 with the combined-field operator (eta = k), N = 4000 piecewise-constant
+elements at uniform parameter spacing, 16-point Gauss with the singular rules at L = 3,
+sigma = 0.15, and one plane-wave right-hand side at a seeded random angle: Galerkin assembly
+A = M + 2K, the right-hand side b = 2<f, phi>, and the dense solve.  Inputs (geometry tables) are
+resident in HBM before the timed region starts.  The same line carries, unless --no-extras, a
+batched run (64 seeded problems) and a short served trace (Poisson arrivals; p50/p99 latency) on
+the mixed and the plain FP64 path.  Prints one JSON line.
+
+--config batched / serving / large prints the line of the 64-problem batch, the full 2000-request
+served trace, or the N = 32000 open arc with 360 right-hand sides instead.
 """
 import argparse
 import json
@@ -24,7 +30,21 @@ sys.path.insert(0, ROOT)
 METRIC = "solves/sec & Gentries/sec assembly, p50/p99 latency; FP32+refine vs FP64 at 1e-10"
 FLOP_PER_PAIRPOINT = 128  # convention for the combined-field kernel evaluation
 PEAK_FP64_TFLOPS = 78.6
+PEAK_FP32_MFMA_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "traffic_assemble_regular_kite4000.md")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the regular assembly kernel on the default workload, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (a line 'traffic_bytes_per_launch: B')."""
+    try:
+        for line in open(TRAFFIC_PROFILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def pair_points(n, nspecial, nq=16, L=3):
@@ -56,6 +76,129 @@ def cpu_baseline(geom, A_host, k, b_host, ncols, seed=7):
     return t_asm, t_solve, t_cols, err
 
 
+def run_batched(ctx, count=64, seed=1):
+    """C3: count seeded problems submitted at once; every stage runs as batched launches."""
+    import time
+    from bem2d_amd import Engine, capi, serving
+    out = {"workload": "%d problems (seed %d): ellipses / polygons, k log-uniform in [5, 200], 10 "
+                       "elements per wavelength, 1 plane wave each, far field at 360 angles"
+                       % (count, seed)}
+    far = {}
+    for prec, name in ((capi.PRECISION_MIXED, "mixed"), (capi.PRECISION_FP64, "plain_fp64")):
+        reqs = serving.batched_problems(count, seed=seed, precision=prec)
+        eng = Engine(ctx, 96 << 30, max_active=count)
+        warm = [eng.submit(q) for q in reqs[:2]]
+        while eng.step() > 0:
+            pass
+        for t in warm:
+            eng.result(t)
+        t0 = time.perf_counter()
+        tickets = [eng.submit(q) for q in reqs]
+        while eng.step() > 0:
+            pass
+        dt = time.perf_counter() - t0
+        res = [eng.result(t, raise_on_fail=False) for t in tickets]
+        st = eng.stats()
+        eng.close()
+        far[name] = [r[1] for r in res]
+        out[name] = {"solves_per_s": count / dt, "makespan_s": dt,
+                     "failed": sum(r[2].status != 0 for r in res),
+                     "lu_step_launches": st["lu_step_launches"],
+                     "assembly_launches": st["assembly_launches"]}
+        out["n_range"] = [min(q["geom"].n for q in reqs), max(q["geom"].n for q in reqs)]
+    errs = [np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in zip(far["mixed"], far["plain_fp64"])]
+    out["farfield_rel_err_mixed_vs_fp64_max"] = float(max(errs))
+    out["mixed_over_plain"] = out["mixed"]["solves_per_s"] / out["plain_fp64"]["solves_per_s"]
+    return out
+
+
+def run_serving(ctx, count, rate, seed=2):
+    """C4: Poisson arrivals at `rate`; latency = completion - arrival; solves = (request, rhs)."""
+    import time
+    from bem2d_amd import Engine, capi, serving
+    out = {"workload": "%d requests (seed %d), Poisson arrivals at %.1f/s: ellipse / kite / polygon, "
+                       "k uniform in [10, 400], 10 elements per wavelength, 1-32 plane waves, far "
+                       "field at 36 angles" % (count, seed, rate), "rate_per_s": rate}
+    for prec, name in ((capi.PRECISION_MIXED, "mixed"), (capi.PRECISION_FP64, "plain_fp64")):
+        reqs, arr = serving.serving_trace(count, rate=rate, seed=seed, precision=prec)
+        eng = Engine(ctx, 160 << 30, max_active=64)
+        t0 = time.perf_counter()
+        tickets = eng.run_trace(reqs, arr)
+        dt = time.perf_counter() - t0
+        res = [eng.result(t, raise_on_fail=False) for t in tickets]
+        eng.close()
+        infos = [r[2] for r in res]
+        nrhs = sum(q["dirs"].shape[0] for q in reqs)
+        lat = serving.latency_summary(infos)
+        out[name] = {"p50_ms": lat["p50_ms"], "p99_ms": lat["p99_ms"], "mean_ms": lat["mean_ms"],
+                     "solves_per_s": nrhs / dt, "requests_per_s": count / dt, "wall_s": dt,
+                     "failed": sum(i.status != 0 for i in infos),
+                     "max_backward_error": max(i.backward_error for i in infos)}
+    out["mixed_over_plain_solves"] = out["mixed"]["solves_per_s"] / out["plain_fp64"]["solves_per_s"]
+    return out
+
+
+def run_large(ctx, n=32000, k=500.0, nrhs=360):
+    """C5: open arc (aperture 0.3 rad), N = 32000, k = 500, 360 plane waves."""
+    import time
+    import torch
+    from bem2d_amd import capi
+    from bem2d_amd import geometry as G
+    geom = G.open_arc(n, 0.3)
+    mesh = ctx.mesh(geom, 16, 3, 0.15)
+    th = 2 * np.pi * np.arange(nrhs) / nrhs
+    dirs = np.stack([np.cos(th), np.sin(th)], 1)
+    A = torch.empty((n, n), dtype=torch.complex128, device="cuda").t()
+    ctx.profile(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.assemble(mesh, capi.KERNEL_COMBINED, k, k, 1.0, 2.0, out=A)
+    torch.cuda.synchronize()
+    t_asm = time.perf_counter() - t0
+    reg_ms, _ = ctx.profile_read(capi.PROF_ASSEMBLE_REGULAR)
+    ctx.profile(False)
+    B = ctx.rhs(mesh, capi.FIELD_COMBINED_PLANE, k, k, dirs, 2.0)
+    lu = ctx.lu32(n)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    lu.factor(A)
+    torch.cuda.synchronize()
+    t_lu = time.perf_counter() - t0
+    X = torch.empty_like(B)
+    t0 = time.perf_counter()
+    rep = lu.refine(A, B, X, tol=1e-12, raise_on_fail=False)
+    torch.cuda.synchronize()
+    t_ref = time.perf_counter() - t0
+    lu.close()
+    X64 = B.clone()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.solve_fp64(A, X64, preserve_A=True)
+    torch.cuda.synchronize()
+    t64 = time.perf_counter() - t0
+    ang = torch.tensor(2 * np.pi * np.arange(360) / 360, device="cuda")
+    fm = ctx.farfield(mesh, k, X, ang)
+    f64 = ctx.farfield(mesh, k, X64, ang)
+    e_ff = (torch.linalg.norm(fm - f64) / torch.linalg.norm(f64)).item()
+    e_x = (torch.linalg.norm(X - X64) / torch.linalg.norm(X64)).item()
+    P_reg, P_sing, _ = pair_points(n, mesh.special_count)
+    mesh.close()
+    del A, B, X, X64
+    torch.cuda.empty_cache()
+    flu = 8.0 / 3.0 * n ** 3
+    return {"workload": "open arc (radius 1, aperture 0.3 rad) N=%d k=%g, %d plane waves" % (n, k, nrhs),
+            "assembly_s": t_asm, "assembly_regular_ms": reg_ms,
+            "gpairpts_per_s": (P_reg + P_sing) / t_asm / 1e9,
+            "lu32_s": t_lu, "lu32_tflops": flu / t_lu / 1e12,
+            "lu32_frac_of_fp32_mfma_peak": flu / t_lu / 1e12 / PEAK_FP32_MFMA_TFLOPS,
+            "refine_s": t_ref, "refine_steps": rep.iterations, "backward_error": rep.backward_error,
+            "fp64_solve_s": t64, "fp64_lu_tflops": flu / t64 / 1e12,
+            "mixed_solve_s": t_lu + t_ref,
+            "solves_per_s_mixed": nrhs / (t_asm + t_lu + t_ref),
+            "solves_per_s_plain_fp64": nrhs / (t_asm + t64),
+            "x_rel_err_mixed_vs_fp64": e_x, "farfield_rel_err_mixed_vs_fp64": e_ff}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,6 +210,14 @@ def main():
     ap.add_argument("--cpu-cols", type=int, default=64)
     ap.add_argument("--path", choices=["both", "plain", "mixed"], default="both",
                     help="profiling aid: time only one solve path (the JSON then reports it)")
+    ap.add_argument("--config", choices=["kite", "batched", "serving", "large"], default="kite")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the batched and served runs on the default line")
+    ap.add_argument("--serving-count", type=int, default=None)
+    ap.add_argument("--serving-rate", type=float, default=27.0,
+                    help="arrivals per second (about 0.8 x the plain path's measured capacity)")
+    ap.add_argument("--probe-traffic", action="store_true",
+                    help="one assembly and one 1 GiB device copy, for rocprofv3 --pmc runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: every box here has a single MI355X, and the path "
@@ -77,6 +228,28 @@ def main():
     import torch
     from bem2d_amd import Context, capi
     from bem2d_amd import geometry as G
+
+    if args.probe_traffic:
+        probe_traffic(args)
+        return
+    if args.config != "kite":
+        ctx = Context(0)
+        if args.config == "batched":
+            res = run_batched(ctx)
+            value, unit = res["mixed"]["solves_per_s"], "solves/s"
+        elif args.config == "serving":
+            res = run_serving(ctx, args.serving_count or 2000, args.serving_rate)
+            value, unit = res["mixed"]["solves_per_s"], "solves/s"
+        else:
+            res = run_large(ctx)
+            value, unit = res["solves_per_s_mixed"], "solves/s"
+        print(json.dumps({"metric": METRIC, "value": value, "unit": unit, "n_gpus": 1,
+                          "steps": 1, "warmup": 0, "higher_is_better": True, "scaling": "weak",
+                          "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+                          "config": {"workload": res["workload"], "name": args.config},
+                          "result": res}), flush=True)
+        ctx.close()
+        return
 
     n, k = args.n, args.k
     rng = np.random.default_rng(0)
@@ -208,12 +381,17 @@ def main():
         "backward_error_fp64": bw,
         "roofline": {
             "kernel": "assemble_regular_kernel<COMBINED,16>",
-            "bound": "fp64-valu",
+            "bound": "mfma",
+            "engine": "FP64 vector ALU: compute-bound (about 2000 FLOP per stored byte); its "
+                      "peak, 78.6 TFLOP/s, is also MI355X's dense FP64 matrix peak",
+            "convention": "128 FLOP per pair-point of the reference algorithm (256 per regular "
+                          "entry); the kernel shares each Hankel evaluation between entries (i,j) "
+                          "and (j,i), so it issues about half of these FLOP in hardware",
             "achieved": achieved,
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64_TFLOPS,
-            "traffic": None,
+            "traffic": measured_traffic(),
             "work_per_launch": "%d FLOP = 128 FLOP x 256 pair-points x %d regular entries" %
                                (FLOP_PER_PAIRPOINT * P_reg, n_reg),
         },
@@ -231,7 +409,35 @@ def main():
                       "(%.2f s); sampled columns agree with the GPU to %.1e" %
                       (args.cpu_cols, n, t_cols, n // args.cpu_cols, t_asm, t_solve, err),
         }
+    mesh.close()
+    del A, B, X, Xm
+    torch.cuda.empty_cache()
+    if not args.no_extras:
+        out["batched"] = run_batched(ctx)
+        sv = run_serving(ctx, args.serving_count or 200, args.serving_rate)
+        out["serving"] = sv
+        out["p50_ms"] = sv["mixed"]["p50_ms"]
+        out["p99_ms"] = sv["mixed"]["p99_ms"]
     print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+def probe_traffic(args):
+    """One regular-assembly launch of the default workload and one 1 GiB device-to-device copy
+    (the calibration kernel of known size) -- the program rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+    runs profile."""
+    import torch
+    from bem2d_amd import Context, capi
+    from bem2d_amd import geometry as G
+    ctx = Context(0)
+    mesh = ctx.mesh(G.kite(args.n), 16, 3, 0.15)
+    A = torch.empty((args.n, args.n), dtype=torch.complex128, device="cuda").t()
+    ctx.assemble(mesh, capi.KERNEL_COMBINED, args.k, args.k, 1.0, 2.0, out=A)
+    src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print("probe done: assembly N=%d + copy of %d bytes" % (args.n, src.numel() * 8), flush=True)
     mesh.close()
     ctx.close()
 
