@@ -153,8 +153,12 @@ __device__ __forceinline__ int batch_find(const int64_t* start, int count, int64
 // test node x on element i and the trial node y on element j, (mr, mi) with test node y on j and
 // trial node x on i.  The two share |x - y| and with it H0 and H1; only the normal factor differs,
 // and the negated difference y - x is exact, so each value is the one kernel_eval gives for its
-// order.  FAR: k|x - y| >= kHankelFarX is known and the asymptotic piece is used without a test.
-template <int KIND, bool FAR>
+// order.  MODE: PAIR_GENERAL evaluates H0, H1 by the piecewise code; PAIR_MID knows
+// k|x - y| >= kHankelMidX and PAIR_FAR knows k|x - y| >= kHankelFarX, and these use the one
+// polynomial set that covers the range, without a test.
+enum { PAIR_GENERAL = 0, PAIR_MID = 1, PAIR_FAR = 2 };
+
+template <int KIND, int MODE>
 __device__ __forceinline__ void kernel_pair(double k, double kinv, double k4, double eta4,
                                             double xx, double xy, double yx, double yy,
                                             double n1x, double n1y, double n2x, double n2y,
@@ -177,11 +181,14 @@ __device__ __forceinline__ void kernel_pair(double k, double kinv, double k4, do
     const double rd = rsqrt_refined(d2);
     const double d = d2 * rd;
     const double x = k * d, invx = kinv * rd;
-    if (FAR) {
+    if (MODE != PAIR_GENERAL) {
         // H0 = A (u0r + i u0i), H1 = A (u1r + i u1i) as in hankel01_combine, with the amplitude
         // A = sqrt(2/(pi x)) folded into the scalar factors of the kernel
         double P0, Q0, P1, Q1, S, C;
-        hankel_pq_far(invx, P0, Q0, P1, Q1);
+        if (MODE == PAIR_FAR)
+            hankel_pq_far(invx, P0, Q0, P1, Q1);
+        else
+            hankel_pq<BEM2D_HANKEL_NPIECE - 1>(invx, P0, Q0, P1, Q1);
         sincos_minus_quarter(x, S, C);
         const double A = rsqrt_refined(0.5 * kPi * x);
         const double u0r = P0 * C - Q0 * S, u0i = P0 * S + Q0 * C;
@@ -263,7 +270,7 @@ __host__ __device__ __forceinline__ int64_t upper_tile_count(int64_t n)
 
 // Node-pair sums of one element pair (i, j): direct (ar, ai) = sum_a w_a sum_b w_b K(x_a, y_b) for
 // entry (i, j), in the order of the reference's Integrate; mirror (mr, mi) for entry (j, i).
-template <int KIND, int NQ, bool FAR>
+template <int KIND, int NQ, int MODE>
 __device__ __forceinline__ void pair_sums(int nq, int64_t n, int64_t i, const double* __restrict__ nodes,
                                           const double* sc, double k, double kinv, double k4,
                                           double eta4, double& ar, double& ai, double& mr,
@@ -280,8 +287,8 @@ __device__ __forceinline__ void pair_sums(int nq, int64_t n, int64_t i, const do
         for (int b = 0; b < nq; ++b) {
             const double* y = sc + b * 5;
             double kr, ki, kmr, kmi;
-            kernel_pair<KIND, FAR>(k, kinv, k4, eta4, xx, xy, y[0], y[1], n1x, n1y, y[2], y[3],
-                                   kr, ki, kmr, kmi);
+            kernel_pair<KIND, MODE>(k, kinv, k4, eta4, xx, xy, y[0], y[1], n1x, n1y, y[2], y[3],
+                                    kr, ki, kmr, kmi);
             br = fma(kr, y[4], br);
             bi = fma(ki, y[4], bi);
             cr = fma(kmr, y[4], cr);
@@ -300,7 +307,7 @@ __device__ __forceinline__ void pair_sums(int nq, int64_t n, int64_t i, const do
 // and A(j, i).
 // The trial element's node data sit in LDS and are read as wave-uniform broadcasts.  A wavefront
 // whose 64 pairs of one column are all at k|x - y| >= kHankelFarX by the bounding circles takes the
-// branch-free asymptotic loop.  Coincident / adjacent pairs are skipped here and written by
+// branch-free asymptotic loop; all at >= kHankelMidX, the branch-free loop of the last piece.  Coincident / adjacent pairs are skipped here and written by
 // assemble_special_kernel.  NQ = nq at compile time (0: runtime nq).
 template <int KIND, int NQ>
 __global__ __launch_bounds__(64 * kWavesPerBlock) void assemble_regular_kernel(const AsmBatch batch)
@@ -359,21 +366,25 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void assemble_regular_kernel(c
         const bool direct = j > i && !(idx_j == idx_i || nx_i == idx_j || pv_i == idx_j);
         const bool mirror = j > i && !(idx_i == idx_j || nx_j == idx_i || pv_j == idx_i);
         const bool any = direct || mirror;
-        bool far = true;
+        bool far = true, mid = true;
         if (any) {
             const double dx = cx_i - bounds[BOUND_CX * n + j];
             const double dy = cy_i - bounds[BOUND_CY * n + j];
-            const double gap = sqrt(dx * dx + dy * dy) - r_i - bounds[BOUND_R * n + j];
-            far = k * gap >= kHankelFarX * (1.0 + 1e-12);
+            const double kgap = k * (sqrt(dx * dx + dy * dy) - r_i - bounds[BOUND_R * n + j]);
+            far = kgap >= kHankelFarX * (1.0 + 1e-12);
+            mid = kgap >= kHankelMidX * (1.0 + 1e-12);
         }
-        const bool all_far = __all(far);
+        const bool all_far = __all(far), all_mid = __all(mid);
         if (!any) continue;
         const double* sc = sh + c * nq * 5;
         double ar, ai, mr, mi;
         if (all_far)
-            pair_sums<KIND, NQ, true>(nq, n, i, nodes, sc, k, kinv, k4, eta4, ar, ai, mr, mi);
+            pair_sums<KIND, NQ, PAIR_FAR>(nq, n, i, nodes, sc, k, kinv, k4, eta4, ar, ai, mr, mi);
+        else if (all_mid)
+            pair_sums<KIND, NQ, PAIR_MID>(nq, n, i, nodes, sc, k, kinv, k4, eta4, ar, ai, mr, mi);
         else
-            pair_sums<KIND, NQ, false>(nq, n, i, nodes, sc, k, kinv, k4, eta4, ar, ai, mr, mi);
+            pair_sums<KIND, NQ, PAIR_GENERAL>(nq, n, i, nodes, sc, k, kinv, k4, eta4, ar, ai, mr,
+                                              mi);
         if (direct) A[i + lda * j] = make_double2(kscale * ar, kscale * ai);
         if (mirror) A[j + lda * i] = make_double2(kscale * mr, kscale * mi);
     }

@@ -14,8 +14,8 @@ Representation (x > 0):
 The header gives every table as an initializer macro, so that the device code can hold them in
 __constant__ memory (read through scalar loads into SGPR operands) and the host code in constexpr
 arrays.  Two more tables serve the far-field fast path of the assembly:
-  FAR   the last piece [PIECES[-1], inf) again, as plain polynomials in w = 1/x^2 (the affine map
-        to z folded into the coefficients);
+  FAR   [FAR_X, inf) = [40, inf), where most far-field pairs of an assembly fall, as plain
+        polynomials of degree 4 in w = 1/x^2 (the affine map to z folded into the coefficients);
   TRIG  sin r = r + r^3 S(r^2), cos r = 1 + r^2 C(r^2) on |r| <= pi/4, S of degree 5 and C of
         degree 6, interpolated at Chebyshev nodes in r^2 (near-minimax; 1.4e-17 relative and
         1.2e-19 absolute), plus 2/pi and pi/2 split into three doubles for the reduction.
@@ -29,8 +29,9 @@ import numpy as np
 
 mp.mp.dps = 40
 HERE = os.path.dirname(os.path.abspath(__file__))
-PIECES = [2.0, 2.6, 3.5, 5.0, 8.0, 20.0, 40.0]  # piece p covers [PIECES[p], PIECES[p+1]); last [40, inf)
-DEGS = [12, 12, 12, 12, 12, 7, 4]             # polynomial degree per piece
+PIECES = [2.0, 2.6, 3.5, 5.0, 8.0]  # piece p covers [PIECES[p], PIECES[p+1]); last [8, inf)
+DEGS = [12, 12, 12, 12, 10]         # polynomial degree per piece
+FAR_X, FAR_DEG = 40.0, 4            # the far table: [FAR_X, inf), polynomials in w = 1/x^2
 DEG = max(DEGS)
 NSER = 14  # series terms k = 0..NSER-1
 
@@ -168,8 +169,8 @@ def main():
         emit("    }, \\")
     emit("}")
     # ---- far piece as polynomials in w = 1/x^2: z = (a^2 w - ym)/yh ------------------------------
-    a = mp.mpf(PIECES[-1])
-    degf = DEGS[-1]
+    a = mp.mpf(FAR_X)
+    degf = FAR_DEG
     far = []
     for nu in (0, 1):
         def fP(y, nu=nu):
@@ -203,7 +204,8 @@ def main():
                 err = max(err, float(abs((approx - exact) / scale)))
             worst = max(worst, err)
             print("far piece w-form: max |err| = %.3g" % err)
-    emit("// far piece [%s, inf): [P0, xQ0, P1, xQ1][ascending powers of w = 1/x^2]" % PIECES[-1])
+    emit("// far table [%s, inf): [P0, xQ0, P1, xQ1][ascending powers of w = 1/x^2]" % FAR_X)
+    emit("#define BEM2D_HANKEL_FAR_X %s" % fmt(FAR_X))
     emit("#define BEM2D_HANKEL_FAR_DEG %d" % degf)
     emit("#define BEM2D_HANKEL_FAR_INIT { \\")
     for cf in far:

@@ -5,8 +5,9 @@
 //   x < 2    ascending series in t = x^2/4 (A&S 9.1.10 and 9.1.11, n = 0, 1);
 //   x >= 2   H_nu = sqrt(2/(pi x)) (P_nu + i Q_nu) exp(i(x - (2nu+1) pi/4)) (A&S 9.2.5-9.2.6),
 //            P_nu and x Q_nu as polynomials of z(y), y = (a/x)^2, on the pieces [2, 2.6),
-//            [2.6, 3.5), [3.5, 5), [5, 8), [8, 20) (degree 12), [20, 40) (degree 7) and
-//            [40, inf) (degree 4, the asymptotic regime where most far-field pairs fall).
+//            [2.6, 3.5), [3.5, 5), [5, 8) (degree 12) and [8, inf) (degree 10), and for
+//            x >= 40 (the asymptotic regime where most far-field pairs fall) degree-4
+//            polynomials in w = 1/x^2.
 // On the device every coefficient lives in one __constant__ table and every index is a
 // compile-time constant, so the coefficients reach v_fma_f64 as SGPR operands through scalar loads
 // (instruction literals would each cost two v_mov_b32 per use).  The last piece is evaluated in
@@ -22,7 +23,8 @@ namespace bem2d_dev {
 constexpr double kPi = 3.14159265358979323846;
 constexpr double kTwoOverPi = 0.63661977236758134308;
 constexpr double kInvPi = 0.31830988618379067154;
-constexpr double kHankelFarX = BEM2D_PIECE_X[BEM2D_HANKEL_NPIECE - 1]; // start of the last piece
+constexpr double kHankelFarX = BEM2D_HANKEL_FAR_X;                      // start of the far table
+constexpr double kHankelMidX = BEM2D_PIECE_X[BEM2D_HANKEL_NPIECE - 1]; // start of the last piece
 
 struct HankelTables {
     double ser[4][BEM2D_HANKEL_NSER];                          // J0, J1, Y0, Y1 series in t
@@ -242,9 +244,7 @@ __host__ __device__ __forceinline__ void hankel01_inv(double x, double invx, dou
     double P0, Q0, P1, Q1;
     if (x >= kHankelFarX)
         hankel_pq_far(invx, P0, Q0, P1, Q1);
-    else if (x >= 20.0)
-        hankel_pq<5>(invx, P0, Q0, P1, Q1);
-    else if (x >= 8.0)
+    else if (x >= kHankelMidX)
         hankel_pq<4>(invx, P0, Q0, P1, Q1);
     else if (x >= 5.0)
         hankel_pq<3>(invx, P0, Q0, P1, Q1);

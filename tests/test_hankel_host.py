@@ -17,7 +17,7 @@ def test_host_hankel_matches_mpmath():
 
 
 def test_host_hankel_piece_boundaries_continuous():
-    for b in (2.0, 2.6, 3.5, 5.0, 8.0, 20.0, 40.0):
+    for b in (2.0, 2.6, 3.5, 5.0, 8.0, 40.0):
         x = np.array([np.nextafter(b, 0), b])
         H0, H1 = capi.hankel01_host(x)
         assert abs(H0[1] - H0[0]) < 2e-15 and abs(H1[1] - H1[0]) < 2e-15
