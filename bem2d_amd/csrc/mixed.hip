@@ -38,6 +38,11 @@ constexpr int NB = 64;             // panel / block width
 constexpr int kMaxSolveRhs = 8;    // right-hand sides handled per triangular sweep
 constexpr int kResidChunks = 64;   // column chunks of the FP64 residual
 constexpr long long kSpinLimit = 1ll << 26;
+// Panels pair up (one depth-128 trailing update per two panels) while the trailing matrix has at
+// least this many 64-wide tiles: the saved pass over it (16 B per entry) then outweighs the
+// diagonal factorisation that the narrow update of the pair leaves exposed (~60 us).  Measured on
+// MI355X: N = 4000 (at most 62 tiles) is faster unpaired, N = 32000 pairs (1.94 -> 1.53 s).
+constexpr int kPairTiles = 80;
 
 typedef float v4f __attribute__((ext_vector_type(4)));
 
@@ -53,12 +58,17 @@ struct LuProblem {
     int64_t k0;             // first row / column of the current diagonal block
     int kb;                 // its size
     int tiles;              // 64-wide tiles of the trailing matrix
+    int mode;               // trailing update: LU_UPDATE_FULL or LU_UPDATE_CROSS
     const float2* Linv;     // inverses of the current diagonal block (panel solves)
     const float2* UinvT;
     float2* Linv_next;      // inverses to produce for the next diagonal block, or null
     float2* UinvT_next;
     int* bad;               // raised on a zero or non-finite pivot
 };
+
+// Trailing update over the full tile grid, or over its first block row and block column only
+// (the "cross": the next panel, updated at once while the rest waits for a depth-128 update).
+enum { LU_UPDATE_FULL = 0, LU_UPDATE_CROSS = 1 };
 
 struct LuBatch {
     int count;
@@ -310,6 +320,18 @@ struct TileAcc {
     v4f re[2][2], im[2][2];
 };
 
+__device__ __forceinline__ void acc_zero(TileAcc& c)
+{
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            c.re[p][q] = v4f{0.f, 0.f, 0.f, 0.f};
+            c.im[p][q] = v4f{0.f, 0.f, 0.f, 0.f};
+        }
+}
+
+// c += Aop Bop over one staged k-chunk of 64
 __device__ __forceinline__ void cgemm_64(const float* __restrict__ Are,
                                          const float* __restrict__ Aim,
                                          const float* __restrict__ Bre,
@@ -319,13 +341,6 @@ __device__ __forceinline__ void cgemm_64(const float* __restrict__ Are,
     const int wave = t >> 6, lane = t & 63;
     const int wr = (wave & 1) * 32, wc = (wave >> 1) * 32;
     const int li = lane % 16, lk = lane / 16;
-#pragma unroll
-    for (int p = 0; p < 2; ++p)
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            c.re[p][q] = v4f{0.f, 0.f, 0.f, 0.f};
-            c.im[p][q] = v4f{0.f, 0.f, 0.f, 0.f};
-        }
     for (int ks = 0; ks < NB; ks += 4) {
         float ar[2], ai[2], br[2], bi[2];
 #pragma unroll
@@ -401,6 +416,7 @@ __global__ __launch_bounds__(256) void panel_trsm_kernel(const LuBatch batch)
     }
     __syncthreads();
     TileAcc c;
+    acc_zero(c);
     cgemm_64(Are, Aim, Bre, Bim, c);
 #pragma unroll
     for (int p = 0; p < 2; ++p)
@@ -419,18 +435,27 @@ __global__ __launch_bounds__(256) void panel_trsm_kernel(const LuBatch batch)
 }
 
 // ---- trailing update on MFMA, fused with the next diagonal block ---------------------------------
-// C[i][j] -= sum_k L[i][k] U[k][j] for i, j in [rest0, n), k in [k0, k0 + kb); workgroup = one
-// 64 x 64 output tile.  Workgroup (0, 0) owns the next diagonal block: after its update it goes on
-// to factorise that block (when Linv_next is given), overlapping the rest of the update.
+// C[i][j] -= sum_k L[i][k] U[k][j] for i, j in [rest0, n), k in [k0, k0 + K), K = kb = 64 or 128
+// (two staged chunks of 64); workgroup = one 64 x 64 output tile of the full grid, or of its
+// cross (first block row and column) for LU_UPDATE_CROSS.  Workgroup 0 owns the next diagonal
+// block: after its update it goes on to factorise that block (when Linv_next is given),
+// overlapping the rest of the update.
 __global__ __launch_bounds__(256) void trailing_update_kernel(const LuBatch batch)
 {
     const int di = find_problem(batch.start, batch.count, blockIdx.x);
     const LuProblem& pb = batch.p[di];
     float2* __restrict__ A = pb.F;
     const int64_t ld = pb.n, n = pb.n, k0 = pb.k0;
-    const int kb = pb.kb;
+    const int K = pb.kb;
     const int bid = (int)((int64_t)blockIdx.x - batch.start[di]);
-    const int bx = bid % pb.tiles, by = bid / pb.tiles;
+    int bx, by;
+    if (pb.mode == LU_UPDATE_CROSS) {
+        bx = bid < pb.tiles ? bid : 0;
+        by = bid < pb.tiles ? 0 : bid - pb.tiles + 1;
+    } else {
+        bx = bid % pb.tiles;
+        by = bid / pb.tiles;
+    }
     __shared__ __align__(16) float lds[kTileLdsFloats];
     static_assert(sizeof(DiagLds) <= sizeof(float) * kTileLdsFloats, "LDS reuse");
     float* Are = lds;
@@ -438,24 +463,29 @@ __global__ __launch_bounds__(256) void trailing_update_kernel(const LuBatch batc
     float* Bre = lds + 2 * NB * KP;
     float* Bim = lds + 3 * NB * KP;
     const int t = threadIdx.x;
-    const int64_t rest0 = k0 + kb;
+    const int64_t rest0 = k0 + K;
     const int64_t r0 = rest0 + (int64_t)bx * NB;
     const int64_t c0 = rest0 + (int64_t)by * NB;
-    for (int e = t; e < NB * NB; e += 256) {
-        const int r = e % NB, k = e / NB;
-        float2 v = make_float2(0.f, 0.f);
-        if (r0 + r < n && k < kb) v = A[(r0 + r) + ld * (k0 + k)];
-        Are[r * KP + k] = v.x;
-        Aim[r * KP + k] = v.y;
-        float2 u = make_float2(0.f, 0.f);
-        const int kk = e % NB, c = e / NB;
-        if (c0 + c < n && kk < kb) u = A[(k0 + kk) + ld * (c0 + c)];
-        Bre[c * KP + kk] = u.x;
-        Bim[c * KP + kk] = u.y;
-    }
-    __syncthreads();
     TileAcc acc;
-    cgemm_64(Are, Aim, Bre, Bim, acc);
+    acc_zero(acc);
+    for (int kc = 0; kc < K; kc += NB) {
+        const int kn = min(NB, K - kc);
+        if (kc > 0) __syncthreads(); // the previous chunk's operand reads are done
+        for (int e = t; e < NB * NB; e += 256) {
+            const int r = e % NB, k = e / NB;
+            float2 v = make_float2(0.f, 0.f);
+            if (r0 + r < n && k < kn) v = A[(r0 + r) + ld * (k0 + kc + k)];
+            Are[r * KP + k] = v.x;
+            Aim[r * KP + k] = v.y;
+            float2 u = make_float2(0.f, 0.f);
+            const int kk = e % NB, c = e / NB;
+            if (c0 + c < n && kk < kn) u = A[(k0 + kc + kk) + ld * (c0 + c)];
+            Bre[c * KP + kk] = u.x;
+            Bim[c * KP + kk] = u.y;
+        }
+        __syncthreads();
+        cgemm_64(Are, Aim, Bre, Bim, acc);
+    }
 #pragma unroll
     for (int p = 0; p < 2; ++p)
 #pragma unroll
@@ -902,6 +932,7 @@ struct bem2d_lu32 {
     int64_t n = 0;
     int nblk = 0;
     int step = -1;                // next panel step; nblk - 1 when factorised; -1 before
+    bool pending_wide = false;    // the last update covered the cross only; the next is depth 128
     bool factored = false;
     float2* F = nullptr;          // n x n factors, ld = n
     float2* Linv = nullptr;       // nblk x 64 x 64
@@ -997,6 +1028,7 @@ int bem2d_internal_lu_begin(hipStream_t s, bem2d_lu32* const* lus, const void* c
         rowsum_max_kernel<<<1, 256, 0, s>>>(n, 16, f->rowpart, f->anorm);
         zero_u64_kernel<<<1, 64, 0, s>>>((unsigned long long*)f->dflag, 1);
         f->step = 0;
+        f->pending_wide = false;
         f->factored = false;
     }
     for (int c0 = 0; c0 < count; c0 += kMaxLuBatch) {
@@ -1033,11 +1065,30 @@ int bem2d_internal_lu_advance(hipStream_t s, bem2d_lu32* const* lus, int count)
             bem2d_lu32* f = act[c0 + d];
             const int b = f->step;
             LuProblem p = lu_problem(f, b);
+            p.mode = LU_UPDATE_FULL;
             p.Linv_next = f->Linv + (size_t)(b + 1) * NB * NB;
             p.UinvT_next = f->UinvT + (size_t)(b + 1) * NB * NB;
-            tr.p[d] = up.p[d] = p;
+            tr.p[d] = p;
             tr.start[d + 1] = tr.start[d] + 2 * p.tiles;
-            up.start[d + 1] = up.start[d] + (int64_t)p.tiles * p.tiles;
+            // Panels pair up: step b updates only the cross (the next panel, which step b + 1
+            // factorises), and step b + 1 applies both panels to the rest at depth 128, so the
+            // trailing matrix is read and written once per two panels.
+            LuProblem u = p;
+            int64_t blocks;
+            if (f->pending_wide) {
+                u.k0 = (int64_t)(b - 1) * NB;
+                u.kb = 2 * NB;
+                f->pending_wide = false;
+                blocks = (int64_t)u.tiles * u.tiles;
+            } else if (b + 2 <= f->nblk - 1 && u.tiles >= kPairTiles) {
+                u.mode = LU_UPDATE_CROSS;
+                f->pending_wide = true;
+                blocks = 2 * (int64_t)u.tiles - 1;
+            } else {
+                blocks = (int64_t)u.tiles * u.tiles;
+            }
+            up.p[d] = u;
+            up.start[d + 1] = up.start[d] + blocks;
         }
         if (up.start[cnt] > (int64_t)INT32_MAX) return FAIL(BEM2D_ERR_UNSUPPORTED, "LU grid too large");
         panel_trsm_kernel<<<(unsigned)tr.start[cnt], 256, 0, s>>>(tr);

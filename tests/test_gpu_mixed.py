@@ -103,3 +103,20 @@ def test_many_right_hand_sides(ctx, nrhs):
     f64 = ctx.farfield(mesh, k, X64, ang)
     assert (torch.linalg.norm(f32 - f64) / torch.linalg.norm(f64)).item() < 1e-10
     mesh.close()
+
+
+def test_paired_panels_large_trailing_matrix(ctx):
+    """n = 5440: the first panels pair up (trailing matrices of >= 80 tiles get one depth-128
+    update per two panels), the later ones do not -- both against the FP64 solve."""
+    n, k = 5440, 60.0
+    print("case paired panels n=%d" % n, flush=True)
+    geom = G.kite(n)
+    mesh = ctx.mesh(geom, 16, 3, 0.15)
+    A = ctx.assemble(mesh, capi.KERNEL_COMBINED, k, k, 1.0, 2.0)
+    B = ctx.rhs(mesh, capi.FIELD_COMBINED_PLANE, k, k, [[0.6, 0.8], [1.0, 0.0]], 2.0)
+    X64 = ctx.solve_fp64(A, B.clone())
+    Xm = B.clone()
+    rep = ctx.solve_mixed(A, Xm, tol=1e-12)
+    mesh.close()
+    assert rep.fallback_fp64 == 0 and rep.backward_error <= 1e-12
+    assert (torch.linalg.norm(Xm - X64) / torch.linalg.norm(X64)).item() < 1e-10
