@@ -54,10 +54,16 @@ extern "C" int bem2d_ctx_create(bem2d_ctx** out, int device)
         return FAIL(BEM2D_ERR_DEVICE, "hipStreamCreate failed");
     }
     c->stream = c->own_stream;
-    if (rocblas_create_handle(&c->blas) != rocblas_status_success) {
+    if (hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
+        rocblas_create_handle(&c->blas) != rocblas_status_success) {
+        if (c->aux) (void)hipStreamDestroy(c->aux);
+        if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+        if (c->ev_join) (void)hipEventDestroy(c->ev_join);
         (void)hipStreamDestroy(c->own_stream);
         delete c;
-        return FAIL(BEM2D_ERR_DEVICE, "rocblas_create_handle failed");
+        return FAIL(BEM2D_ERR_DEVICE, "stream / event / rocBLAS handle creation failed");
     }
     rocblas_set_stream(c->blas, c->stream);
     *out = c;
@@ -75,6 +81,10 @@ extern "C" void bem2d_ctx_destroy(bem2d_ctx* c)
         if (s.ptr) (void)hipFree(s.ptr);
     bem2d_internal_mixed_release(c);
     rocblas_destroy_handle(c->blas);
+    (void)hipStreamSynchronize(c->aux);
+    (void)hipStreamDestroy(c->aux);
+    (void)hipEventDestroy(c->ev_fork);
+    (void)hipEventDestroy(c->ev_join);
     (void)hipStreamDestroy(c->own_stream);
     delete c;
 }
@@ -110,21 +120,21 @@ extern "C" int bem2d_synchronize(bem2d_ctx* c)
 // ------------------------------------------------------------------------------------------------
 // kernel timing
 // ------------------------------------------------------------------------------------------------
-void bem2d_internal_prof_begin(bem2d_ctx* c, int id)
+void bem2d_internal_prof_begin(bem2d_ctx* c, int id, hipStream_t s)
 {
     if (!c->profiling) return;
     hipEvent_t e;
     if (hipEventCreate(&e) != hipSuccess) return;
-    (void)hipEventRecord(e, c->stream);
+    (void)hipEventRecord(e, s ? s : c->stream);
     c->prof[id].pending.push_back(e);
 }
 
-void bem2d_internal_prof_end(bem2d_ctx* c, int id)
+void bem2d_internal_prof_end(bem2d_ctx* c, int id, hipStream_t s)
 {
     if (!c->profiling || c->prof[id].pending.size() % 2 == 0) return;
     hipEvent_t e;
     if (hipEventCreate(&e) != hipSuccess) return;
-    (void)hipEventRecord(e, c->stream);
+    (void)hipEventRecord(e, s ? s : c->stream);
     c->prof[id].pending.push_back(e);
 }
 
@@ -571,21 +581,30 @@ static int launch_assembly(bem2d_ctx* c, hipStream_t s, const AsmProblem* probs,
         if (reg.start[cnt] > (int64_t)INT32_MAX || sp.start[cnt] > (int64_t)INT32_MAX)
             return FAIL(BEM2D_ERR_UNSUPPORTED, "assembly grid too large");
         const size_t shm = sizeof(double) * kColsPerBlock * nq * NODE_PLANES;
+        // The singular-pair kernel writes entries the regular kernel skips: it runs on the second
+        // stream beside the regular kernel (forked from and joined back into s).
+        const bool fork = sp.start[cnt] > 0;
+        if (fork) {
+            if (hipEventRecord(c->ev_fork, s) != hipSuccess ||
+                hipStreamWaitEvent(c->aux, c->ev_fork, 0) != hipSuccess)
+                return FAIL(BEM2D_ERR_DEVICE, "stream fork failed");
+            bem2d_internal_prof_begin(c, BEM2D_PROF_ASSEMBLE_SPECIAL, c->aux);
+            assemble_special_kernel<KIND><<<(unsigned)sp.start[cnt], 256, 0, c->aux>>>(sp);
+            bem2d_internal_prof_end(c, BEM2D_PROF_ASSEMBLE_SPECIAL, c->aux);
+        }
         if (reg.start[cnt] > 0) {
-            bem2d_internal_prof_begin(c, BEM2D_PROF_ASSEMBLE_REGULAR);
+            bem2d_internal_prof_begin(c, BEM2D_PROF_ASSEMBLE_REGULAR, s);
             if (nq == 16)
                 assemble_regular_kernel<KIND, 16>
                     <<<(unsigned)reg.start[cnt], 64 * kWavesPerBlock, shm, s>>>(reg);
             else
                 assemble_regular_kernel<KIND, 0>
                     <<<(unsigned)reg.start[cnt], 64 * kWavesPerBlock, shm, s>>>(reg);
-            bem2d_internal_prof_end(c, BEM2D_PROF_ASSEMBLE_REGULAR);
+            bem2d_internal_prof_end(c, BEM2D_PROF_ASSEMBLE_REGULAR, s);
         }
-        if (sp.start[cnt] > 0) {
-            bem2d_internal_prof_begin(c, BEM2D_PROF_ASSEMBLE_SPECIAL);
-            assemble_special_kernel<KIND><<<(unsigned)sp.start[cnt], 256, 0, s>>>(sp);
-            bem2d_internal_prof_end(c, BEM2D_PROF_ASSEMBLE_SPECIAL);
-        }
+        if (fork && (hipEventRecord(c->ev_join, c->aux) != hipSuccess ||
+                     hipStreamWaitEvent(s, c->ev_join, 0) != hipSuccess))
+            return FAIL(BEM2D_ERR_DEVICE, "stream join failed");
     }
     return hipGetLastError() == hipSuccess ? BEM2D_OK : BEM2D_ERR_DEVICE;
 }

@@ -36,6 +36,8 @@ struct bem2d_ctx {
     int device = 0;
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
+    hipStream_t aux = nullptr;           // second stream: singular-pair assembly beside the regular
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     rocblas_handle blas = nullptr;
     ScratchBuf scratch[SCRATCH_COUNT];
     MixedState* mixed = nullptr;
@@ -44,8 +46,8 @@ struct bem2d_ctx {
 };
 
 // Records a start event for kernel id when profiling is on; returns the event index (or -1).
-void bem2d_internal_prof_begin(bem2d_ctx* c, int id);
-void bem2d_internal_prof_end(bem2d_ctx* c, int id);
+void bem2d_internal_prof_begin(bem2d_ctx* c, int id, hipStream_t s = nullptr);
+void bem2d_internal_prof_end(bem2d_ctx* c, int id, hipStream_t s = nullptr);
 
 struct DevArena;
 
