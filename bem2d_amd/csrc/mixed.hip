@@ -90,8 +90,20 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b)
 }
 __device__ __forceinline__ float2 cfms(float2 acc, float2 a, float2 b) // acc - a b
 {
-    acc.x -= a.x * b.x - a.y * b.y;
-    acc.y -= a.x * b.y + a.y * b.x;
+    // two fused multiply-adds per component (written out: acc - (ab - cd) would keep the inner
+    // difference and cost a third instruction)
+    acc.x = fmaf(-a.x, b.x, acc.x);
+    acc.x = fmaf(a.y, b.y, acc.x);
+    acc.y = fmaf(-a.x, b.y, acc.y);
+    acc.y = fmaf(-a.y, b.x, acc.y);
+    return acc;
+}
+__device__ __forceinline__ float2 cfma(float2 acc, float2 a, float2 b) // acc + a b
+{
+    acc.x = fmaf(a.x, b.x, acc.x);
+    acc.x = fmaf(-a.y, b.y, acc.x);
+    acc.y = fmaf(a.x, b.y, acc.y);
+    acc.y = fmaf(a.y, b.x, acc.y);
     return acc;
 }
 
@@ -615,8 +627,7 @@ __global__ __launch_bounds__(256) void block_trsv_kernel(const RefBatch batch)
                 for (int h = 0; h < kMaxSolveRhs; ++h)
                     if (h < nrhs) {
                         const float2 y = Y[(cb + k0 + m) + ldy * h];
-                        s[h].x += fv[m].x * y.x - fv[m].y * y.y;
-                        s[h].y += fv[m].x * y.y + fv[m].y * y.x;
+                        s[h] = cfma(s[h], fv[m], y);
                     }
             }
         }
@@ -644,8 +655,7 @@ __global__ __launch_bounds__(256) void block_trsv_kernel(const RefBatch batch)
 #pragma unroll
             for (int q = 0; q < 16; ++q) {
                 const float2 a = accs[k0 + q][h];
-                v.x += dv[q].x * a.x - dv[q].y * a.y;
-                v.y += dv[q].x * a.y + dv[q].y * a.x;
+                v = cfma(v, dv[q], a);
             }
         }
         s[h] = v;
