@@ -106,6 +106,18 @@ __device__ __forceinline__ float2 cfma(float2 acc, float2 a, float2 b) // acc + 
     acc.y = fmaf(a.y, b.x, acc.y);
     return acc;
 }
+// Selections on values.  `c ? u : v` between two float2 lvalues selects between their ADDRESSES
+// (a conditional of lvalues is an lvalue), which forces a named constant such as a zero into
+// scratch memory and turns the reads into scratch / flat loads inside the step loops.
+__device__ __forceinline__ float2 csel(bool c, float2 u, float2 v)
+{
+    return make_float2(c ? u.x : v.x, c ? u.y : v.y);
+}
+__device__ __forceinline__ float2 load_or(bool ok, const float2* p, float2 v)
+{
+    if (ok) v = *p;
+    return v;
+}
 
 // ---- cast ---------------------------------------------------------------------------------------
 __global__ void cast_kernel(int64_t n, const double2* __restrict__ A, int64_t lda,
@@ -211,7 +223,7 @@ __device__ __forceinline__ void diag_factor(float2* __restrict__ A, int64_t ld, 
         for (int b = 0; b < 4; ++b) {
             const int r = diag_R(a, l), c = diag_C(b, w, l);
             const float2 e = make_float2(r == c ? 1.f : 0.f, 0.f);
-            T[a][b] = (r < kb && c < kb) ? A[(k0 + r) + ld * (k0 + c)] : e;
+            T[a][b] = load_or(r < kb && c < kb, &A[(k0 + r) + ld * (k0 + c)], e);
             X[a][b] = e;
             Y[a][b] = e;
         }
@@ -254,14 +266,14 @@ __device__ __forceinline__ void diag_factor(float2* __restrict__ A, int64_t ld, 
                 const bool below = a > js || r > j;
                 const float2 cv = S.col[buf][r];
                 const float2 rv = S.row[buf][r];
-                m[a] = below ? cmul(cv, ip) : zero;
-                g[a] = below ? rv : zero;
+                m[a] = csel(below, cmul(cv, ip), zero);
+                g[a] = csel(below, rv, zero);
             }
 #pragma unroll
             for (int b = js; b < 4; ++b) {
                 const int c = diag_C(b, w, l);
                 const float2 rv = S.row[buf][c];
-                u[b] = (b > js || c > j) ? rv : zero;
+                u[b] = csel(b > js || c > j, rv, zero);
             }
 #pragma unroll
             for (int b = 0; b <= js; ++b) {
@@ -607,7 +619,7 @@ __global__ __launch_bounds__(256) void block_trsv_kernel(const RefBatch batch)
         float2 fv[16];
 #pragma unroll
         for (int m = 0; m < 16; ++m)
-            fv[m] = (row < lenb && k0 + m < lenc) ? F[(rb + row) + ld * (cb + k0 + m)] : zero;
+            fv[m] = load_or(row < lenb && k0 + m < lenc, &F[(rb + row) + ld * (cb + k0 + m)], zero);
         if (t == 0) {
             long long spins = 0;
             while (__hip_atomic_load(&flags[c], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
@@ -638,7 +650,7 @@ __global__ __launch_bounds__(256) void block_trsv_kernel(const RefBatch batch)
     __syncthreads();
     for (int e = t; e < NB * nrhs; e += 256) {
         const int r = e % NB, h = e / NB;
-        float2 a = r < lenb ? Y[(rb + r) + ldy * h] : zero;
+        float2 a = load_or(r < lenb, &Y[(rb + r) + ldy * h], zero);
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
             a.x -= part[g][r][h].x;
