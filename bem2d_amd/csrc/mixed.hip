@@ -14,7 +14,7 @@
 //
 // Refinement (bem2d_lu32_refine): x0 = 0, r = b - A64 x in FP64, d = U^{-1} L^{-1} r in FP32
 // (two persistent block-triangular sweeps with the inverted diagonal blocks; blocks hand their
-// solution on through release/acquire flags), x += d in FP64, until the backward error
+// solution on through tagged words), x += d in FP64, until the backward error
 // |b - A x|_inf / (|A|_inf |x|_inf + |b|_inf) <= tol for every right-hand side.
 //
 // Every kernel takes a by-value batch of up to kMaxLuBatch problems, so that several problems of
@@ -553,7 +553,7 @@ struct RefUnit {
     const float2* F;               // FP32 factors (ld n)
     const float2* Linv;
     const float2* UinvT;
-    unsigned long long* flags;     // lower nblk | upper nblk
+    unsigned long long* ty;        // 2 n x nrhs tagged words handed between sweep blocks
     const double* anorm;           // |A|_inf of the factorised matrix
     int* luflags;                  // [0] bad pivot (factorisation), [1] sweep time-out
 };
@@ -574,16 +574,29 @@ struct RefBatch {
 // the transposed inverses UinvT.
 // Each workgroup owns one 64-row block of one unit; a unit's workgroups are consecutive and the
 // first ones dispatched own the blocks without dependencies, so every wait is on a block that is
-// already running or done.  A wait gives up after kSpinLimit polls and raises the unit's time-out
-// flag, so the kernel always terminates.
+// already running or done.
+// A finished block hands its rows on through TY, where every FP32 component sits in a 64-bit word
+// beside the tag of the sweep that wrote it (2 epoch for the lower sweep, 2 epoch + 1 for the upper
+// one).  Words are written and read as relaxed device-scope atomics, so a reader that sees the
+// tag it waits for has the value of the same word: no fence and no flag are needed, and with them
+// go the cache write-back and invalidation that a release / acquire pair costs on every hand-over.
+// A thread gives up waiting after kSpinLimit polls in all and raises the unit's time-out flag, so
+// the kernel always terminates (with wrong values, which the refinement then reports).
 // Thread (row = t % 64, grp = t / 64) owns 16 consecutive k of each 64-wide column block: the
-// matrix block is loaded into registers before the flag wait (it does not depend on it), and the
-// partial sums stay in registers until the end, so a dependency costs one wait and one barrier.
+// matrix block is loaded into registers before the wait (it does not depend on it), the dependency
+// rows go through LDS (double-buffered: one barrier per dependency), and the partial sums stay in
+// registers until the end.
+__device__ __forceinline__ unsigned long long tagged(unsigned tag, float v)
+{
+    return ((unsigned long long)tag << 32) | __float_as_uint(v);
+}
+
 template <bool LOWER>
 __global__ __launch_bounds__(256) void block_trsv_kernel(const RefBatch batch)
 {
     __shared__ float2 part[4][NB][kMaxSolveRhs];
     __shared__ float2 accs[NB][kMaxSolveRhs];
+    __shared__ float ys[2][kMaxSolveRhs][2 * NB];   // dependency rows, (re, im) interleaved
     const int di = find_problem(batch.start, batch.count, blockIdx.x);
     const RefUnit& U = batch.u[di];
     const float2* __restrict__ F = U.F;
@@ -591,8 +604,8 @@ __global__ __launch_bounds__(256) void block_trsv_kernel(const RefBatch batch)
     const int nblk = U.nblk, nrhs = U.nrhs;
     float2* __restrict__ Y = U.Y;
     const int64_t ldy = U.n;
-    unsigned long long* flags = U.flags + (LOWER ? 0 : nblk);
-    const unsigned long long epoch = batch.epoch;
+    unsigned long long* TY = U.ty;
+    const unsigned tag = (unsigned)(2 * batch.epoch + (LOWER ? 0 : 1));
     const int local = (int)((int64_t)blockIdx.x - batch.start[di]);
     const int t = threadIdx.x;
     const int b = LOWER ? local : nblk - 1 - local;
@@ -600,6 +613,7 @@ __global__ __launch_bounds__(256) void block_trsv_kernel(const RefBatch batch)
     const int lenb = (int)min((int64_t)NB, n - rb);
     const int row = t % NB, grp = t / NB, k0 = 16 * grp;
     const float2 zero = make_float2(0.f, 0.f);
+    long long spins = 0;
     float2 s[kMaxSolveRhs];
 #pragma unroll
     for (int h = 0; h < kMaxSolveRhs; ++h) s[h] = zero;
@@ -620,25 +634,41 @@ __global__ __launch_bounds__(256) void block_trsv_kernel(const RefBatch batch)
 #pragma unroll
         for (int m = 0; m < 16; ++m)
             fv[m] = load_or(row < lenb && k0 + m < lenc, &F[(rb + row) + ld * (cb + k0 + m)], zero);
-        if (t == 0) {
-            long long spins = 0;
-            while (__hip_atomic_load(&flags[c], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
-                __builtin_amdgcn_s_sleep(1);
-                if (++spins > kSpinLimit) {
-                    atomicExch(&U.luflags[1], 1);
-                    break;
+        // rows cb .. cb + lenc of every column h: 2 lenc consecutive words at TY + 2 (cb + n h)
+        const int buf = q & 1;
+        // all of this thread's words are requested at once; only those not yet written are polled
+        constexpr int kWords = kMaxSolveRhs / 2;
+        unsigned long long v[kWords];
+#pragma unroll
+        for (int i = 0; i < kWords; ++i) {
+            const int w = t + 256 * i, h = w / (2 * NB), o = w % (2 * NB);
+            v[i] = (unsigned long long)tag << 32;
+            if (h < nrhs && o < 2 * lenc)
+                v[i] = __hip_atomic_load(TY + 2 * (cb + n * h) + o, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+        }
+#pragma unroll
+        for (int i = 0; i < kWords; ++i) {
+            const int w = t + 256 * i, h = w / (2 * NB), o = w % (2 * NB);
+            if (h < nrhs && o < 2 * lenc) {
+                unsigned long long* p = TY + 2 * (cb + n * h) + o;
+                while ((unsigned)(v[i] >> 32) != tag && spins <= kSpinLimit) {
+                    __builtin_amdgcn_s_sleep(1);
+                    if (++spins > kSpinLimit) atomicExch(&U.luflags[1], 1);
+                    v[i] = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 }
+                ys[buf][h][o] = __uint_as_float((unsigned)v[i]);
             }
         }
         __syncthreads();
-        // plain loads: the agent-scope acquire above invalidated this CU's vector L1
 #pragma unroll
         for (int m = 0; m < 16; ++m) {
             if (k0 + m < lenc) {
 #pragma unroll
                 for (int h = 0; h < kMaxSolveRhs; ++h)
                     if (h < nrhs) {
-                        const float2 y = Y[(cb + k0 + m) + ldy * h];
+                        const float2 y = make_float2(ys[buf][h][2 * (k0 + m)],
+                                                     ys[buf][h][2 * (k0 + m) + 1]);
                         s[h] = cfma(s[h], fv[m], y);
                     }
             }
@@ -676,6 +706,8 @@ __global__ __launch_bounds__(256) void block_trsv_kernel(const RefBatch batch)
 #pragma unroll
     for (int h = 0; h < kMaxSolveRhs; ++h) part[grp][row][h] = s[h];
     __syncthreads();
+    // the plain copy serves the next kernel (it reads this block's rows only); the tagged words
+    // serve the blocks of this sweep still waiting
     if (grp == 0 && row < lenb) {
         for (int h = 0; h < nrhs; ++h) {
             float2 v = part[0][row][h];
@@ -684,12 +716,12 @@ __global__ __launch_bounds__(256) void block_trsv_kernel(const RefBatch batch)
                 v.x += part[g][row][h].x;
                 v.y += part[g][row][h].y;
             }
+            unsigned long long* p = TY + 2 * ((rb + row) + n * h);
+            __hip_atomic_store(p, tagged(tag, v.x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(p + 1, tagged(tag, v.y), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             Y[(rb + row) + ldy * h] = v;
         }
     }
-    __threadfence();
-    __syncthreads();
-    if (t == 0) __hip_atomic_store(&flags[b], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ---- FP64 residual, scaling and updates ------------------------------------------------------------
@@ -1177,10 +1209,9 @@ static int ref_chunks(int64_t n)
 size_t bem2d_internal_ref_bytes(int64_t n, int nrhs)
 {
     const int64_t units = (nrhs + kMaxSolveRhs - 1) / kMaxSolveRhs;
-    const int64_t nblk = (n + NB - 1) / NB;
     const size_t per = sizeof(double2) * n * kMaxSolveRhs * (1 + ref_chunks(n)) +
                        sizeof(float2) * n * kMaxSolveRhs + sizeof(double) * kMaxSolveRhs +
-                       sizeof(unsigned long long) * (2 * nblk + kNormSlots) + 6 * 256;
+                       sizeof(unsigned long long) * (2 * n * kMaxSolveRhs + kNormSlots) + 6 * 256;
     return units * per;
 }
 
@@ -1232,7 +1263,7 @@ int bem2d_internal_ref_create(bem2d_ctx* c, DevArena* arena, const RefSpec* spec
             const size_t bytes = sizeof(double2) * n * kMaxSolveRhs * (1 + u.chunks) +
                                  sizeof(float2) * n * kMaxSolveRhs +
                                  sizeof(double) * kMaxSolveRhs +
-                                 sizeof(unsigned long long) * 2 * u.nblk + 4 * 256;
+                                 sizeof(unsigned long long) * 2 * n * kMaxSolveRhs + 4 * 256;
             char* base = (char*)bem2d_internal_dalloc(sp.ws ? sp.ws : arena, bytes);
             if (!base) return fail(BEM2D_ERR_NOMEM, "workspace allocation failed");
             if (!sp.ws) j->blocks.push_back(base); // a spec's own arena is released by its owner
@@ -1245,7 +1276,7 @@ int bem2d_internal_ref_create(bem2d_ctx* c, DevArena* arena, const RefSpec* spec
             u.part = (double2*)carve(sizeof(double2) * n * kMaxSolveRhs * u.chunks);
             u.Y = (float2*)carve(sizeof(float2) * n * kMaxSolveRhs);
             u.scale = (double*)carve(sizeof(double) * kMaxSolveRhs);
-            u.flags = (unsigned long long*)carve(sizeof(unsigned long long) * 2 * u.nblk);
+            u.ty = (unsigned long long*)carve(sizeof(unsigned long long) * 2 * n * kMaxSolveRhs);
             j->units.push_back(u);
             j->unit_spec.push_back(sidx);
             j->tol.push_back(sp.tol);
@@ -1305,8 +1336,8 @@ int bem2d_internal_ref_step(RefJob* j, hipStream_t s)
         zero_u64_kernel<<<(unsigned)((kNormSlots * nu + 255) / 256), 256, 0, s>>>(
             j->dnorms, (int64_t)(kNormSlots * nu));
         for (size_t u = 0; u < nu; ++u)
-            zero_u64_kernel<<<(unsigned)((2 * j->units[u].nblk + 255) / 256), 256, 0, s>>>(
-                j->units[u].flags, 2 * j->units[u].nblk);
+            zero_u64_kernel<<<(unsigned)((2 * j->units[u].n * kMaxSolveRhs + 255) / 256), 256, 0,
+                              s>>>(j->units[u].ty, 2 * j->units[u].n * kMaxSolveRhs);
         for (size_t c0 = 0; c0 < ids.size(); c0 += kMaxLuBatch) {
             RefBatch bt = ref_batch(j, ids, c0, 0);
             ref_init_kernel<<<dim3((unsigned)bt.start[bt.count], kMaxSolveRhs), 256, 0, s>>>(bt);
