@@ -582,21 +582,39 @@ struct RefBatch {
 // go the cache write-back and invalidation that a release / acquire pair costs on every hand-over.
 // A thread gives up waiting after kSpinLimit polls in all and raises the unit's time-out flag, so
 // the kernel always terminates (with wrong values, which the refinement then reports).
-// Thread (row = t % 64, grp = t / 64) owns 16 consecutive k of each 64-wide column block: the
-// matrix block is loaded into registers before the wait (it does not depend on it), the dependency
-// rows go through LDS (double-buffered: one barrier per dependency), and the partial sums stay in
-// registers until the end.
+// Thread t = 4 row + kq owns row `row` of the block and the 16 columns 16 kq .. 16 kq + 15 of each
+// 64-wide column block; the four threads of a row are adjacent lanes and combine their partial
+// sums with lane exchanges.  The matrix block is loaded into registers before the wait (it does
+// not depend on it), the dependency rows go through LDS (double-buffered: one barrier per
+// dependency), and the block's own right-hand side is read at the start, so that after the last
+// dependency arrives only one more barrier stands between it and the block's result.  The products
+// are summed apart from the right-hand side and subtracted once at the end: folding them into an
+// accumulator that starts at y_b loses accuracy to cancellation (measured: a first correction about
+// 3.5 times less accurate on the kite problem).
 __device__ __forceinline__ unsigned long long tagged(unsigned tag, float v)
 {
     return ((unsigned long long)tag << 32) | __float_as_uint(v);
 }
 
+// LDS column of 64 complex rows with one complex of padding after every 16 rows, so that the four
+// 16-row quarters read together by a wavefront fall in different banks
+constexpr int kPadCol = 2 * NB + 8; // floats
+__device__ __forceinline__ int pad_row(int r) { return 2 * r + 2 * (r >> 4); }
+
+__device__ __forceinline__ float2 quad_sum(float2 v)
+{
+    v.x += __shfl_xor(v.x, 1);
+    v.y += __shfl_xor(v.y, 1);
+    v.x += __shfl_xor(v.x, 2);
+    v.y += __shfl_xor(v.y, 2);
+    return v;
+}
+
 template <bool LOWER>
 __global__ __launch_bounds__(256) void block_trsv_kernel(const RefBatch batch)
 {
-    __shared__ float2 part[4][NB][kMaxSolveRhs];
-    __shared__ float2 accs[NB][kMaxSolveRhs];
-    __shared__ float ys[2][kMaxSolveRhs][2 * NB];   // dependency rows, (re, im) interleaved
+    __shared__ float ys[2][kMaxSolveRhs][kPadCol]; // dependency rows, (re, im) interleaved
+    __shared__ float accs[kMaxSolveRhs][kPadCol];  // this block's reduced right-hand side
     const int di = find_problem(batch.start, batch.count, blockIdx.x);
     const RefUnit& U = batch.u[di];
     const float2* __restrict__ F = U.F;
@@ -611,12 +629,16 @@ __global__ __launch_bounds__(256) void block_trsv_kernel(const RefBatch batch)
     const int b = LOWER ? local : nblk - 1 - local;
     const int64_t rb = (int64_t)b * NB;
     const int lenb = (int)min((int64_t)NB, n - rb);
-    const int row = t % NB, grp = t / NB, k0 = 16 * grp;
+    const int row = t >> 2, kq = t & 3, k0 = 16 * kq;
     const float2 zero = make_float2(0.f, 0.f);
     long long spins = 0;
-    float2 s[kMaxSolveRhs];
+    // this block's right-hand side (written by the previous kernel)
+    float2 s[kMaxSolveRhs], yb[kMaxSolveRhs];
 #pragma unroll
-    for (int h = 0; h < kMaxSolveRhs; ++h) s[h] = zero;
+    for (int h = 0; h < kMaxSolveRhs; ++h) {
+        yb[h] = load_or(h < nrhs && row < lenb, &Y[(rb + row) + ldy * h], zero);
+        s[h] = zero;
+    }
     // the diagonal-block inverse, this thread's 16 columns of row `row` (the upper sweep's
     // inverses are stored transposed)
     float2 dv[16];
@@ -634,9 +656,9 @@ __global__ __launch_bounds__(256) void block_trsv_kernel(const RefBatch batch)
 #pragma unroll
         for (int m = 0; m < 16; ++m)
             fv[m] = load_or(row < lenb && k0 + m < lenc, &F[(rb + row) + ld * (cb + k0 + m)], zero);
-        // rows cb .. cb + lenc of every column h: 2 lenc consecutive words at TY + 2 (cb + n h)
+        // rows cb .. cb + lenc of every column h: 2 lenc consecutive words at TY + 2 (cb + n h);
+        // all of this thread's words are requested at once, only those not yet written are polled
         const int buf = q & 1;
-        // all of this thread's words are requested at once; only those not yet written are polled
         constexpr int kWords = kMaxSolveRhs / 2;
         unsigned long long v[kWords];
 #pragma unroll
@@ -657,7 +679,7 @@ __global__ __launch_bounds__(256) void block_trsv_kernel(const RefBatch batch)
                     if (++spins > kSpinLimit) atomicExch(&U.luflags[1], 1);
                     v[i] = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 }
-                ys[buf][h][o] = __uint_as_float((unsigned)v[i]);
+                ys[buf][h][pad_row(o >> 1) + (o & 1)] = __uint_as_float((unsigned)v[i]);
             }
         }
         __syncthreads();
@@ -667,59 +689,42 @@ __global__ __launch_bounds__(256) void block_trsv_kernel(const RefBatch batch)
 #pragma unroll
                 for (int h = 0; h < kMaxSolveRhs; ++h)
                     if (h < nrhs) {
-                        const float2 y = make_float2(ys[buf][h][2 * (k0 + m)],
-                                                     ys[buf][h][2 * (k0 + m) + 1]);
-                        s[h] = cfma(s[h], fv[m], y);
+                        const float* yc = &ys[buf][h][pad_row(k0 + m)];
+                        s[h] = cfma(s[h], fv[m], make_float2(yc[0], yc[1]));
                     }
             }
         }
     }
-    // acc = y_b - sum_c F_bc y_c  (rows of this block)
-#pragma unroll
-    for (int h = 0; h < kMaxSolveRhs; ++h) part[grp][row][h] = s[h];
-    __syncthreads();
-    for (int e = t; e < NB * nrhs; e += 256) {
-        const int r = e % NB, h = e / NB;
-        float2 a = load_or(r < lenb, &Y[(rb + r) + ldy * h], zero);
-#pragma unroll
-        for (int g = 0; g < 4; ++g) {
-            a.x -= part[g][r][h].x;
-            a.y -= part[g][r][h].y;
-        }
-        accs[r][h] = a;
-    }
-    __syncthreads();
-    // y_b = Dinv_b acc: partial over this thread's 16 k, then summed over the 4 groups
+    // acc = y_b - sum_c F_bc y_c for row `row`, then y_b = Dinv_b acc
 #pragma unroll
     for (int h = 0; h < kMaxSolveRhs; ++h) {
-        float2 v = zero;
+        const float2 p = quad_sum(s[h]);
+        const float2 a = make_float2(yb[h].x - p.x, yb[h].y - p.y);
+        if (kq == 0 && h < nrhs) {
+            accs[h][pad_row(row)] = a.x;
+            accs[h][pad_row(row) + 1] = a.y;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int h = 0; h < kMaxSolveRhs; ++h) {
         if (h < nrhs) {
+            float2 v = zero;
 #pragma unroll
             for (int q = 0; q < 16; ++q) {
-                const float2 a = accs[k0 + q][h];
-                v = cfma(v, dv[q], a);
+                const float* ac = &accs[h][pad_row(k0 + q)];
+                v = cfma(v, dv[q], make_float2(ac[0], ac[1]));
             }
-        }
-        s[h] = v;
-    }
-    __syncthreads(); // part[] is reused
-#pragma unroll
-    for (int h = 0; h < kMaxSolveRhs; ++h) part[grp][row][h] = s[h];
-    __syncthreads();
-    // the plain copy serves the next kernel (it reads this block's rows only); the tagged words
-    // serve the blocks of this sweep still waiting
-    if (grp == 0 && row < lenb) {
-        for (int h = 0; h < nrhs; ++h) {
-            float2 v = part[0][row][h];
-#pragma unroll
-            for (int g = 1; g < 4; ++g) {
-                v.x += part[g][row][h].x;
-                v.y += part[g][row][h].y;
+            v = quad_sum(v);
+            // the plain copy serves the next kernel (it reads this block's rows only); the tagged
+            // words serve the blocks of this sweep still waiting
+            if (kq == 0 && row < lenb) {
+                unsigned long long* p = TY + 2 * ((rb + row) + n * h);
+                __hip_atomic_store(p, tagged(tag, v.x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(p + 1, tagged(tag, v.y), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+                Y[(rb + row) + ldy * h] = v;
             }
-            unsigned long long* p = TY + 2 * ((rb + row) + n * h);
-            __hip_atomic_store(p, tagged(tag, v.x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(p + 1, tagged(tag, v.y), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            Y[(rb + row) + ldy * h] = v;
         }
     }
 }
