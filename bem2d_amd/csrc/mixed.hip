@@ -601,20 +601,22 @@ __device__ __forceinline__ unsigned long long tagged(unsigned tag, float v)
 constexpr int kPadCol = 2 * NB + 8; // floats
 __device__ __forceinline__ int pad_row(int r) { return 2 * r + 2 * (r >> 4); }
 
-__device__ __forceinline__ float2 quad_sum(float2 v)
+// Sum over each group of four adjacent lanes: lane ^ 1, then lane ^ 2, as DPP quad permutations
+// (register-to-register; a shuffle would go through the LDS crossbar).
+__device__ __forceinline__ float quad_add(float v)
 {
-    v.x += __shfl_xor(v.x, 1);
-    v.y += __shfl_xor(v.y, 1);
-    v.x += __shfl_xor(v.x, 2);
-    v.y += __shfl_xor(v.y, 2);
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, false));
     return v;
 }
+__device__ __forceinline__ float2 quad_sum(float2 v) { return make_float2(quad_add(v.x), quad_add(v.y)); }
 
 template <bool LOWER>
 __global__ __launch_bounds__(256) void block_trsv_kernel(const RefBatch batch)
 {
-    __shared__ float ys[2][kMaxSolveRhs][kPadCol]; // dependency rows, (re, im) interleaved
-    __shared__ float accs[kMaxSolveRhs][kPadCol];  // this block's reduced right-hand side
+    __shared__ __align__(16) float ys[2][kMaxSolveRhs][kPadCol]; // dependency rows, (re, im)
+    __shared__ __align__(16) float accs[kMaxSolveRhs][kPadCol];  // this block's reduced rhs
+    __shared__ __align__(16) float yl[kMaxSolveRhs][kPadCol];    // this block's result
     const int di = find_problem(batch.start, batch.count, blockIdx.x);
     const RefUnit& U = batch.u[di];
     const float2* __restrict__ F = U.F;
@@ -656,42 +658,48 @@ __global__ __launch_bounds__(256) void block_trsv_kernel(const RefBatch batch)
 #pragma unroll
         for (int m = 0; m < 16; ++m)
             fv[m] = load_or(row < lenb && k0 + m < lenc, &F[(rb + row) + ld * (cb + k0 + m)], zero);
-        // rows cb .. cb + lenc of every column h: 2 lenc consecutive words at TY + 2 (cb + n h);
-        // all of this thread's words are requested at once, only those not yet written are polled
+        // rows cb .. cb + lenc of every column h: 2 lenc consecutive words at TY + 2 (cb + n h).
+        // One wavefront fetches them all (requested at once; only those not yet written are
+        // polled): with a single poller the wait ends on the first poll that finds the block
+        // written, not on the last of several pollers.
         const int buf = q & 1;
-        constexpr int kWords = kMaxSolveRhs / 2;
-        unsigned long long v[kWords];
+        if (t < 64) {
+            constexpr int kWords = 2 * kMaxSolveRhs;
+            unsigned long long v[kWords];
 #pragma unroll
-        for (int i = 0; i < kWords; ++i) {
-            const int w = t + 256 * i, h = w / (2 * NB), o = w % (2 * NB);
-            v[i] = (unsigned long long)tag << 32;
-            if (h < nrhs && o < 2 * lenc)
-                v[i] = __hip_atomic_load(TY + 2 * (cb + n * h) + o, __ATOMIC_RELAXED,
-                                         __HIP_MEMORY_SCOPE_AGENT);
-        }
+            for (int i = 0; i < kWords; ++i) {
+                const int w = t + 64 * i, h = w / (2 * NB), o = w % (2 * NB);
+                v[i] = (unsigned long long)tag << 32;
+                if (h < nrhs && o < 2 * lenc)
+                    v[i] = __hip_atomic_load(TY + 2 * (cb + n * h) + o, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+            }
 #pragma unroll
-        for (int i = 0; i < kWords; ++i) {
-            const int w = t + 256 * i, h = w / (2 * NB), o = w % (2 * NB);
-            if (h < nrhs && o < 2 * lenc) {
-                unsigned long long* p = TY + 2 * (cb + n * h) + o;
-                while ((unsigned)(v[i] >> 32) != tag && spins <= kSpinLimit) {
-                    __builtin_amdgcn_s_sleep(1);
-                    if (++spins > kSpinLimit) atomicExch(&U.luflags[1], 1);
-                    v[i] = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            for (int i = 0; i < kWords; ++i) {
+                const int w = t + 64 * i, h = w / (2 * NB), o = w % (2 * NB);
+                if (h < nrhs && o < 2 * lenc) {
+                    unsigned long long* p = TY + 2 * (cb + n * h) + o;
+                    while ((unsigned)(v[i] >> 32) != tag && spins <= kSpinLimit) {
+                        __builtin_amdgcn_s_sleep(1);
+                        if (++spins > kSpinLimit) atomicExch(&U.luflags[1], 1);
+                        v[i] = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    }
                 }
-                ys[buf][h][pad_row(o >> 1) + (o & 1)] = __uint_as_float((unsigned)v[i]);
+                // rows past the end of block c read as zero (their matrix entries are zero too)
+                if (h < nrhs) ys[buf][h][pad_row(o >> 1) + (o & 1)] = __uint_as_float((unsigned)v[i]);
             }
         }
         __syncthreads();
+        // straight-line per column: the 16 LDS reads issue together, then the products
 #pragma unroll
-        for (int m = 0; m < 16; ++m) {
-            if (k0 + m < lenc) {
+        for (int h = 0; h < kMaxSolveRhs; ++h) {
+            if (h < nrhs) {
+                const float2* yc = reinterpret_cast<const float2*>(&ys[buf][h][0]);
+                float2 yv[16];
 #pragma unroll
-                for (int h = 0; h < kMaxSolveRhs; ++h)
-                    if (h < nrhs) {
-                        const float* yc = &ys[buf][h][pad_row(k0 + m)];
-                        s[h] = cfma(s[h], fv[m], make_float2(yc[0], yc[1]));
-                    }
+                for (int m = 0; m < 16; ++m) yv[m] = yc[pad_row(k0 + m) >> 1];
+#pragma unroll
+                for (int m = 0; m < 16; ++m) s[h] = cfma(s[h], fv[m], yv[m]);
             }
         }
     }
@@ -709,21 +717,35 @@ __global__ __launch_bounds__(256) void block_trsv_kernel(const RefBatch batch)
 #pragma unroll
     for (int h = 0; h < kMaxSolveRhs; ++h) {
         if (h < nrhs) {
+            const float2* ac = reinterpret_cast<const float2*>(&accs[h][0]);
+            float2 av[16];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) av[q] = ac[pad_row(k0 + q) >> 1];
             float2 v = zero;
 #pragma unroll
-            for (int q = 0; q < 16; ++q) {
-                const float* ac = &accs[h][pad_row(k0 + q)];
-                v = cfma(v, dv[q], make_float2(ac[0], ac[1]));
-            }
+            for (int q = 0; q < 16; ++q) v = cfma(v, dv[q], av[q]);
             v = quad_sum(v);
-            // the plain copy serves the next kernel (it reads this block's rows only); the tagged
-            // words serve the blocks of this sweep still waiting
-            if (kq == 0 && row < lenb) {
-                unsigned long long* p = TY + 2 * ((rb + row) + n * h);
-                __hip_atomic_store(p, tagged(tag, v.x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_store(p + 1, tagged(tag, v.y), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-                Y[(rb + row) + ldy * h] = v;
+            // the plain copy serves the next kernel (it reads this block's rows only)
+            if (kq == 0) {
+                yl[h][pad_row(row)] = v.x;
+                yl[h][pad_row(row) + 1] = v.y;
+                if (row < lenb) Y[(rb + row) + ldy * h] = v;
+            }
+        }
+    }
+    // The tagged words serve the blocks of this sweep still waiting.  One wavefront writes them
+    // all, back to back: a reader's poll then rarely finds only part of the block written (each
+    // word it misses costs it another round trip to memory).
+    __syncthreads();
+    if (t < 64) {
+        for (int h = 0; h < nrhs; ++h) {
+#pragma unroll
+            for (int half = 0; half < 2; ++half) {
+                const int o = t + 64 * half, r = o >> 1;
+                if (r < lenb)
+                    __hip_atomic_store(TY + 2 * (rb + n * h) + o,
+                                       tagged(tag, yl[h][pad_row(r) + (o & 1)]), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
             }
         }
     }
