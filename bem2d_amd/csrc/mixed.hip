@@ -58,6 +58,8 @@ struct LuProblem {
     int64_t k0;             // first row / column of the current diagonal block
     int kb;                 // its size
     int tiles;              // 64-wide tiles of the trailing matrix
+    int ntile;              // output tiles of the trailing update
+    int nwg;                // its persistent workgroups
     int mode;               // trailing update: LU_UPDATE_FULL or LU_UPDATE_CROSS
     const float2* Linv;     // inverses of the current diagonal block (panel solves)
     const float2* UinvT;
@@ -113,10 +115,14 @@ __device__ __forceinline__ float2 csel(bool c, float2 u, float2 v)
 {
     return make_float2(c ? u.x : v.x, c ? u.y : v.y);
 }
-__device__ __forceinline__ float2 load_or(bool ok, const float2* p, float2 v)
+// A guarded load without a branch: the load always issues (from `safe`, a valid element, when !ok)
+// and the value is selected afterwards.  With a branch around each load the compiler may copy the
+// result inside the branch and wait for the load there, which turns a batch of guarded loads into
+// one memory round trip each.
+__device__ __forceinline__ float2 load_or(bool ok, const float2* p, const float2* safe, float2 v)
 {
-    if (ok) v = *p;
-    return v;
+    const float2 x = *(ok ? p : safe);
+    return csel(ok, x, v);
 }
 
 // ---- cast ---------------------------------------------------------------------------------------
@@ -223,7 +229,7 @@ __device__ __forceinline__ void diag_factor(float2* __restrict__ A, int64_t ld, 
         for (int b = 0; b < 4; ++b) {
             const int r = diag_R(a, l), c = diag_C(b, w, l);
             const float2 e = make_float2(r == c ? 1.f : 0.f, 0.f);
-            T[a][b] = load_or(r < kb && c < kb, &A[(k0 + r) + ld * (k0 + c)], e);
+            T[a][b] = load_or(r < kb && c < kb, &A[(k0 + r) + ld * (k0 + c)], A, e);
             X[a][b] = e;
             Y[a][b] = e;
         }
@@ -421,22 +427,45 @@ __global__ __launch_bounds__(256) void panel_trsm_kernel(const LuBatch batch)
     const bool rowtile = bid < nrow_tiles;
     const int64_t base = rest0 + (int64_t)(rowtile ? bid : bid - nrow_tiles) * NB;
     const int len = (int)min((int64_t)NB, n - base);
-    for (int e = t; e < NB * NB; e += 256) {
-        const int x = e % NB, y = e / NB;
-        float2 a, b;
-        if (rowtile) { // out = S U^{-1}: Aop[x][y] = S[x][y], Bop[x][y] = U^{-1}[y][x] = UinvT[x][y]
-            a = (x < len && y < kb) ? A[(base + x) + ld * (k0 + y)] : make_float2(0.f, 0.f);
-            b = UinvT[x + NB * y];
-            Bre[x * KP + y] = b.x;
-            Bim[x * KP + y] = b.y;
-        } else {       // out = L^{-1} S: Aop[x][y] = Linv[x][y], Bop[y][x] = S[x][y]
-            a = Linv[x + NB * y];
-            b = (x < kb && y < len) ? A[(k0 + x) + ld * (base + y)] : make_float2(0.f, 0.f);
-            Bre[y * KP + x] = b.x;
-            Bim[y * KP + x] = b.y;
+    // all 32 loads of a thread are requested before the first LDS write; the zero padding of a
+    // short tile is applied when the values are written (a select right after each load would
+    // wait for it there)
+    constexpr int kPer = NB * NB / 256;
+    float2 va[kPer], vb[kPer];
+    if (rowtile) { // out = S U^{-1}: Aop[x][y] = S[x][y], Bop[x][y] = U^{-1}[y][x] = UinvT[x][y]
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) {
+            const int e = t + 256 * i, x = e % NB, y = e / NB;
+            const bool ok = x < len && y < kb;
+            va[i] = *(ok ? &A[(base + x) + ld * (k0 + y)] : A);
+            vb[i] = UinvT[x + NB * y];
         }
-        Are[x * KP + y] = a.x;
-        Aim[x * KP + y] = a.y;
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) {
+            const int e = t + 256 * i, x = e % NB, y = e / NB;
+            const bool ok = x < len && y < kb;
+            Are[x * KP + y] = ok ? va[i].x : 0.f;
+            Aim[x * KP + y] = ok ? va[i].y : 0.f;
+            Bre[x * KP + y] = vb[i].x;
+            Bim[x * KP + y] = vb[i].y;
+        }
+    } else {       // out = L^{-1} S: Aop[x][y] = Linv[x][y], Bop[y][x] = S[x][y]
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) {
+            const int e = t + 256 * i, x = e % NB, y = e / NB;
+            const bool ok = x < kb && y < len;
+            va[i] = Linv[x + NB * y];
+            vb[i] = *(ok ? &A[(k0 + x) + ld * (base + y)] : A);
+        }
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) {
+            const int e = t + 256 * i, x = e % NB, y = e / NB;
+            const bool ok = x < kb && y < len;
+            Are[x * KP + y] = va[i].x;
+            Aim[x * KP + y] = va[i].y;
+            Bre[y * KP + x] = ok ? vb[i].x : 0.f;
+            Bim[y * KP + x] = ok ? vb[i].y : 0.f;
+        }
     }
     __syncthreads();
     TileAcc c;
@@ -460,10 +489,24 @@ __global__ __launch_bounds__(256) void panel_trsm_kernel(const LuBatch batch)
 
 // ---- trailing update on MFMA, fused with the next diagonal block ---------------------------------
 // C[i][j] -= sum_k L[i][k] U[k][j] for i, j in [rest0, n), k in [k0, k0 + K), K = kb = 64 or 128
-// (two staged chunks of 64); workgroup = one 64 x 64 output tile of the full grid, or of its
-// cross (first block row and column) for LU_UPDATE_CROSS.  Workgroup 0 owns the next diagonal
-// block: after its update it goes on to factorise that block (when Linv_next is given),
-// overlapping the rest of the update.
+// (two staged chunks of 64), over 64 x 64 output tiles of the full grid, or of its cross (first
+// block row and column) for LU_UPDATE_CROSS.  The workgroups are persistent (about one per compute
+// unit; the kernel's register use allows one workgroup per unit anyway) and walk their tiles
+// software-pipelined: while the products of one 64-deep chunk run, the operands of the next chunk
+// are already in flight to registers, and a tile's output block is read when the tile starts.
+// Workgroup 0 of a fused update (Linv_next given) takes tile 0 alone -- the next diagonal block --
+// and goes on to factorise it, overlapping the rest of the update.
+__device__ __forceinline__ void update_tile_xy(const LuProblem& pb, int idx, int& bx, int& by)
+{
+    if (pb.mode == LU_UPDATE_CROSS) {
+        bx = idx < pb.tiles ? idx : 0;
+        by = idx < pb.tiles ? 0 : idx - pb.tiles + 1;
+    } else {
+        bx = idx % pb.tiles;
+        by = idx / pb.tiles;
+    }
+}
+
 __global__ __launch_bounds__(256) void trailing_update_kernel(const LuBatch batch)
 {
     const int di = find_problem(batch.start, batch.count, blockIdx.x);
@@ -471,14 +514,12 @@ __global__ __launch_bounds__(256) void trailing_update_kernel(const LuBatch batc
     float2* __restrict__ A = pb.F;
     const int64_t ld = pb.n, n = pb.n, k0 = pb.k0;
     const int K = pb.kb;
-    const int bid = (int)((int64_t)blockIdx.x - batch.start[di]);
-    int bx, by;
-    if (pb.mode == LU_UPDATE_CROSS) {
-        bx = bid < pb.tiles ? bid : 0;
-        by = bid < pb.tiles ? 0 : bid - pb.tiles + 1;
-    } else {
-        bx = bid % pb.tiles;
-        by = bid / pb.tiles;
+    const int w = (int)((int64_t)blockIdx.x - batch.start[di]);
+    const bool fused = pb.Linv_next != nullptr;
+    int first = w, stride = pb.nwg;
+    if (fused) {
+        first = w;
+        stride = w == 0 ? pb.ntile : pb.nwg - 1;
     }
     __shared__ __align__(16) float lds[kTileLdsFloats];
     static_assert(sizeof(DiagLds) <= sizeof(float) * kTileLdsFloats, "LDS reuse");
@@ -488,44 +529,78 @@ __global__ __launch_bounds__(256) void trailing_update_kernel(const LuBatch batc
     float* Bim = lds + 3 * NB * KP;
     const int t = threadIdx.x;
     const int64_t rest0 = k0 + K;
-    const int64_t r0 = rest0 + (int64_t)bx * NB;
-    const int64_t c0 = rest0 + (int64_t)by * NB;
-    TileAcc acc;
-    acc_zero(acc);
-    for (int kc = 0; kc < K; kc += NB) {
+    constexpr int kPer = NB * NB / 256;
+    // Operand element e = t + 256 i of a chunk: A chunk row e % 64, column e / 64; B chunk row
+    // e % 64 of column e / 64 (both walk the column-major matrix down its columns: coalesced).
+    // Out-of-range elements are loaded from A[0] and zeroed when written to LDS (a select right
+    // after a load would wait for it there).
+    float2 va[kPer], vb[kPer];
+    auto fetch = [&](int idx, int kc) __attribute__((always_inline)) {
+        int bx, by;
+        update_tile_xy(pb, idx, bx, by);
+        const int64_t r0 = rest0 + (int64_t)bx * NB, c0 = rest0 + (int64_t)by * NB;
         const int kn = min(NB, K - kc);
-        if (kc > 0) __syncthreads(); // the previous chunk's operand reads are done
-        for (int e = t; e < NB * NB; e += 256) {
-            const int r = e % NB, k = e / NB;
-            float2 v = make_float2(0.f, 0.f);
-            if (r0 + r < n && k < kn) v = A[(r0 + r) + ld * (k0 + kc + k)];
-            Are[r * KP + k] = v.x;
-            Aim[r * KP + k] = v.y;
-            float2 u = make_float2(0.f, 0.f);
-            const int kk = e % NB, c = e / NB;
-            if (c0 + c < n && kk < kn) u = A[(k0 + kc + kk) + ld * (c0 + c)];
-            Bre[c * KP + kk] = u.x;
-            Bim[c * KP + kk] = u.y;
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) {
+            const int e = t + 256 * i, x = e % NB, y = e / NB;
+            va[i] = *(r0 + x < n && y < kn ? &A[(r0 + x) + ld * (k0 + kc + y)] : A);
+            vb[i] = *(c0 + y < n && x < kn ? &A[(k0 + kc + x) + ld * (c0 + y)] : A);
         }
-        __syncthreads();
-        cgemm_64(Are, Aim, Bre, Bim, acc);
-    }
+    };
+    int idx = first;
+    if (idx < pb.ntile) fetch(idx, 0);
+    while (idx < pb.ntile) {
+        int bx, by;
+        update_tile_xy(pb, idx, bx, by);
+        const int64_t r0 = rest0 + (int64_t)bx * NB, c0 = rest0 + (int64_t)by * NB;
+        // the output block, read now and used after the products
+        float2 cv[2][2][4];
 #pragma unroll
-    for (int p = 0; p < 2; ++p)
+        for (int p = 0; p < 2; ++p)
 #pragma unroll
-        for (int q = 0; q < 2; ++q)
+            for (int q = 0; q < 2; ++q)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int64_t i = r0 + acc_row(p, r);
-                const int64_t j = c0 + acc_col(q);
-                if (i < n && j < n) {
-                    float2 v = A[i + ld * j];
-                    v.x -= acc.re[p][q][r];
-                    v.y -= acc.im[p][q][r];
-                    A[i + ld * j] = v;
+                for (int r = 0; r < 4; ++r) {
+                    const int64_t i = r0 + acc_row(p, r), j = c0 + acc_col(q);
+                    cv[p][q][r] = *(i < n && j < n ? &A[i + ld * j] : A);
                 }
+        TileAcc acc;
+        acc_zero(acc);
+        for (int kc = 0; kc < K; kc += NB) {
+            const int kn = min(NB, K - kc);
+            __syncthreads(); // the previous chunk's operand reads are done
+#pragma unroll
+            for (int i = 0; i < kPer; ++i) {
+                const int e = t + 256 * i, x = e % NB, y = e / NB;
+                const bool oka = r0 + x < n && y < kn, okb = c0 + y < n && x < kn;
+                Are[x * KP + y] = oka ? va[i].x : 0.f;
+                Aim[x * KP + y] = oka ? va[i].y : 0.f;
+                Bre[y * KP + x] = okb ? vb[i].x : 0.f;
+                Bim[y * KP + x] = okb ? vb[i].y : 0.f;
             }
-    if (pb.Linv_next && bid == 0) {
+            __syncthreads();
+            // the next chunk (of this tile or of the next one) goes in flight under the products
+            if (kc + NB < K)
+                fetch(idx, kc + NB);
+            else if (idx + stride < pb.ntile)
+                fetch(idx + stride, 0);
+            cgemm_64(Are, Aim, Bre, Bim, acc);
+        }
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+            for (int q = 0; q < 2; ++q)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int64_t i = r0 + acc_row(p, r);
+                    const int64_t j = c0 + acc_col(q);
+                    if (i < n && j < n)
+                        A[i + ld * j] = make_float2(cv[p][q][r].x - acc.re[p][q][r],
+                                                    cv[p][q][r].y - acc.im[p][q][r]);
+                }
+        idx += stride;
+    }
+    if (fused && w == 0) {
         __syncthreads(); // the tile's stores are visible to the workgroup; the LDS is free
         diag_factor(A, ld, rest0, (int)min((int64_t)NB, n - rest0), pb.Linv_next, pb.UinvT_next,
                     pb.bad, *reinterpret_cast<DiagLds*>(lds));
@@ -638,7 +713,7 @@ __global__ __launch_bounds__(256) void block_trsv_kernel(const RefBatch batch)
     float2 s[kMaxSolveRhs], yb[kMaxSolveRhs];
 #pragma unroll
     for (int h = 0; h < kMaxSolveRhs; ++h) {
-        yb[h] = load_or(h < nrhs && row < lenb, &Y[(rb + row) + ldy * h], zero);
+        yb[h] = load_or(h < nrhs && row < lenb, &Y[(rb + row) + ldy * h], Y, zero);
         s[h] = zero;
     }
     // the diagonal-block inverse, this thread's 16 columns of row `row` (the upper sweep's
@@ -655,9 +730,11 @@ __global__ __launch_bounds__(256) void block_trsv_kernel(const RefBatch batch)
         const int64_t cb = (int64_t)c * NB;
         const int lenc = (int)min((int64_t)NB, n - cb);
         float2 fv[16];
+        // (out-of-range entries are loaded from F[0] and zeroed where they are used: a select
+        // here would wait for each load before the dependency rows are requested)
 #pragma unroll
         for (int m = 0; m < 16; ++m)
-            fv[m] = load_or(row < lenb && k0 + m < lenc, &F[(rb + row) + ld * (cb + k0 + m)], zero);
+            fv[m] = *(row < lenb && k0 + m < lenc ? &F[(rb + row) + ld * (cb + k0 + m)] : F);
         // rows cb .. cb + lenc of every column h: 2 lenc consecutive words at TY + 2 (cb + n h).
         // One wavefront fetches them all (requested at once; only those not yet written are
         // polled): with a single poller the wait ends on the first poll that finds the block
@@ -699,7 +776,8 @@ __global__ __launch_bounds__(256) void block_trsv_kernel(const RefBatch batch)
 #pragma unroll
                 for (int m = 0; m < 16; ++m) yv[m] = yc[pad_row(k0 + m) >> 1];
 #pragma unroll
-                for (int m = 0; m < 16; ++m) s[h] = cfma(s[h], fv[m], yv[m]);
+                for (int m = 0; m < 16; ++m)
+                    s[h] = cfma(s[h], csel(row < lenb && k0 + m < lenc, fv[m], zero), yv[m]);
             }
         }
     }
@@ -1015,6 +1093,7 @@ struct bem2d_lu32 {
     int step = -1;                // next panel step; nblk - 1 when factorised; -1 before
     bool pending_wide = false;    // the last update covered the cross only; the next is depth 128
     bool factored = false;
+    int cus = 256;                // compute units of the device (persistent update workgroups)
     float2* F = nullptr;          // n x n factors, ld = n
     float2* Linv = nullptr;       // nblk x 64 x 64
     float2* UinvT = nullptr;      // nblk x (U_bb^{-1})^T
@@ -1053,6 +1132,10 @@ int bem2d_internal_lu32_create(bem2d_ctx* c, int64_t n, DevArena* arena, bem2d_l
     f->arena = arena;
     f->n = n;
     f->nblk = (int)((n + NB - 1) / NB);
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) == hipSuccess &&
+        cus > 0)
+        f->cus = cus;
     const size_t inv_bytes = sizeof(float2) * (size_t)f->nblk * NB * NB;
     f->F = (float2*)bem2d_internal_dalloc(arena, sizeof(float2) * n * n);
     f->Linv = (float2*)bem2d_internal_dalloc(arena, inv_bytes);
@@ -1168,8 +1251,13 @@ int bem2d_internal_lu_advance(hipStream_t s, bem2d_lu32* const* lus, int count)
             } else {
                 blocks = (int64_t)u.tiles * u.tiles;
             }
+            // persistent workgroups: the compute units shared among the problems of the launch
+            // (a fused update needs two: workgroup 0 holds the next diagonal block alone)
+            const int64_t share = std::max<int64_t>(2, act[c0]->cus / cnt);
+            u.ntile = (int)blocks;
+            u.nwg = (int)std::min<int64_t>(blocks, share);
             up.p[d] = u;
-            up.start[d + 1] = up.start[d] + blocks;
+            up.start[d + 1] = up.start[d] + u.nwg;
         }
         if (up.start[cnt] > (int64_t)INT32_MAX) return FAIL(BEM2D_ERR_UNSUPPORTED, "LU grid too large");
         panel_trsm_kernel<<<(unsigned)tr.start[cnt], 256, 0, s>>>(tr);
