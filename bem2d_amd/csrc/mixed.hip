@@ -126,14 +126,21 @@ __device__ __forceinline__ float2 load_or(bool ok, const float2* p, const float2
 }
 
 // ---- cast ---------------------------------------------------------------------------------------
-__global__ void cast_kernel(int64_t n, const double2* __restrict__ A, int64_t lda,
-                            float2* __restrict__ F, int64_t ldf)
+// Column j0 + blockIdx.y of the np x np FP32 matrix: A cast to FP32, bordered by the identity
+// (rows and columns n .. np): the factors of the bordered matrix are those of A bordered by the
+// identity, and every 64 x 64 tile of the factorisation is full.
+__global__ void cast_kernel(int64_t n, int64_t np, int64_t j0, const double2* __restrict__ A,
+                            int64_t lda, float2* __restrict__ F)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t j = blockIdx.y;
-    if (i < n) {
-        const double2 v = A[i + lda * j];
-        F[i + ldf * j] = make_float2((float)v.x, (float)v.y);
+    const int64_t j = j0 + blockIdx.y;
+    if (i < np) {
+        float2 v = make_float2(i == j ? 1.f : 0.f, 0.f);
+        if (i < n && j < n) {
+            const double2 a = A[i + lda * j];
+            v = make_float2((float)a.x, (float)a.y);
+        }
+        F[i + np * j] = v;
     }
 }
 
@@ -507,12 +514,74 @@ __device__ __forceinline__ void update_tile_xy(const LuProblem& pb, int idx, int
     }
 }
 
+// Raw buffer access to the factor matrix: a descriptor per 64-column slab (base in SGPRs) and
+// 32-bit byte offsets, the part common to a thread's loads in a VGPR and the rest in an SGPR, so a
+// load costs no address arithmetic.  Offsets stay below 512 (ld + 1) bytes.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t slab(const float2* base)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float2*>(base), (short)0, 0x7fffffff,
+                                             0x00020000);
+}
+__device__ __forceinline__ float2 slab_load(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+    float2 f;
+    __builtin_memcpy(&f, &v, sizeof(f));
+    return f;
+}
+__device__ __forceinline__ void slab_store(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff,
+                                           float2 f)
+{
+    decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0u, 0u, 0)) v;
+    __builtin_memcpy(&v, &f, sizeof(f));
+    __builtin_amdgcn_raw_buffer_store_b64(v, r, voff, soff, 0);
+}
+
+// The trailing update stages its first operand k-major ([k][row], pitch KPA): the 64 consecutive
+// rows a wavefront writes and the 16 rows x 4 k a MFMA operand reads both fall in distinct banks.
+constexpr int KPA = NB + 16;
+constexpr int kUpdLdsFloats = 2 * NB * KPA + 2 * NB * KP;
+static_assert(sizeof(DiagLds) <= sizeof(float) * kUpdLdsFloats, "LDS reuse");
+
+// c += Aop Bop with Aop in LDS as [k][row] (pitch KPA) and Bop as [col][k] (pitch KP)
+__device__ __forceinline__ void cgemm_64t(const float* __restrict__ Are,
+                                          const float* __restrict__ Aim,
+                                          const float* __restrict__ Bre,
+                                          const float* __restrict__ Bim, TileAcc& c)
+{
+    const int t = threadIdx.x;
+    const int wave = t >> 6, lane = t & 63;
+    const int wr = (wave & 1) * 32, wc = (wave >> 1) * 32;
+    const int li = lane % 16, lk = lane / 16;
+#pragma unroll
+    for (int ks = 0; ks < NB; ks += 4) {
+        float ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            ar[p] = Are[(ks + lk) * KPA + wr + 16 * p + li];
+            ai[p] = Aim[(ks + lk) * KPA + wr + 16 * p + li];
+            br[p] = Bre[(wc + 16 * p + li) * KP + ks + lk];
+            bi[p] = Bim[(wc + 16 * p + li) * KP + ks + lk];
+        }
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                c.re[p][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[p], br[q], c.re[p][q], 0, 0, 0);
+                c.re[p][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(-ai[p], bi[q], c.re[p][q], 0, 0, 0);
+                c.im[p][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[p], bi[q], c.im[p][q], 0, 0, 0);
+                c.im[p][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[p], br[q], c.im[p][q], 0, 0, 0);
+            }
+    }
+}
+
 __global__ __launch_bounds__(256) void trailing_update_kernel(const LuBatch batch)
 {
     const int di = find_problem(batch.start, batch.count, blockIdx.x);
     const LuProblem& pb = batch.p[di];
     float2* __restrict__ A = pb.F;
-    const int64_t ld = pb.n, n = pb.n, k0 = pb.k0;
+    // the bordered matrix: n is a multiple of 64 and every tile is full
+    const int64_t ld = pb.n, k0 = pb.k0;
     const int K = pb.kb;
     const int w = (int)((int64_t)blockIdx.x - batch.start[di]);
     const bool fused = pb.Linv_next != nullptr;
@@ -521,31 +590,48 @@ __global__ __launch_bounds__(256) void trailing_update_kernel(const LuBatch batc
         first = w;
         stride = w == 0 ? pb.ntile : pb.nwg - 1;
     }
-    __shared__ __align__(16) float lds[kTileLdsFloats];
-    static_assert(sizeof(DiagLds) <= sizeof(float) * kTileLdsFloats, "LDS reuse");
+    __shared__ __align__(16) float lds[kUpdLdsFloats];
     float* Are = lds;
-    float* Aim = lds + NB * KP;
-    float* Bre = lds + 2 * NB * KP;
-    float* Bim = lds + 3 * NB * KP;
-    const int t = threadIdx.x;
+    float* Aim = lds + NB * KPA;
+    float* Bre = lds + 2 * NB * KPA;
+    float* Bim = lds + 2 * NB * KPA + NB * KP;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int64_t rest0 = k0 + K;
     constexpr int kPer = NB * NB / 256;
-    // Operand element e = t + 256 i of a chunk: A chunk row e % 64, column e / 64; B chunk row
-    // e % 64 of column e / 64 (both walk the column-major matrix down its columns: coalesced).
-    // Out-of-range elements are loaded from A[0] and zeroed when written to LDS (a select right
-    // after a load would wait for it there).
+    // Operand element e = t + 256 i of a chunk: row x = t % 64 of column y = t / 64 + 4 i of the
+    // slab (coalesced down the columns), at byte offset voff + i soff_step.
+    const unsigned voff = (unsigned)(((t & 63) + ld * (t >> 6)) * 8);
+    const unsigned soff_step = (unsigned)(4 * ld * 8);
+    // output element (p, q, r) of this thread: row wr + 16 p + 4 (lane / 16) + r, column
+    // wc + 16 q + lane % 16 of the tile
+    const int wr = (wave & 1) * 32, wc = (wave >> 1) * 32;
+    const unsigned voffc = (unsigned)(((wr + 4 * (lane >> 4)) + ld * (wc + (lane & 15))) * 8);
     float2 va[kPer], vb[kPer];
     auto fetch = [&](int idx, int kc) __attribute__((always_inline)) {
         int bx, by;
         update_tile_xy(pb, idx, bx, by);
         const int64_t r0 = rest0 + (int64_t)bx * NB, c0 = rest0 + (int64_t)by * NB;
-        const int kn = min(NB, K - kc);
+        const __amdgpu_buffer_rsrc_t ra = slab(A + r0 + ld * (k0 + kc));
+        const __amdgpu_buffer_rsrc_t rb = slab(A + (k0 + kc) + ld * c0);
 #pragma unroll
         for (int i = 0; i < kPer; ++i) {
-            const int e = t + 256 * i, x = e % NB, y = e / NB;
-            va[i] = *(r0 + x < n && y < kn ? &A[(r0 + x) + ld * (k0 + kc + y)] : A);
-            vb[i] = *(c0 + y < n && x < kn ? &A[(k0 + kc + x) + ld * (c0 + y)] : A);
+            va[i] = slab_load(ra, voff, i * soff_step);
+            vb[i] = slab_load(rb, voff, i * soff_step);
         }
+    };
+    // A tile's output is stored during the next tile (after its operands are staged), so the
+    // stores are never the youngest memory operations when a wait for loads comes.
+    float2 res[2][2][4];
+    __amdgpu_buffer_rsrc_t rres = slab(A);
+    bool pending = false;
+    auto store_res = [&]() __attribute__((always_inline)) {
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+            for (int q = 0; q < 2; ++q)
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    slab_store(rres, voffc, (unsigned)((16 * p + r + ld * 16 * q) * 8), res[p][q][r]);
     };
     int idx = first;
     if (idx < pb.ntile) fetch(idx, 0);
@@ -553,57 +639,58 @@ __global__ __launch_bounds__(256) void trailing_update_kernel(const LuBatch batc
         int bx, by;
         update_tile_xy(pb, idx, bx, by);
         const int64_t r0 = rest0 + (int64_t)bx * NB, c0 = rest0 + (int64_t)by * NB;
-        // the output block, read now and used after the products
+        const __amdgpu_buffer_rsrc_t rc = slab(A + r0 + ld * c0);
         float2 cv[2][2][4];
-#pragma unroll
-        for (int p = 0; p < 2; ++p)
-#pragma unroll
-            for (int q = 0; q < 2; ++q)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int64_t i = r0 + acc_row(p, r), j = c0 + acc_col(q);
-                    cv[p][q][r] = *(i < n && j < n ? &A[i + ld * j] : A);
-                }
         TileAcc acc;
         acc_zero(acc);
         for (int kc = 0; kc < K; kc += NB) {
-            const int kn = min(NB, K - kc);
             __syncthreads(); // the previous chunk's operand reads are done
 #pragma unroll
             for (int i = 0; i < kPer; ++i) {
-                const int e = t + 256 * i, x = e % NB, y = e / NB;
-                const bool oka = r0 + x < n && y < kn, okb = c0 + y < n && x < kn;
-                Are[x * KP + y] = oka ? va[i].x : 0.f;
-                Aim[x * KP + y] = oka ? va[i].y : 0.f;
-                Bre[y * KP + x] = okb ? vb[i].x : 0.f;
-                Bim[y * KP + x] = okb ? vb[i].y : 0.f;
+                const int x = t & 63, y = (t >> 6) + 4 * i;
+                Are[y * KPA + x] = va[i].x;
+                Aim[y * KPA + x] = va[i].y;
+                Bre[y * KP + x] = vb[i].x;
+                Bim[y * KP + x] = vb[i].y;
             }
             __syncthreads();
-            // the next chunk (of this tile or of the next one) goes in flight under the products
-            if (kc + NB < K)
-                fetch(idx, kc + NB);
-            else if (idx + stride < pb.ntile)
-                fetch(idx + stride, 0);
-            cgemm_64(Are, Aim, Bre, Bim, acc);
+            // The previous tile's output goes out and this tile's output block comes in (first
+            // chunk); then the next chunk (of this tile or of the next one; past the last, this
+            // tile's chunk again, unused: one call site keeps the wait counts exact) goes in
+            // flight under the products.
+            if (kc == 0) {
+                if (pending) store_res();
+#pragma unroll
+                for (int p = 0; p < 2; ++p)
+#pragma unroll
+                    for (int q = 0; q < 2; ++q)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r)
+                            cv[p][q][r] = slab_load(rc, voffc,
+                                                    (unsigned)((16 * p + r + ld * 16 * q) * 8));
+            }
+            const bool more = kc + NB < K;
+            const int nidx = more ? idx : (idx + stride < pb.ntile ? idx + stride : idx);
+            fetch(nidx, more ? kc + NB : 0);
+            cgemm_64t(Are, Aim, Bre, Bim, acc);
         }
 #pragma unroll
         for (int p = 0; p < 2; ++p)
 #pragma unroll
             for (int q = 0; q < 2; ++q)
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int64_t i = r0 + acc_row(p, r);
-                    const int64_t j = c0 + acc_col(q);
-                    if (i < n && j < n)
-                        A[i + ld * j] = make_float2(cv[p][q][r].x - acc.re[p][q][r],
-                                                    cv[p][q][r].y - acc.im[p][q][r]);
-                }
+                for (int r = 0; r < 4; ++r)
+                    res[p][q][r] = make_float2(cv[p][q][r].x - acc.re[p][q][r],
+                                               cv[p][q][r].y - acc.im[p][q][r]);
+        rres = rc;
+        pending = true;
         idx += stride;
     }
+    if (pending) store_res();
     if (fused && w == 0) {
         __syncthreads(); // the tile's stores are visible to the workgroup; the LDS is free
-        diag_factor(A, ld, rest0, (int)min((int64_t)NB, n - rest0), pb.Linv_next, pb.UinvT_next,
-                    pb.bad, *reinterpret_cast<DiagLds*>(lds));
+        diag_factor(A, ld, rest0, NB, pb.Linv_next, pb.UinvT_next, pb.bad,
+                    *reinterpret_cast<DiagLds*>(lds));
     }
 }
 
@@ -625,7 +712,8 @@ struct RefUnit {
     float2* Y;                     // n x nrhs FP32 correction (ld n)
     double* scale;                 // nrhs
     unsigned long long* norms;     // kNormSlots: rmax | xmax | bmax | |A|_inf | pivot flag | time-out
-    const float2* F;               // FP32 factors (ld n)
+    const float2* F;               // FP32 factors
+    int64_t ldf;                   // their leading dimension
     const float2* Linv;
     const float2* UinvT;
     unsigned long long* ty;        // 2 n x nrhs tagged words handed between sweep blocks
@@ -695,7 +783,7 @@ __global__ __launch_bounds__(256) void block_trsv_kernel(const RefBatch batch)
     const int di = find_problem(batch.start, batch.count, blockIdx.x);
     const RefUnit& U = batch.u[di];
     const float2* __restrict__ F = U.F;
-    const int64_t ld = U.n, n = U.n;
+    const int64_t ld = U.ldf, n = U.n;
     const int nblk = U.nblk, nrhs = U.nrhs;
     float2* __restrict__ Y = U.Y;
     const int64_t ldy = U.n;
@@ -1094,7 +1182,8 @@ struct bem2d_lu32 {
     bool pending_wide = false;    // the last update covered the cross only; the next is depth 128
     bool factored = false;
     int cus = 256;                // compute units of the device (persistent update workgroups)
-    float2* F = nullptr;          // n x n factors, ld = n
+    int64_t np = 0;               // n rounded up to whole 64-wide blocks
+    float2* F = nullptr;          // np x np factors (A bordered by the identity), ld = np
     float2* Linv = nullptr;       // nblk x 64 x 64
     float2* UinvT = nullptr;      // nblk x (U_bb^{-1})^T
     int* dflag = nullptr;         // [0] bad pivot, [1] sweep time-out
@@ -1104,8 +1193,8 @@ struct bem2d_lu32 {
 
 size_t bem2d_internal_lu32_bytes(int64_t n)
 {
-    const int64_t nblk = (n + NB - 1) / NB;
-    return sizeof(float2) * (size_t)(n * n + 2 * nblk * NB * NB) + sizeof(double) * (16 * n + 1) +
+    const int64_t nblk = (n + NB - 1) / NB, np = nblk * NB;
+    return sizeof(float2) * (size_t)(np * np + 2 * nblk * NB * NB) + sizeof(double) * (16 * n + 1) +
            8 * 256;
 }
 
@@ -1132,12 +1221,13 @@ int bem2d_internal_lu32_create(bem2d_ctx* c, int64_t n, DevArena* arena, bem2d_l
     f->arena = arena;
     f->n = n;
     f->nblk = (int)((n + NB - 1) / NB);
+    f->np = (int64_t)f->nblk * NB;
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) == hipSuccess &&
         cus > 0)
         f->cus = cus;
     const size_t inv_bytes = sizeof(float2) * (size_t)f->nblk * NB * NB;
-    f->F = (float2*)bem2d_internal_dalloc(arena, sizeof(float2) * n * n);
+    f->F = (float2*)bem2d_internal_dalloc(arena, sizeof(float2) * f->np * f->np);
     f->Linv = (float2*)bem2d_internal_dalloc(arena, inv_bytes);
     f->UinvT = (float2*)bem2d_internal_dalloc(arena, inv_bytes);
     f->dflag = (int*)bem2d_internal_dalloc(arena, sizeof(int) * 2);
@@ -1163,10 +1253,10 @@ static LuProblem lu_problem(bem2d_lu32* f, int b)
 {
     LuProblem p{};
     p.F = f->F;
-    p.n = f->n;
+    p.n = f->np; // the bordered matrix: every tile full
     p.k0 = (int64_t)b * NB;
-    p.kb = (int)std::min<int64_t>(NB, f->n - p.k0);
-    const int64_t rest = f->n - p.k0 - p.kb;
+    p.kb = NB;
+    const int64_t rest = f->np - p.k0 - p.kb;
     p.tiles = (int)((rest + NB - 1) / NB);
     p.Linv = f->Linv + (size_t)b * NB * NB;
     p.UinvT = f->UinvT + (size_t)b * NB * NB;
@@ -1182,10 +1272,11 @@ int bem2d_internal_lu_begin(hipStream_t s, bem2d_lu32* const* lus, const void* c
         bem2d_lu32* f = lus[d];
         const int64_t n = f->n;
         const double2* A = (const double2*)A64[d];
-        for (int64_t j0 = 0; j0 < n; j0 += 65535) { // grid.y is limited to 65535
-            const int64_t nj = std::min<int64_t>(65535, n - j0);
-            dim3 g((unsigned)((n + 255) / 256), (unsigned)nj);
-            cast_kernel<<<g, 256, 0, s>>>(n, A + lda[d] * j0, lda[d], f->F + n * j0, n);
+        const int64_t np = f->np;
+        for (int64_t j0 = 0; j0 < np; j0 += 65535) { // grid.y is limited to 65535
+            const int64_t nj = std::min<int64_t>(65535, np - j0);
+            dim3 g((unsigned)((np + 255) / 256), (unsigned)nj);
+            cast_kernel<<<g, 256, 0, s>>>(n, np, j0, A, lda[d], f->F);
         }
         dim3 g2((unsigned)((n + 255) / 256), 16);
         rowsum_kernel<<<g2, 256, 0, s>>>(n, A, lda[d], 16, f->rowpart);
@@ -1371,6 +1462,7 @@ int bem2d_internal_ref_create(bem2d_ctx* c, DevArena* arena, const RefSpec* spec
             u.X = (double2*)sp.X + sp.ldx * h0;
             u.ldx = sp.ldx;
             u.F = f->F;
+            u.ldf = f->np;
             u.Linv = f->Linv;
             u.UinvT = f->UinvT;
             u.anorm = f->anorm;
@@ -1662,13 +1754,13 @@ static int refine_wide(bem2d_ctx* c, bem2d_lu32* f, const double2* A, int64_t ld
         rocblas_status st = rocblas_ctrsm(c->blas, rocblas_side_left, rocblas_fill_lower,
                                           rocblas_operation_none, rocblas_diagonal_unit,
                                           (rocblas_int)n, (rocblas_int)nrhs, &one32,
-                                          (const rocblas_float_complex*)f->F, (rocblas_int)n,
+                                          (const rocblas_float_complex*)f->F, (rocblas_int)f->np,
                                           (rocblas_float_complex*)Y, (rocblas_int)n);
         if (st == rocblas_status_success)
             st = rocblas_ctrsm(c->blas, rocblas_side_left, rocblas_fill_upper,
                                rocblas_operation_none, rocblas_diagonal_non_unit, (rocblas_int)n,
                                (rocblas_int)nrhs, &one32, (const rocblas_float_complex*)f->F,
-                               (rocblas_int)n, (rocblas_float_complex*)Y, (rocblas_int)n);
+                               (rocblas_int)f->np, (rocblas_float_complex*)Y, (rocblas_int)n);
         wide_update_kernel<<<g, 256, 0, s>>>(n, nrhs, Y, scale, X, ldx, B, ldb, R);
         if (st == rocblas_status_success)
             st = rocblas_zgemm(c->blas, rocblas_operation_none, rocblas_operation_none,
