@@ -1,13 +1,14 @@
 // Mixed-precision dense complex solve: FP32 LU on the MFMA cores + FP64 iterative refinement.
 //
-// Factorisation (bem2d_lu32_factor): A32 = (complex64) A64, then a blocked right-looking LU with
-// 64-wide panels and no row interchanges.  Step b: the block column below and the block row right
-// of diagonal block b are multiplied by the inverses of its triangles (panel_trsm_kernel, MFMA),
-// and the trailing matrix receives the rank-64 update A22 -= L21 U12 as a complex GEMM on
-// v_mfma_f32_16x16x4f32 with FP32 accumulation (four real MFMA products per complex product).  The
-// workgroup of that update which owns diagonal block b + 1 goes on to factorise it -- LU plus the
-// inverses of both triangles, by one workgroup with the tile in registers (diag_factor) -- while
-// the rest of the update runs, so the panel factorisation overlaps the GEMM.  Without interchanges
+// Factorisation (bem2d_lu32_factor): A32 = (complex64) A64 bordered by the identity to whole
+// 64-blocks, then a blocked right-looking LU with 64-wide panels and no row interchanges.  Step b:
+// the block column below and the block row right of diagonal block b are multiplied by the
+// inverses of its triangles (panel_trsm_kernel, MFMA), and the trailing matrix receives the
+// rank-64 update A22 -= L21 U12 as a complex GEMM on v_mfma_f32_16x16x4f32 with FP32 accumulation
+// (four real MFMA products per complex product).  The workgroup of that update which owns diagonal
+// block b + 1 goes on to factorise it -- LU plus the inverses of both triangles, blocked in
+// 16-column panels inside one workgroup (diag_factor) -- while the rest of the update runs, so the
+// panel factorisation overlaps the GEMM.  Without interchanges
 // the panel has no sequential pivot search, the step that makes partial-pivoting panels latency
 // bound on a GPU.  A zero or non-finite pivot is flagged; refinement then reports failure and the
 // one-shot solver falls back to the FP64 LU.
@@ -179,161 +180,257 @@ __global__ void rowsum_max_kernel(int64_t n, int nchunk, const double* __restric
     if (threadIdx.x == 0) *out = red[0];
 }
 
-// ---- diagonal block: LU without interchanges + inverses of both triangles -----------------------
-// 256 threads.  Thread t = 64 w + l holds T[a][b] = tile(R(a), C(b)) with R(a) = 16 a + (l & 15) and
-// C(b) = 16 b + 4 w + (l >> 4): each wavefront holds a 16 x 4 piece of each of the 4 x 4 sub-blocks
-// of 16, so at step j the sub-blocks above or left of row/column j drop out on every SIMD alike and
-// the work per step shrinks with the active region.
-//
-// Step j eliminates column j (pivot row j = U[j][:], multipliers L[r][j] = A[r][j] / U[j][j]) and,
-// in the same step, advances two forward sweeps that need only row j of U and column j of L:
-//   X = L^{-1}:           X = I;  rows r > j:  X[r][:] -= L[r][j] X[j][:];
-//   Y = (U^T)^{-1}:       Y = I;  Y[j][:] /= U[j][j];  rows r > j:  Y[r][:] -= U[j][r] Y[j][:];
-// so U^{-1} = Y^T comes out of the same 64 steps.  Row and column j+1 of the updated matrix and rows
-// j+1 of X and Y (Y unscaled) are published through ping-pong LDS buffers: one barrier per step.
-// Linv receives X and UinvT receives Y = (U^{-1})^T (column-major, ld 64; identity-padded when
-// kb < 64).
+// Reciprocal of a complex number (hardware reciprocal of |a|^2).
 __device__ __forceinline__ float2 fast_crecip(float2 a)
 {
     const float d = __builtin_amdgcn_rcpf(a.x * a.x + a.y * a.y);
     return make_float2(a.x * d, -a.y * d);
 }
 
+// ---- diagonal block, blocked: LU without interchanges + inverses of both triangles ----------------
+// 256 threads, the tile in LDS.  Four panels of 16 columns, each:
+//   1. one wavefront factorises the panel (rows 16 p .. 63 x 16 columns, a row per lane, pivot row
+//      broadcast by lane reads): 16 dependent steps with no barrier between them;
+//   2. one wavefront solves L_pp [U_p,right | X_pp] = [T_p,right | I] (the block row of U and the
+//      inverse of the unit-lower diagonal block, a column per lane) while another inverts U_pp by
+//      back substitution (Z_pp = U_pp^{-1});
+//   3. the trailing blocks take the rank-16 update on MFMA.
+// Then the off-diagonal blocks of X = L^{-1} and Z = U^{-1}, level by level (i - j = 1, 2, 3):
+//   X_ij = -X_ii sum_{k=j}^{i-1} L_ik X_kj,   Z_ji = -(sum_{k=j}^{i-1} Z_jk U_ki) Z_ii,
+// 16 x 16 complex products on MFMA, one block per wavefront.  Linv receives X and UinvT receives
+// Z^T (column-major, ld 64).  The tile is always a full 64 x 64 block (the factorised matrix is
+// bordered to whole blocks).
+constexpr int TP = NB + 1; // padded row pitch of the LDS tiles (float2)
 struct DiagLds {
-    float2 row[2][NB], col[2][NB];   // pivot row / column of the current matrix
-    float2 xrow[2][NB], yrow[2][NB]; // row j of X and of Y (before its scaling by 1/U[j][j])
+    float2 T[NB][TP];       // the tile; L (unit diagonal implied) and U in place
+    float2 W[NB][TP];       // X = L^{-1} strictly below the diagonal, Z = U^{-1} on and above it
+    float2 S[4][16][17];    // per-wavefront 16 x 16 scratch
 };
 
-__device__ __forceinline__ int diag_R(int a, int l) { return 16 * a + (l & 15); }
-__device__ __forceinline__ int diag_C(int b, int w, int l) { return 16 * b + 4 * w + (l >> 4); }
-
-template <int V> using ic = std::integral_constant<int, V>;
-
-// Lanes holding row j: (l & 15) == (j & 15), sub-block row j >> 4.  Lanes holding column j:
-// wavefront (j >> 2) & 3, (l >> 4) == (j & 3), sub-block column j >> 4.
-__device__ __forceinline__ bool holds_row(int j, int l) { return (l & 15) == (j & 15); }
-__device__ __forceinline__ bool holds_col(int j, int w, int l)
+__device__ __forceinline__ float lane_read(float v, int lane)
 {
-    return w == ((j >> 2) & 3) && (l >> 4) == (j & 3);
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
-// The 64 steps run as four groups of 16, one per sub-block index js = j >> 4: within a group every
-// sub-block index is a compile-time constant, so the register tiles are never indexed at run time
-// (that would place them in scratch memory) and the inactive sub-blocks are dropped at compile time.
+struct Acc16 {
+    v4f re, im;
+};
+__device__ __forceinline__ void acc16_zero(Acc16& c)
+{
+    c.re = v4f{0.f, 0.f, 0.f, 0.f};
+    c.im = v4f{0.f, 0.f, 0.f, 0.f};
+}
+// c += A B, A and B 16 x 16 complex given by element functions a(i, k), b(k, j); lane l supplies
+// A[l % 16][4 s + l / 16] and B[4 s + l / 16][l % 16] to k-slice s of v_mfma_f32_16x16x4f32, and
+// holds C[4 (l / 16) + r][l % 16] in register r.
+template <class FA, class FB>
+__device__ __forceinline__ void mma16(Acc16& c, FA a, FB b)
+{
+    const int l = threadIdx.x & 63, li = l & 15, lk = l >> 4;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const float2 av = a(li, 4 * s + lk);
+        const float2 bv = b(4 * s + lk, li);
+        c.re = __builtin_amdgcn_mfma_f32_16x16x4f32(av.x, bv.x, c.re, 0, 0, 0);
+        c.re = __builtin_amdgcn_mfma_f32_16x16x4f32(-av.y, bv.y, c.re, 0, 0, 0);
+        c.im = __builtin_amdgcn_mfma_f32_16x16x4f32(av.x, bv.y, c.im, 0, 0, 0);
+        c.im = __builtin_amdgcn_mfma_f32_16x16x4f32(av.y, bv.x, c.im, 0, 0, 0);
+    }
+}
+
+// One panel of the blocked factorisation (compile-time panel index P).
+template <int P>
+__device__ __forceinline__ void diag_panel(DiagLds& S, int w, int l, int* __restrict__ bad)
+{
+    constexpr int b0 = 16 * P;
+    const float2 zero = make_float2(0.f, 0.f);
+    // 1. panel factorisation by wavefront 0: lane l holds row b0 + l
+    if (w == 0) {
+        constexpr int nr = NB - b0;
+        const bool act = l < nr;
+        const int row = act ? b0 + l : b0;
+        float2 a[16];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) a[c] = csel(act, S.T[row][b0 + c], zero);
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const float pr = lane_read(a[j].x, j), pi = lane_read(a[j].y, j);
+            const float d = pr * pr + pi * pi;
+            if (l == 0 && !(d > 0.f && isfinite(pr) && isfinite(pi))) *bad = 1;
+            const float rd = __builtin_amdgcn_rcpf(d);
+            const float2 ip = make_float2(pr * rd, -pi * rd);
+            float2 u[16];
+#pragma unroll
+            for (int c = j + 1; c < 16; ++c) u[c] = make_float2(lane_read(a[c].x, j), lane_read(a[c].y, j));
+            if (l > j) {
+                const float2 m = cmul(a[j], ip);
+                a[j] = m;
+#pragma unroll
+                for (int c = j + 1; c < 16; ++c) a[c] = cfms(a[c], m, u[c]);
+            }
+        }
+        if (act) {
+#pragma unroll
+            for (int c = 0; c < 16; ++c) S.T[row][b0 + c] = a[c];
+        }
+    }
+    __syncthreads();
+    // 2. block row of U and X_pp (wavefront 1), Z_pp (wavefront 2)
+    if (w == 1) {
+        constexpr int nu = NB - 16 - b0; // columns right of the diagonal block
+        const bool ucol = l < nu, xcol = l >= nu && l < nu + 16;
+        const int col = ucol ? b0 + 16 + l : b0;
+        const int e = l - nu;
+        float2 x[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+            x[i] = csel(ucol, S.T[b0 + i][col], make_float2(i == e ? 1.f : 0.f, 0.f));
+#pragma unroll
+        for (int j = 0; j < 16; ++j)
+#pragma unroll
+            for (int i = j + 1; i < 16; ++i) x[i] = cfms(x[i], S.T[b0 + i][b0 + j], x[j]);
+        if (ucol) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) S.T[b0 + i][col] = x[i];
+        }
+        if (xcol) {
+#pragma unroll
+            for (int i = 1; i < 16; ++i)
+                if (i > e) S.W[b0 + i][b0 + e] = x[i];
+        }
+    } else if (w == 2) {
+        const int c = l & 15;
+        float2 z[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) z[i] = make_float2(i == c ? 1.f : 0.f, 0.f);
+#pragma unroll
+        for (int i = 15; i >= 0; --i) {
+            z[i] = cmul(z[i], fast_crecip(S.T[b0 + i][b0 + i]));
+#pragma unroll
+            for (int k = 0; k < i; ++k) z[k] = cfms(z[k], S.T[b0 + k][b0 + i], z[i]);
+        }
+        if (l < 16) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i)
+                if (i <= c) S.W[b0 + i][b0 + c] = z[i];
+        }
+    }
+    __syncthreads();
+    // 3. rank-16 update of the trailing blocks (bi, bj), bi, bj > P, one 16 x 16 block per wavefront
+    if (P < 3) {
+        constexpr int m = 3 - P;
+        for (int tile = w; tile < m * m; tile += 4) {
+            const int bi = P + 1 + tile % m, bj = P + 1 + tile / m;
+            Acc16 c;
+            acc16_zero(c);
+            mma16(c, [&](int i, int k) { return S.T[16 * bi + i][b0 + k]; },
+                  [&](int k, int j) { return S.T[b0 + k][16 * bj + j]; });
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                float2& v = S.T[16 * bi + 4 * (l >> 4) + r][16 * bj + (l & 15)];
+                v = make_float2(v.x - c.re[r], v.y - c.im[r]);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Off-diagonal inverse block: kind 0 is X_ij (i > j), kind 1 is Z_ji (j < i).
+__device__ __forceinline__ void diag_inverse_block(DiagLds& S, int w, int kind, int i, int j)
+{
+    const int l = threadIdx.x & 63;
+    // X_kj with its unit diagonal and zero upper part when k == j; Z_jk with its zero lower part
+    auto X = [&](int k, int jj) {
+        return [&S, k, jj](int r, int c) {
+            const float2 v = S.W[16 * k + r][16 * jj + c];
+            if (k > jj || r > c) return v;
+            return make_float2(r == c ? 1.f : 0.f, 0.f);
+        };
+    };
+    auto Z = [&](int jj, int k) {
+        return [&S, jj, k](int r, int c) {
+            const float2 v = S.W[16 * jj + r][16 * k + c];
+            return (jj < k || r <= c) ? v : make_float2(0.f, 0.f);
+        };
+    };
+    auto T = [&](int bi, int bk) {
+        return [&S, bi, bk](int r, int c) { return S.T[16 * bi + r][16 * bk + c]; };
+    };
+    Acc16 s;
+    acc16_zero(s);
+    for (int k = j; k < i; ++k) {
+        if (kind == 0)
+            mma16(s, T(i, k), X(k, j));
+        else
+            mma16(s, Z(j, k), T(k, i));
+    }
+    float2(*sc)[17] = S.S[w];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) sc[4 * (l >> 4) + r][l & 15] = make_float2(s.re[r], s.im[r]);
+    auto SC = [sc](int r, int c) { return sc[r][c]; };
+    Acc16 o;
+    acc16_zero(o);
+    if (kind == 0)
+        mma16(o, X(i, i), SC);
+    else
+        mma16(o, SC, Z(i, i));
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int rr = 4 * (l >> 4) + r, cc = l & 15;
+        const float2 v = make_float2(-o.re[r], -o.im[r]);
+        if (kind == 0)
+            S.W[16 * i + rr][16 * j + cc] = v;
+        else
+            S.W[16 * j + rr][16 * i + cc] = v;
+    }
+}
+
 __device__ __forceinline__ void diag_factor(float2* __restrict__ A, int64_t ld, int64_t k0, int kb,
                                             float2* __restrict__ Linv,
                                             float2* __restrict__ UinvT, int* __restrict__ bad,
                                             DiagLds& S)
 {
+    (void)kb; // always NB
     const int t = threadIdx.x;
     const int w = __builtin_amdgcn_readfirstlane(t >> 6);
     const int l = t & 63;
-    const float2 zero = make_float2(0.f, 0.f);
-    float2 T[4][4], X[4][4], Y[4][4];
+    float2 v[16];
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int i = 0; i < 16; ++i) {
+        const int e = t + 256 * i, r = e & 63, c = e >> 6;
+        v[i] = A[(k0 + r) + ld * (k0 + c)];
+    }
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const int r = diag_R(a, l), c = diag_C(b, w, l);
-            const float2 e = make_float2(r == c ? 1.f : 0.f, 0.f);
-            T[a][b] = load_or(r < kb && c < kb, &A[(k0 + r) + ld * (k0 + c)], A, e);
-            X[a][b] = e;
-            Y[a][b] = e;
-        }
-    // row / column j (sub-block index JS = j >> 4) into buffer buf
-    auto publish = [&](auto JS_, int j, int buf) __attribute__((always_inline)) {
-        constexpr int js = decltype(JS_)::value;
-        if (holds_row(j, l)) {
+    for (int i = 0; i < 16; ++i) {
+        const int e = t + 256 * i, r = e & 63, c = e >> 6;
+        S.T[r][c] = v[i];
+        S.W[r][c] = make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    diag_panel<0>(S, w, l, bad);
+    diag_panel<1>(S, w, l, bad);
+    diag_panel<2>(S, w, l, bad);
+    diag_panel<3>(S, w, l, bad);
+    // off-diagonal inverse blocks by levels: (kind, i, j) per wavefront
+    constexpr int lv1[6][3] = {{0, 1, 0}, {0, 2, 1}, {0, 3, 2}, {1, 1, 0}, {1, 2, 1}, {1, 3, 2}};
+    constexpr int lv2[4][3] = {{0, 2, 0}, {0, 3, 1}, {1, 2, 0}, {1, 3, 1}};
+    constexpr int lv3[2][3] = {{0, 3, 0}, {1, 3, 0}};
+    diag_inverse_block(S, w, lv1[w][0], lv1[w][1], lv1[w][2]);
+    if (w < 2) diag_inverse_block(S, w, lv1[w + 4][0], lv1[w + 4][1], lv1[w + 4][2]);
+    __syncthreads();
+    diag_inverse_block(S, w, lv2[w][0], lv2[w][1], lv2[w][2]);
+    __syncthreads();
+    if (w < 2) diag_inverse_block(S, w, lv3[w][0], lv3[w][1], lv3[w][2]);
+    __syncthreads();
 #pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const int c = diag_C(b, w, l);
-                S.row[buf][c] = T[js][b];
-                if (b <= js) {
-                    S.xrow[buf][c] = X[js][b];
-                    S.yrow[buf][c] = Y[js][b];
-                }
-            }
-        }
-        if (holds_col(j, w, l)) {
-#pragma unroll
-            for (int a = 0; a < 4; ++a) S.col[buf][diag_R(a, l)] = T[a][js];
-        }
-    };
-    auto group = [&](auto JS_) __attribute__((always_inline)) {
-        constexpr int js = decltype(JS_)::value;
-#pragma unroll 1
-        for (int jj = 0; jj < 16; ++jj) {
-            const int j = 16 * js + jj;
-            __syncthreads();
-            const int buf = j & 1;
-            const float2 p = S.row[buf][j];
-            if (t == 0 && j < kb &&
-                !(p.x * p.x + p.y * p.y > 0.f && isfinite(p.x) && isfinite(p.y)))
-                *bad = 1;
-            const float2 ip = fast_crecip(p);
-            // m[a] = L[R(a)][j], g[a] = U[j][R(a)] for rows below j; u[b] = U[j][C(b)] right of j
-            float2 m[4], g[4], u[4], xr[4], yr[4];
-#pragma unroll
-            for (int a = js; a < 4; ++a) {
-                const int r = diag_R(a, l);
-                const bool below = a > js || r > j;
-                const float2 cv = S.col[buf][r];
-                const float2 rv = S.row[buf][r];
-                m[a] = csel(below, cmul(cv, ip), zero);
-                g[a] = csel(below, rv, zero);
-            }
-#pragma unroll
-            for (int b = js; b < 4; ++b) {
-                const int c = diag_C(b, w, l);
-                const float2 rv = S.row[buf][c];
-                u[b] = csel(b > js || c > j, rv, zero);
-            }
-#pragma unroll
-            for (int b = 0; b <= js; ++b) {
-                const int c = diag_C(b, w, l);
-                xr[b] = S.xrow[buf][c];
-                yr[b] = cmul(S.yrow[buf][c], ip);
-            }
-            if (holds_row(j, l)) {
-#pragma unroll
-                for (int b = 0; b <= js; ++b) Y[js][b] = yr[b];
-            }
-#pragma unroll
-            for (int a = js; a < 4; ++a) {
-#pragma unroll
-                for (int b = js; b < 4; ++b) T[a][b] = cfms(T[a][b], m[a], u[b]);
-#pragma unroll
-                for (int b = 0; b <= js; ++b) {
-                    X[a][b] = cfms(X[a][b], m[a], xr[b]);
-                    Y[a][b] = cfms(Y[a][b], g[a], yr[b]);
-                }
-            }
-            // column j takes the multipliers (u is zero there, so the update left it as it was)
-            if (holds_col(j, w, l)) {
-#pragma unroll
-                for (int a = js; a < 4; ++a)
-                    if (a > js || diag_R(a, l) > j) T[a][js] = m[a];
-            }
-            if (jj < 15)
-                publish(ic<js>(), j + 1, buf ^ 1);
-            else if (js < 3)
-                publish(ic<(js < 3 ? js + 1 : 3)>(), j + 1, buf ^ 1);
-        }
-    };
-    publish(ic<0>(), 0, 0);
-    group(ic<0>());
-    group(ic<1>());
-    group(ic<2>());
-    group(ic<3>());
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const int r = diag_R(a, l), c = diag_C(b, w, l);
-            if (r < kb && c < kb) A[(k0 + r) + ld * (k0 + c)] = T[a][b];
-            Linv[r + NB * c] = X[a][b];
-            UinvT[r + NB * c] = Y[a][b];
-        }
+    for (int i = 0; i < 16; ++i) {
+        const int e = t + 256 * i, r = e & 63, c = e >> 6;
+        A[(k0 + r) + ld * (k0 + c)] = S.T[r][c];
+        const float2 x = S.W[r][c];
+        Linv[r + NB * c] = r > c ? x : make_float2(r == c ? 1.f : 0.f, 0.f);
+        const float2 z = S.W[c][r];
+        UinvT[r + NB * c] = c <= r ? z : make_float2(0.f, 0.f);
+    }
 }
 
 // Diagonal block p.k0 .. p.k0 + p.kb of problem blockIdx.x, inverses into Linv_next / UinvT_next.
