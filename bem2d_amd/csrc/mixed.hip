@@ -1270,6 +1270,24 @@ void bem2d_internal_dfree(DevArena* a, void* p)
         (void)hipFree(p);
 }
 
+// Workspace of the single-problem refinement, kept with the factors so that repeated refine calls
+// allocate nothing: one device block handed out by a bump arena (reset per call; a factorisation
+// is refined by one call at a time) and a pinned read-back buffer.
+struct RefCache final : DevArena {
+    char* base = nullptr;
+    size_t cap = 0, off = 0;
+    int nrhs = 0;                      // right-hand sides the workspace is sized for
+    unsigned long long* pinned = nullptr;
+    void* alloc(size_t bytes) override
+    {
+        const size_t a = (off + 255) & ~(size_t)255;
+        if (a + bytes > cap) return nullptr;
+        off = a + bytes;
+        return base + a;
+    }
+    void release(void*) override {}
+};
+
 struct bem2d_lu32 {
     bem2d_ctx* ctx = nullptr;
     DevArena* arena = nullptr;
@@ -1286,6 +1304,7 @@ struct bem2d_lu32 {
     int* dflag = nullptr;         // [0] bad pivot, [1] sweep time-out
     double* anorm = nullptr;      // |A|_inf
     double* rowpart = nullptr;    // 16 x n
+    RefCache* rcache = nullptr;   // refinement workspace of bem2d_lu32_refine
 };
 
 size_t bem2d_internal_lu32_bytes(int64_t n)
@@ -1306,6 +1325,11 @@ extern "C" void bem2d_lu32_destroy(bem2d_lu32* f)
     bem2d_internal_dfree(f->arena, f->dflag);
     bem2d_internal_dfree(f->arena, f->anorm);
     bem2d_internal_dfree(f->arena, f->rowpart);
+    if (f->rcache) {
+        (void)hipFree(f->rcache->base);
+        (void)hipHostFree(f->rcache->pinned);
+        delete f->rcache;
+    }
     delete f;
 }
 
@@ -1917,9 +1941,34 @@ extern "C" int bem2d_lu32_refine(bem2d_ctx* c, bem2d_lu32* f, const void* A64, i
         ldx <= 2000000000)
         return refine_wide(c, f, (const double2*)A64, lda, (const double2*)B, ldb, (double2*)X, ldx,
                            nrhs, tol, maxit, rep);
-    RefSpec sp{f, A64, lda, B, ldb, X, ldx, nrhs, tol, maxit, nullptr};
+    // the workspace is kept with the factors and grows to the largest nrhs seen
+    RefCache*& rcache = f->rcache;
+    if (!rcache || rcache->nrhs < nrhs) {
+        if (rcache) {
+            (void)hipStreamSynchronize(c->stream);
+            (void)hipFree(rcache->base);
+            (void)hipHostFree(rcache->pinned);
+            delete rcache;
+            rcache = nullptr;
+        }
+        RefCache* rc_new = new RefCache();
+        const int units = bem2d_internal_ref_units(nrhs);
+        rc_new->cap = bem2d_internal_ref_bytes(f->n, nrhs) +
+                      sizeof(unsigned long long) * kRefNormSlots * units + 1024;
+        rc_new->nrhs = nrhs;
+        if (hipMalloc(&rc_new->base, rc_new->cap) != hipSuccess ||
+            hipHostMalloc((void**)&rc_new->pinned,
+                          sizeof(unsigned long long) * kRefNormSlots * units) != hipSuccess) {
+            (void)hipFree(rc_new->base);
+            delete rc_new;
+            return FAIL(BEM2D_ERR_NOMEM, "refinement workspace allocation failed");
+        }
+        rcache = rc_new;
+    }
+    rcache->off = 0;
+    RefSpec sp{f, A64, lda, B, ldb, X, ldx, nrhs, tol, maxit, rcache};
     RefJob* j = nullptr;
-    int rc = bem2d_internal_ref_create(c, nullptr, &sp, 1, nullptr, &j);
+    int rc = bem2d_internal_ref_create(c, rcache, &sp, 1, rcache->pinned, &j);
     if (rc) return rc;
     rc = bem2d_internal_refine_sync(c->stream, j);
     bem2d_ir_report r{};
