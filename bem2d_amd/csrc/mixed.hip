@@ -126,50 +126,65 @@ __device__ __forceinline__ float2 load_or(bool ok, const float2* p, const float2
     return csel(ok, x, v);
 }
 
-// ---- cast ---------------------------------------------------------------------------------------
-// Column j0 + blockIdx.y of the np x np FP32 matrix: A cast to FP32, bordered by the identity
-// (rows and columns n .. np): the factors of the bordered matrix are those of A bordered by the
-// identity, and every 64 x 64 tile of the factorisation is full.
-__global__ void cast_kernel(int64_t n, int64_t np, int64_t j0, const double2* __restrict__ A,
-                            int64_t lda, float2* __restrict__ F)
-{
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t j = j0 + blockIdx.y;
-    if (i < np) {
-        float2 v = make_float2(i == j ? 1.f : 0.f, 0.f);
-        if (i < n && j < n) {
-            const double2 a = A[i + lda * j];
-            v = make_float2((float)a.x, (float)a.y);
-        }
-        F[i + np * j] = v;
-    }
-}
+// ---- cast and |A|_inf ------------------------------------------------------------------------------
+// The np x np FP32 matrix: A cast to FP32, bordered by the identity (rows and columns n .. np): the
+// factors of the bordered matrix are those of A bordered by the identity, and every 64 x 64 tile of
+// the factorisation is full.  Workgroup (x, y) covers rows 256 x .. 256 x + 255 of the kCastCols
+// columns of chunk y and leaves the FP64 row sums of |a_ij| over the chunk in part[y][i], so that
+// |A|_inf = max_i sum_j |a_ij| costs no second pass over A.
+constexpr int kCastCols = 64;
 
-// |A|_inf = max_i sum_j |a_ij| (FP64): partial row sums per column chunk, then max.
-__global__ void rowsum_kernel(int64_t n, const double2* __restrict__ A, int64_t lda, int nchunk,
-                              double* __restrict__ part)
+__global__ __launch_bounds__(256) void cast_kernel(int64_t n, int64_t np, const double2* __restrict__ A,
+                                                   int64_t lda, float2* __restrict__ F,
+                                                   double* __restrict__ part)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int c = blockIdx.y;
-    if (i >= n) return;
-    const int64_t j0 = n * c / nchunk, j1 = n * (c + 1) / nchunk;
+    const int64_t j0 = (int64_t)blockIdx.y * kCastCols;
+    if (i >= np) return;
     double s = 0.0;
-    for (int64_t j = j0; j < j1; ++j) {
-        const double2 v = A[i + lda * j];
-        s += hypot(v.x, v.y);
+    constexpr int kBatch = 16;
+    for (int c0 = 0; c0 < kCastCols; c0 += kBatch) {
+        double2 v[kBatch];
+#pragma unroll
+        for (int c = 0; c < kBatch; ++c) {
+            const int64_t j = j0 + c0 + c;
+            v[c] = *(i < n && j < n ? &A[i + lda * j] : A);
+        }
+#pragma unroll
+        for (int c = 0; c < kBatch; ++c) {
+            const int64_t j = j0 + c0 + c;
+            const bool in = i < n && j < n;
+            const double re = in ? v[c].x : (i == j ? 1.0 : 0.0), im = in ? v[c].y : 0.0;
+            F[i + np * j] = make_float2((float)re, (float)im);
+            s += in ? sqrt(fma(re, re, im * im)) : 0.0;
+        }
     }
-    part[c * n + i] = s;
+    if (i < n) part[blockIdx.y * n + i] = s;
 }
 
-__global__ void rowsum_max_kernel(int64_t n, int nchunk, const double* __restrict__ part,
-                                  double* __restrict__ out)
+__device__ __forceinline__ void atomic_max_nonneg(unsigned long long* p, double v)
+{
+    // the bit patterns of non-negative doubles are ordered like the values
+    atomicMax(p, (unsigned long long)__double_as_longlong(v));
+}
+
+// out = max_i sum_c part[c][i] (out zeroed beforehand)
+__global__ __launch_bounds__(256) void rowsum_max_kernel(int64_t n, int nchunk,
+                                                         const double* __restrict__ part,
+                                                         unsigned long long* __restrict__ out)
 {
     __shared__ double red[256];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double m = 0.0;
-    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
-        double s = 0.0;
-        for (int c = 0; c < nchunk; ++c) s += part[c * n + i];
-        m = fmax(m, s);
+    if (i < n) {
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        int c = 0;
+        for (; c + 4 <= nchunk; c += 4) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc[q] += part[(int64_t)(c + q) * n + i];
+        }
+        for (; c < nchunk; ++c) acc[0] += part[(int64_t)c * n + i];
+        m = (acc[0] + acc[1]) + (acc[2] + acc[3]);
     }
     red[threadIdx.x] = m;
     __syncthreads();
@@ -177,7 +192,7 @@ __global__ void rowsum_max_kernel(int64_t n, int nchunk, const double* __restric
         if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) *out = red[0];
+    if (threadIdx.x == 0) atomic_max_nonneg(out, red[0]);
 }
 
 // Reciprocal of a complex number (hardware reciprocal of |a|^2).
@@ -1018,11 +1033,6 @@ __global__ __launch_bounds__(256) void block_trsv_kernel(const RefBatch batch)
 // Row-parallel kernels: workgroup of 256 rows of one unit (start[] counts ceil(n / 256) per unit),
 // blockIdx.y = right-hand side h or residual chunk c.
 
-__device__ __forceinline__ void atomic_max_nonneg(unsigned long long* p, double v)
-{
-    // the bit patterns of non-negative doubles are ordered like the values
-    atomicMax(p, (unsigned long long)__double_as_longlong(v));
-}
 
 __device__ __forceinline__ double wave_max(double m)
 {
@@ -1303,15 +1313,15 @@ struct bem2d_lu32 {
     float2* UinvT = nullptr;      // nblk x (U_bb^{-1})^T
     int* dflag = nullptr;         // [0] bad pivot, [1] sweep time-out
     double* anorm = nullptr;      // |A|_inf
-    double* rowpart = nullptr;    // 16 x n
+    double* rowpart = nullptr;    // (np / kCastCols) x n partial row sums of |a_ij|
     RefCache* rcache = nullptr;   // refinement workspace of bem2d_lu32_refine
 };
 
 size_t bem2d_internal_lu32_bytes(int64_t n)
 {
     const int64_t nblk = (n + NB - 1) / NB, np = nblk * NB;
-    return sizeof(float2) * (size_t)(np * np + 2 * nblk * NB * NB) + sizeof(double) * (16 * n + 1) +
-           8 * 256;
+    return sizeof(float2) * (size_t)(np * np + 2 * nblk * NB * NB) +
+           sizeof(double) * ((np / kCastCols) * n + 1) + 8 * 256;
 }
 
 extern "C" void bem2d_lu32_destroy(bem2d_lu32* f)
@@ -1353,7 +1363,7 @@ int bem2d_internal_lu32_create(bem2d_ctx* c, int64_t n, DevArena* arena, bem2d_l
     f->UinvT = (float2*)bem2d_internal_dalloc(arena, inv_bytes);
     f->dflag = (int*)bem2d_internal_dalloc(arena, sizeof(int) * 2);
     f->anorm = (double*)bem2d_internal_dalloc(arena, sizeof(double));
-    f->rowpart = (double*)bem2d_internal_dalloc(arena, sizeof(double) * 16 * n);
+    f->rowpart = (double*)bem2d_internal_dalloc(arena, sizeof(double) * (f->np / kCastCols) * n);
     if (!f->F || !f->Linv || !f->UinvT || !f->dflag || !f->anorm || !f->rowpart) {
         bem2d_lu32_destroy(f);
         return FAIL(BEM2D_ERR_NOMEM, "bem2d_lu32_create: device allocation of %lld bytes failed",
@@ -1394,14 +1404,12 @@ int bem2d_internal_lu_begin(hipStream_t s, bem2d_lu32* const* lus, const void* c
         const int64_t n = f->n;
         const double2* A = (const double2*)A64[d];
         const int64_t np = f->np;
-        for (int64_t j0 = 0; j0 < np; j0 += 65535) { // grid.y is limited to 65535
-            const int64_t nj = std::min<int64_t>(65535, np - j0);
-            dim3 g((unsigned)((np + 255) / 256), (unsigned)nj);
-            cast_kernel<<<g, 256, 0, s>>>(n, np, j0, A, lda[d], f->F);
-        }
-        dim3 g2((unsigned)((n + 255) / 256), 16);
-        rowsum_kernel<<<g2, 256, 0, s>>>(n, A, lda[d], 16, f->rowpart);
-        rowsum_max_kernel<<<1, 256, 0, s>>>(n, 16, f->rowpart, f->anorm);
+        const int nchunk = (int)(np / kCastCols);
+        zero_u64_kernel<<<1, 64, 0, s>>>((unsigned long long*)f->anorm, 1);
+        cast_kernel<<<dim3((unsigned)((np + 255) / 256), (unsigned)nchunk), 256, 0, s>>>(
+            n, np, A, lda[d], f->F, f->rowpart);
+        rowsum_max_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(
+            n, nchunk, f->rowpart, (unsigned long long*)f->anorm);
         zero_u64_kernel<<<1, 64, 0, s>>>((unsigned long long*)f->dflag, 1);
         f->step = 0;
         f->pending_wide = false;

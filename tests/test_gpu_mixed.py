@@ -41,6 +41,21 @@ def test_random_diagonally_weighted(ctx, n, nrhs):
     assert rel_fro(X, np.linalg.solve(A, B)) < 1e-10
 
 
+@pytest.mark.parametrize("n", [64, 130, 1000])
+def test_matrix_norm_reported(ctx, n):
+    """|A|_inf of the report (row sums formed while the matrix is cast to FP32) against numpy,
+    for sizes on and off the 64-block grid."""
+    print("case n=%d" % n, flush=True)
+    rng = np.random.default_rng(7 + n)
+    A = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) / np.sqrt(n)
+    A += 4 * np.eye(n)
+    A[n // 3, :] *= 5.0  # one dominant row
+    B = rng.standard_normal((n, 1)) + 1j * rng.standard_normal((n, 1))
+    rep = ctx.solve_mixed(_dev(A), _dev(B), tol=1e-12)
+    ref = np.abs(A).sum(axis=1).max()
+    assert abs(rep.matrix_norm_inf - ref) <= 1e-13 * ref, (rep.matrix_norm_inf, ref)
+
+
 @pytest.mark.parametrize("geom_name,n,k", [("circle", 200, 10.0), ("kite", 1000, 50.0),
                                            ("arc", 500, 60.0)])
 def test_bem_matrices(ctx, geom_name, n, k):
