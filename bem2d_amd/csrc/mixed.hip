@@ -456,14 +456,12 @@ __global__ __launch_bounds__(256) void diag_lu_kernel(const LuBatch batch)
     diag_factor(p.F, p.n, p.k0, p.kb, p.Linv_next, p.UinvT_next, p.bad, S);
 }
 
-// ---- 64 x 64 x 64 complex products on MFMA -----------------------------------------------------
-// acc += Aop Bop with Aop in LDS as [row][k] and Bop as [col][k] (pitch KP, real and imaginary
-// planes apart).  Wavefront w computes the 32 x 32 quarter at rows 32 (w & 1), cols 32 (w >> 1)
-// as 2 x 2 tiles of 16 x 16; each complex product is four real v_mfma_f32_16x16x4f32.
+// ---- 64 x 64 output tiles on MFMA -------------------------------------------------------------------
+// Wavefront w of a tile computes the 32 x 32 quarter at rows 32 (w & 1), cols 32 (w >> 1) as 2 x 2
+// blocks of 16 x 16; each complex product is four real v_mfma_f32_16x16x4f32.
 // v_mfma_f32_16x16x4f32 lane layout: A operand lane l holds A[l % 16][l / 16], B operand
 // B[l / 16][l % 16]; accumulator register r of lane l is C[4 (l / 16) + r][l % 16].
-constexpr int KP = NB + 4; // padded k extent of the LDS operand tiles
-constexpr int kTileLdsFloats = 4 * NB * KP;
+constexpr int KP = NB + 4; // padded k extent of the [col][k] LDS operand tile
 
 struct TileAcc {
     v4f re[2][2], im[2][2];
@@ -480,130 +478,68 @@ __device__ __forceinline__ void acc_zero(TileAcc& c)
         }
 }
 
-// c += Aop Bop over one staged k-chunk of 64
-__device__ __forceinline__ void cgemm_64(const float* __restrict__ Are,
-                                         const float* __restrict__ Aim,
-                                         const float* __restrict__ Bre,
-                                         const float* __restrict__ Bim, TileAcc& c)
-{
-    const int t = threadIdx.x;
-    const int wave = t >> 6, lane = t & 63;
-    const int wr = (wave & 1) * 32, wc = (wave >> 1) * 32;
-    const int li = lane % 16, lk = lane / 16;
-    for (int ks = 0; ks < NB; ks += 4) {
-        float ar[2], ai[2], br[2], bi[2];
-#pragma unroll
-        for (int p = 0; p < 2; ++p) {
-            ar[p] = Are[(wr + 16 * p + li) * KP + ks + lk];
-            ai[p] = Aim[(wr + 16 * p + li) * KP + ks + lk];
-            br[p] = Bre[(wc + 16 * p + li) * KP + ks + lk];
-            bi[p] = Bim[(wc + 16 * p + li) * KP + ks + lk];
-        }
-#pragma unroll
-        for (int p = 0; p < 2; ++p)
-#pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                c.re[p][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[p], br[q], c.re[p][q], 0, 0, 0);
-                c.re[p][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(-ai[p], bi[q], c.re[p][q], 0, 0, 0);
-                c.im[p][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[p], bi[q], c.im[p][q], 0, 0, 0);
-                c.im[p][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[p], br[q], c.im[p][q], 0, 0, 0);
-            }
-    }
-}
-
-// Tile-local coordinates of accumulator element (p, q, r) of this thread.
-__device__ __forceinline__ int acc_row(int p, int r)
-{
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    return (wave & 1) * 32 + 16 * p + 4 * (lane / 16) + r;
-}
-__device__ __forceinline__ int acc_col(int q)
-{
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    return (wave >> 1) * 32 + 16 * q + lane % 16;
-}
-
 // ---- panel: L21 = A21 U11^{-1} and U12 = L11^{-1} A12 on MFMA ---------------------------------
-// Workgroup bid < tiles of a problem: a 64-row tile of the block column below the diagonal block;
-// otherwise a 64-column tile of the block row to its right.  In place, through LDS.
+// Four workgroups per 64 x 64 panel tile, each a strip of 16 rows (tiles of the block column below
+// the diagonal block: out = S U^{-1}, row i of out needs row i of S only) or of 16 columns (tiles of
+// the block row to its right: out = L^{-1} S, column j of out needs column j of S only), so the
+// strips are independent and the tile is updated in place.  Wavefront w computes one 16 x 16
+// block of the strip with its MFMA operands loaded straight from memory (all 32 loads of a lane in
+// flight at once; no LDS staging), and writes it after a barrier, once every wavefront of the
+// workgroup has read the strip.
 __global__ __launch_bounds__(256) void panel_trsm_kernel(const LuBatch batch)
 {
     const int di = find_problem(batch.start, batch.count, blockIdx.x);
     const LuProblem& pb = batch.p[di];
     float2* __restrict__ A = pb.F;
-    const int64_t ld = pb.n, n = pb.n, k0 = pb.k0;
-    const int kb = pb.kb, nrow_tiles = pb.tiles;
+    // the bordered matrix: every tile full
+    const int64_t ld = pb.n, k0 = pb.k0;
+    const int nrow_tiles = pb.tiles;
     const float2* __restrict__ Linv = pb.Linv;
     const float2* __restrict__ UinvT = pb.UinvT;
     const int bid = (int)((int64_t)blockIdx.x - batch.start[di]);
-    __shared__ __align__(16) float lds[kTileLdsFloats];
-    float* Are = lds;
-    float* Aim = lds + NB * KP;
-    float* Bre = lds + 2 * NB * KP;
-    float* Bim = lds + 3 * NB * KP;
-    const int t = threadIdx.x;
-    const int64_t rest0 = k0 + kb;
-    const bool rowtile = bid < nrow_tiles;
-    const int64_t base = rest0 + (int64_t)(rowtile ? bid : bid - nrow_tiles) * NB;
-    const int len = (int)min((int64_t)NB, n - base);
-    // all 32 loads of a thread are requested before the first LDS write; the zero padding of a
-    // short tile is applied when the values are written (a select right after each load would
-    // wait for it there)
-    constexpr int kPer = NB * NB / 256;
-    float2 va[kPer], vb[kPer];
-    if (rowtile) { // out = S U^{-1}: Aop[x][y] = S[x][y], Bop[x][y] = U^{-1}[y][x] = UinvT[x][y]
+    const int tile = bid >> 2, strip = bid & 3;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63;
+    const int li = l & 15, lk = l >> 4;
+    const int64_t rest0 = k0 + NB;
+    const bool rowtile = tile < nrow_tiles;
+    const int64_t base = rest0 + (int64_t)(rowtile ? tile : tile - nrow_tiles) * NB;
+    // operand k-slice s of this lane: A(i = li, k = 4 s + lk), B(k = 4 s + lk, j = li)
+    float2 av[16], bv[16];
+    if (rowtile) {
+        // out[16 strip + i][16 w + j] = sum_k S[16 strip + i][k] U^{-1}[k][16 w + j],
+        // U^{-1}[k][c] = UinvT[c + 64 k]
 #pragma unroll
-        for (int i = 0; i < kPer; ++i) {
-            const int e = t + 256 * i, x = e % NB, y = e / NB;
-            const bool ok = x < len && y < kb;
-            va[i] = *(ok ? &A[(base + x) + ld * (k0 + y)] : A);
-            vb[i] = UinvT[x + NB * y];
+        for (int s = 0; s < 16; ++s) {
+            av[s] = A[(base + 16 * strip + li) + ld * (k0 + 4 * s + lk)];
+            bv[s] = UinvT[(16 * w + li) + NB * (4 * s + lk)];
         }
+    } else {
+        // out[16 w + i][16 strip + j] = sum_k L^{-1}[16 w + i][k] S[k][16 strip + j]
 #pragma unroll
-        for (int i = 0; i < kPer; ++i) {
-            const int e = t + 256 * i, x = e % NB, y = e / NB;
-            const bool ok = x < len && y < kb;
-            Are[x * KP + y] = ok ? va[i].x : 0.f;
-            Aim[x * KP + y] = ok ? va[i].y : 0.f;
-            Bre[x * KP + y] = vb[i].x;
-            Bim[x * KP + y] = vb[i].y;
-        }
-    } else {       // out = L^{-1} S: Aop[x][y] = Linv[x][y], Bop[y][x] = S[x][y]
-#pragma unroll
-        for (int i = 0; i < kPer; ++i) {
-            const int e = t + 256 * i, x = e % NB, y = e / NB;
-            const bool ok = x < kb && y < len;
-            va[i] = Linv[x + NB * y];
-            vb[i] = *(ok ? &A[(k0 + x) + ld * (base + y)] : A);
-        }
-#pragma unroll
-        for (int i = 0; i < kPer; ++i) {
-            const int e = t + 256 * i, x = e % NB, y = e / NB;
-            const bool ok = x < kb && y < len;
-            Are[x * KP + y] = va[i].x;
-            Aim[x * KP + y] = va[i].y;
-            Bre[y * KP + x] = ok ? vb[i].x : 0.f;
-            Bim[y * KP + x] = ok ? vb[i].y : 0.f;
+        for (int s = 0; s < 16; ++s) {
+            av[s] = Linv[(16 * w + li) + NB * (4 * s + lk)];
+            bv[s] = A[(k0 + 4 * s + lk) + ld * (base + 16 * strip + li)];
         }
     }
-    __syncthreads();
-    TileAcc c;
-    acc_zero(c);
-    cgemm_64(Are, Aim, Bre, Bim, c);
+    Acc16 c;
+    acc16_zero(c);
 #pragma unroll
-    for (int p = 0; p < 2; ++p)
+    for (int s = 0; s < 16; ++s) {
+        c.re = __builtin_amdgcn_mfma_f32_16x16x4f32(av[s].x, bv[s].x, c.re, 0, 0, 0);
+        c.re = __builtin_amdgcn_mfma_f32_16x16x4f32(-av[s].y, bv[s].y, c.re, 0, 0, 0);
+        c.im = __builtin_amdgcn_mfma_f32_16x16x4f32(av[s].x, bv[s].y, c.im, 0, 0, 0);
+        c.im = __builtin_amdgcn_mfma_f32_16x16x4f32(av[s].y, bv[s].x, c.im, 0, 0, 0);
+    }
+    __syncthreads(); // every wavefront has read the strip
 #pragma unroll
-        for (int q = 0; q < 2; ++q)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int x = acc_row(p, r), y = acc_col(q);
-                const float2 v = make_float2(c.re[p][q][r], c.im[p][q][r]);
-                if (rowtile) {
-                    if (x < len && y < kb) A[(base + x) + ld * (k0 + y)] = v;
-                } else {
-                    if (x < kb && y < len) A[(k0 + x) + ld * (base + y)] = v;
-                }
-            }
+    for (int r = 0; r < 4; ++r) {
+        const int i = 4 * lk + r, j = li;
+        const float2 v = make_float2(c.re[r], c.im[r]);
+        if (rowtile)
+            A[(base + 16 * strip + i) + ld * (k0 + 16 * w + j)] = v;
+        else
+            A[(k0 + 16 * w + i) + ld * (base + 16 * strip + j)] = v;
+    }
 }
 
 // ---- trailing update on MFMA, fused with the next diagonal block ---------------------------------
@@ -1453,7 +1389,7 @@ int bem2d_internal_lu_advance(hipStream_t s, bem2d_lu32* const* lus, int count)
             p.Linv_next = f->Linv + (size_t)(b + 1) * NB * NB;
             p.UinvT_next = f->UinvT + (size_t)(b + 1) * NB * NB;
             tr.p[d] = p;
-            tr.start[d + 1] = tr.start[d] + 2 * p.tiles;
+            tr.start[d + 1] = tr.start[d] + 4 * 2 * p.tiles; // four strips per panel tile
             // Panels pair up: step b updates only the cross (the next panel, which step b + 1
             // factorises), and step b + 1 applies both panels to the rest at depth 128, so the
             // trailing matrix is read and written once per two panels.
