@@ -577,12 +577,23 @@ __device__ __forceinline__ float2 slab_load(__amdgpu_buffer_rsrc_t r, unsigned v
     __builtin_memcpy(&f, &v, sizeof(f));
     return f;
 }
-__device__ __forceinline__ void slab_store(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff,
-                                           float2 f)
+// four consecutive complex values (32 bytes) as two 16-byte accesses
+__device__ __forceinline__ void slab_load4(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff,
+                                           float2* f)
 {
-    decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0u, 0u, 0)) v;
-    __builtin_memcpy(&v, &f, sizeof(f));
-    __builtin_amdgcn_raw_buffer_store_b64(v, r, voff, soff, 0);
+    const auto a = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+    const auto b = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff + 16, 0);
+    __builtin_memcpy(f, &a, 16);
+    __builtin_memcpy(f + 2, &b, 16);
+}
+__device__ __forceinline__ void slab_store4(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff,
+                                            const float2* f)
+{
+    decltype(__builtin_amdgcn_raw_buffer_load_b128(r, 0u, 0u, 0)) a, b;
+    __builtin_memcpy(&a, f, 16);
+    __builtin_memcpy(&b, f + 2, 16);
+    __builtin_amdgcn_raw_buffer_store_b128(a, r, voff, soff, 0);
+    __builtin_amdgcn_raw_buffer_store_b128(b, r, voff, soff + 16, 0);
 }
 
 // The trailing update stages its first operand k-major ([k][row], pitch KPA): the 64 consecutive
@@ -591,7 +602,8 @@ constexpr int KPA = NB + 16;
 constexpr int kUpdLdsFloats = 2 * NB * KPA + 2 * NB * KP;
 static_assert(sizeof(DiagLds) <= sizeof(float) * kUpdLdsFloats, "LDS reuse");
 
-// c += Aop Bop with Aop in LDS as [k][row] (pitch KPA) and Bop as [col][k] (pitch KP)
+// c += Aop Bop with Aop in LDS as [k][row] (pitch KPA) and Bop as [col][k] (pitch KP).  The two
+// MFMAs feeding one accumulator in a k-step are eight apart (no back-to-back dependency).
 __device__ __forceinline__ void cgemm_64t(const float* __restrict__ Are,
                                           const float* __restrict__ Aim,
                                           const float* __restrict__ Bre,
@@ -616,8 +628,13 @@ __device__ __forceinline__ void cgemm_64t(const float* __restrict__ Are,
 #pragma unroll
             for (int q = 0; q < 2; ++q) {
                 c.re[p][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[p], br[q], c.re[p][q], 0, 0, 0);
-                c.re[p][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(-ai[p], bi[q], c.re[p][q], 0, 0, 0);
                 c.im[p][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[p], bi[q], c.im[p][q], 0, 0, 0);
+            }
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                c.re[p][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(-ai[p], bi[q], c.re[p][q], 0, 0, 0);
                 c.im[p][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[p], br[q], c.im[p][q], 0, 0, 0);
             }
     }
@@ -672,14 +689,14 @@ __global__ __launch_bounds__(256) void trailing_update_kernel(const LuBatch batc
     float2 res[2][2][4];
     __amdgpu_buffer_rsrc_t rres = slab(A);
     bool pending = false;
+    // a lane's four accumulator rows of a 16 x 16 block are consecutive in memory: 32-byte accesses,
+    // four lanes covering 128 contiguous bytes of a column
     auto store_res = [&]() __attribute__((always_inline)) {
 #pragma unroll
         for (int p = 0; p < 2; ++p)
 #pragma unroll
             for (int q = 0; q < 2; ++q)
-#pragma unroll
-                for (int r = 0; r < 4; ++r)
-                    slab_store(rres, voffc, (unsigned)((16 * p + r + ld * 16 * q) * 8), res[p][q][r]);
+                slab_store4(rres, voffc, (unsigned)((16 * p + ld * 16 * q) * 8), res[p][q]);
     };
     int idx = first;
     if (idx < pb.ntile) fetch(idx, 0);
@@ -712,10 +729,7 @@ __global__ __launch_bounds__(256) void trailing_update_kernel(const LuBatch batc
                 for (int p = 0; p < 2; ++p)
 #pragma unroll
                     for (int q = 0; q < 2; ++q)
-#pragma unroll
-                        for (int r = 0; r < 4; ++r)
-                            cv[p][q][r] = slab_load(rc, voffc,
-                                                    (unsigned)((16 * p + r + ld * 16 * q) * 8));
+                        slab_load4(rc, voffc, (unsigned)((16 * p + ld * 16 * q) * 8), cv[p][q]);
             }
             const bool more = kc + NB < K;
             const int nidx = more ? idx : (idx + stride < pb.ntile ? idx + stride : idx);
