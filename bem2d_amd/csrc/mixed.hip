@@ -531,15 +531,11 @@ __global__ __launch_bounds__(256) void panel_trsm_kernel(const LuBatch batch)
         c.im = __builtin_amdgcn_mfma_f32_16x16x4f32(av[s].y, bv[s].x, c.im, 0, 0, 0);
     }
     __syncthreads(); // every wavefront has read the strip
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const int i = 4 * lk + r, j = li;
-        const float2 v = make_float2(c.re[r], c.im[r]);
-        if (rowtile)
-            A[(base + 16 * strip + i) + ld * (k0 + 16 * w + j)] = v;
-        else
-            A[(k0 + 16 * w + i) + ld * (base + 16 * strip + j)] = v;
-    }
+    // rows 4 lk .. 4 lk + 3 of column li: 32 contiguous bytes, written as two float4
+    float2* dst = rowtile ? &A[(base + 16 * strip + 4 * lk) + ld * (k0 + 16 * w + li)]
+                          : &A[(k0 + 16 * w + 4 * lk) + ld * (base + 16 * strip + li)];
+    reinterpret_cast<float4*>(dst)[0] = make_float4(c.re[0], c.im[0], c.re[1], c.im[1]);
+    reinterpret_cast<float4*>(dst)[1] = make_float4(c.re[2], c.im[2], c.re[3], c.im[3]);
 }
 
 // ---- trailing update on MFMA, fused with the next diagonal block ---------------------------------
