@@ -203,17 +203,17 @@ __device__ __forceinline__ float2 fast_crecip(float2 a)
 }
 
 // ---- diagonal block, blocked: LU without interchanges + inverses of both triangles ----------------
-// 256 threads, the tile in LDS.  Four panels of 16 columns, each:
-//   1. one wavefront factorises the panel (rows 16 p .. 63 x 16 columns, a row per lane, pivot row
-//      broadcast by lane reads): 16 dependent steps with no barrier between them;
-//   2. one wavefront solves L_pp [U_p,right | X_pp] = [T_p,right | I] (the block row of U and the
-//      inverse of the unit-lower diagonal block, a column per lane) while another inverts U_pp by
-//      back substitution (Z_pp = U_pp^{-1});
-//   3. the trailing blocks take the rank-16 update on MFMA.
-// Then the off-diagonal blocks of X = L^{-1} and Z = U^{-1}, level by level (i - j = 1, 2, 3):
+// 256 threads, the tile in LDS, four panels of 16 columns.  The critical chain runs on wavefront 0:
+// factorise panel P (rows 16 P .. 63, a row per lane, pivot row broadcast by lane reads: 16
+// dependent steps with no barrier), solve the next block of the U row, U_P,P+1 = L_PP^{-1} T_P,P+1,
+// update block column P + 1 (MFMA), factorise panel P + 1.  Meanwhile wavefronts 1 - 3 take the
+// rest of phase P: the other blocks of the U row and their block columns, X_PP = L_PP^{-1},
+// Z_PP = U_PP^{-1} (the 16-column triangular solves use four lanes per column, rows spread over
+// the quad and the solved row broadcast by DPP), and the off-diagonal inverse blocks that have
+// become computable,
 //   X_ij = -X_ii sum_{k=j}^{i-1} L_ik X_kj,   Z_ji = -(sum_{k=j}^{i-1} Z_jk U_ki) Z_ii,
-// 16 x 16 complex products on MFMA, one block per wavefront.  Linv receives X and UinvT receives
-// Z^T (column-major, ld 64).  The tile is always a full 64 x 64 block (the factorised matrix is
+// as 16 x 16 products on MFMA.  One barrier per phase.  Linv receives X and UinvT receives Z^T
+// (column-major, ld 64).  The tile is always a full 64 x 64 block (the factorised matrix is
 // bordered to whole blocks).
 constexpr int TP = NB + 1; // padded row pitch of the LDS tiles (float2)
 struct DiagLds {
@@ -253,100 +253,145 @@ __device__ __forceinline__ void mma16(Acc16& c, FA a, FB b)
     }
 }
 
-// One panel of the blocked factorisation (compile-time panel index P).
+// Panel P factorised by wavefront 0: lane l holds row 16 P + l.
 template <int P>
-__device__ __forceinline__ void diag_panel(DiagLds& S, int w, int l, int* __restrict__ bad)
+__device__ __forceinline__ void diag_a1(DiagLds& S, int l, int* __restrict__ bad)
+{
+    constexpr int b0 = 16 * P;
+    constexpr int nr = NB - b0;
+    const float2 zero = make_float2(0.f, 0.f);
+    const bool act = l < nr;
+    const int row = act ? b0 + l : b0;
+    float2 a[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) a[c] = csel(act, S.T[row][b0 + c], zero);
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const float pr = lane_read(a[j].x, j), pi = lane_read(a[j].y, j);
+        const float d = pr * pr + pi * pi;
+        if (l == 0 && !(d > 0.f && isfinite(pr) && isfinite(pi))) *bad = 1;
+        const float rd = __builtin_amdgcn_rcpf(d);
+        const float2 ip = make_float2(pr * rd, -pi * rd);
+        float2 u[16];
+#pragma unroll
+        for (int c = j + 1; c < 16; ++c) u[c] = make_float2(lane_read(a[c].x, j), lane_read(a[c].y, j));
+        if (l > j) {
+            const float2 m = cmul(a[j], ip);
+            a[j] = m;
+#pragma unroll
+            for (int c = j + 1; c < 16; ++c) a[c] = cfms(a[c], m, u[c]);
+        }
+    }
+    if (act) {
+#pragma unroll
+        for (int c = 0; c < 16; ++c) S.T[row][b0 + c] = a[c];
+    }
+}
+
+// Value of quad lane SRC to all four lanes of each quad (DPP quad_perm [SRC, SRC, SRC, SRC]).
+template <int SRC>
+__device__ __forceinline__ float2 quad_bcast_c(float2 v)
+{
+    constexpr int ctrl = SRC * 0x55;
+    return make_float2(
+        __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v.x), ctrl, 0xF, 0xF, false)),
+        __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v.y), ctrl, 0xF, 0xF, false)));
+}
+// (src is a constant once the caller's loop is unrolled; the switch folds away)
+__device__ __forceinline__ float2 quad_bcast(float2 v, int src)
+{
+    switch (src) {
+    case 0: return quad_bcast_c<0>(v);
+    case 1: return quad_bcast_c<1>(v);
+    case 2: return quad_bcast_c<2>(v);
+    default: return quad_bcast_c<3>(v);
+    }
+}
+
+// x = L_PP^{-1} b for 16 columns by one wavefront: lane l takes column l / 4, rows 4 (l % 4) .. +3.
+// b(r, c) gives the right-hand side, st(r, c, v) stores the solution.
+template <int P, class FB, class FS>
+__device__ __forceinline__ void quad_lower(DiagLds& S, int l, FB b, FS st)
 {
     constexpr int b0 = 16 * P;
     const float2 zero = make_float2(0.f, 0.f);
-    // 1. panel factorisation by wavefront 0: lane l holds row b0 + l
-    if (w == 0) {
-        constexpr int nr = NB - b0;
-        const bool act = l < nr;
-        const int row = act ? b0 + l : b0;
-        float2 a[16];
+    const int c = l >> 2, q = l & 3;
+    float2 x[4];
 #pragma unroll
-        for (int c = 0; c < 16; ++c) a[c] = csel(act, S.T[row][b0 + c], zero);
+    for (int i = 0; i < 4; ++i) x[i] = b(4 * q + i, c);
 #pragma unroll
-        for (int j = 0; j < 16; ++j) {
-            const float pr = lane_read(a[j].x, j), pi = lane_read(a[j].y, j);
-            const float d = pr * pr + pi * pi;
-            if (l == 0 && !(d > 0.f && isfinite(pr) && isfinite(pi))) *bad = 1;
-            const float rd = __builtin_amdgcn_rcpf(d);
-            const float2 ip = make_float2(pr * rd, -pi * rd);
-            float2 u[16];
+    for (int j = 0; j < 16; ++j) {
+        const float2 xj = quad_bcast(x[j & 3], j >> 2);
 #pragma unroll
-            for (int c = j + 1; c < 16; ++c) u[c] = make_float2(lane_read(a[c].x, j), lane_read(a[c].y, j));
-            if (l > j) {
-                const float2 m = cmul(a[j], ip);
-                a[j] = m;
-#pragma unroll
-                for (int c = j + 1; c < 16; ++c) a[c] = cfms(a[c], m, u[c]);
-            }
-        }
-        if (act) {
-#pragma unroll
-            for (int c = 0; c < 16; ++c) S.T[row][b0 + c] = a[c];
+        for (int i = 0; i < 4; ++i) {
+            const float2 lv = csel(4 * q + i > j, S.T[b0 + 4 * q + i][b0 + j], zero);
+            x[i] = cfms(x[i], lv, xj);
         }
     }
-    __syncthreads();
-    // 2. block row of U and X_pp (wavefront 1), Z_pp (wavefront 2)
-    if (w == 1) {
-        constexpr int nu = NB - 16 - b0; // columns right of the diagonal block
-        const bool ucol = l < nu, xcol = l >= nu && l < nu + 16;
-        const int col = ucol ? b0 + 16 + l : b0;
-        const int e = l - nu;
-        float2 x[16];
 #pragma unroll
-        for (int i = 0; i < 16; ++i)
-            x[i] = csel(ucol, S.T[b0 + i][col], make_float2(i == e ? 1.f : 0.f, 0.f));
+    for (int i = 0; i < 4; ++i) st(4 * q + i, c, x[i]);
+}
+
+// Z_PP = U_PP^{-1} by one wavefront (back substitution, same lane layout), into W on and above
+// the diagonal.
+template <int P>
+__device__ __forceinline__ void quad_zinv(DiagLds& S, int l)
+{
+    constexpr int b0 = 16 * P;
+    const float2 zero = make_float2(0.f, 0.f);
+    const int c = l >> 2, q = l & 3;
+    float2 x[4];
 #pragma unroll
-        for (int j = 0; j < 16; ++j)
+    for (int i = 0; i < 4; ++i) x[i] = make_float2(4 * q + i == c ? 1.f : 0.f, 0.f);
 #pragma unroll
-            for (int i = j + 1; i < 16; ++i) x[i] = cfms(x[i], S.T[b0 + i][b0 + j], x[j]);
-        if (ucol) {
+    for (int j = 15; j >= 0; --j) {
+        const float2 inv = fast_crecip(S.T[b0 + j][b0 + j]);
+        x[j & 3] = csel(q == (j >> 2), cmul(x[j & 3], inv), x[j & 3]);
+        const float2 xj = quad_bcast(x[j & 3], j >> 2);
 #pragma unroll
-            for (int i = 0; i < 16; ++i) S.T[b0 + i][col] = x[i];
-        }
-        if (xcol) {
-#pragma unroll
-            for (int i = 1; i < 16; ++i)
-                if (i > e) S.W[b0 + i][b0 + e] = x[i];
-        }
-    } else if (w == 2) {
-        const int c = l & 15;
-        float2 z[16];
-#pragma unroll
-        for (int i = 0; i < 16; ++i) z[i] = make_float2(i == c ? 1.f : 0.f, 0.f);
-#pragma unroll
-        for (int i = 15; i >= 0; --i) {
-            z[i] = cmul(z[i], fast_crecip(S.T[b0 + i][b0 + i]));
-#pragma unroll
-            for (int k = 0; k < i; ++k) z[k] = cfms(z[k], S.T[b0 + k][b0 + i], z[i]);
-        }
-        if (l < 16) {
-#pragma unroll
-            for (int i = 0; i < 16; ++i)
-                if (i <= c) S.W[b0 + i][b0 + c] = z[i];
+        for (int i = 0; i < 4; ++i) {
+            const float2 uv = csel(4 * q + i < j, S.T[b0 + 4 * q + i][b0 + j], zero);
+            x[i] = cfms(x[i], uv, xj);
         }
     }
-    __syncthreads();
-    // 3. rank-16 update of the trailing blocks (bi, bj), bi, bj > P, one 16 x 16 block per wavefront
-    if (P < 3) {
-        constexpr int m = 3 - P;
-        for (int tile = w; tile < m * m; tile += 4) {
-            const int bi = P + 1 + tile % m, bj = P + 1 + tile / m;
-            Acc16 c;
-            acc16_zero(c);
-            mma16(c, [&](int i, int k) { return S.T[16 * bi + i][b0 + k]; },
-                  [&](int k, int j) { return S.T[b0 + k][16 * bj + j]; });
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                float2& v = S.T[16 * bi + 4 * (l >> 4) + r][16 * bj + (l & 15)];
-                v = make_float2(v.x - c.re[r], v.y - c.im[r]);
-            }
-        }
-        __syncthreads();
+    for (int i = 0; i < 4; ++i)
+        if (4 * q + i <= c) S.W[b0 + 4 * q + i][b0 + c] = x[i];
+}
+
+// U_P,Q = L_PP^{-1} T_P,Q in place
+template <int P>
+__device__ __forceinline__ void diag_urow(DiagLds& S, int l, int Q)
+{
+    constexpr int b0 = 16 * P;
+    quad_lower<P>(S, l, [&](int r, int c) { return S.T[b0 + r][16 * Q + c]; },
+                  [&](int r, int c, float2 v) { S.T[b0 + r][16 * Q + c] = v; });
+}
+
+// X_PP = L_PP^{-1}, its strict lower part into W
+template <int P>
+__device__ __forceinline__ void diag_xinv(DiagLds& S, int l)
+{
+    constexpr int b0 = 16 * P;
+    quad_lower<P>(S, l, [](int r, int c) { return make_float2(r == c ? 1.f : 0.f, 0.f); },
+                  [&](int r, int c, float2 v) {
+                      if (r > c) S.W[b0 + r][b0 + c] = v;
+                  });
+}
+
+// T_bi,bj -= L_bi,P U_P,bj (16 x 16 x 16 on MFMA, one wavefront)
+template <int P>
+__device__ __forceinline__ void diag_update(DiagLds& S, int l, int bi, int bj)
+{
+    constexpr int b0 = 16 * P;
+    Acc16 c;
+    acc16_zero(c);
+    mma16(c, [&](int i, int k) { return S.T[16 * bi + i][b0 + k]; },
+          [&](int k, int j) { return S.T[b0 + k][16 * bj + j]; });
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        float2& v = S.T[16 * bi + 4 * (l >> 4) + r][16 * bj + (l & 15)];
+        v = make_float2(v.x - c.re[r], v.y - c.im[r]);
     }
 }
 
@@ -422,20 +467,72 @@ __device__ __forceinline__ void diag_factor(float2* __restrict__ A, int64_t ld, 
         S.W[r][c] = make_float2(0.f, 0.f);
     }
     __syncthreads();
-    diag_panel<0>(S, w, l, bad);
-    diag_panel<1>(S, w, l, bad);
-    diag_panel<2>(S, w, l, bad);
-    diag_panel<3>(S, w, l, bad);
-    // off-diagonal inverse blocks by levels: (kind, i, j) per wavefront
-    constexpr int lv1[6][3] = {{0, 1, 0}, {0, 2, 1}, {0, 3, 2}, {1, 1, 0}, {1, 2, 1}, {1, 3, 2}};
-    constexpr int lv2[4][3] = {{0, 2, 0}, {0, 3, 1}, {1, 2, 0}, {1, 3, 1}};
-    constexpr int lv3[2][3] = {{0, 3, 0}, {1, 3, 0}};
-    diag_inverse_block(S, w, lv1[w][0], lv1[w][1], lv1[w][2]);
-    if (w < 2) diag_inverse_block(S, w, lv1[w + 4][0], lv1[w + 4][1], lv1[w + 4][2]);
+    if (w == 0) diag_a1<0>(S, l, bad);
     __syncthreads();
-    diag_inverse_block(S, w, lv2[w][0], lv2[w][1], lv2[w][2]);
+    // phase 0
+    if (w == 0) {
+        diag_urow<0>(S, l, 1);
+        for (int bi = 1; bi < 4; ++bi) diag_update<0>(S, l, bi, 1);
+        diag_a1<1>(S, l, bad);
+    } else if (w == 1) {
+        diag_urow<0>(S, l, 2);
+        for (int bi = 1; bi < 4; ++bi) diag_update<0>(S, l, bi, 2);
+    } else if (w == 2) {
+        diag_urow<0>(S, l, 3);
+        for (int bi = 1; bi < 4; ++bi) diag_update<0>(S, l, bi, 3);
+    } else {
+        diag_xinv<0>(S, l);
+        quad_zinv<0>(S, l);
+    }
     __syncthreads();
-    if (w < 2) diag_inverse_block(S, w, lv3[w][0], lv3[w][1], lv3[w][2]);
+    // phase 1
+    if (w == 0) {
+        diag_urow<1>(S, l, 2);
+        for (int bi = 2; bi < 4; ++bi) diag_update<1>(S, l, bi, 2);
+        diag_a1<2>(S, l, bad);
+    } else if (w == 1) {
+        diag_urow<1>(S, l, 3);
+        for (int bi = 2; bi < 4; ++bi) diag_update<1>(S, l, bi, 3);
+    } else if (w == 2) {
+        diag_xinv<1>(S, l);
+        diag_inverse_block(S, w, 0, 1, 0);
+    } else {
+        quad_zinv<1>(S, l);
+        diag_inverse_block(S, w, 1, 1, 0);
+    }
+    __syncthreads();
+    // phase 2
+    if (w == 0) {
+        diag_urow<2>(S, l, 3);
+        diag_update<2>(S, l, 3, 3);
+        diag_a1<3>(S, l, bad);
+    } else if (w == 2) {
+        diag_xinv<2>(S, l);
+        diag_inverse_block(S, w, 0, 2, 1);
+        diag_inverse_block(S, w, 0, 2, 0);
+    } else if (w == 3) {
+        quad_zinv<2>(S, l);
+        diag_inverse_block(S, w, 1, 2, 1);
+        diag_inverse_block(S, w, 1, 2, 0);
+    }
+    __syncthreads();
+    // phase 3: the last diagonal inverses, then block row 3 of X and block column 3 of Z
+    if (w == 0)
+        diag_xinv<3>(S, l);
+    else if (w == 1)
+        quad_zinv<3>(S, l);
+    __syncthreads();
+    if (w == 0) {
+        diag_inverse_block(S, w, 0, 3, 0);
+    } else if (w == 1) {
+        diag_inverse_block(S, w, 1, 3, 0);
+    } else if (w == 2) {
+        diag_inverse_block(S, w, 0, 3, 2);
+        diag_inverse_block(S, w, 0, 3, 1);
+    } else {
+        diag_inverse_block(S, w, 1, 3, 2);
+        diag_inverse_block(S, w, 1, 3, 1);
+    }
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
