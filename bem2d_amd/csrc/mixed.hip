@@ -220,6 +220,7 @@ struct DiagLds {
     float2 T[NB][TP];       // the tile; L (unit diagonal implied) and U in place
     float2 W[NB][TP];       // X = L^{-1} strictly below the diagonal, Z = U^{-1} on and above it
     float2 S[4][16][17];    // per-wavefront 16 x 16 scratch
+    float2 S3[6][16][17];   // sums of block row 3 of X (0 - 2) and block column 3 of Z (3 - 5)
 };
 
 __device__ __forceinline__ float lane_read(float v, int lane)
@@ -242,14 +243,18 @@ template <class FA, class FB>
 __device__ __forceinline__ void mma16(Acc16& c, FA a, FB b)
 {
     const int l = threadIdx.x & 63, li = l & 15, lk = l >> 4;
+    float2 av[4], bv[4];
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
-        const float2 av = a(li, 4 * s + lk);
-        const float2 bv = b(4 * s + lk, li);
-        c.re = __builtin_amdgcn_mfma_f32_16x16x4f32(av.x, bv.x, c.re, 0, 0, 0);
-        c.re = __builtin_amdgcn_mfma_f32_16x16x4f32(-av.y, bv.y, c.re, 0, 0, 0);
-        c.im = __builtin_amdgcn_mfma_f32_16x16x4f32(av.x, bv.y, c.im, 0, 0, 0);
-        c.im = __builtin_amdgcn_mfma_f32_16x16x4f32(av.y, bv.x, c.im, 0, 0, 0);
+        av[s] = a(li, 4 * s + lk);
+        bv[s] = b(4 * s + lk, li);
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        c.re = __builtin_amdgcn_mfma_f32_16x16x4f32(av[s].x, bv[s].x, c.re, 0, 0, 0);
+        c.im = __builtin_amdgcn_mfma_f32_16x16x4f32(av[s].x, bv[s].y, c.im, 0, 0, 0);
+        c.re = __builtin_amdgcn_mfma_f32_16x16x4f32(-av[s].y, bv[s].y, c.re, 0, 0, 0);
+        c.im = __builtin_amdgcn_mfma_f32_16x16x4f32(av[s].y, bv[s].x, c.im, 0, 0, 0);
     }
 }
 
@@ -265,11 +270,14 @@ __device__ __forceinline__ void diag_a1(DiagLds& S, int l, int* __restrict__ bad
     float2 a[16];
 #pragma unroll
     for (int c = 0; c < 16; ++c) a[c] = csel(act, S.T[row][b0 + c], zero);
+    // pivot check after the loop: the smallest |p|^2 must be positive and their sum finite
+    float dmin = INFINITY, dsum = 0.f;
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
         const float pr = lane_read(a[j].x, j), pi = lane_read(a[j].y, j);
         const float d = pr * pr + pi * pi;
-        if (l == 0 && !(d > 0.f && isfinite(pr) && isfinite(pi))) *bad = 1;
+        dmin = fminf(dmin, d);
+        dsum += d;
         const float rd = __builtin_amdgcn_rcpf(d);
         const float2 ip = make_float2(pr * rd, -pi * rd);
         float2 u[16];
@@ -286,6 +294,7 @@ __device__ __forceinline__ void diag_a1(DiagLds& S, int l, int* __restrict__ bad
 #pragma unroll
         for (int c = 0; c < 16; ++c) S.T[row][b0 + c] = a[c];
     }
+    if (l == 0 && !(dmin > 0.f && isfinite(dsum))) *bad = 1;
 }
 
 // Value of quad lane SRC to all four lanes of each quad (DPP quad_perm [SRC, SRC, SRC, SRC]).
@@ -309,13 +318,23 @@ __device__ __forceinline__ float2 quad_bcast(float2 v, int src)
 }
 
 // x = L_PP^{-1} b for 16 columns by one wavefront: lane l takes column l / 4, rows 4 (l % 4) .. +3.
-// b(r, c) gives the right-hand side, st(r, c, v) stores the solution.
+// b(r, c) gives the right-hand side, st(r, c, v) stores the solution.  The lane's 64 coefficients
+// L[4 q + i][j] (zero on and above the diagonal) are read before the 16 dependent steps.
 template <int P, class FB, class FS>
 __device__ __forceinline__ void quad_lower(DiagLds& S, int l, FB b, FS st)
 {
     constexpr int b0 = 16 * P;
     const float2 zero = make_float2(0.f, 0.f);
     const int c = l >> 2, q = l & 3;
+    float2 lv[4][16];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 16; ++j) lv[i][j] = S.T[b0 + 4 * q + i][b0 + j];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 16; ++j) lv[i][j] = csel(4 * q + i > j, lv[i][j], zero);
     float2 x[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) x[i] = b(4 * q + i, c);
@@ -323,10 +342,7 @@ __device__ __forceinline__ void quad_lower(DiagLds& S, int l, FB b, FS st)
     for (int j = 0; j < 16; ++j) {
         const float2 xj = quad_bcast(x[j & 3], j >> 2);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const float2 lv = csel(4 * q + i > j, S.T[b0 + 4 * q + i][b0 + j], zero);
-            x[i] = cfms(x[i], lv, xj);
-        }
+        for (int i = 0; i < 4; ++i) x[i] = cfms(x[i], lv[i][j], xj);
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i) st(4 * q + i, c, x[i]);
@@ -340,19 +356,26 @@ __device__ __forceinline__ void quad_zinv(DiagLds& S, int l)
     constexpr int b0 = 16 * P;
     const float2 zero = make_float2(0.f, 0.f);
     const int c = l >> 2, q = l & 3;
+    float2 uv[4][16], inv[16];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 16; ++j) uv[i][j] = S.T[b0 + 4 * q + i][b0 + j];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 16; ++j) uv[i][j] = csel(4 * q + i < j, uv[i][j], zero);
+#pragma unroll
+    for (int j = 0; j < 16; ++j) inv[j] = fast_crecip(S.T[b0 + j][b0 + j]);
     float2 x[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) x[i] = make_float2(4 * q + i == c ? 1.f : 0.f, 0.f);
 #pragma unroll
     for (int j = 15; j >= 0; --j) {
-        const float2 inv = fast_crecip(S.T[b0 + j][b0 + j]);
-        x[j & 3] = csel(q == (j >> 2), cmul(x[j & 3], inv), x[j & 3]);
+        x[j & 3] = csel(q == (j >> 2), cmul(x[j & 3], inv[j]), x[j & 3]);
         const float2 xj = quad_bcast(x[j & 3], j >> 2);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const float2 uv = csel(4 * q + i < j, S.T[b0 + 4 * q + i][b0 + j], zero);
-            x[i] = cfms(x[i], uv, xj);
-        }
+        for (int i = 0; i < 4; ++i) x[i] = cfms(x[i], uv[i][j], xj);
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i)
@@ -395,45 +418,53 @@ __device__ __forceinline__ void diag_update(DiagLds& S, int l, int bi, int bj)
     }
 }
 
-// Off-diagonal inverse block: kind 0 is X_ij (i > j), kind 1 is Z_ji (j < i).
-__device__ __forceinline__ void diag_inverse_block(DiagLds& S, int w, int kind, int i, int j)
+// Off-diagonal inverse blocks: kind 0 is X_ij (i > j), kind 1 is Z_ji (j < i).  inv_sum leaves
+// S_ij = sum_{k=j}^{i-1} L_ik X_kj (kind 0) or sum_{k=j}^{i-1} Z_jk U_ki (kind 1) in a 16 x 16
+// scratch block; inv_apply stores X_ij = -X_ii S_ij or Z_ji = -S_ij Z_ii into W.
+__device__ __forceinline__ auto inv_X(DiagLds& S, int k, int jj)
+{
+    // X_kj with its unit diagonal and zero upper part when k == j
+    return [&S, k, jj](int r, int c) {
+        const float2 v = S.W[16 * k + r][16 * jj + c];
+        if (k > jj || r > c) return v;
+        return make_float2(r == c ? 1.f : 0.f, 0.f);
+    };
+}
+__device__ __forceinline__ auto inv_Z(DiagLds& S, int jj, int k)
+{
+    // Z_jk with its zero lower part when j == k
+    return [&S, jj, k](int r, int c) {
+        const float2 v = S.W[16 * jj + r][16 * k + c];
+        return (jj < k || r <= c) ? v : make_float2(0.f, 0.f);
+    };
+}
+__device__ __forceinline__ void inv_sum(DiagLds& S, int kind, int i, int j, float2 (*sc)[17])
 {
     const int l = threadIdx.x & 63;
-    // X_kj with its unit diagonal and zero upper part when k == j; Z_jk with its zero lower part
-    auto X = [&](int k, int jj) {
-        return [&S, k, jj](int r, int c) {
-            const float2 v = S.W[16 * k + r][16 * jj + c];
-            if (k > jj || r > c) return v;
-            return make_float2(r == c ? 1.f : 0.f, 0.f);
-        };
-    };
-    auto Z = [&](int jj, int k) {
-        return [&S, jj, k](int r, int c) {
-            const float2 v = S.W[16 * jj + r][16 * k + c];
-            return (jj < k || r <= c) ? v : make_float2(0.f, 0.f);
-        };
-    };
-    auto T = [&](int bi, int bk) {
+    auto T = [&S](int bi, int bk) {
         return [&S, bi, bk](int r, int c) { return S.T[16 * bi + r][16 * bk + c]; };
     };
     Acc16 s;
     acc16_zero(s);
     for (int k = j; k < i; ++k) {
         if (kind == 0)
-            mma16(s, T(i, k), X(k, j));
+            mma16(s, T(i, k), inv_X(S, k, j));
         else
-            mma16(s, Z(j, k), T(k, i));
+            mma16(s, inv_Z(S, j, k), T(k, i));
     }
-    float2(*sc)[17] = S.S[w];
 #pragma unroll
     for (int r = 0; r < 4; ++r) sc[4 * (l >> 4) + r][l & 15] = make_float2(s.re[r], s.im[r]);
+}
+__device__ __forceinline__ void inv_apply(DiagLds& S, int kind, int i, int j, float2 (*sc)[17])
+{
+    const int l = threadIdx.x & 63;
     auto SC = [sc](int r, int c) { return sc[r][c]; };
     Acc16 o;
     acc16_zero(o);
     if (kind == 0)
-        mma16(o, X(i, i), SC);
+        mma16(o, inv_X(S, i, i), SC);
     else
-        mma16(o, SC, Z(i, i));
+        mma16(o, SC, inv_Z(S, i, i));
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int rr = 4 * (l >> 4) + r, cc = l & 15;
@@ -443,6 +474,11 @@ __device__ __forceinline__ void diag_inverse_block(DiagLds& S, int w, int kind, 
         else
             S.W[16 * j + rr][16 * i + cc] = v;
     }
+}
+__device__ __forceinline__ void diag_inverse_block(DiagLds& S, int w, int kind, int i, int j)
+{
+    inv_sum(S, kind, i, j, S.S[w]);
+    inv_apply(S, kind, i, j, S.S[w]);
 }
 
 __device__ __forceinline__ void diag_factor(float2* __restrict__ A, int64_t ld, int64_t k0, int kb,
@@ -510,28 +546,37 @@ __device__ __forceinline__ void diag_factor(float2* __restrict__ A, int64_t ld, 
         diag_xinv<2>(S, l);
         diag_inverse_block(S, w, 0, 2, 1);
         diag_inverse_block(S, w, 0, 2, 0);
+        // block row 3 of X needs only X_33 beyond this point: its sums now
+        for (int j = 0; j < 3; ++j) inv_sum(S, 0, 3, j, S.S3[j]);
     } else if (w == 3) {
         quad_zinv<2>(S, l);
         diag_inverse_block(S, w, 1, 2, 1);
         diag_inverse_block(S, w, 1, 2, 0);
     }
     __syncthreads();
-    // phase 3: the last diagonal inverses, then block row 3 of X and block column 3 of Z
-    if (w == 0)
+    // phase 3: the last diagonal inverses and the sums of block column 3 of Z (which need U_23,
+    // solved by wavefront 0 in phase 2), then the six blocks of row 3 of X and column 3 of Z
+    if (w == 0) {
         diag_xinv<3>(S, l);
-    else if (w == 1)
+    } else if (w == 1) {
         quad_zinv<3>(S, l);
+    } else if (w == 2) {
+        inv_sum(S, 1, 3, 0, S.S3[3]);
+    } else {
+        inv_sum(S, 1, 3, 1, S.S3[4]);
+        inv_sum(S, 1, 3, 2, S.S3[5]);
+    }
     __syncthreads();
     if (w == 0) {
-        diag_inverse_block(S, w, 0, 3, 0);
+        inv_apply(S, 0, 3, 0, S.S3[0]);
+        inv_apply(S, 0, 3, 1, S.S3[1]);
     } else if (w == 1) {
-        diag_inverse_block(S, w, 1, 3, 0);
+        inv_apply(S, 1, 3, 0, S.S3[3]);
+        inv_apply(S, 1, 3, 1, S.S3[4]);
     } else if (w == 2) {
-        diag_inverse_block(S, w, 0, 3, 2);
-        diag_inverse_block(S, w, 0, 3, 1);
+        inv_apply(S, 0, 3, 2, S.S3[2]);
     } else {
-        diag_inverse_block(S, w, 1, 3, 2);
-        diag_inverse_block(S, w, 1, 3, 1);
+        inv_apply(S, 1, 3, 2, S.S3[5]);
     }
     __syncthreads();
 #pragma unroll
@@ -692,8 +737,10 @@ __device__ __forceinline__ void slab_store4(__amdgpu_buffer_rsrc_t r, unsigned v
 // The trailing update stages its first operand k-major ([k][row], pitch KPA): the 64 consecutive
 // rows a wavefront writes and the 16 rows x 4 k a MFMA operand reads both fall in distinct banks.
 constexpr int KPA = NB + 16;
-constexpr int kUpdLdsFloats = 2 * NB * KPA + 2 * NB * KP;
-static_assert(sizeof(DiagLds) <= sizeof(float) * kUpdLdsFloats, "LDS reuse");
+constexpr int kUpdOpFloats = 2 * NB * KPA + 2 * NB * KP;
+constexpr int kUpdLdsFloats = kUpdOpFloats > (int)(sizeof(DiagLds) / sizeof(float))
+                                  ? kUpdOpFloats
+                                  : (int)(sizeof(DiagLds) / sizeof(float));
 
 // c += Aop Bop with Aop in LDS as [k][row] (pitch KPA) and Bop as [col][k] (pitch KP).  The two
 // MFMAs feeding one accumulator in a k-step are eight apart (no back-to-back dependency).
