@@ -149,6 +149,37 @@ __device__ __forceinline__ int batch_find(const int64_t* start, int count, int64
     return d;
 }
 
+// Evaluation modes of the regular pairs (kernel_pair).
+enum { PAIR_GENERAL = 0, PAIR_MID = 1, PAIR_FAR = 2 };
+
+// Loop constants of the regular pairs: k, 1/k, k/4, eta/4, and for the modes that know
+// k|x - y| >= kHankelMidX the amplitude factor sqrt(2/(pi k)) folded into k/4, eta/4 and 1/4
+// (H_nu carries sqrt(2/(pi k d)) = sqrt(2/(pi k)) / sqrt(d)) and the operands of hankel_tops.
+struct PairConst {
+    double k, kinv, k4, eta4;
+    double k4a, eta4a, a4;
+    HankelTops h;
+};
+
+template <int MODE>
+__device__ __forceinline__ PairConst pair_const(double k, double eta)
+{
+    PairConst pc;
+    pc.k = k;
+    pc.kinv = 1.0 / k;
+    pc.k4 = 0.25 * k;
+    pc.eta4 = 0.25 * eta;
+    const double amp = sqrt(2.0 * kInvPi * pc.kinv);
+    pc.k4a = pc.k4 * amp;
+    pc.eta4a = pc.eta4 * amp;
+    pc.a4 = 0.25 * amp;
+    if (MODE == PAIR_FAR)
+        pc.h = hankel_tops_vgpr<BEM2D_HANKEL_NPIECE>();
+    else if (MODE == PAIR_MID)
+        pc.h = hankel_tops_vgpr<BEM2D_HANKEL_NPIECE - 1>();
+    return pc;
+}
+
 // Kernel values of one regular node pair for both orders of its element pair: (kr, ki) with the
 // test node x on element i and the trial node y on element j, (mr, mi) with test node y on j and
 // trial node x on i.  The two share |x - y| and with it H0 and H1; only the normal factor differs,
@@ -156,13 +187,11 @@ __device__ __forceinline__ int batch_find(const int64_t* start, int count, int64
 // order.  MODE: PAIR_GENERAL evaluates H0, H1 by the piecewise code; PAIR_MID knows
 // k|x - y| >= kHankelMidX and PAIR_FAR knows k|x - y| >= kHankelFarX, and these use the one
 // polynomial set that covers the range, without a test.
-enum { PAIR_GENERAL = 0, PAIR_MID = 1, PAIR_FAR = 2 };
-
 template <int KIND, int MODE>
-__device__ __forceinline__ void kernel_pair(double k, double kinv, double k4, double eta4,
-                                            double xx, double xy, double yx, double yy,
-                                            double n1x, double n1y, double n2x, double n2y,
-                                            double& kr, double& ki, double& mr, double& mi)
+__device__ __forceinline__ void kernel_pair(const PairConst& pc, double xx, double xy, double yx,
+                                            double yy, double n1x, double n1y, double n2x,
+                                            double n2y, double& kr, double& ki, double& mr,
+                                            double& mi)
 {
     if (KIND == BEM2D_KERNEL_ID || KIND == BEM2D_KERNEL_NONE) {
         kr = mr = (KIND == BEM2D_KERNEL_ID) ? 1.0 : 0.0;
@@ -170,41 +199,50 @@ __device__ __forceinline__ void kernel_pair(double k, double kinv, double k4, do
         return;
     }
     double wx, wy, d2, nw1, nw2;
-    {
+    if (MODE == PAIR_GENERAL) {
+        // near pairs: plain multiply/add as in kernel_eval (n.w cancels for near nodes)
 #pragma clang fp contract(off)
         wx = xx - yx; // x - y
         wy = xy - yy;
         d2 = wx * wx + wy * wy;
         nw1 = n1x * wx + n1y * wy; // n_x.(x - y)
         nw2 = n2x * wx + n2y * wy; // n_y.(x - y)
+    } else {
+        // pairs at k|x - y| >= kHankelMidX: fused (the negation y - x stays exact either way)
+        wx = xx - yx;
+        wy = xy - yy;
+        d2 = fma(wx, wx, wy * wy);
+        nw1 = fma(n1x, wx, n1y * wy);
+        nw2 = fma(n2x, wx, n2y * wy);
     }
     const double rd = rsqrt_refined(d2);
     const double d = d2 * rd;
-    const double x = k * d, invx = kinv * rd;
+    const double x = pc.k * d, invx = pc.kinv * rd;
     if (MODE != PAIR_GENERAL) {
         // H0 = A (u0r + i u0i), H1 = A (u1r + i u1i) as in hankel01_combine, with the amplitude
-        // A = sqrt(2/(pi x)) folded into the scalar factors of the kernel
+        // A = +-sqrt(2/(pi k)) / sqrt(d) (the sign that of the half-turn reduction) and its
+        // constant factor folded into the scalar factors of the kernel
         double P0, Q0, P1, Q1, S, C;
         if (MODE == PAIR_FAR)
-            hankel_pq_far(invx, P0, Q0, P1, Q1);
+            hankel_pq_far(invx, P0, Q0, P1, Q1, pc.h);
         else
-            hankel_pq<BEM2D_HANKEL_NPIECE - 1>(invx, P0, Q0, P1, Q1);
-        sincos_minus_quarter(x, S, C);
-        const double A = rsqrt_refined(0.5 * kPi * x);
+            hankel_pq<BEM2D_HANKEL_NPIECE - 1>(invx, P0, Q0, P1, Q1, pc.h);
+        const int nt = sincos_half_turn(x, S, C, pc.h);
+        const double y = with_parity_sign(rsqrt_refined(d), nt); // A / sqrt(2/(pi k))
         const double u0r = P0 * C - Q0 * S, u0i = P0 * S + Q0 * C;
         const double u1r = P1 * S + Q1 * C, u1i = Q1 * S - P1 * C;
         if (KIND == BEM2D_KERNEL_SL) {
-            const double a4 = 0.25 * A;
+            const double a4 = pc.a4 * y;
             kr = mr = -a4 * u0i;
             ki = mi = a4 * u0r;
             return;
         }
-        const double ka = k4 * A * rd;
+        const double ka = pc.k4a * y * rd;
         const double c = ka * (KIND == BEM2D_KERNEL_DL ? -nw2 : nw1);
         const double cm = ka * (KIND == BEM2D_KERNEL_DL ? nw1 : -nw2);
         double gr = 0.0, gi = 0.0;
         if (KIND == BEM2D_KERNEL_COMBINED) {
-            const double ea = eta4 * A;
+            const double ea = pc.eta4a * y;
             gr = ea * u0r;
             gi = ea * u0i;
         }
@@ -216,6 +254,7 @@ __device__ __forceinline__ void kernel_pair(double k, double kinv, double k4, do
     }
     double h0r, h0i, h1r, h1i;
     hankel01_inv(x, invx, h0r, h0i, h1r, h1i);
+    const double k4 = pc.k4, eta4 = pc.eta4;
     if (KIND == BEM2D_KERNEL_SL) {
         kr = mr = -0.25 * h0i;
         ki = mi = 0.25 * h0r;
@@ -272,10 +311,10 @@ __host__ __device__ __forceinline__ int64_t upper_tile_count(int64_t n)
 // entry (i, j), in the order of the reference's Integrate; mirror (mr, mi) for entry (j, i).
 template <int KIND, int NQ, int MODE>
 __device__ __forceinline__ void pair_sums(int nq, int64_t n, int64_t i, const double* __restrict__ nodes,
-                                          const double* sc, double k, double kinv, double k4,
-                                          double eta4, double& ar, double& ai, double& mr,
-                                          double& mi)
+                                          const double* sc, double k, double eta, double& ar,
+                                          double& ai, double& mr, double& mi)
 {
+    const PairConst pc = pair_const<MODE>(k, eta);
     const int64_t plane = n * nq;
     ar = ai = mr = mi = 0.0;
     for (int a = 0; a < nq; ++a) {
@@ -287,8 +326,8 @@ __device__ __forceinline__ void pair_sums(int nq, int64_t n, int64_t i, const do
         for (int b = 0; b < nq; ++b) {
             const double* y = sc + b * 5;
             double kr, ki, kmr, kmi;
-            kernel_pair<KIND, MODE>(k, kinv, k4, eta4, xx, xy, y[0], y[1], n1x, n1y, y[2], y[3],
-                                    kr, ki, kmr, kmi);
+            kernel_pair<KIND, MODE>(pc, xx, xy, y[0], y[1], n1x, n1y, y[2], y[3], kr, ki, kmr,
+                                    kmi);
             br = fma(kr, y[4], br);
             bi = fma(ki, y[4], bi);
             cr = fma(kmr, y[4], cr);
@@ -354,7 +393,6 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void assemble_regular_kernel(c
     __syncthreads();
     if (i >= n) return;
 
-    const double k4 = 0.25 * k, eta4 = 0.25 * eta, kinv = 1.0 / k;
     const int64_t idx_i = index[i], nx_i = next[i], pv_i = prev[i];
     const double cx_i = bounds[BOUND_CX * n + i], cy_i = bounds[BOUND_CY * n + i];
     const double r_i = bounds[BOUND_R * n + i];
@@ -379,12 +417,11 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void assemble_regular_kernel(c
         const double* sc = sh + c * nq * 5;
         double ar, ai, mr, mi;
         if (all_far)
-            pair_sums<KIND, NQ, PAIR_FAR>(nq, n, i, nodes, sc, k, kinv, k4, eta4, ar, ai, mr, mi);
+            pair_sums<KIND, NQ, PAIR_FAR>(nq, n, i, nodes, sc, k, eta, ar, ai, mr, mi);
         else if (all_mid)
-            pair_sums<KIND, NQ, PAIR_MID>(nq, n, i, nodes, sc, k, kinv, k4, eta4, ar, ai, mr, mi);
+            pair_sums<KIND, NQ, PAIR_MID>(nq, n, i, nodes, sc, k, eta, ar, ai, mr, mi);
         else
-            pair_sums<KIND, NQ, PAIR_GENERAL>(nq, n, i, nodes, sc, k, kinv, k4, eta4, ar, ai, mr,
-                                              mi);
+            pair_sums<KIND, NQ, PAIR_GENERAL>(nq, n, i, nodes, sc, k, eta, ar, ai, mr, mi);
         if (direct) A[i + lda * j] = make_double2(kscale * ar, kscale * ai);
         if (mirror) A[j + lda * i] = make_double2(kscale * mr, kscale * mi);
     }

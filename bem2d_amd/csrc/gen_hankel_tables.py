@@ -18,7 +18,9 @@ arrays.  Two more tables serve the far-field fast path of the assembly:
         polynomials of degree 4 in w = 1/x^2 (the affine map to z folded into the coefficients);
   TRIG  sin r = r + r^3 S(r^2), cos r = 1 + r^2 C(r^2) on |r| <= pi/4, S of degree 5 and C of
         degree 6, interpolated at Chebyshev nodes in r^2 (near-minimax; 1.4e-17 relative and
-        1.2e-19 absolute), plus 2/pi and pi/2 split into three doubles for the reduction.
+        1.2e-19 absolute), plus 2/pi and pi/2 split into three doubles for the reduction;
+  TRIG2 the same on |r| <= pi/2 (the half-turn reduction of the Hankel phase), S and C of degree 7
+        (1.3e-18 and 1.6e-17 absolute), plus 1/pi and pi split into three doubles.
 The fits are Chebyshev interpolants computed at 40 digits from mpmath's hankel1 and converted
 to the monomial basis in z; the script reports the float64 Horner error of every table.
 """
@@ -214,7 +216,7 @@ def main():
     # ---- trig kernels -----------------------------------------------------------------------------
     R2 = (mp.pi / 4) ** 2
 
-    def interp(fn, deg):
+    def interp(fn, deg, R2=R2):
         nodes = [R2 / 2 + R2 / 2 * mp.cos(mp.pi * (j + mp.mpf("0.5")) / (deg + 1))
                  for j in range(deg + 1)]
         A = mp.matrix([[u ** kk for kk in range(deg + 1)] for u in nodes])
@@ -235,6 +237,27 @@ def main():
     emit("#define BEM2D_TRIG_NS 6")
     emit("#define BEM2D_TRIG_NC 7")
     emit("#define BEM2D_TRIG_INIT {%s}" % ", ".join(fmt(v) for v in trig))
+    # half-turn kernels: |r| <= pi/2 (a hair beyond, for the rounding of the reduction)
+    H2 = (mp.pi / 2 * (1 + mp.mpf("1e-9"))) ** 2
+    S2 = interp(lambda u: (mp.sin(mp.sqrt(u)) - mp.sqrt(u)) / (u * mp.sqrt(u)), 7, H2)
+    C2 = interp(lambda u: (mp.cos(mp.sqrt(u)) - 1) / u, 7, H2)
+    for name, fn, cf, scale in (("sin", lambda u: (mp.sin(mp.sqrt(u)) - mp.sqrt(u)) / (u * mp.sqrt(u)),
+                                 S2, lambda u: u * mp.sqrt(u)),
+                                ("cos", lambda u: (mp.cos(mp.sqrt(u)) - 1) / u, C2, lambda u: u)):
+        err = max(abs((fn(u) - sum(cf[kk] * u ** kk for kk in range(len(cf)))) * scale(u))
+                  for u in [H2 * mp.mpf(t) for t in np.linspace(1e-3, 1.0, 400)])
+        print("half-turn %s: max |err| = %.3g" % (name, float(err)))
+    with mp.workdps(80):
+        PI1 = float(mp.pi)
+        PI2 = float(mp.pi - mp.mpf(PI1))
+        PI3 = float(mp.pi - mp.mpf(PI1) - mp.mpf(PI2))
+        oopi = float(1 / mp.pi)
+    trig2 = [oopi, PI1, PI2, PI3] + [float(c) for c in reversed(S2)] + \
+        [float(c) for c in reversed(C2)]
+    emit("// [1/pi, PI1, PI2, PI3, S7..S0, C7..C0]: the same on |r| <= pi/2")
+    emit("#define BEM2D_TRIG2_NS 8")
+    emit("#define BEM2D_TRIG2_NC 8")
+    emit("#define BEM2D_TRIG2_INIT {%s}" % ", ".join(fmt(v) for v in trig2))
     emit("// worst float64 Horner error of the fitted tables (absolute, on P and Q): %.3g" % worst)
     with open(os.path.join(HERE, "hankel_tables.h"), "w") as fh:
         fh.write("\n".join(lines) + "\n")

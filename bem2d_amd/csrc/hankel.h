@@ -11,7 +11,9 @@
 // On the device every coefficient lives in one __constant__ table and every index is a
 // compile-time constant, so the coefficients reach v_fma_f64 as SGPR operands through scalar loads
 // (instruction literals would each cost two v_mov_b32 per use).  The last piece is evaluated in
-// w = 1/x^2 directly (hankel_pq_far).  sin/cos of the phase use sincos_minus_quarter below.  Accuracy: about 3e-16 relative to |H| (pinned against mpmath by the tests).
+// w = 1/x^2 directly (hankel_pq_far).  The phase x - pi/4 is reduced by whole turns of pi
+// (sincos_half_turn), the sign (-1)^n of the reduction folded into the amplitude.  Accuracy: about
+// 3e-16 relative to |H| (pinned against mpmath by the tests).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -34,14 +36,16 @@ struct HankelTables {
     // 2/pi, pi/2 = P1 + P2 + P3, then S (degree 5) and C (degree 6) of
     // sin r = r + r^3 S(r^2), cos r = 1 + r^2 C(r^2), highest power first
     double trig[4 + BEM2D_TRIG_NS + BEM2D_TRIG_NC];
+    // 1/pi, pi = P1 + P2 + P3, then S and C (degree 7 each) of the same on |r| <= pi/2
+    double trig2[4 + BEM2D_TRIG2_NS + BEM2D_TRIG2_NC];
 };
 
 static constexpr HankelTables kHankelHost = {BEM2D_HANKEL_SER_INIT, BEM2D_HANKEL_ZMAP_INIT,
                                              BEM2D_HANKEL_FIT_INIT, BEM2D_HANKEL_FAR_INIT,
-                                             BEM2D_TRIG_INIT};
+                                             BEM2D_TRIG_INIT, BEM2D_TRIG2_INIT};
 static __constant__ HankelTables kHankelDev = {BEM2D_HANKEL_SER_INIT, BEM2D_HANKEL_ZMAP_INIT,
                                                BEM2D_HANKEL_FIT_INIT, BEM2D_HANKEL_FAR_INIT,
-                                               BEM2D_TRIG_INIT};
+                                               BEM2D_TRIG_INIT, BEM2D_TRIG2_INIT};
 
 #ifdef __HIP_DEVICE_COMPILE__
 // Device view of the table: a constant-address-space (4) pointer, so that uniform reads become
@@ -90,16 +94,70 @@ __host__ __device__ __forceinline__ double horner(Ptr c, double z)
     return r;
 }
 
-// 1/a and 1/sqrt(a) for finite a > 0: the hardware estimates (about 2^-23 relative) refined by two
-// Newton steps to within ~1 ulp.  On the host, the C library.
+// Leading operands of the phase and P/Q polynomials.  An instruction reads at most one SGPR
+// operand, so a first Horner step c_top z + c_next with both coefficients from the table costs a
+// multiply and an add (horner above); the quadrature loops hold these few values in VGPRs
+// (hankel_tops_vgpr) and pay one FMA per polynomial instead, and the reduction's -1/4 no longer
+// has to be copied into VGPRs at every use.
+struct HankelTops {
+    double mquarter; // -1/4 of the half-turn reduction
+    double trig[2];  // leading coefficients of the half-turn S and C
+    double pq[4];    // leading coefficients of P0, xQ0, P1, xQ1 of one table
+};
+
+// TABLE: a piece 0 .. BEM2D_HANKEL_NPIECE - 1, or BEM2D_HANKEL_NPIECE for the far table
+template <int TABLE>
+__host__ __device__ __forceinline__ HankelTops hankel_tops()
+{
+    const auto& T = hankel_tables();
+    HankelTops h;
+    h.mquarter = -0.25;
+    h.trig[0] = T.trig2[4];
+    h.trig[1] = T.trig2[4 + BEM2D_TRIG2_NS];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        if constexpr (TABLE == BEM2D_HANKEL_NPIECE)
+            h.pq[i] = T.far[i][BEM2D_HANKEL_FAR_DEG];
+        else
+            h.pq[i] = T.fit[TABLE][i][BEM2D_PIECE_DEG[TABLE]];
+    }
+    return h;
+}
+
+// The same values in VGPRs: the empty asm hides that they are uniform, so they stay in vector
+// registers across the loop that uses them.
+template <int TABLE>
+__host__ __device__ __forceinline__ HankelTops hankel_tops_vgpr()
+{
+    HankelTops h = hankel_tops<TABLE>();
+#ifdef __HIP_DEVICE_COMPILE__
+    asm volatile("" : "+v"(h.mquarter), "+v"(h.trig[0]), "+v"(h.trig[1]));
+    asm volatile("" : "+v"(h.pq[0]), "+v"(h.pq[1]), "+v"(h.pq[2]), "+v"(h.pq[3]));
+#endif
+    return h;
+}
+
+// Horner's rule with the leading coefficient given apart (top z + c[N - 2] as one FMA).
+template <int N, typename Ptr>
+__host__ __device__ __forceinline__ double horner_top(Ptr c, double top, double z)
+{
+    double r = fma_vvs(z, top, c[N - 2]);
+#pragma unroll
+    for (int i = N - 3; i >= 0; --i) r = fma_vvs(r, z, c[i]);
+    return r;
+}
+
+// 1/a and 1/sqrt(a) for finite a > 0: the hardware estimates (V_RCP_F64 / V_RSQ_F64, within 2^29
+// ulp = 2^-23 relative) refined by one third-order step to within ~1 ulp.  With e = 1 - a y
+// (resp. 1 - a y^2), |e| <= 2^-22: 1/a = y (1 + e + e^2 + ...) and 1/sqrt(a) = y (1 + e/2 +
+// 3e^2/8 + ...); the first omitted terms, e^3 and 5/16 e^3, are below 2^-65.  On the host, the C
+// library.
 __host__ __device__ __forceinline__ double rcp_refined(double a)
 {
 #ifdef __HIP_DEVICE_COMPILE__
-    double y = __builtin_amdgcn_rcp(a);
-    double e = fma(-a, y, 1.0);
-    y = fma(y, e, y);
-    e = fma(-a, y, 1.0);
-    return fma(y, e, y);
+    const double y = __builtin_amdgcn_rcp(a);
+    const double e = fma(-a, y, 1.0);
+    return fma(y, fma(e, e, e), y);
 #else
     return 1.0 / a;
 #endif
@@ -108,13 +166,9 @@ __host__ __device__ __forceinline__ double rcp_refined(double a)
 __host__ __device__ __forceinline__ double rsqrt_refined(double a)
 {
 #ifdef __HIP_DEVICE_COMPILE__
-    double y = __builtin_amdgcn_rsq(a);
-    double h = a * y;
-    double e = fma(-h, y, 1.0); // 1 - a y^2
-    y = fma(0.5 * y, e, y);
-    h = a * y;
-    e = fma(-h, y, 1.0);
-    return fma(0.5 * y, e, y);
+    const double y = __builtin_amdgcn_rsq(a);
+    const double e = fma(-(a * y), y, 1.0); // 1 - a y^2
+    return fma(y, e * fma(e, 0.375, 0.5), y);
 #else
     return 1.0 / sqrt(a);
 #endif
@@ -170,62 +224,84 @@ __host__ __device__ __forceinline__ void hankel01_series(double x, double& h0r, 
     h0r = j0; h0i = y0; h1r = j1; h1i = y1;
 }
 
-// P_nu, Q_nu of piece P at 1/x.
+// P_nu, Q_nu of piece P at 1/x; h = hankel_tops<P>() (from the table or held in VGPRs).
 template <int P>
 __host__ __device__ __forceinline__ void hankel_pq(double invx, double& P0, double& Q0, double& P1,
-                                                   double& Q1)
+                                                   double& Q1, const HankelTops& h)
 {
     const auto& T = hankel_tables();
     constexpr int N = BEM2D_PIECE_DEG[P] + 1;
     const double y = T.zmap[P][0] * invx * invx;
     const double z = (y - T.zmap[P][1]) * T.zmap[P][2];
-    P0 = horner<N>(T.fit[P][0], z);
-    Q0 = horner<N>(T.fit[P][1], z) * invx;
-    P1 = horner<N>(T.fit[P][2], z);
-    Q1 = horner<N>(T.fit[P][3], z) * invx;
+    P0 = horner_top<N>(T.fit[P][0], h.pq[0], z);
+    Q0 = horner_top<N>(T.fit[P][1], h.pq[1], z) * invx;
+    P1 = horner_top<N>(T.fit[P][2], h.pq[2], z);
+    Q1 = horner_top<N>(T.fit[P][3], h.pq[3], z) * invx;
 }
 
-// sin and cos of x - pi/4 for 0 <= x < 2^30: the reduction of sincos_reduced applied to
-// x - (n + 1/2) pi/2 with n = rint(2x/pi - 1/2), so the pi/4 shift costs nothing.
-__host__ __device__ __forceinline__ void sincos_minus_quarter(double x, double& s, double& c)
+// sin r and cos r of x - pi/4 = r + n pi, |r| <= pi/2, for 0 <= x < 2^30: Cody-Waite reduction by
+// pi split into three doubles (m = n + 1/4 is exact), then the degree-7 polynomials of the
+// half-turn table.  Returns n; sin(x - pi/4) = (-1)^n s and cos(x - pi/4) = (-1)^n c.  Against
+// the quarter-turn reduction this trades the quadrant selects for three more polynomial terms.
+// h supplies -1/4 and the leading coefficients (hankel_tops).
+__host__ __device__ __forceinline__ int sincos_half_turn(double x, double& s, double& c,
+                                                         const HankelTops& h)
 {
-    const auto& T = hankel_tables().trig;
-    const double n = rint(fma(x, T[0], -0.5));
-    const double m = n + 0.5;
+    const auto& T = hankel_tables().trig2;
+    constexpr int S0 = 4, C0 = 4 + BEM2D_TRIG2_NS;
+    const double n = rint(fma(x, T[0], h.mquarter));
+    const double m = n + 0.25;
     double r = fma(-m, T[1], x);
     r = fma(-m, T[2], r);
     r = fma(-m, T[3], r);
-    sincos_kernel(r, ((int)n) & 3, s, c);
+    const double r2 = r * r;
+    double ps = fma_vvs(r2, h.trig[0], T[S0 + 1]);
+#pragma unroll
+    for (int i = S0 + 2; i < C0; ++i) ps = fma_vvs(ps, r2, T[i]);
+    s = fma(ps * r2, r, r);
+    double pc = fma_vvs(r2, h.trig[1], T[C0 + 1]);
+#pragma unroll
+    for (int i = C0 + 2; i < C0 + BEM2D_TRIG2_NC; ++i) pc = fma_vvs(pc, r2, T[i]);
+    c = fma(pc, r2, 1.0);
+    return (int)n;
+}
+
+// a (-1)^n: the parity of n moved into the sign bit
+__host__ __device__ __forceinline__ double with_parity_sign(double a, int n)
+{
+    const unsigned long long bits = __builtin_bit_cast(unsigned long long, a);
+    return __builtin_bit_cast(double, bits ^ ((unsigned long long)(n & 1) << 63));
 }
 
 // P_nu, Q_nu for x >= BEM2D_PIECE_X[last] as polynomials in w = 1/x^2 (the last piece's map to z
 // folded into the coefficients).
+// h = hankel_tops<BEM2D_HANKEL_NPIECE>().
 __host__ __device__ __forceinline__ void hankel_pq_far(double invx, double& P0, double& Q0,
-                                                       double& P1, double& Q1)
+                                                       double& P1, double& Q1, const HankelTops& h)
 {
     const auto& T = hankel_tables();
     constexpr int N = BEM2D_HANKEL_FAR_DEG + 1;
     const double w = invx * invx;
-    P0 = horner<N>(T.far[0], w);
-    Q0 = horner<N>(T.far[1], w) * invx;
-    P1 = horner<N>(T.far[2], w);
-    Q1 = horner<N>(T.far[3], w) * invx;
+    P0 = horner_top<N>(T.far[0], h.pq[0], w);
+    Q0 = horner_top<N>(T.far[1], h.pq[1], w) * invx;
+    P1 = horner_top<N>(T.far[2], h.pq[2], w);
+    Q1 = horner_top<N>(T.far[3], h.pq[3], w) * invx;
 }
 
 // Modulus/phase combination for x >= 2:
 //   H0 = sqrt(2/(pi x)) (P0 + i Q0) (C + i S),  H1 = sqrt(2/(pi x)) (P1 + i Q1) (S - i C),
-// with C + i S = exp(i (x - pi/4)).
+// with C + i S = exp(i (x - pi/4)) = (-1)^n (c + i s) and the sign carried by the amplitude.
 __host__ __device__ __forceinline__ void hankel01_combine(double x, double invx, double P0,
                                                           double Q0, double P1, double Q1,
                                                           double& h0r, double& h0i, double& h1r,
                                                           double& h1i)
 {
     double S, C;
-    sincos_minus_quarter(x, S, C);
+    const int n = sincos_half_turn(x, S, C, hankel_tops<BEM2D_HANKEL_NPIECE>());
 #ifdef __HIP_DEVICE_COMPILE__
-    const double A = rsqrt_refined(0.5 * kPi * x); // sqrt(2/(pi x))
+    const double A = with_parity_sign(rsqrt_refined(0.5 * kPi * x), n); // +-sqrt(2/(pi x))
 #else
-    const double A = sqrt(2.0 * kInvPi * invx);
+    const double A = with_parity_sign(sqrt(2.0 * kInvPi * invx), n);
 #endif
     h0r = A * (P0 * C - Q0 * S);
     h0i = A * (P0 * S + Q0 * C);
@@ -243,17 +319,17 @@ __host__ __device__ __forceinline__ void hankel01_inv(double x, double invx, dou
     }
     double P0, Q0, P1, Q1;
     if (x >= kHankelFarX)
-        hankel_pq_far(invx, P0, Q0, P1, Q1);
+        hankel_pq_far(invx, P0, Q0, P1, Q1, hankel_tops<BEM2D_HANKEL_NPIECE>());
     else if (x >= kHankelMidX)
-        hankel_pq<4>(invx, P0, Q0, P1, Q1);
+        hankel_pq<4>(invx, P0, Q0, P1, Q1, hankel_tops<4>());
     else if (x >= 5.0)
-        hankel_pq<3>(invx, P0, Q0, P1, Q1);
+        hankel_pq<3>(invx, P0, Q0, P1, Q1, hankel_tops<3>());
     else if (x >= 3.5)
-        hankel_pq<2>(invx, P0, Q0, P1, Q1);
+        hankel_pq<2>(invx, P0, Q0, P1, Q1, hankel_tops<2>());
     else if (x >= 2.6)
-        hankel_pq<1>(invx, P0, Q0, P1, Q1);
+        hankel_pq<1>(invx, P0, Q0, P1, Q1, hankel_tops<1>());
     else
-        hankel_pq<0>(invx, P0, Q0, P1, Q1);
+        hankel_pq<0>(invx, P0, Q0, P1, Q1, hankel_tops<0>());
     hankel01_combine(x, invx, P0, Q0, P1, Q1, h0r, h0i, h1r, h1i);
 }
 
