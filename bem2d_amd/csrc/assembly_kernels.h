@@ -159,12 +159,14 @@ struct PairConst {
     double k, kinv, k4, eta4;
     double k4a, eta4a, a4;
     HankelTops h;
+    const double* tab; // PAIR_GENERAL: the piece tables in LDS (stage_piece_tables)
 };
 
 template <int MODE>
-__device__ __forceinline__ PairConst pair_const(double k, double eta)
+__device__ __forceinline__ PairConst pair_const(double k, double eta, const double* tab)
 {
     PairConst pc;
+    pc.tab = tab;
     pc.k = k;
     pc.kinv = 1.0 / k;
     pc.k4 = 0.25 * k;
@@ -253,7 +255,7 @@ __device__ __forceinline__ void kernel_pair(const PairConst& pc, double xx, doub
         return;
     }
     double h0r, h0i, h1r, h1i;
-    hankel01_inv(x, invx, h0r, h0i, h1r, h1i);
+    hankel01_gather(x, invx, pc.tab, h0r, h0i, h1r, h1i);
     const double k4 = pc.k4, eta4 = pc.eta4;
     if (KIND == BEM2D_KERNEL_SL) {
         kr = mr = -0.25 * h0i;
@@ -311,10 +313,11 @@ __host__ __device__ __forceinline__ int64_t upper_tile_count(int64_t n)
 // entry (i, j), in the order of the reference's Integrate; mirror (mr, mi) for entry (j, i).
 template <int KIND, int NQ, int MODE>
 __device__ __forceinline__ void pair_sums(int nq, int64_t n, int64_t i, const double* __restrict__ nodes,
-                                          const double* sc, double k, double eta, double& ar,
-                                          double& ai, double& mr, double& mi)
+                                          const double* sc, const double* tab, double k,
+                                          double eta, double& ar, double& ai, double& mr,
+                                          double& mi)
 {
-    const PairConst pc = pair_const<MODE>(k, eta);
+    const PairConst pc = pair_const<MODE>(k, eta, tab);
     const int64_t plane = n * nq;
     ar = ai = mr = mi = 0.0;
     for (int a = 0; a < nq; ++a) {
@@ -364,6 +367,8 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void assemble_regular_kernel(c
     double2* __restrict__ A = pb.A;
     const int64_t lda = pb.lda;
     extern __shared__ double sh[]; // [kColsPerBlock][nq][5]
+    __shared__ double piece_tab[kPieceTableDoubles];
+    bool staged = false;
     const int nq = NQ ? NQ : nq_rt;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -416,12 +421,25 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void assemble_regular_kernel(c
         if (!any) continue;
         const double* sc = sh + c * nq * 5;
         double ar, ai, mr, mi;
-        if (all_far)
-            pair_sums<KIND, NQ, PAIR_FAR>(nq, n, i, nodes, sc, k, eta, ar, ai, mr, mi);
-        else if (all_mid)
-            pair_sums<KIND, NQ, PAIR_MID>(nq, n, i, nodes, sc, k, eta, ar, ai, mr, mi);
-        else
-            pair_sums<KIND, NQ, PAIR_GENERAL>(nq, n, i, nodes, sc, k, eta, ar, ai, mr, mi);
+        if (all_far) {
+            pair_sums<KIND, NQ, PAIR_FAR>(nq, n, i, nodes, sc, nullptr, k, eta, ar, ai, mr, mi);
+        } else if (all_mid) {
+            pair_sums<KIND, NQ, PAIR_MID>(nq, n, i, nodes, sc, nullptr, k, eta, ar, ai, mr, mi);
+        } else {
+            // near pairs: the piece tables go to LDS once per workgroup (a single wavefront, so
+            // the branch is uniform over the workgroup and the barrier legal)
+            static_assert(kWavesPerBlock == 1, "the piece tables are staged by a single wavefront");
+            if (!staged) {
+                // lanes past the last row have left: the active lanes share the copy by rank
+                const unsigned long long act = __ballot(1);
+                const int rank = __popcll(act & ((1ull << lane) - 1ull));
+                stage_piece_tables(piece_tab, rank, __popcll(act));
+                __syncthreads();
+                staged = true;
+            }
+            pair_sums<KIND, NQ, PAIR_GENERAL>(nq, n, i, nodes, sc, piece_tab, k, eta, ar, ai, mr,
+                                              mi);
+        }
         if (direct) A[i + lda * j] = make_double2(kscale * ar, kscale * ai);
         if (mirror) A[j + lda * i] = make_double2(kscale * mr, kscale * mi);
     }

@@ -79,6 +79,8 @@ extern "C" void bem2d_ctx_destroy(bem2d_ctx* c)
         for (hipEvent_t e : p.pending) (void)hipEventDestroy(e);
     for (auto& s : c->scratch)
         if (s.ptr) (void)hipFree(s.ptr);
+    if (c->stage) (void)hipHostFree(c->stage);
+    if (c->ev_stage) (void)hipEventDestroy(c->ev_stage);
     bem2d_internal_mixed_release(c);
     rocblas_destroy_handle(c->blas);
     (void)hipStreamSynchronize(c->aux);
@@ -674,15 +676,32 @@ extern "C" int bem2d_rhs(bem2d_ctx* c, const bem2d_mesh* m, int kind, double k, 
         return FAIL(BEM2D_ERR_UNSUPPORTED, "unknown field kind %d", kind);
     if (nrhs == 0) return BEM2D_OK;
     HIPCHK(hipSetDevice(c->device));
-    double* d_dirs = (double*)bem2d_internal_scratch(c, SCRATCH_SMALL, sizeof(double) * 2 * nrhs);
+    const size_t bytes = sizeof(double) * 2 * nrhs;
+    double* d_dirs = (double*)bem2d_internal_scratch(c, SCRATCH_SMALL, bytes);
     if (!d_dirs) return FAIL(BEM2D_ERR_NOMEM, "scratch allocation failed");
-    HIPCHK(hipMemcpyAsync(d_dirs, dirs, sizeof(double) * 2 * nrhs, hipMemcpyHostToDevice, c->stream));
+    // The directions go up through pinned staging memory, so the copy is asynchronous and the call
+    // returns before the kernel runs (the caller's array is free again at once).  The staging
+    // buffer is refilled only after its previous copy has run.
+    if (!c->ev_stage) HIPCHK(hipEventCreateWithFlags(&c->ev_stage, hipEventDisableTiming));
+    HIPCHK(hipEventSynchronize(c->ev_stage));
+    if (c->stage_bytes < bytes) {
+        if (c->stage) HIPCHK(hipHostFree(c->stage));
+        c->stage = nullptr;
+        c->stage_bytes = 0;
+        const size_t want = bytes < 4096 ? 4096 : bytes;
+        if (hipHostMalloc((void**)&c->stage, want) != hipSuccess) {
+            c->stage = nullptr;
+            return FAIL(BEM2D_ERR_NOMEM, "pinned staging allocation failed");
+        }
+        c->stage_bytes = want;
+    }
+    std::memcpy(c->stage, dirs, bytes);
+    HIPCHK(hipMemcpyAsync(d_dirs, c->stage, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipEventRecord(c->ev_stage, c->stream));
     dim3 grid((unsigned)((m->n + 255) / 256), (unsigned)nrhs);
     rhs_kernel<<<grid, 256, 0, c->stream>>>(m->n, m->nq, m->d_nodes, kind, k, eta, d_dirs, scale,
                                             (double2*)B, ldb);
     HIPCHK(hipGetLastError());
-    // dirs was staged from pageable host memory: make the call safe to return before the kernel runs
-    HIPCHK(hipStreamSynchronize(c->stream));
     return BEM2D_OK;
 }
 

@@ -333,6 +333,54 @@ __host__ __device__ __forceinline__ void hankel01_inv(double x, double invx, dou
     hankel01_combine(x, invx, P0, Q0, P1, Q1, h0r, h0i, h1r, h1i);
 }
 
+// The piecewise tables staged in LDS for hankel01_gather: per piece, P0, xQ0, P1, xQ1 with
+// BEM2D_HANKEL_DEGMAX + 1 coefficients each (the last piece zero-padded), then its z map.
+constexpr int kPieceCoefs = BEM2D_HANKEL_DEGMAX + 1;
+constexpr int kPieceStride = 4 * kPieceCoefs + 3;
+constexpr int kPieceTableDoubles = BEM2D_HANKEL_NPIECE * kPieceStride;
+
+// Copies the piece tables into lds (kPieceTableDoubles doubles) with threads lane = 0 .. nthr - 1.
+__device__ __forceinline__ void stage_piece_tables(double* lds, int lane, int nthr)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    const auto& T = hankel_tables();
+    for (int e = lane; e < kPieceTableDoubles; e += nthr) {
+        const int p = e / kPieceStride, o = e - p * kPieceStride;
+        lds[e] = o < 4 * kPieceCoefs ? T.fit[p][o / kPieceCoefs][o % kPieceCoefs]
+                                     : T.zmap[p][o - 4 * kPieceCoefs];
+    }
+#endif
+}
+
+// hankel01_inv for lanes whose x spread over several pieces: every lane with x >= 2 evaluates
+// one set of polynomials with its own piece's coefficients, gathered from the LDS copy (the last
+// piece serves x >= 8, the far table's range included).  Branching per piece instead costs a
+// wavefront every piece that any of its lanes falls in.
+__device__ __forceinline__ void hankel01_gather(double x, double invx, const double* tab,
+                                                double& h0r, double& h0i, double& h1r, double& h1i)
+{
+    if (x < 2.0) {
+        hankel01_series(x, h0r, h0i, h1r, h1i);
+        return;
+    }
+    int p = 0;
+#pragma unroll
+    for (int q = 1; q < BEM2D_HANKEL_NPIECE; ++q) p += x >= BEM2D_PIECE_X[q] ? 1 : 0;
+    const double* c = tab + p * kPieceStride;
+    const double y = c[4 * kPieceCoefs] * invx * invx;
+    const double z = (y - c[4 * kPieceCoefs + 1]) * c[4 * kPieceCoefs + 2];
+    double v[4];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+        const double* cf = c + f * kPieceCoefs;
+        double r = cf[kPieceCoefs - 1];
+#pragma unroll
+        for (int i = kPieceCoefs - 2; i >= 0; --i) r = fma(r, z, cf[i]);
+        v[f] = r;
+    }
+    hankel01_combine(x, invx, v[0], v[1] * invx, v[2], v[3] * invx, h0r, h0i, h1r, h1i);
+}
+
 __host__ __device__ __forceinline__ void hankel01(double x, double& h0r, double& h0i, double& h1r,
                                                   double& h1i)
 {

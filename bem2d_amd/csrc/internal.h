@@ -40,6 +40,9 @@ struct bem2d_ctx {
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     rocblas_handle blas = nullptr;
     ScratchBuf scratch[SCRATCH_COUNT];
+    double* stage = nullptr;             // pinned staging of small host uploads (rhs directions)
+    size_t stage_bytes = 0;
+    hipEvent_t ev_stage = nullptr;       // recorded after the last copy out of stage
     MixedState* mixed = nullptr;
     bool profiling = false;
     ProfSlot prof[BEM2D_PROF_COUNT];

@@ -1601,7 +1601,9 @@ extern "C" int bem2d_lu32_factor(bem2d_ctx* c, bem2d_lu32* f, const void* A64, i
 // split into units of <= kMaxSolveRhs columns, and each step is one batched launch per kernel over
 // the units still running.  Progress is asynchronous: step() enqueues a step and an asynchronous
 // copy of every unit's norms into pinned host memory; ready() polls the step's event; collect()
-// reads the norms and retires the units that met the tolerance.
+// reads the norms and retires the units that met the tolerance.  The first step() enqueues the
+// initial residual and the first correction together: x = 0 meets no tolerance unless b = 0, so
+// waiting for the initial norms would only add a round trip through the host.
 struct RefJob {
     bem2d_ctx* ctx = nullptr;
     DevArena* arena = nullptr;
@@ -1621,7 +1623,7 @@ struct RefJob {
     hipEvent_t ev = nullptr;
     bool pending = false;
     bool started = false;
-    bool last_init = false;               // the pending read-back is the initial residual
+    bool bnorm_read = false;              // |b| per column taken from the first read-back
     int nspec = 0;
 };
 
@@ -1750,7 +1752,7 @@ static RefBatch ref_batch(const RefJob* j, const std::vector<int>& ids, size_t c
     return bt;
 }
 
-// Enqueues one refinement step (or the initial residual when not yet started) for every active
+// Enqueues one refinement step (the first one preceded by the initial residual) for every active
 // unit, then the asynchronous copy of the norms and the job's event.
 int bem2d_internal_ref_step(RefJob* j, hipStream_t s)
 {
@@ -1767,13 +1769,10 @@ int bem2d_internal_ref_step(RefJob* j, hipStream_t s)
         for (size_t c0 = 0; c0 < ids.size(); c0 += kMaxLuBatch) {
             RefBatch bt = ref_batch(j, ids, c0, 0);
             ref_init_kernel<<<dim3((unsigned)bt.start[bt.count], kMaxSolveRhs), 256, 0, s>>>(bt);
-            RefBatch one = ref_batch(j, ids, c0, 1);
-            ref_flags_kernel<<<one.count, 64, 0, s>>>(one);
         }
         j->started = true;
-        j->last_init = true;
-    } else {
-        j->last_init = false;
+    }
+    {
         j->epoch++;
         for (size_t c0 = 0; c0 < ids.size(); c0 += kMaxLuBatch) {
             RefBatch one = ref_batch(j, ids, c0, 1);
@@ -1811,8 +1810,8 @@ static double u64_as_double(unsigned long long v)
     return d;
 }
 
-// Reads the norms of the last step (waits for it) and retires the units that met tol (already
-// at x = 0 when b = 0), that failed (a bad pivot or a sweep time-out) or that used maxit steps.
+// Reads the norms of the last step (waits for it) and retires the units that met tol, that failed
+// (a bad pivot or a sweep time-out) or that used maxit steps.
 // Returns BEM2D_ERR_DEVICE on a sweep time-out.
 int bem2d_internal_ref_collect(RefJob* j, bool* all_done)
 {
@@ -1833,17 +1832,15 @@ int bem2d_internal_ref_collect(RefJob* j, bool* all_done)
             if (nm[kNormTimeout]) rc = BEM2D_ERR_DEVICE;
             continue;
         }
-        if (j->last_init) {
+        if (!j->bnorm_read) // the initial residual's slots, untouched by the correction steps
             for (int h = 0; h < U.nrhs; ++h)
                 j->bnorm[kMaxSolveRhs * u + h] = u64_as_double(nm[2 * kMaxSolveRhs + h]);
-        } else {
-            j->iters[u]++;
-        }
+        j->iters[u]++;
         const double anorm = u64_as_double(nm[kNormA]);
         double w = 0.0;
         for (int h = 0; h < U.nrhs; ++h) {
             const double rn = u64_as_double(nm[h]);
-            const double xn = j->last_init ? 0.0 : u64_as_double(nm[kMaxSolveRhs + h]);
+            const double xn = u64_as_double(nm[kMaxSolveRhs + h]);
             const double den = anorm * xn + j->bnorm[kMaxSolveRhs * u + h];
             const double wh = den > 0 ? rn / den : (rn > 0 ? INFINITY : 0.0);
             w = std::max(w, std::isfinite(wh) ? wh : INFINITY);
@@ -1852,6 +1849,7 @@ int bem2d_internal_ref_collect(RefJob* j, bool* all_done)
         if (w <= j->tol[u] || j->iters[u] >= j->maxit[u]) j->active[u] = 0;
         if (j->active[u]) done = false;
     }
+    j->bnorm_read = true;
     *all_done = done;
     return rc;
 }
@@ -1898,7 +1896,7 @@ int bem2d_internal_refine_sync(hipStream_t s, RefJob* j)
     bool done = false;
     int rc = BEM2D_OK;
     while (!rc && !done) {
-        rc = bem2d_internal_ref_step(j, s); // the initial residual first, then correction steps
+        rc = bem2d_internal_ref_step(j, s); // correction steps (the first after the initial residual)
         if (!rc) rc = bem2d_internal_ref_collect(j, &done);
     }
     return rc;
