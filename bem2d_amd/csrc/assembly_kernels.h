@@ -279,34 +279,25 @@ __device__ __forceinline__ void kernel_pair(const PairConst& pc, double xx, doub
     }
 }
 
-// Tile shape of the regular assembly: one wavefront of 64 rows per trial column gives the
-// dispatcher the finest work units (columns differ in cost by up to 3x: near pairs take the
-// piecewise path); measured on MI355X, kite N = 4000: 4 columns per wave 12.8 ms, 2: 10.6, 1: 9.5.
-#ifndef BEM2D_ASM_WAVES
-#define BEM2D_ASM_WAVES 1
-#endif
-#ifndef BEM2D_ASM_COLS
-#define BEM2D_ASM_COLS 1
-#endif
-constexpr int kRowsPerBlock = 64;   // one wavefront of rows (test elements)
-constexpr int kWavesPerBlock = BEM2D_ASM_WAVES;
-constexpr int kColsPerWave = BEM2D_ASM_COLS; // trial elements handled in turn by each wavefront
-constexpr int kColsPerBlock = kWavesPerBlock * kColsPerWave;
-constexpr int kColBlocksPerRowBlock = kRowsPerBlock / kColsPerBlock;
-static_assert(kRowsPerBlock % kColsPerBlock == 0 && kColBlocksPerRowBlock % 2 == 0,
-              "a row block must hold an even number of column blocks");
+// Tile shape of the regular assembly: one wavefront (one workgroup) takes 8 test x 8 trial
+// elements, lane l the pair (8 p + l % 8, 8 q + l / 8).  The tier of a wavefront (far / mid /
+// general, below) is decided by its nearest pair, and a square tile spans |i - j| over 16 elements
+// rather than 64: near the diagonal fewer wavefronts mix near and far pairs (kite N = 4000, k = 100:
+// about 2% of the wavefronts take the general loop against 5% with 64 x 1 tiles).  Both stores
+// then cover whole 128-byte segments (8 consecutive rows of a column).
+constexpr int kTileSide = 8;
+__host__ __device__ __forceinline__ int node_stride(int nq) { return nq * NODE_PLANES + 1; }
 
-// Tiles of the upper triangle: row block p (64 rows) pairs with column blocks q >= m p, where
-// m = kColBlocksPerRowBlock.  Row block p holds Q - m p tiles, so tile t lies in the largest p with
-// tiles_before(p) = p Q - (m/2) p (p - 1) <= t.
-__host__ __device__ __forceinline__ int64_t tiles_before(int64_t p, int64_t Q)
+// Tiles of the upper triangle of the P x P grid of 8 x 8 blocks: row block p holds the P - p tiles
+// q = p .. P - 1, so tile t lies in the largest p with tiles_before(p) = p P - p (p - 1) / 2 <= t.
+__host__ __device__ __forceinline__ int64_t tiles_before(int64_t p, int64_t P)
 {
-    return p * Q - (int64_t)(kColBlocksPerRowBlock / 2) * p * (p - 1);
+    return p * P - p * (p - 1) / 2;
 }
 __host__ __device__ __forceinline__ int64_t upper_tile_count(int64_t n)
 {
-    const int64_t P = (n + kRowsPerBlock - 1) / kRowsPerBlock;
-    return tiles_before(P, (n + kColsPerBlock - 1) / kColsPerBlock);
+    const int64_t P = (n + kTileSide - 1) / kTileSide;
+    return tiles_before(P, P);
 }
 
 // Node-pair sums of one element pair (i, j): direct (ar, ai) = sum_a w_a sum_b w_b K(x_a, y_b) for
@@ -344,15 +335,15 @@ __device__ __forceinline__ void pair_sums(int nq, int64_t n, int64_t i, const do
 }
 
 // Regular pairs (Integrate with Gauss2D, lib/bem2d_quadrature.h:37-65, :283-286), both orders of
-// each element pair from one evaluation of the Hankel functions: a block takes tile t of the upper
-// triangle (64 rows i x kColsPerBlock columns j, j > i) and writes A(i, j) (lane = row, coalesced)
-// and A(j, i).
-// The trial element's node data sit in LDS and are read as wave-uniform broadcasts.  A wavefront
-// whose 64 pairs of one column are all at k|x - y| >= kHankelFarX by the bounding circles takes the
-// branch-free asymptotic loop; all at >= kHankelMidX, the branch-free loop of the last piece.  Coincident / adjacent pairs are skipped here and written by
-// assemble_special_kernel.  NQ = nq at compile time (0: runtime nq).
+// each element pair from one evaluation of the Hankel functions: workgroup t takes tile t of the
+// upper triangle and writes A(i, j) and A(j, i) for its pairs with j > i.
+// The trial elements' node data sit in LDS (one odd-strided row per element: the eight elements of
+// a wavefront fall in different banks).  A wavefront whose pairs are all at k|x - y| >= kHankelFarX
+// by the bounding circles takes the branch-free asymptotic loop; all at >= kHankelMidX, the
+// branch-free loop of the last piece; otherwise the general loop.  Coincident / adjacent pairs are
+// skipped here and written by assemble_special_kernel.  NQ = nq at compile time (0: runtime nq).
 template <int KIND, int NQ>
-__global__ __launch_bounds__(64 * kWavesPerBlock) void assemble_regular_kernel(const AsmBatch batch)
+__global__ __launch_bounds__(64) void assemble_regular_kernel(const AsmBatch batch)
 {
     const int di = batch_find(batch.start, batch.count, blockIdx.x);
     const AsmProblem& pb = batch.p[di];
@@ -366,83 +357,69 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void assemble_regular_kernel(c
     const double k = pb.k, eta = pb.eta, kscale = pb.kscale;
     double2* __restrict__ A = pb.A;
     const int64_t lda = pb.lda;
-    extern __shared__ double sh[]; // [kColsPerBlock][nq][5]
+    extern __shared__ double sh[]; // [kTileSide][node_stride(nq)]
     __shared__ double piece_tab[kPieceTableDoubles];
-    bool staged = false;
     const int nq = NQ ? NQ : nq_rt;
-    const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int64_t Q = (n + kColsPerBlock - 1) / kColsPerBlock;
-    const int64_t P = (n + kRowsPerBlock - 1) / kRowsPerBlock;
+    const int ns = node_stride(nq);
+    const int lane = threadIdx.x;
+    const int64_t P = (n + kTileSide - 1) / kTileSide;
     const int64_t t = (int64_t)blockIdx.x - batch.start[di];
-    constexpr double m = kColBlocksPerRowBlock;
-    const double qq = (double)Q + 0.5 * m; // smaller root of (m/2) p^2 - (Q + m/2) p + t = 0
-    int64_t p = (int64_t)((qq - sqrt(fmax(qq * qq - 2.0 * m * (double)t, 0.0))) / m);
+    const double qq = (double)P + 0.5; // smaller root of p^2 / 2 - (P + 1/2) p + t = 0
+    int64_t p = (int64_t)(qq - sqrt(fmax(qq * qq - 2.0 * (double)t, 0.0)));
     p = p < 0 ? 0 : (p >= P ? P - 1 : p);
-    while (p > 0 && tiles_before(p, Q) > t) --p;
-    while (p + 1 < P && tiles_before(p + 1, Q) <= t) ++p;
-    const int64_t q = kColBlocksPerRowBlock * p + (t - tiles_before(p, Q));
-    const int64_t i = p * kRowsPerBlock + lane;
-    const int64_t j0 = q * kColsPerBlock;
+    while (p > 0 && tiles_before(p, P) > t) --p;
+    while (p + 1 < P && tiles_before(p + 1, P) <= t) ++p;
+    const int64_t q = p + (t - tiles_before(p, P));
+    const int64_t j0 = q * kTileSide;
     const int64_t plane = n * nq;
 
-    for (int u = threadIdx.x; u < kColsPerBlock * nq; u += blockDim.x) {
+    for (int u = lane; u < kTileSide * nq; u += 64) {
         const int c = u / nq, b = u - c * nq;
         const int64_t j = j0 + c;
         if (j < n) {
             const int64_t o = (int64_t)b * n + j;
 #pragma unroll
-            for (int f = 0; f < NODE_PLANES; ++f) sh[(c * nq + b) * 5 + f] = nodes[f * plane + o];
+            for (int f = 0; f < NODE_PLANES; ++f) sh[c * ns + b * NODE_PLANES + f] = nodes[f * plane + o];
         }
     }
     __syncthreads();
-    if (i >= n) return;
+    const int c = lane / kTileSide;
+    const int64_t i = p * kTileSide + lane % kTileSide, j = j0 + c;
+    if (i >= n || j >= n) return;
 
     const int64_t idx_i = index[i], nx_i = next[i], pv_i = prev[i];
-    const double cx_i = bounds[BOUND_CX * n + i], cy_i = bounds[BOUND_CY * n + i];
-    const double r_i = bounds[BOUND_R * n + i];
-    for (int cc = 0; cc < kColsPerWave; ++cc) {
-        const int c = wave * kColsPerWave + cc;
-        const int64_t j = j0 + c;
-        if (j >= n) break;
-        const int64_t idx_j = index[j], nx_j = next[j], pv_j = prev[j];
-        const bool direct = j > i && !(idx_j == idx_i || nx_i == idx_j || pv_i == idx_j);
-        const bool mirror = j > i && !(idx_i == idx_j || nx_j == idx_i || pv_j == idx_i);
-        const bool any = direct || mirror;
-        bool far = true, mid = true;
-        if (any) {
-            const double dx = cx_i - bounds[BOUND_CX * n + j];
-            const double dy = cy_i - bounds[BOUND_CY * n + j];
-            const double kgap = k * (sqrt(dx * dx + dy * dy) - r_i - bounds[BOUND_R * n + j]);
-            far = kgap >= kHankelFarX * (1.0 + 1e-12);
-            mid = kgap >= kHankelMidX * (1.0 + 1e-12);
-        }
-        const bool all_far = __all(far), all_mid = __all(mid);
-        if (!any) continue;
-        const double* sc = sh + c * nq * 5;
-        double ar, ai, mr, mi;
-        if (all_far) {
-            pair_sums<KIND, NQ, PAIR_FAR>(nq, n, i, nodes, sc, nullptr, k, eta, ar, ai, mr, mi);
-        } else if (all_mid) {
-            pair_sums<KIND, NQ, PAIR_MID>(nq, n, i, nodes, sc, nullptr, k, eta, ar, ai, mr, mi);
-        } else {
-            // near pairs: the piece tables go to LDS once per workgroup (a single wavefront, so
-            // the branch is uniform over the workgroup and the barrier legal)
-            static_assert(kWavesPerBlock == 1, "the piece tables are staged by a single wavefront");
-            if (!staged) {
-                // lanes past the last row have left: the active lanes share the copy by rank
-                const unsigned long long act = __ballot(1);
-                const int rank = __popcll(act & ((1ull << lane) - 1ull));
-                stage_piece_tables(piece_tab, rank, __popcll(act));
-                __syncthreads();
-                staged = true;
-            }
-            pair_sums<KIND, NQ, PAIR_GENERAL>(nq, n, i, nodes, sc, piece_tab, k, eta, ar, ai, mr,
-                                              mi);
-        }
-        if (direct) A[i + lda * j] = make_double2(kscale * ar, kscale * ai);
-        if (mirror) A[j + lda * i] = make_double2(kscale * mr, kscale * mi);
+    const int64_t idx_j = index[j], nx_j = next[j], pv_j = prev[j];
+    const bool direct = j > i && !(idx_j == idx_i || nx_i == idx_j || pv_i == idx_j);
+    const bool mirror = j > i && !(idx_i == idx_j || nx_j == idx_i || pv_j == idx_i);
+    const bool any = direct || mirror;
+    bool far = true, mid = true;
+    if (any) {
+        const double dx = bounds[BOUND_CX * n + i] - bounds[BOUND_CX * n + j];
+        const double dy = bounds[BOUND_CY * n + i] - bounds[BOUND_CY * n + j];
+        const double kgap =
+            k * (sqrt(dx * dx + dy * dy) - bounds[BOUND_R * n + i] - bounds[BOUND_R * n + j]);
+        far = kgap >= kHankelFarX * (1.0 + 1e-12);
+        mid = kgap >= kHankelMidX * (1.0 + 1e-12);
     }
+    const bool all_far = __all(far), all_mid = __all(mid);
+    if (!any) return;
+    const double* sc = sh + c * ns;
+    double ar, ai, mr, mi;
+    if (all_far) {
+        pair_sums<KIND, NQ, PAIR_FAR>(nq, n, i, nodes, sc, nullptr, k, eta, ar, ai, mr, mi);
+    } else if (all_mid) {
+        pair_sums<KIND, NQ, PAIR_MID>(nq, n, i, nodes, sc, nullptr, k, eta, ar, ai, mr, mi);
+    } else {
+        // near pairs: the piece tables go to LDS (the workgroup is this one wavefront, so the
+        // barrier is legal here); lanes that have left share no part of the copy
+        const unsigned long long act = __ballot(1);
+        const int rank = __popcll(act & ((1ull << lane) - 1ull));
+        stage_piece_tables(piece_tab, rank, __popcll(act));
+        __syncthreads();
+        pair_sums<KIND, NQ, PAIR_GENERAL>(nq, n, i, nodes, sc, piece_tab, k, eta, ar, ai, mr, mi);
+    }
+    if (direct) A[i + lda * j] = make_double2(kscale * ar, kscale * ai);
+    if (mirror) A[j + lda * i] = make_double2(kscale * mr, kscale * mi);
 }
 
 // Coincident / adjacent pairs: one block of 256 threads per entry (i, j, rule).

@@ -582,7 +582,7 @@ static int launch_assembly(bem2d_ctx* c, hipStream_t s, const AsmProblem* probs,
         }
         if (reg.start[cnt] > (int64_t)INT32_MAX || sp.start[cnt] > (int64_t)INT32_MAX)
             return FAIL(BEM2D_ERR_UNSUPPORTED, "assembly grid too large");
-        const size_t shm = sizeof(double) * kColsPerBlock * nq * NODE_PLANES;
+        const size_t shm = sizeof(double) * kTileSide * node_stride(nq);
         // The singular-pair kernel writes entries the regular kernel skips: it runs on the second
         // stream beside the regular kernel (forked from and joined back into s).
         const bool fork = sp.start[cnt] > 0;
@@ -598,10 +598,10 @@ static int launch_assembly(bem2d_ctx* c, hipStream_t s, const AsmProblem* probs,
             bem2d_internal_prof_begin(c, BEM2D_PROF_ASSEMBLE_REGULAR, s);
             if (nq == 16)
                 assemble_regular_kernel<KIND, 16>
-                    <<<(unsigned)reg.start[cnt], 64 * kWavesPerBlock, shm, s>>>(reg);
+                    <<<(unsigned)reg.start[cnt], 64, shm, s>>>(reg);
             else
                 assemble_regular_kernel<KIND, 0>
-                    <<<(unsigned)reg.start[cnt], 64 * kWavesPerBlock, shm, s>>>(reg);
+                    <<<(unsigned)reg.start[cnt], 64, shm, s>>>(reg);
             bem2d_internal_prof_end(c, BEM2D_PROF_ASSEMBLE_REGULAR, s);
         }
         if (fork && (hipEventRecord(c->ev_join, c->aux) != hipSuccess ||

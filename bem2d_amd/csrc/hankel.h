@@ -240,8 +240,8 @@ __host__ __device__ __forceinline__ void hankel_pq(double invx, double& P0, doub
 }
 
 // sin r and cos r of x - pi/4 = r + n pi, |r| <= pi/2, for 0 <= x < 2^30: Cody-Waite reduction by
-// pi split into three doubles (m = n + 1/4 is exact), then the degree-7 polynomials of the
-// half-turn table.  Returns n; sin(x - pi/4) = (-1)^n s and cos(x - pi/4) = (-1)^n c.  Against
+// pi = P1 + P2 + P3 (m = n + 1/4 is exact; with |m| < 2^29 the P3 term, |m P3| < 2e-24, lies far
+// below the last bit of r and is left out), then the degree-7 polynomials of the half-turn table.  Returns n; sin(x - pi/4) = (-1)^n s and cos(x - pi/4) = (-1)^n c.  Against
 // the quarter-turn reduction this trades the quadrant selects for three more polynomial terms.
 // h supplies -1/4 and the leading coefficients (hankel_tops).
 __host__ __device__ __forceinline__ int sincos_half_turn(double x, double& s, double& c,
@@ -251,9 +251,7 @@ __host__ __device__ __forceinline__ int sincos_half_turn(double x, double& s, do
     constexpr int S0 = 4, C0 = 4 + BEM2D_TRIG2_NS;
     const double n = rint(fma(x, T[0], h.mquarter));
     const double m = n + 0.25;
-    double r = fma(-m, T[1], x);
-    r = fma(-m, T[2], r);
-    r = fma(-m, T[3], r);
+    const double r = fma(-m, T[2], fma(-m, T[1], x));
     const double r2 = r * r;
     double ps = fma_vvs(r2, h.trig[0], T[S0 + 1]);
 #pragma unroll
