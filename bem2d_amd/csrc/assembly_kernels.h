@@ -287,6 +287,8 @@ __device__ __forceinline__ void kernel_pair(const PairConst& pc, double xx, doub
 // then cover whole 128-byte segments (8 consecutive rows of a column).
 constexpr int kTileSide = 8;
 __host__ __device__ __forceinline__ int node_stride(int nq) { return nq * NODE_PLANES + 1; }
+// launches of fewer tiles than this split every tile over four wavefronts (assemble_regular_kernel)
+constexpr int64_t kSplitTiles = 16384;
 
 // Tiles of the upper triangle of the P x P grid of 8 x 8 blocks: row block p holds the P - p tiles
 // q = p .. P - 1, so tile t lies in the largest p with tiles_before(p) = p P - p (p - 1) / 2 <= t.
@@ -300,18 +302,19 @@ __host__ __device__ __forceinline__ int64_t upper_tile_count(int64_t n)
     return tiles_before(P, P);
 }
 
-// Node-pair sums of one element pair (i, j): direct (ar, ai) = sum_a w_a sum_b w_b K(x_a, y_b) for
-// entry (i, j), in the order of the reference's Integrate; mirror (mr, mi) for entry (j, i).
+// Node-pair sums of one element pair (i, j) over test nodes a0 <= a < a1: direct (ar, ai) =
+// sum_a w_a sum_b w_b K(x_a, y_b) for entry (i, j), in the order of the reference's Integrate;
+// mirror (mr, mi) for entry (j, i).
 template <int KIND, int NQ, int MODE>
-__device__ __forceinline__ void pair_sums(int nq, int64_t n, int64_t i, const double* __restrict__ nodes,
-                                          const double* sc, const double* tab, double k,
-                                          double eta, double& ar, double& ai, double& mr,
-                                          double& mi)
+__device__ __forceinline__ void pair_sums(int nq, int a0, int a1, int64_t n, int64_t i,
+                                          const double* __restrict__ nodes, const double* sc,
+                                          const double* tab, double k, double eta, double& ar,
+                                          double& ai, double& mr, double& mi)
 {
     const PairConst pc = pair_const<MODE>(k, eta, tab);
     const int64_t plane = n * nq;
     ar = ai = mr = mi = 0.0;
-    for (int a = 0; a < nq; ++a) {
+    for (int a = a0; a < a1; ++a) {
         const int64_t o = (int64_t)a * n + i;
         const double xx = nodes[NODE_X * plane + o], xy = nodes[NODE_Y * plane + o];
         const double n1x = nodes[NODE_NX * plane + o], n1y = nodes[NODE_NY * plane + o];
@@ -337,13 +340,18 @@ __device__ __forceinline__ void pair_sums(int nq, int64_t n, int64_t i, const do
 // Regular pairs (Integrate with Gauss2D, lib/bem2d_quadrature.h:37-65, :283-286), both orders of
 // each element pair from one evaluation of the Hankel functions: workgroup t takes tile t of the
 // upper triangle and writes A(i, j) and A(j, i) for its pairs with j > i.
+// SPLIT wavefronts share a tile, wavefront w summing the test nodes a in [w nq / SPLIT,
+// (w + 1) nq / SPLIT) of every pair; their partial sums meet in LDS and are added in wavefront
+// order.  Small problems launch too few tiles to fill the device, and each lane's 256 node pairs
+// then run back to back on an otherwise idle SIMD: SPLIT = 4 cuts that chain fourfold.
 // The trial elements' node data sit in LDS (one odd-strided row per element: the eight elements of
-// a wavefront fall in different banks).  A wavefront whose pairs are all at k|x - y| >= kHankelFarX
-// by the bounding circles takes the branch-free asymptotic loop; all at >= kHankelMidX, the
-// branch-free loop of the last piece; otherwise the general loop.  Coincident / adjacent pairs are
-// skipped here and written by assemble_special_kernel.  NQ = nq at compile time (0: runtime nq).
-template <int KIND, int NQ>
-__global__ __launch_bounds__(64) void assemble_regular_kernel(const AsmBatch batch)
+// a wavefront fall in different banks).  A tile whose pairs are all at k|x - y| >= kHankelFarX by
+// the bounding circles takes the branch-free asymptotic loop; all at >= kHankelMidX, the
+// branch-free loop of the last piece; otherwise the general loop (every wavefront of the tile sees
+// the same pairs and takes the same loop).  Coincident / adjacent pairs are skipped here and
+// written by assemble_special_kernel.  NQ = nq at compile time (0: runtime nq).
+template <int KIND, int NQ, int SPLIT>
+__global__ __launch_bounds__(64 * SPLIT) void assemble_regular_kernel(const AsmBatch batch)
 {
     const int di = batch_find(batch.start, batch.count, blockIdx.x);
     const AsmProblem& pb = batch.p[di];
@@ -359,9 +367,11 @@ __global__ __launch_bounds__(64) void assemble_regular_kernel(const AsmBatch bat
     const int64_t lda = pb.lda;
     extern __shared__ double sh[]; // [kTileSide][node_stride(nq)]
     __shared__ double piece_tab[kPieceTableDoubles];
+    __shared__ double part[SPLIT > 1 ? SPLIT - 1 : 1][4][64];
     const int nq = NQ ? NQ : nq_rt;
     const int ns = node_stride(nq);
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t P = (n + kTileSide - 1) / kTileSide;
     const int64_t t = (int64_t)blockIdx.x - batch.start[di];
     const double qq = (double)P + 0.5; // smaller root of p^2 / 2 - (P + 1/2) p + t = 0
@@ -373,7 +383,7 @@ __global__ __launch_bounds__(64) void assemble_regular_kernel(const AsmBatch bat
     const int64_t j0 = q * kTileSide;
     const int64_t plane = n * nq;
 
-    for (int u = lane; u < kTileSide * nq; u += 64) {
+    for (int u = threadIdx.x; u < kTileSide * nq; u += 64 * SPLIT) {
         const int c = u / nq, b = u - c * nq;
         const int64_t j = j0 + c;
         if (j < n) {
@@ -385,6 +395,8 @@ __global__ __launch_bounds__(64) void assemble_regular_kernel(const AsmBatch bat
     __syncthreads();
     const int c = lane / kTileSide;
     const int64_t i = p * kTileSide + lane % kTileSide, j = j0 + c;
+    // (every wavefront of the tile leaves, or stays, at the same lanes: the barriers below are
+    // reached by all wavefronts or by none)
     if (i >= n || j >= n) return;
 
     const int64_t idx_i = index[i], nx_i = next[i], pv_i = prev[i];
@@ -404,19 +416,39 @@ __global__ __launch_bounds__(64) void assemble_regular_kernel(const AsmBatch bat
     const bool all_far = __all(far), all_mid = __all(mid);
     if (!any) return;
     const double* sc = sh + c * ns;
+    const int a0 = wave * nq / SPLIT, a1 = (wave + 1) * nq / SPLIT;
     double ar, ai, mr, mi;
     if (all_far) {
-        pair_sums<KIND, NQ, PAIR_FAR>(nq, n, i, nodes, sc, nullptr, k, eta, ar, ai, mr, mi);
+        pair_sums<KIND, NQ, PAIR_FAR>(nq, a0, a1, n, i, nodes, sc, nullptr, k, eta, ar, ai, mr, mi);
     } else if (all_mid) {
-        pair_sums<KIND, NQ, PAIR_MID>(nq, n, i, nodes, sc, nullptr, k, eta, ar, ai, mr, mi);
+        pair_sums<KIND, NQ, PAIR_MID>(nq, a0, a1, n, i, nodes, sc, nullptr, k, eta, ar, ai, mr, mi);
     } else {
-        // near pairs: the piece tables go to LDS (the workgroup is this one wavefront, so the
-        // barrier is legal here); lanes that have left share no part of the copy
-        const unsigned long long act = __ballot(1);
-        const int rank = __popcll(act & ((1ull << lane) - 1ull));
-        stage_piece_tables(piece_tab, rank, __popcll(act));
+        // near pairs: the piece tables go to LDS, copied by wavefront 0's remaining lanes
+        if (wave == 0) {
+            const unsigned long long act = __ballot(1);
+            const int rank = __popcll(act & ((1ull << lane) - 1ull));
+            stage_piece_tables(piece_tab, rank, __popcll(act));
+        }
         __syncthreads();
-        pair_sums<KIND, NQ, PAIR_GENERAL>(nq, n, i, nodes, sc, piece_tab, k, eta, ar, ai, mr, mi);
+        pair_sums<KIND, NQ, PAIR_GENERAL>(nq, a0, a1, n, i, nodes, sc, piece_tab, k, eta, ar, ai,
+                                          mr, mi);
+    }
+    if (SPLIT > 1) {
+        if (wave > 0) {
+            part[wave - 1][0][lane] = ar;
+            part[wave - 1][1][lane] = ai;
+            part[wave - 1][2][lane] = mr;
+            part[wave - 1][3][lane] = mi;
+        }
+        __syncthreads();
+        if (wave > 0) return;
+#pragma unroll
+        for (int w = 0; w < SPLIT - 1; ++w) {
+            ar += part[w][0][lane];
+            ai += part[w][1][lane];
+            mr += part[w][2][lane];
+            mi += part[w][3][lane];
+        }
     }
     if (direct) A[i + lda * j] = make_double2(kscale * ar, kscale * ai);
     if (mirror) A[j + lda * i] = make_double2(kscale * mr, kscale * mi);

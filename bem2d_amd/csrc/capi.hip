@@ -596,12 +596,18 @@ static int launch_assembly(bem2d_ctx* c, hipStream_t s, const AsmProblem* probs,
         }
         if (reg.start[cnt] > 0) {
             bem2d_internal_prof_begin(c, BEM2D_PROF_ASSEMBLE_REGULAR, s);
-            if (nq == 16)
-                assemble_regular_kernel<KIND, 16>
-                    <<<(unsigned)reg.start[cnt], 64, shm, s>>>(reg);
+            // few tiles (small problems): four wavefronts per tile, each over a quarter of the
+            // test nodes, so the launch fills the device and every lane's chain is shorter
+            const unsigned grid = (unsigned)reg.start[cnt];
+            const bool split = reg.start[cnt] < kSplitTiles;
+            if (nq == 16 && split)
+                assemble_regular_kernel<KIND, 16, 4><<<grid, 256, shm, s>>>(reg);
+            else if (nq == 16)
+                assemble_regular_kernel<KIND, 16, 1><<<grid, 64, shm, s>>>(reg);
+            else if (split)
+                assemble_regular_kernel<KIND, 0, 4><<<grid, 256, shm, s>>>(reg);
             else
-                assemble_regular_kernel<KIND, 0>
-                    <<<(unsigned)reg.start[cnt], 64, shm, s>>>(reg);
+                assemble_regular_kernel<KIND, 0, 1><<<grid, 64, shm, s>>>(reg);
             bem2d_internal_prof_end(c, BEM2D_PROF_ASSEMBLE_REGULAR, s);
         }
         if (fork && (hipEventRecord(c->ev_join, c->aux) != hipSuccess ||
