@@ -1602,8 +1602,10 @@ extern "C" int bem2d_lu32_factor(bem2d_ctx* c, bem2d_lu32* f, const void* A64, i
 // the units still running.  Progress is asynchronous: step() enqueues a step and an asynchronous
 // copy of every unit's norms into pinned host memory; ready() polls the step's event; collect()
 // reads the norms and retires the units that met the tolerance.  The first step() enqueues the
-// initial residual and the first correction together: x = 0 meets no tolerance unless b = 0, so
-// waiting for the initial norms would only add a round trip through the host.
+// initial residual and the first two corrections together: x = 0 meets no tolerance unless b = 0,
+// and one correction from the FP32 factors leaves a backward error near the FP32 rounding level
+// (~1e-7), far above any FP64 target, so waiting for those norms would only add round trips
+// through the host.  (A unit limited to one step gets one.)
 struct RefJob {
     bem2d_ctx* ctx = nullptr;
     DevArena* arena = nullptr;
@@ -1624,6 +1626,7 @@ struct RefJob {
     bool pending = false;
     bool started = false;
     bool bnorm_read = false;              // |b| per column taken from the first read-back
+    int last_steps = 1;                   // correction steps behind the pending read-back
     int nspec = 0;
 };
 
@@ -1759,7 +1762,10 @@ int bem2d_internal_ref_step(RefJob* j, hipStream_t s)
     std::vector<int> ids;
     for (size_t u = 0; u < j->units.size(); ++u)
         if (j->active[u]) ids.push_back((int)u);
+    int nsteps = 1;
     if (!j->started) {
+        nsteps = 2;
+        for (int u : ids) nsteps = std::min(nsteps, j->maxit[u]);
         const size_t nu = j->units.size();
         zero_u64_kernel<<<(unsigned)((kNormSlots * nu + 255) / 256), 256, 0, s>>>(
             j->dnorms, (int64_t)(kNormSlots * nu));
@@ -1772,7 +1778,7 @@ int bem2d_internal_ref_step(RefJob* j, hipStream_t s)
         }
         j->started = true;
     }
-    {
+    for (int st = 0; st < nsteps; ++st) {
         j->epoch++;
         for (size_t c0 = 0; c0 < ids.size(); c0 += kMaxLuBatch) {
             RefBatch one = ref_batch(j, ids, c0, 1);
@@ -1795,6 +1801,7 @@ int bem2d_internal_ref_step(RefJob* j, hipStream_t s)
         hipEventRecord(j->ev, s) != hipSuccess || hipGetLastError() != hipSuccess)
         return FAIL(BEM2D_ERR_DEVICE, "refinement launch failed");
     j->pending = true;
+    j->last_steps = nsteps;
     return BEM2D_OK;
 }
 
@@ -1835,7 +1842,7 @@ int bem2d_internal_ref_collect(RefJob* j, bool* all_done)
         if (!j->bnorm_read) // the initial residual's slots, untouched by the correction steps
             for (int h = 0; h < U.nrhs; ++h)
                 j->bnorm[kMaxSolveRhs * u + h] = u64_as_double(nm[2 * kMaxSolveRhs + h]);
-        j->iters[u]++;
+        j->iters[u] += j->last_steps;
         const double anorm = u64_as_double(nm[kNormA]);
         double w = 0.0;
         for (int h = 0; h < U.nrhs; ++h) {

@@ -33,18 +33,24 @@ PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_MFMA_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "traffic_assemble_regular_kite4000.md")
+FP64_PROFILE = os.path.join(ROOT, "profiles", "fp64_assemble_regular_kite4000.md")
 
 
-def measured_traffic():
-    """HBM bytes per launch of the regular assembly kernel on the default workload, from the
-    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (a line 'traffic_bytes_per_launch: B')."""
+def profile_value(path, key):
+    """A 'key: value' line of a committed rocprofv3 summary under profiles/, or None."""
     try:
-        for line in open(TRAFFIC_PROFILE):
-            if line.startswith("traffic_bytes_per_launch:"):
+        for line in open(path):
+            if line.startswith(key + ":"):
                 return float(line.split(":", 1)[1])
     except OSError:
         pass
     return None
+
+
+def measured_traffic():
+    """HBM bytes per launch of the regular assembly kernel on the default workload, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary."""
+    return profile_value(TRAFFIC_PROFILE, "traffic_bytes_per_launch")
 
 
 def pair_points(n, nspecial, nq=16, L=3):
@@ -382,6 +388,8 @@ def main():
     lu.close()
 
     achieved = FLOP_PER_PAIRPOINT * P_reg / (reg_avg * 1e-3) / 1e12
+    hw_flop = profile_value(FP64_PROFILE, "hardware_fp64_flop_per_launch")
+    hw_tflops = None if hw_flop is None else hw_flop / (reg_avg * 1e-3) / 1e12
     plain_sps = args.steps / dt if dt else None
     mixed_sps = args.steps / dt_mixed if dt_mixed else None
     use_mixed = mixed_ok and (plain_sps is None or mixed_sps >= plain_sps)
@@ -423,13 +431,14 @@ def main():
         "pair_points": P,
         "backward_error_fp64": bw,
         "roofline": {
-            "kernel": "assemble_regular_kernel<COMBINED,16>",
+            "kernel": "assemble_regular_kernel<COMBINED,16,1>",
             "bound": "mfma",
             "engine": "FP64 vector ALU: compute-bound (about 2000 FLOP per stored byte); its "
                       "peak, 78.6 TFLOP/s, is also MI355X's dense FP64 matrix peak",
             "convention": "128 FLOP per pair-point of the reference algorithm (256 per regular "
                           "entry); the kernel shares each Hankel evaluation between entries (i,j) "
-                          "and (j,i), so it issues about half of these FLOP in hardware",
+                          "and (j,i), so it executes about half of these FLOP and frac can pass 1; "
+                          "hardware_fp64_* give the executed FP64 rate",
             "achieved": achieved,
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
@@ -437,6 +446,12 @@ def main():
             "traffic": measured_traffic(),
             "work_per_launch": "%d FLOP = 128 FLOP x 256 pair-points x %d regular entries" %
                                (FLOP_PER_PAIRPOINT * P_reg, n_reg),
+            # diagnostics beside the convention: FP64 FLOP the hardware executes per launch (SQ
+            # instruction counters of the committed profile, every lane counted, FMA as 2) over
+            # this run's kernel time, and the VALU issue rate the same profile measured
+            "hardware_fp64_tflops": hw_tflops,
+            "hardware_fp64_frac": None if hw_tflops is None else hw_tflops / PEAK_FP64_TFLOPS,
+            "valu_issue_fraction": profile_value(FP64_PROFILE, "valu_issue_fraction"),
         },
     }
     if not args.no_cpu_baseline:
