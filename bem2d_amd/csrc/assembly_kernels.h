@@ -129,7 +129,7 @@ struct AsmProblem {
     int64_t nspecial;
     const int32_t* type;
     const double* param;
-    const double* rules;          // AG1 x|y|w, AG2 x|y|w, AG3 x|y|w
+    const double* rules;          // AG1 x|y|w, AG2 x|y|w, AG3 x|y|w, then the abscissa blocks
     const double* g1d;            // Gauss1D x | w
     double k, eta, kscale, mscale;
     double2* A;
@@ -454,6 +454,31 @@ __global__ __launch_bounds__(64 * SPLIT) void assemble_regular_kernel(const AsmB
     if (mirror) A[j + lda * i] = make_double2(kscale * mr, kscale * mi);
 }
 
+// Abscissa blocks of the singular-pair rules (built with the mesh, after the x|y|w arrays of
+// AG1, AG2, AG3): per rule, the counts nx, ny of its distinct x and y, then kRuleAbscissae slots
+// of each (the distinct values), then per node the indices (ix, iy) of its x and y, all doubles.
+// The element maps at a rule's x (test side) and y (trial side) then need evaluating once per
+// distinct value instead of once per node (AdaptedGauss1 has L N distinct values for N^2 L^2
+// nodes).  A count of 0 (more distinct values than slots) means: evaluate at every node.
+constexpr int kRuleAbscissae = 256;
+__host__ __device__ __forceinline__ int64_t rule_block_doubles(int m)
+{
+    return 2 + 2 * kRuleAbscissae + 2 * (int64_t)m;
+}
+
+// Element map at t for the singular-pair kernel: position, unit normal, |Gamma'|.
+struct ElemPoint {
+    double mx, my, nx, ny, s;
+};
+__device__ __forceinline__ ElemPoint elem_point(int type, const double* p, double t)
+{
+    ElemPoint e;
+    double dx, dy;
+    elem_eval(type, p, t, e.mx, e.my, dx, dy);
+    normal_len(dx, dy, e.nx, e.ny, e.s);
+    return e;
+}
+
 // Coincident / adjacent pairs: one block of 256 threads per entry (i, j, rule).
 // rule 1: IntegrateIdent with AdaptedGauss1 (lib/bem2d_quadrature.h:67-109), plus the EvalIdent
 //         mass entry (lib/bem2d_quadrature.cpp:108-123) scaled by mscale;
@@ -485,42 +510,67 @@ __global__ __launch_bounds__(256) void assemble_special_kernel(const AsmBatch ba
         pj[q] = param[8 * j + q];
     }
     const double k4 = 0.25 * k, eta4 = 0.25 * eta, kinv = 1.0 / k;
+    // element maps at the rule's distinct abscissae: the test element at x (and, for the Duffy
+    // split of rule 1, at 1 - x), the trial element at y for rules 2 and 3
+    __shared__ ElemPoint cache[2][kRuleAbscissae];
     double ar = 0.0, ai = 0.0;
     if (KIND != BEM2D_KERNEL_NONE && kscale != 0.0) {
         const double* rx;
+        const double* blk = rules + 3 * (int64_t)(m1 + m2 + m3);
         int m;
-        if (rule == 1) { rx = rules; m = m1; }
-        else if (rule == 3) { rx = rules + 3 * (int64_t)(m1 + m2); m = m3; }
-        else { rx = rules + 3 * (int64_t)m1; m = m2; }
+        if (rule == 1) {
+            rx = rules;
+            m = m1;
+        } else if (rule == 3) {
+            rx = rules + 3 * (int64_t)(m1 + m2);
+            m = m3;
+            blk += rule_block_doubles(m1) + rule_block_doubles(m2);
+        } else {
+            rx = rules + 3 * (int64_t)m1;
+            m = m2;
+            blk += rule_block_doubles(m1);
+        }
         const double* ry = rx + m;
         const double* rw = ry + m;
+        const int nx = (int)blk[0], ny = (int)blk[1];
+        const double* ux = blk + 2;
+        const double* uy = ux + kRuleAbscissae;
+        const double* ridx = uy + kRuleAbscissae;
+        const bool cached = nx > 0 && ny > 0;
+        if (cached) {
+            for (int u = threadIdx.x; u < nx; u += blockDim.x) {
+                cache[0][u] = elem_point(ti, pi_, ux[u]);
+                if (rule == 1) cache[1][u] = elem_point(ti, pi_, 1 - ux[u]);
+            }
+            if (rule != 1)
+                for (int u = threadIdx.x; u < ny; u += blockDim.x) cache[1][u] = elem_point(tj, pj, uy[u]);
+            __syncthreads();
+        }
         for (int q = threadIdx.x; q < m; q += blockDim.x) {
             const double xq = rx[q], yq = ry[q], wq = rw[q];
-            double mx1, my1, dx1, dy1, mx2, my2, dx2, dy2, n1x, n1y, s1, n2x, n2y, s2, kr, ki;
+            const int ix = cached ? (int)ridx[2 * q] : 0, iy = cached ? (int)ridx[2 * q + 1] : 0;
+            double kr, ki;
             if (rule == 1) {
                 // Duffy split of the coincident pair: (x, (1-y)x) and (1-x, 1-x+yx), Jacobian x
-                elem_eval(ti, pi_, xq, mx1, my1, dx1, dy1);
-                elem_eval(tj, pj, (1 - yq) * xq, mx2, my2, dx2, dy2);
-                normal_len(dx1, dy1, n1x, n1y, s1);
-                normal_len(dx2, dy2, n2x, n2y, s2);
-                kernel_eval<KIND>(k, kinv, k4, eta4, mx1, my1, mx2, my2, n1x, n1y, n2x, n2y, kr, ki);
-                ar += xq * kr * s1 * s2 * wq;
-                ai += xq * ki * s1 * s2 * wq;
-                elem_eval(ti, pi_, 1 - xq, mx1, my1, dx1, dy1);
-                elem_eval(tj, pj, 1 - xq + yq * xq, mx2, my2, dx2, dy2);
-                normal_len(dx1, dy1, n1x, n1y, s1);
-                normal_len(dx2, dy2, n2x, n2y, s2);
-                kernel_eval<KIND>(k, kinv, k4, eta4, mx1, my1, mx2, my2, n1x, n1y, n2x, n2y, kr, ki);
-                ar += xq * kr * s1 * s2 * wq;
-                ai += xq * ki * s1 * s2 * wq;
+                ElemPoint p1 = cached ? cache[0][ix] : elem_point(ti, pi_, xq);
+                ElemPoint p2 = elem_point(tj, pj, (1 - yq) * xq);
+                kernel_eval<KIND>(k, kinv, k4, eta4, p1.mx, p1.my, p2.mx, p2.my, p1.nx, p1.ny, p2.nx,
+                                  p2.ny, kr, ki);
+                ar += xq * kr * p1.s * p2.s * wq;
+                ai += xq * ki * p1.s * p2.s * wq;
+                p1 = cached ? cache[1][ix] : elem_point(ti, pi_, 1 - xq);
+                p2 = elem_point(tj, pj, 1 - xq + yq * xq);
+                kernel_eval<KIND>(k, kinv, k4, eta4, p1.mx, p1.my, p2.mx, p2.my, p1.nx, p1.ny, p2.nx,
+                                  p2.ny, kr, ki);
+                ar += xq * kr * p1.s * p2.s * wq;
+                ai += xq * ki * p1.s * p2.s * wq;
             } else {
-                elem_eval(ti, pi_, xq, mx1, my1, dx1, dy1);
-                elem_eval(tj, pj, yq, mx2, my2, dx2, dy2);
-                normal_len(dx1, dy1, n1x, n1y, s1);
-                normal_len(dx2, dy2, n2x, n2y, s2);
-                kernel_eval<KIND>(k, kinv, k4, eta4, mx1, my1, mx2, my2, n1x, n1y, n2x, n2y, kr, ki);
-                ar += kr * s1 * s2 * wq;
-                ai += ki * s1 * s2 * wq;
+                const ElemPoint p1 = cached ? cache[0][ix] : elem_point(ti, pi_, xq);
+                const ElemPoint p2 = cached ? cache[1][iy] : elem_point(tj, pj, yq);
+                kernel_eval<KIND>(k, kinv, k4, eta4, p1.mx, p1.my, p2.mx, p2.my, p1.nx, p1.ny, p2.nx,
+                                  p2.ny, kr, ki);
+                ar += kr * p1.s * p2.s * wq;
+                ai += ki * p1.s * p2.s * wq;
             }
         }
     }

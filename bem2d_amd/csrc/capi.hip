@@ -4,6 +4,7 @@
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -341,6 +342,38 @@ Rule adapted23(const std::vector<double>& g, const std::vector<double>& gw, int 
     return r;
 }
 
+// The abscissa block of a rule for the singular-pair kernel (layout at kRuleAbscissae): distinct
+// x and y values (compared bit for bit, so a cached map is the one the node would compute) and
+// every node's indices into them.
+void append_abscissa_block(std::vector<double>& out, const Rule& r)
+{
+    const size_t m = r.w.size();
+    std::vector<double> ux, uy, idx(2 * m);
+    std::unordered_map<uint64_t, int> fx, fy;
+    auto find = [](std::unordered_map<uint64_t, int>& f, std::vector<double>& u, double v) {
+        uint64_t key;
+        std::memcpy(&key, &v, sizeof key);
+        auto it = f.find(key);
+        if (it != f.end()) return it->second;
+        f.emplace(key, (int)u.size());
+        u.push_back(v);
+        return (int)u.size() - 1;
+    };
+    for (size_t q = 0; q < m; ++q) {
+        idx[2 * q] = find(fx, ux, r.x[q]);
+        idx[2 * q + 1] = find(fy, uy, r.y[q]);
+    }
+    const bool fits = ux.size() <= (size_t)kRuleAbscissae && uy.size() <= (size_t)kRuleAbscissae;
+    out.push_back(fits ? (double)ux.size() : 0.0);
+    out.push_back(fits ? (double)uy.size() : 0.0);
+    ux.resize(kRuleAbscissae, 0.0);
+    uy.resize(kRuleAbscissae, 0.0);
+    if (!fits) std::fill(idx.begin(), idx.end(), 0.0);
+    out.insert(out.end(), ux.begin(), ux.end());
+    out.insert(out.end(), uy.begin(), uy.end());
+    out.insert(out.end(), idx.begin(), idx.end());
+}
+
 } // namespace
 
 extern "C" int bem2d_gauss_legendre(int n, double* x, double* w)
@@ -383,7 +416,8 @@ size_t bem2d_internal_mesh_bytes(int64_t n, bem2d_quad_option q)
     const int64_t N = q.N, L = q.L;
     const int64_t nodes = N * N * L * L + 6 * N * N * L;
     return sizeof(int32_t) * n + sizeof(double) * 8 * n + sizeof(int64_t) * 3 * n +
-           sizeof(double) * (NODE_PLANES * n * N + BOUND_PLANES * n) + sizeof(double) * (3 * nodes + 1) +
+           sizeof(double) * (NODE_PLANES * n * N + BOUND_PLANES * n) +
+           sizeof(double) * (5 * nodes + 3 * (2 + 2 * (int64_t)kRuleAbscissae) + 1) +
            sizeof(double) * 2 * N + sizeof(int64_t) * (9 * n + 3) + 8 * 256;
 }
 
@@ -467,6 +501,7 @@ int bem2d_internal_mesh_create(bem2d_ctx* c, const bem2d_geometry* g, bem2d_quad
         rules.insert(rules.end(), r->y.begin(), r->y.end());
         rules.insert(rules.end(), r->w.begin(), r->w.end());
     }
+    for (const Rule* r : {&r1, &r2, &r3}) append_abscissa_block(rules, *r);
     std::vector<double> g1d(gx);
     g1d.insert(g1d.end(), gw.begin(), gw.end());
 
