@@ -57,6 +57,10 @@ CASES = [
      0.0, 0.0, 1.0),
     ("ellipse_arc_full", lambda: G.curve_mesh("ellipse_arc", 90, (1.4, 0.6, 0.0, 2 * np.pi)),
      (16, 3, 0.15), capi.KERNEL_COMBINED, 30.0, 30.0, 1.0, 2.0),
+    # 50 Gauss points: the adjacent rules have 300 distinct abscissae per axis, more than the
+    # singular-pair kernel caches, so their element maps are evaluated per node
+    ("ellipse_q50_uncached", lambda: G.curve_mesh("ellipse", 24, (1.3,)), (50, 3, 0.15),
+     capi.KERNEL_COMBINED, 8.0, 8.0, 1.0, 2.0),
     ("trefoil_cdl", lambda: G.curve_mesh("trefoil", 64), (12, 3, 0.15), capi.KERNEL_CDL, 7.0,
      0.0, 0.0, 1.0),
     ("inv_ellipse_sl", lambda: G.curve_mesh("inv_ellipse", 48, (0.3,)), (10, 3, 0.15),
@@ -179,6 +183,25 @@ def test_bad_arguments_raise(ctx):
     with pytest.raises(capi.Bem2dError) as e:
         ctx.mesh(bad, 16, 3, 0.15)
     assert e.value.code == capi.ERR_UNSUPPORTED
+
+
+def test_kite_unsplit_unaligned(ctx, oracle):
+    """N = 1453: enough tiles for one wavefront per 8 x 8 tile (no split over test nodes) and a
+    last tile row and column cut short; sampled columns, the last ones included, against the
+    oracle."""
+    n, k = 1453, 40.0
+    geom = G.kite(n)
+    mesh = ctx.mesh(geom, 16, 3, 0.15)
+    A = ctx.assemble(mesh, capi.KERNEL_COMBINED, k, k, 1.0, 2.0)
+    cols = np.array([0, 5, 700, 1444, 1451, 1452])
+    ref = oracle.assemble(geom, 16, 3, 0.15, oracle.K_COMBINED, k, k, 1.0, 2.0, cols=cols)
+    floor = oracle.noise_floor(geom, 16, 3, 0.15, oracle.K_COMBINED, k, k, 1.0, 2.0, cols=cols)
+    got = _np(A[:, torch.tensor(cols, device="cuda")])
+    err = rel_fro(got, ref)
+    print("kite N=1453 sampled columns: err %.2e, oracle noise floor %.2e" % (err, floor))
+    assert err < max(TOL, 2 * floor), (err, floor)
+    assert torch.isfinite(torch.view_as_real(A)).all()
+    mesh.close()
 
 
 def test_kite_config2_full_size(ctx, oracle):
