@@ -127,7 +127,8 @@ int bem2d_assemble(bem2d_ctx* ctx, const bem2d_mesh* mesh, int kernel_kind, doub
 /* ---- right-hand sides -------------------------------------------------------------------------- */
 /* B[i + ldb*r] = scale * DotProdBasFuns(geom, opts, field(dirs[r]))_i   (lib/bem2d_quadrature.h:
  * 111-156) for r < nrhs; dirs is a HOST array of nrhs (x, y) directions (the field's direction,
- * used as given).  DiscretizeRhs (lib/bem2d_soundsoftscattering.h:105-110) is scale = 2. */
+ * used as given), read before the call returns; B is written in stream order (the call does not
+ * wait for it).  DiscretizeRhs (lib/bem2d_soundsoftscattering.h:105-110) is scale = 2. */
 int bem2d_rhs(bem2d_ctx* ctx, const bem2d_mesh* mesh, int field_kind, double k, double eta,
               const double* dirs, int nrhs, double scale, void* B, int64_t ldb);
 
