@@ -724,22 +724,20 @@ __device__ __forceinline__ void slab_load4(__amdgpu_buffer_rsrc_t r, unsigned vo
     __builtin_memcpy(f, &a, 16);
     __builtin_memcpy(f + 2, &b, 16);
 }
-__device__ __forceinline__ void slab_store4(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff,
-                                            const float2* f)
-{
-    decltype(__builtin_amdgcn_raw_buffer_load_b128(r, 0u, 0u, 0)) a, b;
-    __builtin_memcpy(&a, f, 16);
-    __builtin_memcpy(&b, f + 2, 16);
-    __builtin_amdgcn_raw_buffer_store_b128(a, r, voff, soff, 0);
-    __builtin_amdgcn_raw_buffer_store_b128(b, r, voff, soff + 16, 0);
-}
 
 // The trailing update stages its first operand k-major ([k][row], pitch KPA): the 64 consecutive
 // rows a wavefront writes and the 16 rows x 4 k a MFMA operand reads both fall in distinct banks.
 constexpr int KPA = NB + 16;
 constexpr int kUpdOpFloats = 2 * NB * KPA + 2 * NB * KP;
-constexpr int kUpdLdsFloats = kUpdOpFloats > (int)(sizeof(DiagLds) / sizeof(float))
-                                  ? kUpdOpFloats
+// Each wavefront's 32 x 32 output quarter passes through LDS on its way out, column-major with a
+// padded column (conflict-free writes of the accumulator rows), so that every store instruction
+// covers whole 128-byte column segments: the accumulator layout (four consecutive rows of one
+// column per lane) would otherwise give every store only half of each segment.
+constexpr int kOutPitch = 34;                            // complex values per LDS column
+constexpr int kUpdOutFloats = 4 * 2 * 32 * kOutPitch;    // four wavefronts' quarters
+constexpr int kUpdAllFloats = kUpdOpFloats + kUpdOutFloats;
+constexpr int kUpdLdsFloats = kUpdAllFloats > (int)(sizeof(DiagLds) / sizeof(float))
+                                  ? kUpdAllFloats
                                   : (int)(sizeof(DiagLds) / sizeof(float));
 
 // c += Aop Bop with Aop in LDS as [k][row] (pitch KPA) and Bop as [col][k] (pitch KP).  The two
@@ -829,14 +827,34 @@ __global__ __launch_bounds__(256) void trailing_update_kernel(const LuBatch batc
     float2 res[2][2][4];
     __amdgpu_buffer_rsrc_t rres = slab(A);
     bool pending = false;
-    // a lane's four accumulator rows of a 16 x 16 block are consecutive in memory: 32-byte accesses,
-    // four lanes covering 128 contiguous bytes of a column
+    // this wavefront's output quarter in LDS: column c (0..31) at c * kOutPitch complex values
+    float2* outq = reinterpret_cast<float2*>(lds + kUpdOpFloats) + wave * 32 * kOutPitch;
     auto store_res = [&]() __attribute__((always_inline)) {
+        // accumulator rows 16 p + 4 (lane / 16) .. + 3 of column 16 q + lane % 16: 32 bytes each
 #pragma unroll
         for (int p = 0; p < 2; ++p)
 #pragma unroll
-            for (int q = 0; q < 2; ++q)
-                slab_store4(rres, voffc, (unsigned)((16 * p + ld * 16 * q) * 8), res[p][q]);
+            for (int q = 0; q < 2; ++q) {
+                float2* d = outq + (16 * q + (lane & 15)) * kOutPitch + 16 * p + 4 * (lane >> 4);
+                reinterpret_cast<float4*>(d)[0] = make_float4(res[p][q][0].x, res[p][q][0].y,
+                                                              res[p][q][1].x, res[p][q][1].y);
+                reinterpret_cast<float4*>(d)[1] = make_float4(res[p][q][2].x, res[p][q][2].y,
+                                                              res[p][q][3].x, res[p][q][3].y);
+            }
+        // (the quarter is this wavefront's own: its LDS operations complete in order)
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // store j: columns 4 j .. 4 j + 3, rows 2 (lane % 16), + 1 -- four whole 256-byte columns
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int c = 4 * j + (lane >> 4), r = 2 * (lane & 15);
+            const float4 v = *reinterpret_cast<const float4*>(outq + c * kOutPitch + r);
+            decltype(__builtin_amdgcn_raw_buffer_load_b128(rres, 0u, 0u, 0)) a;
+            __builtin_memcpy(&a, &v, 16);
+            __builtin_amdgcn_raw_buffer_store_b128(
+                a, rres, (unsigned)(((wr + r) + ld * (wc + c)) * 8), 0u, 0);
+        }
     };
     int idx = first;
     if (idx < pb.ntile) fetch(idx, 0);
