@@ -76,6 +76,11 @@ SIGNATURES = {
     "bem2d_quad_rule": (_I, [_I, _I, _I, _D, _P, _P, _P]),
     "bem2d_hankel01": (_I, [_P, _P, _I64, _P]),
     "bem2d_hankel01_host": (_I, [_P, _I64, _P]),
+    "bem2d_hankel01_complex": (_I, [_P, _D, _P, _I64, _P]),
+    "bem2d_hankel01_complex_host": (_I, [_D, _P, _I64, _P]),
+    "bem2d_assemble_complex": (_I, [_P, _P, _I, _D, _D, _D, _D, _D, _P, _I64]),
+    "bem2d_rhs_complex": (_I, [_P, _P, _I, _D, _D, _D, _P, _I, _D, _P, _I64]),
+    "bem2d_eval_single_layer_complex": (_I, [_P, _P, _D, _D, _P, _P, _I64, _P]),
 }
 
 # ---- serving engine ------------------------------------------------------------------------------
@@ -168,6 +173,15 @@ def hankel01_host(x):
     return out[:, 0] + 1j * out[:, 1], out[:, 2] + 1j * out[:, 3]
 
 
+def hankel01_complex_host(phi, z):
+    """Host run of the FP64 complex Hankel routine at complex z (arg z within 0.05 of phi):
+    (n, 4) array of Re H0, Im H0, Re H1, Im H1."""
+    z = np.ascontiguousarray(z, dtype=np.complex128)
+    out = np.zeros((z.size, 4))
+    check(lib().bem2d_hankel01_complex_host(phi, _np_ptr(z), z.size, _np_ptr(out)))
+    return out
+
+
 def quad_rule(which, N, L=3, sigma=0.15):
     """Host rule of the product: Gauss2D (0), AdaptedGauss1 (1), AdaptedGauss2 (2), AdaptedGauss3
     (3) -- the tables the device assembly uses."""
@@ -228,16 +242,31 @@ class Context:
         check(lib().bem2d_hankel01(self.h, _t_ptr(x), x.numel(), _t_ptr(out)))
         return out
 
+    def hankel01_complex(self, phi, z):
+        """Device complex Hankel: z a complex128 cuda tensor (arg z within 0.05 of phi);
+        returns an (n, 4) float64 tensor."""
+        import torch
+        out = torch.empty((z.numel(), 4), dtype=torch.float64, device=z.device)
+        check(lib().bem2d_hankel01_complex(self.h, phi, _t_ptr(z), z.numel(), _t_ptr(out)))
+        return out
+
     def assemble(self, mesh, kind, k, eta=0.0, mass_scale=0.0, kernel_scale=1.0, out=None):
-        """Returns the n x n complex128 matrix as a column-major (transposed-storage) view."""
+        """Returns the n x n complex128 matrix as a column-major (transposed-storage) view.
+        k may be complex (freqtype {re, im}); a nonzero imaginary part takes the complex-k
+        kernels."""
         import torch
         n = mesh.n
         if out is None:
             store = torch.empty((n, n), dtype=torch.complex128, device="cuda:%d" % self.device)
             out = store.t()
         assert out.stride(0) == 1 and out.shape == (n, n)
-        check(lib().bem2d_assemble(self.h, mesh.h, kind, k, eta, mass_scale, kernel_scale,
-                                   _t_ptr(out), _ld(out)))
+        kc = complex(k)
+        if kc.imag != 0.0:
+            check(lib().bem2d_assemble_complex(self.h, mesh.h, kind, kc.real, kc.imag, eta,
+                                               mass_scale, kernel_scale, _t_ptr(out), _ld(out)))
+        else:
+            check(lib().bem2d_assemble(self.h, mesh.h, kind, kc.real, eta, mass_scale,
+                                       kernel_scale, _t_ptr(out), _ld(out)))
         return out
 
     def rhs(self, mesh, kind, k, eta, dirs, scale=1.0, out=None):
@@ -248,8 +277,13 @@ class Context:
         if out is None:
             out = torch.empty((nrhs, n), dtype=torch.complex128,
                               device="cuda:%d" % self.device).t()
-        check(lib().bem2d_rhs(self.h, mesh.h, kind, k, eta, _np_ptr(dirs), nrhs, scale,
-                              _t_ptr(out), _ld(out)))
+        kc = complex(k)
+        if kc.imag != 0.0:
+            check(lib().bem2d_rhs_complex(self.h, mesh.h, kind, kc.real, kc.imag, eta,
+                                          _np_ptr(dirs), nrhs, scale, _t_ptr(out), _ld(out)))
+        else:
+            check(lib().bem2d_rhs(self.h, mesh.h, kind, kc.real, eta, _np_ptr(dirs), nrhs, scale,
+                                  _t_ptr(out), _ld(out)))
         return out
 
     def solve_fp64(self, A, B, preserve_A=True):
@@ -275,8 +309,14 @@ class Context:
         import torch
         pts = pts.contiguous()
         out = torch.empty(pts.shape[0], dtype=torch.complex128, device=pts.device)
-        check(lib().bem2d_eval_single_layer(self.h, mesh.h, k, _t_ptr(coef.contiguous()),
-                                            _t_ptr(pts), pts.shape[0], _t_ptr(out)))
+        kc = complex(k)
+        if kc.imag != 0.0:
+            check(lib().bem2d_eval_single_layer_complex(self.h, mesh.h, kc.real, kc.imag,
+                                                        _t_ptr(coef.contiguous()), _t_ptr(pts),
+                                                        pts.shape[0], _t_ptr(out)))
+        else:
+            check(lib().bem2d_eval_single_layer(self.h, mesh.h, kc.real, _t_ptr(coef.contiguous()),
+                                                _t_ptr(pts), pts.shape[0], _t_ptr(out)))
         return out
 
     def farfield(self, mesh, k, coef, theta):

@@ -64,7 +64,7 @@ __host__ __device__ __forceinline__ void kernel_eval(double k, double kinv, doub
 }
 
 // Gauss-node tables: for element e and node a, position, unit normal and |Gamma'| w_a.
-__global__ void prep_nodes_kernel(int64_t n, int nq, const int32_t* __restrict__ type,
+static __global__ void prep_nodes_kernel(int64_t n, int nq, const int32_t* __restrict__ type,
                                   const double* __restrict__ param, const double* __restrict__ gx,
                                   const double* __restrict__ gw, double* __restrict__ nodes)
 {
@@ -91,7 +91,7 @@ __global__ void prep_nodes_kernel(int64_t n, int nq, const int32_t* __restrict__
 // |x - y| >= |c_e - c_f| - r_e - r_f.
 enum { BOUND_CX = 0, BOUND_CY = 1, BOUND_R = 2, BOUND_PLANES = 3 };
 
-__global__ void prep_bounds_kernel(int64_t n, int nq, const double* __restrict__ nodes,
+static __global__ void prep_bounds_kernel(int64_t n, int nq, const double* __restrict__ nodes,
                                    double* __restrict__ bounds)
 {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -483,8 +483,25 @@ __device__ __forceinline__ ElemPoint elem_point(int type, const double* p, doubl
 // rule 1: IntegrateIdent with AdaptedGauss1 (lib/bem2d_quadrature.h:67-109), plus the EvalIdent
 //         mass entry (lib/bem2d_quadrature.cpp:108-123) scaled by mscale;
 // rule 3: next(i) == index(j), Integrate with AdaptedGauss3; rule 2: prev, AdaptedGauss2.
-template <int KIND>
-__global__ __launch_bounds__(256) void assemble_special_kernel(const AsmBatch batch)
+// Kernel values of the real-k functors for the singular-pair body below.
+struct RealKernelEval {
+    double k, kinv, k4, eta4;
+    __device__ __forceinline__ RealKernelEval(double k_, double eta_)
+        : k(k_), kinv(1.0 / k_), k4(0.25 * k_), eta4(0.25 * eta_)
+    {
+    }
+    template <int KIND>
+    __device__ __forceinline__ void eval(double xx, double xy, double yx, double yy, double n1x,
+                                         double n1y, double n2x, double n2y, double& kr,
+                                         double& ki) const
+    {
+        kernel_eval<KIND>(k, kinv, k4, eta4, xx, xy, yx, yy, n1x, n1y, n2x, n2y, kr, ki);
+    }
+};
+
+// One singular entry per 256-thread block; ev supplies the kernel values (real or complex k).
+template <int KIND, class Eval>
+__device__ __forceinline__ void special_entry(const AsmBatch& batch, const Eval& ev)
 {
     const int di = batch_find(batch.start, batch.count, blockIdx.x);
     const AsmProblem& pb = batch.p[di];
@@ -495,7 +512,7 @@ __global__ __launch_bounds__(256) void assemble_special_kernel(const AsmBatch ba
     const int m1 = pb.m1, m2 = pb.m2, m3 = pb.m3, nq = pb.nq;
     const double* __restrict__ gx = pb.g1d;
     const double* __restrict__ gw = pb.g1d + pb.nq;
-    const double k = pb.k, eta = pb.eta, kscale = pb.kscale, mscale = pb.mscale;
+    const double kscale = pb.kscale, mscale = pb.mscale;
     double2* __restrict__ A = pb.A;
     const int64_t lda = pb.lda;
     __shared__ double red[2][256];
@@ -509,7 +526,6 @@ __global__ __launch_bounds__(256) void assemble_special_kernel(const AsmBatch ba
         pi_[q] = param[8 * i + q];
         pj[q] = param[8 * j + q];
     }
-    const double k4 = 0.25 * k, eta4 = 0.25 * eta, kinv = 1.0 / k;
     // element maps at the rule's distinct abscissae: the test element at x (and, for the Duffy
     // split of rule 1, at 1 - x), the trial element at y for rules 2 and 3
     __shared__ ElemPoint cache[2][kRuleAbscissae];
@@ -554,21 +570,21 @@ __global__ __launch_bounds__(256) void assemble_special_kernel(const AsmBatch ba
                 // Duffy split of the coincident pair: (x, (1-y)x) and (1-x, 1-x+yx), Jacobian x
                 ElemPoint p1 = cached ? cache[0][ix] : elem_point(ti, pi_, xq);
                 ElemPoint p2 = elem_point(tj, pj, (1 - yq) * xq);
-                kernel_eval<KIND>(k, kinv, k4, eta4, p1.mx, p1.my, p2.mx, p2.my, p1.nx, p1.ny, p2.nx,
-                                  p2.ny, kr, ki);
+                ev.template eval<KIND>(p1.mx, p1.my, p2.mx, p2.my, p1.nx, p1.ny, p2.nx, p2.ny,
+                                       kr, ki);
                 ar += xq * kr * p1.s * p2.s * wq;
                 ai += xq * ki * p1.s * p2.s * wq;
                 p1 = cached ? cache[1][ix] : elem_point(ti, pi_, 1 - xq);
                 p2 = elem_point(tj, pj, 1 - xq + yq * xq);
-                kernel_eval<KIND>(k, kinv, k4, eta4, p1.mx, p1.my, p2.mx, p2.my, p1.nx, p1.ny, p2.nx,
-                                  p2.ny, kr, ki);
+                ev.template eval<KIND>(p1.mx, p1.my, p2.mx, p2.my, p1.nx, p1.ny, p2.nx, p2.ny,
+                                       kr, ki);
                 ar += xq * kr * p1.s * p2.s * wq;
                 ai += xq * ki * p1.s * p2.s * wq;
             } else {
                 const ElemPoint p1 = cached ? cache[0][ix] : elem_point(ti, pi_, xq);
                 const ElemPoint p2 = cached ? cache[1][iy] : elem_point(tj, pj, yq);
-                kernel_eval<KIND>(k, kinv, k4, eta4, p1.mx, p1.my, p2.mx, p2.my, p1.nx, p1.ny, p2.nx,
-                                  p2.ny, kr, ki);
+                ev.template eval<KIND>(p1.mx, p1.my, p2.mx, p2.my, p1.nx, p1.ny, p2.nx, p2.ny,
+                                       kr, ki);
                 ar += kr * p1.s * p2.s * wq;
                 ai += ki * p1.s * p2.s * wq;
             }
@@ -599,9 +615,16 @@ __global__ __launch_bounds__(256) void assemble_special_kernel(const AsmBatch ba
     }
 }
 
+template <int KIND>
+__global__ __launch_bounds__(256) void assemble_special_kernel(const AsmBatch batch)
+{
+    const AsmProblem& pb = batch.p[batch_find(batch.start, batch.count, blockIdx.x)];
+    special_entry<KIND>(batch, RealKernelEval(pb.k, pb.eta));
+}
+
 // DotProdBasFuns (lib/bem2d_quadrature.h:111-156) for the plane-wave field functors
 // (lib/bem2d_fun.h:21-24, 45-49, 71-75): thread per (row i, right-hand side r).
-__global__ void rhs_kernel(int64_t n, int nq, const double* __restrict__ nodes, int kind, double k,
+static __global__ void rhs_kernel(int64_t n, int nq, const double* __restrict__ nodes, int kind, double k,
                            double eta, const double* __restrict__ dirs, double scale,
                            double2* __restrict__ B, int64_t ldb)
 {
@@ -634,7 +657,7 @@ __global__ void rhs_kernel(int64_t n, int nq, const double* __restrict__ nodes, 
 }
 
 // EvalKernel<SingleLayer> at points (lib/bem2d_quadrature.h:184-228): thread per point.
-__global__ void eval_sl_kernel(int64_t n, int nq, const double* __restrict__ nodes, double k,
+static __global__ void eval_sl_kernel(int64_t n, int nq, const double* __restrict__ nodes, double k,
                                const double2* __restrict__ coef, const double* __restrict__ pts,
                                int64_t npts, double2* __restrict__ out)
 {
@@ -663,7 +686,7 @@ __global__ void eval_sl_kernel(int64_t n, int nq, const double* __restrict__ nod
 }
 
 // Far-field signature F[a + nang*j] = sum_t exp(-ik xhat_a.y_t) |Gamma_j'| w_t: thread per (a, j).
-__global__ void farfield_signature_kernel(int64_t n, int nq, const double* __restrict__ nodes,
+static __global__ void farfield_signature_kernel(int64_t n, int nq, const double* __restrict__ nodes,
                                           double k, const double* __restrict__ theta, int64_t nang,
                                           double2* __restrict__ F)
 {
@@ -684,7 +707,7 @@ __global__ void farfield_signature_kernel(int64_t n, int nq, const double* __res
 }
 
 // out[a + nang*r] = pref * sum_j F[a + nang*j] coef[j + ldc*r]: thread per (a, r).
-__global__ void farfield_combine_kernel(int64_t n, const double2* __restrict__ F, int64_t nang,
+static __global__ void farfield_combine_kernel(int64_t n, const double2* __restrict__ F, int64_t nang,
                                         const double2* __restrict__ coef, int64_t ldc, double pr,
                                         double pi_, double2* __restrict__ out)
 {

@@ -717,6 +717,18 @@ extern "C" int bem2d_rhs(bem2d_ctx* c, const bem2d_mesh* m, int kind, double k, 
         return FAIL(BEM2D_ERR_UNSUPPORTED, "unknown field kind %d", kind);
     if (nrhs == 0) return BEM2D_OK;
     HIPCHK(hipSetDevice(c->device));
+    const double* d_dirs = nullptr;
+    if (int rc = bem2d_internal_stage_dirs(c, dirs, nrhs, &d_dirs)) return rc;
+    dim3 grid((unsigned)((m->n + 255) / 256), (unsigned)nrhs);
+    rhs_kernel<<<grid, 256, 0, c->stream>>>(m->n, m->nq, m->d_nodes, kind, k, eta, d_dirs, scale,
+                                            (double2*)B, ldb);
+    HIPCHK(hipGetLastError());
+    return BEM2D_OK;
+}
+
+// Copies nrhs host directions (x, y) to device scratch in stream order; *out = the device copy.
+int bem2d_internal_stage_dirs(bem2d_ctx* c, const double* dirs, int nrhs, const double** out)
+{
     const size_t bytes = sizeof(double) * 2 * nrhs;
     double* d_dirs = (double*)bem2d_internal_scratch(c, SCRATCH_SMALL, bytes);
     if (!d_dirs) return FAIL(BEM2D_ERR_NOMEM, "scratch allocation failed");
@@ -739,10 +751,7 @@ extern "C" int bem2d_rhs(bem2d_ctx* c, const bem2d_mesh* m, int kind, double k, 
     std::memcpy(c->stage, dirs, bytes);
     HIPCHK(hipMemcpyAsync(d_dirs, c->stage, bytes, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipEventRecord(c->ev_stage, c->stream));
-    dim3 grid((unsigned)((m->n + 255) / 256), (unsigned)nrhs);
-    rhs_kernel<<<grid, 256, 0, c->stream>>>(m->n, m->nq, m->d_nodes, kind, k, eta, d_dirs, scale,
-                                            (double2*)B, ldb);
-    HIPCHK(hipGetLastError());
+    *out = d_dirs;
     return BEM2D_OK;
 }
 

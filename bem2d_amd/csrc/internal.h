@@ -16,7 +16,8 @@ enum {
     SCRATCH_FARFIELD = 1, // far-field signatures, nang x n complex128
     SCRATCH_FACTOR = 2,   // copy of A when the caller's matrix must be preserved
     SCRATCH_PIVOT = 3,    // pivots + info of the FP64 LU
-    SCRATCH_COUNT = 4
+    SCRATCH_CONTOUR = 4,  // Hankel contour table of a complex wavenumber
+    SCRATCH_COUNT = 5
 };
 
 struct ScratchBuf {
@@ -92,6 +93,7 @@ void bem2d_internal_gauss01(int n, std::vector<double>& x, std::vector<double>& 
 size_t bem2d_internal_mesh_bytes(int64_t n, bem2d_quad_option q);
 int bem2d_internal_mesh_create(bem2d_ctx* c, const bem2d_geometry* g, bem2d_quad_option q,
                                DevArena* arena, hipStream_t s, bem2d_mesh** out);
+int bem2d_internal_stage_dirs(bem2d_ctx* c, const double* dirs, int nrhs, const double** out);
 int bem2d_internal_rhs(const bem2d_mesh* m, int kind, double k, double eta, const double* d_dirs,
                        int nrhs, double scale, void* B, int64_t ldb, hipStream_t s);
 int bem2d_internal_farfield(const bem2d_mesh* m, double k, const void* coef, int64_t ldc,

@@ -38,8 +38,26 @@ complex BesselH1(double x)
     raise(bem2d_hankel01_host(&x, 1, v));
     return complex(v[2], v[3]);
 }
-complex BesselH0(complex) { throw ParameterException(); }
-complex BesselH1(complex) { throw ParameterException(); }
+// BesselH0/H1(complex) (lib/bem2d_mathroutines.cpp:44-75, AMOS zbesh there) on the library's
+// complex Hankel routine (integration path chosen by arg z).
+static void hankel_complex(complex z, double* v)
+{
+    if (!(z.real() > 0)) throw ParameterException();
+    const double zz[2] = {z.real(), z.imag()};
+    raise(bem2d_hankel01_complex_host(std::arg(z), zz, 1, v));
+}
+complex BesselH0(complex z)
+{
+    double v[4];
+    hankel_complex(z, v);
+    return complex(v[0], v[1]);
+}
+complex BesselH1(complex z)
+{
+    double v[4];
+    hankel_complex(z, v);
+    return complex(v[2], v[3]);
+}
 
 // ---- Curve::Length ---------------------------------------------------------------------------
 double Curve::Length(double t1, double t2) const
@@ -169,13 +187,13 @@ static void flatten(const Geometry& geom, QuadOption opts, MeshHandle& h)
 static pMatrix assemble(const Geometry& geom, QuadOption opts, int kind, freqtype k, double eta,
                         double ms, double ks)
 {
-    if (k.im != 0) throw ParameterException();
     MeshHandle h;
     flatten(geom, opts, h);
     const int64_t n = h.n;
     pMatrix out(new Matrix((int)n));
     DevBuf A(sizeof(complex) * n * n);
-    raise(bem2d_assemble(context(), h.mesh, kind, k.re, eta, ms, ks, A.p, n));
+    // k.im != 0: the kernels' complex-k branch (lib/bem2d_kernel.cpp:22-26 and alike)
+    raise(bem2d_assemble_complex(context(), h.mesh, kind, k.re, k.im, eta, ms, ks, A.p, n));
     raise(bem2d_copy_to_host(context(), out->data->data(), A.p, sizeof(complex) * n * n));
     return out;
 }
@@ -189,14 +207,13 @@ pMatrix AssembleCombined(const Geometry& geom, QuadOption opts, freqtype k, doub
 static pMatrix rhs(const Geometry& geom, QuadOption opts, int kind, freqtype k, double eta,
                    Point dir)
 {
-    if (k.im != 0) throw ParameterException();
     MeshHandle h;
     flatten(geom, opts, h);
     const int64_t n = h.n;
     pMatrix out(new Matrix((int)n, 1));
     DevBuf B(sizeof(complex) * n);
     double d[2] = {dir.x, dir.y};
-    raise(bem2d_rhs(context(), h.mesh, kind, k.re, eta, d, 1, 1.0, B.p, n));
+    raise(bem2d_rhs_complex(context(), h.mesh, kind, k.re, k.im, eta, d, 1, 1.0, B.p, n));
     raise(bem2d_copy_to_host(context(), out->data->data(), B.p, sizeof(complex) * n));
     return out;
 }
@@ -245,7 +262,6 @@ void EvalKernel(const Geometry::flat_basis_map& basfuns, const std::vector<Point
                 cvector& vals, Matrix& alpha, SingleLayer g, const Gauss1D& g1d)
 {
     if (points.size() != vals.size()) throw SizeError();
-    if (g.k().im != 0) throw ParameterException();
     std::vector<pElement> elems;
     for (auto& fb : basfuns) elems.push_back(fb.first);
     Geometry geom(elems);
@@ -264,8 +280,8 @@ void EvalKernel(const Geometry::flat_basis_map& basfuns, const std::vector<Point
     }
     raise(bem2d_copy_to_device(device::context(), c.p, alpha.data->data(), sizeof(complex) * n));
     raise(bem2d_copy_to_device(device::context(), P.p, pts.data(), sizeof(double) * 2 * np));
-    raise(bem2d_eval_single_layer(device::context(), h.mesh, g.k().re, c.p, (const double*)P.p, np,
-                                  out.p));
+    raise(bem2d_eval_single_layer_complex(device::context(), h.mesh, g.k().re, g.k().im, c.p,
+                                          (const double*)P.p, np, out.p));
     cvector v(np);
     raise(bem2d_copy_to_host(device::context(), v.data(), out.p, sizeof(complex) * np));
     for (int64_t i = 0; i < np; i++) vals[i] += v[i];

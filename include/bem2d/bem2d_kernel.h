@@ -1,8 +1,9 @@
 // Drop-in replacement of lib/bem2d_kernel.h:12-98 / bem2d_kernel.cpp:7-112.
 // The functors keep the reference's interface (SetNormal + operator()), evaluated on the host with
 // the library's FP64 Hankel routine; DiscreteKernel (bem2d_quadrature.h) dispatches them to the GPU.
-// Complex wavenumbers (k.im != 0) are not supported by this build: operator() and the GPU
-// operators throw ParameterException for them.
+// Complex wavenumbers (k.im != 0) take the reference's complex branches with the library's complex
+// Hankel routine (BesselH0/H1(complex), the stand-in for AMOS zbesh), on the host here and in the
+// GPU operators; they need k.re > 0 (ParameterException otherwise).
 #ifndef BEM2D_DROPIN_KERNEL_H
 #define BEM2D_DROPIN_KERNEL_H
 
@@ -14,7 +15,7 @@ namespace bem2d {
 
 complex BesselH0(double x);
 complex BesselH1(double x);
-complex BesselH0(complex x); // throws ParameterException (complex argument not supported)
+complex BesselH0(complex x); // Re x > 0; ParameterException otherwise
 complex BesselH1(complex x);
 
 class SingleLayer {
@@ -22,8 +23,8 @@ public:
     SingleLayer(freqtype k) : k_(k) {}
     complex operator()(Point x, Point y) const
     {
-        if (k_.im != 0) throw ParameterException();
         double d = length(x - y);
+        if (k_.im != 0) return complex(0, 1) / 4.0 * BesselH0(complex(k_.re, k_.im) * d);
         return complex(0, 1) / 4.0 * BesselH0(k_.re * d);
     }
     inline void SetNormal(Point normal1, Point normal2)
@@ -43,9 +44,12 @@ public:
     DoubleLayer(freqtype k) : k_(k) {}
     complex operator()(Point x, Point y) const
     {
-        if (k_.im != 0) throw ParameterException();
         Point w = y - x;
         double d = length(w);
+        if (k_.im != 0) {
+            complex kc(k_.re, k_.im);
+            return -complex(0, 1) * kc / 4.0 * (BesselH1(kc * d) / d * (n2_.x * w.x + n2_.y * w.y));
+        }
         return -complex(0, 1) * k_.re / 4.0 * (BesselH1(k_.re * d) / d * (n2_.x * w.x + n2_.y * w.y));
     }
     inline void SetNormal(Point normal1, Point normal2)
@@ -65,9 +69,12 @@ public:
     ConjDoubleLayer(freqtype k) : k_(k) {}
     complex operator()(Point x, Point y) const
     {
-        if (k_.im != 0) throw ParameterException();
         Point w = x - y;
         double d = length(w);
+        if (k_.im != 0) {
+            complex kc(k_.re, k_.im);
+            return -complex(0, 1) * kc / 4.0 * (BesselH1(kc * d) / d * (n1_.x * w.x + n1_.y * w.y));
+        }
         return -complex(0, 1) * k_.re / 4.0 * (BesselH1(k_.re * d) / d * (n1_.x * w.x + n1_.y * w.y));
     }
     inline void SetNormal(Point normal1, Point normal2)
@@ -88,10 +95,15 @@ public:
     CombinedSingleConjDouble(freqtype k) : k_(k), eta_(k.re) {}
     complex operator()(Point x, Point y) const
     {
-        if (k_.im != 0) throw ParameterException();
         Point w = x - y;
         double d = length(w);
         complex i(0, 1);
+        if (k_.im != 0) {
+            complex kc(k_.re, k_.im);
+            complex slayer = i / 4.0 * BesselH0(kc * d);
+            complex cdlayer = -i * kc / 4.0 * (BesselH1(kc * d) / d * (n1_.x * w.x + n1_.y * w.y));
+            return cdlayer - i * eta_ * slayer;
+        }
         complex slayer = i / 4.0 * BesselH0(k_.re * d);
         complex cdlayer = -i * k_.re / 4.0 * (BesselH1(k_.re * d) / d * (n1_.x * w.x + n1_.y * w.y));
         return cdlayer - i * eta_ * slayer;

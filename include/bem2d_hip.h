@@ -124,6 +124,25 @@ int64_t bem2d_mesh_special_count(const bem2d_mesh* mesh);
 int bem2d_assemble(bem2d_ctx* ctx, const bem2d_mesh* mesh, int kernel_kind, double k, double eta,
                    double mass_scale, double kernel_scale, void* A, int64_t lda);
 
+/* ---- complex wavenumbers ---------------------------------------------------------------------
+ * The kernels' and fields' branches for freqtype k with k.im != 0 (lib/bem2d_kernel.cpp:15-112,
+ * lib/bem2d_fun.h:21-24, 45-49, 71-75), whose Hankel values come from AMOS zbesh through
+ * BesselH0(complex) / BesselH1(complex) (lib/bem2d_mathroutines.cpp:44-75); here from the
+ * library's own complex Hankel routine.  Same meaning of the other arguments as bem2d_assemble,
+ * bem2d_rhs and bem2d_eval_single_layer; k_im == 0 calls those (the reference's k.im == 0
+ * branch).  Needs k_re > 0 (BEM2D_ERR_UNSUPPORTED otherwise: AMOS covers the whole plane, the
+ * resonance scans of the reference use Re k in [4, 8]); assembly needs n <= 8191.  eta stays real
+ * as in CombinedSingleConjDouble(freqtype k, double eta). */
+int bem2d_assemble_complex(bem2d_ctx* ctx, const bem2d_mesh* mesh, int kernel_kind, double k_re,
+                           double k_im, double eta, double mass_scale, double kernel_scale,
+                           void* A, int64_t lda);
+int bem2d_rhs_complex(bem2d_ctx* ctx, const bem2d_mesh* mesh, int field_kind, double k_re,
+                      double k_im, double eta, const double* dirs, int nrhs, double scale,
+                      void* B, int64_t ldb);
+int bem2d_eval_single_layer_complex(bem2d_ctx* ctx, const bem2d_mesh* mesh, double k_re,
+                                    double k_im, const void* coef, const double* pts, int64_t npts,
+                                    void* out);
+
 /* ---- right-hand sides -------------------------------------------------------------------------- */
 /* B[i + ldb*r] = scale * DotProdBasFuns(geom, opts, field(dirs[r]))_i   (lib/bem2d_quadrature.h:
  * 111-156) for r < nrhs; dirs is a HOST array of nrhs (x, y) directions (the field's direction,
@@ -279,6 +298,13 @@ int bem2d_quad_rule(int which, int N, int L, double sigma, double* x, double* y,
  * FP64 routine on the host (host pointers) and needs no GPU. */
 int bem2d_hankel01(bem2d_ctx* ctx, const double* x, int64_t n, double* out);
 int bem2d_hankel01_host(const double* x, int64_t n, double* out);
+/* BesselH0(complex), BesselH1(complex) (lib/bem2d_mathroutines.cpp:44-75, AMOS zbesh with
+ * kode = 1, n = 1) at the n complex points z (interleaved re, im; Re z > 0); out as for
+ * bem2d_hankel01.  phi (|phi| < pi/2) selects the integration path of the routine's middle region
+ * (2 <= |z| < 20) and must be within 0.05 of every arg z -- one argument per call, as in an
+ * assembly, where z = k d and phi = arg k.  Device and host versions as above. */
+int bem2d_hankel01_complex(bem2d_ctx* ctx, double phi, const double* z, int64_t n, double* out);
+int bem2d_hankel01_complex_host(double phi, const double* z, int64_t n, double* out);
 
 #ifdef __cplusplus
 }

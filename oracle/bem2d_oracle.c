@@ -13,6 +13,7 @@
  *   - curve / element Map, Deriv, Normal               lib/bem2d_curvetypes.h:13-99,
  *                                                      lib/bem2d_element.h:29-34,72-116
  *   - H0^(1), H1^(1) of real argument (GSL J/Y)        lib/bem2d_mathroutines.cpp:20-42
+ *   - H0^(1), H1^(1) of complex argument (AMOS zbesh)  lib/bem2d_mathroutines.cpp:44-75
  *   - SingleLayer, DoubleLayer, ConjDoubleLayer, CombinedSingleConjDouble, IdKernel
  *                                                      lib/bem2d_kernel.cpp:15-112, bem2d_kernel.h:91-98
  *   - Integrate / IntegrateIdent / DiscreteKernel      lib/bem2d_quadrature.h:37-109,231-294
@@ -29,6 +30,8 @@
  *   oracle_rule           node counts, sum of weights, exact integrals of x^a y^b and of log-singular
  *                         integrands over the covered region (closed forms / mpmath)
  *   oracle_hankel         mpmath hankel1 at 50 digits (tests/golden/hankel_mpmath.npz); Wronskian
+ *   oracle_hankel_complex mpmath hankel1 at 30 digits (tests/golden/hankel_complex_mpmath.npz);
+ *                         scipy.special.hankel1 (AMOS zbesh itself); Wronskian
  *   oracle_assemble       entries of tiny meshes evaluated entirely in mpmath with the same rule nodes
  *                         (tests/golden/entries_mpmath.npz); analytic circle series end to end
  *   oracle_rhs            closed form of the 1-D plane-wave integral over flat elements
@@ -46,6 +49,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <complex.h>
+#include <quadmath.h>
 
 #define O_PI 3.14159265358979323846 /* lib/bem2d_defs.h: PI */
 #define O_PI_L 3.141592653589793238462643383279502884L
@@ -332,6 +336,77 @@ void oracle_hankel_many(const double* x, int64_t n, double* out)
 }
 
 /* ------------------------------------------------------------------------------------------ */
+/* H0^(1), H1^(1) of complex z = zr + i zi (stand-in for AMOS zbesh with kode = 1, n = 1,        */
+/* lib/bem2d_mathroutines.cpp:44-75).  Textbook algorithms, chosen apart from the HIP path's:   */
+/*   |z| <= 20  ascending series A&S 9.1.10 (J) and 9.1.11 with n = 0, 1 (Y) in __float128, so   */
+/*              that the cancellation of terms up to e^|z| still leaves ~1e-17 relative;         */
+/*   |z| > 20   Hankel asymptotic expansion A&S 9.2.5-9.2.10 in long double, summed until the    */
+/*              terms fall below 1e-21 of the sum or stop decreasing (smallest term ~e^-2|z|).    */
+/* ------------------------------------------------------------------------------------------ */
+void oracle_hankel_complex(double zr, double zi, double* out4)
+{
+    if (zr * zr + zi * zi <= 400.0) {
+        const __float128 PIQ = M_PIq;
+        const __float128 EULQ = 0.577215664901532860606512090082402431Q;
+        __complex128 z, t, term, J0, S0, J1s, S1;
+        __real__ z = zr;
+        __imag__ z = zi;
+        t = -z * z / 4;
+        term = 1;
+        J0 = 1; S0 = 0; J1s = 1; S1 = 1;
+        __float128 H = 0, big = 1;
+        for (int m = 1; m < 400; m++) {
+            term = term * t / ((__float128)m * m);
+            H += 1.0Q / m;
+            J0 += term;
+            S0 += H * term;
+            J1s += term / (m + 1);
+            S1 += (2 * H + 1.0Q / (m + 1)) * term / (m + 1);
+            __float128 a = cabsq(term);
+            if (a > big) big = a;
+            if (m > 4 && a < 1e-40Q * big) break;
+        }
+        __complex128 J1 = z / 2 * J1s;
+        __complex128 L = clogq(z / 2) + EULQ;
+        __complex128 Y0 = 2 / PIQ * (L * J0 - S0);
+        __complex128 Y1 = 2 / PIQ * L * J1 - 2 / (PIQ * z) - z / (2 * PIQ) * S1;
+        __complex128 H0 = J0 + 1.0Qi * Y0, H1 = J1 + 1.0Qi * Y1;
+        out4[0] = (double)crealq(H0); out4[1] = (double)cimagq(H0);
+        out4[2] = (double)crealq(H1); out4[3] = (double)cimagq(H1);
+        return;
+    }
+    long double complex z = (long double)zr + I * (long double)zi;
+    long double complex s[2];
+    for (int nu = 0; nu < 2; nu++) {
+        long double mu = 4.0L * nu * nu;
+        long double complex term = 1, sum = 1;
+        long double last = 1;
+        for (int k = 1; k < 80; k++) {
+            long double o = (2.0L * k - 1) * (2.0L * k - 1);
+            term = term * I * (mu - o) / (8.0L * k * z);
+            long double a = cabsl(term);
+            if (a > last) break; /* past the smallest term: the series diverges from here */
+            last = a;
+            sum += term;
+            if (a < 1e-21L * cabsl(sum)) break;
+        }
+        s[nu] = sum;
+    }
+    /* sqrt(2/(pi z)) e^{iz} e^{-i pi/4} (and e^{-i pi/2} more for nu = 1) */
+    long double complex f = csqrtl(2.0L / (O_PI_L * z)) * cexpl(I * z) *
+                            (1.0L - I) / sqrtl(2.0L);
+    long double complex H0 = f * s[0], H1 = -I * f * s[1];
+    out4[0] = (double)creall(H0); out4[1] = (double)cimagl(H0);
+    out4[2] = (double)creall(H1); out4[3] = (double)cimagl(H1);
+}
+
+void oracle_hankel_complex_many(const double* z, int64_t n, double* out)
+{
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t i = 0; i < n; i++) oracle_hankel_complex(z[2 * i], z[2 * i + 1], out + 4 * i);
+}
+
+/* ------------------------------------------------------------------------------------------ */
 /* Geometry (lib/bem2d_curvetypes.h:13-99, lib/bem2d_element.h:29-34, 72-116).                    */
 /* The association of the reference's products (e.g. (2*PI)*t) is kept.                          */
 /* ------------------------------------------------------------------------------------------ */
@@ -429,9 +504,57 @@ static void hankel_pair(double x, double complex* h0, double complex* h1)
     *h1 = v[2] + I * v[3];
 }
 
-/* Kernel functors (lib/bem2d_kernel.cpp:15-112, bem2d_kernel.h:91-98), real k. */
-static double complex kernel_eval(int kind, double k, double eta, pt x, pt y, pt n1, pt n2)
+/* Hankel pair of complex argument in double complex */
+static void hankel_pair_c(double complex z, double complex* h0, double complex* h1)
 {
+    double v[4];
+    oracle_hankel_complex(creal(z), cimag(z), v);
+    *h0 = v[0] + I * v[1];
+    *h1 = v[2] + I * v[3];
+}
+
+/* The k.im != 0 branches of the kernel functors (lib/bem2d_kernel.cpp:18-27, 44-53, 71-80, 97-111). */
+static double complex kernel_eval_c(int kind, double complex kc, double eta, pt x, pt y, pt n1,
+                                    pt n2)
+{
+    double complex h0, h1;
+    switch (kind) {
+    case K_SL: {
+        double d = plen((pt){x.x - y.x, x.y - y.y});
+        hankel_pair_c(kc * d, &h0, &h1);
+        return I / 4.0 * h0;
+    }
+    case K_DL: {
+        pt w = {y.x - x.x, y.y - x.y};
+        double d = plen(w);
+        hankel_pair_c(kc * d, &h0, &h1);
+        return -I * kc / 4.0 * (h1 / d * (n2.x * w.x + n2.y * w.y));
+    }
+    case K_CDL: {
+        pt w = {x.x - y.x, x.y - y.y};
+        double d = plen(w);
+        hankel_pair_c(kc * d, &h0, &h1);
+        return -I * kc / 4.0 * (h1 / d * (n1.x * w.x + n1.y * w.y));
+    }
+    case K_COMBINED: {
+        pt w = {x.x - y.x, x.y - y.y};
+        double d = plen(w);
+        hankel_pair_c(kc * d, &h0, &h1);
+        double complex slayer = I / 4.0 * h0;
+        double complex cdlayer = -I * kc / 4.0 * (h1 / d * (n1.x * w.x + n1.y * w.y));
+        return cdlayer - I * eta * slayer;
+    }
+    case K_ID:
+        return 1.0;
+    }
+    return 0.0;
+}
+
+/* Kernel functors (lib/bem2d_kernel.cpp:15-112, bem2d_kernel.h:91-98); ki = Im k (0: real k). */
+static double complex kernel_eval(int kind, double k, double ki, double eta, pt x, pt y, pt n1,
+                                  pt n2)
+{
+    if (ki != 0.0) return kernel_eval_c(kind, k + I * ki, eta, x, y, n1, n2);
     double complex h0, h1;
     switch (kind) {
     case K_SL: {
@@ -467,7 +590,7 @@ static double complex kernel_eval(int kind, double k, double eta, pt x, pt y, pt
 
 /* Integrate<kernel,T> over a regular or adjacent pair (lib/bem2d_quadrature.h:37-65). P0 basis. */
 static double complex integrate_pair(const geom_t* g, int64_t i, int64_t j, int kind, double k,
-                                     double eta, const double* rx, const double* ry,
+                                     double ki, double eta, const double* rx, const double* ry,
                                      const double* rw, int m)
 {
     double complex res = 0;
@@ -475,7 +598,7 @@ static double complex integrate_pair(const geom_t* g, int64_t i, int64_t j, int 
         pt Mx, Dx, My, Dy;
         elem_eval(g, i, rx[q], &Mx, &Dx);
         elem_eval(g, j, ry[q], &My, &Dy);
-        double complex gv = kernel_eval(kind, k, eta, Mx, My, pnormal(Dx), pnormal(Dy));
+        double complex gv = kernel_eval(kind, k, ki, eta, Mx, My, pnormal(Dx), pnormal(Dy));
         double s1 = plen(Dx), s2 = plen(Dy);
         res += gv * s1 * s2 * rw[q];
     }
@@ -484,7 +607,7 @@ static double complex integrate_pair(const geom_t* g, int64_t i, int64_t j, int 
 
 /* IntegrateIdent: Duffy split of the coincident pair (lib/bem2d_quadrature.h:67-109). */
 static double complex integrate_ident(const geom_t* g, int64_t i, int64_t j, int kind, double k,
-                                      double eta, const double* rx, const double* ry,
+                                      double ki, double eta, const double* rx, const double* ry,
                                       const double* rw, int m)
 {
     double complex res = 0;
@@ -494,13 +617,13 @@ static double complex integrate_ident(const geom_t* g, int64_t i, int64_t j, int
         double s2p = (1 - yq) * xq;
         elem_eval(g, i, xq, &Mx, &Dx);
         elem_eval(g, j, s2p, &My, &Dy);
-        double complex gv = kernel_eval(kind, k, eta, Mx, My, pnormal(Dx), pnormal(Dy));
+        double complex gv = kernel_eval(kind, k, ki, eta, Mx, My, pnormal(Dx), pnormal(Dy));
         res += xq * gv * plen(Dx) * plen(Dy) * rw[q];
 
         double s1b = 1 - xq, s2b = 1 - xq + yq * xq;
         elem_eval(g, i, s1b, &Mx, &Dx);
         elem_eval(g, j, s2b, &My, &Dy);
-        gv = kernel_eval(kind, k, eta, Mx, My, pnormal(Dx), pnormal(Dy));
+        gv = kernel_eval(kind, k, ki, eta, Mx, My, pnormal(Dx), pnormal(Dy));
         res += xq * gv * plen(Dx) * plen(Dy) * rw[q];
     }
     return res;
@@ -547,19 +670,20 @@ static void rules_free(rules_t* r)
 /* One entry of DiscreteKernel with the rule selection of lib/bem2d_quadrature.h:271-287,
  * plus the mass-matrix entry of EvalIdent (lib/bem2d_quadrature.cpp:108-123). */
 static double complex entry(const geom_t* g, const rules_t* r, int64_t i, int64_t j, int kind,
-                            double k, double eta, double mass_scale, double kernel_scale)
+                            double k, double ki, double eta, double mass_scale,
+                            double kernel_scale)
 {
     double complex kv = 0, mv = 0;
     int ident = g->index[i] == g->index[j];
     if (kind != K_NONE && kernel_scale != 0.0) {
         if (ident)
-            kv = integrate_ident(g, i, j, kind, k, eta, r->x1, r->y1, r->w1, r->m1);
+            kv = integrate_ident(g, i, j, kind, k, ki, eta, r->x1, r->y1, r->w1, r->m1);
         else if (g->next[i] == g->index[j])
-            kv = integrate_pair(g, i, j, kind, k, eta, r->x3, r->y3, r->w3, r->m3);
+            kv = integrate_pair(g, i, j, kind, k, ki, eta, r->x3, r->y3, r->w3, r->m3);
         else if (g->prev[i] == g->index[j])
-            kv = integrate_pair(g, i, j, kind, k, eta, r->x2, r->y2, r->w2, r->m2);
+            kv = integrate_pair(g, i, j, kind, k, ki, eta, r->x2, r->y2, r->w2, r->m2);
         else
-            kv = integrate_pair(g, i, j, kind, k, eta, r->x0, r->y0, r->w0, r->m0);
+            kv = integrate_pair(g, i, j, kind, k, ki, eta, r->x0, r->y0, r->w0, r->m0);
     }
     if (ident && mass_scale != 0.0) {
         for (int t = 0; t < r->N; t++) {
@@ -576,10 +700,11 @@ static double complex entry(const geom_t* g, const rules_t* r, int64_t i, int64_
 /* DiscreteKernel (+ EvalIdent, + the facade's M + 2K) over the columns cols[0..ncols) (all columns
  * when cols == NULL).  out is column-major with leading dimension ld; column c of out holds matrix
  * column cols[c].  OpenMP over columns as lib/bem2d_quadrature.h:253. */
-int oracle_assemble(int64_t n, const int32_t* type, const double* param, const int64_t* index,
-                    const int64_t* next, const int64_t* prev, int N, int L, double sigma,
-                    int kind, double k, double eta, double mass_scale, double kernel_scale,
-                    const int64_t* cols, int64_t ncols, double* out, int64_t ld)
+int oracle_assemble_ck(int64_t n, const int32_t* type, const double* param, const int64_t* index,
+                       const int64_t* next, const int64_t* prev, int N, int L, double sigma,
+                       int kind, double k, double ki, double eta, double mass_scale,
+                       double kernel_scale, const int64_t* cols, int64_t ncols, double* out,
+                       int64_t ld)
 {
     geom_t g = {type, param, index, next, prev, n};
     rules_t r;
@@ -590,15 +715,34 @@ int oracle_assemble(int64_t n, const int32_t* type, const double* param, const i
     for (int64_t c = 0; c < ncols; c++) {
         int64_t j = cols ? cols[c] : c;
         for (int64_t i = 0; i < n; i++)
-            o[i + ld * c] = entry(&g, &r, i, j, kind, k, eta, mass_scale, kernel_scale);
+            o[i + ld * c] = entry(&g, &r, i, j, kind, k, ki, eta, mass_scale, kernel_scale);
     }
     rules_free(&r);
     return 0;
 }
 
-/* Field functors (lib/bem2d_fun.h:21-24, 45-49, 71-75). */
-static double complex field_eval(int kind, double k, double eta, pt dir, pt p, pt n)
+int oracle_assemble(int64_t n, const int32_t* type, const double* param, const int64_t* index,
+                    const int64_t* next, const int64_t* prev, int N, int L, double sigma,
+                    int kind, double k, double eta, double mass_scale, double kernel_scale,
+                    const int64_t* cols, int64_t ncols, double* out, int64_t ld)
 {
+    return oracle_assemble_ck(n, type, param, index, next, prev, N, L, sigma, kind, k, 0.0, eta,
+                              mass_scale, kernel_scale, cols, ncols, out, ld);
+}
+
+/* Field functors (lib/bem2d_fun.h:21-24, 45-49, 71-75), k = complex(k.re, k.im). */
+static double complex field_eval(int kind, double k, double ki, double eta, pt dir, pt p, pt n)
+{
+    if (ki != 0.0) {
+        double complex kc = k + I * ki;
+        double complex f = cexp(kc * I * (dir.x * p.x + dir.y * p.y));
+        switch (kind) {
+        case F_PLANE: return f;
+        case F_NORMAL_PLANE: return I * kc * f * (n.x * dir.x + n.y * dir.y);
+        case F_COMBINED_PLANE: return I * kc * f * (n.x * dir.x + n.y * dir.y) - I * eta * f;
+        }
+        return 0;
+    }
     double complex f = cexp(k * I * (dir.x * p.x + dir.y * p.y));
     switch (kind) {
     case F_PLANE: return f;
@@ -610,8 +754,9 @@ static double complex field_eval(int kind, double k, double eta, pt dir, pt p, p
 
 /* DotProdBasFuns (lib/bem2d_quadrature.h:111-156) for nrhs directions; out[i + ld*r].
  * scale multiplies the result (the facade's 2.0*rhs, lib/bem2d_soundsoftscattering.h:109). */
-int oracle_rhs(int64_t n, const int32_t* type, const double* param, int N, int kind, double k,
-               double eta, const double* dirs, int nrhs, double scale, double* out, int64_t ld)
+int oracle_rhs_ck(int64_t n, const int32_t* type, const double* param, int N, int kind, double k,
+                  double ki, double eta, const double* dirs, int nrhs, double scale, double* out,
+                  int64_t ld)
 {
     geom_t g = {type, param, NULL, NULL, NULL, n};
     double* gx = malloc(8 * N);
@@ -627,13 +772,19 @@ int oracle_rhs(int64_t n, const int32_t* type, const double* param, int N, int k
                 pt M, D;
                 elem_eval(&g, i, gx[t], &M, &D);
                 double s = plen(D);
-                res += field_eval(kind, k, eta, dir, M, pnormal(D)) * s * gw[t];
+                res += field_eval(kind, k, ki, eta, dir, M, pnormal(D)) * s * gw[t];
             }
             o[i + ld * r] = scale * res;
         }
     }
     free(gx); free(gw);
     return 0;
+}
+
+int oracle_rhs(int64_t n, const int32_t* type, const double* param, int N, int kind, double k,
+               double eta, const double* dirs, int nrhs, double scale, double* out, int64_t ld)
+{
+    return oracle_rhs_ck(n, type, param, N, kind, k, 0.0, eta, dirs, nrhs, scale, out, ld);
 }
 
 /* SolveSystem: LU with partial pivoting (pivot = first max of |re|+|im|, as izamax inside zgetrf),
@@ -688,8 +839,8 @@ int oracle_lu_solve(int64_t n, int64_t nrhs, const double* Ain, double* Bio)
 
 /* EvalKernel<SingleLayer> with Gauss1D(N): vals[p] = sum_j coef[j] sum_t (i/4)H0(k|P-y_t|) s_t w_t
  * (lib/bem2d_quadrature.h:184-228).  EvalSol returns u_inc - vals (soundsoftscattering.h:149-168). */
-int oracle_eval_sl(int64_t n, const int32_t* type, const double* param, int N, double k,
-                   const double* coef, const double* pts, int64_t npts, double* out)
+int oracle_eval_sl_ck(int64_t n, const int32_t* type, const double* param, int N, double k,
+                      double ki, const double* coef, const double* pts, int64_t npts, double* out)
 {
     geom_t g = {type, param, NULL, NULL, NULL, n};
     double* gx = malloc(8 * N);
@@ -706,13 +857,19 @@ int oracle_eval_sl(int64_t n, const int32_t* type, const double* param, int N, d
                 pt M, D;
                 elem_eval(&g, j, gx[t], &M, &D);
                 double s = plen(D);
-                double complex gv = kernel_eval(K_SL, k, 0, P, M, pnormal(D), pnormal(D));
+                double complex gv = kernel_eval(K_SL, k, ki, 0, P, M, pnormal(D), pnormal(D));
                 v += c[j] * gv * s * gw[t];
             }
         o[p] = v;
     }
     free(gx); free(gw);
     return 0;
+}
+
+int oracle_eval_sl(int64_t n, const int32_t* type, const double* param, int N, double k,
+                   const double* coef, const double* pts, int64_t npts, double* out)
+{
+    return oracle_eval_sl_ck(n, type, param, N, k, 0.0, coef, pts, npts, out);
 }
 
 /* Far field of u = u_inc - S phi:  u_inf(theta) = -(e^{i pi/4}/sqrt(8 pi k)) sum_j c_j

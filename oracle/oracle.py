@@ -41,6 +41,12 @@ def lib(variant="plain"):
         L.oracle_lu_solve.argtypes = [i64, i64, P, P]
         L.oracle_eval_sl.argtypes = [i64, P, P, i32, d, P, P, i64, P]
         L.oracle_farfield.argtypes = [i64, P, P, i32, d, P, i32, P, i64, P]
+        L.oracle_assemble_ck.argtypes = [i64, P, P, P, P, P, i32, i32, d, i32, d, d, d, d, d,
+                                         P, i64, P, i64]
+        L.oracle_rhs_ck.argtypes = [i64, P, P, i32, i32, d, d, d, P, i32, d, P, i64]
+        L.oracle_eval_sl_ck.argtypes = [i64, P, P, i32, d, d, P, P, i64, P]
+        L.oracle_hankel_complex_many.argtypes = [P, i64, P]
+        L.oracle_hankel_complex_many.restype = None
         L.oracle_elem_eval.argtypes = [i64, P, P, i64, d, P]
         L.oracle_elem_eval.restype = None
         _LIBS[variant] = L
@@ -76,9 +82,24 @@ def hankel(x):
     return out[:, 0], out[:, 1], out[:, 2], out[:, 3]
 
 
+def hankel_complex(z):
+    """Returns (H0, H1) arrays (complex128) of H0^(1), H1^(1) at complex z."""
+    z = np.ascontiguousarray(z, dtype=np.complex128)
+    zz = np.ascontiguousarray(np.stack([z.real, z.imag], axis=-1).reshape(-1))
+    out = np.zeros((z.size, 4))
+    lib().oracle_hankel_complex_many(_p(zz), z.size, _p(out))
+    return out[:, 0] + 1j * out[:, 1], out[:, 2] + 1j * out[:, 3]
+
+
+def _kparts(k):
+    """(Re k, Im k) of a real or complex wavenumber (freqtype {re, im})."""
+    k = complex(k)
+    return k.real, k.imag
+
+
 def assemble(geom, N, L, sigma, kind, k, eta, mass_scale=0.0, kernel_scale=1.0, cols=None,
              variant="plain"):
-    """Column-major N x ncols complex128 block of mass_scale*M + kernel_scale*K."""
+    """Column-major N x ncols complex128 block of mass_scale*M + kernel_scale*K; k may be complex."""
     n = geom.n
     if cols is None:
         ncols = n
@@ -88,9 +109,10 @@ def assemble(geom, N, L, sigma, kind, k, eta, mass_scale=0.0, kernel_scale=1.0, 
         ncols = cols.size
         cptr = _p(cols)
     out = np.zeros((ncols, n), dtype=np.complex128)  # row c = column cols[c]
-    rc = lib(variant).oracle_assemble(n, _p(geom.type), _p(geom.param), _p(geom.index), _p(geom.next),
-                               _p(geom.prev), N, L, sigma, kind, k, eta, mass_scale, kernel_scale,
-                               cptr, ncols, _p(out), n)
+    kr, ki = _kparts(k)
+    rc = lib(variant).oracle_assemble_ck(n, _p(geom.type), _p(geom.param), _p(geom.index),
+                                         _p(geom.next), _p(geom.prev), N, L, sigma, kind, kr, ki,
+                                         eta, mass_scale, kernel_scale, cptr, ncols, _p(out), n)
     if rc:
         raise ValueError("oracle_assemble: bad quadrature option")
     return out.T  # (n, ncols) view, column-major storage
@@ -100,8 +122,9 @@ def rhs(geom, N, kind, k, eta, dirs, scale=1.0):
     dirs = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1, 2)
     nrhs = dirs.shape[0]
     out = np.zeros((nrhs, geom.n), dtype=np.complex128)
-    lib().oracle_rhs(geom.n, _p(geom.type), _p(geom.param), N, kind, k, eta, _p(dirs), nrhs,
-                     scale, _p(out), geom.n)
+    kr, ki = _kparts(k)
+    lib().oracle_rhs_ck(geom.n, _p(geom.type), _p(geom.param), N, kind, kr, ki, eta, _p(dirs),
+                        nrhs, scale, _p(out), geom.n)
     return out.T
 
 
@@ -122,8 +145,9 @@ def eval_sl(geom, N, k, coef, pts):
     coef = np.ascontiguousarray(coef, dtype=np.complex128)
     pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 2)
     out = np.zeros(pts.shape[0], dtype=np.complex128)
-    lib().oracle_eval_sl(geom.n, _p(geom.type), _p(geom.param), N, k, _p(coef), _p(pts),
-                         pts.shape[0], _p(out))
+    kr, ki = _kparts(k)
+    lib().oracle_eval_sl_ck(geom.n, _p(geom.type), _p(geom.param), N, kr, ki, _p(coef), _p(pts),
+                            pts.shape[0], _p(out))
     return out
 
 

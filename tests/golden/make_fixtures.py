@@ -10,6 +10,14 @@ hankel_mpmath.npz   J0, Y0, J1, Y1 at 50 significant digits, rounded to float64,
                     J0/Y0/J1/Y1.
 gauss_mpmath.npz    Gauss-Legendre nodes and weights on [0, 1] by Newton's method on P_n at 40
                     digits.
+hankel_complex_mpmath.npz
+                    H0^(1), H1^(1) of complex z, |arg z| <= 0.95 pi/2, 1e-6 <= |z| <= 2e3,
+                    -650 < Im z < 120 (plus clusters at the device code's switch radii 2 and 20),
+                    30 significant digits:
+                    mpmath's J + iY loses ~2 Im z / ln 10 digits to cancellation, so the working
+                    precision is raised by that much.
+entries_complex_mpmath.npz
+                    as entries_mpmath.npz for complex wavenumbers (the kernels' k.im != 0 branch).
 entries_mpmath.npz  Galerkin matrices of tiny meshes evaluated entirely in mpmath (30 digits):
                     the reference's quadrature rules are rebuilt from the 40-digit Gauss nodes,
                     geometry, kernels and Hankel functions are evaluated in mpmath.  These pin the
@@ -148,7 +156,7 @@ def normal(D):
 
 def matrix_mp(types, params, index, nxt, prv, N, L, sigma, kind, k, eta):
     n = len(types)
-    k = mp.mpf(k)
+    k = mp.mpc(k.real, k.imag) if isinstance(k, complex) else mp.mpf(k)
     eta = mp.mpf(eta)
     rules = [rule_mp(w, N, L, sigma) for w in range(4)]
     A = np.zeros((n, n), dtype=np.complex128)
@@ -258,6 +266,53 @@ def make_entries():
     np.savez_compressed(os.path.join(HERE, "entries_mpmath.npz"), **out)
 
 
+CASES_COMPLEX = [
+    # name, mesh builder, (N, L, sigma), kinds, k (complex), eta
+    ("circle4_ck", lambda: curve_mesh(1, 4), (4, 2, 0.15), (1, 2, 3, 4), 6.0 - 1.5j, 6.0),
+    ("triangle6_ck", lambda: polygon_mesh([(0, 0), (1.2, 0.1), (0.3, 0.9)], 2), (4, 2, 0.15),
+     (1, 4), 4.0 + 2.0j, 4.0),
+    ("kite5_ck", lambda: curve_mesh(3, 5), (3, 2, 0.2), (4,), 7.5 - 0.8j, 2.0),
+]
+
+
+def make_entries_complex():
+    out = {}
+    mp.mp.dps = 30
+    for name, build, (N, L, sigma), kinds, k, eta in CASES_COMPLEX:
+        types, params, idx, nx, pv = build()
+        out[name + "_type"] = types
+        out[name + "_param"] = params
+        out[name + "_index"] = idx
+        out[name + "_next"] = nx
+        out[name + "_prev"] = pv
+        out[name + "_quad"] = np.array([N, L, sigma])
+        out[name + "_kc"] = np.array([k.real, k.imag, eta])
+        for kind in kinds:
+            out["%s_kind%d" % (name, kind)] = matrix_mp(types, params, idx, nx, pv, N, L, sigma,
+                                                        kind, k, eta)
+            print(name, kind, file=sys.stderr, flush=True)
+    np.savez_compressed(os.path.join(HERE, "entries_complex_mpmath.npz"), **out)
+
+
+def make_hankel_complex():
+    zs = []
+    for ph in np.linspace(-0.95 * np.pi / 2, 0.95 * np.pi / 2, 39):
+        r = list(np.logspace(-6, np.log10(2e3), 60))
+        for c in (2.0, 20.0):
+            r += list(c * (1 + np.linspace(-1e-3, 1e-3, 5)))
+        zs += [x * np.exp(1j * ph) for x in r]
+    # H^(1) is recessive above the real axis (~e^-Im z); the 30-digit values there need
+    # 2 Im z / ln 10 extra digits, so the fixture stops at Im z = 120 (|H| ~ 1e-53)
+    zs = np.array([z for z in zs if -650 < z.imag < 120])
+    vals = np.zeros((zs.size, 4))
+    for i, z in enumerate(zs):
+        Z = mp.mpc(z.real, z.imag)
+        with mp.workdps(30 + int(np.ceil(2 * max(z.imag, 0) / np.log(10)))):
+            h0, h1 = mp.hankel1(0, Z), mp.hankel1(1, Z)
+        vals[i] = [float(h0.real), float(h0.imag), float(h1.real), float(h1.imag)]
+    np.savez_compressed(os.path.join(HERE, "hankel_complex_mpmath.npz"), z=zs, h0h1=vals)
+
+
 def make_hankel():
     mp.mp.dps = 50
     xs = list(np.logspace(-8, np.log10(5e4), 2500))
@@ -287,10 +342,14 @@ def make_gauss():
 
 
 if __name__ == "__main__":
-    what = sys.argv[1:] or ["gauss", "hankel", "entries"]
+    what = sys.argv[1:] or ["gauss", "hankel", "entries", "hankel_complex", "entries_complex"]
     if "gauss" in what:
         make_gauss()
     if "hankel" in what:
         make_hankel()
     if "entries" in what:
         make_entries()
+    if "hankel_complex" in what:
+        make_hankel_complex()
+    if "entries_complex" in what:
+        make_entries_complex()
