@@ -63,6 +63,12 @@ SIGNATURES = {
     "bem2d_lu32_create": (_I, [_P, _I64, ctypes.POINTER(_P)]),
     "bem2d_lu32_destroy": (None, [_P]),
     "bem2d_lu32_size": (_I64, [_P]),
+    "bem2d_lu32_graph_replays": (_I64, [_P]),
+    "bem2d_tune_set_assembly": (_I, [_P, _I, _P, _P]),
+    "bem2d_tune_get_assembly": (_I, [_P, _P, _P]),
+    "bem2d_tune_set_lu_pair_tiles": (_I, [_P, _I]),
+    "bem2d_tune_get_lu_pair_tiles": (_I, [_P]),
+    "bem2d_tune_load": (_I, [_P, ctypes.c_char_p]),
     "bem2d_lu32_factor": (_I, [_P, _P, _P, _I64]),
     "bem2d_lu32_refine": (_I, [_P, _P, _P, _I64, _P, _I64, _P, _I64, _I, _D, _I,
                                ctypes.POINTER(IRReport)]),
@@ -231,6 +237,26 @@ class Context:
         check(lib().bem2d_profile_read(self.h, kernel_id, ctypes.byref(ms), ctypes.byref(cnt)))
         return ms.value, cnt.value
 
+    # ---- launch-shape table ----------------------------------------------------------------
+    def tune_assembly(self, buckets=None):
+        """Gets (and with buckets = [(upper_tiles, split), ...] sets) the assembly table."""
+        if buckets is not None:
+            up = np.array([b[0] for b in buckets], dtype=np.int64)
+            sp = np.array([b[1] for b in buckets], dtype=np.int32)
+            check(lib().bem2d_tune_set_assembly(self.h, len(buckets), _np_ptr(up), _np_ptr(sp)))
+        up = np.zeros(8, dtype=np.int64)
+        sp = np.zeros(8, dtype=np.int32)
+        nb = lib().bem2d_tune_get_assembly(self.h, _np_ptr(up), _np_ptr(sp))
+        return [(int(up[b]), int(sp[b])) for b in range(nb)]
+
+    def tune_lu_pair_tiles(self, tiles=None):
+        if tiles is not None:
+            check(lib().bem2d_tune_set_lu_pair_tiles(self.h, int(tiles)))
+        return lib().bem2d_tune_get_lu_pair_tiles(self.h)
+
+    def tune_load(self, path):
+        check(lib().bem2d_tune_load(self.h, path.encode()))
+
     # ---- operations ------------------------------------------------------------------------
     def mesh(self, geom, N=16, L=3, sigma=0.15):
         return Mesh(self, geom, N, L, sigma)
@@ -342,6 +368,10 @@ class LU32:
 
     def factor(self, A):
         check(lib().bem2d_lu32_factor(self.ctx.h, self.h, _t_ptr(A), _ld(A)))
+
+    def graph_replays(self):
+        """Factorisations served by replaying the captured hipGraph."""
+        return lib().bem2d_lu32_graph_replays(self.h)
 
     def refine(self, A, B, X, tol=1e-12, maxit=30, raise_on_fail=True):
         rep = IRReport()

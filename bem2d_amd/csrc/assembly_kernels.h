@@ -287,8 +287,6 @@ __device__ __forceinline__ void kernel_pair(const PairConst& pc, double xx, doub
 // then cover whole 128-byte segments (8 consecutive rows of a column).
 constexpr int kTileSide = 8;
 __host__ __device__ __forceinline__ int node_stride(int nq) { return nq * NODE_PLANES + 1; }
-// launches of fewer tiles than this split every tile over four wavefronts (assemble_regular_kernel)
-constexpr int64_t kSplitTiles = 16384;
 
 // Tiles of the upper triangle of the P x P grid of 8 x 8 blocks: row block p holds the P - p tiles
 // q = p .. P - 1, so tile t lies in the largest p with tiles_before(p) = p P - p (p - 1) / 2 <= t.

@@ -4,10 +4,13 @@
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
+#include <dlfcn.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <unordered_map>
@@ -67,8 +70,104 @@ extern "C" int bem2d_ctx_create(bem2d_ctx** out, int device)
         return FAIL(BEM2D_ERR_DEVICE, "stream / event / rocBLAS handle creation failed");
     }
     rocblas_set_stream(c->blas, c->stream);
+    // the autotuner's table: $BEM2D_TUNE_FILE, else tuned/<arch>.tune beside this library
+    const char* tf = std::getenv("BEM2D_TUNE_FILE");
+    if (tf && tf[0]) {
+        if (bem2d_tune_load(c, tf) != BEM2D_OK) {
+            bem2d_ctx_destroy(c);
+            return FAIL(BEM2D_ERR_ARG, "BEM2D_TUNE_FILE %s: cannot use it", tf);
+        }
+    } else {
+        Dl_info info;
+        if (dladdr((void*)&bem2d_ctx_create, &info) && info.dli_fname) {
+            std::string path(info.dli_fname);
+            path = path.substr(0, path.find_last_of('/') + 1) + "tuned/gfx950.tune";
+            if (FILE* f = std::fopen(path.c_str(), "r")) {
+                std::fclose(f);
+                if (bem2d_tune_load(c, path.c_str()) != BEM2D_OK) {
+                    bem2d_ctx_destroy(c);
+                    return FAIL(BEM2D_ERR_ARG, "tune table %s: cannot use it", path.c_str());
+                }
+            }
+        }
+    }
     *out = c;
     return BEM2D_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
+// launch-shape table (written by the autotuner, bem2d_amd/autotune.py)
+// ------------------------------------------------------------------------------------------------
+extern "C" int bem2d_tune_set_assembly(bem2d_ctx* c, int nbuckets, const int64_t* upper_tiles,
+                                       const int* split)
+{
+    if (!c || nbuckets < 0 || nbuckets > kMaxTuneBuckets || (nbuckets > 0 && (!upper_tiles || !split)))
+        return FAIL(BEM2D_ERR_ARG, "bem2d_tune_set_assembly: bad argument");
+    for (int b = 0; b < nbuckets; ++b) {
+        if (split[b] != 1 && split[b] != 2 && split[b] != 4)
+            return FAIL(BEM2D_ERR_ARG, "bem2d_tune_set_assembly: split %d not in {1, 2, 4}", split[b]);
+        if (upper_tiles[b] <= 0 || (b > 0 && upper_tiles[b] <= upper_tiles[b - 1]))
+            return FAIL(BEM2D_ERR_ARG, "bem2d_tune_set_assembly: bucket bounds must increase");
+    }
+    c->tune.nasm = nbuckets;
+    for (int b = 0; b < nbuckets; ++b) {
+        c->tune.asm_upper[b] = upper_tiles[b];
+        c->tune.asm_split[b] = split[b];
+    }
+    return BEM2D_OK;
+}
+
+extern "C" int bem2d_tune_get_assembly(const bem2d_ctx* c, int64_t* upper_tiles, int* split)
+{
+    if (!c) return -1;
+    for (int b = 0; b < c->tune.nasm; ++b) {
+        if (upper_tiles) upper_tiles[b] = c->tune.asm_upper[b];
+        if (split) split[b] = c->tune.asm_split[b];
+    }
+    return c->tune.nasm;
+}
+
+extern "C" int bem2d_tune_set_lu_pair_tiles(bem2d_ctx* c, int tiles)
+{
+    if (!c || tiles < 1) return FAIL(BEM2D_ERR_ARG, "bem2d_tune_set_lu_pair_tiles: bad argument");
+    c->tune.lu_pair_tiles = tiles;
+    return BEM2D_OK;
+}
+
+extern "C" int bem2d_tune_get_lu_pair_tiles(const bem2d_ctx* c) { return c ? c->tune.lu_pair_tiles : -1; }
+
+// Text table, one setting per line ('#' starts a comment):
+//   asm_split <upper_tiles> <wavefronts per tile>     (buckets in increasing order)
+//   lu_pair_tiles <tiles>
+extern "C" int bem2d_tune_load(bem2d_ctx* c, const char* path)
+{
+    if (!c || !path) return FAIL(BEM2D_ERR_ARG, "bem2d_tune_load: null argument");
+    FILE* f = std::fopen(path, "r");
+    if (!f) return FAIL(BEM2D_ERR_ARG, "bem2d_tune_load: cannot open %s", path);
+    int64_t upper[kMaxTuneBuckets];
+    int split[kMaxTuneBuckets];
+    int nb = 0, pair = c->tune.lu_pair_tiles, rc = BEM2D_OK;
+    char line[256];
+    while (rc == BEM2D_OK && std::fgets(line, sizeof line, f)) {
+        if (char* h = std::strchr(line, '#')) *h = 0;
+        char key[64];
+        long long a = 0, b = 0;
+        const int got = std::sscanf(line, "%63s %lld %lld", key, &a, &b);
+        if (got <= 0) continue;
+        if (!std::strcmp(key, "asm_split") && got == 3 && nb < kMaxTuneBuckets) {
+            upper[nb] = a;
+            split[nb++] = (int)b;
+        } else if (!std::strcmp(key, "lu_pair_tiles") && got == 2) {
+            pair = (int)a;
+        } else {
+            rc = FAIL(BEM2D_ERR_ARG, "bem2d_tune_load: bad line in %s: %s", path, line);
+        }
+    }
+    std::fclose(f);
+    if (rc) return rc;
+    if (pair < 1) return FAIL(BEM2D_ERR_ARG, "bem2d_tune_load: lu_pair_tiles %d < 1 in %s", pair, path);
+    if ((rc = bem2d_tune_set_assembly(c, nb, upper, split))) return rc;
+    return bem2d_tune_set_lu_pair_tiles(c, pair);
 }
 
 extern "C" void bem2d_ctx_destroy(bem2d_ctx* c)
@@ -80,6 +179,7 @@ extern "C" void bem2d_ctx_destroy(bem2d_ctx* c)
         for (hipEvent_t e : p.pending) (void)hipEventDestroy(e);
     for (auto& s : c->scratch)
         if (s.ptr) (void)hipFree(s.ptr);
+    if (c->capture) (void)hipStreamDestroy(c->capture);
     if (c->stage) (void)hipHostFree(c->stage);
     if (c->ev_stage) (void)hipEventDestroy(c->ev_stage);
     bem2d_internal_mixed_release(c);
@@ -634,13 +734,22 @@ static int launch_assembly(bem2d_ctx* c, hipStream_t s, const AsmProblem* probs,
             // few tiles (small problems): four wavefronts per tile, each over a quarter of the
             // test nodes, so the launch fills the device and every lane's chain is shorter
             const unsigned grid = (unsigned)reg.start[cnt];
-            const bool split = reg.start[cnt] < kSplitTiles;
-            if (nq == 16 && split)
+            int split = 1; // the size bucket's choice (autotuner table)
+            for (int b = 0; b < c->tune.nasm; ++b)
+                if (reg.start[cnt] < c->tune.asm_upper[b]) {
+                    split = c->tune.asm_split[b];
+                    break;
+                }
+            if (nq == 16 && split == 4)
                 assemble_regular_kernel<KIND, 16, 4><<<grid, 256, shm, s>>>(reg);
+            else if (nq == 16 && split == 2)
+                assemble_regular_kernel<KIND, 16, 2><<<grid, 128, shm, s>>>(reg);
             else if (nq == 16)
                 assemble_regular_kernel<KIND, 16, 1><<<grid, 64, shm, s>>>(reg);
-            else if (split)
+            else if (split == 4)
                 assemble_regular_kernel<KIND, 0, 4><<<grid, 256, shm, s>>>(reg);
+            else if (split == 2)
+                assemble_regular_kernel<KIND, 0, 2><<<grid, 128, shm, s>>>(reg);
             else
                 assemble_regular_kernel<KIND, 0, 1><<<grid, 64, shm, s>>>(reg);
             bem2d_internal_prof_end(c, BEM2D_PROF_ASSEMBLE_REGULAR, s);

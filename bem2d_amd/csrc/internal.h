@@ -27,6 +27,18 @@ struct ScratchBuf {
 
 struct MixedState; // mixed-precision solver workspace (mixed.hip)
 
+// Launch-shape choices per size bucket, installed by the autotuner's table (bem2d_tune_*).
+constexpr int kMaxTuneBuckets = 8;
+struct TuneState {
+    // regular assembly: a launch of t tiles runs asm_split[b] wavefronts per tile for the first
+    // bucket b with t < asm_upper[b], one wavefront per tile beyond the last bucket
+    int nasm = 1;
+    int64_t asm_upper[kMaxTuneBuckets] = {16384};
+    int asm_split[kMaxTuneBuckets] = {4};
+    // FP32 LU: panels pair into depth-128 updates while the trailing matrix has >= this many tiles
+    int lu_pair_tiles = 80;
+};
+
 struct ProfSlot {
     std::vector<hipEvent_t> pending; // start/stop pairs not yet read
     double total_ms = 0.0;
@@ -38,6 +50,7 @@ struct bem2d_ctx {
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
     hipStream_t aux = nullptr;           // second stream: singular-pair assembly beside the regular
+    hipStream_t capture = nullptr;       // stream that hipGraph captures record on (never runs work)
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     rocblas_handle blas = nullptr;
     ScratchBuf scratch[SCRATCH_COUNT];
@@ -47,6 +60,7 @@ struct bem2d_ctx {
     MixedState* mixed = nullptr;
     bool profiling = false;
     ProfSlot prof[BEM2D_PROF_COUNT];
+    TuneState tune;
 };
 
 // Records a start event for kernel id when profiling is on; returns the event index (or -1).

@@ -27,6 +27,7 @@
 #include <algorithm>
 #include <type_traits>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -43,7 +44,7 @@ constexpr long long kSpinLimit = 1ll << 26;
 // least this many 64-wide tiles: the saved pass over it (16 B per entry) then outweighs the
 // diagonal factorisation that the narrow update of the pair leaves exposed (~60 us).  Measured on
 // MI355X: N = 4000 (at most 62 tiles) is faster unpaired, N = 32000 pairs (1.94 -> 1.53 s).
-constexpr int kPairTiles = 80;
+constexpr int kPairTiles = 80; // default of TuneState::lu_pair_tiles
 
 typedef float v4f __attribute__((ext_vector_type(4)));
 
@@ -1423,6 +1424,15 @@ struct bem2d_lu32 {
     double* anorm = nullptr;      // |A|_inf
     double* rowpart = nullptr;    // (np / kCastCols) x n partial row sums of |a_ij|
     RefCache* rcache = nullptr;   // refinement workspace of bem2d_lu32_refine
+    // hipGraph of the whole factorisation launch sequence for one matrix buffer (A, lda): built on
+    // the second factorisation of the same buffer, replayed from then on
+    const void* seen_A = nullptr;
+    int64_t seen_lda = 0;
+    hipGraphExec_t graph = nullptr;
+    const void* graph_A = nullptr;
+    int64_t graph_lda = 0;
+    int graph_pair_tiles = 0;     // the panel-pairing choice the graph was captured with
+    int64_t graph_replays = 0;
 };
 
 size_t bem2d_internal_lu32_bytes(int64_t n)
@@ -1443,6 +1453,7 @@ extern "C" void bem2d_lu32_destroy(bem2d_lu32* f)
     bem2d_internal_dfree(f->arena, f->dflag);
     bem2d_internal_dfree(f->arena, f->anorm);
     bem2d_internal_dfree(f->arena, f->rowpart);
+    if (f->graph) (void)hipGraphExecDestroy(f->graph);
     if (f->rcache) {
         (void)hipFree(f->rcache->base);
         (void)hipHostFree(f->rcache->pinned);
@@ -1572,7 +1583,7 @@ int bem2d_internal_lu_advance(hipStream_t s, bem2d_lu32* const* lus, int count)
                 u.kb = 2 * NB;
                 f->pending_wide = false;
                 blocks = (int64_t)u.tiles * u.tiles;
-            } else if (b + 2 <= f->nblk - 1 && u.tiles >= kPairTiles) {
+            } else if (b + 2 <= f->nblk - 1 && u.tiles >= f->ctx->tune.lu_pair_tiles) {
                 u.mode = LU_UPDATE_CROSS;
                 f->pending_wide = true;
                 blocks = 2 * (int64_t)u.tiles - 1;
@@ -1602,17 +1613,82 @@ int bem2d_internal_lu_advance(hipStream_t s, bem2d_lu32* const* lus, int count)
 bool bem2d_internal_lu_done(const bem2d_lu32* f) { return f->factored; }
 int64_t bem2d_internal_lu_n(const bem2d_lu32* f) { return f->n; }
 
+static bool lu_graphs_enabled()
+{
+    static const bool on = [] {
+        const char* e = std::getenv("BEM2D_LU_GRAPH");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
+// Enqueues the factorisation's launch sequence (cast and |A|_inf, first diagonal block, then two
+// kernels per panel step) on s; every launch is decided on the host from n alone.
+static int lu_factor_launches(hipStream_t s, bem2d_lu32* f, const void* A64, int64_t lda)
+{
+    int rc = bem2d_internal_lu_begin(s, &f, &A64, &lda, 1);
+    while (!rc && !f->factored) rc = bem2d_internal_lu_advance(s, &f, 1);
+    return rc;
+}
+
+// A factorisation is ~2 launches per 64 columns (124 at n = 4000) with nothing decided on the
+// device, so a repeated factorisation of the same buffer -- new contents, same address and lda, as
+// in a frequency sweep or a benchmark loop -- replays one hipGraph of the sequence instead of
+// launching it kernel by kernel.  The graph is captured on the second factorisation of a buffer
+// (a one-shot solve never pays for the capture) and holds kernel nodes only.  The capture records
+// on a private stream of the context (the context's own stream may be the null stream, which
+// cannot capture); the graph then runs on the context's stream.  Set BEM2D_LU_GRAPH=0 to launch
+// eagerly always.
 extern "C" int bem2d_lu32_factor(bem2d_ctx* c, bem2d_lu32* f, const void* A64, int64_t lda)
 {
     if (!c || !f || !A64 || lda < f->n) return FAIL(BEM2D_ERR_ARG, "bem2d_lu32_factor: bad argument");
     HIPCHK(hipSetDevice(c->device));
     hipStream_t s = c->stream;
+    const bool graphs = lu_graphs_enabled();
+    const bool current = f->graph && f->graph_A == A64 && f->graph_lda == lda &&
+                         f->graph_pair_tiles == c->tune.lu_pair_tiles;
+    if (graphs && !current && f->seen_A == A64 && f->seen_lda == lda) {
+        if (!c->capture) HIPCHK(hipStreamCreateWithFlags(&c->capture, hipStreamNonBlocking));
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(c->capture, hipStreamCaptureModeThreadLocal));
+        int rc = lu_factor_launches(c->capture, f, A64, lda);
+        const hipError_t e = hipStreamEndCapture(c->capture, &g);
+        if (rc || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc ? rc : FAIL(BEM2D_ERR_DEVICE, "LU graph capture failed: %s", hipGetErrorString(e));
+        }
+        if (f->graph) (void)hipGraphExecDestroy(f->graph);
+        f->graph = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&f->graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            f->graph = nullptr;
+            return FAIL(BEM2D_ERR_DEVICE, "LU graph instantiation failed: %s", hipGetErrorString(ei));
+        }
+        f->graph_A = A64;
+        f->graph_lda = lda;
+        f->graph_pair_tiles = c->tune.lu_pair_tiles;
+    }
     bem2d_internal_prof_begin(c, BEM2D_PROF_LU);
-    int rc = bem2d_internal_lu_begin(s, &f, &A64, &lda, 1);
-    while (!rc && !f->factored) rc = bem2d_internal_lu_advance(s, &f, 1);
+    int rc = BEM2D_OK;
+    if (graphs && f->graph && f->graph_A == A64 && f->graph_lda == lda &&
+        f->graph_pair_tiles == c->tune.lu_pair_tiles) {
+        if (hipGraphLaunch(f->graph, s) != hipSuccess) rc = FAIL(BEM2D_ERR_DEVICE, "LU graph launch failed");
+        // the host-side state the launch sequence leaves behind
+        f->step = f->nblk - 1;
+        f->pending_wide = false;
+        f->factored = true;
+        f->graph_replays++;
+    } else {
+        rc = lu_factor_launches(s, f, A64, lda);
+        f->seen_A = A64;
+        f->seen_lda = lda;
+    }
     bem2d_internal_prof_end(c, BEM2D_PROF_LU);
     return rc;
 }
+
+extern "C" int64_t bem2d_lu32_graph_replays(const bem2d_lu32* f) { return f ? f->graph_replays : -1; }
 
 // ---- refinement jobs --------------------------------------------------------------------------
 // A job refines x = A^{-1} b for several problems at once: every problem's right-hand sides are

@@ -53,6 +53,17 @@ def measured_traffic():
     return profile_value(TRAFFIC_PROFILE, "traffic_bytes_per_launch")
 
 
+def regular_split(ctx, n):
+    """Wavefronts per tile of the regular assembly of an n-element mesh under the context's
+    launch-shape table (the first bucket whose bound exceeds the tile count; 1 beyond them)."""
+    p = (n + 7) // 8
+    tiles = p * (p + 1) // 2
+    for upper, split in ctx.tune_assembly():
+        if tiles < upper:
+            return split
+    return 1
+
+
 def pair_points(n, nspecial, nq=16, L=3):
     """Kernel evaluations of one assembly: 256 per regular entry, 2*N^2 L^2 per coincident entry,
     3 N^2 L per adjacent entry."""
@@ -156,7 +167,8 @@ def run_kite_served(ctx, copies=8, n=4000, k=100.0, seed=3):
 
 
 def run_serving(ctx, count, rate, seed=2):
-    """C4: Poisson arrivals at `rate`; latency = completion - arrival; solves = (request, rhs)."""
+    """C4: Poisson arrivals at `rate`; latency = completion - arrival; solves = (request, rhs).
+    Then each path's capacity: the same requests all present at once."""
     import time
     from bem2d_amd import Engine, capi, serving
     out = {"workload": "%d requests (seed %d), Poisson arrivals at %.1f/s: ellipse / kite / polygon, "
@@ -178,6 +190,21 @@ def run_serving(ctx, count, rate, seed=2):
                      "failed": sum(i.status != 0 for i in infos),
                      "max_backward_error": max(i.backward_error for i in infos)}
     out["mixed_over_plain_solves"] = out["mixed"]["solves_per_s"] / out["plain_fp64"]["solves_per_s"]
+    # capacity: the same requests all present at once, so each path runs as fast as it can
+    for prec, name in ((capi.PRECISION_MIXED, "mixed"), (capi.PRECISION_FP64, "plain_fp64")):
+        reqs, arr = serving.serving_trace(count, rate=rate, seed=seed, precision=prec)
+        eng = Engine(ctx, 160 << 30, max_active=64)
+        t0 = time.perf_counter()
+        tickets = eng.run_trace(reqs, np.zeros_like(arr))
+        dt = time.perf_counter() - t0
+        res = [eng.result(t, raise_on_fail=False) for t in tickets]
+        eng.close()
+        nrhs = sum(q["dirs"].shape[0] for q in reqs)
+        out[name]["capacity_solves_per_s"] = nrhs / dt
+        out[name]["capacity_requests_per_s"] = count / dt
+        out[name]["capacity_failed"] = sum(r[2].status != 0 for r in res)
+    out["mixed_over_plain_capacity"] = (out["mixed"]["capacity_solves_per_s"] /
+                                        out["plain_fp64"]["capacity_solves_per_s"])
     return out
 
 
@@ -431,7 +458,7 @@ def main():
         "pair_points": P,
         "backward_error_fp64": bw,
         "roofline": {
-            "kernel": "assemble_regular_kernel<COMBINED,16,1>",
+            "kernel": "assemble_regular_kernel<COMBINED,16,%d>" % regular_split(ctx, args.n),
             "bound": "mfma",
             "engine": "FP64 vector ALU: compute-bound (about 2000 FLOP per stored byte); its "
                       "peak, 78.6 TFLOP/s, is also MI355X's dense FP64 matrix peak",

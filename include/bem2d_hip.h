@@ -102,6 +102,24 @@ int bem2d_device_free(bem2d_ctx* ctx, void* p);
 int bem2d_copy_to_device(bem2d_ctx* ctx, void* dst, const void* src, int64_t bytes);
 int bem2d_copy_to_host(bem2d_ctx* ctx, void* dst, const void* src, int64_t bytes);
 
+/* ---- launch shapes per size bucket (no counterpart in the reference) ----------------------------
+ * The context's choices of launch shape, measured by the autotuner (bem2d_amd/autotune.py) and
+ * written as a text table; bem2d_ctx_create loads $BEM2D_TUNE_FILE, else tuned/gfx950.tune beside
+ * the library when present, else keeps the built-in defaults.
+ * Assembly: a regular-pair launch of t tiles (8 x 8 elements each) runs split[b] wavefronts per
+ * tile (1, 2 or 4) for the first bucket b with t < upper_tiles[b], one beyond the last bucket; at
+ * most 8 buckets, upper_tiles increasing.  bem2d_tune_get_assembly returns the bucket count and
+ * fills the arrays (8 entries each) when given.
+ * FP32 LU: panels pair into depth-128 trailing updates while the trailing matrix has at least
+ * lu_pair_tiles 64 x 64 tiles.
+ * Table lines: "asm_split <upper_tiles> <split>" and "lu_pair_tiles <tiles>"; '#' comments. */
+int bem2d_tune_set_assembly(bem2d_ctx* ctx, int nbuckets, const int64_t* upper_tiles,
+                            const int* split);
+int bem2d_tune_get_assembly(const bem2d_ctx* ctx, int64_t* upper_tiles, int* split);
+int bem2d_tune_set_lu_pair_tiles(bem2d_ctx* ctx, int tiles);
+int bem2d_tune_get_lu_pair_tiles(const bem2d_ctx* ctx);
+int bem2d_tune_load(bem2d_ctx* ctx, const char* path);
+
 /* ---- mesh: device copy of the geometry, the Gauss-node tables and the singular rules ---------- */
 /* Replaces Geometry::FlatMap + the rule construction at the top of DiscreteKernel
  * (lib/bem2d_quadrature.h:235-242): Gauss2D(N), AdaptedGauss1/2/3(N, L, sigma). */
@@ -176,8 +194,11 @@ typedef struct {
 int bem2d_lu32_create(bem2d_ctx* ctx, int64_t n, bem2d_lu32** out);
 void bem2d_lu32_destroy(bem2d_lu32* lu);
 int64_t bem2d_lu32_size(const bem2d_lu32* lu);
-/* FP32 factorisation of the FP64 device matrix A64 (unchanged). */
+/* FP32 factorisation of the FP64 device matrix A64 (unchanged).  Factorising the same buffer
+ * (address and lda) again replays a hipGraph of the launch sequence, captured on the second call
+ * (BEM2D_LU_GRAPH=0 in the environment: never); bem2d_lu32_graph_replays counts the replays. */
 int bem2d_lu32_factor(bem2d_ctx* ctx, bem2d_lu32* lu, const void* A64, int64_t lda);
+int64_t bem2d_lu32_graph_replays(const bem2d_lu32* lu);
 /* X (n x nrhs, device) = A64^{-1} B by refinement from x0 = 0 with the FP32 factors and FP64
  * residuals, until the backward error is <= tol or maxit steps; BEM2D_ERR_NOT_CONVERGED otherwise. */
 int bem2d_lu32_refine(bem2d_ctx* ctx, bem2d_lu32* lu, const void* A64, int64_t lda, const void* B,

@@ -135,3 +135,35 @@ def test_paired_panels_large_trailing_matrix(ctx):
     mesh.close()
     assert rep.fallback_fp64 == 0 and rep.backward_error <= 1e-12
     assert (torch.linalg.norm(Xm - X64) / torch.linalg.norm(X64)).item() < 1e-10
+
+
+@pytest.mark.parametrize("n", [700, 5440])
+def test_factor_graph_replays_match_eager(ctx, n):
+    """Repeated factorisations of one buffer replay a captured hipGraph (from the second call on);
+    every replay, with new matrix contents and eager work and synchronisations in between, gives
+    the same factors as an eager factorisation of the same matrix on a fresh buffer (compared
+    through one correction x = U^-1 L^-1 b).  n = 5440 covers paired panels in the graph."""
+    print("case factor graph n=%d" % n, flush=True)
+    rng = np.random.default_rng(n + 1)
+    Abuf = torch.empty((n, n), dtype=torch.complex128, device="cuda").t()
+    B = _dev(rng.standard_normal((n, 1)) + 1j * rng.standard_normal((n, 1)))
+    lu = ctx.lu32(n)
+    for t in range(4):
+        A = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) / np.sqrt(n)
+        A += 4 * np.eye(n)
+        Ad = _dev(A)
+        Abuf.copy_(Ad)
+        lu.factor(Abuf)  # t = 0 eager, t = 1 capture + replay, then replays
+        X = torch.zeros_like(B)
+        lu.refine(Abuf, B, X, maxit=1, raise_on_fail=False)
+        ref = ctx.lu32(n)
+        ref.factor(Ad)  # first factorisation of its buffer: eager
+        Xr = torch.zeros_like(B)
+        ref.refine(Ad, B, Xr, maxit=1, raise_on_fail=False)
+        torch.cuda.synchronize()
+        ref.close()
+        d = (torch.linalg.norm(X - Xr) / torch.linalg.norm(Xr)).item()
+        print("  factorisation %d: |x - x_eager|/|x_eager| = %.2e" % (t, d), flush=True)
+        assert d < 1e-12, (t, d)  # the same kernels on the same data: rounding-level at most
+    assert lu.graph_replays() == 3
+    lu.close()

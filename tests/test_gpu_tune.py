@@ -1,0 +1,77 @@
+"""Launch-shape table (bem2d_tune_*): every assembly shape the autotuner can choose gives the
+oracle's matrix, every FP32 LU panel-pairing choice the FP64 solution, and the table loader
+accepts the autotuner's format and rejects malformed tables."""
+import numpy as np
+import pytest
+import torch
+
+from bem2d_amd import capi
+from bem2d_amd import geometry as G
+from conftest import rel_fro
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("name,geom,k", [("circle200", G.unit_circle(200), 10.0),
+                                         ("kite333", G.kite(333), 40.0)])
+def test_every_assembly_split_matches_oracle(ctx, oracle, name, geom, k):
+    saved = ctx.tune_assembly()
+    try:
+        mesh = ctx.mesh(geom, 16, 3, 0.15)
+        R = oracle.assemble(geom, 16, 3, 0.15, oracle.K_COMBINED, k, k, 1.0, 2.0)
+        mats = {}
+        for split in (1, 2, 4):
+            ctx.tune_assembly([(1 << 62, split)])
+            mats[split] = ctx.assemble(mesh, capi.KERNEL_COMBINED, k, k, 1.0, 2.0).cpu().numpy()
+            err = rel_fro(mats[split], R)
+            print(name, "split", split, "%.2e" % err, flush=True)
+            assert err < 1e-12
+        assert rel_fro(mats[2], mats[1]) < 1e-14 and rel_fro(mats[4], mats[1]) < 1e-14
+        mesh.close()
+    finally:
+        ctx.tune_assembly(saved)
+
+
+def test_lu_pairing_choices_solve(ctx):
+    n = 900
+    rng = np.random.default_rng(9)
+    A = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) / np.sqrt(n)
+    A += 4 * np.eye(n)
+    b = rng.standard_normal((n, 1)) + 1j * rng.standard_normal((n, 1))
+    x_ref = np.linalg.solve(A, b)
+    At = torch.tensor(np.ascontiguousarray(A.T), device="cuda").t()
+    saved = ctx.tune_lu_pair_tiles()
+    try:
+        for pair in (1, 10, 1 << 30):  # pair whenever possible, from 10 tiles, never
+            ctx.tune_lu_pair_tiles(pair)
+            B = torch.tensor(np.ascontiguousarray(b.T), device="cuda").t().clone()
+            rep = ctx.solve_mixed(At, B, tol=1e-12)
+            err = rel_fro(B.cpu().numpy(), x_ref)
+            print("pair tiles", pair, "backward %.2e" % rep.backward_error, "err %.2e" % err,
+                  flush=True)
+            assert rep.fallback_fp64 == 0 and rep.backward_error <= 1e-12 and err < 1e-10
+    finally:
+        ctx.tune_lu_pair_tiles(saved)
+
+
+def test_table_load(ctx, tmp_path):
+    saved_a, saved_p = ctx.tune_assembly(), ctx.tune_lu_pair_tiles()
+    try:
+        good = tmp_path / "good.tune"
+        good.write_text("# measured\nasm_split 500 4\nasm_split 9000 2  # comment\n"
+                        "lu_pair_tiles 77\n")
+        ctx.tune_load(str(good))
+        assert ctx.tune_assembly() == [(500, 4), (9000, 2)]
+        assert ctx.tune_lu_pair_tiles() == 77
+        for text in ("asm_split 500 3\n", "asm_split 900 4\nasm_split 500 2\n", "bogus 1\n",
+                     "lu_pair_tiles 0\n"):
+            bad = tmp_path / "bad.tune"
+            bad.write_text(text)
+            with pytest.raises(capi.Bem2dError) as e:
+                ctx.tune_load(str(bad))
+            assert e.value.code == capi.ERR_ARG
+        with pytest.raises(capi.Bem2dError):
+            ctx.tune_load(str(tmp_path / "missing.tune"))
+    finally:
+        ctx.tune_assembly(saved_a)
+        ctx.tune_lu_pair_tiles(saved_p)
