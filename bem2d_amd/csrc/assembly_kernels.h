@@ -704,22 +704,38 @@ static __global__ void farfield_signature_kernel(int64_t n, int nq, const double
     F[t] = make_double2(sr, si);
 }
 
-// out[a + nang*r] = pref * sum_j F[a + nang*j] coef[j + ldc*r]: thread per (a, r).
-static __global__ void farfield_combine_kernel(int64_t n, const double2* __restrict__ F, int64_t nang,
-                                        const double2* __restrict__ coef, int64_t ldc, double pr,
-                                        double pi_, double2* __restrict__ out)
+// out[a + nang*r] = pref * sum_j F[a + nang*j] coef[j + ldc*r]: one 256-thread workgroup per
+// (a, r), its threads striding over j; the partial sums meet by lane exchange and in LDS.  (A thread
+// per (a, r) walked all n elements alone: ~0.8 ms per request at n = 2500 with 36 angles.)
+static __global__ __launch_bounds__(256) void farfield_combine_kernel(
+    int64_t n, const double2* __restrict__ F, int64_t nang, const double2* __restrict__ coef,
+    int64_t ldc, double pr, double pi_, double2* __restrict__ out)
 {
-    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t a = blockIdx.x;
     const int r = blockIdx.y;
-    if (a >= nang) return;
     double vr = 0.0, vi = 0.0;
-    for (int64_t j = 0; j < n; ++j) {
+    for (int64_t j = threadIdx.x; j < n; j += 256) {
         const double2 f = F[a + nang * j];
         const double2 c = coef[j + ldc * r];
         vr += f.x * c.x - f.y * c.y;
         vi += f.x * c.y + f.y * c.x;
     }
-    out[a + nang * r] = make_double2(pr * vr - pi_ * vi, pr * vi + pi_ * vr);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        vr += __shfl_xor(vr, o, 64);
+        vi += __shfl_xor(vi, o, 64);
+    }
+    __shared__ double part[2][4];
+    if (threadIdx.x % 64 == 0) {
+        part[0][threadIdx.x / 64] = vr;
+        part[1][threadIdx.x / 64] = vi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        vr = (part[0][0] + part[0][1]) + (part[0][2] + part[0][3]);
+        vi = (part[1][0] + part[1][1]) + (part[1][2] + part[1][3]);
+        out[a + nang * r] = make_double2(pr * vr - pi_ * vi, pr * vi + pi_ * vr);
+    }
 }
 
 } // namespace bem2d_dev

@@ -885,8 +885,8 @@ int bem2d_internal_farfield(const bem2d_mesh* m, double k, const void* coef, int
         m->n, m->nq, m->d_nodes, k, theta, nang, (double2*)F);
     const double pi = 3.14159265358979323846;
     const double mag = -1.0 / std::sqrt(8.0 * pi * k);
-    dim3 grid((unsigned)((nang + 127) / 128), (unsigned)nrhs);
-    farfield_combine_kernel<<<grid, 128, 0, s>>>(m->n, (const double2*)F, nang,
+    dim3 grid((unsigned)nang, (unsigned)nrhs);
+    farfield_combine_kernel<<<grid, 256, 0, s>>>(m->n, (const double2*)F, nang,
                                                  (const double2*)coef, ldc, mag * std::cos(pi / 4),
                                                  mag * std::sin(pi / 4), (double2*)out);
     return hipGetLastError() == hipSuccess ? BEM2D_OK : FAIL(BEM2D_ERR_DEVICE, "far-field launch failed");
@@ -909,7 +909,7 @@ extern "C" int bem2d_farfield(bem2d_ctx* c, const bem2d_mesh* m, double k, const
                               int64_t ldc, int nrhs, const double* theta, int64_t nang, void* out)
 {
     if (!c || !m || !coef || !theta || !out) return FAIL(BEM2D_ERR_ARG, "bem2d_farfield: null argument");
-    if (ldc < m->n || nrhs < 1 || nrhs > 65535 || nang < 1 || !(k > 0))
+    if (ldc < m->n || nrhs < 1 || nrhs > 65535 || nang < 1 || nang > (1 << 24) || !(k > 0))
         return FAIL(BEM2D_ERR_ARG, "bem2d_farfield: bad sizes");
     HIPCHK(hipSetDevice(c->device));
     double2* F = (double2*)bem2d_internal_scratch(c, SCRATCH_FARFIELD, sizeof(double2) * nang * m->n);

@@ -322,7 +322,8 @@ extern "C" int bem2d_engine_submit(bem2d_engine* e, const bem2d_request* q, doub
     const bem2d_geometry& g = q->geom;
     if (g.n < 1 || !g.type || !g.param || !g.index || !g.next || !g.prev)
         return FAIL(BEM2D_ERR_ARG, "bem2d_engine_submit: bad geometry");
-    if (q->nrhs < 1 || q->nrhs > 65535 || !q->dirs || q->nang < 0 || (q->nang > 0 && !q->theta) ||
+    if (q->nrhs < 1 || q->nrhs > 65535 || !q->dirs || q->nang < 0 || q->nang > (1 << 24) ||
+        (q->nang > 0 && !q->theta) ||
         !(q->k > 0) || (q->precision != BEM2D_PRECISION_MIXED && q->precision != BEM2D_PRECISION_FP64))
         return FAIL(BEM2D_ERR_ARG, "bem2d_engine_submit: bad request");
     if (q->opts.N < 1 || q->opts.N > kMaxNq || q->opts.L < 0 || q->opts.L > 12 ||
