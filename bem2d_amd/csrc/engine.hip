@@ -42,6 +42,7 @@
 namespace {
 
 constexpr size_t kAlign = 256;
+constexpr int kAdmitPerStep = 8; // requests admitted (meshes built) per engine step at most
 size_t round_up(size_t b) { return (b + kAlign - 1) & ~(kAlign - 1); }
 
 // First-fit allocator over one device allocation, with coalescing of free neighbours.
@@ -462,9 +463,13 @@ extern "C" int bem2d_engine_step(bem2d_engine* e, int* unfinished)
     e->active.erase(std::remove_if(e->active.begin(), e->active.end(),
                                    [](Request* r) { return r->stage == DONE; }),
                     e->active.end());
-    // ---- admission (FIFO, by footprint) ----
+    // ---- admission (FIFO, by footprint), at most kAdmitPerStep per step: a mesh is built on the
+    // host (rules, singular-pair list, upload), so admitting a long queue at once would hold the
+    // first launches back until every mesh is ready; in slices the device starts on the first
+    // requests while the host prepares the next ----
     std::vector<Request*> fresh;
-    while (!e->queue.empty() && (int)e->active.size() < e->max_active) {
+    while (!e->queue.empty() && (int)e->active.size() < e->max_active &&
+           (int)fresh.size() < kAdmitPerStep) {
         Request* r = e->queue.front();
         if (r->footprint > e->arena->capacity()) {
             e->queue.pop_front();
