@@ -44,7 +44,6 @@ constexpr long long kSpinLimit = 1ll << 26;
 // least this many 64-wide tiles: the saved pass over it (16 B per entry) then outweighs the
 // diagonal factorisation that the narrow update of the pair leaves exposed (~60 us).  Measured on
 // MI355X: N = 4000 (at most 62 tiles) is faster unpaired, N = 32000 pairs (1.94 -> 1.53 s).
-constexpr int kPairTiles = 80; // default of TuneState::lu_pair_tiles
 
 typedef float v4f __attribute__((ext_vector_type(4)));
 

@@ -27,7 +27,7 @@ int main(int argc, char** argv)
 {
     std::string shape = argc > 1 ? argv[1] : "circle";
     int n = argc > 2 ? std::atoi(argv[2]) : 200;
-    freqtype k = {argc > 3 ? std::atof(argv[3]) : 10.0, 0};
+    freqtype k = {argc > 3 ? std::atof(argv[3]) : 10.0, argc > 5 ? std::atof(argv[5]) : 0.0};
     int N = argc > 4 ? std::atoi(argv[4]) : 16;
 
     pGeometry pgeom;
@@ -61,9 +61,10 @@ int main(int argc, char** argv)
         std::fprintf(stderr, "device error\n");
         return 3;
     }
-    dvector theta(360);
-    for (int a = 0; a < 360; a++) theta[a] = 2 * PI * a / 360;
-    pMatrix ff = soundsoft.FarField(theta);
+    // the far field is defined for real wavenumbers only (an empty list for complex k)
+    dvector theta(k.im == 0 ? 360 : 0);
+    for (std::size_t a = 0; a < theta.size(); a++) theta[a] = 2 * PI * a / 360;
+    pMatrix ff = k.im == 0 ? soundsoft.FarField(theta) : pMatrix(new Matrix(0, 1));
     std::vector<Point> pts{Point(-1.5, 0)};
     pcvector u = soundsoft.EvalSol(pts);
     pMatrix A = soundsoft.GetMatrix();
@@ -73,7 +74,7 @@ int main(int argc, char** argv)
 
     std::printf("{\n\"n\": %zu,\n", nn);
     print_cvec("solution", soundsoft.GetSolution()->data->data(), nn);
-    print_cvec("farfield", ff->data->data(), 360);
+    print_cvec("farfield", ff->data->data(), theta.size());
     print_cvec("u_at_point", u->data(), 1);
     print_cvec("diag", diag.data(), nn);
     print_cvec("col0", A->data->data(), nn, true);
