@@ -565,15 +565,23 @@ __device__ __forceinline__ void special_entry(const AsmBatch& batch, const Eval&
             const int ix = cached ? (int)ridx[2 * q] : 0, iy = cached ? (int)ridx[2 * q + 1] : 0;
             double kr, ki;
             if (rule == 1) {
-                // Duffy split of the coincident pair: (x, (1-y)x) and (1-x, 1-x+yx), Jacobian x
+                // Duffy split of the coincident pair: (x, (1-y)x) and (1-x, 1-x+yx), Jacobian x.
+                // The trial parameters are rounded as the reference rounds them (no fused
+                // multiply-add): near-coincident nodes make the entry depend on their last bits.
+                double u2a, u2b;
+                {
+#pragma clang fp contract(off)
+                    u2a = (1 - yq) * xq;
+                    u2b = 1 - xq + yq * xq;
+                }
                 ElemPoint p1 = cached ? cache[0][ix] : elem_point(ti, pi_, xq);
-                ElemPoint p2 = elem_point(tj, pj, (1 - yq) * xq);
+                ElemPoint p2 = elem_point(tj, pj, u2a);
                 ev.template eval<KIND>(p1.mx, p1.my, p2.mx, p2.my, p1.nx, p1.ny, p2.nx, p2.ny,
                                        kr, ki);
                 ar += xq * kr * p1.s * p2.s * wq;
                 ai += xq * ki * p1.s * p2.s * wq;
                 p1 = cached ? cache[1][ix] : elem_point(ti, pi_, 1 - xq);
-                p2 = elem_point(tj, pj, 1 - xq + yq * xq);
+                p2 = elem_point(tj, pj, u2b);
                 ev.template eval<KIND>(p1.mx, p1.my, p2.mx, p2.my, p1.nx, p1.ny, p2.nx, p2.ny,
                                        kr, ki);
                 ar += xq * kr * p1.s * p2.s * wq;

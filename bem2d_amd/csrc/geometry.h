@@ -23,12 +23,112 @@ struct Complex2 {
     double re, im;
 };
 
-// sin and cos of a curve angle: the reduced-argument routine on the device (few registers), the C
-// library on the host.
+// ---- correctly rounded sin and cos of curve angles ---------------------------------------------
+// Near-coincident quadrature nodes make the double-layer factor n.(x - y) ~ kappa d^2 / 2 cancel,
+// so that a one-ulp difference in a node position moves near-field entries by ~ulp / (kappa d^2)
+// (1e-9 relative at ten elements per wavelength on N = 7000 elements).  The reference evaluates
+// its curves with the C library's sin and cos, which round correctly in all but a tiny fraction of
+// cases; so do these: the argument is reduced by pi/2 in double-double arithmetic (pi/2 to 159
+// bits), the series of sin and cos are summed with their leading terms in double-double (error
+// below 2^-70 relative), and the result is rounded once.  Contraction is off throughout: a product
+// fused into a two-sum would break its error-free transformation.
+struct DD {
+    double hi, lo;
+};
+__host__ __device__ __forceinline__ DD dd_two_sum(double a, double b)
+{
+#pragma clang fp contract(off)
+    const double s = a + b, bb = s - a;
+    return DD{s, (a - (s - bb)) + (b - bb)};
+}
+__host__ __device__ __forceinline__ DD dd_fast(double a, double b) // |a| >= |b|
+{
+#pragma clang fp contract(off)
+    const double s = a + b;
+    return DD{s, b - (s - a)};
+}
+__host__ __device__ __forceinline__ DD dd_prod(double a, double b)
+{
+#pragma clang fp contract(off)
+    const double p = a * b;
+    return DD{p, fma(a, b, -p)};
+}
+__host__ __device__ __forceinline__ DD dd_add(DD a, DD b)
+{
+#pragma clang fp contract(off)
+    const DD s = dd_two_sum(a.hi, b.hi);
+    return dd_fast(s.hi, s.lo + (a.lo + b.lo));
+}
+__host__ __device__ __forceinline__ DD dd_mul(DD a, DD b)
+{
+#pragma clang fp contract(off)
+    const DD p = dd_prod(a.hi, b.hi);
+    return dd_fast(p.hi, p.lo + (a.hi * b.lo + a.lo * b.hi));
+}
+
+__host__ __device__ __forceinline__ void sincos_cr(double x, double& s, double& c)
+{
+#pragma clang fp contract(off)
+    // pi/2 = P1 + P2 + P3, 1/k! as hi + lo (k <= 7) and as doubles beyond
+    constexpr double P1 = 1.5707963267948966, P2 = 6.123233995736766e-17,
+                     P3 = -1.4973849048591698e-33, TWO_OVER_PI = 0.6366197723675814;
+    const double n = rint(x * TWO_OVER_PI);
+    // r = x - n (P1 + P2 + P3): the products exact as double-double pairs
+    const DD a1 = dd_prod(n, P1), a2 = dd_prod(n, P2);
+    DD r = dd_two_sum(x, -a1.hi);
+    r = dd_add(r, DD{-a1.lo, 0.0});
+    r = dd_add(r, DD{-a2.hi, -a2.lo});
+    r = dd_add(r, DD{-n * P3, 0.0});
+    const DD r2 = dd_mul(r, r);
+    const double z = r2.hi;
+    // sin r = r + r^3 (S3 + r^2 (S5 + r^2 (S7 + r^2 ts)))
+    double ts;
+    {
+        // the double tail: S9 .. S29 (S29 first)
+        const double t[11] = {1.1309962886447716e-31, -9.183689863795546e-29,
+                              6.446950284384474e-26, -3.868170170630684e-23,
+                              1.9572941063391263e-20, -8.22063524662433e-18,
+                              2.8114572543455206e-15, -7.647163731819816e-13,
+                              1.6059043836821613e-10, -2.505210838544172e-08,
+                              2.7557319223985893e-06};
+        ts = t[0];
+#pragma unroll
+        for (int i = 1; i < 11; ++i) ts = fma(ts, z, t[i]);
+    }
+    DD us = dd_add(DD{-0.0001984126984126984, -1.7209558293420705e-22}, DD{ts * z, 0.0});
+    us = dd_add(DD{0.008333333333333333, 1.1564823173178714e-19}, dd_mul(r2, us));
+    us = dd_add(DD{-0.16666666666666666, -9.25185853854297e-18}, dd_mul(r2, us));
+    const DD sr = dd_add(r, dd_mul(dd_mul(r, r2), us));
+    // cos r = 1 + r^2 (C2 + r^2 (C4 + r^2 (C6 + r^2 tc)))
+    double tc;
+    {
+        const double t[11] = {3.279889237069838e-30, -2.4795962632247976e-27,
+                              1.6117375710961184e-24, -8.896791392450574e-22,
+                              4.110317623312165e-19, -1.5619206968586225e-16,
+                              4.779477332387385e-14, -1.1470745597729725e-11,
+                              2.08767569878681e-09, -2.755731922398589e-07,
+                              2.48015873015873e-05};
+        tc = t[0];
+#pragma unroll
+        for (int i = 1; i < 11; ++i) tc = fma(tc, z, t[i]);
+    }
+    DD uc = dd_add(DD{-0.001388888888888889, 5.300543954373577e-20}, DD{tc * z, 0.0});
+    uc = dd_add(DD{0.041666666666666664, 2.3129646346357427e-18}, dd_mul(r2, uc));
+    uc = dd_add(DD{-0.5, 0.0}, dd_mul(r2, uc));
+    const DD cr = dd_add(DD{1.0, 0.0}, dd_mul(r2, uc));
+    const double sv = sr.hi + sr.lo, cv = cr.hi + cr.lo;
+    const int q = ((int)(long long)n) & 3;
+    const double a = (q & 1) ? cv : sv; // sin before the sign
+    const double b = (q & 1) ? sv : cv; // cos before the sign
+    s = (q & 2) ? -a : a;
+    c = ((q + 1) & 2) ? -b : b;
+}
+
+// sin and cos of a curve angle: correctly rounded on the device, the C library on the host.
 __host__ __device__ __forceinline__ void angle_sincos(double a, double& s, double& c)
 {
 #ifdef __HIP_DEVICE_COMPILE__
-    sincos_reduced(a, s, c);
+    sincos_cr(a, s, c);
 #else
     s = sin(a);
     c = cos(a);

@@ -409,39 +409,50 @@ void oracle_hankel_complex_many(const double* z, int64_t n, double* out)
 /* ------------------------------------------------------------------------------------------ */
 /* Geometry (lib/bem2d_curvetypes.h:13-99, lib/bem2d_element.h:29-34, 72-116).                    */
 /* The association of the reference's products (e.g. (2*PI)*t) is kept.                          */
+/* The curves' sin and cos are the C library's, as in the reference; built with ORACLE_CR_TRIG    */
+/* they are correctly rounded instead (evaluated in __float128 and rounded once): the distance     */
+/* between the two builds measures how far the reference's own matrix moves with the libm it is   */
+/* linked against (near-field double-layer entries amplify a one-ulp change of a node position).  */
 /* ------------------------------------------------------------------------------------------ */
+#ifdef ORACLE_CR_TRIG
+#define CSIN(x) ((double)sinq((__float128)(x)))
+#define CCOS(x) ((double)cosq((__float128)(x)))
+#else
+#define CSIN(x) sin(x)
+#define CCOS(x) cos(x)
+#endif
 static void curve_eval(int type, const double* c, double t, pt* M, pt* D)
 {
     const double PI = O_PI;
     switch (type) {
     case E_CIRCLE:
-        M->x = cos(2 * PI * t); M->y = sin(2 * PI * t);
-        D->x = 2 * PI * (-1.0 * sin(2 * PI * t)); D->y = 2 * PI * cos(2 * PI * t);
+        M->x = CCOS(2 * PI * t); M->y = CSIN(2 * PI * t);
+        D->x = 2 * PI * (-1.0 * CSIN(2 * PI * t)); D->y = 2 * PI * CCOS(2 * PI * t);
         break;
     case E_TREFOIL: {
-        double r = 1 + 0.3 * cos(3 * 2 * PI * t);
-        M->x = r * cos(2 * PI * t); M->y = r * sin(2 * PI * t);
-        double rp = -0.3 * 3 * 2 * PI * sin(3 * 2 * PI * t);
-        D->x = rp * cos(2 * PI * t) - r * 2 * PI * sin(2 * PI * t);
-        D->y = rp * sin(2 * PI * t) + r * 2 * PI * cos(2 * PI * t);
+        double r = 1 + 0.3 * CCOS(3 * 2 * PI * t);
+        M->x = r * CCOS(2 * PI * t); M->y = r * CSIN(2 * PI * t);
+        double rp = -0.3 * 3 * 2 * PI * CSIN(3 * 2 * PI * t);
+        D->x = rp * CCOS(2 * PI * t) - r * 2 * PI * CSIN(2 * PI * t);
+        D->y = rp * CSIN(2 * PI * t) + r * 2 * PI * CCOS(2 * PI * t);
         break;
     }
     case E_KITE:
-        M->x = cos(2 * PI * t) + 0.65 * cos(4 * PI * t) - 0.65; M->y = 1.5 * sin(2 * PI * t);
-        D->x = -2 * PI * sin(2 * PI * t) - 0.65 * 4 * PI * sin(4 * PI * t);
-        D->y = 1.5 * 2 * PI * cos(2 * PI * t);
+        M->x = CCOS(2 * PI * t) + 0.65 * CCOS(4 * PI * t) - 0.65; M->y = 1.5 * CSIN(2 * PI * t);
+        D->x = -2 * PI * CSIN(2 * PI * t) - 0.65 * 4 * PI * CSIN(4 * PI * t);
+        D->y = 1.5 * 2 * PI * CCOS(2 * PI * t);
         break;
     case E_ELLIPSE: {
         double rho = c[0];
-        M->x = rho * cos(2 * PI * t); M->y = 1 / rho * sin(2 * PI * t);
-        D->x = -rho * 2 * PI * sin(2 * PI * t); D->y = 2 * PI / rho * cos(2 * PI * t);
+        M->x = rho * CCOS(2 * PI * t); M->y = 1 / rho * CSIN(2 * PI * t);
+        D->x = -rho * 2 * PI * CSIN(2 * PI * t); D->y = 2 * PI / rho * CCOS(2 * PI * t);
         break;
     }
     case E_ELLIPSE_ARC: {
         double a = c[0], b = c[1], t1 = c[2], t2 = c[3];
-        M->x = a * cos(t1 + t * (t2 - t1)); M->y = b * sin(t1 + t * (t2 - t1));
-        D->x = -a * (t2 - t1) * sin(t1 + t * (t2 - t1));
-        D->y = b * (t2 - t1) * cos(t1 + t * (t2 - t1));
+        M->x = a * CCOS(t1 + t * (t2 - t1)); M->y = b * CSIN(t1 + t * (t2 - t1));
+        D->x = -a * (t2 - t1) * CSIN(t1 + t * (t2 - t1));
+        D->y = b * (t2 - t1) * CCOS(t1 + t * (t2 - t1));
         break;
     }
     case E_INV_ELLIPSE: {

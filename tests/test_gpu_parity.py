@@ -230,3 +230,26 @@ def test_kite_config2_full_size(ctx, oracle):
     r = torch.linalg.norm(B - A @ X) / (torch.linalg.norm(A) * torch.linalg.norm(X) +
                                          torch.linalg.norm(B))
     assert r.item() < 1e-15
+
+
+def test_near_field_double_layer_on_a_near_circle(ctx, oracle):
+    """Near-coincident nodes of a smooth curve make n.(x - y) cancel, so the coincident and adjacent
+    adjoint double-layer entries depend on the last bit of every node position: the device's curve
+    sin / cos must round as the C library's do (correctly, in all but ~0.1% of arguments).  A
+    near-circular ellipse at N = 7159 (the largest problem of the served request stream), k = 175:
+    the entries around the diagonal of three columns against the oracle to 1e-12 each (before the
+    device rounded its curve trigonometry correctly, the coincident entries were 1.4e-7 off)."""
+    from bem2d_amd import serving
+    geom = serving.ellipse(1.9399066438338748, 1.8867428378418813, 7159)
+    n, k = geom.n, 175.0
+    mesh = ctx.mesh(geom, 16, 3, 0.15)
+    A = ctx.assemble(mesh, capi.KERNEL_CDL, k, k, 0.0, 1.0)
+    cols = np.array([0, 2386, 7158])
+    ref = oracle.assemble(geom, 16, 3, 0.15, oracle.K_CDL, k, k, 0.0, 1.0, cols=cols)
+    got = _np(A[:, torch.tensor(cols, device="cuda")])
+    for c, j in enumerate(cols):
+        rows = (j + np.arange(-3, 4)) % n
+        rel = np.abs(got[rows, c] - ref[rows, c]) / np.abs(ref[rows, c])
+        print("column %d: entry errors around the diagonal %s" % (j, " ".join("%.1e" % v for v in rel)))
+        assert rel.max() < 1e-12, (j, rel)
+    mesh.close()
