@@ -37,8 +37,8 @@ class QuadOption(ctypes.Structure):
 
 class IRReport(ctypes.Structure):
     """Outcome of the mixed-precision solve (bem2d_ir_report in include/bem2d_hip.h)."""
-    _fields_ = [("iterations", _I), ("fallback_fp64", _I), ("converged", _I), ("reserved", _I),
-                ("backward_error", _D), ("matrix_norm_inf", _D)]
+    _fields_ = [("iterations", _I), ("fallback_fp64", _I), ("converged", _I), ("gmres_steps", _I),
+                ("backward_error", _D), ("matrix_norm_inf", _D), ("growth", _D)]
 
 
 SIGNATURES = {
@@ -101,7 +101,8 @@ class Request(ctypes.Structure):
 
 class ResultInfo(ctypes.Structure):
     _fields_ = [("status", _I), ("precision", _I), ("n", _I64), ("nrhs", _I), ("iterations", _I),
-                ("backward_error", _D), ("t_arrival", _D), ("t_admit", _D), ("t_done", _D)]
+                ("backward_error", _D), ("t_arrival", _D), ("t_admit", _D), ("t_done", _D),
+                ("fallback_fp64", _I)]
 
 
 class EngineStats(ctypes.Structure):

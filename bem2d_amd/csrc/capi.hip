@@ -881,6 +881,9 @@ int bem2d_internal_farfield(const bem2d_mesh* m, double k, const void* coef, int
                             hipStream_t s)
 {
     const int64_t tot = nang * m->n;
+    if (nang < 1 || nang >= (1 << 24) || tot > (int64_t)UINT32_MAX - 256)
+        return FAIL(BEM2D_ERR_UNSUPPORTED, "far field: %lld angles x %lld elements exceed one launch",
+                    (long long)nang, (long long)m->n);
     farfield_signature_kernel<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(
         m->n, m->nq, m->d_nodes, k, theta, nang, (double2*)F);
     const double pi = 3.14159265358979323846;
@@ -909,7 +912,8 @@ extern "C" int bem2d_farfield(bem2d_ctx* c, const bem2d_mesh* m, double k, const
                               int64_t ldc, int nrhs, const double* theta, int64_t nang, void* out)
 {
     if (!c || !m || !coef || !theta || !out) return FAIL(BEM2D_ERR_ARG, "bem2d_farfield: null argument");
-    if (ldc < m->n || nrhs < 1 || nrhs > 65535 || nang < 1 || nang > (1 << 24) || !(k > 0))
+    // one 256-thread workgroup per angle: nang * 256 work-items must stay below 2^32
+    if (ldc < m->n || nrhs < 1 || nrhs > 65535 || nang < 1 || nang >= (1 << 24) || !(k > 0))
         return FAIL(BEM2D_ERR_ARG, "bem2d_farfield: bad sizes");
     HIPCHK(hipSetDevice(c->device));
     double2* F = (double2*)bem2d_internal_scratch(c, SCRATCH_FARFIELD, sizeof(double2) * nang * m->n);

@@ -155,11 +155,13 @@ struct Request {
     double2* ff = nullptr;
     double2* sig = nullptr;
     bem2d_lu32* lu = nullptr;
-    int* piv = nullptr;
+    int* piv = nullptr;     // pivots + info of the FP64 LU (FP64 requests and the fallback)
+    bool fp64_ran = false;  // the FP64 LU solved this request (its info decides the status)
     hipEvent_t ev = nullptr;
     // results
     double2* hx = nullptr;  // pinned
     double2* hff = nullptr; // pinned
+    int* hinfo = nullptr;   // pinned copy of the FP64 LU's info
     bem2d_ir_report rep{};
 };
 
@@ -184,6 +186,7 @@ struct bem2d_engine {
     RefJob* job = nullptr;
     std::vector<Request*> job_reqs;
     unsigned long long* pinned = nullptr; // refinement read-back buffer, kept for the engine's life
+    unsigned long long* dnorms = nullptr; // its device side (outside the arena, grown with it)
     size_t pinned_units = 0;
     bool launched = false;                // the last step launched work
     std::chrono::steady_clock::time_point t0;
@@ -228,13 +231,12 @@ extern "C" int64_t bem2d_engine_footprint(int64_t n, int nrhs, int64_t nang, int
     b += round_up(sizeof(double) * 2 * nrhs) + round_up(sizeof(double) * (nang + 1));
     b += round_up(sizeof(double2) * (nang + 1) * nrhs);       // far field
     b += round_up(sizeof(double2) * (nang + 1) * n);          // far-field signatures
-    if (precision == BEM2D_PRECISION_FP64) {
-        b += round_up(sizeof(int) * (n + 1));
-    } else {
+    b += round_up(sizeof(int) * (n + 1)); // FP64 LU pivots (also the mixed path's fallback)
+    if (precision == BEM2D_PRECISION_MIXED) {
         b += bem2d_internal_lu32_bytes(n) + 8 * kAlign;
         b += bem2d_internal_ref_bytes(n, nrhs);
     }
-    return (int64_t)(b + 16 * kAlign);
+    return (int64_t)round_up(b + 16 * kAlign); // the arena hands out whole kAlign units
 }
 
 extern "C" int bem2d_engine_create(bem2d_ctx* c, int64_t hbm_bytes, int max_active,
@@ -290,6 +292,7 @@ static void free_request(bem2d_engine* e, Request* r)
     release_request_device(e, r);
     if (r->hx) (void)hipHostFree(r->hx);
     if (r->hff) (void)hipHostFree(r->hff);
+    if (r->hinfo) (void)hipHostFree(r->hinfo);
     delete r;
 }
 
@@ -307,6 +310,7 @@ extern "C" void bem2d_engine_destroy(bem2d_engine* e)
     if (e->blas) (void)rocblas_destroy_handle(e->blas);
     if (e->blas_ws) (void)hipFree(e->blas_ws);
     if (e->pinned) (void)hipHostFree(e->pinned);
+    if (e->dnorms) (void)hipFree(e->dnorms);
     for (hipStream_t s : {e->s_setup, e->s_asm, e->s_lu, e->s_ref})
         if (s) (void)hipStreamDestroy(s);
     e->arena.reset();
@@ -323,7 +327,7 @@ extern "C" int bem2d_engine_submit(bem2d_engine* e, const bem2d_request* q, doub
     const bem2d_geometry& g = q->geom;
     if (g.n < 1 || !g.type || !g.param || !g.index || !g.next || !g.prev)
         return FAIL(BEM2D_ERR_ARG, "bem2d_engine_submit: bad geometry");
-    if (q->nrhs < 1 || q->nrhs > 65535 || !q->dirs || q->nang < 0 || q->nang > (1 << 24) ||
+    if (q->nrhs < 1 || q->nrhs > 65535 || !q->dirs || q->nang < 0 || q->nang >= (1 << 24) ||
         (q->nang > 0 && !q->theta) ||
         !(q->k > 0) || (q->precision != BEM2D_PRECISION_MIXED && q->precision != BEM2D_PRECISION_FP64))
         return FAIL(BEM2D_ERR_ARG, "bem2d_engine_submit: bad request");
@@ -372,13 +376,14 @@ static int admit(bem2d_engine* e, Request* r)
     r->d_theta = (double*)b.alloc(sizeof(double) * (r->nang + 1));
     r->ff = (double2*)b.alloc(sizeof(double2) * (r->nang + 1) * r->nrhs);
     r->sig = (double2*)b.alloc(sizeof(double2) * (r->nang + 1) * n);
-    if (r->precision == BEM2D_PRECISION_FP64) r->piv = (int*)b.alloc(sizeof(int) * (n + 1));
-    if (!r->A || !r->B || !r->X || !r->d_dirs || !r->d_theta || !r->ff || !r->sig ||
-        (r->precision == BEM2D_PRECISION_FP64 && !r->piv))
+    r->piv = (int*)b.alloc(sizeof(int) * (n + 1));
+    if (!r->A || !r->B || !r->X || !r->d_dirs || !r->d_theta || !r->ff || !r->sig || !r->piv)
         return FAIL(BEM2D_ERR_NOMEM, "request footprint too small");
     if (hipHostMalloc((void**)&r->hx, sizeof(double2) * n * r->nrhs) != hipSuccess ||
-        hipHostMalloc((void**)&r->hff, sizeof(double2) * (r->nang + 1) * r->nrhs) != hipSuccess)
+        hipHostMalloc((void**)&r->hff, sizeof(double2) * (r->nang + 1) * r->nrhs) != hipSuccess ||
+        hipHostMalloc((void**)&r->hinfo, sizeof(int)) != hipSuccess)
         return FAIL(BEM2D_ERR_NOMEM, "pinned result buffers");
+    *r->hinfo = 0;
     HIPCHK(hipMemcpyAsync(r->d_dirs, r->dirs.data(), sizeof(double) * 2 * r->nrhs,
                           hipMemcpyHostToDevice, e->s_setup));
     if (r->nang > 0)
@@ -412,14 +417,21 @@ static void fail_request(bem2d_engine* e, Request* r, int code)
 static void finish(bem2d_engine* e, Request* r, hipStream_t s)
 {
     const int64_t n = r->n;
+    if (r->nang > 0) {
+        const int rc = bem2d_internal_farfield(r->mesh, r->k, r->X, n, r->nrhs, r->d_theta,
+                                               r->nang, r->ff, r->sig, s);
+        if (rc && r->status == BEM2D_OK) r->status = rc;
+    }
+    bool ok = hipMemcpyAsync(r->hx, r->X, sizeof(double2) * n * r->nrhs, hipMemcpyDeviceToHost,
+                             s) == hipSuccess;
     if (r->nang > 0)
-        (void)bem2d_internal_farfield(r->mesh, r->k, r->X, n, r->nrhs, r->d_theta, r->nang, r->ff,
-                                      r->sig, s);
-    (void)hipMemcpyAsync(r->hx, r->X, sizeof(double2) * n * r->nrhs, hipMemcpyDeviceToHost, s);
-    if (r->nang > 0)
-        (void)hipMemcpyAsync(r->hff, r->ff, sizeof(double2) * r->nang * r->nrhs,
-                             hipMemcpyDeviceToHost, s);
-    (void)hipEventRecord(r->ev, s);
+        ok &= hipMemcpyAsync(r->hff, r->ff, sizeof(double2) * r->nang * r->nrhs,
+                             hipMemcpyDeviceToHost, s) == hipSuccess;
+    if (r->fp64_ran)
+        ok &= hipMemcpyAsync(r->hinfo, r->piv + n, sizeof(int), hipMemcpyDeviceToHost, s) ==
+              hipSuccess;
+    ok &= hipEventRecord(r->ev, s) == hipSuccess;
+    if (!ok && r->status == BEM2D_OK) r->status = BEM2D_ERR_DEVICE;
     r->stage = FINISHING;
 }
 
@@ -436,7 +448,6 @@ static int ensure_blas_workspace(bem2d_engine* e, int64_t n, int nrhs)
     if (need <= e->blas_ws_bytes) return BEM2D_OK;
     for (hipStream_t s : {e->s_lu}) (void)hipStreamSynchronize(s);
     if (e->blas_ws) (void)hipFree(e->blas_ws);
-    if (e->pinned) (void)hipHostFree(e->pinned);
     e->blas_ws = nullptr;
     e->blas_ws_bytes = 0;
     if (hipMalloc(&e->blas_ws, need) != hipSuccess) return BEM2D_ERR_NOMEM;
@@ -444,6 +455,34 @@ static int ensure_blas_workspace(bem2d_engine* e, int64_t n, int nrhs)
     return rocblas_set_workspace(e->blas, e->blas_ws, need) == rocblas_status_success
                ? BEM2D_OK
                : BEM2D_ERR_DEVICE;
+}
+
+// The plain FP64 solve of a request on the LU stream (rocSOLVER zgetrf + zgetrs, the reference's
+// SolveSystem), then finish.  A is factorised in place.  Serves FP64 requests and the fallback of
+// mixed requests whose FP32 factors could not reach the tolerance.
+static void solve_fp64(bem2d_engine* e, Request* r)
+{
+    int rc = ensure_blas_workspace(e, r->n, r->nrhs);
+    if (rc) {
+        fail_request(e, r, rc);
+        return;
+    }
+    (void)hipMemcpyAsync(r->X, r->B, sizeof(double2) * r->n * r->nrhs, hipMemcpyDeviceToDevice,
+                         e->s_lu);
+    rocblas_status st = rocsolver_zgetrf(e->blas, (rocblas_int)r->n, (rocblas_int)r->n,
+                                         (rocblas_double_complex*)r->A, (rocblas_int)r->n, r->piv,
+                                         r->piv + r->n);
+    if (st == rocblas_status_success)
+        st = rocsolver_zgetrs(e->blas, rocblas_operation_none, (rocblas_int)r->n,
+                              (rocblas_int)r->nrhs, (rocblas_double_complex*)r->A,
+                              (rocblas_int)r->n, r->piv, (rocblas_double_complex*)r->X,
+                              (rocblas_int)r->n);
+    if (st != rocblas_status_success) {
+        fail_request(e, r, BEM2D_ERR_DEVICE);
+        return;
+    }
+    r->fp64_ran = true;
+    finish(e, r, e->s_lu);
 }
 
 extern "C" int bem2d_engine_step(bem2d_engine* e, int* unfinished)
@@ -457,8 +496,13 @@ extern "C" int bem2d_engine_step(bem2d_engine* e, int* unfinished)
         if (r->stage == FINISHING && event_done(r->ev)) {
             r->stage = DONE;
             r->t_done = now;
+            if (r->fp64_ran && *r->hinfo > 0 && r->status == BEM2D_OK)
+                r->status = BEM2D_ERR_SINGULAR; // zgetrf info > 0: LapackError
             release_request_device(e, r);
-            e->stats.completed++;
+            if (r->status == BEM2D_OK)
+                e->stats.completed++;
+            else
+                e->stats.failed++;
         }
     e->active.erase(std::remove_if(e->active.begin(), e->active.end(),
                                    [](Request* r) { return r->stage == DONE; }),
@@ -524,26 +568,7 @@ extern "C" int bem2d_engine_step(bem2d_engine* e, int* unfinished)
             begin_ld.push_back(r->n);
             r->stage = FACTORING;
         } else {
-            int rc = ensure_blas_workspace(e, r->n, r->nrhs);
-            if (rc) {
-                fail_request(e, r, rc);
-                continue;
-            }
-            (void)hipMemcpyAsync(r->X, r->B, sizeof(double2) * r->n * r->nrhs,
-                                 hipMemcpyDeviceToDevice, e->s_lu);
-            rocblas_status st = rocsolver_zgetrf(e->blas, (rocblas_int)r->n, (rocblas_int)r->n,
-                                                 (rocblas_double_complex*)r->A, (rocblas_int)r->n,
-                                                 r->piv, r->piv + r->n);
-            if (st == rocblas_status_success)
-                st = rocsolver_zgetrs(e->blas, rocblas_operation_none, (rocblas_int)r->n,
-                                      (rocblas_int)r->nrhs, (rocblas_double_complex*)r->A,
-                                      (rocblas_int)r->n, r->piv, (rocblas_double_complex*)r->X,
-                                      (rocblas_int)r->n);
-            if (st != rocblas_status_success) {
-                fail_request(e, r, BEM2D_ERR_DEVICE);
-                continue;
-            }
-            finish(e, r, e->s_lu);
+            solve_fp64(e, r);
         }
     }
     if (!begin_lu.empty()) {
@@ -583,9 +608,23 @@ extern "C" int bem2d_engine_step(bem2d_engine* e, int* unfinished)
             Request* r = e->job_reqs[q];
             if (r->stage != REFINING || bem2d_internal_ref_spec_active(e->job, (int)q)) continue;
             bem2d_internal_ref_report(e->job, (int)q, &r->rep);
-            if (bem2d_internal_ref_bad_pivot(e->job, (int)q) || !r->rep.converged)
-                r->status = rc ? BEM2D_ERR_DEVICE : BEM2D_ERR_NOT_CONVERGED;
-            finish(e, r, e->s_ref);
+            if (rc) {
+                r->status = BEM2D_ERR_DEVICE;
+                finish(e, r, e->s_ref);
+            } else if (bem2d_internal_ref_bad_pivot(e->job, (int)q) || !r->rep.converged) {
+                // FP32 factors unusable (zero pivot, growth) or refinement stalled: the pivoted
+                // FP64 LU solves the request.  Its refinement work has completed (the read-back
+                // that retired it came after), and the LU stream waits for the refinement stream.
+                r->rep.fallback_fp64 = 1;
+                if (hipEventRecord(r->ev, e->s_ref) != hipSuccess ||
+                    hipStreamWaitEvent(e->s_lu, r->ev, 0) != hipSuccess) {
+                    fail_request(e, r, BEM2D_ERR_DEVICE);
+                    continue;
+                }
+                solve_fp64(e, r);
+            } else {
+                finish(e, r, e->s_ref);
+            }
         }
         if (all_done) {
             bem2d_internal_ref_destroy(e->job); // its workspace lives in the requests' blocks
@@ -611,18 +650,23 @@ extern "C" int bem2d_engine_step(bem2d_engine* e, int* unfinished)
             }
         if (!specs.empty()) {
             if (units > e->pinned_units) {
-                if (e->pinned) (void)hipHostFree(e->pinned);
-                e->pinned = nullptr;
-                e->pinned_units = 0;
+                // no job is in flight here (a new one is about to start), so both buffers are free
                 const size_t want = std::max<size_t>(units, 2 * e->pinned_units + 64);
+                if (e->pinned) (void)hipHostFree(e->pinned);
+                if (e->dnorms) (void)hipFree(e->dnorms);
+                e->pinned = nullptr;
+                e->dnorms = nullptr;
+                e->pinned_units = 0;
                 if (hipHostMalloc((void**)&e->pinned, sizeof(unsigned long long) * kRefNormSlots *
-                                                          want) != hipSuccess)
-                    return FAIL(BEM2D_ERR_NOMEM, "pinned read-back buffer");
+                                                          want) != hipSuccess ||
+                    hipMalloc((void**)&e->dnorms, sizeof(unsigned long long) * kRefNormSlots *
+                                                      want) != hipSuccess)
+                    return FAIL(BEM2D_ERR_NOMEM, "refinement read-back buffers");
                 e->pinned_units = want;
             }
             RefJob* j = nullptr;
             const int rc = bem2d_internal_ref_create(e->ctx, e->arena.get(), specs.data(),
-                                                     (int)specs.size(), e->pinned, &j);
+                                                     (int)specs.size(), e->pinned, e->dnorms, &j);
             if (rc) {
                 for (Request* r : reqs) fail_request(e, r, rc);
             } else {
@@ -667,6 +711,7 @@ extern "C" int bem2d_engine_result(bem2d_engine* e, int64_t ticket, void* x, voi
         info->precision = r->precision;
         info->iterations = r->rep.iterations;
         info->backward_error = r->rep.backward_error;
+        info->fallback_fp64 = r->rep.fallback_fp64;
         info->t_arrival = r->t_arrival;
         info->t_admit = r->t_admit;
         info->t_done = r->t_done;

@@ -153,12 +153,13 @@ struct RefSpec { // x = A^{-1} b by refinement with the FP32 factors lu
 };
 struct RefJob;
 size_t bem2d_internal_ref_bytes(int64_t n, int nrhs);
-// pinned: optional host buffer of >= kRefNormSlots x (units) 64-bit words for the read-backs
-// (null: the job allocates its own)
+// pinned / dnorms: optional host (pinned) and device buffers of >= kRefNormSlots x (units) 64-bit
+// words for the read-backs (null: the job allocates its own, the device one from arena)
 constexpr int kRefNormSlots = 32;
 int bem2d_internal_ref_units(int nrhs);
 int bem2d_internal_ref_create(bem2d_ctx* c, DevArena* arena, const RefSpec* specs, int count,
-                              unsigned long long* pinned, RefJob** out);
+                              unsigned long long* pinned, unsigned long long* dnorms,
+                              RefJob** out);
 void bem2d_internal_ref_destroy(RefJob* j);
 // enqueue the initial residual (first call) or one correction step, then an asynchronous read-back
 int bem2d_internal_ref_step(RefJob* j, hipStream_t s);
@@ -168,3 +169,19 @@ bool bem2d_internal_ref_spec_active(const RefJob* j, int sidx);
 void bem2d_internal_ref_report(const RefJob* j, int sidx, bem2d_ir_report* r);
 bool bem2d_internal_ref_bad_pivot(const RefJob* j, int sidx);
 int bem2d_internal_refine_sync(hipStream_t s, RefJob* j);
+
+// Read-only view of FP32 factors for code outside mixed.hip: the n x n leading block of F (ld np)
+// holds L (unit diagonal implied) and U of P A, where row i of P A is row perm[i] of A.
+struct Lu32View {
+    const float2* F;
+    int64_t n, np;
+    const int* perm;
+    const double* anorm; // |A|_inf (device)
+};
+Lu32View bem2d_internal_lu32_view(const bem2d_lu32* f);
+// GMRES-IR (FP64 GMRES on M^{-1} A d = M^{-1} r, M = the FP32 factors) from the current X for the
+// columns whose backward error is above tol; rep->backward_error / converged / gmres_steps are
+// updated.  Allocates its workspace; runs on the context's stream.
+int bem2d_internal_gmres_ir(bem2d_ctx* c, const bem2d_lu32* f, const void* A64, int64_t lda,
+                            const void* B, int64_t ldb, void* X, int64_t ldx, int nrhs, double tol,
+                            bem2d_ir_report* rep);

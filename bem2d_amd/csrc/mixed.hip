@@ -135,12 +135,15 @@ __device__ __forceinline__ float2 load_or(bool ok, const float2* p, const float2
 constexpr int kCastCols = 64;
 
 __global__ __launch_bounds__(256) void cast_kernel(int64_t n, int64_t np, const double2* __restrict__ A,
-                                                   int64_t lda, float2* __restrict__ F,
+                                                   int64_t lda, const int* __restrict__ perm,
+                                                   float2* __restrict__ F,
                                                    double* __restrict__ part)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t j0 = (int64_t)blockIdx.y * kCastCols;
     if (i >= np) return;
+    // row i of the factorised matrix is row perm[i] of A (the static pivoting order)
+    const int64_t pi = i < n ? (int64_t)perm[i] : 0;
     double s = 0.0;
     constexpr int kBatch = 16;
     for (int c0 = 0; c0 < kCastCols; c0 += kBatch) {
@@ -148,7 +151,7 @@ __global__ __launch_bounds__(256) void cast_kernel(int64_t n, int64_t np, const 
 #pragma unroll
         for (int c = 0; c < kBatch; ++c) {
             const int64_t j = j0 + c0 + c;
-            v[c] = *(i < n && j < n ? &A[i + lda * j] : A);
+            v[c] = *(i < n && j < n ? &A[pi + lda * j] : A);
         }
 #pragma unroll
         for (int c = 0; c < kBatch; ++c) {
@@ -193,6 +196,158 @@ __global__ __launch_bounds__(256) void rowsum_max_kernel(int64_t n, int nchunk,
         __syncthreads();
     }
     if (threadIdx.x == 0) atomic_max_nonneg(out, red[0]);
+}
+
+// ---- static pivoting ------------------------------------------------------------------------------
+// The factorisation runs without interchanges on the rows of A in an order fixed beforehand from
+// the matrix itself (static pivoting, as in SuperLU_DIST: X. S. Li and J. W. Demmel, "Making
+// sparse Gaussian elimination scalable by static pivoting", SC 1998, where a row permutation puts
+// large entries on the diagonal, the factorisation then never pivots, and iterative refinement
+// recovers the accuracy).  Column j claims the row of its largest entry; a row claimed by several
+// columns goes to the column in which its entry is largest (ties: the smaller column index); the
+// positions left over take the unclaimed rows in increasing order.  So a matrix whose largest
+// entry of every column lies in a different row -- a BEM matrix in its natural order, or any row
+// permutation of one -- gets exactly those rows on its diagonal, and any matrix gets a
+// permutation.  perm[i] = row of A at position i of the factorised matrix (i < n).
+// Key of a claim: |a_rj| as float bits above the complement of j (larger value first, then the
+// smaller j); never 0, so 0 marks an unclaimed row.
+
+// One workgroup per column: colarg[j] = argmax_i |a_ij| (the smallest i on ties), the claim of
+// that row, and amax = max_ij |a_ij| (NaN entries make it NaN).
+__global__ __launch_bounds__(256) void pivot_colmax_kernel(int64_t n, const double2* __restrict__ A,
+                                                           int64_t lda, int* __restrict__ colarg,
+                                                           unsigned long long* __restrict__ claim,
+                                                           unsigned long long* __restrict__ amax)
+{
+    __shared__ double sb[4];
+    __shared__ int sa[4];
+    const int64_t j = blockIdx.x;
+    const double2* __restrict__ col = A + lda * j;
+    double best = -1.0;
+    int arg = 0;
+    bool nan = false;
+    for (int64_t i = threadIdx.x; i < n; i += 256) {
+        const double2 a = col[i];
+        const double m = fma(a.x, a.x, a.y * a.y);
+        nan |= !(m == m);
+        if (m > best) { // strictly: the smallest i of this thread's rows on ties
+            best = m;
+            arg = (int)i;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const double ob = __shfl_xor(best, off);
+        const int oa = __shfl_xor(arg, off);
+        if (ob > best || (ob == best && oa < arg)) {
+            best = ob;
+            arg = oa;
+        }
+    }
+    const unsigned long long anynan = __ballot(nan);
+    __shared__ int snan[4];
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        sb[w] = best;
+        sa[w] = arg;
+        snan[w] = anynan != 0;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        bool bad = false;
+        for (int q = 0; q < 4; ++q) {
+            bad |= snan[q] != 0;
+            if (sb[q] > best || (sb[q] == best && sa[q] < arg)) {
+                best = sb[q];
+                arg = sa[q];
+            }
+        }
+        const double v = best > 0.0 ? sqrt(best) : 0.0;
+        colarg[j] = arg;
+        const unsigned long long key = ((unsigned long long)__float_as_uint((float)v) << 32) |
+                                       (unsigned long long)(0xffffffffu - (unsigned)j);
+        atomicMax(&claim[arg], key);
+        atomicMax(amax, (unsigned long long)__double_as_longlong(bad ? (double)NAN : v));
+    }
+}
+
+// Exclusive prefix count of flag over a 1024-thread workgroup (wave ballots + 16 wave totals);
+// *total = the count over the workgroup.
+__device__ __forceinline__ int block_scan_1024(bool flag, int* wave_tot, int* total)
+{
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const unsigned long long b = __ballot(flag);
+    const int pre = __popcll(b & ((1ull << lane) - 1ull));
+    if (lane == 0) wave_tot[w] = __popcll(b);
+    __syncthreads();
+    int off = 0, tot = 0;
+    for (int q = 0; q < 16; ++q) {
+        const int v = wave_tot[q];
+        off += q < w ? v : 0;
+        tot += v;
+    }
+    __syncthreads();
+    *total = tot;
+    return off + pre;
+}
+
+// One workgroup of 1024 threads: positions whose claim won keep their row; the others, in
+// increasing order, take the unclaimed rows in increasing order (freepos / freerow: n ints each).
+__global__ __launch_bounds__(1024) void pivot_assign_kernel(int64_t n, const int* __restrict__ colarg,
+                                                            const unsigned long long* __restrict__ claim,
+                                                            int* __restrict__ perm,
+                                                            int* __restrict__ freepos,
+                                                            int* __restrict__ freerow)
+{
+    __shared__ int wave_tot[16];
+    const int t = threadIdx.x;
+    int nfp = 0, nfr = 0;
+    for (int64_t base = 0; base < n; base += 1024) {
+        const int64_t j = base + t;
+        bool taken = false, unclaimed = false;
+        if (j < n) {
+            const int r = colarg[j];
+            taken = (unsigned)(claim[r] & 0xffffffffull) == 0xffffffffu - (unsigned)j;
+            if (taken) perm[j] = r;
+            unclaimed = claim[j] == 0ull;
+        }
+        int tot = 0;
+        const int fp = block_scan_1024(j < n && !taken, wave_tot, &tot);
+        if (j < n && !taken) freepos[nfp + fp] = (int)j;
+        nfp += tot;
+        const int fr = block_scan_1024(unclaimed, wave_tot, &tot);
+        if (unclaimed) freerow[nfr + fr] = (int)j;
+        nfr += tot;
+    }
+    __syncthreads(); // the lists written above are read by other threads of this workgroup
+    for (int q = t; q < nfp; q += 1024) perm[freepos[q]] = freerow[q];
+}
+
+// Element growth of the factors: gmax[0] = max |U_ij| (i <= j < n), gmax[1] = max |L_ij|
+// (j < i < n); NaN or infinite entries count as infinite.  Grid: rows in 256s x 64-column chunks.
+__global__ __launch_bounds__(256) void growth_kernel(int64_t n, int64_t np, const float2* __restrict__ F,
+                                                     unsigned long long* __restrict__ gmax)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t j0 = (int64_t)blockIdx.y * 64;
+    float mu = 0.f, ml = 0.f;
+    if (i < n) {
+        const int64_t j1 = min(j0 + 64, n);
+        for (int64_t j = j0; j < j1; ++j) {
+            const float2 v = F[i + np * j];
+            float a = sqrtf(fmaf(v.x, v.x, v.y * v.y));
+            if (!(a <= 3.0e38f)) a = INFINITY; // NaN and overflow
+            if (i <= j) mu = fmaxf(mu, a);
+            else ml = fmaxf(ml, a);
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        mu = fmaxf(mu, __shfl_xor(mu, off));
+        ml = fmaxf(ml, __shfl_xor(ml, off));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomic_max_nonneg(&gmax[0], (double)mu);
+        atomic_max_nonneg(&gmax[1], (double)ml);
+    }
 }
 
 // Reciprocal of a complex number (hardware reciprocal of |a|^2).
@@ -939,10 +1094,13 @@ struct RefUnit {
     unsigned long long* ty;        // 2 n x nrhs tagged words handed between sweep blocks
     const double* anorm;           // |A|_inf of the factorised matrix
     int* luflags;                  // [0] bad pivot (factorisation), [1] sweep time-out
+    const int* perm;               // static pivoting order: factor row i is row perm[i] of A
+    const unsigned long long* gmax; // max |U|, max |L|, max |a_ij| (non-negative double bits)
 };
 constexpr int kNormSlots = 4 * kMaxSolveRhs;
 static_assert(kNormSlots == kRefNormSlots, "read-back slots per unit");
 constexpr int kNormA = 3 * kMaxSolveRhs, kNormBad = kNormA + 1, kNormTimeout = kNormA + 2;
+constexpr int kNormGrowthU = kNormA + 3, kNormGrowthL = kNormA + 4, kNormAmax = kNormA + 5;
 
 struct RefBatch {
     int count;
@@ -1184,7 +1342,7 @@ __global__ void ref_scale_kernel(const RefBatch batch)
     }
 }
 
-// Y (FP32) = R * scale[h]
+// Y (FP32) = P R * scale[h]
 __global__ void ref_demote_kernel(const RefBatch batch)
 {
     const int di = find_problem(batch.start, batch.count, blockIdx.x);
@@ -1193,7 +1351,7 @@ __global__ void ref_demote_kernel(const RefBatch batch)
     const int h = blockIdx.y;
     if (h >= U.nrhs || i >= U.n) return;
     const double s = U.scale[h];
-    const double2 r = U.R[i + U.n * h];
+    const double2 r = U.R[(int64_t)U.perm[i] + U.n * h]; // P r: the factors are those of P A
     U.Y[i + U.n * h] = make_float2((float)(r.x * s), (float)(r.y * s));
 }
 
@@ -1278,8 +1436,8 @@ __global__ void ref_residual_reduce_kernel(const RefBatch batch)
     if ((threadIdx.x & 63) == 0) atomic_max_nonneg(&U.norms[h], m);
 }
 
-// Copies |A|_inf and the factorisation's pivot / time-out flags into the norm slots, so that one
-// read-back carries everything the host needs.
+// Copies |A|_inf, the factorisation's pivot / time-out flags and its element growth into the norm
+// slots, so that one read-back carries everything the host needs.
 __global__ void ref_flags_kernel(const RefBatch batch)
 {
     const RefUnit& U = batch.u[blockIdx.x];
@@ -1287,6 +1445,9 @@ __global__ void ref_flags_kernel(const RefBatch batch)
         U.norms[kNormA] = (unsigned long long)__double_as_longlong(*U.anorm);
         U.norms[kNormBad] = (unsigned long long)U.luflags[0];
         U.norms[kNormTimeout] = (unsigned long long)U.luflags[1];
+        U.norms[kNormGrowthU] = U.gmax[0];
+        U.norms[kNormGrowthL] = U.gmax[1];
+        U.norms[kNormAmax] = U.gmax[2];
     }
 }
 
@@ -1324,8 +1485,9 @@ __global__ void wide_init_kernel(int64_t n, int nrhs, const double2* __restrict_
     X[i + ldx * h] = make_double2(0.0, 0.0);
 }
 
-// Y = R / max(rmax, tiny) in FP32; s[h] records the scale.
+// Y = P R / max(rmax, tiny) in FP32; s[h] records the scale.
 __global__ void wide_demote_kernel(int64_t n, int nrhs, const double2* __restrict__ R,
+                                   const int* __restrict__ perm,
                                    const unsigned long long* __restrict__ rmax,
                                    double* __restrict__ scale, float2* __restrict__ Y)
 {
@@ -1335,7 +1497,7 @@ __global__ void wide_demote_kernel(int64_t n, int nrhs, const double2* __restric
     const double sc = (m > 1e-300 && m < INFINITY) ? 1.0 / m : 1.0;
     if (i == 0) scale[h] = sc;
     if (i >= n) return;
-    const double2 r = R[i + n * h];
+    const double2 r = R[(int64_t)perm[i] + n * h];
     Y[i + n * h] = make_float2((float)(r.x * sc), (float)(r.y * sc));
 }
 
@@ -1422,6 +1584,10 @@ struct bem2d_lu32 {
     int* dflag = nullptr;         // [0] bad pivot, [1] sweep time-out
     double* anorm = nullptr;      // |A|_inf
     double* rowpart = nullptr;    // (np / kCastCols) x n partial row sums of |a_ij|
+    int* perm = nullptr;          // np: static pivoting order (factor row i = row perm[i] of A)
+    int* pivwork = nullptr;       // colarg | freepos | freerow, n each
+    unsigned long long* claim = nullptr; // n claims of the rows
+    unsigned long long* gmax = nullptr;  // max |U|, max |L|, max |a_ij|
     RefCache* rcache = nullptr;   // refinement workspace of bem2d_lu32_refine
     // hipGraph of the whole factorisation launch sequence for one matrix buffer (A, lda): built on
     // the second factorisation of the same buffer, replayed from then on
@@ -1438,7 +1604,8 @@ size_t bem2d_internal_lu32_bytes(int64_t n)
 {
     const int64_t nblk = (n + NB - 1) / NB, np = nblk * NB;
     return sizeof(float2) * (size_t)(np * np + 2 * nblk * NB * NB) +
-           sizeof(double) * ((np / kCastCols) * n + 1) + 8 * 256;
+           sizeof(double) * ((np / kCastCols) * n + 1) + sizeof(int) * (np + 3 * n) +
+           sizeof(unsigned long long) * (n + 3) + 12 * 256;
 }
 
 extern "C" void bem2d_lu32_destroy(bem2d_lu32* f)
@@ -1452,6 +1619,10 @@ extern "C" void bem2d_lu32_destroy(bem2d_lu32* f)
     bem2d_internal_dfree(f->arena, f->dflag);
     bem2d_internal_dfree(f->arena, f->anorm);
     bem2d_internal_dfree(f->arena, f->rowpart);
+    bem2d_internal_dfree(f->arena, f->perm);
+    bem2d_internal_dfree(f->arena, f->pivwork);
+    bem2d_internal_dfree(f->arena, f->claim);
+    bem2d_internal_dfree(f->arena, f->gmax);
     if (f->graph) (void)hipGraphExecDestroy(f->graph);
     if (f->rcache) {
         (void)hipFree(f->rcache->base);
@@ -1482,7 +1653,12 @@ int bem2d_internal_lu32_create(bem2d_ctx* c, int64_t n, DevArena* arena, bem2d_l
     f->dflag = (int*)bem2d_internal_dalloc(arena, sizeof(int) * 2);
     f->anorm = (double*)bem2d_internal_dalloc(arena, sizeof(double));
     f->rowpart = (double*)bem2d_internal_dalloc(arena, sizeof(double) * (f->np / kCastCols) * n);
-    if (!f->F || !f->Linv || !f->UinvT || !f->dflag || !f->anorm || !f->rowpart) {
+    f->perm = (int*)bem2d_internal_dalloc(arena, sizeof(int) * f->np);
+    f->pivwork = (int*)bem2d_internal_dalloc(arena, sizeof(int) * 3 * n);
+    f->claim = (unsigned long long*)bem2d_internal_dalloc(arena, sizeof(unsigned long long) * n);
+    f->gmax = (unsigned long long*)bem2d_internal_dalloc(arena, sizeof(unsigned long long) * 3);
+    if (!f->F || !f->Linv || !f->UinvT || !f->dflag || !f->anorm || !f->rowpart || !f->perm ||
+        !f->pivwork || !f->claim || !f->gmax) {
         bem2d_lu32_destroy(f);
         return FAIL(BEM2D_ERR_NOMEM, "bem2d_lu32_create: device allocation of %lld bytes failed",
                     (long long)(8 * n * n));
@@ -1497,6 +1673,11 @@ extern "C" int bem2d_lu32_create(bem2d_ctx* c, int64_t n, bem2d_lu32** out)
 }
 
 extern "C" int64_t bem2d_lu32_size(const bem2d_lu32* f) { return f ? f->n : -1; }
+
+Lu32View bem2d_internal_lu32_view(const bem2d_lu32* f)
+{
+    return Lu32View{f->F, f->n, f->np, f->perm, f->anorm};
+}
 
 static LuProblem lu_problem(bem2d_lu32* f, int b)
 {
@@ -1513,7 +1694,15 @@ static LuProblem lu_problem(bem2d_lu32* f, int b)
     return p;
 }
 
-// Cast, |A|_inf and the factorisation of diagonal block 0 for count problems.
+// Element growth of finished factors (read by the refinement's first read-back).
+static void launch_growth(hipStream_t s, const bem2d_lu32* f)
+{
+    growth_kernel<<<dim3((unsigned)((f->n + 255) / 256), (unsigned)((f->n + 63) / 64)), 256, 0, s>>>(
+        f->n, f->np, f->F, f->gmax);
+}
+
+// Static pivoting order, cast, |A|_inf and the factorisation of diagonal block 0 for count
+// problems.
 int bem2d_internal_lu_begin(hipStream_t s, bem2d_lu32* const* lus, const void* const* A64,
                             const int64_t* lda, int count)
 {
@@ -1524,8 +1713,15 @@ int bem2d_internal_lu_begin(hipStream_t s, bem2d_lu32* const* lus, const void* c
         const int64_t np = f->np;
         const int nchunk = (int)(np / kCastCols);
         zero_u64_kernel<<<1, 64, 0, s>>>((unsigned long long*)f->anorm, 1);
+        // static pivoting order, then the cast of the rows in that order
+        zero_u64_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(f->claim, n);
+        zero_u64_kernel<<<1, 64, 0, s>>>(f->gmax, 3);
+        pivot_colmax_kernel<<<(unsigned)n, 256, 0, s>>>(n, A, lda[d], f->pivwork, f->claim,
+                                                        f->gmax + 2);
+        pivot_assign_kernel<<<1, 1024, 0, s>>>(n, f->pivwork, f->claim, f->perm, f->pivwork + n,
+                                               f->pivwork + 2 * n);
         cast_kernel<<<dim3((unsigned)((np + 255) / 256), (unsigned)nchunk), 256, 0, s>>>(
-            n, np, A, lda[d], f->F, f->rowpart);
+            n, np, A, lda[d], f->perm, f->F, f->rowpart);
         rowsum_max_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(
             n, nchunk, f->rowpart, (unsigned long long*)f->anorm);
         zero_u64_kernel<<<1, 64, 0, s>>>((unsigned long long*)f->dflag, 1);
@@ -1546,9 +1742,12 @@ int bem2d_internal_lu_begin(hipStream_t s, bem2d_lu32* const* lus, const void* c
         }
         diag_lu_kernel<<<bt.count, 256, 0, s>>>(bt);
     }
-    if (hipGetLastError() != hipSuccess) return FAIL(BEM2D_ERR_DEVICE, "LU launch failed");
     for (int d = 0; d < count; ++d)
-        if (lus[d]->nblk == 1) lus[d]->factored = true;
+        if (lus[d]->nblk == 1) {
+            lus[d]->factored = true;
+            launch_growth(s, lus[d]);
+        }
+    if (hipGetLastError() != hipSuccess) return FAIL(BEM2D_ERR_DEVICE, "LU launch failed");
     return BEM2D_OK;
 }
 
@@ -1604,8 +1803,12 @@ int bem2d_internal_lu_advance(hipStream_t s, bem2d_lu32* const* lus, int count)
     if (hipGetLastError() != hipSuccess) return FAIL(BEM2D_ERR_DEVICE, "LU launch failed");
     for (bem2d_lu32* f : act) {
         f->step++;
-        if (f->step == f->nblk - 1) f->factored = true;
+        if (f->step == f->nblk - 1) {
+            f->factored = true;
+            launch_growth(s, f);
+        }
     }
+    if (hipGetLastError() != hipSuccess) return FAIL(BEM2D_ERR_DEVICE, "LU launch failed");
     return BEM2D_OK;
 }
 
@@ -1710,6 +1913,10 @@ struct RefJob {
     std::vector<double> bnorm;            // |b_h|_inf per unit and column (kMaxSolveRhs each)
     std::vector<double> tol;              // backward-error target per unit
     std::vector<int> maxit;               // step limit per unit
+    std::vector<double> prev_omega;       // backward error of the previous read-back per unit
+    std::vector<double> growth;           // element growth of the unit's factors
+    std::vector<char> bad;                // zero / non-finite pivot or growth above the limit
+    std::vector<char> stalled;            // the backward error stopped falling
     bool own_pinned = true;
     std::vector<void*> blocks;            // device allocations of the workspace
     unsigned long long* dnorms = nullptr; // units x kNormSlots
@@ -1750,7 +1957,8 @@ void bem2d_internal_ref_destroy(RefJob* j)
 int bem2d_internal_ref_units(int nrhs) { return (nrhs + kMaxSolveRhs - 1) / kMaxSolveRhs; }
 
 int bem2d_internal_ref_create(bem2d_ctx* c, DevArena* arena, const RefSpec* specs, int count,
-                              unsigned long long* pinned, RefJob** out)
+                              unsigned long long* pinned, unsigned long long* dnorms,
+                              RefJob** out)
 {
     RefJob* j = new RefJob();
     j->ctx = c;
@@ -1784,6 +1992,8 @@ int bem2d_internal_ref_create(bem2d_ctx* c, DevArena* arena, const RefSpec* spec
             u.UinvT = f->UinvT;
             u.anorm = f->anorm;
             u.luflags = f->dflag;
+            u.perm = f->perm;
+            u.gmax = f->gmax;
             const size_t bytes = sizeof(double2) * n * kMaxSolveRhs * (1 + u.chunks) +
                                  sizeof(float2) * n * kMaxSolveRhs +
                                  sizeof(double) * kMaxSolveRhs +
@@ -1808,10 +2018,14 @@ int bem2d_internal_ref_create(bem2d_ctx* c, DevArena* arena, const RefSpec* spec
         }
     }
     const size_t nu = j->units.size();
-    j->dnorms = (unsigned long long*)bem2d_internal_dalloc(arena, sizeof(unsigned long long) *
-                                                                      kNormSlots * nu);
-    if (!j->dnorms) return fail(BEM2D_ERR_NOMEM, "workspace allocation failed");
-    j->blocks.push_back(j->dnorms);
+    if (dnorms) {
+        j->dnorms = dnorms;
+    } else {
+        j->dnorms = (unsigned long long*)bem2d_internal_dalloc(arena, sizeof(unsigned long long) *
+                                                                          kNormSlots * nu);
+        if (!j->dnorms) return fail(BEM2D_ERR_NOMEM, "workspace allocation failed");
+        j->blocks.push_back(j->dnorms);
+    }
     for (size_t u = 0; u < nu; ++u) j->units[u].norms = j->dnorms + kNormSlots * u;
     if (pinned) {
         j->hnorms = pinned;
@@ -1826,6 +2040,10 @@ int bem2d_internal_ref_create(bem2d_ctx* c, DevArena* arena, const RefSpec* spec
     j->active.assign(nu, 1);
     j->iters.assign(nu, 0);
     j->omega.assign(nu, INFINITY);
+    j->prev_omega.assign(nu, INFINITY);
+    j->growth.assign(nu, 0.0);
+    j->bad.assign(nu, 0);
+    j->stalled.assign(nu, 0);
     j->bnorm.assign(nu * kMaxSolveRhs, 0.0);
     *out = j;
     return BEM2D_OK;
@@ -1910,6 +2128,18 @@ static double u64_as_double(unsigned long long v)
     return d;
 }
 
+// Element growth of FP32 factors without interchanges: max(max|U| / max|a_ij|, max|L|).  Above
+// kGrowthLimit the FP32 backward error (about n u32 growth |A|) leaves refinement nothing to
+// converge from, and the solve goes to the pivoted FP64 LU at once.
+constexpr double kGrowthLimit = 1e4;
+constexpr double kStallRatio = 0.5;
+static double factor_growth(double gu, double gl, double amax)
+{
+    if (!(gu == gu) || !(gl == gl) || !(amax == amax)) return INFINITY;
+    const double ru = amax > 0 ? gu / amax : (gu > 0 ? INFINITY : 0.0);
+    return std::max(ru, gl);
+}
+
 // Reads the norms of the last step (waits for it) and retires the units that met tol, that failed
 // (a bad pivot or a sweep time-out) or that used maxit steps.
 // Returns BEM2D_ERR_DEVICE on a sweep time-out.
@@ -1926,9 +2156,14 @@ int bem2d_internal_ref_collect(RefJob* j, bool* all_done)
         if (!j->active[u]) continue;
         const unsigned long long* nm = j->hnorms + kNormSlots * u;
         const RefUnit& U = j->units[u];
-        if (nm[kNormTimeout] || nm[kNormBad]) {
+        const double g = factor_growth(u64_as_double(nm[kNormGrowthU]),
+                                       u64_as_double(nm[kNormGrowthL]),
+                                       u64_as_double(nm[kNormAmax]));
+        j->growth[u] = g;
+        if (nm[kNormTimeout] || nm[kNormBad] || !(g <= kGrowthLimit)) {
             j->active[u] = 0;
             j->omega[u] = INFINITY;
+            j->bad[u] = !nm[kNormTimeout];
             if (nm[kNormTimeout]) rc = BEM2D_ERR_DEVICE;
             continue;
         }
@@ -1946,7 +2181,15 @@ int bem2d_internal_ref_collect(RefJob* j, bool* all_done)
             w = std::max(w, std::isfinite(wh) ? wh : INFINITY);
         }
         j->omega[u] = w;
-        if (w <= j->tol[u] || j->iters[u] >= j->maxit[u]) j->active[u] = 0;
+        if (w <= j->tol[u] || j->iters[u] >= j->maxit[u]) {
+            j->active[u] = 0;
+        } else if (j->iters[u] >= 3 && !(w <= kStallRatio * j->prev_omega[u])) {
+            // plain refinement converges linearly; a step that no longer halves the backward
+            // error marks a factorisation too far from A for it (GMRES-IR or FP64 take over)
+            j->active[u] = 0;
+            j->stalled[u] = 1;
+        }
+        j->prev_omega[u] = w;
         if (j->active[u]) done = false;
     }
     j->bnorm_read = true;
@@ -1965,7 +2208,7 @@ bool bem2d_internal_ref_spec_active(const RefJob* j, int sidx)
 void bem2d_internal_ref_report(const RefJob* j, int sidx, bem2d_ir_report* r)
 {
     double tol = INFINITY;
-    double worst = 0.0;
+    double worst = 0.0, growth = 0.0;
     int it = 0;
     double anorm = 0.0;
     bool any = false;
@@ -1975,18 +2218,21 @@ void bem2d_internal_ref_report(const RefJob* j, int sidx, bem2d_ir_report* r)
         tol = std::min(tol, j->tol[u]);
         it = std::max(it, j->iters[u]);
         anorm = u64_as_double(j->hnorms[kNormSlots * u + kNormA]);
+        growth = std::max(growth, j->growth[u]);
         any = true;
     }
     r->iterations = it;
     r->backward_error = worst;
     r->converged = worst <= tol;
     r->matrix_norm_inf = anorm;
+    r->growth = growth;
 }
 
+// A zero or non-finite pivot, or element growth above kGrowthLimit, in the factors of spec sidx.
 bool bem2d_internal_ref_bad_pivot(const RefJob* j, int sidx)
 {
     for (size_t u = 0; u < j->units.size(); ++u)
-        if (j->unit_spec[u] == sidx && j->hnorms[kNormSlots * u + kNormBad]) return true;
+        if (j->unit_spec[u] == sidx && j->bad[u]) return true;
     return false;
 }
 
@@ -2034,12 +2280,14 @@ static int refine_wide(bem2d_ctx* c, bem2d_lu32* f, const double2* A, int64_t ld
     std::vector<unsigned long long> h(3 * nrhs);
     int dflag[2] = {0, 0};
     double anorm = 0.0;
+    double gm[3] = {0.0, 0.0, 0.0};
     const dim3 g((unsigned)((n + 255) / 256), (unsigned)nrhs);
     wide_init_kernel<<<g, 256, 0, s>>>(n, nrhs, B, ldb, R, X, ldx);
     zero_u64_kernel<<<(unsigned)((3 * nrhs + 255) / 256), 256, 0, s>>>(nm, 3 * nrhs);
     colmax_kernel<<<g, 256, 0, s>>>(n, nrhs, R, n, nm + 2 * nrhs);
     (void)hipMemcpyAsync(&anorm, f->anorm, sizeof(double), hipMemcpyDeviceToHost, s);
     (void)hipMemcpyAsync(dflag, f->dflag, sizeof(int) * 2, hipMemcpyDeviceToHost, s);
+    (void)hipMemcpyAsync(gm, f->gmax, sizeof(double) * 3, hipMemcpyDeviceToHost, s);
     (void)hipMemcpyAsync(h.data(), nm, sizeof(unsigned long long) * 3 * nrhs,
                          hipMemcpyDeviceToHost, s);
     if (hipStreamSynchronize(s) != hipSuccess) {
@@ -2048,11 +2296,14 @@ static int refine_wide(bem2d_ctx* c, bem2d_lu32* f, const double2* A, int64_t ld
     }
     bem2d_ir_report r{};
     r.matrix_norm_inf = anorm;
-    if (dflag[0]) {
+    r.growth = factor_growth(gm[0], gm[1], gm[2]);
+    if (dflag[0] || !(r.growth <= kGrowthLimit)) {
         cleanup();
         r.backward_error = INFINITY;
         if (rep) *rep = r;
-        return FAIL(BEM2D_ERR_NOT_CONVERGED, "FP32 LU met a zero or non-finite pivot");
+        return FAIL(BEM2D_ERR_NOT_CONVERGED, dflag[0] ? "FP32 LU met a zero or non-finite pivot"
+                                                      : "FP32 LU element growth %.3g above %.0g",
+                    r.growth, kGrowthLimit);
     }
     std::vector<double> bnorm(nrhs);
     for (int q = 0; q < nrhs; ++q) {
@@ -2063,11 +2314,11 @@ static int refine_wide(bem2d_ctx* c, bem2d_lu32* f, const double2* A, int64_t ld
                          hipMemcpyDeviceToDevice, s);
     const rocblas_float_complex one32 = {1.f, 0.f};
     const rocblas_double_complex mone = {-1.0, 0.0}, one = {1.0, 0.0};
-    double omega = INFINITY;
+    double omega = INFINITY, prev = INFINITY;
     int it = 0;
     int rc = BEM2D_OK;
     for (; it < maxit; ++it) {
-        wide_demote_kernel<<<g, 256, 0, s>>>(n, nrhs, R, nm, scale, Y);
+        wide_demote_kernel<<<g, 256, 0, s>>>(n, nrhs, R, f->perm, nm, scale, Y);
         rocblas_status st = rocblas_ctrsm(c->blas, rocblas_side_left, rocblas_fill_lower,
                                           rocblas_operation_none, rocblas_diagonal_unit,
                                           (rocblas_int)n, (rocblas_int)nrhs, &one32,
@@ -2107,20 +2358,23 @@ static int refine_wide(bem2d_ctx* c, bem2d_lu32* f, const double2* A, int64_t ld
             const double w = den > 0 ? rn / den : (rn > 0 ? INFINITY : 0.0);
             omega = std::max(omega, std::isfinite(w) ? w : INFINITY);
         }
-        if (omega <= tol) {
-            ++it;
+        if (omega <= tol || (it >= 2 && !(omega <= kStallRatio * prev))) {
+            ++it; // converged, or stalled (GMRES-IR takes over below)
             break;
         }
+        prev = omega;
     }
     cleanup();
     r.iterations = it;
     r.backward_error = omega;
     r.converged = omega <= tol;
+    if (!rc && !r.converged)
+        rc = bem2d_internal_gmres_ir(c, f, A, lda, B, ldb, X, ldx, nrhs, tol, &r);
     if (rep) *rep = r;
     if (rc) return rc;
     if (!r.converged)
         return FAIL(BEM2D_ERR_NOT_CONVERGED, "refinement reached backward error %.3e > %.3e",
-                    omega, tol);
+                    r.backward_error, tol);
     return BEM2D_OK;
 }
 
@@ -2164,7 +2418,7 @@ extern "C" int bem2d_lu32_refine(bem2d_ctx* c, bem2d_lu32* f, const void* A64, i
     rcache->off = 0;
     RefSpec sp{f, A64, lda, B, ldb, X, ldx, nrhs, tol, maxit, rcache};
     RefJob* j = nullptr;
-    int rc = bem2d_internal_ref_create(c, rcache, &sp, 1, rcache->pinned, &j);
+    int rc = bem2d_internal_ref_create(c, rcache, &sp, 1, rcache->pinned, nullptr, &j);
     if (rc) return rc;
     rc = bem2d_internal_refine_sync(c->stream, j);
     bem2d_ir_report r{};
@@ -2182,6 +2436,11 @@ extern "C" int bem2d_lu32_refine(bem2d_ctx* c, bem2d_lu32* f, const void* A64, i
         return FAIL(BEM2D_ERR_NOT_CONVERGED, "FP32 LU met a zero or non-finite pivot");
     }
     if (rc) return rc;
+    if (!r.converged) { // plain refinement stalled or ran out of steps: GMRES-IR from here
+        rc = bem2d_internal_gmres_ir(c, f, A64, lda, B, ldb, X, ldx, nrhs, tol, &r);
+        if (rep) *rep = r;
+        if (rc) return rc;
+    }
     if (!r.converged)
         return FAIL(BEM2D_ERR_NOT_CONVERGED, "refinement reached backward error %.3e > %.3e",
                     r.backward_error, tol);

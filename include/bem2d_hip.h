@@ -180,15 +180,23 @@ int bem2d_solve_fp64(bem2d_ctx* ctx, int64_t n, int64_t nrhs, void* A, int64_t l
 /* ---- mixed-precision solve (FP32 LU on the MFMA cores + FP64 iterative refinement) -------------
  * Not in the reference, whose only solve is SolveSystem; it answers the same question to a stated
  * backward error.  bem2d_lu32 holds FP32 factors of an n x n matrix resident on the device
- * (8 n^2 bytes), so that one factorisation serves many right-hand sides and refinement passes. */
+ * (8 n^2 bytes), so that one factorisation serves many right-hand sides and refinement passes.
+ * Row order: static pivoting -- before the factorisation, which then makes no interchanges, each
+ * column claims the row of its largest entry (contested rows go to the column where the entry is
+ * largest, left-over positions take the unclaimed rows in order), so any row permutation of a
+ * matrix whose column maxima lie in distinct rows is factorised in the order of those maxima.
+ * The refinement reads the element growth of the factors; above 1e4 (or at a zero / non-finite
+ * pivot) it reports BEM2D_ERR_NOT_CONVERGED at once; when plain refinement stalls (a step that
+ * does not halve the backward error) GMRES-IR preconditioned by the FP32 factors continues. */
 typedef struct bem2d_lu32 bem2d_lu32;
 typedef struct {
     int iterations;          /* refinement steps taken (max over right-hand-side groups) */
     int fallback_fp64;       /* 1 if bem2d_solve_mixed fell back to the FP64 LU */
     int converged;           /* 1 if every column met tol */
-    int reserved;
+    int gmres_steps;         /* GMRES iterations of GMRES-IR (0: plain refinement sufficed) */
     double backward_error;   /* max over columns of |b - A x|_inf / (|A|_inf |x|_inf + |b|_inf) */
     double matrix_norm_inf;  /* |A|_inf of the FP64 matrix */
+    double growth;           /* element growth of the FP32 factors, max(max|U| / max|a_ij|, max|L|) */
 } bem2d_ir_report;
 
 int bem2d_lu32_create(bem2d_ctx* ctx, int64_t n, bem2d_lu32** out);
@@ -204,8 +212,9 @@ int64_t bem2d_lu32_graph_replays(const bem2d_lu32* lu);
 int bem2d_lu32_refine(bem2d_ctx* ctx, bem2d_lu32* lu, const void* A64, int64_t lda, const void* B,
                       int64_t ldb, void* X, int64_t ldx, int nrhs, double tol, int maxit,
                       bem2d_ir_report* report);
-/* One-shot: factor, refine, and fall back to bem2d_solve_fp64 if refinement does not converge.
- * B is overwritten with the solution; A64 is unchanged. */
+/* One-shot: factor, refine (GMRES-IR when plain refinement stalls), and fall back to
+ * bem2d_solve_fp64 -- the pivoted LU of SolveSystem -- if the factors are unusable or the
+ * tolerance is missed.  B is overwritten with the solution; A64 is unchanged. */
 int bem2d_solve_mixed(bem2d_ctx* ctx, int64_t n, int64_t nrhs, const void* A64, int64_t lda,
                       void* B, int64_t ldb, double tol, bem2d_ir_report* report);
 
@@ -233,7 +242,11 @@ int bem2d_farfield(bem2d_ctx* ctx, const bem2d_mesh* mesh, double k, const void*
  * preallocated HBM arena and admits queued requests first-in first-out as their footprint fits;
  * each engine step advances every admitted request by one stage with one batched launch per
  * kernel per stage, so requests of different sizes and arrival times share launches.  Results
- * (x and u_inf) are copied to host memory; the device memory returns to the arena at once. */
+ * (x and u_inf) are copied to host memory; the device memory returns to the arena at once.
+ * A mixed request whose FP32 factors are unusable (zero pivot, element growth above 1e4) or whose
+ * refinement stalls or misses its tolerance is solved by the FP64 LU instead (fallback_fp64 in
+ * its result).  A request whose FP64 LU meets an exactly zero pivot (zgetrf info > 0) finishes
+ * with BEM2D_ERR_SINGULAR, the reference's LapackError. */
 #define BEM2D_PRECISION_MIXED 0
 #define BEM2D_PRECISION_FP64 1
 
@@ -258,8 +271,10 @@ typedef struct {
     int64_t n;
     int nrhs;
     int iterations;          /* refinement steps (mixed path) */
-    double backward_error;   /* mixed path: max over columns */
+    double backward_error;   /* mixed path: max over columns (the last refinement step's) */
     double t_arrival, t_admit, t_done; /* engine clock (s): latency = t_done - t_arrival */
+    int fallback_fp64;       /* mixed request solved by the FP64 LU after its FP32 factors had a
+                                zero pivot or too much growth, or refinement stalled */
 } bem2d_result_info;
 
 typedef struct {

@@ -162,3 +162,53 @@ def test_poisson_trace_latency(ctx):
     # continuous batching: fewer LU launches than requests x panel steps
     steps = sum(math.ceil(q["geom"].n / 64) for q in reqs)
     assert st["lu_step_launches"] < steps
+
+
+def test_mixed_and_fp64_requests_share_one_engine(ctx):
+    """MIXED, then a larger FP64 request (it grows the rocSOLVER workspace), then MIXED again in
+    one engine: every result equals its single-problem path and the engine closes cleanly."""
+    print("case mixed and FP64 in one engine", flush=True)
+    reqs = [_req(G.kite(400), 30.0, 2, seed=21),
+            _req(G.kite(1500), 60.0, 3, seed=22, precision=capi.PRECISION_FP64),
+            _req(G.unit_circle(300), 20.0, 9, seed=23),
+            _req(G.kite(2000), 70.0, 1, seed=24, precision=capi.PRECISION_FP64),
+            _req(G.open_arc(500), 40.0, 4, seed=25)]
+    eng = Engine(ctx, 4 * GB)
+    res = []
+    for q in reqs:  # one after another, so each FP64 request finds the other's state behind it
+        res += _run(eng, [q])
+    res_mix = _run(eng, reqs)  # and all at once
+    eng.close()
+    for q, (x, ff, info), (xm, fm, im) in zip(reqs, res, res_mix):
+        assert info.status == capi.OK and im.status == capi.OK
+        xs, ffs = _single(ctx, q)
+        assert rel_fro(x, xs) <= 1e-13 and rel_fro(xm, xs) <= 1e-13, q["geom"].n
+        assert rel_fro(ff, ffs) <= 1e-13 and rel_fro(fm, ffs) <= 1e-13
+
+
+def test_arena_of_one_footprint(ctx):
+    """An arena of exactly one request's footprint serves several such requests one after
+    another (the refinement's read-back slots live outside the arena)."""
+    print("case arena of one footprint", flush=True)
+    reqs = [_req(G.kite(600), 30.0, 3, seed=30 + s) for s in range(3)]
+    eng = Engine(ctx, Engine.footprint(600, 3, 36))
+    res = _run(eng, reqs)
+    eng.close()
+    for x, ff, info in res:
+        assert info.status == capi.OK and info.backward_error <= 1e-12
+
+
+def test_unreachable_tolerance_falls_back_to_fp64(ctx):
+    """A mixed request whose tolerance plain refinement cannot reach (1e-30) stalls, and the
+    engine solves it by the pivoted FP64 LU: status OK, fallback flagged, the FP64 result."""
+    print("case engine FP64 fallback", flush=True)
+    q = _req(G.kite(700), 40.0, 2, seed=41, tol=1e-30)
+    ok = _req(G.kite(500), 30.0, 1, seed=42)
+    eng = Engine(ctx, 2 * GB)
+    (x, ff, info), (x2, ff2, info2) = _run(eng, [q, ok])
+    eng.close()
+    assert info.status == capi.OK and info.fallback_fp64 == 1
+    assert info2.status == capi.OK and info2.fallback_fp64 == 0
+    q64 = dict(q, precision=capi.PRECISION_FP64)
+    xs, ffs = _single(ctx, q64)
+    assert rel_fro(x, xs) <= 1e-13 and rel_fro(ff, ffs) <= 1e-13

@@ -82,16 +82,35 @@ def test_bem_matrices(ctx, geom_name, n, k):
     lu.close()
 
 
-def test_zero_pivot_falls_back_to_fp64(ctx):
+def test_zero_diagonal_entry_needs_no_fallback(ctx):
+    """A zero on the diagonal of an otherwise diagonally weighted matrix: the static row order puts
+    another row's large entry there, and the mixed path converges without the FP64 LU."""
     n = 130
     rng = np.random.default_rng(1)
     A = rng.standard_normal((n, n)) + 4 * np.eye(n) + 0j
-    A[0, 0] = 0.0  # no-pivot LU meets an exactly zero pivot
+    A[0, 0] = 0.0
+    B = rng.standard_normal((n, 2)) + 0j
+    Bd = _dev(B)
+    rep = ctx.solve_mixed(_dev(A), Bd, tol=1e-12)
+    assert rep.fallback_fp64 == 0 and rep.converged == 1
+    assert _backward(A, Bd.cpu().numpy(), B) <= 2e-12
+
+
+def test_zero_pivot_falls_back_to_fp64(ctx):
+    """A leading block [[1, 2], [2, 4 + 1e-10]]: column 1 claims row 1, column 0 takes row 0, and
+    the FP32 cast of 4 + 1e-10 leaves an exactly zero second pivot, while the FP64 matrix is
+    regular.  The one-shot solve falls back to the pivoted FP64 LU."""
+    n = 130
+    rng = np.random.default_rng(1)
+    A = 4 * np.eye(n) + 0.01 * rng.standard_normal((n, n)) + 0j
+    A[:2, :] = 0.0
+    A[:, :2] = 0.0
+    A[:2, :2] = [[1.0, 2.0], [2.0, 4.0 + 1e-10]]
     B = rng.standard_normal((n, 2)) + 0j
     Bd = _dev(B)
     rep = ctx.solve_mixed(_dev(A), Bd, tol=1e-12)
     assert rep.fallback_fp64 == 1
-    assert rel_fro(Bd.cpu().numpy(), np.linalg.solve(A, B)) < 1e-12
+    assert _backward(A, Bd.cpu().numpy(), B) <= 1e-14
 
 
 @pytest.mark.parametrize("nrhs", [64, 360])
@@ -121,18 +140,24 @@ def test_many_right_hand_sides(ctx, nrhs):
 
 
 def test_paired_panels_large_trailing_matrix(ctx):
-    """n = 5440: the first panels pair up (trailing matrices of >= 80 tiles get one depth-128
-    update per two panels), the later ones do not -- both against the FP64 solve."""
+    """n = 5440 (85 tiles) with panels pairing while the trailing matrix has >= 40 tiles: the first
+    panels pair up (one depth-128 update per two panels), the later ones do not -- both against
+    the FP64 solve."""
     n, k = 5440, 60.0
     print("case paired panels n=%d" % n, flush=True)
-    geom = G.kite(n)
-    mesh = ctx.mesh(geom, 16, 3, 0.15)
-    A = ctx.assemble(mesh, capi.KERNEL_COMBINED, k, k, 1.0, 2.0)
-    B = ctx.rhs(mesh, capi.FIELD_COMBINED_PLANE, k, k, [[0.6, 0.8], [1.0, 0.0]], 2.0)
-    X64 = ctx.solve_fp64(A, B.clone())
-    Xm = B.clone()
-    rep = ctx.solve_mixed(A, Xm, tol=1e-12)
-    mesh.close()
+    saved = ctx.tune_lu_pair_tiles()
+    ctx.tune_lu_pair_tiles(40)
+    try:
+        geom = G.kite(n)
+        mesh = ctx.mesh(geom, 16, 3, 0.15)
+        A = ctx.assemble(mesh, capi.KERNEL_COMBINED, k, k, 1.0, 2.0)
+        B = ctx.rhs(mesh, capi.FIELD_COMBINED_PLANE, k, k, [[0.6, 0.8], [1.0, 0.0]], 2.0)
+        X64 = ctx.solve_fp64(A, B.clone())
+        Xm = B.clone()
+        rep = ctx.solve_mixed(A, Xm, tol=1e-12)
+        mesh.close()
+    finally:
+        ctx.tune_lu_pair_tiles(saved)
     assert rep.fallback_fp64 == 0 and rep.backward_error <= 1e-12
     assert (torch.linalg.norm(Xm - X64) / torch.linalg.norm(X64)).item() < 1e-10
 
@@ -142,8 +167,18 @@ def test_factor_graph_replays_match_eager(ctx, n):
     """Repeated factorisations of one buffer replay a captured hipGraph (from the second call on);
     every replay, with new matrix contents and eager work and synchronisations in between, gives
     the same factors as an eager factorisation of the same matrix on a fresh buffer (compared
-    through one correction x = U^-1 L^-1 b).  n = 5440 covers paired panels in the graph."""
+    through one correction x = U^-1 L^-1 b).  n = 5440 (85 tiles), with pairing from 40 tiles,
+    covers paired panels in the graph."""
     print("case factor graph n=%d" % n, flush=True)
+    saved = ctx.tune_lu_pair_tiles()
+    ctx.tune_lu_pair_tiles(40)
+    try:
+        _factor_graph_replays(ctx, n)
+    finally:
+        ctx.tune_lu_pair_tiles(saved)
+
+
+def _factor_graph_replays(ctx, n):
     rng = np.random.default_rng(n + 1)
     Abuf = torch.empty((n, n), dtype=torch.complex128, device="cuda").t()
     B = _dev(rng.standard_normal((n, 1)) + 1j * rng.standard_normal((n, 1)))
@@ -167,3 +202,95 @@ def test_factor_graph_replays_match_eager(ctx, n):
         assert d < 1e-12, (t, d)  # the same kernels on the same data: rounding-level at most
     assert lu.graph_replays() == 3
     lu.close()
+
+
+def _kite_system(ctx, n=1000, k=50.0):
+    geom = G.kite(n)
+    mesh = ctx.mesh(geom, 16, 3, 0.15)
+    A = ctx.assemble(mesh, capi.KERNEL_COMBINED, k, k, 1.0, 2.0)
+    ang = np.array([0.3, 2.0])
+    B = ctx.rhs(mesh, capi.FIELD_COMBINED_PLANE, k, k, np.stack([np.cos(ang), np.sin(ang)], 1), 2.0)
+    torch.cuda.synchronize()
+    mesh.close()
+    return A.cpu().numpy(), B.cpu().numpy()
+
+
+@pytest.mark.parametrize("order", ["reversed", "random"])
+def test_row_permuted_bem_matrix(ctx, oracle, order):
+    """Static pivoting: the rows of the kite N=1000 matrix reversed or randomly permuted (the
+    natural order puts every column's largest entry on the diagonal, so a factorisation without
+    interchanges in the given order would meet tiny pivots).  The mixed path recovers the order,
+    converges without falling back, and matches the oracle's pivoted LU (zgetrf semantics) on the
+    permuted system to 1e-10."""
+    print("case row-permuted kite", order, flush=True)
+    A, B = _kite_system(ctx)
+    n = A.shape[0]
+    p = np.arange(n)[::-1] if order == "reversed" else np.random.default_rng(11).permutation(n)
+    Ap, Bp = A[p, :], B[p, :]
+    xr = oracle.lu_solve(Ap, Bp)
+    Bd = _dev(Bp)
+    rep = ctx.solve_mixed(_dev(Ap), Bd, tol=1e-12)
+    X = Bd.cpu().numpy()
+    print("  iterations %d gmres %d growth %.3g backward %.2e" % (
+        rep.iterations, rep.gmres_steps, rep.growth, rep.backward_error), flush=True)
+    assert rep.fallback_fp64 == 0 and rep.converged == 1
+    assert rep.backward_error <= 1e-12
+    assert rel_fro(X, xr) < 1e-10
+
+
+def test_static_pivoting_undoes_a_row_swap(ctx):
+    """Every column of the BEM matrix claims a different row (its diagonal), so the static order
+    of the matrix with two rows swapped is the natural order again: the factors, hence one
+    correction from x = 0, are bit-identical to those of the unswapped matrix."""
+    A, B = _kite_system(ctx, n=700, k=40.0)
+    n = A.shape[0]
+    p = np.arange(n)
+    p[[5, 600]] = p[[600, 5]]
+    X = []
+    for M, R in ((A, B), (A[p, :], B[p, :])):
+        lu = ctx.lu32(n)
+        Md = _dev(M)
+        lu.factor(Md)
+        Xd = torch.zeros((R.shape[1], n), dtype=torch.complex128, device="cuda").t()
+        lu.refine(Md, _dev(R), Xd, maxit=1, raise_on_fail=False)
+        torch.cuda.synchronize()
+        X.append(Xd.cpu().numpy())
+        lu.close()
+    assert np.array_equal(X[0], X[1])
+
+
+def test_gmres_ir_on_ill_conditioned_matrix(ctx):
+    """cond(A) = 1e7: one FP32 correction gains too little for plain refinement (it stalls), and
+    GMRES-IR preconditioned by the FP32 factors reaches the FP64 target without the FP64 LU."""
+    n = 300
+    rng = np.random.default_rng(3)
+    Q1, _ = np.linalg.qr(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    Q2, _ = np.linalg.qr(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    A = (Q1 * np.logspace(0, -7, n)) @ Q2.conj().T
+    B = rng.standard_normal((n, 2)) + 1j * rng.standard_normal((n, 2))
+    Bd = _dev(B)
+    rep = ctx.solve_mixed(_dev(A), Bd, tol=1e-12)
+    print("  iterations %d gmres %d growth %.3g backward %.2e fallback %d" % (
+        rep.iterations, rep.gmres_steps, rep.growth, rep.backward_error, rep.fallback_fp64),
+        flush=True)
+    assert rep.converged == 1 and rep.backward_error <= 1e-12
+    assert rep.fallback_fp64 == 0 and rep.gmres_steps > 0
+    assert _backward(A, Bd.cpu().numpy(), B) <= 2e-12
+
+
+def test_growth_routes_to_fp64(ctx):
+    """Every column's largest entry in row 0 (a dominant first row): the static order keeps the
+    first row on the diagonal once and leaves tiny pivots behind it, the factors' element growth
+    exceeds 1e4, and the one-shot solve goes to the pivoted FP64 LU without refining."""
+    n = 200
+    rng = np.random.default_rng(5)
+    A = 1e-6 * (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    A[0, :] = 1.0 + rng.uniform(size=n)
+    A[1:, 0] += 1.0
+    B = rng.standard_normal((n, 1)) + 0j
+    Bd = _dev(B)
+    rep = ctx.solve_mixed(_dev(A), Bd, tol=1e-12)
+    print("  growth %.3g fallback %d" % (rep.growth, rep.fallback_fp64), flush=True)
+    assert rep.fallback_fp64 == 1 and rep.growth > 1e4
+    # the FP64 LU is backward stable (the forward error of this cond ~1e8 matrix is not a test)
+    assert _backward(A, Bd.cpu().numpy(), B) <= 1e-14
