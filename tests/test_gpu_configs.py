@@ -1,0 +1,252 @@
+"""The benchmark configurations at their full sizes on the GPU, against the oracle.
+
+C2  kite N=4000, k=100, one plane wave at a seeded angle: the complete oracle matrix (plain and
+    with fused multiply-adds, whose distance is the rounding-noise floor), the solutions of the
+    plain FP64 path and of the mixed path against the oracle's LAPACK zgesv solution, and the
+    mixed far field against the oracle's far field.
+C3  the 64 seeded problems (seed 1) through the engine on both paths: every request solved, the
+    mixed path to backward error 1e-12, far fields within 1e-10 of the FP64 path, and the
+    largest problem's matrix on sampled columns against the oracle.
+C4  a slice of the seeded request stream (seed 2): the largest N, the largest k (~400), requests
+    of 32 right-hand sides and small ones, submitted together; each against the single-problem
+    path and its matrix on sampled columns against the oracle.
+C5  open arc N=32000, k=500, 360 plane waves: sampled columns (both arc ends among them) against
+    the oracle, the mixed solve to backward error 1e-12 on every right-hand side, and its far
+    field within 1e-10 of the FP64 LU's.
+
+Matrix parity bar: relative Frobenius error <= max(1e-12, 2 x the oracle's noise floor) on the
+same columns; solutions the same with the solution-level noise floor.
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from bem2d_amd import Engine, capi, serving
+from bem2d_amd import geometry as G
+from conftest import rel_fro
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+GB = 1 << 30
+_C2 = {}
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _sampled_columns(ctx, oracle, geom, k, A, cols, label):
+    ref = oracle.assemble(geom, 16, 3, 0.15, oracle.K_COMBINED, k, k, 1.0, 2.0, cols=cols)
+    floor = oracle.noise_floor(geom, 16, 3, 0.15, oracle.K_COMBINED, k, k, 1.0, 2.0, cols=cols)
+    got = _np(A[:, torch.tensor(cols, device="cuda")])
+    err = rel_fro(got, ref)
+    print("%s sampled columns %s: err %.2e, oracle noise floor %.2e" % (label, list(cols), err,
+                                                                     floor), flush=True)
+    assert err <= max(TOL, 2 * floor), (label, err, floor)
+
+
+def _oracle_blocks(oracle, geom, k, variant):
+    """The complete oracle matrix, built in column blocks (progress printed per block)."""
+    n = geom.n
+    out = np.empty((n, n), dtype=np.complex128, order="F")
+    step = 500
+    for c0 in range(0, n, step):
+        cols = np.arange(c0, min(n, c0 + step))
+        out[:, cols] = oracle.assemble(geom, 16, 3, 0.15, oracle.K_COMBINED, k, k, 1.0, 2.0,
+                                       cols=cols, variant=variant)
+        print("  oracle (%s) columns %d..%d" % (variant, c0, cols[-1]), flush=True)
+    return out
+
+
+def _c2_gpu(ctx):
+    n, k = 4000, 100.0
+    geom = G.kite(n)
+    mesh = ctx.mesh(geom, 16, 3, 0.15)
+    A = ctx.assemble(mesh, capi.KERNEL_COMBINED, k, k, 1.0, 2.0)
+    th0 = 2 * math.pi * np.random.default_rng(0).uniform()  # the benchmark's seeded angle
+    d = [[math.cos(th0), math.sin(th0)]]
+    B = ctx.rhs(mesh, capi.FIELD_COMBINED_PLANE, k, k, d, 2.0)
+    X64 = ctx.solve_fp64(A, B.clone())
+    lu = ctx.lu32(n)
+    lu.factor(A)
+    Xm = torch.empty_like(B)
+    rep = lu.refine(A, B, Xm, tol=1e-12)
+    lu.close()
+    th = torch.tensor(2 * np.pi * np.arange(360) / 360, device="cuda")
+    ffm = ctx.farfield(mesh, k, Xm, th)
+    ff64 = ctx.farfield(mesh, k, X64, th)
+    torch.cuda.synchronize()
+    out = {"geom": geom, "k": k, "dirs": np.array(d), "A": _np(A), "X64": _np(X64),
+           "Xm": _np(Xm), "ffm": _np(ffm)[:, 0], "ff64": _np(ff64)[:, 0], "th": _np(th),
+           "rep": rep}
+    mesh.close()
+    return out
+
+
+def test_c2_full_matrix_plain_oracle(ctx, oracle):
+    """C2, first half: the GPU matrix and both solutions against the complete oracle matrix and
+    the LAPACK zgesv solution of the oracle's system (the reference's SolveSystem)."""
+    import scipy.linalg.lapack as lapack
+    print("case C2 kite N=4000 k=100: full oracle matrix", flush=True)
+    g = _c2_gpu(ctx)
+    Ao = _oracle_blocks(oracle, g["geom"], g["k"], "plain")
+    bo = oracle.rhs(g["geom"], 16, oracle.F_COMBINED_PLANE, g["k"], g["k"], g["dirs"], 2.0)
+    _, _, xo, info = lapack.zgesv(Ao.copy(order="F"), bo.copy())
+    assert info == 0
+    g.update(Ao=Ao, bo=bo, xo=xo)
+    _C2.update(g)
+    ea = rel_fro(g["A"], Ao)
+    ex64 = rel_fro(g["X64"], xo)
+    exm = rel_fro(g["Xm"], xo)
+    print("C2 |A - A_oracle| %.2e, |x64 - x_oracle| %.2e, |x_mixed - x_oracle| %.2e, "
+          "mixed backward error %.2e" % (ea, ex64, exm, g["rep"].backward_error), flush=True)
+    # the final bounds need the noise floor (next test); these catch gross errors on their own
+    assert ea < 1e-11 and ex64 < 1e-10 and exm < 1e-10
+    assert g["rep"].backward_error <= 1e-12
+
+
+def test_c2_against_noise_floor(ctx, oracle):
+    """C2, second half: the oracle with fused multiply-adds, whose distance from the plain oracle
+    is how far rounding alone moves the reference's own matrix and solution; the GPU results
+    within max(1e-12, 2 x floor), and the mixed far field within 1e-10 of the oracle's."""
+    import scipy.linalg.lapack as lapack
+    if "Ao" not in _C2:
+        pytest.skip("needs test_c2_full_matrix_plain_oracle in the same session")
+    g = _C2
+    print("case C2 noise floor: oracle with FMA", flush=True)
+    Af = _oracle_blocks(oracle, g["geom"], g["k"], "fma")
+    _, _, xf, info = lapack.zgesv(Af.copy(order="F"), g["bo"].copy())
+    assert info == 0
+    floor_a = rel_fro(Af, g["Ao"])
+    floor_x = rel_fro(xf, g["xo"])
+    ea = rel_fro(g["A"], g["Ao"])
+    ex64 = rel_fro(g["X64"], g["xo"])
+    exm = rel_fro(g["Xm"], g["xo"])
+    ffo = oracle.farfield(g["geom"], 16, g["k"], g["xo"], g["th"])[:, 0]
+    eff = rel_fro(g["ffm"], ffo)
+    eff64 = rel_fro(g["ff64"], ffo)
+    print("C2 matrix: err %.2e floor %.2e | x FP64: err %.2e, x mixed: err %.2e, floor %.2e | "
+          "far field mixed %.2e, FP64 %.2e" % (ea, floor_a, ex64, exm, floor_x, eff, eff64),
+          flush=True)
+    assert ea <= max(TOL, 2 * floor_a)
+    assert ex64 <= max(TOL, 2 * floor_x)
+    assert exm <= max(1e-10, 2 * floor_x)
+    assert eff <= 1e-10
+    _C2.clear()
+
+
+def test_c3_all_problems_through_the_engine(ctx, oracle):
+    print("case C3: 64 seeded problems", flush=True)
+    res = {}
+    for prec in (capi.PRECISION_MIXED, capi.PRECISION_FP64):
+        reqs = serving.batched_problems(64, seed=1, precision=prec)
+        eng = Engine(ctx, 64 * GB, max_active=64)
+        tickets = [eng.submit(q) for q in reqs]
+        while eng.step() > 0:
+            pass
+        res[prec] = [eng.result(t) for t in tickets]
+        eng.close()
+    worst_bw, worst_ff = 0.0, 0.0
+    for (xm, fm, im), (x6, f6, i6) in zip(res[capi.PRECISION_MIXED], res[capi.PRECISION_FP64]):
+        assert im.status == capi.OK and i6.status == capi.OK
+        assert im.fallback_fp64 == 0 and im.backward_error <= 1e-12
+        worst_bw = max(worst_bw, im.backward_error)
+        worst_ff = max(worst_ff, rel_fro(fm, f6))
+    print("C3 worst backward error %.2e, worst far field mixed vs FP64 %.2e" % (worst_bw, worst_ff),
+          flush=True)
+    assert worst_ff <= 1e-10
+    # the largest problem: engine result against the single-problem path, matrix against the oracle
+    reqs = serving.batched_problems(64, seed=1)
+    q = max(reqs, key=lambda r: r["geom"].n)
+    i = reqs.index(q)
+    mesh = ctx.mesh(q["geom"], 16, 3, 0.15)
+    A = ctx.assemble(mesh, capi.KERNEL_COMBINED, q["k"], q["eta"], 1.0, 2.0)
+    B = ctx.rhs(mesh, capi.FIELD_COMBINED_PLANE, q["k"], q["eta"], q["dirs"], 2.0)
+    lu = ctx.lu32(q["geom"].n)
+    lu.factor(A)
+    X = torch.empty_like(B)
+    lu.refine(A, B, X, tol=1e-12)
+    lu.close()
+    assert rel_fro(res[capi.PRECISION_MIXED][i][0], _np(X)) <= 1e-13
+    n = q["geom"].n
+    _sampled_columns(ctx, oracle, q["geom"], q["k"], A, np.array([0, 1, n // 2, n - 2, n - 1]),
+                     "C3 largest (%s, N=%d, k=%.1f)" % (q["name"], n, q["k"]))
+    mesh.close()
+
+
+def test_c4_trace_slice(ctx, oracle):
+    print("case C4: slice of the seeded request stream", flush=True)
+    reqs, _ = serving.serving_trace(2000, rate=27.0, seed=2)
+    ns = np.array([q["geom"].n for q in reqs])
+    ks = np.array([q["k"] for q in reqs])
+    nr = np.array([q["dirs"].shape[0] for q in reqs])
+    pick = [int(ns.argmax()), int(ks.argmax())] + [int(i) for i in np.where(nr == 32)[0][:2]]
+    pick += [int(i) for i in np.argsort(ns)[:2]]
+    pick = list(dict.fromkeys(pick))
+    sl = [reqs[i] for i in pick]
+    print("  requests", [(reqs[i]["name"], int(ns[i]), round(ks[i], 1), int(nr[i])) for i in pick],
+          flush=True)
+    eng = Engine(ctx, 64 * GB, max_active=64)
+    tickets = [eng.submit(q) for q in sl]
+    while eng.step() > 0:
+        pass
+    res = [eng.result(t) for t in tickets]
+    eng.close()
+    for q, (x, ff, info) in zip(sl, res):
+        assert info.status == capi.OK and info.backward_error <= 1e-12 and info.fallback_fp64 == 0
+        n = q["geom"].n
+        mesh = ctx.mesh(q["geom"], 16, 3, 0.15)
+        A = ctx.assemble(mesh, capi.KERNEL_COMBINED, q["k"], q["eta"], 1.0, 2.0)
+        B = ctx.rhs(mesh, capi.FIELD_COMBINED_PLANE, q["k"], q["eta"], q["dirs"], 2.0)
+        lu = ctx.lu32(n)
+        lu.factor(A)
+        X = torch.empty_like(B)
+        lu.refine(A, B, X, tol=1e-12)
+        lu.close()
+        X64 = ctx.solve_fp64(A, B.clone())
+        th = torch.tensor(q["theta"], device="cuda")
+        f64 = _np(ctx.farfield(mesh, q["k"], X64, th))
+        assert rel_fro(x, _np(X)) <= 1e-13, n
+        assert rel_fro(ff, f64) <= 1e-10, n
+        _sampled_columns(ctx, oracle, q["geom"], q["k"], A, np.array([0, n // 3, n - 1]),
+                         "C4 %s N=%d k=%.1f nrhs=%d" % (q["name"], n, q["k"], q["dirs"].shape[0]))
+        mesh.close()
+        del A, B, X, X64
+    torch.cuda.empty_cache()
+
+
+def test_c5_open_arc_full_size(ctx, oracle):
+    n, k, nrhs = 32000, 500.0, 360
+    print("case C5 open arc N=%d k=%g, %d right-hand sides" % (n, k, nrhs), flush=True)
+    geom = G.open_arc(n, 0.3)
+    mesh = ctx.mesh(geom, 16, 3, 0.15)
+    A = ctx.assemble(mesh, capi.KERNEL_COMBINED, k, k, 1.0, 2.0)
+    rng = np.random.default_rng(5)
+    cols = np.unique(np.concatenate([[0, 1, n // 2, n - 2, n - 1],
+                                     rng.choice(n, size=11, replace=False)]))
+    _sampled_columns(ctx, oracle, geom, k, A, cols, "C5")
+    th = 2 * np.pi * np.arange(nrhs) / nrhs
+    B = ctx.rhs(mesh, capi.FIELD_COMBINED_PLANE, k, k, np.stack([np.cos(th), np.sin(th)], 1), 2.0)
+    lu = ctx.lu32(n)
+    lu.factor(A)
+    X = torch.empty_like(B)
+    rep = lu.refine(A, B, X, tol=1e-12)
+    lu.close()
+    print("C5 mixed: %d refinement steps, backward error %.2e, growth %.3g" % (
+        rep.iterations, rep.backward_error, rep.growth), flush=True)
+    assert rep.converged == 1 and rep.backward_error <= 1e-12
+    X64 = B.clone()
+    ctx.solve_fp64(A, X64, preserve_A=True)
+    ang = torch.tensor(2 * np.pi * np.arange(360) / 360, device="cuda")
+    fm = ctx.farfield(mesh, k, X, ang)
+    f64 = ctx.farfield(mesh, k, X64, ang)
+    e_ff = (torch.linalg.norm(fm - f64) / torch.linalg.norm(f64)).item()
+    e_x = (torch.linalg.norm(X - X64) / torch.linalg.norm(X64)).item()
+    print("C5 far field mixed vs FP64 %.2e, solutions %.2e" % (e_ff, e_x), flush=True)
+    assert e_ff <= 1e-10 and e_x <= 1e-10
+    mesh.close()
+    del A, B, X, X64
+    torch.cuda.empty_cache()
