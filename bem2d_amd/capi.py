@@ -17,7 +17,7 @@ ELEM_FLAT, ELEM_CIRCLE, ELEM_TREFOIL, ELEM_KITE, ELEM_ELLIPSE, ELEM_ELLIPSE_ARC,
     range(7)
 KERNEL_NONE, KERNEL_SL, KERNEL_DL, KERNEL_CDL, KERNEL_COMBINED, KERNEL_ID = range(6)
 FIELD_PLANE, FIELD_NORMAL_PLANE, FIELD_COMBINED_PLANE = range(3)
-PROF_ASSEMBLE_REGULAR, PROF_ASSEMBLE_SPECIAL, PROF_LU = range(3)
+PROF_ASSEMBLE_REGULAR, PROF_ASSEMBLE_SPECIAL, PROF_LU, PROF_LU_UPDATE = range(4)
 
 # Exported symbols, with (restype, argtypes).  Kept in the order of include/bem2d_hip.h.
 _P = ctypes.c_void_p
@@ -48,6 +48,7 @@ SIGNATURES = {
     "bem2d_ctx_reset_stream": (_I, [_P]),
     "bem2d_ctx_stream": (_P, [_P]),
     "bem2d_synchronize": (_I, [_P]),
+    "bem2d_ctx_release_scratch": (_I, [_P]),
     "bem2d_last_error": (ctypes.c_char_p, []),
     "bem2d_device_alloc": (_I, [_P, _I64, ctypes.POINTER(_P)]),
     "bem2d_device_free": (_I, [_P, _P]),
@@ -230,6 +231,13 @@ class Context:
     def profile(self, on=True):
         check(lib().bem2d_profile_enable(self.h, 1 if on else 0))
         check(lib().bem2d_profile_reset(self.h))
+
+    def profile_reset(self):
+        check(lib().bem2d_profile_reset(self.h))
+
+    def release_scratch(self):
+        """Frees the context's scratch buffers (they grow back on demand)."""
+        check(lib().bem2d_ctx_release_scratch(self.h))
 
     def profile_read(self, kernel_id):
         """(total ms, launches) of an instrumented kernel since the last reset."""

@@ -212,6 +212,19 @@ extern "C" int bem2d_ctx_reset_stream(bem2d_ctx* c)
 
 extern "C" void* bem2d_ctx_stream(bem2d_ctx* c) { return c ? (void*)c->stream : nullptr; }
 
+extern "C" int bem2d_ctx_release_scratch(bem2d_ctx* c)
+{
+    if (!c) return FAIL(BEM2D_ERR_ARG, "null context");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (auto& s : c->scratch) {
+        if (s.ptr) HIPCHK(hipFree(s.ptr));
+        s.ptr = nullptr;
+        s.bytes = 0;
+    }
+    return BEM2D_OK;
+}
+
 extern "C" int bem2d_synchronize(bem2d_ctx* c)
 {
     if (!c) return FAIL(BEM2D_ERR_ARG, "null context");

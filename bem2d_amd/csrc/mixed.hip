@@ -1798,7 +1798,14 @@ int bem2d_internal_lu_advance(hipStream_t s, bem2d_lu32* const* lus, int count)
         }
         if (up.start[cnt] > (int64_t)INT32_MAX) return FAIL(BEM2D_ERR_UNSUPPORTED, "LU grid too large");
         panel_trsm_kernel<<<(unsigned)tr.start[cnt], 256, 0, s>>>(tr);
+        // HIP events around the update when the context profiles (not inside a graph capture)
+        bem2d_ctx* pc = act[c0]->ctx;
+        hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+        const bool prof = pc->profiling && hipStreamIsCapturing(s, &cap) == hipSuccess &&
+                          cap == hipStreamCaptureStatusNone;
+        if (prof) bem2d_internal_prof_begin(pc, BEM2D_PROF_LU_UPDATE, s);
         trailing_update_kernel<<<(unsigned)up.start[cnt], 256, 0, s>>>(up);
+        if (prof) bem2d_internal_prof_end(pc, BEM2D_PROF_LU_UPDATE, s);
     }
     if (hipGetLastError() != hipSuccess) return FAIL(BEM2D_ERR_DEVICE, "LU launch failed");
     for (bem2d_lu32* f : act) {

@@ -1,19 +1,23 @@
 """Benchmark of the bem2d hot path on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-extras]
-    python bench.py --config batched|serving|large
+    python bench.py --config large|kite|batched|serving
 
-Default workload (one step): sound-soft scattering by the kite x = cos t + 0.65 cos 2t - 0.65,
-y = 1.5 sin t at k = 100 with the combined-field operator (eta = k), N = 4000 piecewise-constant
-elements at uniform parameter spacing, 16-point Gauss with the singular rules at L = 3,
-sigma = 0.15, and one plane-wave right-hand side at a seeded random angle: Galerkin assembly
-A = M + 2K, the right-hand side b = 2<f, phi>, and the dense solve.  Inputs (geometry tables) are
-resident in HBM before the timed region starts.  The same line carries, unless --no-extras, a
-batched run (64 seeded problems) and a short served trace (Poisson arrivals; p50/p99 latency) on
-the mixed and the plain FP64 path.  Prints one JSON line.
+Headline workload (one step): the large multi-right-hand-side configuration -- sound-soft
+scattering by an open circular-arc cavity (radius 1, aperture 0.3 rad centred on +x) at k = 500,
+combined-field operator (eta = k), N = 32000 piecewise-constant elements at uniform parameter
+spacing, 16-point Gauss with the singular rules at L = 3, sigma = 0.15, and 360 plane-wave
+right-hand sides at equispaced angles: Galerkin assembly A = M + 2K (FP64), the right-hand sides
+b = 2<f, phi>, the FP32 MFMA LU and FP64 refinement to backward error 1e-12 (the mixed path).  The
+plain FP64 path (FP64 assembly + rocSOLVER zgetrf/zgetrs) is timed beside it.  Geometry tables are
+resident in HBM before the timed region starts.  value = right-hand sides solved per second.
 
---config batched / serving / large prints the line of the 64-problem batch, the full 2000-request
-served trace, or the N = 32000 open arc with 360 right-hand sides instead.
+The same line carries, unless --no-extras, the kite configuration (N = 4000, k = 100, one right-hand
+side) on both paths, the 64-problem batch, the full 2000-request served trace (Poisson arrivals;
+p50 / p99 latency) and eight kite problems served together.  Progress lines go to stderr; stdout
+carries the one JSON line.
+
+--config large / kite / batched / serving prints the line of that configuration alone.
 """
 import argparse
 import json
@@ -34,6 +38,33 @@ PEAK_FP32_MFMA_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "traffic_assemble_regular_kite4000.md")
 FP64_PROFILE = os.path.join(ROOT, "profiles", "fp64_assemble_regular_kite4000.md")
+LU_TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "traffic_trailing_update_arc32000.md")
+ASM_C5_PROFILE = os.path.join(ROOT, "profiles", "fp64_assemble_regular_arc32000.md")
+
+
+def progress(msg):
+    print("bench: " + msg, file=sys.stderr, flush=True)
+
+
+def lu_update_flops(n, pair_tiles):
+    """Algorithmic FLOP of the FP32 LU's trailing-update launches for an n x n matrix: 8 real FLOP
+    per complex multiply-add, in the launch schedule of bem2d_internal_lu_advance (64-wide panels on
+    the matrix bordered to whole 64-blocks; while the trailing matrix has >= pair_tiles tiles two
+    panels share one depth-128 update, the first of them updating only the cross -- the next block
+    row and column).  Returns (FLOP, launches)."""
+    nblk = (n + 63) // 64
+    total, pending = 0, False
+    for b in range(nblk - 1):
+        tiles = nblk - b - 1
+        if pending:
+            total += 8 * 128 * (64 * tiles) ** 2
+            pending = False
+        elif b + 2 <= nblk - 1 and tiles >= pair_tiles:
+            total += 8 * 64 * 64 * 64 * (2 * tiles - 1)
+            pending = True
+        else:
+            total += 8 * 64 * (64 * tiles) ** 2
+    return total, nblk - 1
 
 
 def profile_value(path, key):
@@ -71,26 +102,6 @@ def pair_points(n, nspecial, nq=16, L=3):
     n_adj = nspecial - n
     n_reg = n * n - nspecial
     return n_reg * nq * nq, n_ident * 2 * nq * nq * L * L + n_adj * 3 * nq * nq * L, n_reg
-
-
-def cpu_baseline(geom, A_host, k, b_host, ncols, seed=7):
-    """The oracle (C restatement, OpenMP over columns) timed on a seeded sample of columns and
-    extrapolated to the full matrix, plus LAPACK zgesv (scipy's OpenBLAS) on the full matrix."""
-    from oracle import oracle as O
-    import scipy.linalg.lapack as lapack
-    n = geom.n
-    rng = np.random.default_rng(seed)
-    cols = np.sort(rng.choice(n, size=ncols, replace=False))
-    O.assemble(geom, 16, 3, 0.15, O.K_COMBINED, k, k, 1.0, 2.0, cols=cols[:2])  # warm
-    t0 = time.perf_counter()
-    blk = O.assemble(geom, 16, 3, 0.15, O.K_COMBINED, k, k, 1.0, 2.0, cols=cols)
-    t_cols = time.perf_counter() - t0
-    t_asm = t_cols * n / ncols
-    err = np.linalg.norm(A_host[:, cols] - blk) / np.linalg.norm(blk)
-    t0 = time.perf_counter()
-    _, _, x, info = lapack.zgesv(np.asfortranarray(A_host), b_host.copy())
-    t_solve = time.perf_counter() - t0
-    return t_asm, t_solve, t_cols, err
 
 
 def run_batched(ctx, count=64, seed=1):
@@ -188,7 +199,9 @@ def run_serving(ctx, count, rate, seed=2):
         out[name] = {"p50_ms": lat["p50_ms"], "p99_ms": lat["p99_ms"], "mean_ms": lat["mean_ms"],
                      "solves_per_s": nrhs / dt, "requests_per_s": count / dt, "wall_s": dt,
                      "failed": sum(i.status != 0 for i in infos),
+                     "fallback_fp64": sum(i.fallback_fp64 for i in infos),
                      "max_backward_error": max(i.backward_error for i in infos)}
+        progress("C4 trace, %s path: %d requests in %.1f s" % (name, count, dt))
     out["mixed_over_plain_solves"] = out["mixed"]["solves_per_s"] / out["plain_fp64"]["solves_per_s"]
     # capacity: the same requests all present at once, so each path runs as fast as it can
     for prec, name in ((capi.PRECISION_MIXED, "mixed"), (capi.PRECISION_FP64, "plain_fp64")):
@@ -203,14 +216,40 @@ def run_serving(ctx, count, rate, seed=2):
         out[name]["capacity_solves_per_s"] = nrhs / dt
         out[name]["capacity_requests_per_s"] = count / dt
         out[name]["capacity_failed"] = sum(r[2].status != 0 for r in res)
+        progress("C4 capacity, %s path: %.1f s" % (name, dt))
     out["mixed_over_plain_capacity"] = (out["mixed"]["capacity_solves_per_s"] /
                                         out["plain_fp64"]["capacity_solves_per_s"])
     return out
 
 
-def run_large(ctx, n=32000, k=500.0, nrhs=360):
-    """C5: open arc (aperture 0.3 rad), N = 32000, k = 500, 360 plane waves."""
-    import time
+def cpu_baseline_large(geom, k, nrhs, ncols=16, n_lu=8000, seed=7):
+    """The oracle (C restatement, OpenMP over columns) timed on a seeded sample of ncols columns of
+    the N = 32000 matrix and extrapolated to all columns, plus LAPACK zgesv (scipy's OpenBLAS) of an
+    n_lu x n_lu system with the same right-hand sides, extrapolated by (N / n_lu)^3."""
+    from oracle import oracle as O
+    import scipy.linalg.lapack as lapack
+    n = geom.n
+    rng = np.random.default_rng(seed)
+    cols = np.sort(rng.choice(n, size=ncols, replace=False))
+    O.assemble(geom, 16, 3, 0.15, O.K_COMBINED, k, k, 1.0, 2.0, cols=cols[:1])  # warm
+    t0 = time.perf_counter()
+    blk = O.assemble(geom, 16, 3, 0.15, O.K_COMBINED, k, k, 1.0, 2.0, cols=cols)
+    t_cols = time.perf_counter() - t0
+    M = (rng.standard_normal((n_lu, n_lu)) + 1j * rng.standard_normal((n_lu, n_lu))) / np.sqrt(n_lu)
+    M += 4 * np.eye(n_lu)
+    R = rng.standard_normal((n_lu, nrhs)) + 1j * rng.standard_normal((n_lu, nrhs))
+    M = np.asfortranarray(M)
+    t0 = time.perf_counter()
+    _, _, _, info = lapack.zgesv(M, R, overwrite_a=True, overwrite_b=True)
+    t_lu = time.perf_counter() - t0
+    return {"cols": cols, "block": blk, "t_cols": t_cols, "t_asm": t_cols * n / ncols,
+            "t_lu_small": t_lu, "t_lu": t_lu * (n / n_lu) ** 3, "n_lu": n_lu}
+
+
+def run_large(ctx, steps, warmup, n=32000, k=500.0, nrhs=360, plain_steps=3, cpu=True):
+    """C5, the headline: open arc (aperture 0.3 rad), N = 32000, k = 500, 360 plane waves.  One
+    step = FP64 assembly + right-hand sides + FP32 MFMA LU + FP64 refinement to 1e-12; the plain
+    FP64 path (assembly + rocSOLVER zgetrf/zgetrs) timed over plain_steps steps beside it."""
     import torch
     from bem2d_amd import capi
     from bem2d_amd import geometry as G
@@ -219,118 +258,184 @@ def run_large(ctx, n=32000, k=500.0, nrhs=360):
     th = 2 * np.pi * np.arange(nrhs) / nrhs
     dirs = np.stack([np.cos(th), np.sin(th)], 1)
     A = torch.empty((n, n), dtype=torch.complex128, device="cuda").t()
-    ctx.profile(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ctx.assemble(mesh, capi.KERNEL_COMBINED, k, k, 1.0, 2.0, out=A)
-    torch.cuda.synchronize()
-    t_asm = time.perf_counter() - t0
-    reg_ms, _ = ctx.profile_read(capi.PROF_ASSEMBLE_REGULAR)
-    ctx.profile(False)
-    B = ctx.rhs(mesh, capi.FIELD_COMBINED_PLANE, k, k, dirs, 2.0)
-    lu = ctx.lu32(n)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    lu.factor(A)
-    torch.cuda.synchronize()
-    t_lu = time.perf_counter() - t0
+    B = torch.empty((nrhs, n), dtype=torch.complex128, device="cuda").t()
     X = torch.empty_like(B)
+    lu = ctx.lu32(n)
+
+    def step_mixed():
+        ctx.assemble(mesh, capi.KERNEL_COMBINED, k, k, 1.0, 2.0, out=A)
+        ctx.rhs(mesh, capi.FIELD_COMBINED_PLANE, k, k, dirs, 2.0, out=B)
+        lu.factor(A)
+        return lu.refine(A, B, X, tol=1e-12, raise_on_fail=False)
+
+    for _ in range(warmup):
+        step_mixed()
+    torch.cuda.synchronize()
+    ctx.profile(True)
+    ctx.profile_reset()
     t0 = time.perf_counter()
+    reports = [step_mixed() for _ in range(steps)]
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    reg_ms, reg_n = ctx.profile_read(capi.PROF_ASSEMBLE_REGULAR)
+    sp_ms, sp_n = ctx.profile_read(capi.PROF_ASSEMBLE_SPECIAL)
+    lu_ms, lu_n = ctx.profile_read(capi.PROF_LU)
+    up_ms, up_n = ctx.profile_read(capi.PROF_LU_UPDATE)
+    ctx.profile(False)
+    progress("C5 mixed path: %d steps in %.2f s" % (steps, dt))
+
+    # the plain FP64 path: assembly, right-hand sides, zgetrf + zgetrs (A overwritten)
+    X64 = torch.empty_like(B)
+
+    def step_plain():
+        ctx.assemble(mesh, capi.KERNEL_COMBINED, k, k, 1.0, 2.0, out=A)
+        ctx.rhs(mesh, capi.FIELD_COMBINED_PLANE, k, k, dirs, 2.0, out=X64)
+        ctx.solve_fp64(A, X64, preserve_A=False)
+
+    step_plain()
+    torch.cuda.synchronize()
+    ctx.profile(True)
+    ctx.profile_reset()
+    t0 = time.perf_counter()
+    for _ in range(plain_steps):
+        step_plain()
+    torch.cuda.synchronize()
+    dt64 = time.perf_counter() - t0
+    lu64_ms, lu64_n = ctx.profile_read(capi.PROF_LU)
+    ctx.profile(False)
+    progress("C5 plain FP64 path: %d steps in %.2f s" % (plain_steps, dt64))
+
+    # checks outside the timed regions: both solutions of one assembly, far fields at 360 angles
+    ctx.assemble(mesh, capi.KERNEL_COMBINED, k, k, 1.0, 2.0, out=A)
+    ctx.rhs(mesh, capi.FIELD_COMBINED_PLANE, k, k, dirs, 2.0, out=B)
+    lu.factor(A)
     rep = lu.refine(A, B, X, tol=1e-12, raise_on_fail=False)
-    torch.cuda.synchronize()
-    t_ref = time.perf_counter() - t0
-    lu.close()
-    X64 = B.clone()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
+    X64.copy_(B)
     ctx.solve_fp64(A, X64, preserve_A=True)
-    torch.cuda.synchronize()
-    t64 = time.perf_counter() - t0
     ang = torch.tensor(2 * np.pi * np.arange(360) / 360, device="cuda")
     fm = ctx.farfield(mesh, k, X, ang)
     f64 = ctx.farfield(mesh, k, X64, ang)
     e_ff = (torch.linalg.norm(fm - f64) / torch.linalg.norm(f64)).item()
     e_x = (torch.linalg.norm(X - X64) / torch.linalg.norm(X64)).item()
-    P_reg, P_sing, _ = pair_points(n, mesh.special_count)
+    P_reg, P_sing, n_reg = pair_points(n, mesh.special_count)
+    cpu_res = None
+    if cpu:
+        cpu_res = cpu_baseline_large(geom, k, nrhs)
+        got = A[:, torch.tensor(cpu_res["cols"], device="cuda")].cpu().numpy()
+        cpu_res["gpu_vs_oracle_cols"] = float(np.linalg.norm(got - cpu_res["block"]) /
+                                              np.linalg.norm(cpu_res["block"]))
+        progress("C5 CPU baseline sample done")
+    lu.close()
     mesh.close()
     del A, B, X, X64
+    ctx.release_scratch()
     torch.cuda.empty_cache()
+
+    pair_tiles = ctx.tune_lu_pair_tiles()
+    upd_flop, upd_launches = lu_update_flops(n, pair_tiles)
     flu = 8.0 / 3.0 * n ** 3
-    return {"workload": "open arc (radius 1, aperture 0.3 rad) N=%d k=%g, %d plane waves" % (n, k, nrhs),
-            "assembly_s": t_asm, "assembly_regular_ms": reg_ms,
-            "gpairpts_per_s": (P_reg + P_sing) / t_asm / 1e9,
-            "lu32_s": t_lu, "lu32_tflops": flu / t_lu / 1e12,
-            "lu32_frac_of_fp32_mfma_peak": flu / t_lu / 1e12 / PEAK_FP32_MFMA_TFLOPS,
-            "refine_s": t_ref, "refine_steps": rep.iterations, "backward_error": rep.backward_error,
-            "fp64_solve_s": t64, "fp64_lu_tflops": flu / t64 / 1e12,
-            "mixed_solve_s": t_lu + t_ref,
-            "solves_per_s_mixed": nrhs / (t_asm + t_lu + t_ref),
-            "solves_per_s_plain_fp64": nrhs / (t_asm + t64),
-            "x_rel_err_mixed_vs_fp64": e_x, "farfield_rel_err_mixed_vs_fp64": e_ff}
+    reg_avg = reg_ms / max(reg_n, 1)
+    up_avg = up_ms / max(up_n, 1)
+    lu_avg = lu_ms / max(lu_n, 1)
+    achieved_upd = upd_flop * steps / (up_ms * 1e-3) / 1e12 if up_ms > 0 else None
+    asm_achieved = FLOP_PER_PAIRPOINT * P_reg / (reg_avg * 1e-3) / 1e12
+    out = {
+        "workload": "open arc (radius 1, aperture 0.3 rad) N=%d k=%g combined-field (eta=k), P0, "
+                    "16-pt Gauss, L=3 sigma=0.15, %d plane waves at equispaced angles; one step = "
+                    "FP64 assembly A=M+2K + right-hand sides + FP32 MFMA LU + FP64 refinement to "
+                    "backward error 1e-12" % (n, k, nrhs),
+        "n": n, "k": k, "nrhs": nrhs,
+        "steps": steps, "warmup": warmup, "time_s": dt,
+        "solves_per_s_mixed": nrhs * steps / dt,
+        "ms_per_step_mixed": 1e3 * dt / steps,
+        "solves_per_s_plain_fp64": nrhs * plain_steps / dt64,
+        "ms_per_step_plain_fp64": 1e3 * dt64 / plain_steps,
+        "plain_steps": plain_steps,
+        "mixed_over_plain": (nrhs * steps / dt) / (nrhs * plain_steps / dt64),
+        "converged_all_steps": all(r.converged for r in reports),
+        "fallback_any_step": any(r.fallback_fp64 for r in reports),
+        "backward_error_max": max(r.backward_error for r in reports),
+        "refinement_steps_max": max(r.iterations for r in reports),
+        "growth": rep.growth,
+        "x_rel_err_mixed_vs_fp64": e_x,
+        "farfield_rel_err_mixed_vs_fp64": e_ff,
+        "assembly_regular_ms": reg_avg,
+        "assembly_special_ms": sp_ms / max(sp_n, 1),
+        "gentries_per_s": n * n / (reg_avg * 1e-3) / 1e9,
+        "lu32_ms": lu_avg,
+        "lu32_tflops": flu / (lu_avg * 1e-3) / 1e12,
+        "lu32_frac_of_fp32_mfma_peak": flu / (lu_avg * 1e-3) / 1e12 / PEAK_FP32_MFMA_TFLOPS,
+        "lu_fp64_ms": lu64_ms / max(lu64_n, 1),
+        "lu_fp64_tflops": flu / (lu64_ms / max(lu64_n, 1) * 1e-3) / 1e12,
+        "roofline": {
+            "kernel": "trailing_update_kernel (FP32 LU, complex rank-64/128 update on "
+                      "v_mfma_f32_16x16x4f32)",
+            "bound": "mfma",
+            "achieved": achieved_upd,
+            "peak": PEAK_FP32_MFMA_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": None if achieved_upd is None else achieved_upd / PEAK_FP32_MFMA_TFLOPS,
+            "traffic": profile_value(LU_TRAFFIC_PROFILE, "traffic_bytes_per_launch"),
+            "work_per_launch": "%.6e FLOP per factorisation over %d launches = %.6e per launch "
+                               "(8 real FLOP per complex multiply-add; panel pairing from %d "
+                               "tiles)" % (upd_flop, upd_launches, upd_flop / upd_launches,
+                                           pair_tiles),
+            "launches": up_n, "avg_launch_ms": up_avg,
+            "measured": "HIP events around each trailing_update_kernel launch on the LU stream "
+                        "over the timed steps (eager launches)",
+        },
+        "roofline_assembly": {
+            "kernel": "assemble_regular_kernel<COMBINED,16,%d>" % regular_split(ctx, n),
+            "bound": "fp64-valu",
+            "achieved": asm_achieved,
+            "peak": PEAK_FP64_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": asm_achieved / PEAK_FP64_TFLOPS,
+            "convention": "128 FLOP per pair-point of the reference algorithm (256 per regular "
+                          "entry); the kernel evaluates each Hankel pair once for the entries "
+                          "(i,j) and (j,i), so it executes about 0.6 of the convention's FLOP",
+            "work_per_launch": "%d FLOP = 128 x 256 pair-points x %d regular entries" %
+                               (FLOP_PER_PAIRPOINT * P_reg, n_reg),
+            "hardware_fp64_flop_per_launch": profile_value(ASM_C5_PROFILE,
+                                                           "hardware_fp64_flop_per_launch"),
+            "valu_issue_fraction": profile_value(ASM_C5_PROFILE, "valu_issue_fraction"),
+            "avg_launch_ms": reg_avg,
+        },
+    }
+    hw = out["roofline_assembly"]["hardware_fp64_flop_per_launch"]
+    if hw:
+        out["roofline_assembly"]["hardware_fp64_frac"] = hw / (reg_avg * 1e-3) / 1e12 / PEAK_FP64_TFLOPS
+    if cpu_res is not None:
+        t_cpu = cpu_res["t_asm"] + cpu_res["t_lu"]
+        cores = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
+        out["cpu_baseline"] = {
+            "value": nrhs / t_cpu,
+            "unit": "solves/s",
+            "cores": cores,
+            "kind": "port",
+            "sample": "oracle assembly of %d seeded random columns of the N=%d matrix (%.1f s, "
+                      "extrapolated x%d to %.0f s; the GPU matrix agrees with it on those columns "
+                      "to %.1e) + scipy LAPACK zgesv of an N=%d system with the %d right-hand "
+                      "sides (%.2f s, extrapolated x%d to %.0f s)" %
+                      (len(cpu_res["cols"]), n, cpu_res["t_cols"], n // len(cpu_res["cols"]),
+                       cpu_res["t_asm"], cpu_res["gpu_vs_oracle_cols"], cpu_res["n_lu"], nrhs,
+                       cpu_res["t_lu_small"], (n // cpu_res["n_lu"]) ** 3, cpu_res["t_lu"]),
+        }
+    return out
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--n", type=int, default=4000)
-    ap.add_argument("--k", type=float, default=100.0)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-cols", type=int, default=64)
-    ap.add_argument("--path", choices=["both", "plain", "mixed"], default="both",
-                    help="profiling aid: time only one solve path (the JSON then reports it)")
-    ap.add_argument("--config", choices=["kite", "batched", "serving", "large"], default="kite")
-    ap.add_argument("--no-extras", action="store_true",
-                    help="skip the batched and served runs on the default line")
-    ap.add_argument("--serving-count", type=int, default=None)
-    ap.add_argument("--serving-rate", type=float, default=27.0,
-                    help="arrivals per second (about 0.8 x the plain path's measured capacity)")
-    ap.add_argument("--probe-traffic", action="store_true",
-                    help="one assembly and one 1 GiB device copy, for rocprofv3 --pmc runs")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one GPU only: every box here has a single MI355X, and the path "
-              "would shard across GPUs by problem (no collective); run with --gpus 1.",
-              file=sys.stderr)
-        sys.exit(2)
-
+def run_kite(ctx, steps, warmup, n=4000, k=100.0, cpu_cols=0):
+    """C2: kite N=4000, k=100, one plane wave at a seeded random angle, both paths; the solutions'
+    distance from each other (their distance from the oracle is a GPU test of its own)."""
     import torch
-    from bem2d_amd import Context, capi
+    from bem2d_amd import capi
     from bem2d_amd import geometry as G
-
-    if args.probe_traffic:
-        probe_traffic(args)
-        return
-    if args.config != "kite":
-        ctx = Context(0)
-        if args.config == "batched":
-            res = run_batched(ctx)
-            value, unit = res["mixed"]["solves_per_s"], "solves/s"
-        elif args.config == "serving":
-            res = run_serving(ctx, args.serving_count or 2000, args.serving_rate)
-            value, unit = res["mixed"]["solves_per_s"], "solves/s"
-        else:
-            res = run_large(ctx)
-            value, unit = res["solves_per_s_mixed"], "solves/s"
-        print(json.dumps({"metric": METRIC, "value": value, "unit": unit, "n_gpus": 1,
-                          "steps": 1, "warmup": 0, "higher_is_better": True, "scaling": "weak",
-                          "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-                          "config": {"workload": res["workload"], "name": args.config},
-                          "result": res}), flush=True)
-        ctx.close()
-        return
-
-    n, k = args.n, args.k
     rng = np.random.default_rng(0)
     theta0 = 2 * math.pi * rng.uniform()
     dirs = [[math.cos(theta0), math.sin(theta0)]]
     geom = G.kite(n)
-    ctx = Context(0)
     mesh = ctx.mesh(geom, 16, 3, 0.15)
     P_reg, P_sing, n_reg = pair_points(n, mesh.special_count)
-    P = P_reg + P_sing
-
     A = torch.empty((n, n), dtype=torch.complex128, device="cuda").t()
     B = torch.empty((1, n), dtype=torch.complex128, device="cuda").t()
 
@@ -339,39 +444,19 @@ def main():
         ctx.rhs(mesh, capi.FIELD_COMBINED_PLANE, k, k, dirs, 2.0, out=B)
         ctx.solve_fp64(A, B, preserve_A=False)
 
-    dt = None
-    lu_avg = None
-    if args.path in ("both", "plain"):
-        for _ in range(args.warmup):
-            step()
-        torch.cuda.synchronize()
-        ctx.profile(True)
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            step()
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t0
-        lu_ms, lu_n = ctx.profile_read(capi.PROF_LU)
-        lu_avg = lu_ms / max(lu_n, 1)
-        ctx.profile(False)
-    # assembly kernels timed on their own (HIP events on the streams they run on; the singular-pair
-    # kernel runs beside the regular one, so the assembly's time is taken around the whole call)
-    ctx.profile(True)
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(args.steps):
-        ctx.assemble(mesh, capi.KERNEL_COMBINED, k, k, 1.0, 2.0, out=A)
-    e1.record()
-    reg_ms, reg_n = ctx.profile_read(capi.PROF_ASSEMBLE_REGULAR)
-    sp_ms, sp_n = ctx.profile_read(capi.PROF_ASSEMBLE_SPECIAL)
-    ctx.profile(False)
+    for _ in range(warmup):
+        step()
     torch.cuda.synchronize()
-    reg_avg = reg_ms / max(reg_n, 1)
-    sp_avg = sp_ms / max(sp_n, 1)
-    asm_ms = e0.elapsed_time(e1) / args.steps
+    ctx.profile(True)
+    ctx.profile_reset()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    lu_ms, lu_n = ctx.profile_read(capi.PROF_LU)
+    ctx.profile(False)
 
-    # ---- the chosen form: same assembly + FP32 MFMA LU with FP64 refinement -------------------
     lu = ctx.lu32(n)
     X = torch.empty_like(B)
 
@@ -381,126 +466,163 @@ def main():
         lu.factor(A)
         return lu.refine(A, B, X, tol=1e-12, raise_on_fail=False)
 
-    dt_mixed = None
-    mixed_ok, mixed_bw, mixed_it, lu32_ms, lu32_n = False, None, None, 0.0, 0
-    if args.path in ("both", "mixed"):
-        for _ in range(args.warmup):
-            step_mixed()
-        torch.cuda.synchronize()
-        ctx.profile(True)
-        t0 = time.perf_counter()
-        reports = [step_mixed() for _ in range(args.steps)]
-        torch.cuda.synchronize()
-        dt_mixed = time.perf_counter() - t0
-        lu32_ms, lu32_n = ctx.profile_read(capi.PROF_LU)
-        ctx.profile(False)
-        mixed_ok = all(r.converged for r in reports)
-        mixed_bw = max(r.backward_error for r in reports)
-        mixed_it = max(r.iterations for r in reports)
+    for _ in range(warmup):
+        step_mixed()
+    torch.cuda.synchronize()
+    ctx.profile(True)
+    ctx.profile_reset()
+    t0 = time.perf_counter()
+    reports = [step_mixed() for _ in range(steps)]
+    torch.cuda.synchronize()
+    dt_mixed = time.perf_counter() - t0
+    lu32_ms, lu32_n = ctx.profile_read(capi.PROF_LU)
+    reg_ms, reg_n = ctx.profile_read(capi.PROF_ASSEMBLE_REGULAR)
+    sp_ms, sp_n = ctx.profile_read(capi.PROF_ASSEMBLE_SPECIAL)
+    ctx.profile(False)
 
-    # one more assembly for the checks and the CPU baseline (outside the timed region)
     ctx.assemble(mesh, capi.KERNEL_COMBINED, k, k, 1.0, 2.0, out=A)
     ctx.rhs(mesh, capi.FIELD_COMBINED_PLANE, k, k, dirs, 2.0, out=B)
-    A_host = A.cpu().numpy()
-    b_host = B.cpu().numpy()
-    X = B.clone()
-    ctx.solve_fp64(A.clone(), X, preserve_A=False)
-    torch.cuda.synchronize()
-    bw = (torch.linalg.norm(B - A @ X) / (torch.linalg.norm(A) * torch.linalg.norm(X) +
-                                          torch.linalg.norm(B))).item()
-    Xm = torch.empty_like(B)
+    X64 = B.clone()
+    ctx.solve_fp64(A, X64, preserve_A=True)
     lu.factor(A)
-    lu.refine(A, B, Xm, tol=1e-12, raise_on_fail=False)
-    mixed_vs_fp64 = (torch.linalg.norm(Xm - X) / torch.linalg.norm(X)).item()
+    lu.refine(A, B, X, tol=1e-12, raise_on_fail=False)
+    torch.cuda.synchronize()
+    bw = (torch.linalg.norm(B - A @ X64) / (torch.linalg.norm(A) * torch.linalg.norm(X64) +
+                                            torch.linalg.norm(B))).item()
+    mixed_vs_fp64 = (torch.linalg.norm(X - X64) / torch.linalg.norm(X64)).item()
     lu.close()
-
+    mesh.close()
+    del A, B, X, X64
+    ctx.release_scratch()
+    torch.cuda.empty_cache()
+    reg_avg = reg_ms / max(reg_n, 1)
     achieved = FLOP_PER_PAIRPOINT * P_reg / (reg_avg * 1e-3) / 1e12
     hw_flop = profile_value(FP64_PROFILE, "hardware_fp64_flop_per_launch")
-    hw_tflops = None if hw_flop is None else hw_flop / (reg_avg * 1e-3) / 1e12
-    plain_sps = args.steps / dt if dt else None
-    mixed_sps = args.steps / dt_mixed if dt_mixed else None
-    use_mixed = mixed_ok and (plain_sps is None or mixed_sps >= plain_sps)
+    return {
+        "workload": "kite N=%d k=%g combined-field (eta=k), P0, 16-pt Gauss, L=3 sigma=0.15, 1 "
+                    "plane-wave right-hand side at a seeded random angle; one step = FP64 "
+                    "assembly + right-hand side + solve" % (n, k),
+        "steps": steps,
+        "solves_per_s_mixed": steps / dt_mixed,
+        "solves_per_s_plain_fp64": steps / dt,
+        "mixed_over_plain": dt / dt_mixed,
+        "ms_per_step_mixed": 1e3 * dt_mixed / steps,
+        "ms_per_step_plain_fp64": 1e3 * dt / steps,
+        "mixed_converged": all(r.converged for r in reports),
+        "mixed_backward_error": max(r.backward_error for r in reports),
+        "mixed_refinement_steps": max(r.iterations for r in reports),
+        "mixed_vs_fp64_solution_rel_err": mixed_vs_fp64,
+        "backward_error_fp64": bw,
+        "lu32_ms": lu32_ms / max(lu32_n, 1),
+        "lu_fp64_ms": lu_ms / max(lu_n, 1),
+        "assembly_regular_ms": reg_avg,
+        "assembly_special_ms": sp_ms / max(sp_n, 1),
+        "gentries_per_s": n * n / (reg_avg * 1e-3) / 1e9,
+        "gpairpts_per_s": (P_reg + P_sing) / (reg_avg * 1e-3) / 1e9,
+        "pair_points": P_reg + P_sing,
+        "roofline_assembly": {
+            "kernel": "assemble_regular_kernel<COMBINED,16,%d>" % regular_split(ctx, n),
+            "bound": "fp64-valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+            "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
+            "traffic": measured_traffic(),
+            "hardware_fp64_frac": None if hw_flop is None else
+            hw_flop / (reg_avg * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
+            "valu_issue_fraction": profile_value(FP64_PROFILE, "valu_issue_fraction"),
+            "avg_launch_ms": reg_avg,
+        },
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--plain-steps", type=int, default=3,
+                    help="timed steps of the plain FP64 path on the large configuration")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--config", choices=["all", "large", "kite", "batched", "serving"],
+                    default="all")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="the large configuration only (no kite, batched and served legs)")
+    ap.add_argument("--serving-count", type=int, default=2000)
+    ap.add_argument("--serving-rate", type=float, default=27.0,
+                    help="arrivals per second (about 0.8 x the plain path's measured capacity)")
+    ap.add_argument("--probe-traffic", action="store_true",
+                    help="one kite assembly and one 1 GiB device copy, for rocprofv3 --pmc runs")
+    ap.add_argument("--probe-lu", action="store_true",
+                    help="one N=32000 open-arc assembly and FP32 factorisation and one 1 GiB "
+                         "device copy, for rocprofv3 --pmc runs")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: every box here has a single MI355X, and the path "
+              "would shard across GPUs by problem (no collective); run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+    # per-kernel HIP events need the factorisation's launches made eagerly (a replayed hipGraph of
+    # the launch sequence records none; it measured no faster, the launches being queued ahead)
+    os.environ.setdefault("BEM2D_LU_GRAPH", "0")
+
+    import torch
+    from bem2d_amd import Context
+
+    if args.probe_traffic:
+        probe_traffic(args)
+        return
+    if args.probe_lu:
+        probe_lu()
+        return
+    ctx = Context(0)
+    cpu = not args.no_cpu_baseline
+    if args.config in ("kite", "batched", "serving"):
+        if args.config == "kite":
+            res = run_kite(ctx, args.steps, args.warmup)
+            value, t = res["solves_per_s_mixed"], res["ms_per_step_mixed"]
+        elif args.config == "batched":
+            res = run_batched(ctx)
+            value, t = res["mixed"]["solves_per_s"], None
+        else:
+            res = run_serving(ctx, args.serving_count, args.serving_rate)
+            value, t = res["mixed"]["solves_per_s"], None
+        print(json.dumps({"metric": METRIC, "value": value, "unit": "solves/s", "n_gpus": 1,
+                          "steps": args.steps, "warmup": args.warmup, "ms_per_step": t,
+                          "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                          "dtype": "f64-assembly+fp32-mfma-lu+f64-refinement",
+                          "data": "synthetic",
+                          "config": {"workload": res["workload"], "name": args.config},
+                          "result": res}), flush=True)
+        ctx.close()
+        return
+
+    large = run_large(ctx, args.steps, args.warmup, plain_steps=args.plain_steps, cpu=cpu)
     out = {
         "metric": METRIC,
-        "value": mixed_sps if use_mixed else plain_sps,
+        "value": large["solves_per_s_mixed"],
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * (dt_mixed if use_mixed else dt) / args.steps,
+        "ms_per_step": large["ms_per_step_mixed"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64-assembly+fp32-mfma-lu+f64-refinement",
         "data": "synthetic",
-        "config": {
-            "workload": "kite N=%d k=%g combined-field (eta=k), P0, 16-pt Gauss, L=3 sigma=0.15, "
-                        "1 plane-wave RHS at a seeded random angle; one step = FP64 assembly "
-                        "A=M+2K + RHS + solve; value = the faster of the mixed path (FP32 MFMA "
-                        "LU + FP64 refinement to backward error 1e-12) and the plain FP64 path "
-                        "(rocSOLVER zgetrf/zgetrs)" % (n, k),
-            "n": n, "k": k, "nrhs": 1, "path": "mixed" if use_mixed else "plain-fp64",
-        },
-        "solves_per_s_mixed": mixed_sps,
-        "solves_per_s_plain_fp64": plain_sps,
-        "mixed_over_plain": mixed_sps / plain_sps if (mixed_sps and plain_sps) else None,
-        "mixed_converged": mixed_ok,
-        "mixed_backward_error": mixed_bw,
-        "mixed_refinement_steps": mixed_it,
-        "mixed_vs_fp64_solution_rel_err": mixed_vs_fp64,
-        "lu32_ms": lu32_ms / max(lu32_n, 1),
-        "gentries_per_s": n * n / (asm_ms * 1e-3) / 1e9,
-        "gpairpts_per_s": P / (asm_ms * 1e-3) / 1e9,
-        "assembly_ms": asm_ms,
-        "assembly_regular_ms": reg_avg,
-        "assembly_special_ms": sp_avg,
-        "lu_fp64_ms": lu_avg,
-        "pair_points": P,
-        "backward_error_fp64": bw,
-        "roofline": {
-            "kernel": "assemble_regular_kernel<COMBINED,16,%d>" % regular_split(ctx, args.n),
-            "bound": "mfma",
-            "engine": "FP64 vector ALU: compute-bound (about 2000 FLOP per stored byte); its "
-                      "peak, 78.6 TFLOP/s, is also MI355X's dense FP64 matrix peak",
-            "convention": "128 FLOP per pair-point of the reference algorithm (256 per regular "
-                          "entry); the kernel shares each Hankel evaluation between entries (i,j) "
-                          "and (j,i), so it executes about half of these FLOP and frac can pass 1; "
-                          "hardware_fp64_* give the executed FP64 rate",
-            "achieved": achieved,
-            "peak": PEAK_FP64_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": achieved / PEAK_FP64_TFLOPS,
-            "traffic": measured_traffic(),
-            "work_per_launch": "%d FLOP = 128 FLOP x 256 pair-points x %d regular entries" %
-                               (FLOP_PER_PAIRPOINT * P_reg, n_reg),
-            # diagnostics beside the convention: FP64 FLOP the hardware executes per launch (SQ
-            # instruction counters of the committed profile, every lane counted, FMA as 2) over
-            # this run's kernel time, and the VALU issue rate the same profile measured
-            "hardware_fp64_tflops": hw_tflops,
-            "hardware_fp64_frac": None if hw_tflops is None else hw_tflops / PEAK_FP64_TFLOPS,
-            "valu_issue_fraction": profile_value(FP64_PROFILE, "valu_issue_fraction"),
-        },
+        "config": {"workload": large["workload"], "name": "large", "n": large["n"],
+                   "k": large["k"], "nrhs": large["nrhs"], "path": "mixed"},
+        "roofline": large.pop("roofline"),
     }
-    if not args.no_cpu_baseline:
-        t_asm, t_solve, t_cols, err = cpu_baseline(geom, A_host, k, b_host, args.cpu_cols)
-        cores = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
-        out["cpu_baseline"] = {
-            "value": 1.0 / (t_asm + t_solve),
-            "unit": "solves/s",
-            "cores": cores,
-            "kind": "port",
-            "sample": "oracle assembly of %d seeded random columns of the N=%d matrix (%.1f s, "
-                      "extrapolated x%d to %.1f s) + scipy LAPACK zgesv on the full matrix "
-                      "(%.2f s); sampled columns agree with the GPU to %.1e" %
-                      (args.cpu_cols, n, t_cols, n // args.cpu_cols, t_asm, t_solve, err),
-        }
-    mesh.close()
-    del A, B, X, Xm
-    torch.cuda.empty_cache()
-    if not args.no_extras:
+    if "cpu_baseline" in large:
+        out["cpu_baseline"] = large.pop("cpu_baseline")
+    out["large"] = large
+    if not args.no_extras and args.config == "all":
+        out["kite"] = run_kite(ctx, args.steps, args.warmup)
+        progress("C2 kite done")
         out["kite_served"] = run_kite_served(ctx)
+        progress("kite served done")
         out["batched"] = run_batched(ctx)
-        sv = run_serving(ctx, args.serving_count or 200, args.serving_rate)
+        progress("C3 batch done")
+        sv = run_serving(ctx, args.serving_count, args.serving_rate)
         out["serving"] = sv
         out["p50_ms"] = sv["mixed"]["p50_ms"]
         out["p99_ms"] = sv["mixed"]["p99_ms"]
@@ -509,21 +631,47 @@ def main():
 
 
 def probe_traffic(args):
-    """One regular-assembly launch of the default workload and one 1 GiB device-to-device copy
-    (the calibration kernel of known size) -- the program rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
-    runs profile."""
+    """One regular-assembly launch of the kite workload (N=4000, k=100) and one 1 GiB
+    device-to-device copy (the calibration kernel of known size) -- the program rocprofv3 --pmc
+    FETCH_SIZE / WRITE_SIZE runs profile."""
     import torch
     from bem2d_amd import Context, capi
     from bem2d_amd import geometry as G
     ctx = Context(0)
-    mesh = ctx.mesh(G.kite(args.n), 16, 3, 0.15)
-    A = torch.empty((args.n, args.n), dtype=torch.complex128, device="cuda").t()
-    ctx.assemble(mesh, capi.KERNEL_COMBINED, args.k, args.k, 1.0, 2.0, out=A)
+    mesh = ctx.mesh(G.kite(4000), 16, 3, 0.15)
+    A = torch.empty((4000, 4000), dtype=torch.complex128, device="cuda").t()
+    ctx.assemble(mesh, capi.KERNEL_COMBINED, 100.0, 100.0, 1.0, 2.0, out=A)
     src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
     dst = torch.empty_like(src)
     dst.copy_(src)
     torch.cuda.synchronize()
-    print("probe done: assembly N=%d + copy of %d bytes" % (args.n, src.numel() * 8), flush=True)
+    print("probe done: assembly N=4000 + copy of %d bytes" % (src.numel() * 8), flush=True)
+    mesh.close()
+    ctx.close()
+
+
+def probe_lu():
+    """One assembly and one FP32 factorisation of the N=32000 open arc, then one 1 GiB
+    device-to-device copy -- the program the rocprofv3 --pmc runs of the large configuration
+    profile."""
+    import torch
+    from bem2d_amd import Context, capi
+    from bem2d_amd import geometry as G
+    n, k = 32000, 500.0
+    ctx = Context(0)
+    mesh = ctx.mesh(G.open_arc(n, 0.3), 16, 3, 0.15)
+    A = torch.empty((n, n), dtype=torch.complex128, device="cuda").t()
+    ctx.assemble(mesh, capi.KERNEL_COMBINED, k, k, 1.0, 2.0, out=A)
+    lu = ctx.lu32(n)
+    lu.factor(A)
+    torch.cuda.synchronize()
+    print("probe: assembly and FP32 factorisation N=%d done" % n, flush=True)
+    src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print("probe done: copy of %d bytes" % (src.numel() * 8), flush=True)
+    lu.close()
     mesh.close()
     ctx.close()
 

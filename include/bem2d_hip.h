@@ -94,6 +94,9 @@ int bem2d_ctx_set_stream(bem2d_ctx* ctx, void* stream);
 int bem2d_ctx_reset_stream(bem2d_ctx* ctx);
 void* bem2d_ctx_stream(bem2d_ctx* ctx);
 int bem2d_synchronize(bem2d_ctx* ctx);
+/* Frees the context's scratch memory (factor copies, far-field signatures, pivots), after the work
+ * queued on its stream; it is allocated again on demand.  No counterpart in the reference. */
+int bem2d_ctx_release_scratch(bem2d_ctx* ctx);
 const char* bem2d_last_error(void);
 
 /* Device memory helpers, so that host code above this layer needs no HIP headers. */
@@ -313,7 +316,9 @@ int bem2d_engine_run_trace(bem2d_engine* eng, const bem2d_request* reqs, const d
 #define BEM2D_PROF_ASSEMBLE_REGULAR 0 /* regular-pair assembly kernel */
 #define BEM2D_PROF_ASSEMBLE_SPECIAL 1 /* coincident / adjacent assembly kernel */
 #define BEM2D_PROF_LU 2               /* LU factorisation (FP64 rocSOLVER or the FP32 MFMA LU) */
-#define BEM2D_PROF_COUNT 3
+#define BEM2D_PROF_LU_UPDATE 3        /* FP32 LU trailing-update kernel (launches made eagerly; a
+                                         factorisation replayed from a hipGraph records none) */
+#define BEM2D_PROF_COUNT 4
 int bem2d_profile_enable(bem2d_ctx* ctx, int on);
 int bem2d_profile_reset(bem2d_ctx* ctx);
 int bem2d_profile_read(bem2d_ctx* ctx, int kernel_id, double* total_ms, int64_t* launches);
