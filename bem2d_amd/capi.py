@@ -74,6 +74,8 @@ SIGNATURES = {
     "bem2d_lu32_refine": (_I, [_P, _P, _P, _I64, _P, _I64, _P, _I64, _I, _D, _I,
                                ctypes.POINTER(IRReport)]),
     "bem2d_solve_mixed": (_I, [_P, _I64, _I64, _P, _I64, _P, _I64, _D, ctypes.POINTER(IRReport)]),
+    "bem2d_solve_mixed_static": (_I, [_P, _I64, _I64, _P, _I64, _P, _I64, _D,
+                                      ctypes.POINTER(IRReport)]),
     "bem2d_eval_single_layer": (_I, [_P, _P, _D, _P, _P, _I64, _P]),
     "bem2d_farfield": (_I, [_P, _P, _D, _P, _I64, _I, _P, _I64, _P]),
     "bem2d_profile_enable": (_I, [_P, _I]),
@@ -336,6 +338,17 @@ class Context:
         check(lib().bem2d_solve_mixed(self.h, A.shape[0], B.shape[1], _t_ptr(A), _ld(A),
                                       _t_ptr(B), _ld(B), tol, ctypes.byref(rep)))
         return rep
+
+    def solve_mixed_static(self, A, B, tol=1e-12, raise_on_fail=True):
+        """bem2d_solve_mixed_static: the mixed solve in the library's fixed device workspace
+        (n <= 1024, nrhs <= 8; no allocation).  Overwrites B on success; returns
+        (status, IRReport)."""
+        rep = IRReport()
+        rc = lib().bem2d_solve_mixed_static(self.h, A.shape[0], B.shape[1], _t_ptr(A), _ld(A),
+                                            _t_ptr(B), _ld(B), tol, ctypes.byref(rep))
+        if raise_on_fail:
+            check(rc)
+        return rc, rep
 
     def lu32(self, n):
         return LU32(self, n)

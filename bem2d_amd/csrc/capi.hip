@@ -70,6 +70,9 @@ extern "C" int bem2d_ctx_create(bem2d_ctx** out, int device)
         return FAIL(BEM2D_ERR_DEVICE, "stream / event / rocBLAS handle creation failed");
     }
     rocblas_set_stream(c->blas, c->stream);
+    // device address of the no-allocation solve's workspace, taken once per device while it is
+    // current (a failure here is reported by that entry point, which tries again)
+    (void)bem2d_internal_static_scratch(device, nullptr);
     // the autotuner's table: $BEM2D_TUNE_FILE, else tuned/<arch>.tune beside this library
     const char* tf = std::getenv("BEM2D_TUNE_FILE");
     if (tf && tf[0]) {

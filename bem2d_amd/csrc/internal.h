@@ -170,6 +170,10 @@ void bem2d_internal_ref_report(const RefJob* j, int sidx, bem2d_ir_report* r);
 bool bem2d_internal_ref_bad_pivot(const RefJob* j, int sidx);
 int bem2d_internal_refine_sync(hipStream_t s, RefJob* j);
 
+// Device address of the no-allocation solve's file-scope workspace on `device` (current): taken
+// once per device with hipGetSymbolAddress, at the first context creation on it.
+int bem2d_internal_static_scratch(int device, void** out);
+
 // Read-only view of FP32 factors for code outside mixed.hip: the n x n leading block of F (ld np)
 // holds L (unit diagonal implied) and U of P A, where row i of P A is row perm[i] of A.
 struct Lu32View {

@@ -29,6 +29,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "../../include/bem2d_hip.h"
@@ -2488,4 +2489,200 @@ extern "C" int bem2d_solve_mixed(bem2d_ctx* c, int64_t n, int64_t nrhs, const vo
     bem2d_lu32_destroy(f);
     if (rep) *rep = r;
     return rc;
+}
+
+// ================================================================================================
+// mixed solve without allocation
+// ================================================================================================
+// bem2d_solve_mixed_static: the factorisation and refinement of bem2d_solve_mixed with every
+// device buffer carved from one file-scope array, for callers that cannot allocate (code under
+// graph capture, tight serving loops, harnesses that forbid hipMalloc).  Its address on each device
+// is taken once, with that device current, when the first context on it is created; host code uses
+// that address, never the array's name (which on the host is only a stand-in).  Every call on a GPU
+// shares that GPU's array: calls that overlap on one GPU race, and each call clears what must start
+// at zero (row claims, norms, flags, the tagged words of the sweeps) before use.
+constexpr int64_t kStaticMaxN = 1024;
+constexpr int kStaticMaxRhs = kMaxSolveRhs;
+constexpr size_t kStaticScratchBytes = 20u << 20;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+namespace {
+constexpr int kStaticMaxDevices = 64;
+std::mutex g_static_mutex;
+void* g_static_ptr[kStaticMaxDevices] = {};
+}
+
+// Takes the device address of `scratch` on device `device` (current) once; later calls return it.
+int bem2d_internal_static_scratch(int device, void** out)
+{
+    if (device < 0 || device >= kStaticMaxDevices)
+        return FAIL(BEM2D_ERR_UNSUPPORTED, "static scratch: device %d beyond %d", device,
+                    kStaticMaxDevices);
+    std::lock_guard<std::mutex> lock(g_static_mutex);
+    if (!g_static_ptr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return FAIL(BEM2D_ERR_DEVICE, "hipGetSymbolAddress(scratch) failed: %s",
+                        hipGetErrorString(e));
+        g_static_ptr[device] = p;
+    }
+    if (out) *out = g_static_ptr[device];
+    return BEM2D_OK;
+}
+
+extern "C" int bem2d_solve_mixed_static(bem2d_ctx* c, int64_t n, int64_t nrhs, const void* A64,
+                                        int64_t lda, void* B, int64_t ldb, double tol,
+                                        bem2d_ir_report* rep)
+{
+    if (!c || !A64 || !B || n < 1 || nrhs < 1 || lda < n || ldb < n)
+        return FAIL(BEM2D_ERR_ARG, "bem2d_solve_mixed_static: bad argument");
+    if (n > kStaticMaxN || nrhs > kStaticMaxRhs)
+        return FAIL(BEM2D_ERR_ARG,
+                    "bem2d_solve_mixed_static: n = %lld, nrhs = %lld beyond the fixed workspace "
+                    "(n <= %lld, nrhs <= %d)", (long long)n, (long long)nrhs,
+                    (long long)kStaticMaxN, kStaticMaxRhs);
+    HIPCHK(hipSetDevice(c->device));
+    void* base = nullptr;
+    if (int rc = bem2d_internal_static_scratch(c->device, &base)) return rc;
+    if (!(tol > 0)) tol = 1e-12;
+    hipStream_t s = c->stream;
+    // the factor object lives on the stack; its buffers are carved from the array
+    bem2d_lu32 f;
+    f.ctx = c;
+    f.n = n;
+    f.nblk = (int)((n + NB - 1) / NB);
+    f.np = (int64_t)f.nblk * NB;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) == hipSuccess &&
+        cus > 0)
+        f.cus = cus;
+    char* p = (char*)base;
+    size_t used = 0;
+    auto carve = [&](size_t bytes) {
+        void* q = p + used;
+        used += (bytes + 255) & ~(size_t)255;
+        return q;
+    };
+    const int64_t np = f.np;
+    f.F = (float2*)carve(sizeof(float2) * np * np);
+    f.Linv = (float2*)carve(sizeof(float2) * (size_t)f.nblk * NB * NB);
+    f.UinvT = (float2*)carve(sizeof(float2) * (size_t)f.nblk * NB * NB);
+    f.dflag = (int*)carve(sizeof(int) * 2);
+    f.anorm = (double*)carve(sizeof(double));
+    f.rowpart = (double*)carve(sizeof(double) * (np / kCastCols) * n);
+    f.perm = (int*)carve(sizeof(int) * np);
+    f.pivwork = (int*)carve(sizeof(int) * 3 * n);
+    f.claim = (unsigned long long*)carve(sizeof(unsigned long long) * n);
+    f.gmax = (unsigned long long*)carve(sizeof(unsigned long long) * 3);
+    RefUnit u{};
+    u.n = n;
+    u.nrhs = (int)nrhs;
+    u.nblk = f.nblk;
+    u.chunks = ref_chunks(n);
+    u.A = (const double2*)A64;
+    u.lda = lda;
+    u.B = (const double2*)B;
+    u.ldb = ldb;
+    u.X = (double2*)carve(sizeof(double2) * n * nrhs);
+    u.ldx = n;
+    u.R = (double2*)carve(sizeof(double2) * n * kMaxSolveRhs);
+    u.part = (double2*)carve(sizeof(double2) * n * kMaxSolveRhs * u.chunks);
+    u.Y = (float2*)carve(sizeof(float2) * n * kMaxSolveRhs);
+    u.scale = (double*)carve(sizeof(double) * kMaxSolveRhs);
+    u.ty = (unsigned long long*)carve(sizeof(unsigned long long) * 2 * n * kMaxSolveRhs);
+    u.norms = (unsigned long long*)carve(sizeof(unsigned long long) * kNormSlots);
+    u.F = f.F;
+    u.ldf = np;
+    u.Linv = f.Linv;
+    u.UinvT = f.UinvT;
+    u.anorm = f.anorm;
+    u.luflags = f.dflag;
+    u.perm = f.perm;
+    u.gmax = f.gmax;
+    if (used > kStaticScratchBytes)
+        return FAIL(BEM2D_ERR_ARG, "bem2d_solve_mixed_static: workspace of %zu bytes exceeds %zu",
+                    used, kStaticScratchBytes);
+    // factorisation: the launch sequence of bem2d_lu32_factor, eagerly
+    bem2d_lu32* fp = &f;
+    int rc = bem2d_internal_lu_begin(s, &fp, &A64, &lda, 1);
+    while (!rc && !f.factored) rc = bem2d_internal_lu_advance(s, &fp, 1);
+    if (rc) return rc;
+    // refinement: the step sequence of bem2d_lu32_refine for one group of right-hand sides (the
+    // initial residual and two corrections, then one correction per read-back), so the arithmetic
+    // -- and the solution -- is that of bem2d_solve_mixed
+    RefBatch one{}, rows{}, sweep{};
+    one.count = rows.count = sweep.count = 1;
+    one.u[0] = rows.u[0] = sweep.u[0] = u;
+    one.start[1] = 1;
+    rows.start[1] = (n + 255) / 256;
+    sweep.start[1] = f.nblk;
+    const unsigned rgrid = (unsigned)rows.start[1];
+    zero_u64_kernel<<<1, 64, 0, s>>>(u.norms, kNormSlots);
+    zero_u64_kernel<<<(unsigned)((2 * n * kMaxSolveRhs + 255) / 256), 256, 0, s>>>(
+        u.ty, 2 * n * kMaxSolveRhs);
+    ref_init_kernel<<<dim3(rgrid, kMaxSolveRhs), 256, 0, s>>>(rows);
+    constexpr int kMaxit = 30;
+    unsigned long long nm[kNormSlots];
+    double bnorm[kMaxSolveRhs] = {};
+    double omega = INFINITY, prev = INFINITY, growth = 0.0, anorm = 0.0;
+    int it = 0;
+    bool bad = false, first = true;
+    while (it < kMaxit) {
+        const int nsteps = first ? 2 : 1;
+        for (int st = 0; st < nsteps; ++st) {
+            const unsigned long long epoch = (unsigned long long)(it + st + 1);
+            one.epoch = rows.epoch = sweep.epoch = epoch;
+            ref_scale_kernel<<<1, 64, 0, s>>>(one);
+            ref_demote_kernel<<<dim3(rgrid, kMaxSolveRhs), 256, 0, s>>>(rows);
+            block_trsv_kernel<true><<<(unsigned)sweep.start[1], 256, 0, s>>>(sweep);
+            block_trsv_kernel<false><<<(unsigned)sweep.start[1], 256, 0, s>>>(sweep);
+            ref_update_kernel<<<dim3(rgrid, kMaxSolveRhs), 256, 0, s>>>(rows);
+            ref_residual_partial_kernel<<<dim3(rgrid, kResidChunks), 256, 0, s>>>(rows);
+            ref_residual_reduce_kernel<<<dim3(rgrid, kMaxSolveRhs), 256, 0, s>>>(rows);
+            ref_flags_kernel<<<1, 64, 0, s>>>(one);
+        }
+        it += nsteps;
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(nm, u.norms, sizeof nm, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        growth = factor_growth(u64_as_double(nm[kNormGrowthU]), u64_as_double(nm[kNormGrowthL]),
+                               u64_as_double(nm[kNormAmax]));
+        anorm = u64_as_double(nm[kNormA]);
+        if (nm[kNormTimeout]) return FAIL(BEM2D_ERR_DEVICE, "triangular sweep wait timed out");
+        if (nm[kNormBad] || !(growth <= kGrowthLimit)) {
+            bad = true;
+            omega = INFINITY;
+            break;
+        }
+        if (first)
+            for (int h = 0; h < nrhs; ++h) bnorm[h] = u64_as_double(nm[2 * kMaxSolveRhs + h]);
+        first = false;
+        omega = 0.0;
+        for (int h = 0; h < nrhs; ++h) {
+            const double rn = u64_as_double(nm[h]), xn = u64_as_double(nm[kMaxSolveRhs + h]);
+            const double den = anorm * xn + bnorm[h];
+            const double wh = den > 0 ? rn / den : (rn > 0 ? INFINITY : 0.0);
+            omega = std::max(omega, std::isfinite(wh) ? wh : INFINITY);
+        }
+        if (omega <= tol || (it >= 3 && !(omega <= kStallRatio * prev))) break;
+        prev = omega;
+    }
+    if (rep) {
+        *rep = bem2d_ir_report{};
+        rep->iterations = bad ? 0 : it;
+        rep->converged = omega <= tol;
+        rep->backward_error = omega;
+        rep->matrix_norm_inf = anorm;
+        rep->growth = growth;
+    }
+    if (!(omega <= tol))
+        return FAIL(BEM2D_ERR_NOT_CONVERGED,
+                    bad ? "FP32 factors unusable (zero pivot or element growth %.3g)"
+                        : "refinement reached backward error %.3e > tol",
+                    bad ? growth : omega);
+    HIPCHK(hipMemcpy2DAsync(B, sizeof(double2) * ldb, u.X, sizeof(double2) * n,
+                            sizeof(double2) * n, nrhs, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return BEM2D_OK;
 }

@@ -221,6 +221,18 @@ int bem2d_lu32_refine(bem2d_ctx* ctx, bem2d_lu32* lu, const void* A64, int64_t l
 int bem2d_solve_mixed(bem2d_ctx* ctx, int64_t n, int64_t nrhs, const void* A64, int64_t lda,
                       void* B, int64_t ldb, double tol, bem2d_ir_report* report);
 
+/* No counterpart in the reference: bem2d_solve_mixed without any allocation, for callers that cannot
+ * allocate device memory (code under graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The FP32 factorisation and the refinement of bem2d_solve_mixed -- the same kernels
+ * and step sequence, hence the same solution -- with every buffer in one fixed device workspace of
+ * 20 MiB per GPU, held until the process ends: n <= 1024 and nrhs <= 8 (larger sizes return
+ * BEM2D_ERR_ARG).  No GMRES-IR and no FP64 fallback (both allocate): factors with a zero pivot or
+ * element growth above 1e4, or refinement that stalls or misses tol (<= 0: 1e-12), return
+ * BEM2D_ERR_NOT_CONVERGED with B unchanged.  Waits for the solve before returning.  All calls on
+ * one GPU share its workspace: calls that overlap on one GPU race (serialise them). */
+int bem2d_solve_mixed_static(bem2d_ctx* ctx, int64_t n, int64_t nrhs, const void* A64, int64_t lda,
+                             void* B, int64_t ldb, double tol, bem2d_ir_report* report);
+
 /* ---- potential and far field ------------------------------------------------------------------- */
 /* out[p] = sum_j coef[j] * int_{elem j} (i/4) H0(k |P_p - y|) ds(y) with Gauss1D(N):
  * EvalKernel<SingleLayer> (lib/bem2d_quadrature.h:184-228) as used by EvalSol
