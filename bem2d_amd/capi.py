@@ -76,6 +76,11 @@ SIGNATURES = {
     "bem2d_solve_mixed": (_I, [_P, _I64, _I64, _P, _I64, _P, _I64, _D, ctypes.POINTER(IRReport)]),
     "bem2d_solve_mixed_static": (_I, [_P, _I64, _I64, _P, _I64, _P, _I64, _D,
                                       ctypes.POINTER(IRReport)]),
+    "bem2d_change_basis": (_I, [_P, _I64, _P, _I64, _P, _I64]),
+    "bem2d_singular_values": (_I, [_P, _I64, _P, _I64, _P]),
+    "bem2d_l2_norm_cond": (_I, [_P, _I64, _P, _I64, ctypes.POINTER(_D), ctypes.POINTER(_D)]),
+    "bem2d_hermitian_eigenvalues": (_I, [_P, _I64, _P, _I64, _P]),
+    "bem2d_numerical_range": (_I, [_P, _I64, _P, _I64, _I, _P]),
     "bem2d_eval_single_layer": (_I, [_P, _P, _D, _P, _P, _I64, _P]),
     "bem2d_farfield": (_I, [_P, _P, _D, _P, _I64, _I, _P, _I64, _P]),
     "bem2d_profile_enable": (_I, [_P, _I]),
@@ -352,6 +357,37 @@ class Context:
 
     def lu32(self, n):
         return LU32(self, n)
+
+    # ---- spectral drivers (device matrices: complex128, column-major views) -----------------
+    def change_basis(self, K, M):
+        """K <- U^{-H} K U^{-1} with M = U^H U (ChangeBasis); K is overwritten and returned."""
+        check(lib().bem2d_change_basis(self.h, K.shape[0], _t_ptr(K), _ld(K), _t_ptr(M), _ld(M)))
+        return K
+
+    def singular_values(self, A):
+        import torch
+        s = torch.empty(A.shape[0], dtype=torch.float64, device=A.device)
+        check(lib().bem2d_singular_values(self.h, A.shape[0], _t_ptr(A), _ld(A), _t_ptr(s)))
+        return s
+
+    def l2_norm_cond(self, A):
+        nrm, cnd = _D(), _D()
+        check(lib().bem2d_l2_norm_cond(self.h, A.shape[0], _t_ptr(A), _ld(A), ctypes.byref(nrm),
+                                       ctypes.byref(cnd)))
+        return nrm.value, cnd.value
+
+    def hermitian_eigenvalues(self, A):
+        import torch
+        w = torch.empty(A.shape[0], dtype=torch.float64, device=A.device)
+        check(lib().bem2d_hermitian_eigenvalues(self.h, A.shape[0], _t_ptr(A), _ld(A), _t_ptr(w)))
+        return w
+
+    def numerical_range(self, A, nsteps):
+        """The 2 nsteps + 1 boundary points (complex, numpy) of NumRange."""
+        pts = np.zeros(2 * (2 * nsteps + 1))
+        check(lib().bem2d_numerical_range(self.h, A.shape[0], _t_ptr(A), _ld(A), int(nsteps),
+                                          _np_ptr(pts)))
+        return pts[0::2] + 1j * pts[1::2]
 
     def eval_single_layer(self, mesh, k, coef, pts):
         import torch

@@ -17,7 +17,8 @@ enum {
     SCRATCH_FACTOR = 2,   // copy of A when the caller's matrix must be preserved
     SCRATCH_PIVOT = 3,    // pivots + info of the FP64 LU
     SCRATCH_CONTOUR = 4,  // Hankel contour table of a complex wavenumber
-    SCRATCH_COUNT = 5
+    SCRATCH_SPECTRAL = 5, // rocSOLVER info, off-diagonals, eigenvalues of the spectral drivers
+    SCRATCH_COUNT = 6
 };
 
 struct ScratchBuf {

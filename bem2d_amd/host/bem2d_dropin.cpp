@@ -4,6 +4,7 @@
 // exception tags.
 #include <cstdio>
 #include <cstdlib>
+#include <fstream>
 #include <cstring>
 #include <mutex>
 
@@ -304,6 +305,80 @@ pMatrix FarField(const Geometry& geom, QuadOption opts, freqtype k, Matrix& alph
     raise(bem2d_farfield(device::context(), h.mesh, k.re, c.p, n, nrhs, (const double*)th.p, na, o.p));
     raise(bem2d_copy_to_host(device::context(), out->data->data(), o.p, sizeof(complex) * na * nrhs));
     return out;
+}
+
+// ---- spectral drivers (lib/bem2d_mathroutines.cpp:208-320, 353-410, 534-653) on the GPU ---------
+static void square(const Matrix& m)
+{
+    if (m.dim[0] != m.dim[1]) throw ArrayMismatch();
+}
+
+void L2NormCond(const Matrix& stiff, double& norm, double& cond)
+{
+    square(stiff);
+    const int64_t n = stiff.dim[0];
+    device::DevBuf A(sizeof(complex) * n * n);
+    raise(bem2d_copy_to_device(device::context(), A.p, stiff.data->data(), sizeof(complex) * n * n));
+    raise(bem2d_l2_norm_cond(device::context(), n, A.p, n, &norm, &cond));
+}
+
+void SingularValues(const Matrix& stiff, pdvector& singvals)
+{
+    square(stiff);
+    const int64_t n = stiff.dim[0];
+    singvals = pdvector(new dvector(n));
+    device::DevBuf A(sizeof(complex) * n * n), S(sizeof(double) * n);
+    raise(bem2d_copy_to_device(device::context(), A.p, stiff.data->data(), sizeof(complex) * n * n));
+    raise(bem2d_singular_values(device::context(), n, A.p, n, (double*)S.p));
+    raise(bem2d_copy_to_host(device::context(), singvals->data(), S.p, sizeof(double) * n));
+}
+
+void HermitianEigenvalues(const Matrix& k, pdvector& evalues)
+{
+    square(k);
+    const int64_t n = k.dim[0];
+    evalues = pdvector(new dvector(n));
+    device::DevBuf A(sizeof(complex) * n * n), W(sizeof(double) * n);
+    raise(bem2d_copy_to_device(device::context(), A.p, k.data->data(), sizeof(complex) * n * n));
+    raise(bem2d_hermitian_eigenvalues(device::context(), n, A.p, n, (double*)W.p));
+    raise(bem2d_copy_to_host(device::context(), evalues->data(), W.p, sizeof(double) * n));
+}
+
+Matrix ChangeBasis(const Matrix& k, const Matrix& m)
+{
+    square(k);
+    square(m);
+    if (k.dim[0] != m.dim[0]) throw ArrayMismatch();
+    const int64_t n = k.dim[0];
+    Matrix out(k);
+    device::DevBuf K(sizeof(complex) * n * n), M(sizeof(complex) * n * n);
+    raise(bem2d_copy_to_device(device::context(), K.p, k.data->data(), sizeof(complex) * n * n));
+    raise(bem2d_copy_to_device(device::context(), M.p, m.data->data(), sizeof(complex) * n * n));
+    raise(bem2d_change_basis(device::context(), n, K.p, n, M.p, n));
+    raise(bem2d_copy_to_host(device::context(), out.data->data(), K.p, sizeof(complex) * n * n));
+    return out;
+}
+
+void NumRange(const Matrix& m, int n, std::string filename)
+{
+    square(m);
+    const int64_t N = m.dim[0];
+    std::vector<double> pts(2 * (2 * (std::size_t)n + 1));
+    device::DevBuf A(sizeof(complex) * N * N);
+    raise(bem2d_copy_to_device(device::context(), A.p, m.data->data(), sizeof(complex) * N * N));
+    raise(bem2d_numerical_range(device::context(), N, A.p, N, n, pts.data()));
+    std::ofstream out(filename.c_str());
+    for (int i = 0; i < 2 * n + 1; ++i) out << pts[2 * i] << " " << pts[2 * i + 1] << std::endl;
+}
+
+Matrix ConjTranspose(const Matrix& m)
+{
+    const int M = m.dim[0], N = m.dim[1];
+    Matrix r(N, M);
+    for (int j = 0; j < N; ++j)
+        for (int i = 0; i < M; ++i)
+            (*r.data)[j + (std::size_t)N * i] = std::conj((*m.data)[i + (std::size_t)M * j]);
+    return r;
 }
 
 pMatrix SolveSystem(Matrix& m, Matrix& rhs)

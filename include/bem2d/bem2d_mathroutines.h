@@ -5,6 +5,8 @@
 #ifndef BEM2D_DROPIN_MATHROUTINES_H
 #define BEM2D_DROPIN_MATHROUTINES_H
 
+#include <string>
+
 #include "bem2d_defs.h"
 #include "bem2d_exceptions.h"
 #include "bem2d_kernel.h"
@@ -52,6 +54,18 @@ Matrix operator*(const Matrix& lhs, const Matrix& rhs);
 
 // X = m^{-1} rhs (FP64 LU with partial pivoting on the GPU); m is not modified.
 pMatrix SolveSystem(Matrix& m, Matrix& rhs);
+
+// Spectral drivers (lib/bem2d_mathroutines.h:48-73), on the GPU (rocSOLVER / rocBLAS):
+// largest singular value and condition number; all singular values (descending); eigenvalues of a
+// Hermitian matrix (ascending); k in the orthonormal basis of the mass matrix m, U^{-H} k U^{-1}
+// with m = U^H U; the boundary of the numerical range at n angles written to filename as
+// "re im" lines; the conjugate transpose.  Eigenvalues of a general matrix are not provided.
+void L2NormCond(const Matrix& stiff, double& norm, double& cond);
+void SingularValues(const Matrix& stiff, pdvector& singvals);
+void HermitianEigenvalues(const Matrix& k, pdvector& evalues);
+Matrix ChangeBasis(const Matrix& k, const Matrix& m);
+void NumRange(const Matrix& m, int n, std::string filename);
+Matrix ConjTranspose(const Matrix& m);
 
 } // namespace bem2d
 

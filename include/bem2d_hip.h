@@ -233,6 +233,30 @@ int bem2d_solve_mixed(bem2d_ctx* ctx, int64_t n, int64_t nrhs, const void* A64, 
 int bem2d_solve_mixed_static(bem2d_ctx* ctx, int64_t n, int64_t nrhs, const void* A64, int64_t lda,
                              void* B, int64_t ldb, double tol, bem2d_ir_report* report);
 
+/* ---- spectral drivers on assembled matrices -------------------------------------------------------
+ * The reference's research routines (lib/bem2d_mathroutines.cpp), on the device with rocSOLVER /
+ * rocBLAS.  Matrices are device pointers and are not modified except where stated. */
+/* ChangeBasis (lib/bem2d_mathroutines.cpp:534-588): with the Cholesky factor M = U^H U (upper) of the
+ * Hermitian positive definite mass matrix M, K is overwritten by U^{-H} K U^{-1} -- the matrix of K
+ * in the M-orthonormal basis.  BEM2D_ERR_SINGULAR if M is not positive definite (the reference
+ * does not check zpotrf's info). */
+int bem2d_change_basis(bem2d_ctx* ctx, int64_t n, void* K, int64_t ldk, const void* M, int64_t ldm);
+/* SingularValues (:266-320): s (device, n doubles) = the singular values of A, descending (zgesvd
+ * without vectors). */
+int bem2d_singular_values(bem2d_ctx* ctx, int64_t n, const void* A, int64_t lda, double* s);
+/* L2NormCond (:208-264): *norm = s_max, *cond = s_max / s_min (host outputs). */
+int bem2d_l2_norm_cond(bem2d_ctx* ctx, int64_t n, const void* A, int64_t lda, double* norm,
+                       double* cond);
+/* HermitianEigenvalues (:353-410): w (device, n doubles) = the eigenvalues of the Hermitian A,
+ * ascending (zheev without vectors, upper triangle referenced). */
+int bem2d_hermitian_eigenvalues(bem2d_ctx* ctx, int64_t n, const void* A, int64_t lda, double* w);
+/* NumRange (:590-653) without the file: the extreme eigenvalues of (e^{i t} A + (e^{i t} A)^H) / 2 at
+ * t = i pi / nsteps, i < nsteps, and the 2 nsteps + 1 boundary points (re, im) of the numerical
+ * range built from them as the reference builds them; points is a HOST array of 2 (2 nsteps + 1)
+ * doubles.  n <= 65535. */
+int bem2d_numerical_range(bem2d_ctx* ctx, int64_t n, const void* A, int64_t lda, int nsteps,
+                          double* points);
+
 /* ---- potential and far field ------------------------------------------------------------------- */
 /* out[p] = sum_j coef[j] * int_{elem j} (i/4) H0(k |P_p - y|) ds(y) with Gauss1D(N):
  * EvalKernel<SingleLayer> (lib/bem2d_quadrature.h:184-228) as used by EvalSol
