@@ -18,6 +18,8 @@ ELEM_FLAT, ELEM_CIRCLE, ELEM_TREFOIL, ELEM_KITE, ELEM_ELLIPSE, ELEM_ELLIPSE_ARC,
 KERNEL_NONE, KERNEL_SL, KERNEL_DL, KERNEL_CDL, KERNEL_COMBINED, KERNEL_ID = range(6)
 FIELD_PLANE, FIELD_NORMAL_PLANE, FIELD_COMBINED_PLANE = range(3)
 PROF_ASSEMBLE_REGULAR, PROF_ASSEMBLE_SPECIAL, PROF_LU, PROF_LU_UPDATE = range(4)
+BASIS_CONST, BASIS_POL, BASIS_LIN, BASIS_WAVE, BASIS_WAVEPOL = range(5)
+MAX_BASES = 8
 
 # Exported symbols, with (restype, argtypes).  Kept in the order of include/bem2d_hip.h.
 _P = ctypes.c_void_p
@@ -33,6 +35,79 @@ class Geometry(ctypes.Structure):
 
 class QuadOption(ctypes.Structure):
     _fields_ = [("L", _I), ("N", _I), ("sigma", _D)]
+
+
+class Basis(ctypes.Structure):
+    """bem2d_basis: one basis function per element (kind BASIS_*; degree = the power of POL /
+    WAVEPOL, the sign of LIN; direction and k of WAVE / WAVEPOL)."""
+    _fields_ = [("kind", _I), ("degree", _I), ("direction", _D), ("k_re", _D), ("k_im", _D)]
+
+
+def const_basis():
+    """ConstBasis (lib/bem2d_basistypes.h:16-25)."""
+    return [(BASIS_CONST, 0, 0.0, 0.0)]
+
+
+def pol_bases(degree):
+    """PolBasis::AddBasis(degree, geom): t^0 .. t^degree (lib/bem2d_basistypes.cpp:12-17)."""
+    return [(BASIS_POL, d, 0.0, 0.0) for d in range(degree + 1)]
+
+
+def lin_bases():
+    """LinBasis::AddBasis(geom): t, then 1 - t (lib/bem2d_basistypes.cpp:21-26)."""
+    return [(BASIS_LIN, 1, 0.0, 0.0), (BASIS_LIN, -1, 0.0, 0.0)]
+
+
+def wave_bases(k):
+    """WaveBasis::AddBasis(k, geom): exp(ikt), exp(-ikt) (lib/bem2d_basistypes.cpp:30-34)."""
+    return [(BASIS_WAVE, 0, 1.0, k), (BASIS_WAVE, 0, -1.0, k)]
+
+
+def wavepol_bases(degree, k):
+    """WavePolBasis::AddBasis(degree, k, geom): exp(+-ikt) t^d, d = 0 .. degree, the + direction
+    first for each d (lib/bem2d_basistypes.cpp:38-44)."""
+    out = []
+    for d in range(degree + 1):
+        out += [(BASIS_WAVEPOL, d, 1.0, k), (BASIS_WAVEPOL, d, -1.0, k)]
+    return out
+
+
+def basis_array(bases):
+    """ctypes array of bem2d_basis from a list of (kind, degree, direction, k) tuples."""
+    arr = (Basis * len(bases))()
+    for i, (kind, deg, dr, k) in enumerate(bases):
+        kc = complex(k)
+        arr[i] = Basis(int(kind), int(deg), float(dr), kc.real, kc.imag)
+    return arr
+
+
+def basis_values(bases, t):
+    """phi_b(t_q) on the host: (len(t), nb) complex array."""
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    out = np.zeros((t.size, len(bases), 2))
+    check(lib().bem2d_basis_values(basis_array(bases), len(bases), _np_ptr(t), t.size,
+                                   _np_ptr(out)))
+    return out[..., 0] + 1j * out[..., 1]
+
+
+def curve_length(curve_type, args, t0=0.0, t1=1.0):
+    """Curve::Length(t0, t1) of curve type ELEM_CIRCLE .. ELEM_INV_ELLIPSE."""
+    a = np.zeros(4)
+    a[:len(args)] = args
+    out = _D()
+    check(lib().bem2d_curve_length(int(curve_type), _np_ptr(a), t0, t1, ctypes.byref(out)))
+    return out.value
+
+
+def arclength_params(curve_type, args, s):
+    """Curve parameters t_i at arc lengths s_i (non-decreasing, from t = 0)."""
+    a = np.zeros(4)
+    a[:len(args)] = args
+    s = np.ascontiguousarray(s, dtype=np.float64)
+    t = np.zeros(s.size)
+    check(lib().bem2d_arclength_params(int(curve_type), _np_ptr(a), _np_ptr(s), s.size,
+                                       _np_ptr(t)))
+    return t
 
 
 class IRReport(ctypes.Structure):
@@ -83,6 +158,13 @@ SIGNATURES = {
     "bem2d_numerical_range": (_I, [_P, _I64, _P, _I64, _I, _P]),
     "bem2d_eval_single_layer": (_I, [_P, _P, _D, _P, _P, _I64, _P]),
     "bem2d_farfield": (_I, [_P, _P, _D, _P, _I64, _I, _P, _I64, _P]),
+    "bem2d_assemble_basis": (_I, [_P, _P, _P, _I, _I, _D, _D, _D, _D, _D, _P, _I64]),
+    "bem2d_rhs_basis": (_I, [_P, _P, _P, _I, _I, _D, _D, _D, _P, _I, _D, _P, _I64]),
+    "bem2d_eval_single_layer_basis": (_I, [_P, _P, _P, _I, _D, _D, _P, _P, _I64, _P]),
+    "bem2d_farfield_basis": (_I, [_P, _P, _P, _I, _D, _P, _I64, _I, _P, _I64, _P]),
+    "bem2d_basis_values": (_I, [_P, _I, _P, _I64, _P]),
+    "bem2d_curve_length": (_I, [_I, _P, _D, _D, _P]),
+    "bem2d_arclength_params": (_I, [_I, _P, _P, _I64, _P]),
     "bem2d_profile_enable": (_I, [_P, _I]),
     "bem2d_profile_reset": (_I, [_P]),
     "bem2d_profile_read": (_I, [_P, _I, ctypes.POINTER(_D), ctypes.POINTER(_I64)]),
@@ -411,6 +493,58 @@ class Context:
         out = torch.empty((nrhs, nang), dtype=torch.complex128, device=theta.device).t()
         check(lib().bem2d_farfield(self.h, mesh.h, k, _t_ptr(coef), _ld(coef), nrhs,
                                    _t_ptr(theta.contiguous()), nang, _t_ptr(out)))
+        return out
+
+
+    # ---- several bases per element (bases: list of (kind, degree, direction, k)) -------------
+    def assemble_basis(self, mesh, bases, kind, k, eta=0.0, mass_scale=0.0, kernel_scale=1.0,
+                       out=None):
+        """The (n nb) x (n nb) matrix, row / column element * nb + basis, column-major view."""
+        import torch
+        nn = mesh.n * len(bases)
+        if out is None:
+            out = torch.empty((nn, nn), dtype=torch.complex128,
+                              device="cuda:%d" % self.device).t()
+        assert out.stride(0) == 1 and out.shape == (nn, nn)
+        kc = complex(k)
+        check(lib().bem2d_assemble_basis(self.h, mesh.h, basis_array(bases), len(bases), kind,
+                                         kc.real, kc.imag, eta, mass_scale, kernel_scale,
+                                         _t_ptr(out), _ld(out)))
+        return out
+
+    def rhs_basis(self, mesh, bases, kind, k, eta, dirs, scale=1.0, out=None):
+        import torch
+        dirs = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1, 2)
+        nrhs = dirs.shape[0]
+        nn = mesh.n * len(bases)
+        if out is None:
+            out = torch.empty((nrhs, nn), dtype=torch.complex128,
+                              device="cuda:%d" % self.device).t()
+        kc = complex(k)
+        check(lib().bem2d_rhs_basis(self.h, mesh.h, basis_array(bases), len(bases), kind,
+                                    kc.real, kc.imag, eta, _np_ptr(dirs), nrhs, scale,
+                                    _t_ptr(out), _ld(out)))
+        return out
+
+    def eval_single_layer_basis(self, mesh, bases, k, coef, pts):
+        import torch
+        pts = pts.contiguous()
+        out = torch.empty(pts.shape[0], dtype=torch.complex128, device=pts.device)
+        kc = complex(k)
+        check(lib().bem2d_eval_single_layer_basis(self.h, mesh.h, basis_array(bases), len(bases),
+                                                  kc.real, kc.imag, _t_ptr(coef.contiguous()),
+                                                  _t_ptr(pts), pts.shape[0], _t_ptr(out)))
+        return out
+
+    def farfield_basis(self, mesh, bases, k, coef, theta):
+        """coef: (n nb) x nrhs column-major; returns nang x nrhs column-major."""
+        import torch
+        nrhs = coef.shape[1]
+        nang = theta.numel()
+        out = torch.empty((nrhs, nang), dtype=torch.complex128, device=theta.device).t()
+        check(lib().bem2d_farfield_basis(self.h, mesh.h, basis_array(bases), len(bases), k,
+                                         _t_ptr(coef), _ld(coef), nrhs,
+                                         _t_ptr(theta.contiguous()), nang, _t_ptr(out)))
         return out
 
 

@@ -13,63 +13,12 @@
 
 #pragma clang diagnostic ignored "-Wunused-function" // real-k kernels of the shared header
 #include "assembly_kernels.h"
-#include "hankel_complex.h"
+#include "complex_kernels.h"
 #include "internal.h"
 
 using namespace bem2d_dev;
 
 namespace {
-
-// Kernel functors for complex k = (kr, ki); tab / nt = contour table of arg k.
-template <int KIND>
-__host__ __device__ __forceinline__ void kernel_eval_ck(double kr, double ki, double eta,
-                                                        const double* tab, int nt, double xx,
-                                                        double xy, double yx, double yy,
-                                                        double n1x, double n1y, double n2x,
-                                                        double n2y, double& vr, double& vi)
-{
-    double wx, wy, d2, nw;
-    {
-#pragma clang fp contract(off)
-        wx = xx - yx; // x - y
-        wy = xy - yy;
-        d2 = wx * wx + wy * wy;
-        if (KIND == BEM2D_KERNEL_DL)
-            nw = n2x * wx + n2y * wy;
-        else
-            nw = n1x * wx + n1y * wy;
-    }
-    const double d = sqrt(d2);
-    double h0r, h0i, h1r, h1i;
-    hankel01_complex(kr * d, ki * d, tab, nt, h0r, h0i, h1r, h1i);
-    if (KIND == BEM2D_KERNEL_SL) { // (i/4) H0(kd)
-        vr = -0.25 * h0i;
-        vi = 0.25 * h0r;
-        return;
-    }
-    // -(i k/4) H1(kd) c, c = n_y.(y - x)/d for DL, n_x.(x - y)/d otherwise
-    const double c = (KIND == BEM2D_KERNEL_DL ? -nw : nw) / d;
-    const double ar = kr * h1r - ki * h1i, ai = kr * h1i + ki * h1r; // k H1
-    vr = 0.25 * c * ai;
-    vi = -0.25 * c * ar;
-    if (KIND == BEM2D_KERNEL_COMBINED) { // CDL - i eta (i/4) H0
-        vr += 0.25 * eta * h0r;
-        vi += 0.25 * eta * h0i;
-    }
-}
-
-struct ComplexKernelEval {
-    double kr, ki, eta;
-    const double* tab;
-    int nt;
-    template <int KIND>
-    __device__ __forceinline__ void eval(double xx, double xy, double yx, double yy, double n1x,
-                                         double n1y, double n2x, double n2y, double& vr,
-                                         double& vi) const
-    {
-        kernel_eval_ck<KIND>(kr, ki, eta, tab, nt, xx, xy, yx, yy, n1x, n1y, n2x, n2y, vr, vi);
-    }
-};
 
 template <int KIND>
 __global__ __launch_bounds__(256) void assemble_special_ck_kernel(const AsmBatch batch, double ki,
@@ -208,8 +157,16 @@ __global__ void hankel01_complex_kernel(const double* __restrict__ tab, int nt,
 
 bool wavenumber_ok(double kr, double ki) { return kr > 0.0 && std::isfinite(kr) && std::isfinite(ki); }
 
-// Contour table of arg k in device scratch, uploaded in stream order on c->stream.
 int upload_contour(bem2d_ctx* c, double kr, double ki, const double** d_tab, int* nt)
+{
+    return bem2d_internal_upload_contour(c, kr, ki, d_tab, nt);
+}
+
+} // namespace
+
+// Contour table of arg k in device scratch, uploaded in stream order on c->stream.
+int bem2d_internal_upload_contour(bem2d_ctx* c, double kr, double ki, const double** d_tab,
+                                  int* nt)
 {
     double tab[kContourMaxNodes * kContourDoubles];
     *nt = contour_table(std::atan2(ki, kr), tab);
@@ -221,6 +178,8 @@ int upload_contour(bem2d_ctx* c, double kr, double ki, const double** d_tab, int
     *d_tab = d;
     return BEM2D_OK;
 }
+
+namespace {
 
 template <int KIND>
 int launch_ck(bem2d_ctx* c, const bem2d_mesh* m, double kr, double ki, double eta, double mscale,

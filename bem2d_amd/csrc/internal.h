@@ -103,6 +103,10 @@ void bem2d_internal_dfree(DevArena* a, void* p);
 
 int bem2d_internal_fail(int code, const char* fmt, ...);
 void* bem2d_internal_scratch(bem2d_ctx* c, int slot, size_t bytes);
+// Contour table of the complex wavenumber kr + i ki in device scratch (SCRATCH_CONTOUR), uploaded in
+// stream order on c->stream; *nt = its node count.
+int bem2d_internal_upload_contour(bem2d_ctx* c, double kr, double ki, const double** d_tab,
+                                  int* nt);
 void bem2d_internal_mixed_release(bem2d_ctx* c);
 void bem2d_internal_gauss01(int n, std::vector<double>& x, std::vector<double>& w);
 size_t bem2d_internal_mesh_bytes(int64_t n, bem2d_quad_option q);

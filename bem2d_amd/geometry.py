@@ -7,6 +7,10 @@ mirror the reference's element constructions:
   curve_mesh         AnalyticCurveElement(i/n, (i+1)/n, curve) with ring adjacency, as built by hand
                      in the reference's drivers (normcomps/cracknormcond.cpp:67-75) -- uniform in
                      the curve parameter, not by arc length
+  analytic_curve     AnalyticCurve(n, curve, closed): n pieces of equal arc length
+                     (lib/bem2d_shape.cpp:252-276)
+  analytic_curve_ppw AnalyticCurve(ppw, k, curve, closed, L, sigma): ppw pieces per wavelength,
+                     the first and last graded (lib/bem2d_shape.cpp:278-331)
   disk_chords        DiskShapePiecewiseConst(n, r)        lib/bem2d_shape.cpp:11-34
   polygon            Polygon(points, n)                   lib/bem2d_shape.cpp:114-138
   line               Line(p0, p1, n) with open ends        lib/bem2d_shape.cpp:225-236
@@ -15,6 +19,7 @@ import math
 
 import numpy as np
 
+from . import capi
 from .capi import (ELEM_CIRCLE, ELEM_ELLIPSE, ELEM_ELLIPSE_ARC, ELEM_FLAT, ELEM_INV_ELLIPSE,
                    ELEM_KITE, ELEM_TREFOIL)
 
@@ -59,6 +64,65 @@ def curve_mesh(curve, n, params=(0.0, 0.0, 0.0, 0.0), closed=True):
     param[:, 1] = (i + 1) / n
     param[:, 2:2 + len(params)] = params
     return Geometry(np.full(n, ty), param, *_ring(n, closed))
+
+
+def _curve(curve, params):
+    ty = CURVES[curve] if isinstance(curve, str) else int(curve)
+    args = np.zeros(4)
+    args[:len(params)] = params
+    return ty, args
+
+
+def _arclength_mesh(ty, args, partition, closed):
+    """Elements between the curve parameters of the arc lengths partition[0 .. n] (partition[0] =
+    0); the last element ends at t = 1 (AnalyticCurve::ParameterizeArc sets it)."""
+    n = len(partition) - 1
+    t = np.zeros(n + 1)
+    t[1:] = capi.arclength_params(ty, args, np.asarray(partition[1:], dtype=np.float64))
+    t[n] = 1.0
+    param = np.zeros((n, 8))
+    param[:, 0] = t[:-1]
+    param[:, 1] = t[1:]
+    param[:, 2:6] = args
+    return Geometry(np.full(n, ty), param, *_ring(n, closed))
+
+
+def analytic_curve(curve, n, params=(0.0, 0.0, 0.0, 0.0), closed=True):
+    """AnalyticCurve(n, curve, closed): n elements of equal arc length."""
+    ty, args = _curve(curve, params)
+    L = capi.curve_length(ty, args)
+    return _arclength_mesh(ty, args, [i * L / n for i in range(n + 1)], closed)
+
+
+def analytic_curve_ppw(curve, ppw, k, params=(0.0, 0.0, 0.0, 0.0), closed=True, L=0, sigma=0.15):
+    """AnalyticCurve(ppw, k, curve, closed, L, sigma): c = max(ceil(ppw k len / 2 pi), 10) pieces
+    of equal arc length, the first and the last split into L + 1 pieces graded by sigma towards
+    the curve's ends."""
+    ty, args = _curve(curve, params)
+    length = capi.curve_length(ty, args)
+    c = max(int(math.ceil(ppw * float(k) * length / 2.0 / math.pi)), 10)
+    part = [0.0]
+    if L == 0:
+        part.append(length / c)
+    else:
+        ts = 1.0
+        for _ in range(L):
+            ts = ts * sigma
+        part.append(ts * length / c)
+        for _ in range(L):
+            ts = ts / sigma
+            part.append(ts * length / c)
+    part += [length / c * j for j in range(2, c)]
+    if L == 0:
+        part.append(length)
+    else:
+        td = length / c * (c - 1)
+        ts = 1.0
+        for _ in range(L):
+            ts = ts * sigma
+            part.append(td + length / c * (1 - ts))
+        part.append(length)
+    return _arclength_mesh(ty, args, part, closed)
 
 
 def disk_chords(n, radius=1.0):

@@ -2,6 +2,8 @@
 // Host code only: flattens geometries into the C-ABI's element format, moves matrices between the
 // reference's host Matrix and device buffers, and maps C-ABI statuses onto the reference's
 // exception tags.
+#include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -60,9 +62,18 @@ complex BesselH1(complex z)
     return complex(v[2], v[3]);
 }
 
-// ---- Curve::Length ---------------------------------------------------------------------------
+// ---- Curve::Length (lib/bem2d_curve.cpp:11-25) ---------------------------------------------
+// Curves the library knows: its adaptive Gauss-Legendre rule (rounding-level accurate); other
+// curves: composite 16-point Gauss-Legendre on 64 panels of the virtual Deriv.
 double Curve::Length(double t1, double t2) const
 {
+    int type = -1;
+    double args[4] = {0, 0, 0, 0};
+    if (device_desc(type, args) && t1 <= t2) {
+        double out = 0;
+        raise(bem2d_curve_length(type, args, t1, t2, &out));
+        return out;
+    }
     const int panels = 64, q = 16;
     double x[q], w[q];
     bem2d_gauss_legendre(q, x, w);
@@ -148,41 +159,75 @@ struct DevBuf {
     DevBuf& operator=(const DevBuf&) = delete;
 };
 
-// Flattened geometry + its bem2d_mesh.
+// Flattened geometry + its bem2d_mesh: n elements, each carrying the same nb bases (the FlatMap
+// rows / columns are element * nb + basis).
 struct MeshHandle {
     bem2d_mesh* mesh = nullptr;
     int64_t n = 0;
+    std::vector<bem2d_basis> bases;
+    int nb() const { return (int)bases.size(); }
+    int64_t size() const { return n * nb(); }
+    // one ConstBasis or PolBasis(0) per element: the P0 kernels
+    bool p0() const
+    {
+        return bases.size() == 1 && (bases[0].kind == BEM2D_BASIS_CONST ||
+                                     (bases[0].kind == BEM2D_BASIS_POL && bases[0].degree == 0));
+    }
     ~MeshHandle()
     {
         if (mesh) bem2d_mesh_destroy(mesh);
     }
 };
 
-static void flatten(const Geometry& geom, QuadOption opts, MeshHandle& h)
+static bool same_basis(const bem2d_basis& a, const bem2d_basis& b)
 {
-    auto flat = geom.FlatMap();
-    const std::size_t n = flat->size();
-    if (n == 0) throw ParameterException();
-    std::vector<int32_t> type(n);
-    std::vector<double> param(8 * n);
-    std::vector<int64_t> index(n), next(n), prev(n);
-    for (std::size_t i = 0; i < n; i++) {
-        const pElement& e = (*flat)[i].first;
-        const pBasis& b = (*flat)[i].second;
-        // The device path is P0: one PolBasis(0) per element, FlatMap order = element order.
-        if (!b || b->pol_degree() != 0 || (i > 0 && (*flat)[i - 1].first == e))
+    return a.kind == b.kind && a.degree == b.degree && a.direction == b.direction &&
+           a.k_re == b.k_re && a.k_im == b.k_im;
+}
+
+// The FlatMap must be element-major with the same basis list on every element (what AddBasis on
+// the whole geometry produces); other layouts throw ParameterException.
+static void flatten(const Geometry::flat_basis_map& flat, QuadOption opts, MeshHandle& h)
+{
+    if (flat.empty()) throw ParameterException();
+    std::vector<int32_t> type;
+    std::vector<double> param;
+    std::vector<int64_t> index, next, prev;
+    for (std::size_t i = 0; i < flat.size();) {
+        const pElement& e = flat[i].first;
+        std::vector<bem2d_basis> list;
+        std::size_t j = i;
+        for (; j < flat.size() && flat[j].first == e; ++j) {
+            bem2d_basis d;
+            if (!flat[j].second || !flat[j].second->device_desc(d)) throw ParameterException();
+            list.push_back(d);
+        }
+        if (type.empty()) {
+            if (list.size() > BEM2D_MAX_BASES) throw ParameterException();
+            h.bases = list;
+        } else if (list.size() != h.bases.size() ||
+                   !std::equal(list.begin(), list.end(), h.bases.begin(), same_basis)) {
             throw ParameterException();
+        }
         int t = -1;
-        if (!e->device_desc(t, &param[8 * i])) throw ParameterException();
-        type[i] = t;
-        index[i] = (int64_t)e->index();
-        next[i] = e->next() == (std::size_t)-1 ? -1 : (int64_t)e->next();
-        prev[i] = e->prev() == (std::size_t)-1 ? -1 : (int64_t)e->prev();
+        param.resize(param.size() + 8);
+        if (!e->device_desc(t, &param[param.size() - 8])) throw ParameterException();
+        type.push_back(t);
+        index.push_back((int64_t)e->index());
+        next.push_back(e->next() == (std::size_t)-1 ? -1 : (int64_t)e->next());
+        prev.push_back(e->prev() == (std::size_t)-1 ? -1 : (int64_t)e->prev());
+        i = j;
     }
-    bem2d_geometry g{(int64_t)n, type.data(), param.data(), index.data(), next.data(), prev.data()};
+    const int64_t n = (int64_t)type.size();
+    bem2d_geometry g{n, type.data(), param.data(), index.data(), next.data(), prev.data()};
     bem2d_quad_option q{opts.L, opts.N, opts.sigma};
     raise(bem2d_mesh_create(context(), &g, q, &h.mesh));
-    h.n = (int64_t)n;
+    h.n = n;
+}
+
+static void flatten(const Geometry& geom, QuadOption opts, MeshHandle& h)
+{
+    flatten(*geom.FlatMap(), opts, h);
 }
 
 static pMatrix assemble(const Geometry& geom, QuadOption opts, int kind, freqtype k, double eta,
@@ -190,11 +235,15 @@ static pMatrix assemble(const Geometry& geom, QuadOption opts, int kind, freqtyp
 {
     MeshHandle h;
     flatten(geom, opts, h);
-    const int64_t n = h.n;
+    const int64_t n = h.size();
     pMatrix out(new Matrix((int)n));
     DevBuf A(sizeof(complex) * n * n);
     // k.im != 0: the kernels' complex-k branch (lib/bem2d_kernel.cpp:22-26 and alike)
-    raise(bem2d_assemble_complex(context(), h.mesh, kind, k.re, k.im, eta, ms, ks, A.p, n));
+    if (h.p0())
+        raise(bem2d_assemble_complex(context(), h.mesh, kind, k.re, k.im, eta, ms, ks, A.p, n));
+    else
+        raise(bem2d_assemble_basis(context(), h.mesh, h.bases.data(), h.nb(), kind, k.re, k.im,
+                                   eta, ms, ks, A.p, n));
     raise(bem2d_copy_to_host(context(), out->data->data(), A.p, sizeof(complex) * n * n));
     return out;
 }
@@ -210,11 +259,15 @@ static pMatrix rhs(const Geometry& geom, QuadOption opts, int kind, freqtype k, 
 {
     MeshHandle h;
     flatten(geom, opts, h);
-    const int64_t n = h.n;
+    const int64_t n = h.size();
     pMatrix out(new Matrix((int)n, 1));
     DevBuf B(sizeof(complex) * n);
     double d[2] = {dir.x, dir.y};
-    raise(bem2d_rhs_complex(context(), h.mesh, kind, k.re, k.im, eta, d, 1, 1.0, B.p, n));
+    if (h.p0())
+        raise(bem2d_rhs_complex(context(), h.mesh, kind, k.re, k.im, eta, d, 1, 1.0, B.p, n));
+    else
+        raise(bem2d_rhs_basis(context(), h.mesh, h.bases.data(), h.nb(), kind, k.re, k.im, eta, d,
+                              1, 1.0, B.p, n));
     raise(bem2d_copy_to_host(context(), out->data->data(), B.p, sizeof(complex) * n));
     return out;
 }
@@ -263,14 +316,10 @@ void EvalKernel(const Geometry::flat_basis_map& basfuns, const std::vector<Point
                 cvector& vals, Matrix& alpha, SingleLayer g, const Gauss1D& g1d)
 {
     if (points.size() != vals.size()) throw SizeError();
-    std::vector<pElement> elems;
-    for (auto& fb : basfuns) elems.push_back(fb.first);
-    Geometry geom(elems);
-    geom.AddBasis(basfuns.empty() ? pBasis() : basfuns[0].second);
     QuadOption opts{1, g1d.Size(), 0.5};
     device::MeshHandle h;
-    device::flatten(geom, opts, h);
-    const int64_t n = h.n, np = (int64_t)points.size();
+    device::flatten(basfuns, opts, h);
+    const int64_t n = h.size(), np = (int64_t)points.size();
     if ((int64_t)alpha.data->size() < n) throw SizeError();
     if (np == 0) return;
     device::DevBuf c(sizeof(complex) * n), P(sizeof(double) * 2 * np), out(sizeof(complex) * np);
@@ -281,8 +330,13 @@ void EvalKernel(const Geometry::flat_basis_map& basfuns, const std::vector<Point
     }
     raise(bem2d_copy_to_device(device::context(), c.p, alpha.data->data(), sizeof(complex) * n));
     raise(bem2d_copy_to_device(device::context(), P.p, pts.data(), sizeof(double) * 2 * np));
-    raise(bem2d_eval_single_layer_complex(device::context(), h.mesh, g.k().re, g.k().im, c.p,
-                                          (const double*)P.p, np, out.p));
+    if (h.p0())
+        raise(bem2d_eval_single_layer_complex(device::context(), h.mesh, g.k().re, g.k().im, c.p,
+                                              (const double*)P.p, np, out.p));
+    else
+        raise(bem2d_eval_single_layer_basis(device::context(), h.mesh, h.bases.data(), h.nb(),
+                                            g.k().re, g.k().im, c.p, (const double*)P.p, np,
+                                            out.p));
     cvector v(np);
     raise(bem2d_copy_to_host(device::context(), v.data(), out.p, sizeof(complex) * np));
     for (int64_t i = 0; i < np; i++) vals[i] += v[i];
@@ -294,7 +348,7 @@ pMatrix FarField(const Geometry& geom, QuadOption opts, freqtype k, Matrix& alph
     if (k.im != 0) throw ParameterException();
     device::MeshHandle h;
     device::flatten(geom, opts, h);
-    const int64_t n = h.n, na = (int64_t)theta.size();
+    const int64_t n = h.size(), na = (int64_t)theta.size();
     const int nrhs = alpha.dim[1];
     if (alpha.dim[0] != n || na == 0) throw SizeError();
     pMatrix out(new Matrix((int)na, nrhs));
@@ -302,10 +356,134 @@ pMatrix FarField(const Geometry& geom, QuadOption opts, freqtype k, Matrix& alph
         o(sizeof(complex) * na * nrhs);
     raise(bem2d_copy_to_device(device::context(), c.p, alpha.data->data(), sizeof(complex) * n * nrhs));
     raise(bem2d_copy_to_device(device::context(), th.p, theta.data(), sizeof(double) * na));
-    raise(bem2d_farfield(device::context(), h.mesh, k.re, c.p, n, nrhs, (const double*)th.p, na, o.p));
+    if (h.p0())
+        raise(bem2d_farfield(device::context(), h.mesh, k.re, c.p, n, nrhs, (const double*)th.p, na,
+                             o.p));
+    else
+        raise(bem2d_farfield_basis(device::context(), h.mesh, h.bases.data(), h.nb(), k.re, c.p, n,
+                                   nrhs, (const double*)th.p, na, o.p));
     raise(bem2d_copy_to_host(device::context(), out->data->data(), o.p, sizeof(complex) * na * nrhs));
     return out;
 }
+
+// ---- point in polygon (InPolygon, lib/bem2d_mathroutines.cpp:732-758) -------------------------
+// Crossing-number test: a horizontal ray from the point crosses the closed polygon through the
+// vertices an odd number of times iff the point is inside.
+static int inside(const PointVector& v, const Point& p)
+{
+    int c = 0;
+    const std::size_t m = v.size();
+    for (std::size_t i = 0, j = m - 1; i < m; j = i++) {
+        const bool straddle = (v[i].y > p.y) != (v[j].y > p.y);
+        if (straddle && p.x < (v[j].x - v[i].x) * (p.y - v[i].y) / (v[j].y - v[i].y) + v[i].x)
+            c = !c;
+    }
+    return c;
+}
+
+void InPolygon(const std::vector<pGeometry> polygons, const std::vector<Point>& testpoints,
+               std::vector<int>& inpoly)
+{
+    inpoly.assign(testpoints.size(), 0);
+    for (const pGeometry& g : polygons) {
+        const PointVector& v = g->points();
+        if (v.empty()) continue;
+        for (std::size_t j = 0; j < testpoints.size(); ++j) inpoly[j] += inside(v, testpoints[j]);
+    }
+}
+
+void InPolygon(const pGeometry polygon, const std::vector<Point>& testpoints,
+               std::vector<int>& inpoly)
+{
+    InPolygon(std::vector<pGeometry>{polygon}, testpoints, inpoly);
+}
+
+// ---- AnalyticCurve (lib/bem2d_shape.cpp:247-331) -----------------------------------------------
+static void curve_desc(const pCurve& curve, int& type, double* args)
+{
+    if (!curve || !curve->device_desc(type, args)) throw ParameterException();
+}
+
+AnalyticCurve::AnalyticCurve(int n, pCurve curve, int closed) : curve_(curve)
+{
+    if (n < 2) throw ParameterException();
+    const double curveLength = curve_->Length();
+    dvector partition(n + 1);
+    for (int i = 0; i <= n; i++) partition[i] = i * curveLength / n;
+    ParameterizeArc(partition);
+    build(closed);
+}
+
+AnalyticCurve::AnalyticCurve(int ppw, freqtype k, pCurve curve, int closed, int L, double sigma)
+    : curve_(curve)
+{
+    const double curveLength = curve_->Length();
+    dvector partition;
+    partition.push_back(0);
+    int c = (int)std::ceil(ppw * (double)k.re * curveLength / 2.0 / PI);
+    c = std::max(c, 10);
+    // graded towards both ends: L pieces of ratio sigma inside the first and the last interval
+    if (L == 0) {
+        partition.push_back(curveLength / c);
+    } else {
+        double tsigma = 1;
+        for (int m = 0; m < L; m++) tsigma = tsigma * sigma;
+        partition.push_back(tsigma * curveLength / c);
+        for (int m = 0; m < L; m++) {
+            tsigma = tsigma / sigma;
+            partition.push_back(tsigma * curveLength / c);
+        }
+    }
+    for (int j = 2; j < c; j++) partition.push_back(curveLength / c * j);
+    if (L == 0) {
+        partition.push_back(curveLength);
+    } else {
+        const double td = curveLength / c * (c - 1);
+        double tsigma = 1;
+        for (int m = 0; m < L; m++) {
+            tsigma = tsigma * sigma;
+            partition.push_back(td + curveLength / c * (1 - tsigma));
+        }
+        partition.push_back(curveLength);
+    }
+    ParameterizeArc(partition);
+    build(closed);
+}
+
+void AnalyticCurve::ParameterizeArc(dvector& partition)
+{
+    const int n = (int)partition.size() - 1;
+    if (n < 1) throw ParameterException();
+    int type = -1;
+    double args[4] = {0, 0, 0, 0};
+    curve_desc(curve_, type, args);
+    arclengthparam.assign(n + 1, 0.0);
+    raise(bem2d_arclength_params(type, args, &partition[1], n, &arclengthparam[1]));
+    arclengthparam[0] = 0;
+    arclengthparam[n] = 1; // the curve's end, as the reference sets it
+}
+
+void AnalyticCurve::build(int closed)
+{
+    const int n = (int)arclengthparam.size() - 1;
+    elements_.resize(n);
+    for (int i = 0; i < n - 1; i++)
+        elements_[i] = pElement(new AnalyticCurveElement(arclengthparam[i], arclengthparam[i + 1],
+                                                         curve_, i));
+    elements_[n - 1] = pElement(new AnalyticCurveElement(arclengthparam[n - 1], 1, curve_, n - 1));
+    for (int i = 1; i < n - 1; i++) {
+        elements_[i]->set_next(i + 1);
+        elements_[i]->set_prev(i - 1);
+    }
+    elements_[0]->set_next(1);
+    elements_[n - 1]->set_prev(n - 2);
+    if (closed) {
+        elements_[0]->set_prev(n - 1);
+        elements_[n - 1]->set_next(0);
+    }
+}
+
+pGeometry AnalyticCurve::GetGeometry() { return pGeometry(new Geometry(elements_)); }
 
 // ---- spectral drivers (lib/bem2d_mathroutines.cpp:208-320, 353-410, 534-653) on the GPU ---------
 static void square(const Matrix& m)

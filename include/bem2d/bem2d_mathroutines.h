@@ -52,6 +52,14 @@ Matrix operator*(const Matrix& lhs, const double& alpha);
 Matrix operator*(const double& alpha, const Matrix& rhs);
 Matrix operator*(const Matrix& lhs, const Matrix& rhs);
 
+// inpoly[j] = number of the polygons (closed vertex lists Geometry::points(), the first points of
+// the elements) that contain testpoints[j] (lib/bem2d_mathroutines.cpp:732-758).
+class Geometry;
+void InPolygon(const std::vector<std::shared_ptr<Geometry>> polygons,
+               const std::vector<Point>& testpoints, std::vector<int>& inpoly);
+void InPolygon(const std::shared_ptr<Geometry> polygon, const std::vector<Point>& testpoints,
+               std::vector<int>& inpoly);
+
 // X = m^{-1} rhs (FP64 LU with partial pivoting on the GPU); m is not modified.
 pMatrix SolveSystem(Matrix& m, Matrix& rhs);
 

@@ -1,13 +1,17 @@
-// Drop-in replacement of the element builders of lib/bem2d_shape.h:15-47 / bem2d_shape.cpp:11-236
-// that need no GSL: DiskShapePiecewiseConst(n, r), Polygon(points, n), Line(p0, p1, n).
-// (The arc-length builders AnalyticCurve(...) and the ppw-graded Polygon/Line are not part of this
-// build; meshes on analytic curves are built from AnalyticCurveElement at uniform parameter.)
+// Drop-in replacement of the element builders of lib/bem2d_shape.h:15-62 / bem2d_shape.cpp:11-368:
+// DiskShapePiecewiseConst(n, r), Polygon(points, n), Line(p0, p1, n) and AnalyticCurve (n pieces
+// of equal arc length, or ppw points per wavelength graded towards the ends).  AnalyticCurve's
+// arc-length parameterisation comes from the library (bem2d_arclength_params: adaptive
+// Gauss-Legendre and Newton's method, to rounding level) instead of GSL's RK2 integration of
+// dt/ds to 1e-10; it needs a curve the library knows (Curve::device_desc), ParameterException
+// otherwise.  (The ppw-graded Polygon/Line builders are not part of this build.)
 #ifndef BEM2D_DROPIN_SHAPE_H
 #define BEM2D_DROPIN_SHAPE_H
 
 #include <cmath>
 #include <vector>
 
+#include "bem2d_curve.h"
 #include "bem2d_element.h"
 #include "bem2d_geometry.h"
 
@@ -59,6 +63,21 @@ public:
 
 private:
     std::vector<pElement> elements_;
+};
+
+class AnalyticCurve {
+public:
+    AnalyticCurve(int n, pCurve curve, int closed = 1);
+    AnalyticCurve(int ppw, freqtype k, pCurve curve, int closed = 1, int L = 0,
+                  double sigma = 0.15);
+    pGeometry GetGeometry();
+    void ParameterizeArc(dvector& partition);
+
+private:
+    void build(int closed);
+    std::vector<pElement> elements_;
+    pCurve curve_;
+    dvector arclengthparam;
 };
 
 class Line {

@@ -2,7 +2,8 @@
 // DiscretizeMatrix assembles A = M + 2 K_combined on the GPU in one pass (the reference's
 // DiscreteKernel + EvalIdent + (*pA_)=(*pId_)+2.0*(*pA_), lib/bem2d_soundsoftscattering.h:95-103);
 // Solve uses the FP64 GPU LU.
-// Plotting/output members (SetOutput, Write*) and NormCond are not part of this build.
+// The interior mask of EvalIncident (set_polygons, set_plotInterior) is kept; the plotting/output
+// members (SetOutput, Write*) and NormCond are not part of this build.
 #ifndef BEM2D_DROPIN_SOUNDSOFTSCATTERING_H
 #define BEM2D_DROPIN_SOUNDSOFTSCATTERING_H
 
@@ -61,12 +62,21 @@ public:
     inline pMatrix GetIdent() const { return pId_; }
     inline pMatrix GetSolution() const { return psol_; }
 
+    // Points inside the polygons given by set_polygons get NaN once set_plotInterior was called
+    // (lib/bem2d_soundsoftscattering.h:128-145).
     pcvector EvalIncident(const std::vector<Point>& points)
     {
         pcvector pz(new cvector(points.size()));
-        for (std::size_t i = 0; i < points.size(); i++) (*pz)[i] = incident_(points[i]);
+        std::vector<int> inpoly(points.size(), 0);
+        if (plotInterior_ && !polygons_.empty()) InPolygon(polygons_, points, inpoly);
+        for (std::size_t i = 0; i < points.size(); i++)
+            (*pz)[i] = inpoly[i] == 0 ? incident_(points[i])
+                                      : complex(std::numeric_limits<double>::quiet_NaN(), 0.0);
         return pz;
     }
+    void set_polygons(std::vector<pGeometry>& polygons) { polygons_ = polygons; }
+    void set_polygons(pGeometry polygon) { polygons_ = std::vector<pGeometry>{polygon}; }
+    void set_plotInterior() { plotInterior_ = true; }
 
     // u = u_inc - S phi at the points (lib/bem2d_soundsoftscattering.h:149-168).
     pcvector EvalSol(const std::vector<Point>& points)
@@ -91,6 +101,8 @@ private:
     T2 frhs_;
     QuadOption quadopts_;
     pMatrix pA_, pId_, prhs_, psol_;
+    std::vector<pGeometry> polygons_;
+    bool plotInterior_ = false;
 };
 
 } // namespace bem2d

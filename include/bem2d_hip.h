@@ -272,6 +272,68 @@ int bem2d_eval_single_layer(bem2d_ctx* ctx, const bem2d_mesh* mesh, double k, co
 int bem2d_farfield(bem2d_ctx* ctx, const bem2d_mesh* mesh, double k, const void* coef,
                    int64_t ldc, int nrhs, const double* theta, int64_t nang, void* out);
 
+/* ---- several basis functions per element -------------------------------------------------------
+ * ConstBasis, PolBasis, LinBasis, WaveBasis, WavePolBasis (lib/bem2d_basistypes.h:16-99) with the
+ * AddBasis helpers' lists (lib/bem2d_basistypes.cpp:12-46):
+ *   PolBasis::AddBasis(d)        {POL 0, POL 1, ..., POL d}
+ *   LinBasis::AddBasis           {LIN +1 (t), LIN -1 (1 - t)}
+ *   WaveBasis::AddBasis(k)       {WAVE dir +1, WAVE dir -1}
+ *   WavePolBasis::AddBasis(d, k) {WAVEPOL (0, +1), (0, -1), (1, +1), (1, -1), ..., (d, -1)}
+ * The same nb bases sit on every element; row / column e * nb + b is basis b of element e
+ * (Geometry::FlatMap, lib/bem2d_geometry.h:33-57, when each AddBasis call adds its basis to all
+ * elements).  nb <= BEM2D_MAX_BASES. */
+#define BEM2D_BASIS_CONST 0   /* 1 */
+#define BEM2D_BASIS_POL 1     /* t^degree */
+#define BEM2D_BASIS_LIN 2     /* t (degree = sign = +1), 1 - t (sign = -1) */
+#define BEM2D_BASIS_WAVE 3    /* exp(i k direction t), k = k_re + i k_im */
+#define BEM2D_BASIS_WAVEPOL 4 /* exp(i k direction t) t^degree */
+#define BEM2D_MAX_BASES 8
+typedef struct {
+    int kind;         /* BEM2D_BASIS_* */
+    int degree;       /* POL / WAVEPOL degree (0..64); LIN: the sign, -1 for 1 - t */
+    double direction; /* WAVE / WAVEPOL */
+    double k_re, k_im; /* WAVE / WAVEPOL wavenumber (freqtype) */
+} bem2d_basis;
+
+/* The (n nb) x (n nb) Galerkin matrix mass_scale * M + kernel_scale * K with the bases: entries
+ * sum_q conj(phi_a(x_q)) K(x_q, y_q) phi_b(y_q) |Gamma_i'| |Gamma_j'| w_q over the same rules as
+ * bem2d_assemble (Integrate / IntegrateIdent, lib/bem2d_quadrature.h:37-109, DiscreteKernel
+ * :231-294) and the mass matrix sum_t |Gamma'| conj(phi_a) phi_b w_t (EvalIdent,
+ * lib/bem2d_quadrature.cpp:79-127).  k = k_re + i k_im (k_im != 0: the complex-k kernels, Re k > 0,
+ * n <= 8191).  lda >= n nb. */
+int bem2d_assemble_basis(bem2d_ctx* ctx, const bem2d_mesh* mesh, const bem2d_basis* bases, int nb,
+                         int kernel_kind, double k_re, double k_im, double eta,
+                         double mass_scale, double kernel_scale, void* A, int64_t lda);
+/* B[(i nb + b) + ldb r] = scale * <f_r, phi_b> on element i (DotProdBasFuns with bases,
+ * lib/bem2d_quadrature.h:111-128); dirs as bem2d_rhs; complex k as bem2d_rhs_complex. */
+int bem2d_rhs_basis(bem2d_ctx* ctx, const bem2d_mesh* mesh, const bem2d_basis* bases, int nb,
+                    int field_kind, double k_re, double k_im, double eta, const double* dirs,
+                    int nrhs, double scale, void* B, int64_t ldb);
+/* EvalKernel<SingleLayer> with bases (lib/bem2d_quadrature.h:184-228): out[p] =
+ * sum_j int_{elem j} (i/4) H0(k |P_p - y|) sum_b coef[j nb + b] phi_b ds(y). */
+int bem2d_eval_single_layer_basis(bem2d_ctx* ctx, const bem2d_mesh* mesh, const bem2d_basis* bases,
+                                  int nb, double k_re, double k_im, const void* coef,
+                                  const double* pts, int64_t npts, void* out);
+/* bem2d_farfield with bases: the density on element j is sum_b coef[j nb + b + ldc r] phi_b. */
+int bem2d_farfield_basis(bem2d_ctx* ctx, const bem2d_mesh* mesh, const bem2d_basis* bases, int nb,
+                         double k, const void* coef, int64_t ldc, int nrhs, const double* theta,
+                         int64_t nang, void* out);
+/* Host evaluation of the bases: out[2 (q nb + b) + {0, 1}] = phi_b(t[q]) (Basis::operator()). */
+int bem2d_basis_values(const bem2d_basis* bases, int nb, const double* t, int64_t count,
+                       double* out);
+
+/* ---- arc-length parameterisation (host) --------------------------------------------------------
+ * Curve::Length(t0, t1) (lib/bem2d_curve.cpp:11-25) and the parameters t_i of arc lengths s_i
+ * (AnalyticCurve::ParameterizeArc, lib/bem2d_shape.cpp:332-366) of curve type BEM2D_ELEM_CIRCLE ..
+ * BEM2D_ELEM_INV_ELLIPSE with constructor arguments args[4].  The reference integrates |Gamma'| by
+ * GSL's qag (relative tolerance 1e-4) and dt/ds = 1/|Gamma'| by GSL's adaptive RK2 (absolute
+ * tolerance 1e-10); here both are computed to rounding level (adaptive Gauss-Legendre, Newton's
+ * method), so parameters agree with the reference's to its ODE tolerance.  s must be
+ * non-decreasing in [0, Length]; s_i >= Length gives t_i = 1. */
+int bem2d_curve_length(int curve_type, const double* args, double t0, double t1, double* out);
+int bem2d_arclength_params(int curve_type, const double* args, const double* s, int64_t count,
+                           double* t);
+
 /* ---- serving engine: continuous batching of scattering solves ----------------------------------
  * Not in the reference, whose only driver is one SoundSoftScattering at a time
  * (lib/bem2d_soundsoftscattering.h:74-126).  A request is one such problem -- DiscretizeMatrix

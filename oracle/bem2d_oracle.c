@@ -925,3 +925,348 @@ void oracle_elem_eval(int64_t n, const int32_t* type, const double* param, int64
     pt nn = pnormal(D);
     out6[0] = M.x; out6[1] = M.y; out6[2] = D.x; out6[3] = D.y; out6[4] = nn.x; out6[5] = nn.y;
 }
+
+/* ------------------------------------------------------------------------------------------ */
+/* Several basis functions per element (lib/bem2d_basistypes.h:16-99, lib/bem2d_geometry.h:48-57) */
+/* Basis kinds: 0 ConstBasis (1), 1 PolBasis(deg) (t^deg), 2 LinBasis(sign) (t for sign 1, 1-t    */
+/* for sign -1; the sign travels in deg), 3 WaveBasis(dir, k) (exp(i k dir t)), 4                 */
+/* WavePolBasis(deg, dir, k) (exp(i k dir t) t^deg), at the element-local parameter t.  Every      */
+/* element carries the same nb bases, so FlatMap's order is row = element * nb + basis (the       */
+/* order AddBasis gives when each basis is added to all elements, lib/bem2d_geometry.h:33-39).     */
+/* ------------------------------------------------------------------------------------------ */
+enum { B_CONST = 0, B_POL = 1, B_LIN = 2, B_WAVE = 3, B_WAVEPOL = 4 };
+typedef struct {
+    int kind, deg;
+    double dir, kr, ki;
+} basis_t;
+
+static double complex basis_eval(const basis_t* b, double t)
+{
+    double r = 1;
+    switch (b->kind) {
+    case B_CONST:
+        return 1.0;
+    case B_POL:
+        for (int i = 0; i < b->deg; i++) r = r * t;
+        return r;
+    case B_LIN:
+        return b->deg == -1 ? 1 - t : t;
+    case B_WAVE:
+        return cexp(I * (b->kr + I * b->ki) * b->dir * t);
+    case B_WAVEPOL:
+        for (int i = 0; i < b->deg; i++) r = r * t;
+        return cexp(I * (b->kr + I * b->ki) * b->dir * t) * r;
+    }
+    return NAN;
+}
+
+static void bases_make(basis_t* out, int nb, const int32_t* kind, const int32_t* deg,
+                       const double* dir, const double* kk)
+{
+    for (int b = 0; b < nb; b++) {
+        out[b].kind = kind[b];
+        out[b].deg = deg[b];
+        out[b].dir = dir[b];
+        out[b].kr = kk[2 * b];
+        out[b].ki = kk[2 * b + 1];
+    }
+}
+
+/* Integrate<kernel,T> with basis functions (lib/bem2d_quadrature.h:37-65). */
+static double complex integrate_pair_b(const geom_t* g, int64_t i, int64_t j, const basis_t* fa,
+                                       const basis_t* fb, int kind, double k, double ki,
+                                       double eta, const double* rx, const double* ry,
+                                       const double* rw, int m)
+{
+    double complex res = 0;
+    for (int q = 0; q < m; q++) {
+        pt Mx, Dx, My, Dy;
+        elem_eval(g, i, rx[q], &Mx, &Dx);
+        elem_eval(g, j, ry[q], &My, &Dy);
+        double complex gv = kernel_eval(kind, k, ki, eta, Mx, My, pnormal(Dx), pnormal(Dy));
+        double complex f1 = conj(basis_eval(fa, rx[q]));
+        double complex f2 = basis_eval(fb, ry[q]);
+        double s1 = plen(Dx), s2 = plen(Dy);
+        res += gv * f1 * f2 * s1 * s2 * rw[q];
+    }
+    return res;
+}
+
+/* IntegrateIdent with basis functions (lib/bem2d_quadrature.h:67-109). */
+static double complex integrate_ident_b(const geom_t* g, int64_t i, int64_t j, const basis_t* fa,
+                                        const basis_t* fb, int kind, double k, double ki,
+                                        double eta, const double* rx, const double* ry,
+                                        const double* rw, int m)
+{
+    double complex res = 0;
+    for (int q = 0; q < m; q++) {
+        double xq = rx[q], yq = ry[q];
+        pt Mx, Dx, My, Dy;
+        double s2p = (1 - yq) * xq;
+        elem_eval(g, i, xq, &Mx, &Dx);
+        elem_eval(g, j, s2p, &My, &Dy);
+        double complex gv = kernel_eval(kind, k, ki, eta, Mx, My, pnormal(Dx), pnormal(Dy));
+        double complex f1 = conj(basis_eval(fa, xq)), f2 = basis_eval(fb, s2p);
+        res += xq * gv * f1 * f2 * plen(Dx) * plen(Dy) * rw[q];
+
+        double s1b = 1 - xq, s2b = 1 - xq + yq * xq;
+        elem_eval(g, i, s1b, &Mx, &Dx);
+        elem_eval(g, j, s2b, &My, &Dy);
+        gv = kernel_eval(kind, k, ki, eta, Mx, My, pnormal(Dx), pnormal(Dy));
+        f1 = conj(basis_eval(fa, s1b));
+        f2 = basis_eval(fb, s2b);
+        res += xq * gv * f1 * f2 * plen(Dx) * plen(Dy) * rw[q];
+    }
+    return res;
+}
+
+/* Entry (row, col) of mass_scale*M + kernel_scale*K with nb bases per element: the rule selection
+ * of lib/bem2d_quadrature.h:271-287 by the two elements, EvalIdent's entry
+ * sum_t |Gamma_j'(x_t)| conj(f_a(x_t)) f_b(x_t) w_t for elements with the same index
+ * (lib/bem2d_quadrature.cpp:108-123). */
+static double complex entry_b(const geom_t* g, const rules_t* r, const basis_t* bs, int nb,
+                              int64_t row, int64_t col, int kind, double k, double ki, double eta,
+                              double mass_scale, double kernel_scale)
+{
+    const int64_t i = row / nb, j = col / nb;
+    const basis_t* fa = bs + row % nb;
+    const basis_t* fb = bs + col % nb;
+    double complex kv = 0, mv = 0;
+    int ident = g->index[i] == g->index[j];
+    if (kind != K_NONE && kernel_scale != 0.0) {
+        if (ident)
+            kv = integrate_ident_b(g, i, j, fa, fb, kind, k, ki, eta, r->x1, r->y1, r->w1, r->m1);
+        else if (g->next[i] == g->index[j])
+            kv = integrate_pair_b(g, i, j, fa, fb, kind, k, ki, eta, r->x3, r->y3, r->w3, r->m3);
+        else if (g->prev[i] == g->index[j])
+            kv = integrate_pair_b(g, i, j, fa, fb, kind, k, ki, eta, r->x2, r->y2, r->w2, r->m2);
+        else
+            kv = integrate_pair_b(g, i, j, fa, fb, kind, k, ki, eta, r->x0, r->y0, r->w0, r->m0);
+    }
+    if (ident && mass_scale != 0.0) {
+        for (int t = 0; t < r->N; t++) {
+            pt M, D;
+            elem_eval(g, j, r->gx[t], &M, &D);
+            double s = plen(D);
+            mv += s * conj(basis_eval(fa, r->gx[t])) * basis_eval(fb, r->gx[t]) * r->gw[t];
+        }
+    }
+    return kernel_scale * kv + mass_scale * mv;
+}
+
+/* The (n nb) x (n nb) matrix (columns cols[0..ncols) of it, all when cols == NULL), column c of
+ * out = matrix column cols[c], leading dimension ld; real k. */
+int oracle_assemble_basis(int64_t n, const int32_t* type, const double* param,
+                          const int64_t* index, const int64_t* next, const int64_t* prev, int nb,
+                          const int32_t* bkind, const int32_t* bdeg, const double* bdir,
+                          const double* bk, int N, int L, double sigma, int kind, double k,
+                          double ki, double eta, double mass_scale, double kernel_scale,
+                          const int64_t* cols, int64_t ncols, double* out, int64_t ld)
+{
+    if (nb < 1 || nb > 64) return 2;
+    geom_t g = {type, param, index, next, prev, n};
+    rules_t r;
+    if (rules_make(&r, N, L, sigma)) return 1;
+    basis_t bs[64];
+    bases_make(bs, nb, bkind, bdeg, bdir, bk);
+    const int64_t nn = n * nb;
+    if (!cols) ncols = nn;
+    double complex* o = (double complex*)out;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t c = 0; c < ncols; c++) {
+        int64_t col = cols ? cols[c] : c;
+        for (int64_t row = 0; row < nn; row++)
+            o[row + ld * c] = entry_b(&g, &r, bs, nb, row, col, kind, k, ki, eta, mass_scale,
+                                      kernel_scale);
+    }
+    rules_free(&r);
+    return 0;
+}
+
+/* DotProdBasFuns with basis functions (lib/bem2d_quadrature.h:111-128): out[(i nb + a) + ld r]. */
+int oracle_rhs_basis(int64_t n, const int32_t* type, const double* param, int nb,
+                     const int32_t* bkind, const int32_t* bdeg, const double* bdir,
+                     const double* bk, int N, int kind, double k, double ki, double eta,
+                     const double* dirs, int nrhs, double scale, double* out, int64_t ld)
+{
+    if (nb < 1 || nb > 64) return 2;
+    geom_t g = {type, param, NULL, NULL, NULL, n};
+    basis_t bs[64];
+    bases_make(bs, nb, bkind, bdeg, bdir, bk);
+    double* gx = malloc(8 * N);
+    double* gw = malloc(8 * N);
+    oracle_gauss(N, gx, gw);
+    double complex* o = (double complex*)out;
+#pragma omp parallel for
+    for (int64_t i = 0; i < n; i++)
+        for (int a = 0; a < nb; a++)
+            for (int r = 0; r < nrhs; r++) {
+                pt dir = {dirs[2 * r], dirs[2 * r + 1]};
+                double complex res = 0;
+                for (int t = 0; t < N; t++) {
+                    pt M, D;
+                    elem_eval(&g, i, gx[t], &M, &D);
+                    double s = plen(D);
+                    double complex tf = conj(basis_eval(bs + a, gx[t]));
+                    res += field_eval(kind, k, ki, eta, dir, M, pnormal(D)) * s * tf * gw[t];
+                }
+                o[(i * nb + a) + ld * r] = scale * res;
+            }
+    free(gx); free(gw);
+    return 0;
+}
+
+/* EvalKernel<SingleLayer> with basis functions (lib/bem2d_quadrature.h:184-228). */
+int oracle_eval_sl_basis(int64_t n, const int32_t* type, const double* param, int nb,
+                         const int32_t* bkind, const int32_t* bdeg, const double* bdir,
+                         const double* bk, int N, double k, double ki, const double* coef,
+                         const double* pts, int64_t npts, double* out)
+{
+    if (nb < 1 || nb > 64) return 2;
+    geom_t g = {type, param, NULL, NULL, NULL, n};
+    basis_t bs[64];
+    bases_make(bs, nb, bkind, bdeg, bdir, bk);
+    double* gx = malloc(8 * N);
+    double* gw = malloc(8 * N);
+    oracle_gauss(N, gx, gw);
+    const double complex* c = (const double complex*)coef;
+    double complex* o = (double complex*)out;
+#pragma omp parallel for
+    for (int64_t p = 0; p < npts; p++) {
+        pt P = {pts[2 * p], pts[2 * p + 1]};
+        double complex v = 0;
+        for (int64_t j = 0; j < n; j++)
+            for (int b = 0; b < nb; b++)
+                for (int t = 0; t < N; t++) {
+                    pt M, D;
+                    elem_eval(&g, j, gx[t], &M, &D);
+                    double s = plen(D);
+                    double complex gv = kernel_eval(K_SL, k, ki, 0, P, M, pnormal(D), pnormal(D));
+                    v += c[j * nb + b] * gv * s * basis_eval(bs + b, gx[t]) * gw[t];
+                }
+        o[p] = v;
+    }
+    free(gx); free(gw);
+    return 0;
+}
+
+/* Far field of u = u_inc - S phi with basis functions (as oracle_farfield). */
+int oracle_farfield_basis(int64_t n, const int32_t* type, const double* param, int nb,
+                          const int32_t* bkind, const int32_t* bdeg, const double* bdir,
+                          const double* bk, int N, double k, const double* coef, int nrhs,
+                          const double* theta, int64_t nang, double* out)
+{
+    if (nb < 1 || nb > 64) return 2;
+    geom_t g = {type, param, NULL, NULL, NULL, n};
+    basis_t bs[64];
+    bases_make(bs, nb, bkind, bdeg, bdir, bk);
+    double* gx = malloc(8 * N);
+    double* gw = malloc(8 * N);
+    oracle_gauss(N, gx, gw);
+    const double complex* c = (const double complex*)coef;
+    double complex* o = (double complex*)out;
+    double complex pref = -cexp(I * O_PI / 4.0) / sqrt(8.0 * O_PI * k);
+    const int64_t nn = n * nb;
+#pragma omp parallel for collapse(2)
+    for (int r = 0; r < nrhs; r++)
+        for (int64_t a = 0; a < nang; a++) {
+            double cx = cos(theta[a]), cy = sin(theta[a]);
+            double complex v = 0;
+            for (int64_t j = 0; j < n; j++)
+                for (int b = 0; b < nb; b++) {
+                    double complex sj = 0;
+                    for (int t = 0; t < N; t++) {
+                        pt M, D;
+                        elem_eval(&g, j, gx[t], &M, &D);
+                        sj += cexp(-I * k * (cx * M.x + cy * M.y)) * plen(D) *
+                              basis_eval(bs + b, gx[t]) * gw[t];
+                    }
+                    v += c[(j * nb + b) + nn * r] * sj;
+                }
+            o[a + nang * r] = pref * v;
+        }
+    free(gx); free(gw);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Arc-length parameterisation of a curve (AnalyticCurve, lib/bem2d_shape.cpp:247-368, and      */
+/* Curve::Length, lib/bem2d_curve.cpp:11-25), restated without GSL: the arc length            */
+/* s(t) = int_0^t |Gamma'(u)| du by adaptive Gauss-Legendre (20 points per panel, panels halved */
+/* until two levels agree to 1e-15 relative), and t_i with s(t_i) = i L / n by Newton's method  */
+/* on s(t) - s_i (s' = |Gamma'|), bracketed by bisection.  The reference integrates            */
+/* dt/ds = 1/|Gamma'| with GSL's adaptive RK2 to an absolute tolerance of 1e-10 and takes L by  */
+/* GSL qag to 1e-4 relative; this restatement computes the exact parameterisation instead.     */
+/* ------------------------------------------------------------------------------------------ */
+static double curve_speed(int type, const double* c, double t)
+{
+    pt M, D;
+    curve_eval(type, c, t, &M, &D);
+    return plen(D);
+}
+
+static double gl20(int type, const double* c, double a, double b)
+{
+    static double x[20], w[20];
+    static int ready = 0;
+    if (!ready) { /* benign race: every thread writes the same values */
+        double gx[20], gw[20];
+        oracle_gauss(20, gx, gw);
+        for (int i = 0; i < 20; i++) { x[i] = gx[i]; w[i] = gw[i]; }
+        ready = 1;
+    }
+    double s = 0;
+    for (int i = 0; i < 20; i++) s += w[i] * curve_speed(type, c, a + (b - a) * x[i]);
+    return s * (b - a);
+}
+
+static double arc_adapt(int type, const double* c, double a, double b, double whole, int depth)
+{
+    const double m = 0.5 * (a + b);
+    const double l = gl20(type, c, a, m), r = gl20(type, c, m, b);
+    if (depth >= 30 || fabs(l + r - whole) <= 1e-15 * fabs(l + r)) return l + r;
+    return arc_adapt(type, c, a, m, l, depth + 1) + arc_adapt(type, c, m, b, r, depth + 1);
+}
+
+static double arc_length(int type, const double* c, double a, double b)
+{
+    if (b <= a) return 0.0;
+    /* 16 panels first, so that a periodic integrand is resolved before the adaptive test */
+    double s = 0;
+    for (int p = 0; p < 16; p++) {
+        const double pa = a + (b - a) * p / 16.0, pb = a + (b - a) * (p + 1) / 16.0;
+        s += arc_adapt(type, c, pa, pb, gl20(type, c, pa, pb), 0);
+    }
+    return s;
+}
+
+/* Curve length (Curve::Length) of curve type `type` (1..6) with arguments c[4]. */
+double oracle_curve_length(int type, const double* c) { return arc_length(type, c, 0.0, 1.0); }
+
+/* The parameters t[0..n] with arc length s(t_i) = i L / n, t_0 = 0, t_n = 1 (AnalyticCurve(n)). */
+int oracle_arclength_partition(int type, const double* c, int n, double* t)
+{
+    if (n < 1) return 1;
+    const double L = arc_length(type, c, 0.0, 1.0);
+    t[0] = 0.0;
+    double tprev = 0.0, sprev = 0.0;
+    for (int i = 1; i < n; i++) {
+        const double target = i * L / n;
+        double lo = tprev, hi = 1.0, x = tprev + (target - sprev) / L;
+        if (x <= lo || x >= hi) x = 0.5 * (lo + hi);
+        for (int it = 0; it < 100; it++) {
+            const double f = sprev + arc_length(type, c, tprev, x) - target;
+            if (f > 0) hi = x; else lo = x;
+            const double step = f / curve_speed(type, c, x);
+            double xn = x - step;
+            if (!(xn > lo && xn < hi)) xn = 0.5 * (lo + hi);
+            if (fabs(xn - x) <= 1e-16 * fabs(x) + 1e-300) { x = xn; break; }
+            x = xn;
+        }
+        sprev += arc_length(type, c, tprev, x);
+        tprev = x;
+        t[i] = x;
+    }
+    t[n] = 1.0;
+    return 0;
+}

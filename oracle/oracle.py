@@ -14,6 +14,7 @@ _LIBS = {}
 
 R_GAUSS2D, R_AG1, R_AG2, R_AG3 = 0, 1, 2, 3
 K_NONE, K_SL, K_DL, K_CDL, K_COMBINED, K_ID = 0, 1, 2, 3, 4, 5
+B_CONST, B_POL, B_LIN, B_WAVE, B_WAVEPOL = 0, 1, 2, 3, 4
 F_PLANE, F_NORMAL_PLANE, F_COMBINED_PLANE = 0, 1, 2
 
 
@@ -51,6 +52,16 @@ def lib(variant="plain"):
         L.oracle_hankel_complex_many.restype = None
         L.oracle_elem_eval.argtypes = [i64, P, P, i64, d, P]
         L.oracle_elem_eval.restype = None
+        L.oracle_assemble_basis.argtypes = [i64, P, P, P, P, P, i32, P, P, P, P, i32, i32, d,
+                                            i32, d, d, d, d, d, P, i64, P, i64]
+        L.oracle_rhs_basis.argtypes = [i64, P, P, i32, P, P, P, P, i32, i32, d, d, d, P, i32, d,
+                                       P, i64]
+        L.oracle_eval_sl_basis.argtypes = [i64, P, P, i32, P, P, P, P, i32, d, d, P, P, i64, P]
+        L.oracle_farfield_basis.argtypes = [i64, P, P, i32, P, P, P, P, i32, d, P, i32, P, i64,
+                                            P]
+        L.oracle_curve_length.argtypes = [i32, P]
+        L.oracle_curve_length.restype = d
+        L.oracle_arclength_partition.argtypes = [i32, P, i32, P]
         _LIBS[variant] = L
     return _LIBS[variant]
 
@@ -186,3 +197,106 @@ def noise_floor(geom, N, L, sigma, kind, k, eta, mass_scale=0.0, kernel_scale=1.
         b = assemble(geom, N, L, sigma, kind, k, eta, mass_scale, kernel_scale, cols, variant=v)
         out = max(out, float(np.linalg.norm(a - b) / na))
     return out
+
+
+def _bases(bases):
+    """Arrays of a basis list [(kind, deg, dir, k), ...] (kind B_*; deg: the power of PolBasis and
+    WavePolBasis, the sign +1 / -1 of LinBasis; dir and k of WaveBasis / WavePolBasis)."""
+    kind = np.ascontiguousarray([b[0] for b in bases], dtype=np.int32)
+    deg = np.ascontiguousarray([b[1] for b in bases], dtype=np.int32)
+    dr = np.ascontiguousarray([b[2] for b in bases], dtype=np.float64)
+    kk = np.ascontiguousarray([[complex(b[3]).real, complex(b[3]).imag] for b in bases],
+                              dtype=np.float64).reshape(-1)
+    return kind, deg, dr, kk
+
+
+def assemble_basis(geom, bases, N, L, sigma, kind, k, eta, mass_scale=0.0, kernel_scale=1.0,
+                   cols=None, variant="plain"):
+    """(n nb) x ncols block with nb basis functions per element, row = element * nb + basis."""
+    nb = len(bases)
+    nn = geom.n * nb
+    if cols is None:
+        ncols, cptr = nn, None
+    else:
+        cols = np.ascontiguousarray(cols, dtype=np.int64)
+        ncols, cptr = cols.size, _p(cols)
+    bk, bd, bdir, bkk = _bases(bases)
+    out = np.zeros((ncols, nn), dtype=np.complex128)
+    kr, ki = _kparts(k)
+    rc = lib(variant).oracle_assemble_basis(geom.n, _p(geom.type), _p(geom.param), _p(geom.index),
+                                     _p(geom.next), _p(geom.prev), nb, _p(bk), _p(bd), _p(bdir),
+                                     _p(bkk), N, L, sigma, kind, kr, ki, eta, mass_scale,
+                                     kernel_scale, cptr, ncols, _p(out), nn)
+    if rc:
+        raise ValueError("oracle_assemble_basis: bad option")
+    return out.T
+
+
+def rhs_basis(geom, bases, N, kind, k, eta, dirs, scale=1.0):
+    nb = len(bases)
+    dirs = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1, 2)
+    nrhs = dirs.shape[0]
+    out = np.zeros((nrhs, geom.n * nb), dtype=np.complex128)
+    bk, bd, bdir, bkk = _bases(bases)
+    kr, ki = _kparts(k)
+    lib().oracle_rhs_basis(geom.n, _p(geom.type), _p(geom.param), nb, _p(bk), _p(bd), _p(bdir),
+                           _p(bkk), N, kind, kr, ki, eta, _p(dirs), nrhs, scale, _p(out),
+                           geom.n * nb)
+    return out.T
+
+
+def eval_sl_basis(geom, bases, N, k, coef, pts):
+    nb = len(bases)
+    coef = np.ascontiguousarray(coef, dtype=np.complex128)
+    pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 2)
+    out = np.zeros(pts.shape[0], dtype=np.complex128)
+    bk, bd, bdir, bkk = _bases(bases)
+    kr, ki = _kparts(k)
+    lib().oracle_eval_sl_basis(geom.n, _p(geom.type), _p(geom.param), nb, _p(bk), _p(bd),
+                               _p(bdir), _p(bkk), N, kr, ki, _p(coef), _p(pts), pts.shape[0],
+                               _p(out))
+    return out
+
+
+def farfield_basis(geom, bases, N, k, coef, theta):
+    nb = len(bases)
+    coef = np.asfortranarray(coef, dtype=np.complex128)
+    if coef.ndim == 1:
+        coef = coef.reshape(-1, 1, order="F")
+    theta = np.ascontiguousarray(theta, dtype=np.float64)
+    nrhs = coef.shape[1]
+    out = np.zeros((nrhs, theta.size), dtype=np.complex128)
+    bk, bd, bdir, bkk = _bases(bases)
+    lib().oracle_farfield_basis(geom.n, _p(geom.type), _p(geom.param), nb, _p(bk), _p(bd),
+                                _p(bdir), _p(bkk), N, float(k), _p(coef), nrhs, _p(theta),
+                                theta.size, _p(out))
+    return out.T
+
+
+def noise_floor_basis(geom, bases, N, L, sigma, kind, k, eta, mass_scale=0.0, kernel_scale=1.0,
+                      cols=None):
+    """noise_floor for the matrix with bases."""
+    a = assemble_basis(geom, bases, N, L, sigma, kind, k, eta, mass_scale, kernel_scale, cols)
+    na = np.linalg.norm(a)
+    if na == 0:
+        return 0.0
+    out = 0.0
+    for v in ("fma", "crtrig"):
+        b = assemble_basis(geom, bases, N, L, sigma, kind, k, eta, mass_scale, kernel_scale, cols,
+                           variant=v)
+        out = max(out, float(np.linalg.norm(a - b) / na))
+    return out
+
+
+def curve_length(ctype, c):
+    c = np.ascontiguousarray(c, dtype=np.float64)
+    return lib().oracle_curve_length(ctype, _p(c))
+
+
+def arclength_partition(ctype, c, n):
+    """Parameters t_0 = 0 < ... < t_n = 1 splitting the curve into n pieces of equal arc length."""
+    c = np.ascontiguousarray(c, dtype=np.float64)
+    t = np.zeros(n + 1)
+    if lib().oracle_arclength_partition(ctype, _p(c), n, _p(t)):
+        raise ValueError("arclength_partition: n < 1")
+    return t

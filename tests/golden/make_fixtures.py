@@ -2,7 +2,7 @@
 
     python tests/golden/make_fixtures.py
 
-Nothing here reads or runs the reference.  Three fixture files are produced:
+Nothing here reads or runs the reference.  The fixture files produced:
 
 hankel_mpmath.npz   J0, Y0, J1, Y1 at 50 significant digits, rounded to float64, at log-spaced
                     x in [1e-8, 5e4] plus dense clusters around the algorithm switch points of
@@ -18,6 +18,11 @@ hankel_complex_mpmath.npz
                     precision is raised by that much.
 entries_complex_mpmath.npz
                     as entries_mpmath.npz for complex wavenumbers (the kernels' k.im != 0 branch).
+entries_basis_mpmath.npz
+                    as entries_mpmath.npz with several basis functions per element (PolBasis,
+                    LinBasis, WaveBasis, WavePolBasis; row = element * nb + basis): the entries
+                    carry conj(phi_a(test parameter)) phi_b(trial parameter), the mass matrix
+                    conj(phi_a) phi_b, evaluated in mpmath at 30 digits.
 entries_mpmath.npz  Galerkin matrices of tiny meshes evaluated entirely in mpmath (30 digits):
                     the reference's quadrature rules are rebuilt from the 40-digit Gauss nodes,
                     geometry, kernels and Hankel functions are evaluated in mpmath.  These pin the
@@ -294,6 +299,114 @@ def make_entries_complex():
     np.savez_compressed(os.path.join(HERE, "entries_complex_mpmath.npz"), **out)
 
 
+def basis_mp(b, t):
+    """phi_b(t) of a basis tuple (kind, degree, direction, k): 0 Const, 1 Pol, 2 Lin, 3 Wave,
+    4 WavePol (lib/bem2d_basistypes.h:19-94)."""
+    kind, deg, dr, k = b
+    if kind == 0:
+        return mp.mpc(1)
+    if kind == 2:
+        return mp.mpc(1 - t) if deg == -1 else mp.mpc(t)
+    r = t ** deg if kind in (1, 4) else mp.mpf(1)
+    if kind == 1:
+        return mp.mpc(r)
+    kk = mp.mpc(complex(k).real, complex(k).imag)
+    return mp.exp(mp.mpc(0, 1) * kk * dr * t) * r
+
+
+def matrix_mp_basis(types, params, index, nxt, prv, N, L, sigma, kind, k, eta, bases):
+    """(n nb) x (n nb) Galerkin matrix with the bases (kind 0: the mass matrix)."""
+    n, nb = len(types), len(bases)
+    A = np.zeros((n * nb, n * nb), dtype=np.complex128)
+    if kind == 0:
+        xt, wt = gauss_mp(N)
+        for e in range(n):
+            acc = [[mp.mpc(0)] * nb for _ in range(nb)]
+            for t in range(N):
+                _, D = elem(types[e], params[e], xt[t])
+                s = mp.sqrt(D[0] ** 2 + D[1] ** 2) * wt[t]
+                for a in range(nb):
+                    fa = mp.conj(basis_mp(bases[a], xt[t]))
+                    for b in range(nb):
+                        acc[a][b] += s * fa * basis_mp(bases[b], xt[t])
+            for a in range(nb):
+                for b in range(nb):
+                    A[e * nb + a, e * nb + b] = complex(acc[a][b])
+        return A
+    k = mp.mpc(k.real, k.imag) if isinstance(k, complex) else mp.mpf(k)
+    eta = mp.mpf(eta)
+    rules = [rule_mp(w, N, L, sigma) for w in range(4)]
+    for i in range(n):
+        for j in range(n):
+            acc = [[mp.mpc(0)] * nb for _ in range(nb)]
+            if index[i] == index[j]:
+                X, Y, W = rules[1]
+                pts = []
+                for q in range(len(X)):
+                    for (s, t) in ((X[q], (1 - Y[q]) * X[q]), (1 - X[q], 1 - X[q] + Y[q] * X[q])):
+                        pts.append((s, t, X[q] * W[q]))
+            else:
+                r = 0
+                if nxt[i] == index[j]:
+                    r = 3
+                elif prv[i] == index[j]:
+                    r = 2
+                X, Y, W = rules[r]
+                pts = [(X[q], Y[q], W[q]) for q in range(len(X))]
+            for (s, t, w) in pts:
+                Mx, Dx = elem(types[i], params[i], s)
+                My, Dy = elem(types[j], params[j], t)
+                n1, s1 = normal(Dx)
+                n2, s2 = normal(Dy)
+                g = kernel(kind, k, eta, Mx, My, n1, n2) * s1 * s2 * w
+                fa = [mp.conj(basis_mp(bb, s)) for bb in bases]
+                fb = [basis_mp(bb, t) for bb in bases]
+                for a in range(nb):
+                    for b in range(nb):
+                        acc[a][b] += g * fa[a] * fb[b]
+            for a in range(nb):
+                for b in range(nb):
+                    A[i * nb + a, j * nb + b] = complex(acc[a][b])
+    return A
+
+
+LIN = [(2, 1, 0.0, 0.0), (2, -1, 0.0, 0.0)]
+POL2 = [(1, 0, 0.0, 0.0), (1, 1, 0.0, 0.0), (1, 2, 0.0, 0.0)]
+
+CASES_BASIS = [
+    # name, mesh builder, (N, L, sigma), kinds, k, eta, bases
+    ("circle4_lin", lambda: curve_mesh(1, 4), (4, 2, 0.15), (0, 1, 3, 4), 3.0, 3.0, LIN),
+    ("triangle6_pol2", lambda: polygon_mesh([(0, 0), (1.2, 0.1), (0.3, 0.9)], 2), (4, 2, 0.15),
+     (0, 2), 4.0, 4.0, POL2),
+    ("arc4open_wavepol1", lambda: curve_mesh(5, 4, (1.3, 0.7, 0.3, 5.5), closed=False),
+     (4, 2, 0.15), (0, 4), 2.5, 1.5,
+     [(4, 0, 1.0, 1.7), (4, 0, -1.0, 1.7), (4, 1, 1.0, 1.7), (4, 1, -1.0, 1.7)]),
+    ("circle3_wave_ck", lambda: curve_mesh(1, 3), (4, 2, 0.15), (1, 4), 6.0 - 1.5j, 6.0,
+     [(3, 0, 1.0, 2.0 - 0.5j), (3, 0, -1.0, 2.0 - 0.5j)]),
+]
+
+
+def make_entries_basis():
+    out = {}
+    mp.mp.dps = 30
+    for name, build, (N, L, sigma), kinds, k, eta, bases in CASES_BASIS:
+        types, params, idx, nx, pv = build()
+        out[name + "_type"] = types
+        out[name + "_param"] = params
+        out[name + "_index"] = idx
+        out[name + "_next"] = nx
+        out[name + "_prev"] = pv
+        out[name + "_quad"] = np.array([N, L, sigma])
+        out[name + "_kc"] = np.array([complex(k).real, complex(k).imag, eta])
+        out[name + "_bases"] = np.array([[b[0], b[1], b[2], complex(b[3]).real,
+                                          complex(b[3]).imag] for b in bases])
+        for kind in kinds:
+            out["%s_kind%d" % (name, kind)] = matrix_mp_basis(types, params, idx, nx, pv, N, L,
+                                                              sigma, kind, k, eta, bases)
+            print(name, kind, file=sys.stderr, flush=True)
+    np.savez_compressed(os.path.join(HERE, "entries_basis_mpmath.npz"), **out)
+
+
 def make_hankel_complex():
     zs = []
     for ph in np.linspace(-0.95 * np.pi / 2, 0.95 * np.pi / 2, 39):
@@ -342,7 +455,8 @@ def make_gauss():
 
 
 if __name__ == "__main__":
-    what = sys.argv[1:] or ["gauss", "hankel", "entries", "hankel_complex", "entries_complex"]
+    what = sys.argv[1:] or ["gauss", "hankel", "entries", "hankel_complex", "entries_complex",
+                            "entries_basis"]
     if "gauss" in what:
         make_gauss()
     if "hankel" in what:
@@ -353,3 +467,5 @@ if __name__ == "__main__":
         make_hankel_complex()
     if "entries_complex" in what:
         make_entries_complex()
+    if "entries_basis" in what:
+        make_entries_basis()
