@@ -1,5 +1,5 @@
 // Device kernels of the FP64 hot path: Gauss-node tables, regular-pair and singular-pair Galerkin
-// assembly, plane-wave right-hand sides, single-layer potential and far field.
+// assembly, plane-wave right-hand sides and far field.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -660,35 +660,6 @@ static __global__ void rhs_kernel(int64_t n, int nq, const double* __restrict__ 
         ai = fma(fi, sw, ai);
     }
     B[i + ldb * r] = make_double2(scale * ar, scale * ai);
-}
-
-// EvalKernel<SingleLayer> at points (lib/bem2d_quadrature.h:184-228): thread per point.
-static __global__ void eval_sl_kernel(int64_t n, int nq, const double* __restrict__ nodes, double k,
-                               const double2* __restrict__ coef, const double* __restrict__ pts,
-                               int64_t npts, double2* __restrict__ out)
-{
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= npts) return;
-    const double px = pts[2 * p], py = pts[2 * p + 1];
-    const int64_t plane = n * nq;
-    double vr = 0.0, vi = 0.0;
-    for (int64_t j = 0; j < n; ++j) {
-        double sr = 0.0, si = 0.0;
-        for (int a = 0; a < nq; ++a) {
-            const int64_t o = (int64_t)a * n + j;
-            double kr, ki;
-            kernel_eval<BEM2D_KERNEL_SL>(k, 1.0 / k, 0.25 * k, 0.0, px, py,
-                                         nodes[NODE_X * plane + o], nodes[NODE_Y * plane + o], 0,
-                                         0, 0, 0, kr, ki);
-            const double sw = nodes[NODE_SW * plane + o];
-            sr = fma(kr, sw, sr);
-            si = fma(ki, sw, si);
-        }
-        const double2 c = coef[j];
-        vr += c.x * sr - c.y * si;
-        vi += c.x * si + c.y * sr;
-    }
-    out[p] = make_double2(vr, vi);
 }
 
 // Far-field signature F[a + nang*j] = sum_t exp(-ik xhat_a.y_t) |Gamma_j'| w_t: thread per (a, j).

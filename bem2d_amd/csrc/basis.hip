@@ -24,59 +24,17 @@
 #pragma clang diagnostic ignored "-Wunused-function" // kernels of the shared headers
 #include "assembly_kernels.h"
 #include "complex_kernels.h"
+#include "basis.h"
 #include "internal.h"
 
 using namespace bem2d_dev;
 
 namespace {
 
+using bem2d_dev::BasisSet;
+using bem2d_dev::basis_value;
 constexpr int kMaxBases = BEM2D_MAX_BASES;
-constexpr int kGroup = 4;
-
-// The bases of a launch, by value in the kernel arguments; entries past nb (padding of the last
-// group) are zero functions and their entries are not written.
-struct BasisSet {
-    int nb;     // bases per element
-    int G;      // bases per group
-    int ngroup; // groups (G * ngroup >= nb)
-    int kind[kMaxBases + kGroup];
-    int deg[kMaxBases + kGroup];
-    double dir[kMaxBases + kGroup], kr[kMaxBases + kGroup], ki[kMaxBases + kGroup];
-};
-
-// phi_b(t), lib/bem2d_basistypes.h:19-94: 1, t^deg, t (sign 1) / 1 - t (sign -1),
-// exp(i k dir t), exp(i k dir t) t^deg; zero past nb.
-__host__ __device__ inline void basis_value(const BasisSet& s, int b, double t, double& vr,
-                                            double& vi)
-{
-    vi = 0.0;
-    if (b >= s.nb) {
-        vr = 0.0;
-        return;
-    }
-    const int kind = s.kind[b];
-    if (kind == BEM2D_BASIS_CONST) {
-        vr = 1.0;
-        return;
-    }
-    if (kind == BEM2D_BASIS_LIN) {
-        vr = s.deg[b] == -1 ? 1 - t : t;
-        return;
-    }
-    double r = 1.0;
-    for (int i = 0; i < s.deg[b]; ++i) r = r * t;
-    if (kind == BEM2D_BASIS_POL) {
-        vr = r;
-        return;
-    }
-    // i k dir t = (-ki dir t) + i (kr dir t)
-    const double er = -s.ki[b] * s.dir[b] * t, ei = s.kr[b] * s.dir[b] * t;
-    double sn, cs;
-    sincos(ei, &sn, &cs);
-    const double m = exp(er);
-    vr = m * cs * r;
-    vi = m * sn * r;
-}
+constexpr int kGroup = bem2d_dev::kBasisGroup;
 
 // Basis table at the Gauss nodes: bt[2 (a * nbp + b) + {0, 1}], nbp = G * ngroup columns.
 __device__ __forceinline__ void stage_basis_table(const BasisSet& s, int nq,
@@ -500,54 +458,6 @@ __global__ void rhs_basis_kernel(int64_t n, int nq, const double* __restrict__ n
     B[u + ldb * r] = make_double2(scale * ar, scale * ai);
 }
 
-// out[p] = sum_j sum_a (i/4) H0(k |P - y_a|) |Gamma_j'(t_a)| w_a sum_b coef[j nb + b] phi_b(t_a):
-// thread per point; real k (CK = false) or complex k.
-template <bool CK>
-__global__ void eval_sl_basis_kernel(int64_t n, int nq, const double* __restrict__ nodes,
-                                     const double* __restrict__ gx, const BasisSet bs, double kr,
-                                     double ki, const double* __restrict__ tab, int nt,
-                                     const double2* __restrict__ coef,
-                                     const double* __restrict__ pts, int64_t npts,
-                                     double2* __restrict__ out)
-{
-    extern __shared__ double bt[];
-    const int nb = bs.nb, nbp = bs.G * bs.ngroup;
-    stage_basis_table(bs, nq, gx, bt, threadIdx.x, blockDim.x);
-    __syncthreads();
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= npts) return;
-    const double px = pts[2 * p], py = pts[2 * p + 1];
-    const int64_t plane = n * nq;
-    double vr = 0.0, vi = 0.0;
-    for (int64_t j = 0; j < n; ++j) {
-        for (int a = 0; a < nq; ++a) {
-            const int64_t o = (int64_t)a * n + j;
-            double kvr, kvi;
-            if (CK)
-                kernel_eval_ck<BEM2D_KERNEL_SL>(kr, ki, 0.0, tab, nt, px, py,
-                                                nodes[NODE_X * plane + o],
-                                                nodes[NODE_Y * plane + o], 0, 0, 0, 0, kvr, kvi);
-            else
-                kernel_eval<BEM2D_KERNEL_SL>(kr, 1.0 / kr, 0.25 * kr, 0.0, px, py,
-                                             nodes[NODE_X * plane + o], nodes[NODE_Y * plane + o],
-                                             0, 0, 0, 0, kvr, kvi);
-            const double sw = nodes[NODE_SW * plane + o];
-            double dr = 0.0, di = 0.0; // density sum_b c_b phi_b(t_a)
-            for (int b = 0; b < nb; ++b) {
-                const double2 c = coef[j * nb + b];
-                const double fr = bt[2 * (a * nbp + b)], fi = bt[2 * (a * nbp + b) + 1];
-                dr += c.x * fr - c.y * fi;
-                di += c.x * fi + c.y * fr;
-            }
-            kvr *= sw;
-            kvi *= sw;
-            vr += kvr * dr - kvi * di;
-            vi += kvr * di + kvi * dr;
-        }
-    }
-    out[p] = make_double2(vr, vi);
-}
-
 // Far-field signature with bases: F[a + nang (j nb + b)] = sum_t exp(-i k xhat_a.y_t)
 // |Gamma_j'(t)| w_t phi_b(t): thread per (a, j).
 __global__ void farfield_signature_basis_kernel(int64_t n, int nq, const double* __restrict__ nodes,
@@ -587,36 +497,6 @@ __global__ void farfield_signature_basis_kernel(int64_t n, int nq, const double*
 }
 
 // ---- host side -----------------------------------------------------------------------------------
-int make_basis_set(const bem2d_basis* bases, int nb, BasisSet* out)
-{
-    if (!bases || nb < 1 || nb > kMaxBases)
-        return FAIL(BEM2D_ERR_ARG, "basis list: need 1 <= nb <= %d (got %d)", kMaxBases, nb);
-    BasisSet s{};
-    s.nb = nb;
-    s.ngroup = (nb + kGroup - 1) / kGroup;
-    s.G = (nb + s.ngroup - 1) / s.ngroup;
-    for (int b = 0; b < nb; ++b) {
-        const bem2d_basis& x = bases[b];
-        if (x.kind < BEM2D_BASIS_CONST || x.kind > BEM2D_BASIS_WAVEPOL)
-            return FAIL(BEM2D_ERR_ARG, "basis %d: unknown kind %d", b, x.kind);
-        if ((x.kind == BEM2D_BASIS_POL || x.kind == BEM2D_BASIS_WAVEPOL) &&
-            (x.degree < 0 || x.degree > 64))
-            return FAIL(BEM2D_ERR_ARG, "basis %d: degree %d outside [0, 64]", b, x.degree);
-        if ((x.kind == BEM2D_BASIS_WAVE || x.kind == BEM2D_BASIS_WAVEPOL) &&
-            !(std::isfinite(x.direction) && std::isfinite(x.k_re) && std::isfinite(x.k_im)))
-            return FAIL(BEM2D_ERR_ARG, "basis %d: direction or wavenumber not finite", b);
-        s.kind[b] = x.kind;
-        s.deg[b] = (x.kind == BEM2D_BASIS_POL || x.kind == BEM2D_BASIS_WAVEPOL || x.kind == BEM2D_BASIS_LIN)
-                       ? x.degree
-                       : 0;
-        s.dir[b] = x.direction;
-        s.kr[b] = x.k_re;
-        s.ki[b] = x.k_im;
-    }
-    *out = s;
-    return BEM2D_OK;
-}
-
 AsmProblem basis_problem(const bem2d_mesh* m, double k, double eta, double mscale, double kscale,
                          void* A, int64_t lda)
 {
@@ -654,12 +534,21 @@ int launch_basis(bem2d_ctx* c, const AsmProblem& p, const BasisSet& s, double ki
 {
     const unsigned gp = (unsigned)(s.ngroup * s.ngroup);
     hipStream_t st = c->stream;
-    if (p.nspecial > 0) {
+    // the singular entries run on the second stream beside the regular pairs (disjoint entries),
+    // forked from and joined back into the caller's stream
+    const bool fork = p.nspecial > 0;
+    if (fork) {
+        if (hipEventRecord(c->ev_fork, st) != hipSuccess ||
+            hipStreamWaitEvent(c->aux, c->ev_fork, 0) != hipSuccess)
+            return FAIL(BEM2D_ERR_DEVICE, "stream fork failed");
+        bem2d_internal_prof_begin(c, BEM2D_PROF_ASSEMBLE_SPECIAL, c->aux);
         if (ki == 0.0)
-            assemble_special_basis_kernel<KIND, G><<<dim3((unsigned)p.nspecial, gp), 256, 0, st>>>(p, s);
+            assemble_special_basis_kernel<KIND, G>
+                <<<dim3((unsigned)p.nspecial, gp), 256, 0, c->aux>>>(p, s);
         else
             assemble_special_basis_ck_kernel<KIND, G>
-                <<<dim3((unsigned)p.nspecial, gp), 256, 0, st>>>(p, s, ki, tab, nt);
+                <<<dim3((unsigned)p.nspecial, gp), 256, 0, c->aux>>>(p, s, ki, tab, nt);
+        bem2d_internal_prof_end(c, BEM2D_PROF_ASSEMBLE_SPECIAL, c->aux);
     }
     if (ki == 0.0) {
         const size_t shm = sizeof(double) * kTileSide * node_stride(p.nq) + btab;
@@ -671,6 +560,9 @@ int launch_basis(bem2d_ctx* c, const AsmProblem& p, const BasisSet& s, double ki
         assemble_regular_basis_ck_kernel<KIND, G>
             <<<dim3((unsigned)(p.n * p.n), gp), 64, btab, st>>>(p, s, ki, tab, nt);
     }
+    if (fork && (hipEventRecord(c->ev_join, c->aux) != hipSuccess ||
+                 hipStreamWaitEvent(st, c->ev_join, 0) != hipSuccess))
+        return FAIL(BEM2D_ERR_DEVICE, "stream join failed");
     return hipGetLastError() == hipSuccess
                ? BEM2D_OK
                : FAIL(BEM2D_ERR_DEVICE, "basis assembly launch failed");
@@ -734,13 +626,44 @@ bool curve_ok(int type) { return type >= BEM2D_ELEM_CIRCLE && type <= BEM2D_ELEM
 
 } // namespace
 
+// Validated device descriptor of a basis list: groups of at most kGroup bases, G per group.
+int bem2d_dev::bem2d_internal_make_basis_set(const bem2d_basis* bases, int nb, BasisSet* out)
+{
+    if (!bases || nb < 1 || nb > kMaxBases)
+        return FAIL(BEM2D_ERR_ARG, "basis list: need 1 <= nb <= %d (got %d)", kMaxBases, nb);
+    BasisSet s{};
+    s.nb = nb;
+    s.ngroup = (nb + kGroup - 1) / kGroup;
+    s.G = (nb + s.ngroup - 1) / s.ngroup;
+    for (int b = 0; b < nb; ++b) {
+        const bem2d_basis& x = bases[b];
+        if (x.kind < BEM2D_BASIS_CONST || x.kind > BEM2D_BASIS_WAVEPOL)
+            return FAIL(BEM2D_ERR_ARG, "basis %d: unknown kind %d", b, x.kind);
+        if ((x.kind == BEM2D_BASIS_POL || x.kind == BEM2D_BASIS_WAVEPOL) &&
+            (x.degree < 0 || x.degree > 64))
+            return FAIL(BEM2D_ERR_ARG, "basis %d: degree %d outside [0, 64]", b, x.degree);
+        if ((x.kind == BEM2D_BASIS_WAVE || x.kind == BEM2D_BASIS_WAVEPOL) &&
+            !(std::isfinite(x.direction) && std::isfinite(x.k_re) && std::isfinite(x.k_im)))
+            return FAIL(BEM2D_ERR_ARG, "basis %d: direction or wavenumber not finite", b);
+        s.kind[b] = x.kind;
+        s.deg[b] = (x.kind == BEM2D_BASIS_POL || x.kind == BEM2D_BASIS_WAVEPOL || x.kind == BEM2D_BASIS_LIN)
+                       ? x.degree
+                       : 0;
+        s.dir[b] = x.direction;
+        s.kr[b] = x.k_re;
+        s.ki[b] = x.k_im;
+    }
+    *out = s;
+    return BEM2D_OK;
+}
+
 extern "C" int bem2d_assemble_basis(bem2d_ctx* c, const bem2d_mesh* m, const bem2d_basis* bases,
                                     int nb, int kind, double k_re, double k_im, double eta,
                                     double mscale, double kscale, void* A, int64_t lda)
 {
     if (!c || !m || !A) return FAIL(BEM2D_ERR_ARG, "bem2d_assemble_basis: null argument");
     BasisSet s;
-    if (int rc = make_basis_set(bases, nb, &s)) return rc;
+    if (int rc = bem2d_internal_make_basis_set(bases, nb, &s)) return rc;
     if (lda < m->n * nb)
         return FAIL(BEM2D_ERR_ARG, "bem2d_assemble_basis: lda %lld < n nb %lld", (long long)lda,
                     (long long)(m->n * nb));
@@ -787,7 +710,7 @@ extern "C" int bem2d_rhs_basis(bem2d_ctx* c, const bem2d_mesh* m, const bem2d_ba
     if (!c || !m || !B || (!dirs && nrhs > 0))
         return FAIL(BEM2D_ERR_ARG, "bem2d_rhs_basis: null argument");
     BasisSet s;
-    if (int rc = make_basis_set(bases, nb, &s)) return rc;
+    if (int rc = bem2d_internal_make_basis_set(bases, nb, &s)) return rc;
     if (nrhs < 0 || nrhs > 65535 || ldb < m->n * nb)
         return FAIL(BEM2D_ERR_ARG, "bem2d_rhs_basis: bad sizes");
     if (kind < BEM2D_FIELD_PLANE || kind > BEM2D_FIELD_COMBINED_PLANE)
@@ -815,30 +738,13 @@ extern "C" int bem2d_eval_single_layer_basis(bem2d_ctx* c, const bem2d_mesh* m,
     if (!c || !m || !coef || (!pts && npts > 0) || (!out && npts > 0))
         return FAIL(BEM2D_ERR_ARG, "bem2d_eval_single_layer_basis: null argument");
     BasisSet s;
-    if (int rc = make_basis_set(bases, nb, &s)) return rc;
+    if (int rc = bem2d_internal_make_basis_set(bases, nb, &s)) return rc;
     if (k_im == 0.0 ? !(k_re > 0.0) : !(k_re > 0.0 && std::isfinite(k_re) && std::isfinite(k_im)))
         return FAIL(BEM2D_ERR_UNSUPPORTED, "bem2d_eval_single_layer_basis: needs Re k > 0 (got %g%+gi)",
                     k_re, k_im);
     if (npts <= 0) return BEM2D_OK;
-    if ((npts + 127) / 128 > (int64_t)UINT32_MAX / 128)
-        return FAIL(BEM2D_ERR_UNSUPPORTED, "bem2d_eval_single_layer_basis: too many points");
     HIPCHK(hipSetDevice(c->device));
-    const double* tab = nullptr;
-    int nt = 0;
-    if (k_im != 0.0)
-        if (int rc = bem2d_internal_upload_contour(c, k_re, k_im, &tab, &nt)) return rc;
-    const size_t btab = basis_table_bytes(m, s);
-    const unsigned grid = (unsigned)((npts + 127) / 128);
-    if (k_im == 0.0)
-        eval_sl_basis_kernel<false><<<grid, 128, btab, c->stream>>>(
-            m->n, m->nq, m->d_nodes, m->d_g1d, s, k_re, 0.0, nullptr, 0, (const double2*)coef, pts,
-            npts, (double2*)out);
-    else
-        eval_sl_basis_kernel<true><<<grid, 128, btab, c->stream>>>(
-            m->n, m->nq, m->d_nodes, m->d_g1d, s, k_re, k_im, tab, nt, (const double2*)coef, pts,
-            npts, (double2*)out);
-    HIPCHK(hipGetLastError());
-    return BEM2D_OK;
+    return bem2d_internal_potential(c, m, s, k_re, k_im, coef, pts, npts, out);
 }
 
 extern "C" int bem2d_farfield_basis(bem2d_ctx* c, const bem2d_mesh* m, const bem2d_basis* bases,
@@ -848,7 +754,7 @@ extern "C" int bem2d_farfield_basis(bem2d_ctx* c, const bem2d_mesh* m, const bem
     if (!c || !m || !coef || !theta || !out)
         return FAIL(BEM2D_ERR_ARG, "bem2d_farfield_basis: null argument");
     BasisSet s;
-    if (int rc = make_basis_set(bases, nb, &s)) return rc;
+    if (int rc = bem2d_internal_make_basis_set(bases, nb, &s)) return rc;
     const int64_t nn = m->n * nb;
     if (ldc < nn || nrhs < 1 || nrhs > 65535 || nang < 1 || nang >= (1 << 24) || !(k > 0))
         return FAIL(BEM2D_ERR_ARG, "bem2d_farfield_basis: bad sizes");
@@ -876,7 +782,7 @@ extern "C" int bem2d_basis_values(const bem2d_basis* bases, int nb, const double
                                   double* out)
 {
     BasisSet s;
-    if (int rc = make_basis_set(bases, nb, &s)) return rc;
+    if (int rc = bem2d_internal_make_basis_set(bases, nb, &s)) return rc;
     if (count < 0 || (count > 0 && (!t || !out)))
         return FAIL(BEM2D_ERR_ARG, "bem2d_basis_values: bad arguments");
     for (int64_t q = 0; q < count; ++q)

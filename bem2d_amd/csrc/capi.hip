@@ -18,6 +18,7 @@
 
 #include "../../include/bem2d_hip.h"
 #include "assembly_kernels.h"
+#include "basis.h"
 #include "internal.h"
 
 using namespace bem2d_dev;
@@ -916,12 +917,13 @@ extern "C" int bem2d_eval_single_layer(bem2d_ctx* c, const bem2d_mesh* m, double
 {
     if (!c || !m || !coef || (!pts && npts > 0) || (!out && npts > 0))
         return FAIL(BEM2D_ERR_ARG, "bem2d_eval_single_layer: null argument");
+    if (!(k > 0.0)) return FAIL(BEM2D_ERR_ARG, "bem2d_eval_single_layer: wavenumber must be positive");
     if (npts <= 0) return BEM2D_OK;
     HIPCHK(hipSetDevice(c->device));
-    eval_sl_kernel<<<(unsigned)((npts + 127) / 128), 128, 0, c->stream>>>(
-        m->n, m->nq, m->d_nodes, k, (const double2*)coef, pts, npts, (double2*)out);
-    HIPCHK(hipGetLastError());
-    return BEM2D_OK;
+    const bem2d_basis p0{BEM2D_BASIS_CONST, 0, 0.0, 0.0, 0.0};
+    BasisSet bs;
+    if (int rc = bem2d_internal_make_basis_set(&p0, 1, &bs)) return rc;
+    return bem2d_internal_potential(c, m, bs, k, 0.0, coef, pts, npts, out);
 }
 
 extern "C" int bem2d_farfield(bem2d_ctx* c, const bem2d_mesh* m, double k, const void* coef,

@@ -13,6 +13,7 @@
 
 #pragma clang diagnostic ignored "-Wunused-function" // real-k kernels of the shared header
 #include "assembly_kernels.h"
+#include "basis.h"
 #include "complex_kernels.h"
 #include "internal.h"
 
@@ -110,35 +111,6 @@ __global__ void rhs_ck_kernel(int64_t n, int nq, const double* __restrict__ node
         ai = fma(fi, sw, ai);
     }
     B[i + ldb * r] = make_double2(scale * ar, scale * ai);
-}
-
-// EvalKernel<SingleLayer> with complex k at points: thread per point.
-__global__ void eval_sl_ck_kernel(int64_t n, int nq, const double* __restrict__ nodes, double kr,
-                                  double ki, const double* __restrict__ tab, int nt,
-                                  const double2* __restrict__ coef, const double* __restrict__ pts,
-                                  int64_t npts, double2* __restrict__ out)
-{
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= npts) return;
-    const double px = pts[2 * p], py = pts[2 * p + 1];
-    const int64_t plane = n * nq;
-    double vr = 0.0, vi = 0.0;
-    for (int64_t j = 0; j < n; ++j) {
-        double sr = 0.0, si = 0.0;
-        for (int a = 0; a < nq; ++a) {
-            const int64_t o = (int64_t)a * n + j;
-            double kvr, kvi;
-            kernel_eval_ck<BEM2D_KERNEL_SL>(kr, ki, 0.0, tab, nt, px, py, nodes[NODE_X * plane + o],
-                                            nodes[NODE_Y * plane + o], 0, 0, 0, 0, kvr, kvi);
-            const double sw = nodes[NODE_SW * plane + o];
-            sr = fma(kvr, sw, sr);
-            si = fma(kvi, sw, si);
-        }
-        const double2 c = coef[j];
-        vr += c.x * sr - c.y * si;
-        vi += c.x * si + c.y * sr;
-    }
-    out[p] = make_double2(vr, vi);
 }
 
 __global__ void hankel01_complex_kernel(const double* __restrict__ tab, int nt,
@@ -288,14 +260,10 @@ extern "C" int bem2d_eval_single_layer_complex(bem2d_ctx* c, const bem2d_mesh* m
                     k_im);
     if (npts <= 0) return BEM2D_OK;
     HIPCHK(hipSetDevice(c->device));
-    const double* tab = nullptr;
-    int nt = 0;
-    if (int rc = upload_contour(c, k_re, k_im, &tab, &nt)) return rc;
-    eval_sl_ck_kernel<<<(unsigned)((npts + 127) / 128), 128, 0, c->stream>>>(
-        m->n, m->nq, m->d_nodes, k_re, k_im, tab, nt, (const double2*)coef, pts, npts,
-        (double2*)out);
-    HIPCHK(hipGetLastError());
-    return BEM2D_OK;
+    const bem2d_basis p0{BEM2D_BASIS_CONST, 0, 0.0, 0.0, 0.0};
+    BasisSet bs;
+    if (int rc = bem2d_internal_make_basis_set(&p0, 1, &bs)) return rc;
+    return bem2d_internal_potential(c, m, bs, k_re, k_im, coef, pts, npts, out);
 }
 
 extern "C" int bem2d_hankel01_complex(bem2d_ctx* c, double phi, const double* z, int64_t n,

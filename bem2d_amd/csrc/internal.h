@@ -18,7 +18,8 @@ enum {
     SCRATCH_PIVOT = 3,    // pivots + info of the FP64 LU
     SCRATCH_CONTOUR = 4,  // Hankel contour table of a complex wavenumber
     SCRATCH_SPECTRAL = 5, // rocSOLVER info, off-diagonals, eigenvalues of the spectral drivers
-    SCRATCH_COUNT = 6
+    SCRATCH_EVAL = 6,     // potential: source-node weights and per-chunk partial sums
+    SCRATCH_COUNT = 7
 };
 
 struct ScratchBuf {
@@ -107,6 +108,14 @@ void* bem2d_internal_scratch(bem2d_ctx* c, int slot, size_t bytes);
 // stream order on c->stream; *nt = its node count.
 int bem2d_internal_upload_contour(bem2d_ctx* c, double kr, double ki, const double** d_tab,
                                   int* nt);
+namespace bem2d_dev {
+struct BasisSet;
+}
+// Single-layer potential of the density sum_b coef[j nb + b] phi_b at npts device points (real
+// k_im == 0, or complex k), on c->stream.
+int bem2d_internal_potential(bem2d_ctx* c, const bem2d_mesh* m, const bem2d_dev::BasisSet& bs,
+                             double k_re, double k_im, const void* coef, const double* pts,
+                             int64_t npts, void* out);
 void bem2d_internal_mixed_release(bem2d_ctx* c);
 void bem2d_internal_gauss01(int n, std::vector<double>& x, std::vector<double>& w);
 size_t bem2d_internal_mesh_bytes(int64_t n, bem2d_quad_option q);

@@ -533,6 +533,89 @@ def run_kite(ctx, steps, warmup, n=4000, k=100.0, cpu_cols=0):
     }
 
 
+def run_disk(ctx, steps, warmup, k=1.0, ppw=2000, L=7, N=30, cpu=True):
+    """The reference's examples/disk.cpp at its largest setting (k = 1, ppw = 2000): the unit circle
+    as AnalyticCurve(n, Circle) with n = int(Length k ppw / 2 pi) pieces of equal arc length,
+    LinBasis (two per element), SoundSoftScattering with SetQuadOption(7, 30, 0.15); one step =
+    assembly of A = M + 2 K_combined (eta = k) with the bases + right-hand side + FP64 solve +
+    EvalSol at (-1.5, 0).  Accuracy: EvalSol against the analytic series of the sound-soft disk."""
+    import torch
+    import scipy.special as sp
+    from bem2d_amd import capi
+    from bem2d_amd import geometry as G
+    n = int(capi.curve_length(capi.ELEM_CIRCLE, [0, 0, 0, 0]) * k * ppw / 2.0 / math.pi)
+    geom = G.analytic_curve("circle", n)
+    bases = capi.lin_bases()
+    nn = n * len(bases)
+    mesh = ctx.mesh(geom, N, L, 0.15)
+    A = torch.empty((nn, nn), dtype=torch.complex128, device="cuda").t()
+    B = torch.empty((1, nn), dtype=torch.complex128, device="cuda").t()
+    pt = torch.tensor([[-1.5, 0.0]], dtype=torch.float64, device="cuda")
+    dirs = [[1.0, 0.0]]
+
+    def step():
+        ctx.assemble_basis(mesh, bases, capi.KERNEL_COMBINED, k, k, 1.0, 2.0, out=A)
+        ctx.rhs_basis(mesh, bases, capi.FIELD_COMBINED_PLANE, k, k, dirs, 2.0, out=B)
+        ctx.solve_fp64(A, B, preserve_A=False)
+        return ctx.eval_single_layer_basis(mesh, bases, k, B[:, 0].contiguous(), pt)
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    ctx.profile(True)
+    ctx.profile_reset()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s_u = step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    reg_ms, reg_n = ctx.profile_read(capi.PROF_ASSEMBLE_REGULAR)
+    sp_ms, sp_n = ctx.profile_read(capi.PROF_ASSEMBLE_SPECIAL)
+    lu_ms, lu_n = ctx.profile_read(capi.PROF_LU)
+    ctx.profile(False)
+    u = complex(np.exp(1j * k * -1.5) - s_u[0].item())
+    # sound-soft disk: u = u_inc - sum_m i^m J_m(k) / H_m(k) H_m(k r) e^{i m theta}, r = 1.5,
+    # theta = pi
+    ms = np.arange(-60, 61)
+    u_ser = complex(np.exp(1j * k * -1.5) - np.sum(
+        (1j ** ms) * sp.jv(ms, k) / sp.hankel1(ms, k) * sp.hankel1(ms, 1.5 * k) *
+        np.exp(1j * ms * math.pi)))
+    nspecial = mesh.special_count
+    P_reg = (n * n - nspecial) * N * N
+    reg_avg = reg_ms / max(reg_n, 1)
+    res = {
+        "workload": "examples/disk.cpp at k=%g ppw=%d: unit circle AnalyticCurve n=%d (arc "
+                    "length), LinBasis (%d unknowns), combined field eta=k, N=%d, L=%d, sigma=0.15; "
+                    "step = basis assembly + right-hand side + FP64 solve + EvalSol at (-1.5, 0)"
+                    % (k, ppw, n, nn, N, L),
+        "n_elements": n, "unknowns": nn, "steps": steps,
+        "examples_per_s": steps / dt,
+        "ms_per_step": 1e3 * dt / steps,
+        "assembly_regular_ms": reg_avg,
+        "assembly_special_ms": sp_ms / max(sp_n, 1),
+        "lu_fp64_ms": lu_ms / max(lu_n, 1),
+        "gentries_per_s": nn * nn / (reg_avg * 1e-3) / 1e9,
+        "gpairpts_per_s": P_reg / (reg_avg * 1e-3) / 1e9,
+        "evalsol": [u.real, u.imag],
+        "evalsol_vs_series_rel_err": abs(u - u_ser) / abs(u_ser),
+    }
+    if cpu:
+        # the oracle (OpenMP over columns) on a bounded column sample, scaled to the whole matrix
+        from oracle import oracle as O
+        cols = np.linspace(0, nn - 1, 8).astype(np.int64)
+        t0 = time.perf_counter()
+        O.assemble_basis(geom, bases, N, L, 0.15, O.K_COMBINED, k, k, 1.0, 2.0, cols=cols)
+        tc = time.perf_counter() - t0
+        res["cpu_assembly_estimate_s"] = tc * nn / cols.size
+        res["cpu_sample"] = "oracle assemble_basis of 8 of the %d columns in %.2f s on %d threads, " \
+            "scaled to the whole matrix" % (nn, tc, os.cpu_count() or 1)
+    mesh.close()
+    del A, B
+    ctx.release_scratch()
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -541,7 +624,7 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=3,
                     help="timed steps of the plain FP64 path on the large configuration")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--config", choices=["all", "large", "kite", "batched", "serving"],
+    ap.add_argument("--config", choices=["all", "large", "kite", "batched", "serving", "disk"],
                     default="all")
     ap.add_argument("--no-extras", action="store_true",
                     help="the large configuration only (no kite, batched and served legs)")
@@ -574,10 +657,13 @@ def main():
         return
     ctx = Context(0)
     cpu = not args.no_cpu_baseline
-    if args.config in ("kite", "batched", "serving"):
+    if args.config in ("kite", "batched", "serving", "disk"):
         if args.config == "kite":
             res = run_kite(ctx, args.steps, args.warmup)
             value, t = res["solves_per_s_mixed"], res["ms_per_step_mixed"]
+        elif args.config == "disk":
+            res = run_disk(ctx, min(args.steps, 5), min(args.warmup, 1), cpu=cpu)
+            value, t = res["examples_per_s"], res["ms_per_step"]
         elif args.config == "batched":
             res = run_batched(ctx)
             value, t = res["mixed"]["solves_per_s"], None
@@ -622,6 +708,8 @@ def main():
         progress("kite served done")
         out["batched"] = run_batched(ctx)
         progress("C3 batch done")
+        out["disk_lin"] = run_disk(ctx, 3, 1, cpu=cpu)
+        progress("disk.cpp (LinBasis) leg done")
         sv = run_serving(ctx, args.serving_count, args.serving_rate)
         out["serving"] = sv
         out["p50_ms"] = sv["mixed"]["p50_ms"]
