@@ -156,6 +156,7 @@ SIGNATURES = {
     "bem2d_l2_norm_cond": (_I, [_P, _I64, _P, _I64, ctypes.POINTER(_D), ctypes.POINTER(_D)]),
     "bem2d_hermitian_eigenvalues": (_I, [_P, _I64, _P, _I64, _P]),
     "bem2d_numerical_range": (_I, [_P, _I64, _P, _I64, _I, _P]),
+    "bem2d_eigenvalues": (_I, [_P, _I64, _P, _I64, _P]),
     "bem2d_eval_single_layer": (_I, [_P, _P, _D, _P, _P, _I64, _P]),
     "bem2d_farfield": (_I, [_P, _P, _D, _P, _I64, _I, _P, _I64, _P]),
     "bem2d_assemble_basis": (_I, [_P, _P, _P, _I, _I, _D, _D, _D, _D, _D, _P, _I64]),
@@ -462,6 +463,13 @@ class Context:
         import torch
         w = torch.empty(A.shape[0], dtype=torch.float64, device=A.device)
         check(lib().bem2d_hermitian_eigenvalues(self.h, A.shape[0], _t_ptr(A), _ld(A), _t_ptr(w)))
+        return w
+
+    def eigenvalues(self, A):
+        """All eigenvalues of the general complex matrix A (device, unchanged): complex128 tensor."""
+        import torch
+        w = torch.empty(A.shape[0], dtype=torch.complex128, device=A.device)
+        check(lib().bem2d_eigenvalues(self.h, A.shape[0], _t_ptr(A), _ld(A), _t_ptr(w)))
         return w
 
     def numerical_range(self, A, nsteps):

@@ -522,6 +522,19 @@ void HermitianEigenvalues(const Matrix& k, pdvector& evalues)
     raise(bem2d_copy_to_host(device::context(), evalues->data(), W.p, sizeof(double) * n));
 }
 
+void Eigenvalues(const Matrix& k, pcvector& evalues)
+{
+    square(k);
+    const int64_t n = k.dim[0];
+    evalues = pcvector(new cvector(n));
+    device::DevBuf A(sizeof(complex) * n * n), W(sizeof(complex) * n);
+    raise(bem2d_copy_to_device(device::context(), A.p, k.data->data(), sizeof(complex) * n * n));
+    const int rc = bem2d_eigenvalues(device::context(), n, A.p, n, W.p);
+    if (rc == BEM2D_ERR_NOT_CONVERGED) throw LapackError(); // zgeev's info > 0
+    raise(rc);
+    raise(bem2d_copy_to_host(device::context(), evalues->data(), W.p, sizeof(complex) * n));
+}
+
 Matrix ChangeBasis(const Matrix& k, const Matrix& m)
 {
     square(k);

@@ -63,14 +63,17 @@ void InPolygon(const std::shared_ptr<Geometry> polygon, const std::vector<Point>
 // X = m^{-1} rhs (FP64 LU with partial pivoting on the GPU); m is not modified.
 pMatrix SolveSystem(Matrix& m, Matrix& rhs);
 
-// Spectral drivers (lib/bem2d_mathroutines.h:48-73), on the GPU (rocSOLVER / rocBLAS):
-// largest singular value and condition number; all singular values (descending); eigenvalues of a
-// Hermitian matrix (ascending); k in the orthonormal basis of the mass matrix m, U^{-H} k U^{-1}
-// with m = U^H U; the boundary of the numerical range at n angles written to filename as
-// "re im" lines; the conjugate transpose.  Eigenvalues of a general matrix are not provided.
+// Spectral drivers (lib/bem2d_mathroutines.h:48-85), on the GPU (rocSOLVER / rocBLAS and the
+// library's Hessenberg + multishift QR): largest singular value and condition number; all singular
+// values (descending); eigenvalues of a Hermitian matrix (ascending); eigenvalues of a general matrix
+// (zgeev's set, in the order of the computed triangular form); k in the orthonormal basis of the
+// mass matrix m, U^{-H} k U^{-1} with m = U^H U; the boundary of the numerical range at n angles
+// written to filename as "re im" lines; the conjugate transpose.  The eigenvector overload
+// Eigenvalues(k, evalues, evectors) is not provided.
 void L2NormCond(const Matrix& stiff, double& norm, double& cond);
 void SingularValues(const Matrix& stiff, pdvector& singvals);
 void HermitianEigenvalues(const Matrix& k, pdvector& evalues);
+void Eigenvalues(const Matrix& k, pcvector& evalues);
 Matrix ChangeBasis(const Matrix& k, const Matrix& m);
 void NumRange(const Matrix& m, int n, std::string filename);
 Matrix ConjTranspose(const Matrix& m);

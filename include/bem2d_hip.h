@@ -250,6 +250,12 @@ int bem2d_l2_norm_cond(bem2d_ctx* ctx, int64_t n, const void* A, int64_t lda, do
 /* HermitianEigenvalues (:353-410): w (device, n doubles) = the eigenvalues of the Hermitian A,
  * ascending (zheev without vectors, upper triangle referenced). */
 int bem2d_hermitian_eigenvalues(bem2d_ctx* ctx, int64_t n, const void* A, int64_t lda, double* w);
+/* Eigenvalues (lib/bem2d_mathroutines.cpp:438-500, zgeev with no eigenvectors there): all n
+ * eigenvalues of the general complex matrix A (device, unchanged) into w (device, n complex128), in
+ * the order of the diagonal of the computed triangular form (LAPACK's order may differ; the set is
+ * the same).  Hessenberg reduction + multishift QR on the GPU.  BEM2D_ERR_NOT_CONVERGED when the QR
+ * iteration does not converge. */
+int bem2d_eigenvalues(bem2d_ctx* ctx, int64_t n, const void* A, int64_t lda, void* w);
 /* NumRange (:590-653) without the file: the extreme eigenvalues of (e^{i t} A + (e^{i t} A)^H) / 2 at
  * t = i pi / nsteps, i < nsteps, and the 2 nsteps + 1 boundary points (re, im) of the numerical
  * range built from them as the reference builds them; points is a HOST array of 2 (2 nsteps + 1)

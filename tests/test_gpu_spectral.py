@@ -112,3 +112,109 @@ def test_numerical_range_of_a_normal_matrix(ctx):
     # L tan(pi/(2m)) / 2 beyond it
     edge = 2 * 2.0 * np.sin(np.pi / 7)
     assert off.max() <= edge * np.tan(np.pi / (2 * m)) / 2 + 1e-10
+
+
+# ---- Eigenvalues (zgeev, eigenvalues only) ----------------------------------------------------
+def _match(got, ref):
+    """Largest distance between the two eigenvalue sets under the best one-to-one matching."""
+    from scipy.optimize import linear_sum_assignment
+    d = np.abs(np.asarray(got)[:, None] - np.asarray(ref)[None, :])
+    r, c = linear_sum_assignment(d)
+    return d[r, c].max()
+
+
+@pytest.mark.parametrize("n", [3, 10, 64, 65, 100, 257])
+def test_eigenvalues_random(ctx, n):
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    print("eigenvalues n", n, flush=True)
+    w = _np(ctx.eigenvalues(_dev(A)))
+    ref = np.linalg.eigvals(A)
+    err = _match(w, ref)
+    print("n %d: max matched distance %.2e" % (n, err))
+    assert err < 1e-10 * np.linalg.norm(A, 2)
+
+
+def test_eigenvalues_of_bem_operators(ctx, oracle):
+    """The numerical-range drivers' operator (numrangecomps/disk.cpp:92-99): Id + 2 CDL - 2 i eta SL
+    in the mass matrix's orthonormal basis, with P0 and with PolBasis(2) on a kite."""
+    K, Id = _operators(oracle, n=150)
+    Kd = _dev(K)
+    KB = ctx.change_basis(Kd.clone(), _dev(Id))
+    w = _np(ctx.eigenvalues(KB))
+    ref = np.linalg.eigvals(_np(KB))
+    assert _match(w, ref) < 1e-11 * np.abs(ref).max()
+    geom = G.analytic_curve("kite", 100)
+    bases = [(1, 0, 0.0, 0.0), (1, 1, 0.0, 0.0), (1, 2, 0.0, 0.0)]
+    k = 8.0
+    cdl = oracle.assemble_basis(geom, bases, 5, 3, 0.15, oracle.K_CDL, k, 0.0, 0.0, 1.0)
+    sl = oracle.assemble_basis(geom, bases, 5, 3, 0.15, oracle.K_SL, k, 0.0, 0.0, 1.0)
+    M = oracle.assemble_basis(geom, bases, 5, 3, 0.15, oracle.K_NONE, 1.0, 0.0, 1.0, 0.0)
+    A = M + 2 * cdl - 2j * k * sl
+    AB = ctx.change_basis(_dev(A), _dev(M))
+    w = _np(ctx.eigenvalues(AB))
+    ref = np.linalg.eigvals(_np(AB))
+    err = _match(w, ref)
+    print("kite P2 (300 x 300): %.2e" % err)
+    assert err < 1e-10 * np.abs(ref).max()
+
+
+def test_eigenvalues_closed_forms(ctx):
+    """A normal matrix with a known spectrum (repeated eigenvalues included), a triangular matrix
+    (already in Schur form) and a companion matrix of a polynomial with known roots."""
+    rng = np.random.default_rng(5)
+    n = 120
+    lam = np.exp(2j * np.pi * rng.uniform(size=n)) * rng.uniform(0.5, 2.0, size=n)
+    lam[10:20] = lam[0]                       # a tenfold eigenvalue
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    A = (Q * lam) @ Q.conj().T
+    assert _match(_np(ctx.eigenvalues(_dev(A))), lam) < 1e-12 * np.abs(lam).max()
+    T = np.triu(rng.standard_normal((80, 80)) + 1j * rng.standard_normal((80, 80)))
+    assert _match(_np(ctx.eigenvalues(_dev(T))), np.diag(T)) < 1e-13 * np.abs(T).max()
+    roots = np.exp(2j * np.pi * np.arange(12) / 12) * (1 + 0.1 * np.arange(12))
+    coef = np.poly(roots)
+    C = np.zeros((12, 12), dtype=complex)
+    C[0, :] = -coef[1:]
+    C[1:, :-1] = np.eye(11)
+    assert _match(_np(ctx.eigenvalues(_dev(C))), roots) < 1e-9
+
+
+def test_numrange_driver_matches_numpy(ctx, oracle, tmp_path):
+    """The drop-in driver shaped like numrangecomps/disk.cpp (AnalyticCurve, PolBasis::AddBasis(2),
+    DiscreteKernel SL / CDL, EvalIdent, ChangeBasis, Eigenvalues, L2NormCond, NumRange) against
+    numpy on the oracle's matrices."""
+    import json
+    import os
+    import subprocess
+    from conftest import ROOT
+    drv = os.path.join(ROOT, "bem2d_amd", "host", "numrange_driver")
+    if not os.path.exists(drv):
+        pytest.fail("numrange_driver is not built")
+    n, k, nsteps = 60, 6.0, 40
+    f = str(tmp_path / "nr.txt")
+    out = subprocess.run([drv, "kite", str(n), str(k), str(nsteps), f], capture_output=True,
+                         text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    d = json.loads(out.stdout)
+    geom = G.analytic_curve("kite", n)
+    bases = [(1, 0, 0.0, 0.0), (1, 1, 0.0, 0.0), (1, 2, 0.0, 0.0)]
+    sl = oracle.assemble_basis(geom, bases, 5, 3, 0.15, oracle.K_SL, k, 0.0, 0.0, 1.0)
+    cdl = oracle.assemble_basis(geom, bases, 5, 3, 0.15, oracle.K_CDL, k, 0.0, 0.0, 1.0)
+    M = oracle.assemble_basis(geom, bases, 5, 3, 0.15, oracle.K_NONE, 1.0, 0.0, 1.0, 0.0)
+    A = M + 2 * cdl - 2j * k * sl
+    L = np.linalg.cholesky(M)
+    KB = np.linalg.solve(L, np.linalg.solve(L, A.conj().T).conj().T)
+    ref = np.linalg.eigvals(KB)
+    got = np.array(d["eigenvalues"])[:, 0] + 1j * np.array(d["eigenvalues"])[:, 1]
+    assert d["size"] == 3 * n
+    assert _match(got, ref) < 1e-10 * np.abs(ref).max()
+    s = np.linalg.svd(KB, compute_uv=False)
+    assert abs(d["norm"] - s[0]) < 1e-11 * s[0] and abs(d["cond"] - s[0] / s[-1]) < 1e-9 * d["cond"]
+    z = np.loadtxt(f)
+    z = z[:, 0] + 1j * z[:, 1]
+    assert z.size == 2 * nsteps + 1
+    th = np.arange(nsteps) * np.pi / nsteps
+    for t in range(0, nsteps, 7):
+        e = np.linalg.eigvalsh(0.5 * (np.exp(1j * th[t]) * KB + (np.exp(1j * th[t]) * KB).conj().T))
+        # the file holds 6 significant digits (the stream's default precision)
+        assert abs((np.exp(1j * th[t]) * z[t]).real - e[0]) < 1e-4 * np.abs(e).max()
