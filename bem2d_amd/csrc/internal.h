@@ -39,6 +39,7 @@ struct TuneState {
     int asm_split[kMaxTuneBuckets] = {4};
     // FP32 LU: panels pair into depth-128 updates while the trailing matrix has >= this many tiles
     int lu_pair_tiles = 80;
+    int lu_split_tiles = 64;   // trailing tiles from which the diagonal block is factorised apart
 };
 
 struct ProfSlot {

@@ -934,7 +934,8 @@ __device__ __forceinline__ void cgemm_64t(const float* __restrict__ Are,
     }
 }
 
-__global__ __launch_bounds__(256) void trailing_update_kernel(const LuBatch batch)
+template <bool FUSED>
+__global__ __launch_bounds__(256, FUSED ? 1 : 2) void trailing_update_kernel(const LuBatch batch)
 {
     const int di = find_problem(batch.start, batch.count, blockIdx.x);
     const LuProblem& pb = batch.p[di];
@@ -943,13 +944,15 @@ __global__ __launch_bounds__(256) void trailing_update_kernel(const LuBatch batc
     const int64_t ld = pb.n, k0 = pb.k0;
     const int K = pb.kb;
     const int w = (int)((int64_t)blockIdx.x - batch.start[di]);
-    const bool fused = pb.Linv_next != nullptr;
+    const bool fused = FUSED && pb.Linv_next != nullptr;
     int first = w, stride = pb.nwg;
     if (fused) {
         first = w;
         stride = w == 0 ? pb.ntile : pb.nwg - 1;
     }
-    __shared__ __align__(16) float lds[kUpdLdsFloats];
+    // the unfused update stores straight from the accumulators (no output quarters in LDS), so
+    // that two workgroups fit on a compute unit
+    __shared__ __align__(16) float lds[FUSED ? kUpdLdsFloats : kUpdOpFloats];
     float* Are = lds;
     float* Aim = lds + NB * KPA;
     float* Bre = lds + 2 * NB * KPA;
@@ -984,8 +987,27 @@ __global__ __launch_bounds__(256) void trailing_update_kernel(const LuBatch batc
     __amdgpu_buffer_rsrc_t rres = slab(A);
     bool pending = false;
     // this wavefront's output quarter in LDS: column c (0..31) at c * kOutPitch complex values
-    float2* outq = reinterpret_cast<float2*>(lds + kUpdOpFloats) + wave * 32 * kOutPitch;
+    float2* outq = reinterpret_cast<float2*>(lds + (FUSED ? kUpdOpFloats : 0)) + wave * 32 * kOutPitch;
     auto store_res = [&]() __attribute__((always_inline)) {
+        if constexpr (!FUSED) {
+            // rows 16 p + 4 (lane / 16) .. + 3 of column 16 q + lane % 16: two 16-byte stores
+#pragma unroll
+            for (int p = 0; p < 2; ++p)
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    const unsigned off = (unsigned)(((wr + 16 * p + 4 * (lane >> 4)) +
+                                                     ld * (wc + 16 * q + (lane & 15))) * 8);
+                    const float4 v0 = make_float4(res[p][q][0].x, res[p][q][0].y, res[p][q][1].x,
+                                                  res[p][q][1].y);
+                    const float4 v1 = make_float4(res[p][q][2].x, res[p][q][2].y, res[p][q][3].x,
+                                                  res[p][q][3].y);
+                    decltype(__builtin_amdgcn_raw_buffer_load_b128(rres, 0u, 0u, 0)) a0, a1;
+                    __builtin_memcpy(&a0, &v0, 16);
+                    __builtin_memcpy(&a1, &v1, 16);
+                    __builtin_amdgcn_raw_buffer_store_b128(a0, rres, off, 0u, 0);
+                    __builtin_amdgcn_raw_buffer_store_b128(a1, rres, off + 16u, 0u, 0);
+                }
+        } else {
         // accumulator rows 16 p + 4 (lane / 16) .. + 3 of column 16 q + lane % 16: 32 bytes each
 #pragma unroll
         for (int p = 0; p < 2; ++p)
@@ -1010,6 +1032,7 @@ __global__ __launch_bounds__(256) void trailing_update_kernel(const LuBatch batc
             __builtin_memcpy(&a, &v, 16);
             __builtin_amdgcn_raw_buffer_store_b128(
                 a, rres, (unsigned)(((wr + r) + ld * (wc + c)) * 8), 0u, 0);
+        }
         }
     };
     int idx = first;
@@ -1063,10 +1086,12 @@ __global__ __launch_bounds__(256) void trailing_update_kernel(const LuBatch batc
         idx += stride;
     }
     if (pending) store_res();
-    if (fused && w == 0) {
-        __syncthreads(); // the tile's stores are visible to the workgroup; the LDS is free
-        diag_factor(A, ld, rest0, NB, pb.Linv_next, pb.UinvT_next, pb.bad,
-                    *reinterpret_cast<DiagLds*>(lds));
+    if constexpr (FUSED) {
+        if (fused && w == 0) {
+            __syncthreads(); // the tile's stores are visible to the workgroup; the LDS is free
+            diag_factor(A, ld, rest0, NB, pb.Linv_next, pb.UinvT_next, pb.bad,
+                        *reinterpret_cast<DiagLds*>(lds));
+        }
     }
 }
 
@@ -1598,6 +1623,7 @@ struct bem2d_lu32 {
     const void* graph_A = nullptr;
     int64_t graph_lda = 0;
     int graph_pair_tiles = 0;     // the panel-pairing choice the graph was captured with
+    int graph_split_tiles = 0;    // and the diagonal-split choice
     int64_t graph_replays = 0;
 };
 
@@ -1753,16 +1779,25 @@ int bem2d_internal_lu_begin(hipStream_t s, bem2d_lu32* const* lus, const void* c
 }
 
 // One panel step for each of count problems (each at its own step): the panel solves of block b,
-// then the trailing update, whose first workgroup also factorises diagonal block b + 1.
+// then the trailing update, whose first workgroup also factorises diagonal block b + 1 (or, for
+// large trailing matrices, the update followed by that factorisation on its own).
 int bem2d_internal_lu_advance(hipStream_t s, bem2d_lu32* const* lus, int count)
 {
     std::vector<bem2d_lu32*> act;
     for (int d = 0; d < count; ++d)
         if (lus[d]->step >= 0 && lus[d]->step < lus[d]->nblk - 1) act.push_back(lus[d]);
     for (size_t c0 = 0; c0 < act.size(); c0 += kMaxLuBatch) {
-        LuBatch tr{}, up{};
+        LuBatch tr{}, up{}, dg{};
         const int cnt = (int)std::min<size_t>(kMaxLuBatch, act.size() - c0);
         tr.count = up.count = cnt;
+        // Trailing matrices of at least tune.lu_split_tiles tiles per side are updated without the
+        // fused diagonal block: the update then fits two workgroups per compute unit, and
+        // diag_lu_kernel factorises the block after it.  One launch shape for the whole batch:
+        // split only when every problem is that large.
+        bool split = true;
+        for (int d = 0; d < cnt; ++d)
+            if (lu_problem(act[c0 + d], act[c0 + d]->step).tiles < act[c0]->ctx->tune.lu_split_tiles)
+                split = false;
         for (int d = 0; d < cnt; ++d) {
             bem2d_lu32* f = act[c0 + d];
             const int b = f->step;
@@ -1791,12 +1826,23 @@ int bem2d_internal_lu_advance(hipStream_t s, bem2d_lu32* const* lus, int count)
             }
             // persistent workgroups: the compute units shared among the problems of the launch
             // (a fused update needs two: workgroup 0 holds the next diagonal block alone)
-            const int64_t share = std::max<int64_t>(2, act[c0]->cus / cnt);
+            // (two per compute unit for the unfused update, which fits two)
+            const int64_t share = std::max<int64_t>(2, (split ? 2 : 1) * act[c0]->cus / cnt);
             u.ntile = (int)blocks;
             u.nwg = (int)std::min<int64_t>(blocks, share);
+            if (split) { // the update alone; diag_lu_kernel factorises block b + 1 after it
+                u.Linv_next = nullptr;
+                u.UinvT_next = nullptr;
+                LuProblem q = lu_problem(f, b + 1);
+                q.Linv_next = p.Linv_next;
+                q.UinvT_next = p.UinvT_next;
+                dg.p[d] = q;
+                dg.start[d + 1] = d + 1;
+            }
             up.p[d] = u;
             up.start[d + 1] = up.start[d] + u.nwg;
         }
+        dg.count = split ? cnt : 0;
         if (up.start[cnt] > (int64_t)INT32_MAX) return FAIL(BEM2D_ERR_UNSUPPORTED, "LU grid too large");
         panel_trsm_kernel<<<(unsigned)tr.start[cnt], 256, 0, s>>>(tr);
         // HIP events around the update when the context profiles (not inside a graph capture)
@@ -1805,8 +1851,12 @@ int bem2d_internal_lu_advance(hipStream_t s, bem2d_lu32* const* lus, int count)
         const bool prof = pc->profiling && hipStreamIsCapturing(s, &cap) == hipSuccess &&
                           cap == hipStreamCaptureStatusNone;
         if (prof) bem2d_internal_prof_begin(pc, BEM2D_PROF_LU_UPDATE, s);
-        trailing_update_kernel<<<(unsigned)up.start[cnt], 256, 0, s>>>(up);
+        if (split)
+            trailing_update_kernel<false><<<(unsigned)up.start[cnt], 256, 0, s>>>(up);
+        else
+            trailing_update_kernel<true><<<(unsigned)up.start[cnt], 256, 0, s>>>(up);
         if (prof) bem2d_internal_prof_end(pc, BEM2D_PROF_LU_UPDATE, s);
+        if (split) diag_lu_kernel<<<dg.count, 256, 0, s>>>(dg);
     }
     if (hipGetLastError() != hipSuccess) return FAIL(BEM2D_ERR_DEVICE, "LU launch failed");
     for (bem2d_lu32* f : act) {
@@ -1856,7 +1906,8 @@ extern "C" int bem2d_lu32_factor(bem2d_ctx* c, bem2d_lu32* f, const void* A64, i
     hipStream_t s = c->stream;
     const bool graphs = lu_graphs_enabled();
     const bool current = f->graph && f->graph_A == A64 && f->graph_lda == lda &&
-                         f->graph_pair_tiles == c->tune.lu_pair_tiles;
+                         f->graph_pair_tiles == c->tune.lu_pair_tiles &&
+                         f->graph_split_tiles == c->tune.lu_split_tiles;
     if (graphs && !current && f->seen_A == A64 && f->seen_lda == lda) {
         if (!c->capture) HIPCHK(hipStreamCreateWithFlags(&c->capture, hipStreamNonBlocking));
         hipGraph_t g = nullptr;
@@ -1878,11 +1929,13 @@ extern "C" int bem2d_lu32_factor(bem2d_ctx* c, bem2d_lu32* f, const void* A64, i
         f->graph_A = A64;
         f->graph_lda = lda;
         f->graph_pair_tiles = c->tune.lu_pair_tiles;
+        f->graph_split_tiles = c->tune.lu_split_tiles;
     }
     bem2d_internal_prof_begin(c, BEM2D_PROF_LU);
     int rc = BEM2D_OK;
     if (graphs && f->graph && f->graph_A == A64 && f->graph_lda == lda &&
-        f->graph_pair_tiles == c->tune.lu_pair_tiles) {
+        f->graph_pair_tiles == c->tune.lu_pair_tiles &&
+        f->graph_split_tiles == c->tune.lu_split_tiles) {
         if (hipGraphLaunch(f->graph, s) != hipSuccess) rc = FAIL(BEM2D_ERR_DEVICE, "LU graph launch failed");
         // the host-side state the launch sequence leaves behind
         f->step = f->nblk - 1;
