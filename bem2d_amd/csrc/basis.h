@@ -60,4 +60,11 @@ __host__ __device__ inline void basis_value(const BasisSet& s, int b, double t, 
 // Checks a basis list and fills its descriptor (groups of at most kBasisGroup bases).
 int bem2d_internal_make_basis_set(const bem2d_basis* bases, int nb, BasisSet* out);
 
+struct AsmProblem;
+// The regular pairs of a P0 assembly with complex k (k = p.k + i ki, contour table tab / nt) on the
+// tiled kernel, when the mesh has enough tiles to fill the device; false (nothing launched)
+// otherwise.
+bool bem2d_internal_regular_ck_tiles(hipStream_t st, const AsmProblem& p, int kind, double ki,
+                                     const double* tab, int nt);
+
 } // namespace bem2d_dev

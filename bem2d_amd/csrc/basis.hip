@@ -53,14 +53,82 @@ __device__ __forceinline__ void stage_basis_table(const BasisSet& s, int nq,
 // = sum_a conj(phi_{g1+al}(x_a)) sw_a sum_b K(x_a, y_b) sw_b phi_{g2+be}(y_b); mirror
 // M[be][al] = entry (j nb + g1 + be, i nb + g2 + al) = sum_a phi_{g2+al}(x_a) sw_a
 // sum_b K(y_b, x_a) sw_b conj(phi_{g1+be}(y_b)).
-template <int KIND, int G, int MODE>
+// Pair evaluators: kernel values of one node pair for both orders (kr, ki: test x, trial y;
+// mr, mi: test y, trial x).  Real k: kernel_pair of the tier MODE.
+template <int KIND, int MODE>
+struct RealPair {
+    PairConst pc;
+    __device__ __forceinline__ RealPair(double k, double eta, const double* tab)
+        : pc(pair_const<MODE>(k, eta, tab))
+    {
+    }
+    __device__ __forceinline__ void operator()(double xx, double xy, double yx, double yy,
+                                               double n1x, double n1y, double n2x, double n2y,
+                                               double& kr, double& ki, double& mr,
+                                               double& mi) const
+    {
+        kernel_pair<KIND, MODE>(pc, xx, xy, yx, yy, n1x, n1y, n2x, n2y, kr, ki, mr, mi);
+    }
+};
+
+// Complex k = kr + i ki (the kernels' k.im != 0 branches, lib/bem2d_kernel.cpp:15-112): H0(kd),
+// H1(kd) once for both orders; the normal factor of the swapped order has the normals traded and
+// x - y negated, as in kernel_pair.  tab / nt: the contour table of arg k.
+template <int KIND>
+struct ComplexPair {
+    double kr, ki, eta;
+    const double* tab;
+    int nt;
+    __device__ __forceinline__ void operator()(double xx, double xy, double yx, double yy,
+                                               double n1x, double n1y, double n2x, double n2y,
+                                               double& vr, double& vi, double& mr,
+                                               double& mi) const
+    {
+        if (KIND == BEM2D_KERNEL_ID || KIND == BEM2D_KERNEL_NONE) {
+            vr = mr = (KIND == BEM2D_KERNEL_ID) ? 1.0 : 0.0;
+            vi = mi = 0.0;
+            return;
+        }
+        double wx, wy, d2, nw1, nw2;
+        {
+#pragma clang fp contract(off)
+            wx = xx - yx;
+            wy = xy - yy;
+            d2 = wx * wx + wy * wy;
+            nw1 = n1x * wx + n1y * wy; // n_x.(x - y)
+            nw2 = n2x * wx + n2y * wy; // n_y.(x - y)
+        }
+        const double d = sqrt(d2);
+        double h0r, h0i, h1r, h1i;
+        hankel01_complex(kr * d, ki * d, tab, nt, h0r, h0i, h1r, h1i);
+        if (KIND == BEM2D_KERNEL_SL) {
+            vr = mr = -0.25 * h0i;
+            vi = mi = 0.25 * h0r;
+            return;
+        }
+        const double c = (KIND == BEM2D_KERNEL_DL ? -nw2 : nw1) / d;
+        const double cm = (KIND == BEM2D_KERNEL_DL ? nw1 : -nw2) / d;
+        const double ar = kr * h1r - ki * h1i, ai = kr * h1i + ki * h1r; // k H1
+        vr = 0.25 * c * ai;
+        vi = -0.25 * c * ar;
+        mr = 0.25 * cm * ai;
+        mi = -0.25 * cm * ar;
+        if (KIND == BEM2D_KERNEL_COMBINED) {
+            vr += 0.25 * eta * h0r;
+            vi += 0.25 * eta * h0i;
+            mr += 0.25 * eta * h0r;
+            mi += 0.25 * eta * h0i;
+        }
+    }
+};
+
+template <int G, class Pair>
 __device__ __forceinline__ void pair_sums_basis(int nq, int64_t n, int64_t i,
                                                 const double* __restrict__ nodes, const double* sc,
                                                 const double* bt, int nbp, int g1, int g2,
-                                                const double* tab, double k, double eta,
-                                                double (&D)[G][G][2], double (&M)[G][G][2])
+                                                const Pair& pair, double (&D)[G][G][2],
+                                                double (&M)[G][G][2])
 {
-    const PairConst pc = pair_const<MODE>(k, eta, tab);
     const int64_t plane = n * nq;
 #pragma unroll
     for (int u = 0; u < G; ++u)
@@ -77,8 +145,7 @@ __device__ __forceinline__ void pair_sums_basis(int nq, int64_t n, int64_t i,
         for (int b = 0; b < nq; ++b) {
             const double* y = sc + b * 5;
             double kr, ki, kmr, kmi;
-            kernel_pair<KIND, MODE>(pc, xx, xy, y[0], y[1], n1x, n1y, y[2], y[3], kr, ki, kmr,
-                                    kmi);
+            pair(xx, xy, y[0], y[1], n1x, n1y, y[2], y[3], kr, ki, kmr, kmi);
             const double w = y[4];
             const double dr = kr * w, di = ki * w, mr = kmr * w, mi = kmi * w;
             const double* row = bt + 2 * b * nbp;
@@ -178,19 +245,19 @@ __global__ __launch_bounds__(64) void assemble_regular_basis_kernel(const AsmPro
     const double* sc = sh + c * ns;
     double D[G][G][2], M[G][G][2];
     if (all_far) {
-        pair_sums_basis<KIND, G, PAIR_FAR>(nq, n, i, nodes, sc, bt, nbp, g1, g2, nullptr, k, eta, D,
-                                           M);
+        pair_sums_basis<G>(nq, n, i, nodes, sc, bt, nbp, g1, g2,
+                           RealPair<KIND, PAIR_FAR>(k, eta, nullptr), D, M);
     } else if (all_mid) {
-        pair_sums_basis<KIND, G, PAIR_MID>(nq, n, i, nodes, sc, bt, nbp, g1, g2, nullptr, k, eta, D,
-                                           M);
+        pair_sums_basis<G>(nq, n, i, nodes, sc, bt, nbp, g1, g2,
+                           RealPair<KIND, PAIR_MID>(k, eta, nullptr), D, M);
     } else {
         // near pairs: the piece tables go to LDS, copied by the lanes still here
         const unsigned long long act = __ballot(1);
         const int rank = __popcll(act & ((1ull << lane) - 1ull));
         stage_piece_tables(piece_tab, rank, __popcll(act));
         __syncthreads();
-        pair_sums_basis<KIND, G, PAIR_GENERAL>(nq, n, i, nodes, sc, bt, nbp, g1, g2, piece_tab, k,
-                                               eta, D, M);
+        pair_sums_basis<G>(nq, n, i, nodes, sc, bt, nbp, g1, g2,
+                           RealPair<KIND, PAIR_GENERAL>(k, eta, piece_tab), D, M);
     }
 #pragma unroll
     for (int u = 0; u < G; ++u)
@@ -204,6 +271,94 @@ __global__ __launch_bounds__(64) void assemble_regular_basis_kernel(const AsmPro
                     make_double2(kscale * M[u][v][0], kscale * M[u][v][1]);
         }
 }
+
+// ---- regular pairs, complex k, tiled ---------------------------------------------------------------
+// The real-k tile kernel's shape for complex k: one lane per element pair of an 8 x 8 tile, both
+// orders of the pair from one complex Hankel evaluation, one loop for all pairs (the complex
+// routine picks its region per argument).  Dynamic LDS: the tile's trial node data, the basis
+// table, then the contour table of arg k (read by every evaluation in the middle region).
+template <int KIND, int G>
+__global__ __launch_bounds__(64) void assemble_regular_basis_ckt_kernel(
+    const AsmProblem pb, const BasisSet bs, double ki, const double* __restrict__ gtab, int nt)
+{
+    const int64_t n = pb.n;
+    const int nq = pb.nq;
+    const double* __restrict__ nodes = pb.nodes;
+    const int64_t* __restrict__ index = pb.adj;
+    const int64_t* __restrict__ next = pb.adj + n;
+    const int64_t* __restrict__ prev = pb.adj + 2 * n;
+    double2* __restrict__ A = pb.A;
+    const int64_t lda = pb.lda;
+    const int nb = bs.nb, nbp = G * bs.ngroup;
+    const int g1 = (int)(blockIdx.y / bs.ngroup) * G, g2 = (int)(blockIdx.y % bs.ngroup) * G;
+    extern __shared__ double sh[];
+    const int ns = node_stride(nq);
+    double* bt = sh + kTileSide * ns;
+    double* ctab = bt + 2 * nq * nbp;
+    const int lane = threadIdx.x;
+    const int64_t P = (n + kTileSide - 1) / kTileSide;
+    const int64_t t = blockIdx.x;
+    const double qq = (double)P + 0.5;
+    int64_t p = (int64_t)(qq - sqrt(fmax(qq * qq - 2.0 * (double)t, 0.0)));
+    p = p < 0 ? 0 : (p >= P ? P - 1 : p);
+    while (p > 0 && tiles_before(p, P) > t) --p;
+    while (p + 1 < P && tiles_before(p + 1, P) <= t) ++p;
+    const int64_t q = p + (t - tiles_before(p, P));
+    const int64_t j0 = q * kTileSide;
+    const int64_t plane = n * nq;
+    for (int u = lane; u < kTileSide * nq; u += 64) {
+        const int c = u / nq, b = u - c * nq;
+        const int64_t j = j0 + c;
+        if (j < n) {
+            const int64_t o = (int64_t)b * n + j;
+#pragma unroll
+            for (int f = 0; f < NODE_PLANES; ++f) sh[c * ns + b * NODE_PLANES + f] = nodes[f * plane + o];
+        }
+    }
+    stage_basis_table(bs, nq, pb.g1d, bt, lane, 64);
+    for (int u = lane; u < nt * kContourDoubles; u += 64) ctab[u] = gtab[u];
+    __syncthreads();
+    const int c = lane / kTileSide;
+    const int64_t i = p * kTileSide + lane % kTileSide, j = j0 + c;
+    if (i >= n || j >= n) return;
+    const int64_t idx_i = index[i], nx_i = next[i], pv_i = prev[i];
+    const int64_t idx_j = index[j], nx_j = next[j], pv_j = prev[j];
+    const bool direct = j > i && !(idx_j == idx_i || nx_i == idx_j || pv_i == idx_j);
+    const bool mirror = j > i && !(idx_i == idx_j || nx_j == idx_i || pv_j == idx_i);
+    if (!(direct || mirror)) return;
+    double D[G][G][2], M[G][G][2];
+    pair_sums_basis<G>(nq, n, i, nodes, sh + c * ns, bt, nbp, g1, g2,
+                       ComplexPair<KIND>{pb.k, ki, pb.eta, ctab, nt}, D, M);
+    const double kscale = pb.kscale;
+#pragma unroll
+    for (int u = 0; u < G; ++u)
+#pragma unroll
+        for (int v = 0; v < G; ++v) {
+            if (direct && g1 + u < nb && g2 + v < nb)
+                A[(i * nb + g1 + u) + lda * (j * nb + g2 + v)] =
+                    make_double2(kscale * D[u][v][0], kscale * D[u][v][1]);
+            if (mirror && g1 + u < nb && g2 + v < nb)
+                A[(j * nb + g1 + u) + lda * (i * nb + g2 + v)] =
+                    make_double2(kscale * M[u][v][0], kscale * M[u][v][1]);
+        }
+}
+
+// Launches the tiled complex-k regular kernel (all gp group pairs) on stream s.
+template <int KIND, int G>
+void launch_ckt(hipStream_t s, const AsmProblem& p, const BasisSet& bs, double ki,
+                const double* tab, int nt)
+{
+    const size_t shm = sizeof(double) * ((size_t)kTileSide * node_stride(p.nq) +
+                                         2 * (size_t)p.nq * G * bs.ngroup +
+                                         (size_t)kContourDoubles * nt);
+    const unsigned gp = (unsigned)(bs.ngroup * bs.ngroup);
+    assemble_regular_basis_ckt_kernel<KIND, G>
+        <<<dim3((unsigned)upper_tile_count(p.n), gp), 64, shm, s>>>(p, bs, ki, tab, nt);
+}
+
+// Launches of fewer workgroups than this take the one-wavefront-per-entry complex-k kernel, whose
+// grid of n^2 entries fills the device when the tiles cannot.
+constexpr int64_t kCktMinTiles = 2048;
 
 // ---- regular pairs, complex k --------------------------------------------------------------------
 __device__ __forceinline__ double wave_sum(double v)
@@ -556,6 +711,8 @@ int launch_basis(bem2d_ctx* c, const AsmProblem& p, const BasisSet& s, double ki
         assemble_regular_basis_kernel<KIND, G>
             <<<dim3((unsigned)upper_tile_count(p.n), gp), 64, shm, st>>>(p, s);
         bem2d_internal_prof_end(c, BEM2D_PROF_ASSEMBLE_REGULAR, st);
+    } else if (upper_tile_count(p.n) * gp >= kCktMinTiles) {
+        launch_ckt<KIND, G>(st, p, s, ki, tab, nt);
     } else {
         assemble_regular_basis_ck_kernel<KIND, G>
             <<<dim3((unsigned)(p.n * p.n), gp), 64, btab, st>>>(p, s, ki, tab, nt);
@@ -655,6 +812,26 @@ int bem2d_dev::bem2d_internal_make_basis_set(const bem2d_basis* bases, int nb, B
     }
     *out = s;
     return BEM2D_OK;
+}
+
+// The regular pairs of a P0 complex-k assembly through the tiled kernel (one ConstBasis), when the
+// mesh gives enough tiles; returns false (nothing launched) otherwise.
+bool bem2d_dev::bem2d_internal_regular_ck_tiles(hipStream_t st, const AsmProblem& p, int kind,
+                                                double ki, const double* tab, int nt)
+{
+    if (upper_tile_count(p.n) < kCktMinTiles) return false;
+    const bem2d_basis c0{BEM2D_BASIS_CONST, 0, 0.0, 0.0, 0.0};
+    BasisSet bs;
+    if (bem2d_internal_make_basis_set(&c0, 1, &bs)) return false;
+    switch (kind) {
+    case BEM2D_KERNEL_SL: launch_ckt<BEM2D_KERNEL_SL, 1>(st, p, bs, ki, tab, nt); return true;
+    case BEM2D_KERNEL_DL: launch_ckt<BEM2D_KERNEL_DL, 1>(st, p, bs, ki, tab, nt); return true;
+    case BEM2D_KERNEL_CDL: launch_ckt<BEM2D_KERNEL_CDL, 1>(st, p, bs, ki, tab, nt); return true;
+    case BEM2D_KERNEL_COMBINED:
+        launch_ckt<BEM2D_KERNEL_COMBINED, 1>(st, p, bs, ki, tab, nt);
+        return true;
+    default: return false;
+    }
 }
 
 extern "C" int bem2d_assemble_basis(bem2d_ctx* c, const bem2d_mesh* m, const bem2d_basis* bases,

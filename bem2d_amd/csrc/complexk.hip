@@ -184,7 +184,9 @@ int launch_ck(bem2d_ctx* c, const bem2d_mesh* m, double kr, double ki, double et
     if (m->nspecial > 0)
         assemble_special_ck_kernel<KIND><<<(unsigned)m->nspecial, 256, 0, c->stream>>>(sp, ki, tab,
                                                                                       nt);
-    assemble_regular_ck_kernel<KIND><<<(unsigned)(m->n * m->n), 64, 0, c->stream>>>(p, ki, tab, nt);
+    if (!bem2d_internal_regular_ck_tiles(c->stream, p, KIND, ki, tab, nt))
+        assemble_regular_ck_kernel<KIND><<<(unsigned)(m->n * m->n), 64, 0, c->stream>>>(p, ki, tab,
+                                                                                      nt);
     HIPCHK(hipGetLastError());
     return BEM2D_OK;
 }

@@ -43,6 +43,32 @@ __global__ void numrange_hermitian_kernel(int64_t n, const double2* __restrict__
     H[(int64_t)b * n * n + i + n * j] = make_double2(re, im);
 }
 
+// s[0 .. n) into decreasing order in place: odd-even transposition passes by one workgroup until a
+// pass swaps nothing (the divide-and-conquer SVD leaves near-equal values out of order by rounding,
+// so one or two passes do; n passes at most).
+__global__ __launch_bounds__(1024) void sort_desc_kernel(double* __restrict__ s, int64_t n)
+{
+    __shared__ int swapped;
+    for (int64_t pass = 0; pass < n + 2; ++pass) {
+        if (threadIdx.x == 0) swapped = 0;
+        __syncthreads();
+        for (int parity = 0; parity < 2; ++parity) {
+            for (int64_t i = 2 * (int64_t)threadIdx.x + parity; i + 1 < n; i += 2 * blockDim.x) {
+                const double a = s[i], b = s[i + 1];
+                if (a < b) {
+                    s[i] = b;
+                    s[i + 1] = a;
+                    swapped = 1;
+                }
+            }
+            __syncthreads();
+        }
+        const int any = swapped;
+        __syncthreads();
+        if (!any) break;
+    }
+}
+
 int check_args(bem2d_ctx* c, int64_t n, const void* A, int64_t lda, const char* who)
 {
     if (!c || !A || n < 1 || lda < n || n > 2000000000 || lda > 2000000000)
@@ -108,16 +134,43 @@ extern "C" int bem2d_singular_values(bem2d_ctx* c, int64_t n, const void* A, int
     spectral_copy_kernel<<<dim3((unsigned)((n + 255) / 256), (unsigned)n), 256, 0, c->stream>>>(
         n, (const double2*)A, lda, W);
     HIPCHK(hipGetLastError());
-    const rocblas_status st = rocsolver_zgesvd(c->blas, rocblas_svect_none, rocblas_svect_none,
-                                               (rocblas_int)n, (rocblas_int)n,
-                                               (rocblas_double_complex*)W, (rocblas_int)n, s,
-                                               nullptr, (rocblas_int)n, nullptr, (rocblas_int)n, E,
-                                               rocblas_outofplace, info);
-    if (st != rocblas_status_success) return FAIL(BEM2D_ERR_DEVICE, "rocsolver_zgesvd status %d", (int)st);
+    const rocblas_int N = (rocblas_int)n;
+    rocblas_double_complex* Wz = (rocblas_double_complex*)W;
+    // Divide and conquer (zgesdd) first; the bidiagonal QR iteration of zgesvd, then one-sided
+    // Jacobi (zgesvdj), on a fresh copy if it does not converge (strongly graded operators can
+    // stall the QR iteration).  All three return the singular values in decreasing order.
+    rocblas_status st = rocsolver_zgesdd(c->blas, rocblas_svect_none, rocblas_svect_none, N, N, Wz,
+                                         N, s, nullptr, N, nullptr, N, info);
+    if (st != rocblas_status_success) return FAIL(BEM2D_ERR_DEVICE, "rocsolver_zgesdd status %d", (int)st);
     int h_info = 0;
     if (int rc = read_info(c, info, &h_info)) return rc;
+    if (h_info == 0) {
+        sort_desc_kernel<<<1, 1024, 0, c->stream>>>(s, n);
+        HIPCHK(hipGetLastError());
+        return BEM2D_OK;
+    }
+    const int first_info = h_info;
+    spectral_copy_kernel<<<dim3((unsigned)((n + 255) / 256), (unsigned)n), 256, 0, c->stream>>>(
+        n, (const double2*)A, lda, W);
+    st = rocsolver_zgesvd(c->blas, rocblas_svect_none, rocblas_svect_none, N, N, Wz, N, s, nullptr,
+                          N, nullptr, N, E, rocblas_outofplace, info);
+    if (st != rocblas_status_success) return FAIL(BEM2D_ERR_DEVICE, "rocsolver_zgesvd status %d", (int)st);
+    if (int rc = read_info(c, info, &h_info)) return rc;
+    if (h_info == 0) return BEM2D_OK;
+    spectral_copy_kernel<<<dim3((unsigned)((n + 255) / 256), (unsigned)n), 256, 0, c->stream>>>(
+        n, (const double2*)A, lda, W);
+    double* residual = (double*)(small + 8);
+    int* sweeps = (int*)(small + 16);
+    st = rocsolver_zgesvdj(c->blas, rocblas_svect_none, rocblas_svect_none, N, N, Wz, N, 0.0,
+                           residual, 100, sweeps, s, nullptr, N, nullptr, N, info);
+    if (st != rocblas_status_success) return FAIL(BEM2D_ERR_DEVICE, "rocsolver_zgesvdj status %d", (int)st);
+    if (int rc = read_info(c, info, &h_info)) return rc;
     if (h_info != 0)
-        return FAIL(BEM2D_ERR_SINGULAR, "bem2d_singular_values: SVD did not converge (%d)", h_info);
+        return FAIL(BEM2D_ERR_SINGULAR,
+                    "bem2d_singular_values: SVD did not converge (zgesdd %d, zgesvd and zgesvdj %d)",
+                    first_info, h_info);
+    sort_desc_kernel<<<1, 1024, 0, c->stream>>>(s, n);
+    HIPCHK(hipGetLastError());
     return BEM2D_OK;
 }
 

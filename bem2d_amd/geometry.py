@@ -14,6 +14,8 @@ mirror the reference's element constructions:
   disk_chords        DiskShapePiecewiseConst(n, r)        lib/bem2d_shape.cpp:11-34
   polygon            Polygon(points, n)                   lib/bem2d_shape.cpp:114-138
   line               Line(p0, p1, n) with open ends        lib/bem2d_shape.cpp:225-236
+  polygon_ppw        Polygon(points, ppw, k, L, sigma)     lib/bem2d_shape.cpp:36-111
+  line_ppw           Line(p0, p1, ppw, k, L, sigma)        lib/bem2d_shape.cpp:147-222
 """
 import math
 
@@ -152,6 +154,67 @@ def polygon(points, per_side):
                   pts[s][1] + (1.0 * (j + 1)) / per_side * dy)
             param[s * per_side + j, :4] = (p1[0], p1[1], p2[0] - p1[0], p2[1] - p1[1])
     return Geometry(np.full(m * per_side, ELEM_FLAT), param, *_ring(m * per_side))
+
+
+def _graded_side(params, p, q, n, L, sigma):
+    """Pieces of the side p -> q: n of |q - p| / n, the first and the last split into L + 1 pieces
+    of ratio sigma (as the ppw-graded Polygon and Line place them)."""
+    p = np.asarray(p, dtype=np.float64)
+    q = np.asarray(q, dtype=np.float64)
+    d = q - p
+
+    def add(a, b):
+        params.append([a[0], a[1], b[0] - a[0], b[1] - a[1], 0, 0, 0, 0])
+
+    if L == 0:
+        add(p, p + (1.0 / n) * d)
+    else:
+        ts = 1.0
+        for _ in range(L):
+            ts = ts * sigma
+        add(p, p + (ts / n) * d)
+        for _ in range(L):
+            a = p + ts / n * d
+            ts = ts / sigma
+            add(a, p + ts / n * d)
+    for j in range(1, n - 1):
+        add(p + (1.0 * j) / n * d, p + (1.0 * (j + 1)) / n * d)
+    if L == 0:
+        add(p + (1.0 * (n - 1)) / n * d, q)
+    else:
+        ts = 1.0
+        for _ in range(L):
+            a = p + (1.0 * (n - 1)) / n * d + (1 - ts) / n * d
+            ts *= sigma
+            add(a, p + (1.0 * (n - 1)) / n * d + (1 - ts) / n * d)
+        add(p + (1.0 * (n - 1)) / n * d + (1 - ts) / n * d, q)
+
+
+def polygon_ppw(points, ppw, k, L=0, sigma=0.15):
+    """Polygon(points, ppw, k, L, sigma): max(150, ceil(ppw Re k |side| / 2 pi)) pieces per side,
+    graded into the corners; closed ring."""
+    pts = [np.asarray(p, dtype=np.float64) for p in points]
+    params = []
+    for i in range(len(pts)):
+        a, b = pts[i], pts[(i + 1) % len(pts)]
+        n = max(150, int(math.ceil(ppw * float(complex(k).real) * math.hypot(*(b - a)) / 2.0 /
+                                   math.pi)))
+        _graded_side(params, a, b, n, L, sigma)
+    m = len(params)
+    return Geometry(np.full(m, ELEM_FLAT), np.array(params), *_ring(m))
+
+
+def line_ppw(p0, p1, ppw, k, L=0, sigma=0.15):
+    """Line(p0, p1, ppw, k, L, sigma): max(10, ceil(ppw Re k |p1 - p0| / 2 pi)) pieces graded
+    towards both ends; the reference links them into a closed ring."""
+    p0 = np.asarray(p0, dtype=np.float64)
+    p1 = np.asarray(p1, dtype=np.float64)
+    n = max(10, int(math.ceil(ppw * float(complex(k).real) * math.hypot(*(p1 - p0)) / 2.0 /
+                              math.pi)))
+    params = []
+    _graded_side(params, p0, p1, n, L, sigma)
+    m = len(params)
+    return Geometry(np.full(m, ELEM_FLAT), np.array(params), *_ring(m))
 
 
 def line(p0, p1, n):

@@ -616,6 +616,96 @@ def run_disk(ctx, steps, warmup, k=1.0, ppw=2000, L=7, N=30, cpu=True):
     return res
 
 
+TRAPPING = [(0.0, 0.0), (-1.0, 0.0), (-1.0, -0.69), (0.69, -0.69), (0.69, 1.31), (-1.0, 1.31),
+            (-1.0, 0.62), (0.0, 0.62)]  # resonances/trapresonance.cpp:21-35, a = 0.31, c = 1
+
+
+def resonance_grid():
+    """The reference's resonance grid: kr in [4, 8), ki in [-2, 0), 10 x 10 points."""
+    return [complex(4 + i * 0.4, -2 + j * 0.2) for i in range(10) for j in range(10)]
+
+
+def run_resonance(ctx, npts=4, nasm=10, cpu=True):
+    """resonances/trapresonance.cpp:44-80: the trapping polygon as Polygon(points, ppw=10,
+    kmax=8, L=10, sigma=0.15) (1360 elements), PolBasis(2) (4080 unknowns), N=5, L=3; per complex
+    wavenumber k of the 10 x 10 grid: Id + 2 CDL - 2 i |k| SL assembled in one pass (complex-k
+    kernels with bases), ChangeBasis with the mass matrix, SingularValues, the smallest singular
+    value.  Times: the whole pipeline on npts grid points, the assembly alone on nasm."""
+    import torch
+    from bem2d_amd import capi
+    from bem2d_amd import geometry as G
+    geom = G.polygon_ppw(TRAPPING, 10, 8.0, 10, 0.15)
+    bases = capi.pol_bases(2)
+    nn = geom.n * len(bases)
+    mesh = ctx.mesh(geom, 5, 3, 0.15)
+    grid = resonance_grid()
+    M = ctx.assemble_basis(mesh, bases, capi.KERNEL_NONE, 1.0, 0.0, 1.0, 0.0)
+    A = torch.empty((nn, nn), dtype=torch.complex128, device="cuda").t()
+    ctx.profile(True)
+    ctx.profile_reset()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in grid[:nasm]:
+        ctx.assemble_basis(mesh, bases, capi.KERNEL_COMBINED, k, abs(k), 1.0, 2.0, out=A)
+    torch.cuda.synchronize()
+    t_asm = (time.perf_counter() - t0) / nasm
+    progress("resonance: %d assemblies of %d unknowns, %.1f ms each" % (nasm, nn, 1e3 * t_asm))
+    reg_ms, reg_n = ctx.profile_read(capi.PROF_ASSEMBLE_REGULAR)
+    sp_ms, sp_n = ctx.profile_read(capi.PROF_ASSEMBLE_SPECIAL)
+    ctx.profile(False)
+    smin = []
+    t0 = time.perf_counter()
+    t_cb = t_svd = 0.0
+    for k in grid[:npts]:
+        ctx.assemble_basis(mesh, bases, capi.KERNEL_COMBINED, k, abs(k), 1.0, 2.0, out=A)
+        torch.cuda.synchronize()
+        ta = time.perf_counter()
+        ctx.change_basis(A, M)
+        torch.cuda.synchronize()
+        tb = time.perf_counter()
+        progress("resonance: k = %s change of basis %.1f ms; singular values..." %
+                 (k, 1e3 * (tb - ta)))
+        s = ctx.singular_values(A)
+        smin.append(float(s[-1].item()))
+        tc = time.perf_counter()
+        progress("resonance: singular values %.1f s" % (tc - tb))
+        t_cb += tb - ta
+        t_svd += tc - tb
+    t_all = (time.perf_counter() - t0) / npts
+    res = {
+        "workload": "resonances/trapresonance.cpp: trapping polygon, Polygon(ppw=10, kmax=8, L=10, "
+                    "sigma=0.15) = %d elements, PolBasis(2) = %d unknowns, N=5, L=3; complex k on "
+                    "the 10 x 10 grid kr in [4, 8), ki in [-2, 0); per k: Id + 2 CDL - 2i|k| SL, "
+                    "ChangeBasis, SingularValues" % (geom.n, nn),
+        "unknowns": nn,
+        "assembly_ms": 1e3 * t_asm,
+        "assembly_regular_ms": reg_ms / max(reg_n, 1),
+        "assembly_special_ms": sp_ms / max(sp_n, 1),
+        "assembly_matrices_per_s": 1.0 / t_asm,
+        "change_basis_ms": 1e3 * t_cb / npts,
+        "svd_ms": 1e3 * t_svd / npts,
+        "pipeline_points_per_s": 1.0 / t_all,
+        "grid_points_timed": npts,
+        "smallest_singular_values": smin,
+        "k": [[k.real, k.imag] for k in grid[:npts]],
+    }
+    if cpu:
+        from oracle import oracle as O
+        cols = np.linspace(0, nn - 1, 4).astype(np.int64)
+        t0 = time.perf_counter()
+        O.assemble_basis(geom, bases, 5, 3, 0.15, O.K_COMBINED, grid[0], abs(grid[0]), 1.0, 2.0,
+                         cols=cols)
+        tc = time.perf_counter() - t0
+        res["cpu_assembly_estimate_s"] = tc * nn / cols.size
+        res["cpu_sample"] = "oracle assemble_basis (complex k) of 4 of the %d columns in %.2f s on " \
+            "%d threads, scaled to the whole matrix" % (nn, tc, os.cpu_count() or 1)
+    mesh.close()
+    del A, M
+    ctx.release_scratch()
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -624,7 +714,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=3,
                     help="timed steps of the plain FP64 path on the large configuration")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--config", choices=["all", "large", "kite", "batched", "serving", "disk"],
+    ap.add_argument("--config", choices=["all", "large", "kite", "batched", "serving", "disk",
+                                         "resonance"],
                     default="all")
     ap.add_argument("--no-extras", action="store_true",
                     help="the large configuration only (no kite, batched and served legs)")
@@ -657,13 +748,16 @@ def main():
         return
     ctx = Context(0)
     cpu = not args.no_cpu_baseline
-    if args.config in ("kite", "batched", "serving", "disk"):
+    if args.config in ("kite", "batched", "serving", "disk", "resonance"):
         if args.config == "kite":
             res = run_kite(ctx, args.steps, args.warmup)
             value, t = res["solves_per_s_mixed"], res["ms_per_step_mixed"]
         elif args.config == "disk":
             res = run_disk(ctx, min(args.steps, 5), min(args.warmup, 1), cpu=cpu)
             value, t = res["examples_per_s"], res["ms_per_step"]
+        elif args.config == "resonance":
+            res = run_resonance(ctx, npts=1, cpu=cpu)
+            value, t = res["pipeline_points_per_s"], None
         elif args.config == "batched":
             res = run_batched(ctx)
             value, t = res["mixed"]["solves_per_s"], None
@@ -710,6 +804,8 @@ def main():
         progress("C3 batch done")
         out["disk_lin"] = run_disk(ctx, 3, 1, cpu=cpu)
         progress("disk.cpp (LinBasis) leg done")
+        out["resonance"] = run_resonance(ctx, npts=1, nasm=4, cpu=cpu)
+        progress("resonance (complex k, PolBasis(2)) leg done")
         sv = run_serving(ctx, args.serving_count, args.serving_rate)
         out["serving"] = sv
         out["p50_ms"] = sv["mixed"]["p50_ms"]

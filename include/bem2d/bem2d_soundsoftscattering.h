@@ -41,10 +41,12 @@ public:
         quadopts_.N = N;
         quadopts_.sigma = sigma;
     }
+    // The mass matrix the reference keeps beside A (pId_) is folded into A's assembly; GetIdent
+    // assembles it on first use.
     virtual void DiscretizeMatrix()
     {
         pA_ = device::AssembleCombined(*pgeom_, quadopts_, k_, frhs_.eta(), 1.0, 2.0);
-        pId_ = EvalIdent(*pgeom_, quadopts_);
+        pId_.reset();
     }
     virtual void DiscretizeRhs()
     {
@@ -59,7 +61,11 @@ public:
     void Solve() { psol_ = SolveSystem(*pA_, *prhs_); }
 
     inline pMatrix GetMatrix() const { return pA_; }
-    inline pMatrix GetIdent() const { return pId_; }
+    inline pMatrix GetIdent()
+    {
+        if (!pId_ && pA_) pId_ = EvalIdent(*pgeom_, quadopts_);
+        return pId_;
+    }
     inline pMatrix GetSolution() const { return psol_; }
 
     // Points inside the polygons given by set_polygons get NaN once set_plotInterior was called

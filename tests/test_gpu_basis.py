@@ -199,3 +199,65 @@ def test_disk_driver_matches_oracle(oracle, ctx, basis, ppw, k, L, N, curve, gra
     if curve == "circle":
         ser = analytic_farfield(k, th)
         assert np.abs(d["farfield"] - ser).max() / np.abs(ser).max() < 1e-4
+
+
+@pytest.mark.parametrize("kind", [1, 2, 3, 4])
+def test_complex_k_tiled_kernel(ctx, oracle, kind):
+    """Meshes with enough 8 x 8 tiles take the tiled complex-k kernel (both orders of a pair from
+    one complex Hankel evaluation): P0 and LinBasis against sampled oracle columns, including the
+    columns next to the diagonal (singular entries) and both ends of an open arc."""
+    k = 9.0 - 1.1j
+    for geom, bases in ((G.kite(720), None), (G.open_arc(600, 0.4), LIN)):
+        mesh = ctx.mesh(geom, 6, 3, 0.15)
+        nn = geom.n * (1 if bases is None else len(bases))
+        cols = np.array([0, 1, 2, nn // 3, nn // 2 + 1, nn - 2, nn - 1])
+        if bases is None:
+            A = _np(ctx.assemble(mesh, kind, k, 9.0, 1.0, 2.0))
+            ref = oracle.assemble(geom, 6, 3, 0.15, kind, k, 9.0, 1.0, 2.0, cols=cols)
+        else:
+            A = _np(ctx.assemble_basis(mesh, bases, kind, k, 9.0, 1.0, 2.0))
+            ref = oracle.assemble_basis(geom, bases, 6, 3, 0.15, kind, k, 9.0, 1.0, 2.0,
+                                        cols=cols)
+        err = rel_fro(A[:, cols], ref)
+        print("tiled complex k kind %d n %d nb %s: %.2e" % (kind, geom.n, bases and len(bases),
+                                                          err), flush=True)
+        assert err < 1e-12
+        # the whole matrix against the per-entry kernel's entries through a real-k identity:
+        # k = 9 - 1.1i is no special case, so check symmetry of SL (Gauss2D is symmetric)
+        if kind == 1 and bases is None:
+            n = geom.n
+            d = np.abs(np.subtract.outer(np.arange(n), np.arange(n)))
+            reg = (d > 1) & (d < n - 1)
+            S = A - np.diag(np.diag(A))  # mass on the diagonal
+            assert np.abs((S - S.T)[reg]).max() <= 1e-14 * np.abs(S).max()
+
+
+def test_resonance_pipeline(ctx, oracle):
+    """resonances/trapresonance.cpp on a graded triangle (Polygon(points, ppw, kmax, L, sigma),
+    PolBasis(2)): Id + 2 CDL - 2i|k| SL with complex k, ChangeBasis, SingularValues.  The mass
+    matrix against the oracle in full, the operator on sampled columns (the oracle's complex-k
+    matrix of this size takes minutes), and the smallest singular value against numpy's SVD of the
+    same operator in the mass matrix's orthonormal basis."""
+    tri = [(0.0, 0.0), (1.0, 0.1), (0.3, 0.8)]
+    geom = G.polygon_ppw(tri, 10, 8.0, 2, 0.15)
+    assert geom.n == 3 * (150 + 4)
+    bases = capi.pol_bases(2)
+    nn = geom.n * 3
+    mesh = ctx.mesh(geom, 5, 3, 0.15)
+    M = ctx.assemble_basis(mesh, bases, 0, 1.0, 0.0, 1.0, 0.0)
+    Mref = oracle.assemble_basis(geom, bases, 5, 3, 0.15, oracle.K_NONE, 1.0, 0.0, 1.0, 0.0)
+    assert rel_fro(_np(M), Mref) < 1e-13
+    cols = np.array([0, 1, 2, 3, 4, 5, 460, 700, nn - 3, nn - 2, nn - 1])
+    for k in (5.2 - 0.6j, 7.6 - 1.8j):
+        A = ctx.assemble_basis(mesh, bases, 4, k, abs(k), 1.0, 2.0)
+        An = _np(A)
+        ref = oracle.assemble_basis(geom, bases, 5, 3, 0.15, oracle.K_COMBINED, k, abs(k), 1.0,
+                                    2.0, cols=cols)
+        assert rel_fro(An[:, cols], ref) < 1e-12
+        s = _np(ctx.singular_values(ctx.change_basis(A, M)))
+        L = np.linalg.cholesky(Mref)
+        KB = np.linalg.solve(L, np.linalg.solve(L, An.conj().T).conj().T)
+        sref = np.linalg.svd(KB, compute_uv=False)
+        print("k %s: smallest singular value %.12f (numpy %.12f)" % (k, s[-1], sref[-1]),
+              flush=True)
+        assert abs(s[-1] - sref[-1]) <= 1e-10 * sref[0]
