@@ -711,11 +711,14 @@ int launch_basis(bem2d_ctx* c, const AsmProblem& p, const BasisSet& s, double ki
         assemble_regular_basis_kernel<KIND, G>
             <<<dim3((unsigned)upper_tile_count(p.n), gp), 64, shm, st>>>(p, s);
         bem2d_internal_prof_end(c, BEM2D_PROF_ASSEMBLE_REGULAR, st);
-    } else if (upper_tile_count(p.n) * gp >= kCktMinTiles) {
-        launch_ckt<KIND, G>(st, p, s, ki, tab, nt);
     } else {
-        assemble_regular_basis_ck_kernel<KIND, G>
-            <<<dim3((unsigned)(p.n * p.n), gp), 64, btab, st>>>(p, s, ki, tab, nt);
+        bem2d_internal_prof_begin(c, BEM2D_PROF_ASSEMBLE_REGULAR, st);
+        if (upper_tile_count(p.n) * gp >= kCktMinTiles)
+            launch_ckt<KIND, G>(st, p, s, ki, tab, nt);
+        else
+            assemble_regular_basis_ck_kernel<KIND, G>
+                <<<dim3((unsigned)(p.n * p.n), gp), 64, btab, st>>>(p, s, ki, tab, nt);
+        bem2d_internal_prof_end(c, BEM2D_PROF_ASSEMBLE_REGULAR, st);
     }
     if (fork && (hipEventRecord(c->ev_join, c->aux) != hipSuccess ||
                  hipStreamWaitEvent(st, c->ev_join, 0) != hipSuccess))
@@ -945,14 +948,8 @@ extern "C" int bem2d_farfield_basis(bem2d_ctx* c, const bem2d_mesh* m, const bem
     const size_t btab = basis_table_bytes(m, s);
     farfield_signature_basis_kernel<<<(unsigned)((tot + 255) / 256), 256, btab, c->stream>>>(
         m->n, m->nq, m->d_nodes, m->d_g1d, s, k, theta, nang, F);
-    const double pi = 3.14159265358979323846;
-    const double mag = -1.0 / std::sqrt(8.0 * pi * k);
-    dim3 grid((unsigned)nang, (unsigned)nrhs);
-    farfield_combine_kernel<<<grid, 256, 0, c->stream>>>(nn, F, nang, (const double2*)coef, ldc,
-                                                         mag * std::cos(pi / 4),
-                                                         mag * std::sin(pi / 4), (double2*)out);
     HIPCHK(hipGetLastError());
-    return BEM2D_OK;
+    return bem2d_internal_farfield_gemm(c, nn, k, F, nang, coef, ldc, nrhs, out);
 }
 
 extern "C" int bem2d_basis_values(const bem2d_basis* bases, int nb, const double* t, int64_t count,

@@ -951,7 +951,35 @@ extern "C" int bem2d_farfield(bem2d_ctx* c, const bem2d_mesh* m, double k, const
     HIPCHK(hipSetDevice(c->device));
     double2* F = (double2*)bem2d_internal_scratch(c, SCRATCH_FARFIELD, sizeof(double2) * nang * m->n);
     if (!F) return FAIL(BEM2D_ERR_NOMEM, "scratch allocation failed");
-    return bem2d_internal_farfield(m, k, coef, ldc, nrhs, theta, nang, out, F, c->stream);
+    const int64_t tot = nang * m->n;
+    if (tot > (int64_t)UINT32_MAX - 256)
+        return FAIL(BEM2D_ERR_UNSUPPORTED, "far field: %lld angles x %lld elements exceed one launch",
+                    (long long)nang, (long long)m->n);
+    farfield_signature_kernel<<<(unsigned)((tot + 255) / 256), 256, 0, c->stream>>>(
+        m->n, m->nq, m->d_nodes, k, theta, nang, (double2*)F);
+    HIPCHK(hipGetLastError());
+    return bem2d_internal_farfield_gemm(c, m->n, k, F, nang, coef, ldc, nrhs, out);
+}
+
+// out (nang x nrhs, ld nang) = -(exp(i pi/4) / sqrt(8 pi k)) F coef: the signature matrix F
+// (nang x ncols, ld nang) times the coefficients (ncols x nrhs, ld ldc) on rocBLAS zgemm, on the
+// context's stream.
+int bem2d_internal_farfield_gemm(bem2d_ctx* c, int64_t ncols, double k, const void* F, int64_t nang,
+                                 const void* coef, int64_t ldc, int nrhs, void* out)
+{
+    if (ncols > INT32_MAX || nang > INT32_MAX || ldc > INT32_MAX)
+        return FAIL(BEM2D_ERR_UNSUPPORTED, "far field: sizes beyond rocBLAS's int range");
+    const double pi = 3.14159265358979323846;
+    const double mag = -1.0 / std::sqrt(8.0 * pi * k);
+    const rocblas_double_complex alpha(mag * std::cos(pi / 4), mag * std::sin(pi / 4));
+    const rocblas_double_complex beta(0.0, 0.0);
+    const rocblas_status st = rocblas_zgemm(
+        c->blas, rocblas_operation_none, rocblas_operation_none, (rocblas_int)nang,
+        (rocblas_int)nrhs, (rocblas_int)ncols, &alpha, (const rocblas_double_complex*)F,
+        (rocblas_int)nang, (const rocblas_double_complex*)coef, (rocblas_int)ldc, &beta,
+        (rocblas_double_complex*)out, (rocblas_int)nang);
+    return st == rocblas_status_success ? BEM2D_OK
+                                        : FAIL(BEM2D_ERR_DEVICE, "far field zgemm status %d", (int)st);
 }
 
 // ------------------------------------------------------------------------------------------------

@@ -112,6 +112,10 @@ int bem2d_internal_upload_contour(bem2d_ctx* c, double kr, double ki, const doub
 namespace bem2d_dev {
 struct BasisSet;
 }
+// out = -(exp(i pi/4) / sqrt(8 pi k)) F coef on rocBLAS zgemm (F: nang x ncols far-field
+// signatures, ld nang), on c->stream.
+int bem2d_internal_farfield_gemm(bem2d_ctx* c, int64_t ncols, double k, const void* F, int64_t nang,
+                                 const void* coef, int64_t ldc, int nrhs, void* out);
 // Single-layer potential of the density sum_b coef[j nb + b] phi_b at npts device points (real
 // k_im == 0, or complex k), on c->stream.
 int bem2d_internal_potential(bem2d_ctx* c, const bem2d_mesh* m, const bem2d_dev::BasisSet& bs,
