@@ -202,6 +202,7 @@ extern "C" void bem2d_ctx_destroy(bem2d_ctx* c)
     if (c->stage) (void)hipHostFree(c->stage);
     if (c->ev_stage) (void)hipEventDestroy(c->ev_stage);
     bem2d_internal_mixed_release(c);
+    bem2d_internal_rulesets_release(c);
     rocblas_destroy_handle(c->blas);
     (void)hipStreamSynchronize(c->aux);
     (void)hipStreamDestroy(c->aux);
@@ -557,13 +558,137 @@ extern "C" int bem2d_mesh_create(bem2d_ctx* c, const bem2d_geometry* g, bem2d_qu
                                  bem2d_mesh** out)
 {
     if (!c) return FAIL(BEM2D_ERR_ARG, "bem2d_mesh_create: null argument");
-    return bem2d_internal_mesh_create(c, g, q, nullptr, c->stream, out);
+    return bem2d_internal_mesh_create(c, g, q, nullptr, c->stream, nullptr, out);
 }
 
-// Mesh whose device arrays come from arena (null: hipMalloc), built on stream s (synchronised
-// before returning: the host staging vectors are local).
+namespace {
+
+// The context's rule tables of option q, built and uploaded at the first mesh of that option.
+const RuleSet* ruleset(bem2d_ctx* c, bem2d_quad_option q)
+{
+    std::lock_guard<std::mutex> lock(c->rules_mu);
+    for (const RuleSet* r : c->rulesets)
+        if (r->N == q.N && r->L == q.L && r->sigma == q.sigma) return r;
+    std::vector<double> gx, gw;
+    bem2d_internal_gauss01(q.N, gx, gw);
+    Rule r1 = adapted1(gx, gw, q.L, q.sigma);
+    Rule r2 = adapted23(gx, gw, q.L, q.sigma, false);
+    Rule r3 = adapted23(gx, gw, q.L, q.sigma, true);
+    std::vector<double> rules;
+    for (const Rule* r : {&r1, &r2, &r3}) {
+        rules.insert(rules.end(), r->x.begin(), r->x.end());
+        rules.insert(rules.end(), r->y.begin(), r->y.end());
+        rules.insert(rules.end(), r->w.begin(), r->w.end());
+    }
+    for (const Rule* r : {&r1, &r2, &r3}) append_abscissa_block(rules, *r);
+    std::vector<double> g1d(gx);
+    g1d.insert(g1d.end(), gw.begin(), gw.end());
+    RuleSet* rs = new RuleSet();
+    rs->N = q.N;
+    rs->L = q.L;
+    rs->sigma = q.sigma;
+    rs->m1 = (int)r1.w.size();
+    rs->m2 = (int)r2.w.size();
+    rs->m3 = (int)r3.w.size();
+    double* dr = nullptr;
+    double* dg = nullptr;
+    if (hipMalloc(&dr, sizeof(double) * (rules.size() + 1)) != hipSuccess ||
+        hipMalloc(&dg, sizeof(double) * g1d.size()) != hipSuccess ||
+        hipMemcpy(dr, rules.data(), sizeof(double) * rules.size(), hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        hipMemcpy(dg, g1d.data(), sizeof(double) * g1d.size(), hipMemcpyHostToDevice) !=
+            hipSuccess) {
+        (void)hipFree(dr);
+        (void)hipFree(dg);
+        delete rs;
+        return nullptr;
+    }
+    rs->d_rules = dr;
+    rs->d_g1d = dg;
+    c->rulesets.push_back(rs);
+    return rs;
+}
+
+// Special (coincident / adjacent) entries with the precedence of lib/bem2d_quadrature.h:271-287:
+// identical index -> rule 1; else next(i) == index(j) -> rule 3; else prev(i) == index(j) -> 2.
+// Elements are looked up by index value through a bucket array over the range of the indices when
+// that range is at most a few times n (the usual numbering), else through a hash map.
+void special_pairs(const bem2d_geometry* g, std::vector<int64_t>& special)
+{
+    const int64_t n = g->n;
+    int64_t lo = g->index[0], hi = g->index[0];
+    for (int64_t e = 1; e < n; ++e) {
+        lo = std::min(lo, g->index[e]);
+        hi = std::max(hi, g->index[e]);
+    }
+    const bool dense = hi - lo < 4 * n + 64;
+    std::vector<int64_t> start, members;
+    std::unordered_map<int64_t, std::vector<int64_t>> pos;
+    if (dense) {
+        const int64_t range = hi - lo + 1;
+        start.assign((size_t)range + 1, 0);
+        for (int64_t e = 0; e < n; ++e) ++start[(size_t)(g->index[e] - lo) + 1];
+        for (int64_t v = 0; v < range; ++v) start[(size_t)v + 1] += start[(size_t)v];
+        members.resize((size_t)n);
+        std::vector<int64_t> fill(start.begin(), start.end() - 1);
+        for (int64_t e = 0; e < n; ++e) members[(size_t)fill[(size_t)(g->index[e] - lo)]++] = e;
+    } else {
+        pos.reserve((size_t)n * 2);
+        for (int64_t e = 0; e < n; ++e) pos[g->index[e]].push_back(e);
+    }
+    special.clear();
+    special.reserve((size_t)n * 9);
+    std::vector<int64_t> seen;
+    for (int64_t i = 0; i < n; ++i) {
+        seen.clear();
+        auto add = [&](int64_t idx, int rule) {
+            const int64_t* first = nullptr;
+            const int64_t* last = nullptr;
+            if (dense) {
+                if (idx < lo || idx > hi) return;
+                first = members.data() + start[(size_t)(idx - lo)];
+                last = members.data() + start[(size_t)(idx - lo) + 1];
+            } else {
+                auto it = pos.find(idx);
+                if (it == pos.end()) return;
+                first = it->second.data();
+                last = first + it->second.size();
+            }
+            for (const int64_t* p = first; p != last; ++p) {
+                const int64_t j = *p;
+                bool dup = false;
+                for (int64_t s : seen) dup |= (s == j);
+                if (dup) continue;
+                seen.push_back(j);
+                special.push_back(i);
+                special.push_back(j);
+                special.push_back(rule);
+            }
+        };
+        add(g->index[i], 1);
+        add(g->next[i], 3);
+        add(g->prev[i], 2);
+    }
+}
+
+size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+} // namespace
+
+void bem2d_internal_rulesets_release(bem2d_ctx* c)
+{
+    std::lock_guard<std::mutex> lock(c->rules_mu);
+    for (RuleSet* r : c->rulesets) {
+        (void)hipFree(r->d_rules);
+        (void)hipFree(r->d_g1d);
+        delete r;
+    }
+    c->rulesets.clear();
+}
+
 int bem2d_internal_mesh_create(bem2d_ctx* c, const bem2d_geometry* g, bem2d_quad_option q,
-                               DevArena* arena, hipStream_t s, bem2d_mesh** out)
+                               DevArena* arena, hipStream_t s, HostStager* stager,
+                               bem2d_mesh** out)
 {
     if (!c || !g || !out) return FAIL(BEM2D_ERR_ARG, "bem2d_mesh_create: null argument");
     if (g->n < 1) return FAIL(BEM2D_ERR_ARG, "bem2d_mesh_create: empty geometry");
@@ -580,39 +705,10 @@ int bem2d_internal_mesh_create(bem2d_ctx* c, const bem2d_geometry* g, bem2d_quad
             return FAIL(BEM2D_ERR_UNSUPPORTED, "element %lld has unknown type %d", (long long)e,
                         g->type[e]);
     HIPCHK(hipSetDevice(c->device));
-
-    std::vector<double> gx, gw;
-    bem2d_internal_gauss01(q.N, gx, gw);
-    Rule r1 = adapted1(gx, gw, q.L, q.sigma);
-    Rule r2 = adapted23(gx, gw, q.L, q.sigma, false);
-    Rule r3 = adapted23(gx, gw, q.L, q.sigma, true);
-
-    // Special (coincident / adjacent) entries with the precedence of lib/bem2d_quadrature.h:271-287:
-    // identical index -> rule 1; else next(i) == index(j) -> rule 3; else prev(i) == index(j) -> 2.
-    std::unordered_map<int64_t, std::vector<int64_t>> pos;
-    pos.reserve((size_t)n * 2);
-    for (int64_t e = 0; e < n; ++e) pos[g->index[e]].push_back(e);
+    const RuleSet* rs = ruleset(c, q);
+    if (!rs) return FAIL(BEM2D_ERR_NOMEM, "bem2d_mesh_create: rule tables");
     std::vector<int64_t> special;
-    std::vector<int64_t> seen;
-    for (int64_t i = 0; i < n; ++i) {
-        seen.clear();
-        auto add = [&](int64_t idx, int rule) {
-            auto it = pos.find(idx);
-            if (it == pos.end()) return;
-            for (int64_t j : it->second) {
-                bool dup = false;
-                for (int64_t s : seen) dup |= (s == j);
-                if (dup) continue;
-                seen.push_back(j);
-                special.push_back(i);
-                special.push_back(j);
-                special.push_back(rule);
-            }
-        };
-        add(g->index[i], 1);
-        add(g->next[i], 3);
-        add(g->prev[i], 2);
-    }
+    special_pairs(g, special);
 
     bem2d_mesh* m = new bem2d_mesh();
     m->ctx = c;
@@ -620,64 +716,62 @@ int bem2d_internal_mesh_create(bem2d_ctx* c, const bem2d_geometry* g, bem2d_quad
     m->n = n;
     m->opts = q;
     m->nq = q.N;
-    m->m1 = (int)r1.w.size();
-    m->m2 = (int)r2.w.size();
-    m->m3 = (int)r3.w.size();
+    m->m1 = rs->m1;
+    m->m2 = rs->m2;
+    m->m3 = rs->m3;
     m->nspecial = (int64_t)special.size() / 3;
     m->h_type.assign(g->type, g->type + n);
     m->h_param.assign(g->param, g->param + 8 * n);
-
-    std::vector<double> rules;
-    for (const Rule* r : {&r1, &r2, &r3}) {
-        rules.insert(rules.end(), r->x.begin(), r->x.end());
-        rules.insert(rules.end(), r->y.begin(), r->y.end());
-        rules.insert(rules.end(), r->w.begin(), r->w.end());
-    }
-    for (const Rule* r : {&r1, &r2, &r3}) append_abscissa_block(rules, *r);
-    std::vector<double> g1d(gx);
-    g1d.insert(g1d.end(), gw.begin(), gw.end());
+    m->d_rules = rs->d_rules;
+    m->d_g1d = rs->d_g1d;
 
     auto cleanup = [&](int code, const char* what) {
         bem2d_mesh_destroy(m);
         return FAIL(code, "bem2d_mesh_create: %s", what);
     };
-    size_t bytes_nodes = sizeof(double) * (NODE_PLANES * (size_t)n * q.N + BOUND_PLANES * (size_t)n);
-    m->d_type = (int32_t*)bem2d_internal_dalloc(arena, sizeof(int32_t) * n);
-    m->d_param = (double*)bem2d_internal_dalloc(arena, sizeof(double) * 8 * n);
-    m->d_adj = (int64_t*)bem2d_internal_dalloc(arena, sizeof(int64_t) * 3 * n);
+    // type | param | index, next, prev | special triples: one block, one upload
+    const size_t o_param = round256(sizeof(int32_t) * n);
+    const size_t o_adj = o_param + round256(sizeof(double) * 8 * n);
+    const size_t o_special = o_adj + round256(sizeof(int64_t) * 3 * n);
+    const size_t bytes_block = o_special + round256(sizeof(int64_t) * (special.size() + 3));
+    const size_t bytes_nodes =
+        sizeof(double) * (NODE_PLANES * (size_t)n * q.N + BOUND_PLANES * (size_t)n);
+    m->d_block = bem2d_internal_dalloc(arena, bytes_block);
     m->d_nodes = (double*)bem2d_internal_dalloc(arena, bytes_nodes);
-    m->d_rules = (double*)bem2d_internal_dalloc(arena, sizeof(double) * (rules.size() + 1));
-    m->d_g1d = (double*)bem2d_internal_dalloc(arena, sizeof(double) * g1d.size());
-    m->d_special = (int64_t*)bem2d_internal_dalloc(arena, sizeof(int64_t) * (special.size() + 3));
-    if (!m->d_type || !m->d_param || !m->d_adj || !m->d_nodes || !m->d_rules || !m->d_g1d ||
-        !m->d_special)
-        return cleanup(BEM2D_ERR_NOMEM, "device allocation failed");
-    std::vector<int64_t> adj(3 * n);
-    std::memcpy(adj.data(), g->index, sizeof(int64_t) * n);
-    std::memcpy(adj.data() + n, g->next, sizeof(int64_t) * n);
-    std::memcpy(adj.data() + 2 * n, g->prev, sizeof(int64_t) * n);
-    bool ok = hipMemcpyAsync(m->d_type, g->type, sizeof(int32_t) * n, hipMemcpyHostToDevice, s) ==
-                  hipSuccess &&
-              hipMemcpyAsync(m->d_param, g->param, sizeof(double) * 8 * n, hipMemcpyHostToDevice,
-                             s) == hipSuccess &&
-              hipMemcpyAsync(m->d_adj, adj.data(), sizeof(int64_t) * 3 * n, hipMemcpyHostToDevice,
-                             s) == hipSuccess &&
-              (rules.empty() || hipMemcpyAsync(m->d_rules, rules.data(), sizeof(double) * rules.size(),
-                                               hipMemcpyHostToDevice, s) == hipSuccess) &&
-              hipMemcpyAsync(m->d_g1d, g1d.data(), sizeof(double) * g1d.size(),
-                             hipMemcpyHostToDevice, s) == hipSuccess &&
-              (special.empty() ||
-               hipMemcpyAsync(m->d_special, special.data(), sizeof(int64_t) * special.size(),
-                              hipMemcpyHostToDevice, s) == hipSuccess);
-    if (!ok) return cleanup(BEM2D_ERR_DEVICE, "upload failed");
+    if (!m->d_block || !m->d_nodes) return cleanup(BEM2D_ERR_NOMEM, "device allocation failed");
+    char* db = (char*)m->d_block;
+    m->d_type = (int32_t*)db;
+    m->d_param = (double*)(db + o_param);
+    m->d_adj = (int64_t*)(db + o_adj);
+    m->d_special = (int64_t*)(db + o_special);
+    std::vector<char> local;
+    char* h = nullptr;
+    if (stager) {
+        h = (char*)stager->get(bytes_block);
+        if (!h) return cleanup(BEM2D_ERR_NOMEM, "pinned staging");
+    } else {
+        local.resize(bytes_block);
+        h = local.data();
+    }
+    std::memcpy(h, g->type, sizeof(int32_t) * n);
+    std::memcpy(h + o_param, g->param, sizeof(double) * 8 * n);
+    std::memcpy(h + o_adj, g->index, sizeof(int64_t) * n);
+    std::memcpy(h + o_adj + sizeof(int64_t) * n, g->next, sizeof(int64_t) * n);
+    std::memcpy(h + o_adj + 2 * sizeof(int64_t) * n, g->prev, sizeof(int64_t) * n);
+    if (!special.empty())
+        std::memcpy(h + o_special, special.data(), sizeof(int64_t) * special.size());
+    if (hipMemcpyAsync(m->d_block, h, o_special + sizeof(int64_t) * special.size(),
+                       hipMemcpyHostToDevice, s) != hipSuccess)
+        return cleanup(BEM2D_ERR_DEVICE, "upload failed");
     const int64_t total = n * q.N;
     prep_nodes_kernel<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(
         n, q.N, m->d_type, m->d_param, m->d_g1d, m->d_g1d + q.N, m->d_nodes);
     prep_bounds_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, q.N, m->d_nodes,
                                                                    m->bounds());
     if (hipGetLastError() != hipSuccess) return cleanup(BEM2D_ERR_DEVICE, "prep kernel launch");
-    // The host vectors above go out of scope on return: wait for the uploads.
-    if (hipStreamSynchronize(s) != hipSuccess) return cleanup(BEM2D_ERR_DEVICE, "prep kernel");
+    // pageable staging goes out of scope on return: wait for the upload
+    if (!stager && hipStreamSynchronize(s) != hipSuccess)
+        return cleanup(BEM2D_ERR_DEVICE, "prep kernel");
     *out = m;
     return BEM2D_OK;
 }
@@ -688,13 +782,8 @@ extern "C" void bem2d_mesh_destroy(bem2d_mesh* m)
     (void)hipSetDevice(m->ctx->device);
     if (!m->arena) (void)hipStreamSynchronize(m->ctx->stream);
     DevArena* a = m->arena;
-    bem2d_internal_dfree(a, m->d_type);
-    bem2d_internal_dfree(a, m->d_param);
-    bem2d_internal_dfree(a, m->d_adj);
+    bem2d_internal_dfree(a, m->d_block);
     bem2d_internal_dfree(a, m->d_nodes);
-    bem2d_internal_dfree(a, m->d_rules);
-    bem2d_internal_dfree(a, m->d_g1d);
-    bem2d_internal_dfree(a, m->d_special);
     delete m;
 }
 

@@ -158,10 +158,12 @@ struct Request {
     int* piv = nullptr;     // pivots + info of the FP64 LU (FP64 requests and the fallback)
     bool fp64_ran = false;  // the FP64 LU solved this request (its info decides the status)
     hipEvent_t ev = nullptr;
-    // results
-    double2* hx = nullptr;  // pinned
-    double2* hff = nullptr; // pinned
-    int* hinfo = nullptr;   // pinned copy of the FP64 LU's info
+    // results, in one pinned block of the engine's pool
+    char* hblock = nullptr;
+    size_t hblock_bytes = 0;
+    double2* hx = nullptr;
+    double2* hff = nullptr;
+    int* hinfo = nullptr;   // copy of the FP64 LU's info
     bem2d_ir_report rep{};
 };
 
@@ -185,6 +187,18 @@ struct bem2d_engine {
     std::vector<hipEvent_t> event_pool;
     RefJob* job = nullptr;
     std::vector<Request*> job_reqs;
+    // Pinned host memory, allocated once and reused: hipHostMalloc costs tens to hundreds of
+    // microseconds a call, which would stall every admission.  Result blocks are kept by size class
+    // (powers of two) for later requests; staging slots carry the uploads of a step's new meshes
+    // and are reused once the setup stream has passed the event recorded after their copies.
+    std::multimap<size_t, char*> pinned_free;
+    struct StageSlot {
+        char* ptr = nullptr;
+        size_t bytes = 0;
+        hipEvent_t ev = nullptr;
+        bool busy = false;                // handed out in the current step, event not yet recorded
+    };
+    std::vector<StageSlot> stage_slots;
     unsigned long long* pinned = nullptr; // refinement read-back buffer, kept for the engine's life
     unsigned long long* dnorms = nullptr; // its device side (outside the arena, grown with it)
     size_t pinned_units = 0;
@@ -218,6 +232,63 @@ void give_event(bem2d_engine* e, hipEvent_t v)
 }
 
 bool event_done(hipEvent_t v) { return !v || hipEventQuery(v) == hipSuccess; }
+
+// A pinned block of at least `bytes` (size class: the next power of two from 4 KiB), or null.
+char* take_pinned(bem2d_engine* e, size_t bytes, size_t* cls)
+{
+    size_t c = 4096;
+    while (c < bytes) c <<= 1;
+    *cls = c;
+    auto it = e->pinned_free.find(c);
+    if (it != e->pinned_free.end()) {
+        char* p = it->second;
+        e->pinned_free.erase(it);
+        return p;
+    }
+    char* p = nullptr;
+    return hipHostMalloc((void**)&p, c) == hipSuccess ? p : nullptr;
+}
+
+void give_pinned(bem2d_engine* e, char* p, size_t cls)
+{
+    if (p) e->pinned_free.emplace(cls, p);
+}
+
+// Staging of the step's mesh uploads: a free slot (never handed out, or its copies done) of at
+// least `bytes`, else a new one.
+struct EngineStager : HostStager {
+    bem2d_engine* e;
+    explicit EngineStager(bem2d_engine* eng) : e(eng) {}
+    void* get(size_t bytes) override
+    {
+        for (auto& sl : e->stage_slots)
+            if (!sl.busy && sl.bytes >= bytes && event_done(sl.ev)) {
+                sl.busy = true;
+                return sl.ptr;
+            }
+        bem2d_engine::StageSlot sl;
+        sl.bytes = std::max<size_t>(bytes, (size_t)1 << 20);
+        if (hipHostMalloc((void**)&sl.ptr, sl.bytes) != hipSuccess) return nullptr;
+        if (hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming) != hipSuccess) {
+            (void)hipHostFree(sl.ptr);
+            return nullptr;
+        }
+        sl.busy = true;
+        e->stage_slots.push_back(sl);
+        return sl.ptr;
+    }
+    // the copies out of every slot handed out so far are enqueued on s: arm their events
+    int arm(hipStream_t s)
+    {
+        int rc = BEM2D_OK;
+        for (auto& sl : e->stage_slots)
+            if (sl.busy) {
+                if (hipEventRecord(sl.ev, s) != hipSuccess) rc = BEM2D_ERR_DEVICE;
+                sl.busy = false;
+            }
+        return rc;
+    }
+};
 
 } // namespace
 
@@ -290,9 +361,7 @@ static void release_request_device(bem2d_engine* e, Request* r)
 static void free_request(bem2d_engine* e, Request* r)
 {
     release_request_device(e, r);
-    if (r->hx) (void)hipHostFree(r->hx);
-    if (r->hff) (void)hipHostFree(r->hff);
-    if (r->hinfo) (void)hipHostFree(r->hinfo);
+    give_pinned(e, r->hblock, r->hblock_bytes);
     delete r;
 }
 
@@ -311,6 +380,11 @@ extern "C" void bem2d_engine_destroy(bem2d_engine* e)
     if (e->blas_ws) (void)hipFree(e->blas_ws);
     if (e->pinned) (void)hipHostFree(e->pinned);
     if (e->dnorms) (void)hipFree(e->dnorms);
+    for (auto& kv : e->pinned_free) (void)hipHostFree(kv.second);
+    for (auto& sl : e->stage_slots) {
+        (void)hipHostFree(sl.ptr);
+        (void)hipEventDestroy(sl.ev);
+    }
     for (hipStream_t s : {e->s_setup, e->s_asm, e->s_lu, e->s_ref})
         if (s) (void)hipStreamDestroy(s);
     e->arena.reset();
@@ -361,8 +435,9 @@ extern "C" int bem2d_engine_submit(bem2d_engine* e, const bem2d_request* q, doub
     return BEM2D_OK;
 }
 
-// Reserves the request's block and builds its mesh and right-hand sides on the setup stream.
-static int admit(bem2d_engine* e, Request* r)
+// Reserves the request's block and builds its mesh and right-hand sides on the setup stream; the
+// uploads leave from the stager's pinned slots, so nothing here waits for the device.
+static int admit(bem2d_engine* e, Request* r, EngineStager& stager)
 {
     r->block = (char*)e->arena->alloc(r->footprint);
     if (!r->block) return BEM2D_ERR_NOMEM;
@@ -372,26 +447,31 @@ static int admit(bem2d_engine* e, Request* r)
     r->A = (double2*)b.alloc(sizeof(double2) * n * n);
     r->B = (double2*)b.alloc(sizeof(double2) * n * r->nrhs);
     r->X = (double2*)b.alloc(sizeof(double2) * n * r->nrhs);
-    r->d_dirs = (double*)b.alloc(sizeof(double) * 2 * r->nrhs);
-    r->d_theta = (double*)b.alloc(sizeof(double) * (r->nang + 1));
+    r->d_dirs = (double*)b.alloc(sizeof(double) * (2 * r->nrhs + r->nang + 1));
+    r->d_theta = r->d_dirs + 2 * r->nrhs;
     r->ff = (double2*)b.alloc(sizeof(double2) * (r->nang + 1) * r->nrhs);
     r->sig = (double2*)b.alloc(sizeof(double2) * (r->nang + 1) * n);
     r->piv = (int*)b.alloc(sizeof(int) * (n + 1));
-    if (!r->A || !r->B || !r->X || !r->d_dirs || !r->d_theta || !r->ff || !r->sig || !r->piv)
+    if (!r->A || !r->B || !r->X || !r->d_dirs || !r->ff || !r->sig || !r->piv)
         return FAIL(BEM2D_ERR_NOMEM, "request footprint too small");
-    if (hipHostMalloc((void**)&r->hx, sizeof(double2) * n * r->nrhs) != hipSuccess ||
-        hipHostMalloc((void**)&r->hff, sizeof(double2) * (r->nang + 1) * r->nrhs) != hipSuccess ||
-        hipHostMalloc((void**)&r->hinfo, sizeof(int)) != hipSuccess)
-        return FAIL(BEM2D_ERR_NOMEM, "pinned result buffers");
+    const size_t bx = round_up(sizeof(double2) * n * r->nrhs);
+    const size_t bff = round_up(sizeof(double2) * (r->nang + 1) * r->nrhs);
+    r->hblock = take_pinned(e, bx + bff + sizeof(int), &r->hblock_bytes);
+    if (!r->hblock) return FAIL(BEM2D_ERR_NOMEM, "pinned result buffers");
+    r->hx = (double2*)r->hblock;
+    r->hff = (double2*)(r->hblock + bx);
+    r->hinfo = (int*)(r->hblock + bx + bff);
     *r->hinfo = 0;
-    HIPCHK(hipMemcpyAsync(r->d_dirs, r->dirs.data(), sizeof(double) * 2 * r->nrhs,
-                          hipMemcpyHostToDevice, e->s_setup));
-    if (r->nang > 0)
-        HIPCHK(hipMemcpyAsync(r->d_theta, r->theta.data(), sizeof(double) * r->nang,
-                              hipMemcpyHostToDevice, e->s_setup));
+    const size_t bdir = sizeof(double) * (2 * r->nrhs + r->nang);
+    double* hd = (double*)stager.get(bdir);
+    if (!hd) return FAIL(BEM2D_ERR_NOMEM, "pinned staging");
+    std::memcpy(hd, r->dirs.data(), sizeof(double) * 2 * r->nrhs);
+    if (r->nang > 0) std::memcpy(hd + 2 * r->nrhs, r->theta.data(), sizeof(double) * r->nang);
+    HIPCHK(hipMemcpyAsync(r->d_dirs, hd, bdir, hipMemcpyHostToDevice, e->s_setup));
     bem2d_geometry g{n, r->type.data(), r->param.data(), r->index.data(), r->next.data(),
                      r->prev.data()};
-    int rc = bem2d_internal_mesh_create(e->ctx, &g, r->opts, &r->bump, e->s_setup, &r->mesh);
+    int rc = bem2d_internal_mesh_create(e->ctx, &g, r->opts, &r->bump, e->s_setup, &stager,
+                                        &r->mesh);
     if (rc) return rc;
     rc = bem2d_internal_rhs(r->mesh, BEM2D_FIELD_COMBINED_PLANE, r->k, r->eta, r->d_dirs, r->nrhs,
                             2.0, r->B, n, e->s_setup);
@@ -512,6 +592,7 @@ extern "C" int bem2d_engine_step(bem2d_engine* e, int* unfinished)
     // first launches back until every mesh is ready; in slices the device starts on the first
     // requests while the host prepares the next ----
     std::vector<Request*> fresh;
+    EngineStager stager(e);
     while (!e->queue.empty() && (int)e->active.size() < e->max_active &&
            (int)fresh.size() < kAdmitPerStep) {
         Request* r = e->queue.front();
@@ -520,7 +601,7 @@ extern "C" int bem2d_engine_step(bem2d_engine* e, int* unfinished)
             fail_request(e, r, BEM2D_ERR_NOMEM);
             continue;
         }
-        const int rc = admit(e, r);
+        const int rc = admit(e, r, stager);
         if (rc == BEM2D_ERR_NOMEM && r->block == nullptr) break; // wait for memory
         e->queue.pop_front();
         if (rc) {
@@ -532,6 +613,10 @@ extern "C" int bem2d_engine_step(bem2d_engine* e, int* unfinished)
         e->active.push_back(r);
         fresh.push_back(r);
         e->stats.admitted++;
+    }
+    {
+        const int rc = stager.arm(e->s_setup);
+        if (rc) return FAIL(rc, "staging event");
     }
     // ---- batched assembly of the new requests ----
     if (!fresh.empty()) {

@@ -5,6 +5,7 @@
 #include <rocblas/rocblas.h>
 
 #include <cstdint>
+#include <mutex>
 #include <vector>
 
 #include "../../include/bem2d_hip.h"
@@ -48,6 +49,16 @@ struct ProfSlot {
     int64_t launches = 0;
 };
 
+// Quadrature rule tables of one QuadOption (AdaptedGauss1/2/3 and Gauss1D), built and uploaded once
+// per context and shared by every mesh of that option: they depend on (N, L, sigma) only.
+struct RuleSet {
+    int N = 0, L = 0;
+    double sigma = 0.0;
+    int m1 = 0, m2 = 0, m3 = 0;          // node counts of AdaptedGauss1/2/3
+    double* d_rules = nullptr;           // AG1 x|y|w, AG2 x|y|w, AG3 x|y|w, abscissa blocks
+    double* d_g1d = nullptr;             // Gauss1D x | w
+};
+
 struct bem2d_ctx {
     int device = 0;
     hipStream_t own_stream = nullptr;
@@ -64,6 +75,8 @@ struct bem2d_ctx {
     bool profiling = false;
     ProfSlot prof[BEM2D_PROF_COUNT];
     TuneState tune;
+    std::mutex rules_mu;
+    std::vector<RuleSet*> rulesets;      // owned; freed with the context
 };
 
 // Records a start event for kernel id when profiling is on; returns the event index (or -1).
@@ -87,10 +100,19 @@ struct bem2d_mesh {
     int64_t* d_adj = nullptr;            // index | next | prev, n each
     double* d_nodes = nullptr;           // 5 planes of nq*n (x, y, nx, ny, |Gamma'| w), then
                                          // 3 planes of n (bounding circle cx, cy, r)
-    double* d_rules = nullptr;           // AG1 x|y|w, AG2 x|y|w, AG3 x|y|w
-    double* d_g1d = nullptr;             // Gauss1D x | w
+    const double* d_rules = nullptr;     // the context's rule tables of opts (RuleSet)
+    const double* d_g1d = nullptr;
     int64_t* d_special = nullptr;        // (i, j, rule) triples
+    void* d_block = nullptr;             // one allocation holding d_type, d_param, d_adj, d_special
     double* bounds() const { return d_nodes + 5 * (size_t)n * nq; }
+};
+
+// Pinned host memory for the uploads of a mesh built inside a stream-ordered pipeline: get(bytes)
+// returns a buffer that stays untouched until the work enqueued on the mesh's stream has passed
+// the copies out of it (the caller's promise), so mesh creation neither waits nor stages.
+struct HostStager {
+    virtual void* get(size_t bytes) = 0;
+    virtual ~HostStager() {}
 };
 
 // Device memory source for meshes, factorisations and refinement workspaces; a null arena means
@@ -124,8 +146,13 @@ int bem2d_internal_potential(bem2d_ctx* c, const bem2d_mesh* m, const bem2d_dev:
 void bem2d_internal_mixed_release(bem2d_ctx* c);
 void bem2d_internal_gauss01(int n, std::vector<double>& x, std::vector<double>& w);
 size_t bem2d_internal_mesh_bytes(int64_t n, bem2d_quad_option q);
+// Mesh whose device arrays come from arena (null: hipMalloc), built on stream s.  With a stager the
+// uploads leave from its pinned buffer and nothing is waited for; without one they leave from
+// pageable memory and the call synchronises s before returning.
 int bem2d_internal_mesh_create(bem2d_ctx* c, const bem2d_geometry* g, bem2d_quad_option q,
-                               DevArena* arena, hipStream_t s, bem2d_mesh** out);
+                               DevArena* arena, hipStream_t s, HostStager* stager,
+                               bem2d_mesh** out);
+void bem2d_internal_rulesets_release(bem2d_ctx* c);
 int bem2d_internal_stage_dirs(bem2d_ctx* c, const double* dirs, int nrhs, const double** out);
 int bem2d_internal_rhs(const bem2d_mesh* m, int kind, double k, double eta, const double* d_dirs,
                        int nrhs, double scale, void* B, int64_t ldb, hipStream_t s);

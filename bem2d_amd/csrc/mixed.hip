@@ -87,6 +87,27 @@ __device__ __forceinline__ int find_problem(const int64_t* start, int count, int
     return d;
 }
 
+// The start of a factorisation (zeroed flags, static pivoting order, cast, |A|_inf) for up to
+// kMaxLuBatch problems per launch; workgroup b belongs to problem d with start[d] <= b < start[d+1]
+// (each kernel fills start[] with its own per-problem workgroup counts).
+struct BeginProblem {
+    int64_t n, np, lda;
+    const double2* A;
+    int* perm;
+    float2* F;
+    double* part;                 // (np / kCastCols) x n partial row sums
+    int* pivwork;                 // colarg | freepos | freerow, n each
+    unsigned long long* claim;    // n
+    unsigned long long* gmax;     // max |U|, max |L|, max |a_ij|
+    unsigned long long* anorm;    // |A|_inf (double bits)
+    int* dflag;                   // [0] bad pivot, [1] sweep time-out
+};
+struct BeginBatch {
+    int count;
+    int64_t start[kMaxLuBatch + 1];
+    BeginProblem p[kMaxLuBatch];
+};
+
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b)
 {
@@ -135,13 +156,20 @@ __device__ __forceinline__ float2 load_or(bool ok, const float2* p, const float2
 // |A|_inf = max_i sum_j |a_ij| costs no second pass over A.
 constexpr int kCastCols = 64;
 
-__global__ __launch_bounds__(256) void cast_kernel(int64_t n, int64_t np, const double2* __restrict__ A,
-                                                   int64_t lda, const int* __restrict__ perm,
-                                                   float2* __restrict__ F,
-                                                   double* __restrict__ part)
+// Workgroups of a problem: ceil(np / 256) row groups x (np / kCastCols) column chunks.
+__global__ __launch_bounds__(256) void cast_kernel(const BeginBatch bt)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t j0 = (int64_t)blockIdx.y * kCastCols;
+    const int di = find_problem(bt.start, bt.count, blockIdx.x);
+    const BeginProblem& pb = bt.p[di];
+    const int64_t n = pb.n, np = pb.np, lda = pb.lda;
+    const double2* __restrict__ A = pb.A;
+    const int* __restrict__ perm = pb.perm;
+    float2* __restrict__ F = pb.F;
+    double* __restrict__ part = pb.part;
+    const int64_t lb = (int64_t)blockIdx.x - bt.start[di], nx = (np + 255) / 256;
+    const int64_t bx = lb % nx, by = lb / nx;
+    const int64_t i = bx * 256 + threadIdx.x;
+    const int64_t j0 = by * kCastCols;
     if (i >= np) return;
     // row i of the factorised matrix is row perm[i] of A (the static pivoting order)
     const int64_t pi = i < n ? (int64_t)perm[i] : 0;
@@ -163,7 +191,7 @@ __global__ __launch_bounds__(256) void cast_kernel(int64_t n, int64_t np, const 
             s += in ? sqrt(fma(re, re, im * im)) : 0.0;
         }
     }
-    if (i < n) part[blockIdx.y * n + i] = s;
+    if (i < n) part[by * n + i] = s;
 }
 
 __device__ __forceinline__ void atomic_max_nonneg(unsigned long long* p, double v)
@@ -172,13 +200,31 @@ __device__ __forceinline__ void atomic_max_nonneg(unsigned long long* p, double 
     atomicMax(p, (unsigned long long)__double_as_longlong(v));
 }
 
-// out = max_i sum_c part[c][i] (out zeroed beforehand)
-__global__ __launch_bounds__(256) void rowsum_max_kernel(int64_t n, int nchunk,
-                                                         const double* __restrict__ part,
-                                                         unsigned long long* __restrict__ out)
+// Zeroes the claims of the static pivoting, |A|_inf, the growth maxima and the flags;
+// ceil(n / 256) workgroups per problem.
+__global__ __launch_bounds__(256) void begin_init_kernel(const BeginBatch bt)
+{
+    const int di = find_problem(bt.start, bt.count, blockIdx.x);
+    const BeginProblem& pb = bt.p[di];
+    const int64_t lb = (int64_t)blockIdx.x - bt.start[di];
+    const int64_t i = lb * blockDim.x + threadIdx.x;
+    if (i < pb.n) pb.claim[i] = 0ull;
+    if (lb == 0 && threadIdx.x < 3) pb.gmax[threadIdx.x] = 0ull;
+    if (lb == 0 && threadIdx.x == 3) *pb.anorm = 0ull;
+    if (lb == 0 && threadIdx.x < 2) pb.dflag[threadIdx.x] = 0;
+}
+
+// anorm = max_i sum_c part[c][i] (zeroed beforehand); ceil(n / 256) workgroups per problem.
+__global__ __launch_bounds__(256) void rowsum_max_kernel(const BeginBatch bt)
 {
     __shared__ double red[256];
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int di = find_problem(bt.start, bt.count, blockIdx.x);
+    const BeginProblem& pb = bt.p[di];
+    const int64_t n = pb.n;
+    const int nchunk = (int)(pb.np / kCastCols);
+    const double* __restrict__ part = pb.part;
+    unsigned long long* __restrict__ out = pb.anorm;
+    const int64_t i = ((int64_t)blockIdx.x - bt.start[di]) * blockDim.x + threadIdx.x;
     double m = 0.0;
     if (i < n) {
         double acc[4] = {0.0, 0.0, 0.0, 0.0};
@@ -215,14 +261,19 @@ __global__ __launch_bounds__(256) void rowsum_max_kernel(int64_t n, int nchunk,
 
 // One workgroup per column: colarg[j] = argmax_i |a_ij| (the smallest i on ties), the claim of
 // that row, and amax = max_ij |a_ij| (NaN entries make it NaN).
-__global__ __launch_bounds__(256) void pivot_colmax_kernel(int64_t n, const double2* __restrict__ A,
-                                                           int64_t lda, int* __restrict__ colarg,
-                                                           unsigned long long* __restrict__ claim,
-                                                           unsigned long long* __restrict__ amax)
+// n workgroups per problem.
+__global__ __launch_bounds__(256) void pivot_colmax_kernel(const BeginBatch bt)
 {
     __shared__ double sb[4];
     __shared__ int sa[4];
-    const int64_t j = blockIdx.x;
+    const int di = find_problem(bt.start, bt.count, blockIdx.x);
+    const BeginProblem& pb = bt.p[di];
+    const int64_t n = pb.n, lda = pb.lda;
+    const double2* __restrict__ A = pb.A;
+    int* __restrict__ colarg = pb.pivwork;
+    unsigned long long* __restrict__ claim = pb.claim;
+    unsigned long long* __restrict__ amax = pb.gmax + 2;
+    const int64_t j = (int64_t)blockIdx.x - bt.start[di];
     const double2* __restrict__ col = A + lda * j;
     double best = -1.0;
     int arg = 0;
@@ -293,13 +344,17 @@ __device__ __forceinline__ int block_scan_1024(bool flag, int* wave_tot, int* to
 
 // One workgroup of 1024 threads: positions whose claim won keep their row; the others, in
 // increasing order, take the unclaimed rows in increasing order (freepos / freerow: n ints each).
-__global__ __launch_bounds__(1024) void pivot_assign_kernel(int64_t n, const int* __restrict__ colarg,
-                                                            const unsigned long long* __restrict__ claim,
-                                                            int* __restrict__ perm,
-                                                            int* __restrict__ freepos,
-                                                            int* __restrict__ freerow)
+// One workgroup per problem (start[] unused).
+__global__ __launch_bounds__(1024) void pivot_assign_kernel(const BeginBatch bt)
 {
     __shared__ int wave_tot[16];
+    const BeginProblem& pb = bt.p[blockIdx.x];
+    const int64_t n = pb.n;
+    const int* __restrict__ colarg = pb.pivwork;
+    const unsigned long long* __restrict__ claim = pb.claim;
+    int* __restrict__ perm = pb.perm;
+    int* __restrict__ freepos = pb.pivwork + n;
+    int* __restrict__ freerow = pb.pivwork + 2 * n;
     const int t = threadIdx.x;
     int nfp = 0, nfr = 0;
     for (int64_t base = 0; base < n; base += 1024) {
@@ -324,30 +379,96 @@ __global__ __launch_bounds__(1024) void pivot_assign_kernel(int64_t n, const int
 }
 
 // Element growth of the factors: gmax[0] = max |U_ij| (i <= j < n), gmax[1] = max |L_ij|
-// (j < i < n); NaN or infinite entries count as infinite.  Grid: rows in 256s x 64-column chunks.
-__global__ __launch_bounds__(256) void growth_kernel(int64_t n, int64_t np, const float2* __restrict__ F,
-                                                     unsigned long long* __restrict__ gmax)
+// (j < i < n); NaN or infinite entries count as infinite.  One pass over the factors after the last
+// step, HBM-bound: workgroup (x, y) reads rows 1024 x .. 1024 x + 1023 of the 32 columns of chunk y,
+// each thread four consecutive rows of a column per load (two 16-byte accesses), and leaves its two
+// maxima in part[2 wg], part[2 wg + 1]; growth_reduce_kernel then takes the maxima over the
+// workgroups (no atomics on two hot words).
+constexpr int kGrowthRows = 1024, kGrowthCols = 32;
+__device__ __forceinline__ float growth_abs(float2 v)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t j0 = (int64_t)blockIdx.y * 64;
+    const float a = sqrtf(fmaf(v.x, v.x, v.y * v.y));
+    return a <= 3.0e38f ? a : INFINITY; // NaN and overflow
+}
+__global__ __launch_bounds__(256) void growth_kernel(int64_t n, int64_t np, const float2* __restrict__ F,
+                                                     float* __restrict__ part)
+{
+    __shared__ float red[2][4];
+    const int64_t nx = (np + kGrowthRows - 1) / kGrowthRows;
+    const int64_t bx = blockIdx.x % nx, by = blockIdx.x / nx;
+    const int64_t i0 = bx * kGrowthRows + 4 * threadIdx.x;
+    const int64_t j0 = by * kGrowthCols;
     float mu = 0.f, ml = 0.f;
-    if (i < n) {
-        const int64_t j1 = min(j0 + 64, n);
-        for (int64_t j = j0; j < j1; ++j) {
-            const float2 v = F[i + np * j];
-            float a = sqrtf(fmaf(v.x, v.x, v.y * v.y));
-            if (!(a <= 3.0e38f)) a = INFINITY; // NaN and overflow
-            if (i <= j) mu = fmaxf(mu, a);
-            else ml = fmaxf(ml, a);
+    if (i0 < n) {
+        constexpr int kUnroll = 8;
+        for (int c0 = 0; c0 < kGrowthCols; c0 += kUnroll) {
+            float4 v[kUnroll][2];
+#pragma unroll
+            for (int c = 0; c < kUnroll; ++c) {
+                const int64_t j = min(j0 + c0 + c, n - 1);
+                const float4* p = reinterpret_cast<const float4*>(F + i0 + np * j);
+                v[c][0] = p[0];
+                v[c][1] = p[1];
+            }
+#pragma unroll
+            for (int c = 0; c < kUnroll; ++c) {
+                const int64_t j = j0 + c0 + c;
+                const float2 e[4] = {make_float2(v[c][0].x, v[c][0].y), make_float2(v[c][0].z, v[c][0].w),
+                                     make_float2(v[c][1].x, v[c][1].y), make_float2(v[c][1].z, v[c][1].w)};
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int64_t i = i0 + r;
+                    const float a = i < n && j < n ? growth_abs(e[r]) : 0.f;
+                    if (i <= j) mu = fmaxf(mu, a);
+                    else ml = fmaxf(ml, a);
+                }
+            }
         }
     }
     for (int off = 32; off > 0; off >>= 1) {
         mu = fmaxf(mu, __shfl_xor(mu, off));
         ml = fmaxf(ml, __shfl_xor(ml, off));
     }
+    const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
-        atomic_max_nonneg(&gmax[0], (double)mu);
-        atomic_max_nonneg(&gmax[1], (double)ml);
+        red[0][w] = mu;
+        red[1][w] = ml;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = fmaxf(fmaxf(red[0][0], red[0][1]), fmaxf(red[0][2], red[0][3]));
+        part[2 * blockIdx.x + 1] = fmaxf(fmaxf(red[1][0], red[1][1]), fmaxf(red[1][2], red[1][3]));
+    }
+}
+
+// gmax[0], gmax[1] = the maxima of the nwg workgroup pairs in part (one workgroup of 1024).
+__global__ __launch_bounds__(1024) void growth_reduce_kernel(int64_t nwg, const float* __restrict__ part,
+                                                             unsigned long long* __restrict__ gmax)
+{
+    __shared__ float red[2][16];
+    float mu = 0.f, ml = 0.f;
+    for (int64_t q = threadIdx.x; q < nwg; q += 1024) {
+        mu = fmaxf(mu, part[2 * q]);
+        ml = fmaxf(ml, part[2 * q + 1]);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        mu = fmaxf(mu, __shfl_xor(mu, off));
+        ml = fmaxf(ml, __shfl_xor(ml, off));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = mu;
+        red[1][threadIdx.x >> 6] = ml;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < 16; ++q) {
+            mu = fmaxf(mu, red[0][q]);
+            ml = fmaxf(ml, red[1][q]);
+        }
+        mu = fmaxf(mu, red[0][0]);
+        ml = fmaxf(ml, red[1][0]);
+        gmax[0] = (unsigned long long)__double_as_longlong((double)mu);
+        gmax[1] = (unsigned long long)__double_as_longlong((double)ml);
     }
 }
 
@@ -1722,39 +1843,60 @@ static LuProblem lu_problem(bem2d_lu32* f, int b)
 }
 
 // Element growth of finished factors (read by the refinement's first read-back).
-static void launch_growth(hipStream_t s, const bem2d_lu32* f)
-{
-    growth_kernel<<<dim3((unsigned)((f->n + 255) / 256), (unsigned)((f->n + 63) / 64)), 256, 0, s>>>(
-        f->n, f->np, f->F, f->gmax);
-}
 
 // Static pivoting order, cast, |A|_inf and the factorisation of diagonal block 0 for count
 // problems.
+// Element growth of finished factors (the cast's row-sum buffer, free by then, holds the partials).
+static void launch_growth(hipStream_t s, const bem2d_lu32* f)
+{
+    const int64_t nwg = ((f->np + kGrowthRows - 1) / kGrowthRows) * ((f->n + kGrowthCols - 1) / kGrowthCols);
+    float* part = reinterpret_cast<float*>(f->rowpart);
+    growth_kernel<<<(unsigned)nwg, 256, 0, s>>>(f->n, f->np, f->F, part);
+    growth_reduce_kernel<<<1, 1024, 0, s>>>(nwg, part, f->gmax);
+}
+
 int bem2d_internal_lu_begin(hipStream_t s, bem2d_lu32* const* lus, const void* const* A64,
                             const int64_t* lda, int count)
 {
-    for (int d = 0; d < count; ++d) {
-        bem2d_lu32* f = lus[d];
-        const int64_t n = f->n;
-        const double2* A = (const double2*)A64[d];
-        const int64_t np = f->np;
-        const int nchunk = (int)(np / kCastCols);
-        zero_u64_kernel<<<1, 64, 0, s>>>((unsigned long long*)f->anorm, 1);
+    for (int c0 = 0; c0 < count; c0 += kMaxLuBatch) {
+        BeginBatch bt{};
+        bt.count = std::min(kMaxLuBatch, count - c0);
+        for (int d = 0; d < bt.count; ++d) {
+            bem2d_lu32* f = lus[c0 + d];
+            BeginProblem& p = bt.p[d];
+            p.n = f->n;
+            p.np = f->np;
+            p.lda = lda[c0 + d];
+            p.A = (const double2*)A64[c0 + d];
+            p.perm = f->perm;
+            p.F = f->F;
+            p.part = f->rowpart;
+            p.pivwork = f->pivwork;
+            p.claim = f->claim;
+            p.gmax = f->gmax;
+            p.anorm = (unsigned long long*)f->anorm;
+            p.dflag = f->dflag;
+            f->step = 0;
+            f->pending_wide = false;
+            f->factored = false;
+        }
+        // workgroups per problem of each kernel -> start[]
+        auto launch = [&](auto per_problem) {
+            bt.start[0] = 0;
+            for (int d = 0; d < bt.count; ++d) bt.start[d + 1] = bt.start[d] + per_problem(bt.p[d]);
+            return (unsigned)bt.start[bt.count];
+        };
+        const unsigned g_rows = launch([](const BeginProblem& p) { return (p.n + 255) / 256; });
+        begin_init_kernel<<<g_rows, 256, 0, s>>>(bt);
         // static pivoting order, then the cast of the rows in that order
-        zero_u64_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(f->claim, n);
-        zero_u64_kernel<<<1, 64, 0, s>>>(f->gmax, 3);
-        pivot_colmax_kernel<<<(unsigned)n, 256, 0, s>>>(n, A, lda[d], f->pivwork, f->claim,
-                                                        f->gmax + 2);
-        pivot_assign_kernel<<<1, 1024, 0, s>>>(n, f->pivwork, f->claim, f->perm, f->pivwork + n,
-                                               f->pivwork + 2 * n);
-        cast_kernel<<<dim3((unsigned)((np + 255) / 256), (unsigned)nchunk), 256, 0, s>>>(
-            n, np, A, lda[d], f->perm, f->F, f->rowpart);
-        rowsum_max_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(
-            n, nchunk, f->rowpart, (unsigned long long*)f->anorm);
-        zero_u64_kernel<<<1, 64, 0, s>>>((unsigned long long*)f->dflag, 1);
-        f->step = 0;
-        f->pending_wide = false;
-        f->factored = false;
+        const unsigned g_cols = launch([](const BeginProblem& p) { return p.n; });
+        pivot_colmax_kernel<<<g_cols, 256, 0, s>>>(bt);
+        pivot_assign_kernel<<<(unsigned)bt.count, 1024, 0, s>>>(bt);
+        const unsigned g_cast =
+            launch([](const BeginProblem& p) { return ((p.np + 255) / 256) * (p.np / kCastCols); });
+        cast_kernel<<<g_cast, 256, 0, s>>>(bt);
+        launch([](const BeginProblem& p) { return (p.n + 255) / 256; });
+        rowsum_max_kernel<<<g_rows, 256, 0, s>>>(bt);
     }
     for (int c0 = 0; c0 < count; c0 += kMaxLuBatch) {
         LuBatch bt{};
