@@ -672,7 +672,11 @@ extern "C" int bem2d_engine_step(bem2d_engine* e, int* unfinished)
     for (Request* r : e->active)
         if (r->stage == FACTORING && !bem2d_internal_lu_done(r->lu)) stepping.push_back(r->lu);
     if (!stepping.empty() && e->lu_inflight.size() < 2) {
-        const int rc = bem2d_internal_lu_advance(e->s_lu, stepping.data(), (int)stepping.size());
+        // every update split from its diagonal block: the fused update holds a whole compute unit
+        // (all registers of its SIMDs, 110 KB of LDS), the split one leaves room beside it for the
+        // assembly wavefronts of other requests (400 served requests: 3.34 -> 2.83 s)
+        const int rc = bem2d_internal_lu_advance(e->s_lu, stepping.data(), (int)stepping.size(),
+                                                 1);
         if (rc) return rc;
         hipEvent_t v = take_event(e);
         HIPCHK(hipEventRecord(v, e->s_lu));

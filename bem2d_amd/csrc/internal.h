@@ -180,7 +180,11 @@ size_t bem2d_internal_lu32_bytes(int64_t n);
 // cast + |A|_inf + diagonal block 0, then one panel step per call (each problem at its own step)
 int bem2d_internal_lu_begin(hipStream_t s, bem2d_lu32* const* lus, const void* const* A64,
                             const int64_t* lda, int count);
-int bem2d_internal_lu_advance(hipStream_t s, bem2d_lu32* const* lus, int count);
+// split_tiles > 0 overrides the context's lu_split_tiles (the engine splits every update: its LU
+// steps share the device with other requests' assemblies, and the split update leaves room on a
+// compute unit for their wavefronts)
+int bem2d_internal_lu_advance(hipStream_t s, bem2d_lu32* const* lus, int count,
+                              int split_tiles = 0);
 bool bem2d_internal_lu_done(const bem2d_lu32* f);
 int64_t bem2d_internal_lu_n(const bem2d_lu32* f);
 

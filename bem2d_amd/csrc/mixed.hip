@@ -1923,7 +1923,7 @@ int bem2d_internal_lu_begin(hipStream_t s, bem2d_lu32* const* lus, const void* c
 // One panel step for each of count problems (each at its own step): the panel solves of block b,
 // then the trailing update, whose first workgroup also factorises diagonal block b + 1 (or, for
 // large trailing matrices, the update followed by that factorisation on its own).
-int bem2d_internal_lu_advance(hipStream_t s, bem2d_lu32* const* lus, int count)
+int bem2d_internal_lu_advance(hipStream_t s, bem2d_lu32* const* lus, int count, int split_tiles)
 {
     std::vector<bem2d_lu32*> act;
     for (int d = 0; d < count; ++d)
@@ -1936,10 +1936,10 @@ int bem2d_internal_lu_advance(hipStream_t s, bem2d_lu32* const* lus, int count)
         // fused diagonal block: the update then fits two workgroups per compute unit, and
         // diag_lu_kernel factorises the block after it.  One launch shape for the whole batch:
         // split only when every problem is that large.
+        const int split_from = split_tiles > 0 ? split_tiles : act[c0]->ctx->tune.lu_split_tiles;
         bool split = true;
         for (int d = 0; d < cnt; ++d)
-            if (lu_problem(act[c0 + d], act[c0 + d]->step).tiles < act[c0]->ctx->tune.lu_split_tiles)
-                split = false;
+            if (lu_problem(act[c0 + d], act[c0 + d]->step).tiles < split_from) split = false;
         for (int d = 0; d < cnt; ++d) {
             bem2d_lu32* f = act[c0 + d];
             const int b = f->step;
