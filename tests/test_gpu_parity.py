@@ -92,6 +92,30 @@ def test_assembly_matches_oracle(ctx, oracle, case):
     assert np.all(np.isfinite(A))
 
 
+def test_sparse_element_numbering(ctx, oracle):
+    """The singular-pair list is built through a bucket array when the element indices span about
+    n values and through a hash map otherwise: the same mesh numbered 0..n-1 and numbered
+    1000 e + 7 (next / prev renumbered alike, -1 kept) gives the same matrix bit for bit, and the
+    sparse numbering matches the oracle."""
+    dense = G.open_arc(120)
+
+    def renum(a):
+        return np.where(a >= 0, 1000 * a + 7, a)
+
+    sparse = G.Geometry(dense.type, dense.param, renum(dense.index), renum(dense.next),
+                        renum(dense.prev))
+    args = (16, 3, 0.15)
+    A1 = _np(ctx.assemble(ctx.mesh(dense, *args), capi.KERNEL_COMBINED, 25.0, 25.0, 1.0, 2.0))
+    m2 = ctx.mesh(sparse, *args)
+    assert m2.special_count() == ctx.mesh(dense, *args).special_count()
+    A2 = _np(ctx.assemble(m2, capi.KERNEL_COMBINED, 25.0, 25.0, 1.0, 2.0))
+    assert np.array_equal(A1, A2)
+    ref = oracle.assemble(sparse, *args, oracle.K_COMBINED, 25.0, 25.0, 1.0, 2.0)
+    err = rel_fro(A2, ref)
+    print("sparse numbering: |A-A_oracle|/|A_oracle| = %.2e" % err, flush=True)
+    assert err < TOL
+
+
 def test_entries_match_mpmath_fixture(ctx):
     f = golden("entries_mpmath.npz")
     for name in ("circle4", "kite5", "triangle6", "arc4open", "circle3_q16"):
