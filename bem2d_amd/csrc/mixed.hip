@@ -880,7 +880,11 @@ __global__ __launch_bounds__(256) void diag_lu_kernel(const LuBatch batch)
 // blocks of 16 x 16; each complex product is four real v_mfma_f32_16x16x4f32.
 // v_mfma_f32_16x16x4f32 lane layout: A operand lane l holds A[l % 16][l / 16], B operand
 // B[l / 16][l % 16]; accumulator register r of lane l is C[4 (l / 16) + r][l % 16].
-constexpr int KP = NB + 4; // padded k extent of the [col][k] LDS operand tile
+// Padded k extent of the [col][k] LDS operand tile.  A wavefront's MFMA operand read takes 16
+// columns x 4 k per instruction, served 32 lanes at a time (16 columns x 2 k): with a pitch of
+// 2 mod 32 floats those 32 words fall in 32 different banks (a pitch of 4 mod 32 put columns c and
+// c + 8 in one bank: two-way conflicts, a quarter of the update kernel's LDS cycles).
+constexpr int KP = NB + 2;
 
 struct TileAcc {
     v4f re[2][2], im[2][2];
