@@ -6,6 +6,7 @@ when the HIP library has not been built -- there is no fallback.
 """
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -302,6 +303,9 @@ class Context:
         h = _P()
         check(lib().bem2d_ctx_create(ctypes.byref(h), device))
         self.h = h
+        # meshes, factorisations and engines of this context: closed before it, so that none
+        # outlives the bem2d_ctx it points to (e.g. objects kept alive by a traceback)
+        self._children = weakref.WeakSet()
         if use_torch_stream:
             s = torch.cuda.current_stream(device)
             check(lib().bem2d_ctx_set_stream(self.h, _P(s.cuda_stream)))
@@ -311,6 +315,8 @@ class Context:
 
     def close(self):
         if self.h:
+            for child in list(self._children):
+                child.close()
             lib().bem2d_ctx_destroy(self.h)
             self.h = None
 
@@ -574,6 +580,7 @@ class LU32:
         h = _P()
         check(lib().bem2d_lu32_create(ctx.h, n, ctypes.byref(h)))
         self.h = h
+        ctx._children.add(self)
 
     def factor(self, A):
         check(lib().bem2d_lu32_factor(self.ctx.h, self.h, _t_ptr(A), _ld(A)))
@@ -615,6 +622,7 @@ class Mesh:
         check(lib().bem2d_mesh_create(ctx.h, ctypes.byref(g), QuadOption(L, N, sigma),
                                       ctypes.byref(h)))
         self.h = h
+        ctx._children.add(self)
         self.N, self.L, self.sigma = N, L, sigma
 
     @property
@@ -646,6 +654,7 @@ class Engine:
         check(lib().bem2d_engine_create(ctx.h, int(hbm_bytes), int(max_active), ctypes.byref(h)))
         self.h = h
         self._meta = {}
+        ctx._children.add(self)
 
     @staticmethod
     def footprint(n, nrhs, nang, precision=PRECISION_MIXED, N=16, L=3, sigma=0.15):
