@@ -5,10 +5,10 @@
 // the block column below and the block row right of diagonal block b are multiplied by the
 // inverses of its triangles (panel_trsm_kernel, MFMA), and the trailing matrix receives the
 // rank-64 update A22 -= L21 U12 as a complex GEMM on v_mfma_f32_16x16x4f32 with FP32 accumulation
-// (four real MFMA products per complex product).  The workgroup of that update which owns diagonal
-// block b + 1 goes on to factorise it -- LU plus the inverses of both triangles, blocked in
-// 16-column panels inside one workgroup (diag_factor) -- while the rest of the update runs, so the
-// panel factorisation overlaps the GEMM.  Without interchanges
+// (three real MFMA products per complex product, Gauss's form).  The workgroup of that update
+// which owns diagonal block b + 1 goes on to factorise it -- LU plus the inverses of both
+// triangles, blocked in 16-column panels inside one workgroup (diag_factor) -- while the rest of
+// the update runs, so the panel factorisation overlaps the GEMM.  Without interchanges
 // the panel has no sequential pivot search, the step that makes partial-pivoting panels latency
 // bound on a GPU.  A zero or non-finite pivot is flagged; refinement then reports failure and the
 // one-shot solver falls back to the FP64 LU.
@@ -886,8 +886,14 @@ __global__ __launch_bounds__(256) void diag_lu_kernel(const LuBatch batch)
 // c + 8 in one bank: two-way conflicts, a quarter of the update kernel's LDS cycles).
 constexpr int KP = NB + 2;
 
+// The trailing update forms each complex product from three real products (Gauss's method):
+// with A = Ar + i Ai and B = Br + i Bi, t1 = Ar Br, t2 = Ai Bi and t3 = (Ar + Ai)(Br + Bi) give
+// Re AB = t1 - t2 and Im AB = t3 - t1 - t2, three MFMAs where the direct form takes four.  Its
+// error bound is normwise, not componentwise, in the imaginary part (Higham, "Stability of a
+// method for multiplying complex matrices with three real matrix multiplications", SIMAX 13
+// (1992)): a backward error of the factorisation of the same order, which refinement removes.
 struct TileAcc {
-    v4f re[2][2], im[2][2];
+    v4f t1[2][2], t2[2][2], t3[2][2];
 };
 
 __device__ __forceinline__ void acc_zero(TileAcc& c)
@@ -896,8 +902,9 @@ __device__ __forceinline__ void acc_zero(TileAcc& c)
     for (int p = 0; p < 2; ++p)
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
-            c.re[p][q] = v4f{0.f, 0.f, 0.f, 0.f};
-            c.im[p][q] = v4f{0.f, 0.f, 0.f, 0.f};
+            c.t1[p][q] = v4f{0.f, 0.f, 0.f, 0.f};
+            c.t2[p][q] = v4f{0.f, 0.f, 0.f, 0.f};
+            c.t3[p][q] = v4f{0.f, 0.f, 0.f, 0.f};
         }
 }
 
@@ -1021,8 +1028,8 @@ constexpr int kUpdLdsFloats = kUpdAllFloats > (int)(sizeof(DiagLds) / sizeof(flo
                                   ? kUpdAllFloats
                                   : (int)(sizeof(DiagLds) / sizeof(float));
 
-// c += Aop Bop with Aop in LDS as [k][row] (pitch KPA) and Bop as [col][k] (pitch KP).  The two
-// MFMAs feeding one accumulator in a k-step are eight apart (no back-to-back dependency).
+// c += Aop Bop with Aop in LDS as [k][row] (pitch KPA) and Bop as [col][k] (pitch KP), in the
+// three-product form of TileAcc.  The MFMAs feeding one accumulator are twelve apart.
 __device__ __forceinline__ void cgemm_64t(const float* __restrict__ Are,
                                           const float* __restrict__ Aim,
                                           const float* __restrict__ Bre,
@@ -1034,28 +1041,31 @@ __device__ __forceinline__ void cgemm_64t(const float* __restrict__ Are,
     const int li = lane % 16, lk = lane / 16;
 #pragma unroll
     for (int ks = 0; ks < NB; ks += 4) {
-        float ar[2], ai[2], br[2], bi[2];
+        float ar[2], ai[2], br[2], bi[2], as[2], bs[2];
 #pragma unroll
         for (int p = 0; p < 2; ++p) {
             ar[p] = Are[(ks + lk) * KPA + wr + 16 * p + li];
             ai[p] = Aim[(ks + lk) * KPA + wr + 16 * p + li];
             br[p] = Bre[(wc + 16 * p + li) * KP + ks + lk];
             bi[p] = Bim[(wc + 16 * p + li) * KP + ks + lk];
+            as[p] = ar[p] + ai[p];
+            bs[p] = br[p] + bi[p];
         }
 #pragma unroll
         for (int p = 0; p < 2; ++p)
 #pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                c.re[p][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[p], br[q], c.re[p][q], 0, 0, 0);
-                c.im[p][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[p], bi[q], c.im[p][q], 0, 0, 0);
-            }
+            for (int q = 0; q < 2; ++q)
+                c.t1[p][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[p], br[q], c.t1[p][q], 0, 0, 0);
 #pragma unroll
         for (int p = 0; p < 2; ++p)
 #pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                c.re[p][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(-ai[p], bi[q], c.re[p][q], 0, 0, 0);
-                c.im[p][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[p], br[q], c.im[p][q], 0, 0, 0);
-            }
+            for (int q = 0; q < 2; ++q)
+                c.t2[p][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[p], bi[q], c.t2[p][q], 0, 0, 0);
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+            for (int q = 0; q < 2; ++q)
+                c.t3[p][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(as[p], bs[q], c.t3[p][q], 0, 0, 0);
     }
 }
 
@@ -1204,8 +1214,9 @@ __global__ __launch_bounds__(256, FUSED ? 1 : 2) void trailing_update_kernel(con
             for (int q = 0; q < 2; ++q)
 #pragma unroll
                 for (int r = 0; r < 4; ++r)
-                    res[p][q][r] = make_float2(cv[p][q][r].x - acc.re[p][q][r],
-                                               cv[p][q][r].y - acc.im[p][q][r]);
+                    res[p][q][r] = make_float2(
+                        cv[p][q][r].x - (acc.t1[p][q][r] - acc.t2[p][q][r]),
+                        cv[p][q][r].y - (acc.t3[p][q][r] - acc.t1[p][q][r] - acc.t2[p][q][r]));
         rres = rc;
         pending = true;
         idx += stride;
