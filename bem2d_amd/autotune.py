@@ -128,6 +128,7 @@ def main():
         print("LU N=%5d  %s" % (n, "  ".join("pair>=%s: %.3f ms" % (
             c if c < 1 << 30 else "never", t) for c, t in row.items())), flush=True)
     pair = min(lu_tot, key=lu_tot.get)
+    quad = ctx.tune_lu_quad_tiles()
     props = __import__("torch").cuda.get_device_properties(0)
     os.makedirs(os.path.dirname(args.out), exist_ok=True)
     with open(args.out, "w") as f:
@@ -144,6 +145,12 @@ def main():
             LU_SIZES, ", ".join("%s: %.3f" % (c if c < 1 << 30 else "never", v)
                                 for c, v in lu_tot.items())))
         f.write("lu_pair_tiles %d\n" % pair)
+        f.write("# FP32 LU, four panels per depth-256 update from this many tiles (carried over: the\n"
+                "# sizes above stay below it; measured on the open arc N=32000)\n")
+        f.write("lu_quad_tiles %d\n" % quad)
+        f.write("# FP32 LU, diagonal block factorised apart from the update from this many trailing\n"
+                "# tiles (carried over; measured on the open arc N=32000 and the kite N=4000)\n")
+        f.write("lu_split_tiles %d\n" % ctx.tune_lu_split_tiles())
     out = {"table": args.out, "assembly_buckets": asm, "lu_pair_tiles": pair,
            "measurements": meas}
     print(json.dumps(out), flush=True)

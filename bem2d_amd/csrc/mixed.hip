@@ -41,10 +41,13 @@ constexpr int NB = 64;             // panel / block width
 constexpr int kMaxSolveRhs = 8;    // right-hand sides handled per triangular sweep
 constexpr int kResidChunks = 64;   // column chunks of the FP64 residual
 constexpr long long kSpinLimit = 1ll << 26;
-// Panels pair up (one depth-128 trailing update per two panels) while the trailing matrix has at
-// least this many 64-wide tiles: the saved pass over it (16 B per entry) then outweighs the
-// diagonal factorisation that the narrow update of the pair leaves exposed (~60 us).  Measured on
-// MI355X: N = 4000 (at most 62 tiles) is faster unpaired, N = 32000 pairs (1.94 -> 1.53 s).
+// Panels group in twos (one depth-128 trailing update per two panels) or fours (depth 256) while
+// the trailing matrix has at least tune.lu_pair_tiles / tune.lu_quad_tiles 64-wide tiles: the
+// saved passes over it (16 B per entry each) and the longer runs of products per output tile then
+// outweigh the narrow cross updates and diagonal factorisations the group leaves exposed (~60 us
+// each).  Measured on MI355X: N = 4000 (at most 62 tiles) is fastest ungrouped; N = 32000 pairs
+// (1.94 -> 1.53 s, four-product updates) and groups in fours from 200 tiles (700 -> 632 ms,
+// three-product updates).
 
 typedef float v4f __attribute__((ext_vector_type(4)));
 
@@ -71,7 +74,7 @@ struct LuProblem {
 };
 
 // Trailing update over the full tile grid, or over its first block row and block column only
-// (the "cross": the next panel, updated at once while the rest waits for a depth-128 update).
+// (the "cross": the next panel, updated at once while the rest waits for the group's update).
 enum { LU_UPDATE_FULL = 0, LU_UPDATE_CROSS = 1 };
 
 struct LuBatch {
@@ -969,10 +972,10 @@ __global__ __launch_bounds__(256) void panel_trsm_kernel(const LuBatch batch)
 }
 
 // ---- trailing update on MFMA, fused with the next diagonal block ---------------------------------
-// C[i][j] -= sum_k L[i][k] U[k][j] for i, j in [rest0, n), k in [k0, k0 + K), K = kb = 64 or 128
-// (two staged chunks of 64), over 64 x 64 output tiles of the full grid, or of its cross (first
-// block row and column) for LU_UPDATE_CROSS.  The workgroups are persistent (about one per compute
-// unit; the kernel's register use allows one workgroup per unit anyway) and walk their tiles
+// C[i][j] -= sum_k L[i][k] U[k][j] for i, j in [rest0, n), k in [k0, k0 + K), K = kb = 64 .. 256
+// (staged in chunks of 64), over 64 x 64 output tiles of the full grid, or of its cross (first
+// block row and column) for LU_UPDATE_CROSS.  The workgroups are persistent (one per compute unit
+// for the fused kernel, whose registers and LDS allow one; two for the unfused) and walk their tiles
 // software-pipelined: while the products of one 64-deep chunk run, the operands of the next chunk
 // are already in flight to registers, and a tile's output block is read when the tile starts.
 // Workgroup 0 of a fused update (Linv_next given) takes tile 0 alone -- the next diagonal block --
@@ -1736,7 +1739,8 @@ struct bem2d_lu32 {
     int64_t n = 0;
     int nblk = 0;
     int step = -1;                // next panel step; nblk - 1 when factorised; -1 before
-    bool pending_wide = false;    // the last update covered the cross only; the next is depth 128
+    int pend = 0;                 // panels of the current group applied to the cross only so far
+    int group = 1;                // panels per trailing update in the current group (1, 2 or 4)
     bool factored = false;
     int cus = 256;                // compute units of the device (persistent update workgroups)
     int64_t np = 0;               // n rounded up to whole 64-wide blocks
@@ -1758,7 +1762,8 @@ struct bem2d_lu32 {
     hipGraphExec_t graph = nullptr;
     const void* graph_A = nullptr;
     int64_t graph_lda = 0;
-    int graph_pair_tiles = 0;     // the panel-pairing choice the graph was captured with
+    int graph_pair_tiles = 0;     // the panel-grouping choices the graph was captured with
+    int graph_quad_tiles = 0;
     int graph_split_tiles = 0;    // and the diagonal-split choice
     int64_t graph_replays = 0;
 };
@@ -1892,7 +1897,7 @@ int bem2d_internal_lu_begin(hipStream_t s, bem2d_lu32* const* lus, const void* c
             p.anorm = (unsigned long long*)f->anorm;
             p.dflag = f->dflag;
             f->step = 0;
-            f->pending_wide = false;
+            f->pend = 0;
             f->factored = false;
         }
         // workgroups per problem of each kernel -> start[]
@@ -1964,21 +1969,27 @@ int bem2d_internal_lu_advance(hipStream_t s, bem2d_lu32* const* lus, int count, 
             p.UinvT_next = f->UinvT + (size_t)(b + 1) * NB * NB;
             tr.p[d] = p;
             tr.start[d + 1] = tr.start[d] + 4 * 2 * p.tiles; // four strips per panel tile
-            // Panels pair up: step b updates only the cross (the next panel, which step b + 1
-            // factorises), and step b + 1 applies both panels to the rest at depth 128, so the
-            // trailing matrix is read and written once per two panels.
+            // Panels group in twos or fours: each step of a group but the last updates only the
+            // cross (the next panel, which the next step factorises) with every panel of the group
+            // so far, and the last step applies the whole group to the rest at depth 128 or 256,
+            // so the trailing matrix is read and written once per group.
             LuProblem u = p;
             int64_t blocks;
-            if (f->pending_wide) {
-                u.k0 = (int64_t)(b - 1) * NB;
-                u.kb = 2 * NB;
-                f->pending_wide = false;
-                blocks = (int64_t)u.tiles * u.tiles;
-            } else if (b + 2 <= f->nblk - 1 && u.tiles >= f->ctx->tune.lu_pair_tiles) {
+            int pend = f->pend;
+            if (pend == 0) {
+                const TuneState& tn = f->ctx->tune;
+                f->group = 1;
+                if (b + 4 <= f->nblk - 1 && u.tiles >= tn.lu_quad_tiles) f->group = 4;
+                else if (b + 2 <= f->nblk - 1 && u.tiles >= tn.lu_pair_tiles) f->group = 2;
+            }
+            u.k0 = (int64_t)(b - pend) * NB;
+            u.kb = (pend + 1) * NB;
+            if (pend + 1 < f->group) {
                 u.mode = LU_UPDATE_CROSS;
-                f->pending_wide = true;
+                f->pend = pend + 1;
                 blocks = 2 * (int64_t)u.tiles - 1;
             } else {
+                f->pend = 0;
                 blocks = (int64_t)u.tiles * u.tiles;
             }
             // persistent workgroups: the compute units shared among the problems of the launch
@@ -2064,6 +2075,7 @@ extern "C" int bem2d_lu32_factor(bem2d_ctx* c, bem2d_lu32* f, const void* A64, i
     const bool graphs = lu_graphs_enabled();
     const bool current = f->graph && f->graph_A == A64 && f->graph_lda == lda &&
                          f->graph_pair_tiles == c->tune.lu_pair_tiles &&
+                         f->graph_quad_tiles == c->tune.lu_quad_tiles &&
                          f->graph_split_tiles == c->tune.lu_split_tiles;
     if (graphs && !current && f->seen_A == A64 && f->seen_lda == lda) {
         if (!c->capture) HIPCHK(hipStreamCreateWithFlags(&c->capture, hipStreamNonBlocking));
@@ -2086,17 +2098,19 @@ extern "C" int bem2d_lu32_factor(bem2d_ctx* c, bem2d_lu32* f, const void* A64, i
         f->graph_A = A64;
         f->graph_lda = lda;
         f->graph_pair_tiles = c->tune.lu_pair_tiles;
+        f->graph_quad_tiles = c->tune.lu_quad_tiles;
         f->graph_split_tiles = c->tune.lu_split_tiles;
     }
     bem2d_internal_prof_begin(c, BEM2D_PROF_LU);
     int rc = BEM2D_OK;
     if (graphs && f->graph && f->graph_A == A64 && f->graph_lda == lda &&
         f->graph_pair_tiles == c->tune.lu_pair_tiles &&
+        f->graph_quad_tiles == c->tune.lu_quad_tiles &&
         f->graph_split_tiles == c->tune.lu_split_tiles) {
         if (hipGraphLaunch(f->graph, s) != hipSuccess) rc = FAIL(BEM2D_ERR_DEVICE, "LU graph launch failed");
         // the host-side state the launch sequence leaves behind
         f->step = f->nblk - 1;
-        f->pending_wide = false;
+        f->pend = 0;
         f->factored = true;
         f->graph_replays++;
     } else {

@@ -140,13 +140,14 @@ def test_many_right_hand_sides(ctx, nrhs):
 
 
 def test_paired_panels_large_trailing_matrix(ctx):
-    """n = 5440 (85 tiles) with panels pairing while the trailing matrix has >= 40 tiles: the first
-    panels pair up (one depth-128 update per two panels), the later ones do not -- both against
-    the FP64 solve."""
+    """n = 5440 (85 tiles) with panels in fours while the trailing matrix has >= 60 tiles and in
+    twos while it has >= 40: the first panels group in fours (one depth-256 update per four
+    panels), the next ones in twos, the last ones not at all -- against the FP64 solve."""
     n, k = 5440, 60.0
-    print("case paired panels n=%d" % n, flush=True)
-    saved = ctx.tune_lu_pair_tiles()
+    print("case grouped panels n=%d" % n, flush=True)
+    saved, saved_q = ctx.tune_lu_pair_tiles(), ctx.tune_lu_quad_tiles()
     ctx.tune_lu_pair_tiles(40)
+    ctx.tune_lu_quad_tiles(60)
     try:
         geom = G.kite(n)
         mesh = ctx.mesh(geom, 16, 3, 0.15)
@@ -158,6 +159,7 @@ def test_paired_panels_large_trailing_matrix(ctx):
         mesh.close()
     finally:
         ctx.tune_lu_pair_tiles(saved)
+        ctx.tune_lu_quad_tiles(saved_q)
     assert rep.fallback_fp64 == 0 and rep.backward_error <= 1e-12
     assert (torch.linalg.norm(Xm - X64) / torch.linalg.norm(X64)).item() < 1e-10
 
@@ -167,15 +169,17 @@ def test_factor_graph_replays_match_eager(ctx, n):
     """Repeated factorisations of one buffer replay a captured hipGraph (from the second call on);
     every replay, with new matrix contents and eager work and synchronisations in between, gives
     the same factors as an eager factorisation of the same matrix on a fresh buffer (compared
-    through one correction x = U^-1 L^-1 b).  n = 5440 (85 tiles), with pairing from 40 tiles,
-    covers paired panels in the graph."""
+    through one correction x = U^-1 L^-1 b).  n = 5440 (85 tiles), with panels in fours from 60
+    tiles and in twos from 40, covers both groupings in the graph."""
     print("case factor graph n=%d" % n, flush=True)
-    saved = ctx.tune_lu_pair_tiles()
+    saved, saved_q = ctx.tune_lu_pair_tiles(), ctx.tune_lu_quad_tiles()
     ctx.tune_lu_pair_tiles(40)
+    ctx.tune_lu_quad_tiles(60)
     try:
         _factor_graph_replays(ctx, n)
     finally:
         ctx.tune_lu_pair_tiles(saved)
+        ctx.tune_lu_quad_tiles(saved_q)
 
 
 def _factor_graph_replays(ctx, n):

@@ -40,24 +40,29 @@ def test_lu_pairing_choices_solve(ctx):
     b = rng.standard_normal((n, 1)) + 1j * rng.standard_normal((n, 1))
     x_ref = np.linalg.solve(A, b)
     At = torch.tensor(np.ascontiguousarray(A.T), device="cuda").t()
-    saved = ctx.tune_lu_pair_tiles()
+    saved, saved_q = ctx.tune_lu_pair_tiles(), ctx.tune_lu_quad_tiles()
+    never = 1 << 30
     try:
-        for pair in (1, 10, 1 << 30):  # pair whenever possible, from 10 tiles, never
+        # (pair from, fours from): pairs whenever possible, from 10 tiles, never; fours whenever
+        # possible; fours from 8 tiles with pairs below
+        for pair, quad in ((1, never), (10, never), (never, never), (never, 1), (1, 8)):
             ctx.tune_lu_pair_tiles(pair)
+            ctx.tune_lu_quad_tiles(quad)
             B = torch.tensor(np.ascontiguousarray(b.T), device="cuda").t().clone()
             rep = ctx.solve_mixed(At, B, tol=1e-12)
             err = rel_fro(B.cpu().numpy(), x_ref)
-            print("pair tiles", pair, "backward %.2e" % rep.backward_error, "err %.2e" % err,
-                  flush=True)
+            print("pair tiles", pair, "quad tiles", quad, "backward %.2e" % rep.backward_error,
+                  "err %.2e" % err, flush=True)
             assert rep.fallback_fp64 == 0 and rep.backward_error <= 1e-12 and err < 1e-10
     finally:
         ctx.tune_lu_pair_tiles(saved)
+        ctx.tune_lu_quad_tiles(saved_q)
 
 
 def test_lu_split_diagonal_matches_fused(ctx):
     """The FP32 LU with the diagonal block factorised apart from the update (two workgroups per
     compute unit) gives the fused path's solution bit for bit (the same products, summed in the
-    same order), with and without paired panels and on a replayed graph."""
+    same order), with panels single, in twos and in fours, and on a replayed graph."""
     n = 1500  # 24 block columns: split on every step with lu_split_tiles = 1
     rng = np.random.default_rng(11)
     A = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) / np.sqrt(n)
@@ -66,9 +71,11 @@ def test_lu_split_diagonal_matches_fused(ctx):
     x_ref = np.linalg.solve(A, b)
     At = torch.tensor(np.ascontiguousarray(A.T), device="cuda").t()
     saved_s, saved_p = ctx.tune_lu_split_tiles(), ctx.tune_lu_pair_tiles()
+    saved_q = ctx.tune_lu_quad_tiles()
     try:
-        for pair in (1 << 30, 4):
+        for pair, quad in ((1 << 30, 1 << 30), (4, 1 << 30), (4, 8)):
             ctx.tune_lu_pair_tiles(pair)
+            ctx.tune_lu_quad_tiles(quad)
             facs = []
             for split in (1 << 30, 1):
                 ctx.tune_lu_split_tiles(split)
@@ -80,25 +87,28 @@ def test_lu_split_diagonal_matches_fused(ctx):
                 rep = lu.refine(At, B, X, tol=1e-12)
                 assert rep.converged and rel_fro(X.cpu().numpy(), x_ref) < 1e-10
                 facs.append(X.cpu().numpy())
-                print("pair", pair, "split", split, "replays", lu.graph_replays(), flush=True)
+                print("pair", pair, "quad", quad, "split", split, "replays", lu.graph_replays(),
+                      flush=True)
                 lu.close()
             assert np.array_equal(facs[0], facs[1])
     finally:
         ctx.tune_lu_split_tiles(saved_s)
         ctx.tune_lu_pair_tiles(saved_p)
+        ctx.tune_lu_quad_tiles(saved_q)
 
 
 def test_table_load(ctx, tmp_path):
     saved_a, saved_p = ctx.tune_assembly(), ctx.tune_lu_pair_tiles()
+    saved_q = ctx.tune_lu_quad_tiles()
     try:
         good = tmp_path / "good.tune"
         good.write_text("# measured\nasm_split 500 4\nasm_split 9000 2  # comment\n"
-                        "lu_pair_tiles 77\n")
+                        "lu_pair_tiles 77\nlu_quad_tiles 150\n")
         ctx.tune_load(str(good))
         assert ctx.tune_assembly() == [(500, 4), (9000, 2)]
-        assert ctx.tune_lu_pair_tiles() == 77
+        assert ctx.tune_lu_pair_tiles() == 77 and ctx.tune_lu_quad_tiles() == 150
         for text in ("asm_split 500 3\n", "asm_split 900 4\nasm_split 500 2\n", "bogus 1\n",
-                     "lu_pair_tiles 0\n"):
+                     "lu_pair_tiles 0\n", "lu_quad_tiles 0\n"):
             bad = tmp_path / "bad.tune"
             bad.write_text(text)
             with pytest.raises(capi.Bem2dError) as e:
@@ -109,3 +119,4 @@ def test_table_load(ctx, tmp_path):
     finally:
         ctx.tune_assembly(saved_a)
         ctx.tune_lu_pair_tiles(saved_p)
+        ctx.tune_lu_quad_tiles(saved_q)
