@@ -999,13 +999,6 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t slab(const float2* base)
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<float2*>(base), (short)0, 0x7fffffff,
                                              0x00020000);
 }
-__device__ __forceinline__ float2 slab_load(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
-{
-    const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
-    float2 f;
-    __builtin_memcpy(&f, &v, sizeof(f));
-    return f;
-}
 // four consecutive complex values (32 bytes) as two 16-byte accesses
 __device__ __forceinline__ void slab_load4(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff,
                                            float2* f)
@@ -1097,16 +1090,17 @@ __global__ __launch_bounds__(256, FUSED ? 1 : 2) void trailing_update_kernel(con
     float* Bim = lds + 2 * NB * KPA + NB * KP;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int64_t rest0 = k0 + K;
-    constexpr int kPer = NB * NB / 256;
-    // Operand element e = t + 256 i of a chunk: row x = t % 64 of column y = t / 64 + 4 i of the
-    // slab (coalesced down the columns), at byte offset voff + i soff_step.
-    const unsigned voff = (unsigned)(((t & 63) + ld * (t >> 6)) * 8);
-    const unsigned soff_step = (unsigned)(4 * ld * 8);
+    constexpr int kPer = NB * NB / 512;
+    // Operand element pair i of a chunk: rows x = 2 (t % 32), x + 1 of column y = t / 32 + 8 i of
+    // the slab (one 16-byte load per lane, coalesced down the columns), at byte offset
+    // voff + i soff_step; staged as two 8-byte LDS writes per operand (real parts, imaginary parts).
+    const unsigned voff = (unsigned)((2 * (t & 31) + ld * (t >> 5)) * 8);
+    const unsigned soff_step = (unsigned)(8 * ld * 8);
     // output element (p, q, r) of this thread: row wr + 16 p + 4 (lane / 16) + r, column
     // wc + 16 q + lane % 16 of the tile
     const int wr = (wave & 1) * 32, wc = (wave >> 1) * 32;
     const unsigned voffc = (unsigned)(((wr + 4 * (lane >> 4)) + ld * (wc + (lane & 15))) * 8);
-    float2 va[kPer], vb[kPer];
+    float4 va[kPer], vb[kPer];
     auto fetch = [&](int idx, int kc) __attribute__((always_inline)) {
         int bx, by;
         update_tile_xy(pb, idx, bx, by);
@@ -1115,8 +1109,10 @@ __global__ __launch_bounds__(256, FUSED ? 1 : 2) void trailing_update_kernel(con
         const __amdgpu_buffer_rsrc_t rb = slab(A + (k0 + kc) + ld * c0);
 #pragma unroll
         for (int i = 0; i < kPer; ++i) {
-            va[i] = slab_load(ra, voff, i * soff_step);
-            vb[i] = slab_load(rb, voff, i * soff_step);
+            const auto a = __builtin_amdgcn_raw_buffer_load_b128(ra, voff, i * soff_step, 0);
+            const auto b = __builtin_amdgcn_raw_buffer_load_b128(rb, voff, i * soff_step, 0);
+            __builtin_memcpy(&va[i], &a, 16);
+            __builtin_memcpy(&vb[i], &b, 16);
         }
     };
     // A tile's output is stored during the next tile (after its operands are staged), so the
@@ -1187,11 +1183,11 @@ __global__ __launch_bounds__(256, FUSED ? 1 : 2) void trailing_update_kernel(con
             __syncthreads(); // the previous chunk's operand reads are done
 #pragma unroll
             for (int i = 0; i < kPer; ++i) {
-                const int x = t & 63, y = (t >> 6) + 4 * i;
-                Are[y * KPA + x] = va[i].x;
-                Aim[y * KPA + x] = va[i].y;
-                Bre[y * KP + x] = vb[i].x;
-                Bim[y * KP + x] = vb[i].y;
+                const int x = 2 * (t & 31), y = (t >> 5) + 8 * i;
+                *reinterpret_cast<float2*>(Are + y * KPA + x) = make_float2(va[i].x, va[i].z);
+                *reinterpret_cast<float2*>(Aim + y * KPA + x) = make_float2(va[i].y, va[i].w);
+                *reinterpret_cast<float2*>(Bre + y * KP + x) = make_float2(vb[i].x, vb[i].z);
+                *reinterpret_cast<float2*>(Bim + y * KP + x) = make_float2(vb[i].y, vb[i].w);
             }
             __syncthreads();
             // The previous tile's output goes out and this tile's output block comes in (first
