@@ -148,6 +148,8 @@ def main():
         f.write("# FP32 LU, four panels per depth-256 update from this many tiles (carried over: the\n"
                 "# sizes above stay below it; measured on the open arc N=32000)\n")
         f.write("lu_quad_tiles %d\n" % quad)
+        f.write("# FP32 LU, eight panels per depth-512 update from this many tiles (carried over)\n")
+        f.write("lu_oct_tiles %d\n" % ctx.tune_lu_oct_tiles())
         f.write("# FP32 LU, diagonal block factorised apart from the update from this many trailing\n"
                 "# tiles (carried over; measured on the open arc N=32000 and the kite N=4000)\n")
         f.write("lu_split_tiles %d\n" % ctx.tune_lu_split_tiles())

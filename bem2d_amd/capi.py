@@ -147,6 +147,8 @@ SIGNATURES = {
     "bem2d_tune_get_lu_pair_tiles": (_I, [_P]),
     "bem2d_tune_set_lu_quad_tiles": (_I, [_P, _I]),
     "bem2d_tune_get_lu_quad_tiles": (_I, [_P]),
+    "bem2d_tune_set_lu_oct_tiles": (_I, [_P, _I]),
+    "bem2d_tune_get_lu_oct_tiles": (_I, [_P]),
     "bem2d_tune_set_lu_split_tiles": (_I, [_P, _I]),
     "bem2d_tune_get_lu_split_tiles": (_I, [_P]),
     "bem2d_tune_load": (_I, [_P, ctypes.c_char_p]),
@@ -369,6 +371,13 @@ class Context:
         if tiles is not None:
             check(lib().bem2d_tune_set_lu_quad_tiles(self.h, int(tiles)))
         return lib().bem2d_tune_get_lu_quad_tiles(self.h)
+
+    def tune_lu_oct_tiles(self, tiles=None):
+        """Trailing-matrix size (64 x 64 tiles per side) from which the FP32 LU groups eight panels
+        into one depth-512 trailing update; returns the current value."""
+        if tiles is not None:
+            check(lib().bem2d_tune_set_lu_oct_tiles(self.h, int(tiles)))
+        return lib().bem2d_tune_get_lu_oct_tiles(self.h)
 
     def tune_lu_split_tiles(self, tiles=None):
         """Trailing-matrix size (64 x 64 tiles per side) from which the FP32 LU factorises the

@@ -149,6 +149,15 @@ extern "C" int bem2d_tune_set_lu_quad_tiles(bem2d_ctx* c, int tiles)
 
 extern "C" int bem2d_tune_get_lu_quad_tiles(const bem2d_ctx* c) { return c ? c->tune.lu_quad_tiles : -1; }
 
+extern "C" int bem2d_tune_set_lu_oct_tiles(bem2d_ctx* c, int tiles)
+{
+    if (!c || tiles < 1) return FAIL(BEM2D_ERR_ARG, "bem2d_tune_set_lu_oct_tiles: bad argument");
+    c->tune.lu_oct_tiles = tiles;
+    return BEM2D_OK;
+}
+
+extern "C" int bem2d_tune_get_lu_oct_tiles(const bem2d_ctx* c) { return c ? c->tune.lu_oct_tiles : -1; }
+
 extern "C" int bem2d_tune_set_lu_split_tiles(bem2d_ctx* c, int tiles)
 {
     if (!c || tiles < 1) return FAIL(BEM2D_ERR_ARG, "bem2d_tune_set_lu_split_tiles: bad argument");
@@ -162,6 +171,7 @@ extern "C" int bem2d_tune_get_lu_split_tiles(const bem2d_ctx* c) { return c ? c-
 //   asm_split <upper_tiles> <wavefronts per tile>     (buckets in increasing order)
 //   lu_pair_tiles <tiles>
 //   lu_quad_tiles <tiles>
+//   lu_oct_tiles <tiles>
 //   lu_split_tiles <tiles>
 extern "C" int bem2d_tune_load(bem2d_ctx* c, const char* path)
 {
@@ -171,6 +181,7 @@ extern "C" int bem2d_tune_load(bem2d_ctx* c, const char* path)
     int64_t upper[kMaxTuneBuckets];
     int split[kMaxTuneBuckets];
     int nb = 0, pair = c->tune.lu_pair_tiles, quad = c->tune.lu_quad_tiles;
+    int oct = c->tune.lu_oct_tiles;
     int lusplit = c->tune.lu_split_tiles, rc = BEM2D_OK;
     char line[256];
     while (rc == BEM2D_OK && std::fgets(line, sizeof line, f)) {
@@ -186,6 +197,8 @@ extern "C" int bem2d_tune_load(bem2d_ctx* c, const char* path)
             pair = (int)a;
         } else if (!std::strcmp(key, "lu_quad_tiles") && got == 2) {
             quad = (int)a;
+        } else if (!std::strcmp(key, "lu_oct_tiles") && got == 2) {
+            oct = (int)a;
         } else if (!std::strcmp(key, "lu_split_tiles") && got == 2) {
             lusplit = (int)a;
         } else {
@@ -196,11 +209,13 @@ extern "C" int bem2d_tune_load(bem2d_ctx* c, const char* path)
     if (rc) return rc;
     if (pair < 1) return FAIL(BEM2D_ERR_ARG, "bem2d_tune_load: lu_pair_tiles %d < 1 in %s", pair, path);
     if (quad < 1) return FAIL(BEM2D_ERR_ARG, "bem2d_tune_load: lu_quad_tiles %d < 1 in %s", quad, path);
+    if (oct < 1) return FAIL(BEM2D_ERR_ARG, "bem2d_tune_load: lu_oct_tiles %d < 1 in %s", oct, path);
     if (lusplit < 1)
         return FAIL(BEM2D_ERR_ARG, "bem2d_tune_load: lu_split_tiles %d < 1 in %s", lusplit, path);
     if ((rc = bem2d_tune_set_assembly(c, nb, upper, split))) return rc;
     if ((rc = bem2d_tune_set_lu_split_tiles(c, lusplit))) return rc;
     if ((rc = bem2d_tune_set_lu_quad_tiles(c, quad))) return rc;
+    if ((rc = bem2d_tune_set_lu_oct_tiles(c, oct))) return rc;
     return bem2d_tune_set_lu_pair_tiles(c, pair);
 }
 

@@ -40,8 +40,9 @@ struct TuneState {
     int asm_split[kMaxTuneBuckets] = {4};
     // FP32 LU: panels pair into depth-128 updates while the trailing matrix has >= this many tiles
     int lu_pair_tiles = 80;
-    // and four panels into depth-256 updates while it has >= this many
+    // four panels into depth-256 updates while it has >= this many, eight into depth-512 from
     int lu_quad_tiles = 200;
+    int lu_oct_tiles = 250;
     int lu_split_tiles = 64;   // trailing tiles from which the diagonal block is factorised apart
 };
 

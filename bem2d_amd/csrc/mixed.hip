@@ -41,13 +41,14 @@ constexpr int NB = 64;             // panel / block width
 constexpr int kMaxSolveRhs = 8;    // right-hand sides handled per triangular sweep
 constexpr int kResidChunks = 64;   // column chunks of the FP64 residual
 constexpr long long kSpinLimit = 1ll << 26;
-// Panels group in twos (one depth-128 trailing update per two panels) or fours (depth 256) while
-// the trailing matrix has at least tune.lu_pair_tiles / tune.lu_quad_tiles 64-wide tiles: the
-// saved passes over it (16 B per entry each) and the longer runs of products per output tile then
-// outweigh the narrow cross updates and diagonal factorisations the group leaves exposed (~60 us
-// each).  Measured on MI355X: N = 4000 (at most 62 tiles) is fastest ungrouped; N = 32000 pairs
-// (1.94 -> 1.53 s, four-product updates) and groups in fours from 200 tiles (700 -> 632 ms,
-// three-product updates).
+// Panels group in twos (one depth-128 trailing update per two panels), fours (depth 256) or eights
+// (depth 512) while the trailing matrix has at least tune.lu_pair_tiles / lu_quad_tiles /
+// lu_oct_tiles 64-wide tiles: the saved passes over it (16 B per entry each) and the longer runs of
+// products per output tile then outweigh the narrow cross updates and diagonal factorisations the
+// group leaves exposed (~60 us each).  Measured on MI355X: N = 4000 (at most 62 tiles) is fastest
+// ungrouped; N = 32000 pairs (1.94 -> 1.53 s, four-product updates), groups in fours from 200 tiles
+// (700 -> 632 ms, three-product updates) and in eights from 250 (635 -> 616 ms; sixteens from 350
+// tiles gained nothing more).
 
 typedef float v4f __attribute__((ext_vector_type(4)));
 
@@ -972,7 +973,7 @@ __global__ __launch_bounds__(256) void panel_trsm_kernel(const LuBatch batch)
 }
 
 // ---- trailing update on MFMA, fused with the next diagonal block ---------------------------------
-// C[i][j] -= sum_k L[i][k] U[k][j] for i, j in [rest0, n), k in [k0, k0 + K), K = kb = 64 .. 256
+// C[i][j] -= sum_k L[i][k] U[k][j] for i, j in [rest0, n), k in [k0, k0 + K), K = kb = 64 .. 512
 // (staged in chunks of 64), over 64 x 64 output tiles of the full grid, or of its cross (first
 // block row and column) for LU_UPDATE_CROSS.  The workgroups are persistent (one per compute unit
 // for the fused kernel, whose registers and LDS allow one; two for the unfused) and walk their tiles
@@ -1736,7 +1737,7 @@ struct bem2d_lu32 {
     int nblk = 0;
     int step = -1;                // next panel step; nblk - 1 when factorised; -1 before
     int pend = 0;                 // panels of the current group applied to the cross only so far
-    int group = 1;                // panels per trailing update in the current group (1, 2 or 4)
+    int group = 1;                // panels per trailing update in the current group (1, 2, 4, 8)
     bool factored = false;
     int cus = 256;                // compute units of the device (persistent update workgroups)
     int64_t np = 0;               // n rounded up to whole 64-wide blocks
@@ -1760,6 +1761,7 @@ struct bem2d_lu32 {
     int64_t graph_lda = 0;
     int graph_pair_tiles = 0;     // the panel-grouping choices the graph was captured with
     int graph_quad_tiles = 0;
+    int graph_oct_tiles = 0;
     int graph_split_tiles = 0;    // and the diagonal-split choice
     int64_t graph_replays = 0;
 };
@@ -1965,17 +1967,18 @@ int bem2d_internal_lu_advance(hipStream_t s, bem2d_lu32* const* lus, int count, 
             p.UinvT_next = f->UinvT + (size_t)(b + 1) * NB * NB;
             tr.p[d] = p;
             tr.start[d + 1] = tr.start[d] + 4 * 2 * p.tiles; // four strips per panel tile
-            // Panels group in twos or fours: each step of a group but the last updates only the
-            // cross (the next panel, which the next step factorises) with every panel of the group
-            // so far, and the last step applies the whole group to the rest at depth 128 or 256,
-            // so the trailing matrix is read and written once per group.
+            // Panels group in twos, fours or eights: each step of a group but the last updates only
+            // the cross (the next panel, which the next step factorises) with every panel of the
+            // group so far, and the last step applies the whole group to the rest at depth 128 to
+            // 512, so the trailing matrix is read and written once per group.
             LuProblem u = p;
             int64_t blocks;
             int pend = f->pend;
             if (pend == 0) {
                 const TuneState& tn = f->ctx->tune;
                 f->group = 1;
-                if (b + 4 <= f->nblk - 1 && u.tiles >= tn.lu_quad_tiles) f->group = 4;
+                if (b + 8 <= f->nblk - 1 && u.tiles >= tn.lu_oct_tiles) f->group = 8;
+                else if (b + 4 <= f->nblk - 1 && u.tiles >= tn.lu_quad_tiles) f->group = 4;
                 else if (b + 2 <= f->nblk - 1 && u.tiles >= tn.lu_pair_tiles) f->group = 2;
             }
             u.k0 = (int64_t)(b - pend) * NB;
@@ -2072,6 +2075,7 @@ extern "C" int bem2d_lu32_factor(bem2d_ctx* c, bem2d_lu32* f, const void* A64, i
     const bool current = f->graph && f->graph_A == A64 && f->graph_lda == lda &&
                          f->graph_pair_tiles == c->tune.lu_pair_tiles &&
                          f->graph_quad_tiles == c->tune.lu_quad_tiles &&
+                         f->graph_oct_tiles == c->tune.lu_oct_tiles &&
                          f->graph_split_tiles == c->tune.lu_split_tiles;
     if (graphs && !current && f->seen_A == A64 && f->seen_lda == lda) {
         if (!c->capture) HIPCHK(hipStreamCreateWithFlags(&c->capture, hipStreamNonBlocking));
@@ -2095,6 +2099,7 @@ extern "C" int bem2d_lu32_factor(bem2d_ctx* c, bem2d_lu32* f, const void* A64, i
         f->graph_lda = lda;
         f->graph_pair_tiles = c->tune.lu_pair_tiles;
         f->graph_quad_tiles = c->tune.lu_quad_tiles;
+        f->graph_oct_tiles = c->tune.lu_oct_tiles;
         f->graph_split_tiles = c->tune.lu_split_tiles;
     }
     bem2d_internal_prof_begin(c, BEM2D_PROF_LU);
@@ -2102,6 +2107,7 @@ extern "C" int bem2d_lu32_factor(bem2d_ctx* c, bem2d_lu32* f, const void* A64, i
     if (graphs && f->graph && f->graph_A == A64 && f->graph_lda == lda &&
         f->graph_pair_tiles == c->tune.lu_pair_tiles &&
         f->graph_quad_tiles == c->tune.lu_quad_tiles &&
+        f->graph_oct_tiles == c->tune.lu_oct_tiles &&
         f->graph_split_tiles == c->tune.lu_split_tiles) {
         if (hipGraphLaunch(f->graph, s) != hipSuccess) rc = FAIL(BEM2D_ERR_DEVICE, "LU graph launch failed");
         // the host-side state the launch sequence leaves behind

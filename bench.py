@@ -46,11 +46,11 @@ def progress(msg):
     print("bench: " + msg, file=sys.stderr, flush=True)
 
 
-def lu_update_flops(n, pair_tiles, quad_tiles=1 << 30):
+def lu_update_flops(n, pair_tiles, quad_tiles=1 << 30, oct_tiles=1 << 30):
     """Algorithmic FLOP of the FP32 LU's trailing-update launches for an n x n matrix: 8 real FLOP
     per complex multiply-add, in the launch schedule of bem2d_internal_lu_advance (64-wide panels on
-    the matrix bordered to whole 64-blocks; while the trailing matrix has >= quad_tiles tiles four
-    panels, else while it has >= pair_tiles two, share one trailing update of depth 256 or 128,
+    the matrix bordered to whole 64-blocks; while the trailing matrix has >= oct_tiles tiles eight
+    panels, >= quad_tiles four, >= pair_tiles two share one trailing update of depth 512, 256, 128,
     each step of the group but the last updating only the cross -- the next block row and column --
     with the group's panels so far).  The total does not depend on the grouping.  Returns (FLOP,
     launches)."""
@@ -60,7 +60,9 @@ def lu_update_flops(n, pair_tiles, quad_tiles=1 << 30):
         tiles = nblk - b - 1
         if pend == 0:
             group = 1
-            if b + 4 <= nblk - 1 and tiles >= quad_tiles:
+            if b + 8 <= nblk - 1 and tiles >= oct_tiles:
+                group = 8
+            elif b + 4 <= nblk - 1 and tiles >= quad_tiles:
                 group = 4
             elif b + 2 <= nblk - 1 and tiles >= pair_tiles:
                 group = 2
@@ -340,7 +342,8 @@ def run_large(ctx, steps, warmup, n=32000, k=500.0, nrhs=360, plain_steps=3, cpu
 
     pair_tiles = ctx.tune_lu_pair_tiles()
     quad_tiles = ctx.tune_lu_quad_tiles()
-    upd_flop, upd_launches = lu_update_flops(n, pair_tiles, quad_tiles)
+    oct_tiles = ctx.tune_lu_oct_tiles()
+    upd_flop, upd_launches = lu_update_flops(n, pair_tiles, quad_tiles, oct_tiles)
     flu = 8.0 / 3.0 * n ** 3
     reg_avg = reg_ms / max(reg_n, 1)
     up_avg = up_ms / max(up_n, 1)
@@ -376,7 +379,7 @@ def run_large(ctx, steps, warmup, n=32000, k=500.0, nrhs=360, plain_steps=3, cpu
         "lu_fp64_ms": lu64_ms / max(lu64_n, 1),
         "lu_fp64_tflops": flu / (lu64_ms / max(lu64_n, 1) * 1e-3) / 1e12,
         "roofline": {
-            "kernel": "trailing_update_kernel (FP32 LU, complex rank-64..256 update on "
+            "kernel": "trailing_update_kernel (FP32 LU, complex rank-64..512 update on "
                       "v_mfma_f32_16x16x4f32, three real products per complex product)",
             "bound": "mfma",
             "achieved": achieved_upd,
@@ -386,9 +389,9 @@ def run_large(ctx, steps, warmup, n=32000, k=500.0, nrhs=360, plain_steps=3, cpu
             "traffic": profile_value(LU_TRAFFIC_PROFILE, "traffic_bytes_per_launch"),
             "work_per_launch": "%.6e FLOP per factorisation over %d launches = %.6e per launch "
                                "(8 real FLOP per complex multiply-add; panels in twos from %d "
-                               "tiles, in fours from %d)" % (upd_flop, upd_launches,
-                                                             upd_flop / upd_launches, pair_tiles,
-                                                             quad_tiles),
+                               "tiles, in fours from %d, in eights from %d)" % (
+                                   upd_flop, upd_launches, upd_flop / upd_launches, pair_tiles,
+                                   quad_tiles, oct_tiles),
             "executed_frac": None if achieved_upd is None else
             0.75 * achieved_upd / PEAK_FP32_MFMA_TFLOPS,
             "executed_note": "the kernel forms each complex product from three real MFMA products "

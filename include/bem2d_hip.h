@@ -114,12 +114,13 @@ int bem2d_copy_to_host(bem2d_ctx* ctx, void* dst, const void* src, int64_t bytes
  * most 8 buckets, upper_tiles increasing.  bem2d_tune_get_assembly returns the bucket count and
  * fills the arrays (8 entries each) when given.
  * FP32 LU: panels pair into depth-128 trailing updates while the trailing matrix has at least
- * lu_pair_tiles 64 x 64 tiles, and group in fours into depth-256 updates while it has at least
- * lu_quad_tiles (the larger choice wins where both apply); trailing matrices of at least lu_split_tiles tiles per side are
+ * lu_pair_tiles 64 x 64 tiles, group in fours into depth-256 updates while it has at least
+ * lu_quad_tiles and in eights into depth-512 updates from lu_oct_tiles (the largest group whose
+ * threshold is met wins); trailing matrices of at least lu_split_tiles tiles per side are
  * updated without the fused diagonal block (two workgroups per compute unit), the block then being
  * factorised by a kernel of its own.
- * Table lines: "asm_split <upper_tiles> <split>", "lu_pair_tiles <tiles>", "lu_quad_tiles <tiles>"
- * and "lu_split_tiles <tiles>"; '#' comments. */
+ * Table lines: "asm_split <upper_tiles> <split>", "lu_pair_tiles <tiles>", "lu_quad_tiles <tiles>",
+ * "lu_oct_tiles <tiles>" and "lu_split_tiles <tiles>"; '#' comments. */
 int bem2d_tune_set_assembly(bem2d_ctx* ctx, int nbuckets, const int64_t* upper_tiles,
                             const int* split);
 int bem2d_tune_get_assembly(const bem2d_ctx* ctx, int64_t* upper_tiles, int* split);
@@ -127,6 +128,8 @@ int bem2d_tune_set_lu_pair_tiles(bem2d_ctx* ctx, int tiles);
 int bem2d_tune_get_lu_pair_tiles(const bem2d_ctx* ctx);
 int bem2d_tune_set_lu_quad_tiles(bem2d_ctx* ctx, int tiles);
 int bem2d_tune_get_lu_quad_tiles(const bem2d_ctx* ctx);
+int bem2d_tune_set_lu_oct_tiles(bem2d_ctx* ctx, int tiles);
+int bem2d_tune_get_lu_oct_tiles(const bem2d_ctx* ctx);
 int bem2d_tune_set_lu_split_tiles(bem2d_ctx* ctx, int tiles);
 int bem2d_tune_get_lu_split_tiles(const bem2d_ctx* ctx);
 int bem2d_tune_load(bem2d_ctx* ctx, const char* path);

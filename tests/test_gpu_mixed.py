@@ -140,14 +140,17 @@ def test_many_right_hand_sides(ctx, nrhs):
 
 
 def test_paired_panels_large_trailing_matrix(ctx):
-    """n = 5440 (85 tiles) with panels in fours while the trailing matrix has >= 60 tiles and in
-    twos while it has >= 40: the first panels group in fours (one depth-256 update per four
-    panels), the next ones in twos, the last ones not at all -- against the FP64 solve."""
+    """n = 5440 (85 tiles) with panels in eights while the trailing matrix has >= 75 tiles, in
+    fours while it has >= 60 and in twos while it has >= 40: the first panels group in eights (one
+    depth-512 update per eight panels), the next ones in fours, then twos, the last ones not at
+    all -- against the FP64 solve."""
     n, k = 5440, 60.0
     print("case grouped panels n=%d" % n, flush=True)
     saved, saved_q = ctx.tune_lu_pair_tiles(), ctx.tune_lu_quad_tiles()
+    saved_o = ctx.tune_lu_oct_tiles()
     ctx.tune_lu_pair_tiles(40)
     ctx.tune_lu_quad_tiles(60)
+    ctx.tune_lu_oct_tiles(75)
     try:
         geom = G.kite(n)
         mesh = ctx.mesh(geom, 16, 3, 0.15)
@@ -160,6 +163,7 @@ def test_paired_panels_large_trailing_matrix(ctx):
     finally:
         ctx.tune_lu_pair_tiles(saved)
         ctx.tune_lu_quad_tiles(saved_q)
+        ctx.tune_lu_oct_tiles(saved_o)
     assert rep.fallback_fp64 == 0 and rep.backward_error <= 1e-12
     assert (torch.linalg.norm(Xm - X64) / torch.linalg.norm(X64)).item() < 1e-10
 
@@ -169,17 +173,20 @@ def test_factor_graph_replays_match_eager(ctx, n):
     """Repeated factorisations of one buffer replay a captured hipGraph (from the second call on);
     every replay, with new matrix contents and eager work and synchronisations in between, gives
     the same factors as an eager factorisation of the same matrix on a fresh buffer (compared
-    through one correction x = U^-1 L^-1 b).  n = 5440 (85 tiles), with panels in fours from 60
-    tiles and in twos from 40, covers both groupings in the graph."""
+    through one correction x = U^-1 L^-1 b).  n = 5440 (85 tiles), with panels in eights from 75
+    tiles, fours from 60 and twos from 40, covers every grouping in the graph."""
     print("case factor graph n=%d" % n, flush=True)
     saved, saved_q = ctx.tune_lu_pair_tiles(), ctx.tune_lu_quad_tiles()
+    saved_o = ctx.tune_lu_oct_tiles()
     ctx.tune_lu_pair_tiles(40)
     ctx.tune_lu_quad_tiles(60)
+    ctx.tune_lu_oct_tiles(75)
     try:
         _factor_graph_replays(ctx, n)
     finally:
         ctx.tune_lu_pair_tiles(saved)
         ctx.tune_lu_quad_tiles(saved_q)
+        ctx.tune_lu_oct_tiles(saved_o)
 
 
 def _factor_graph_replays(ctx, n):

@@ -40,29 +40,33 @@ def test_lu_pairing_choices_solve(ctx):
     b = rng.standard_normal((n, 1)) + 1j * rng.standard_normal((n, 1))
     x_ref = np.linalg.solve(A, b)
     At = torch.tensor(np.ascontiguousarray(A.T), device="cuda").t()
-    saved, saved_q = ctx.tune_lu_pair_tiles(), ctx.tune_lu_quad_tiles()
+    saved = ctx.tune_lu_pair_tiles(), ctx.tune_lu_quad_tiles(), ctx.tune_lu_oct_tiles()
     never = 1 << 30
     try:
-        # (pair from, fours from): pairs whenever possible, from 10 tiles, never; fours whenever
-        # possible; fours from 8 tiles with pairs below
-        for pair, quad in ((1, never), (10, never), (never, never), (never, 1), (1, 8)):
+        # (twos from, fours from, eights from): pairs whenever possible, from 10 tiles, never; fours
+        # whenever possible; fours from 8 tiles with pairs below; eights whenever possible; eights
+        # from 12 tiles, fours from 8, pairs below
+        for pair, quad, oct_ in ((1, never, never), (10, never, never), (never, never, never),
+                                 (never, 1, never), (1, 8, never), (never, never, 1), (1, 8, 12)):
             ctx.tune_lu_pair_tiles(pair)
             ctx.tune_lu_quad_tiles(quad)
+            ctx.tune_lu_oct_tiles(oct_)
             B = torch.tensor(np.ascontiguousarray(b.T), device="cuda").t().clone()
             rep = ctx.solve_mixed(At, B, tol=1e-12)
             err = rel_fro(B.cpu().numpy(), x_ref)
-            print("pair tiles", pair, "quad tiles", quad, "backward %.2e" % rep.backward_error,
-                  "err %.2e" % err, flush=True)
+            print("pair tiles", pair, "quad tiles", quad, "oct tiles", oct_,
+                  "backward %.2e" % rep.backward_error, "err %.2e" % err, flush=True)
             assert rep.fallback_fp64 == 0 and rep.backward_error <= 1e-12 and err < 1e-10
     finally:
-        ctx.tune_lu_pair_tiles(saved)
-        ctx.tune_lu_quad_tiles(saved_q)
+        ctx.tune_lu_pair_tiles(saved[0])
+        ctx.tune_lu_quad_tiles(saved[1])
+        ctx.tune_lu_oct_tiles(saved[2])
 
 
 def test_lu_split_diagonal_matches_fused(ctx):
     """The FP32 LU with the diagonal block factorised apart from the update (two workgroups per
     compute unit) gives the fused path's solution bit for bit (the same products, summed in the
-    same order), with panels single, in twos and in fours, and on a replayed graph."""
+    same order), with panels single, in twos, fours and eights, and on a replayed graph."""
     n = 1500  # 24 block columns: split on every step with lu_split_tiles = 1
     rng = np.random.default_rng(11)
     A = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) / np.sqrt(n)
@@ -71,11 +75,13 @@ def test_lu_split_diagonal_matches_fused(ctx):
     x_ref = np.linalg.solve(A, b)
     At = torch.tensor(np.ascontiguousarray(A.T), device="cuda").t()
     saved_s, saved_p = ctx.tune_lu_split_tiles(), ctx.tune_lu_pair_tiles()
-    saved_q = ctx.tune_lu_quad_tiles()
+    saved_q, saved_o = ctx.tune_lu_quad_tiles(), ctx.tune_lu_oct_tiles()
     try:
-        for pair, quad in ((1 << 30, 1 << 30), (4, 1 << 30), (4, 8)):
+        for pair, quad, oct_ in ((1 << 30, 1 << 30, 1 << 30), (4, 1 << 30, 1 << 30),
+                                 (4, 8, 1 << 30), (4, 8, 12)):
             ctx.tune_lu_pair_tiles(pair)
             ctx.tune_lu_quad_tiles(quad)
+            ctx.tune_lu_oct_tiles(oct_)
             facs = []
             for split in (1 << 30, 1):
                 ctx.tune_lu_split_tiles(split)
@@ -87,28 +93,30 @@ def test_lu_split_diagonal_matches_fused(ctx):
                 rep = lu.refine(At, B, X, tol=1e-12)
                 assert rep.converged and rel_fro(X.cpu().numpy(), x_ref) < 1e-10
                 facs.append(X.cpu().numpy())
-                print("pair", pair, "quad", quad, "split", split, "replays", lu.graph_replays(),
-                      flush=True)
+                print("pair", pair, "quad", quad, "oct", oct_, "split", split, "replays",
+                      lu.graph_replays(), flush=True)
                 lu.close()
             assert np.array_equal(facs[0], facs[1])
     finally:
         ctx.tune_lu_split_tiles(saved_s)
         ctx.tune_lu_pair_tiles(saved_p)
         ctx.tune_lu_quad_tiles(saved_q)
+        ctx.tune_lu_oct_tiles(saved_o)
 
 
 def test_table_load(ctx, tmp_path):
     saved_a, saved_p = ctx.tune_assembly(), ctx.tune_lu_pair_tiles()
-    saved_q = ctx.tune_lu_quad_tiles()
+    saved_q, saved_o = ctx.tune_lu_quad_tiles(), ctx.tune_lu_oct_tiles()
     try:
         good = tmp_path / "good.tune"
         good.write_text("# measured\nasm_split 500 4\nasm_split 9000 2  # comment\n"
-                        "lu_pair_tiles 77\nlu_quad_tiles 150\n")
+                        "lu_pair_tiles 77\nlu_quad_tiles 150\nlu_oct_tiles 260\n")
         ctx.tune_load(str(good))
         assert ctx.tune_assembly() == [(500, 4), (9000, 2)]
         assert ctx.tune_lu_pair_tiles() == 77 and ctx.tune_lu_quad_tiles() == 150
+        assert ctx.tune_lu_oct_tiles() == 260
         for text in ("asm_split 500 3\n", "asm_split 900 4\nasm_split 500 2\n", "bogus 1\n",
-                     "lu_pair_tiles 0\n", "lu_quad_tiles 0\n"):
+                     "lu_pair_tiles 0\n", "lu_quad_tiles 0\n", "lu_oct_tiles 0\n"):
             bad = tmp_path / "bad.tune"
             bad.write_text(text)
             with pytest.raises(capi.Bem2dError) as e:
@@ -120,3 +128,4 @@ def test_table_load(ctx, tmp_path):
         ctx.tune_assembly(saved_a)
         ctx.tune_lu_pair_tiles(saved_p)
         ctx.tune_lu_quad_tiles(saved_q)
+        ctx.tune_lu_oct_tiles(saved_o)
