@@ -113,12 +113,14 @@ def test_zero_pivot_falls_back_to_fp64(ctx):
     assert _backward(A, Bd.cpu().numpy(), B) <= 1e-14
 
 
-@pytest.mark.parametrize("nrhs", [64, 360])
-def test_many_right_hand_sides(ctx, nrhs):
-    """The library-GEMM refinement path (ctrsm on the FP32 factors, zgemm residual) against the
-    FP64 solve, with the far field checked at 1e-10 as on the large configuration."""
-    print("case many rhs", nrhs, flush=True)
-    n, k = 700, 40.0
+@pytest.mark.parametrize("n,nrhs", [(700, 64), (700, 360), (1024, 64)])
+def test_many_right_hand_sides(ctx, n, nrhs):
+    """The library-GEMM refinement path (triangular solves by 512-wide super-blocks on the FP32
+    factors -- one full super-block and a partial one at n = 700, two full ones at n = 1024 --
+    zgemm residual) against the FP64 solve, with the far field checked at 1e-10 as on the large
+    configuration."""
+    print("case many rhs n=%d nrhs=%d" % (n, nrhs), flush=True)
+    k = 40.0
     geom = G.open_arc(n)
     mesh = ctx.mesh(geom, 16, 3, 0.15)
     A = ctx.assemble(mesh, capi.KERNEL_COMBINED, k, k, 1.0, 2.0)
@@ -137,6 +139,29 @@ def test_many_right_hand_sides(ctx, nrhs):
     f64 = ctx.farfield(mesh, k, X64, ang)
     assert (torch.linalg.norm(f32 - f64) / torch.linalg.norm(f64)).item() < 1e-10
     mesh.close()
+
+
+def test_many_right_hand_sides_after_refactorisation(ctx):
+    """The super-block inverses of the many-right-hand-side solve belong to one factorisation: a
+    second factorisation of another matrix in the same factor object, then a solve, must use the
+    new factors (and a second solve with the same factors reuses the inverses)."""
+    n, nrhs = 700, 64
+    print("case many rhs refactorisation n=%d" % n, flush=True)
+    rng = np.random.default_rng(21)
+    lu = ctx.lu32(n)
+    for t in range(2):
+        A = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) / np.sqrt(n)
+        A += (3 + t) * np.eye(n)
+        B = rng.standard_normal((n, nrhs)) + 1j * rng.standard_normal((n, nrhs))
+        Ad = torch.tensor(np.ascontiguousarray(A.T), device="cuda").t()
+        Bd = torch.tensor(np.ascontiguousarray(B.T), device="cuda").t()
+        lu.factor(Ad)
+        for _ in range(2):
+            X = torch.empty_like(Bd)
+            rep = lu.refine(Ad, Bd, X, tol=1e-12)
+            assert rep.converged == 1 and rep.backward_error <= 1e-12
+            assert rel_fro(X.cpu().numpy(), np.linalg.solve(A, B)) < 1e-10
+    lu.close()
 
 
 def test_paired_panels_large_trailing_matrix(ctx):
