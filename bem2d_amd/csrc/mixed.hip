@@ -1752,9 +1752,6 @@ struct bem2d_lu32 {
     unsigned long long* claim = nullptr; // n claims of the rows
     unsigned long long* gmax = nullptr;  // max |U|, max |L|, max |a_ij|
     RefCache* rcache = nullptr;   // refinement workspace of bem2d_lu32_refine
-    int64_t gen = 0;              // factorisations begun (the super-block inverses follow it)
-    float2* sbinv = nullptr;      // inverses of the 512-wide diagonal super-blocks: L's, then U's
-    int64_t sbinv_gen = -1;       // the factorisation they belong to
     // hipGraph of the whole factorisation launch sequence for one matrix buffer (A, lda): built on
     // the second factorisation of the same buffer, replayed from then on
     const void* seen_A = nullptr;
@@ -1798,7 +1795,6 @@ extern "C" void bem2d_lu32_destroy(bem2d_lu32* f)
         (void)hipHostFree(f->rcache->pinned);
         delete f->rcache;
     }
-    (void)hipFree(f->sbinv);
     delete f;
 }
 
@@ -1900,7 +1896,6 @@ int bem2d_internal_lu_begin(hipStream_t s, bem2d_lu32* const* lus, const void* c
             p.dflag = f->dflag;
             f->step = 0;
             f->pend = 0;
-            f->gen++;
             f->factored = false;
         }
         // workgroups per problem of each kernel -> start[]
@@ -2118,7 +2113,6 @@ extern "C" int bem2d_lu32_factor(bem2d_ctx* c, bem2d_lu32* f, const void* A64, i
         // the host-side state the launch sequence leaves behind
         f->step = f->nblk - 1;
         f->pend = 0;
-        f->gen++;
         f->factored = true;
         f->graph_replays++;
     } else {
@@ -2488,100 +2482,10 @@ int bem2d_internal_refine_sync(hipStream_t s, RefJob* j)
     return rc;
 }
 
-// Refinement of many right-hand sides at once (nrhs >= kWideRhs): the triangular solves run on
-// the FP32 factors by 512-wide super-blocks (wide_trsm) and the FP64 residual R = B - A X is one
-// zgemm, so A64 and the factors are read once per step for all columns instead of once per group
-// of kMaxSolveRhs.
+// Refinement of many right-hand sides at once (nrhs >= kWideRhs): the triangular solves are
+// rocBLAS ctrsm on the FP32 factors and the FP64 residual R = B - A X is one zgemm, so A64 and
+// the factors are read once per step for all columns instead of once per group of kMaxSolveRhs.
 constexpr int kWideRhs = 64;
-constexpr int kSuperBlock = 512;
-
-// Y <- U^{-1} L^{-1} Y (n x nrhs, ld n) with the FP32 factors, blocked by 512-wide super-blocks:
-// each super-block's rows are multiplied by the inverse of its diagonal triangle (rocBLAS ctrtri,
-// once per factorisation) and the rest of Y is updated by one cgemm of depth 512 -- 63 steps per
-// triangle at n = 32000 where rocBLAS ctrsm takes 250 of depth 128 (measured on MI355X with 360
-// right-hand sides: cgemm runs at 111 TFLOP/s at depth 512 against 94 at depth 128).  T is a
-// 512 x nrhs workspace.
-static int wide_trsm(bem2d_ctx* c, bem2d_lu32* f, float2* Y, int nrhs, float2* T)
-{
-    const int64_t n = f->n, np = f->np, bs = kSuperBlock;
-    const int64_t ns = (n + bs - 1) / bs, nfull = n / bs;
-    const rocblas_float_complex one = {1.f, 0.f}, zero = {0.f, 0.f}, mone = {-1.f, 0.f};
-    auto F = [&](int64_t r, int64_t col) { return (rocblas_float_complex*)(f->F + r + np * col); };
-    float2* invL = nullptr;
-    float2* invU = nullptr;
-    if (!f->sbinv) {
-        if (hipMalloc(&f->sbinv, sizeof(float2) * 2 * ns * bs * bs) != hipSuccess) {
-            f->sbinv = nullptr;
-            return FAIL(BEM2D_ERR_NOMEM, "super-block inverse allocation failed");
-        }
-        f->sbinv_gen = -1;
-    }
-    invL = f->sbinv;
-    invU = f->sbinv + ns * bs * bs;
-    rocblas_status st = rocblas_status_success;
-    if (f->sbinv_gen != f->gen) {
-        if (nfull > 0) {
-            st = rocblas_ctrtri_strided_batched(c->blas, rocblas_fill_lower, rocblas_diagonal_unit,
-                                                (rocblas_int)bs, F(0, 0), (rocblas_int)np,
-                                                (rocblas_stride)(bs * (np + 1)),
-                                                (rocblas_float_complex*)invL, (rocblas_int)bs,
-                                                (rocblas_stride)(bs * bs), (rocblas_int)nfull);
-            if (st == rocblas_status_success)
-                st = rocblas_ctrtri_strided_batched(
-                    c->blas, rocblas_fill_upper, rocblas_diagonal_non_unit, (rocblas_int)bs, F(0, 0),
-                    (rocblas_int)np, (rocblas_stride)(bs * (np + 1)), (rocblas_float_complex*)invU,
-                    (rocblas_int)bs, (rocblas_stride)(bs * bs), (rocblas_int)nfull);
-        }
-        if (st == rocblas_status_success && nfull < ns) {
-            const int64_t s0 = nfull * bs, b = n - s0;
-            st = rocblas_ctrtri(c->blas, rocblas_fill_lower, rocblas_diagonal_unit, (rocblas_int)b,
-                                F(s0, s0), (rocblas_int)np,
-                                (rocblas_float_complex*)(invL + nfull * bs * bs), (rocblas_int)bs);
-            if (st == rocblas_status_success)
-                st = rocblas_ctrtri(c->blas, rocblas_fill_upper, rocblas_diagonal_non_unit,
-                                    (rocblas_int)b, F(s0, s0), (rocblas_int)np,
-                                    (rocblas_float_complex*)(invU + nfull * bs * bs), (rocblas_int)bs);
-        }
-        if (st != rocblas_status_success)
-            return FAIL(BEM2D_ERR_DEVICE, "rocBLAS status %d in the super-block inverses", (int)st);
-        f->sbinv_gen = f->gen;
-    }
-    auto Yp = [&](int64_t r) { return (rocblas_float_complex*)(Y + r); };
-    // one super-block: Y_S <- inv_S Y_S through T
-    auto diag = [&](const float2* inv, int64_t s0, int64_t b) {
-        rocblas_status s = rocblas_cgemm(c->blas, rocblas_operation_none, rocblas_operation_none,
-                                         (rocblas_int)b, nrhs, (rocblas_int)b, &one,
-                                         (const rocblas_float_complex*)inv, (rocblas_int)bs, Yp(s0),
-                                         (rocblas_int)n, &zero, (rocblas_float_complex*)T,
-                                         (rocblas_int)bs);
-        if (s == rocblas_status_success &&
-            hipMemcpy2DAsync(Y + s0, sizeof(float2) * n, T, sizeof(float2) * bs,
-                             sizeof(float2) * b, (size_t)nrhs, hipMemcpyDeviceToDevice,
-                             c->stream) != hipSuccess)
-            s = rocblas_status_internal_error;
-        return s;
-    };
-    for (int64_t S = 0; S < ns && st == rocblas_status_success; ++S) { // L y = r, top down
-        const int64_t s0 = S * bs, b = std::min(bs, n - s0), rest = n - s0 - b;
-        st = diag(invL + S * bs * bs, s0, b);
-        if (st == rocblas_status_success && rest > 0)
-            st = rocblas_cgemm(c->blas, rocblas_operation_none, rocblas_operation_none,
-                               (rocblas_int)rest, nrhs, (rocblas_int)b, &mone, F(s0 + b, s0),
-                               (rocblas_int)np, Yp(s0), (rocblas_int)n, &one, Yp(s0 + b),
-                               (rocblas_int)n);
-    }
-    for (int64_t S = ns - 1; S >= 0 && st == rocblas_status_success; --S) { // U x = y, bottom up
-        const int64_t s0 = S * bs, b = std::min(bs, n - s0);
-        st = diag(invU + S * bs * bs, s0, b);
-        if (st == rocblas_status_success && s0 > 0)
-            st = rocblas_cgemm(c->blas, rocblas_operation_none, rocblas_operation_none,
-                               (rocblas_int)s0, nrhs, (rocblas_int)b, &mone, F(0, s0),
-                               (rocblas_int)np, Yp(s0), (rocblas_int)n, &one, Yp(0), (rocblas_int)n);
-    }
-    if (st != rocblas_status_success)
-        return FAIL(BEM2D_ERR_DEVICE, "rocBLAS status %d in the super-block solve", (int)st);
-    return BEM2D_OK;
-}
 
 static int refine_wide(bem2d_ctx* c, bem2d_lu32* f, const double2* A, int64_t lda,
                        const double2* B, int64_t ldb, double2* X, int64_t ldx, int nrhs,
@@ -2591,20 +2495,17 @@ static int refine_wide(bem2d_ctx* c, bem2d_lu32* f, const double2* A, int64_t ld
     hipStream_t s = c->stream;
     double2* R = nullptr;
     float2* Y = nullptr;
-    float2* T = nullptr;
     double* scale = nullptr;
     unsigned long long* nm = nullptr; // rmax | xmax | bmax
     auto cleanup = [&]() {
         (void)hipStreamSynchronize(s);
         (void)hipFree(R);
         (void)hipFree(Y);
-        (void)hipFree(T);
         (void)hipFree(scale);
         (void)hipFree(nm);
     };
     if (hipMalloc(&R, sizeof(double2) * n * nrhs) != hipSuccess ||
         hipMalloc(&Y, sizeof(float2) * n * nrhs) != hipSuccess ||
-        hipMalloc(&T, sizeof(float2) * kSuperBlock * nrhs) != hipSuccess ||
         hipMalloc(&scale, sizeof(double) * nrhs) != hipSuccess ||
         hipMalloc(&nm, sizeof(unsigned long long) * 3 * nrhs) != hipSuccess) {
         cleanup();
@@ -2645,19 +2546,30 @@ static int refine_wide(bem2d_ctx* c, bem2d_lu32* f, const double2* A, int64_t ld
     // rmax = bmax for the first scaling
     (void)hipMemcpyAsync(nm, nm + 2 * nrhs, sizeof(unsigned long long) * nrhs,
                          hipMemcpyDeviceToDevice, s);
+    const rocblas_float_complex one32 = {1.f, 0.f};
     const rocblas_double_complex mone = {-1.0, 0.0}, one = {1.0, 0.0};
     double omega = INFINITY, prev = INFINITY;
     int it = 0;
     int rc = BEM2D_OK;
     for (; it < maxit; ++it) {
         wide_demote_kernel<<<g, 256, 0, s>>>(n, nrhs, R, f->perm, nm, scale, Y);
-        if ((rc = wide_trsm(c, f, Y, nrhs, T))) break;
+        rocblas_status st = rocblas_ctrsm(c->blas, rocblas_side_left, rocblas_fill_lower,
+                                          rocblas_operation_none, rocblas_diagonal_unit,
+                                          (rocblas_int)n, (rocblas_int)nrhs, &one32,
+                                          (const rocblas_float_complex*)f->F, (rocblas_int)f->np,
+                                          (rocblas_float_complex*)Y, (rocblas_int)n);
+        if (st == rocblas_status_success)
+            st = rocblas_ctrsm(c->blas, rocblas_side_left, rocblas_fill_upper,
+                               rocblas_operation_none, rocblas_diagonal_non_unit, (rocblas_int)n,
+                               (rocblas_int)nrhs, &one32, (const rocblas_float_complex*)f->F,
+                               (rocblas_int)f->np, (rocblas_float_complex*)Y, (rocblas_int)n);
         wide_update_kernel<<<g, 256, 0, s>>>(n, nrhs, Y, scale, X, ldx, B, ldb, R);
-        const rocblas_status st =
-            rocblas_zgemm(c->blas, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n,
-                          (rocblas_int)nrhs, (rocblas_int)n, &mone, (const rocblas_double_complex*)A,
-                          (rocblas_int)lda, (const rocblas_double_complex*)X, (rocblas_int)ldx,
-                          &one, (rocblas_double_complex*)R, (rocblas_int)n);
+        if (st == rocblas_status_success)
+            st = rocblas_zgemm(c->blas, rocblas_operation_none, rocblas_operation_none,
+                               (rocblas_int)n, (rocblas_int)nrhs, (rocblas_int)n, &mone,
+                               (const rocblas_double_complex*)A, (rocblas_int)lda,
+                               (const rocblas_double_complex*)X, (rocblas_int)ldx, &one,
+                               (rocblas_double_complex*)R, (rocblas_int)n);
         if (st != rocblas_status_success) {
             rc = FAIL(BEM2D_ERR_DEVICE, "rocBLAS status %d in refinement", (int)st);
             break;

@@ -115,10 +115,8 @@ def test_zero_pivot_falls_back_to_fp64(ctx):
 
 @pytest.mark.parametrize("n,nrhs", [(700, 64), (700, 360), (1024, 64)])
 def test_many_right_hand_sides(ctx, n, nrhs):
-    """The library-GEMM refinement path (triangular solves by 512-wide super-blocks on the FP32
-    factors -- one full super-block and a partial one at n = 700, two full ones at n = 1024 --
-    zgemm residual) against the FP64 solve, with the far field checked at 1e-10 as on the large
-    configuration."""
+    """The library-GEMM refinement path (rocBLAS ctrsm on the FP32 factors, zgemm residual) against
+    the FP64 solve, with the far field checked at 1e-10 as on the large configuration."""
     print("case many rhs n=%d nrhs=%d" % (n, nrhs), flush=True)
     k = 40.0
     geom = G.open_arc(n)
@@ -142,9 +140,8 @@ def test_many_right_hand_sides(ctx, n, nrhs):
 
 
 def test_many_right_hand_sides_after_refactorisation(ctx):
-    """The super-block inverses of the many-right-hand-side solve belong to one factorisation: a
-    second factorisation of another matrix in the same factor object, then a solve, must use the
-    new factors (and a second solve with the same factors reuses the inverses)."""
+    """A second factorisation of another matrix in the same factor object, then a many-right-hand-
+    side solve, uses the new factors; repeated solves with the same factors agree."""
     n, nrhs = 700, 64
     print("case many rhs refactorisation n=%d" % n, flush=True)
     rng = np.random.default_rng(21)
