@@ -101,6 +101,37 @@ def test_each_result_same_alone_and_in_the_mix(ctx):
         assert rel_fro(fm, fa) <= 1e-14
 
 
+def test_grouped_panels_in_batched_launches(ctx):
+    """With panels grouped in twos from 2 trailing tiles, fours from 4 and eights from 6, the
+    problems of one batched LU launch sit at different points of their groups (different update
+    depths in one launch): results still match the single-problem path, which groups the same way,
+    and are the same alone and in the mix."""
+    print("case engine grouped panels", flush=True)
+    saved = ctx.tune_lu_pair_tiles(), ctx.tune_lu_quad_tiles(), ctx.tune_lu_oct_tiles()
+    ctx.tune_lu_pair_tiles(2)
+    ctx.tune_lu_quad_tiles(4)
+    ctx.tune_lu_oct_tiles(6)
+    try:
+        reqs = _mix()
+        eng = Engine(ctx, 4 * GB)
+        res = _run(eng, reqs)
+        alone = [_run(eng, [q])[0] for q in reqs[:2]]
+        st = eng.stats()
+        eng.close()
+        assert st["failed"] == 0, st
+        for q, (x, ff, info) in zip(reqs, res):
+            assert info.status == capi.OK and info.backward_error <= 1e-12
+            xs, ffs = _single(ctx, q)
+            assert rel_fro(x, xs) <= 1e-13, (q["geom"].n, rel_fro(x, xs))
+            assert rel_fro(ff, ffs) <= 1e-13
+        for (xa, fa, _), (xm, fm, _) in zip(alone, res[:2]):
+            assert rel_fro(xm, xa) <= 1e-14 and rel_fro(fm, fa) <= 1e-14
+    finally:
+        ctx.tune_lu_pair_tiles(saved[0])
+        ctx.tune_lu_quad_tiles(saved[1])
+        ctx.tune_lu_oct_tiles(saved[2])
+
+
 def test_circle_far_field_against_series(ctx):
     print("case circle far field", flush=True)
     q = _req(G.unit_circle(200), 10.0, 1, nang=360)
