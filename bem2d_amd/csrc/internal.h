@@ -39,10 +39,10 @@ struct TuneState {
     int64_t asm_upper[kMaxTuneBuckets] = {16384};
     int asm_split[kMaxTuneBuckets] = {4};
     // FP32 LU: panels pair into depth-128 updates while the trailing matrix has >= this many tiles
-    int lu_pair_tiles = 80;
+    int lu_pair_tiles = 64;
     // four panels into depth-256 updates while it has >= this many, eight into depth-512 from
-    int lu_quad_tiles = 200;
-    int lu_oct_tiles = 250;
+    int lu_quad_tiles = 96;
+    int lu_oct_tiles = 150;
     int lu_split_tiles = 64;   // trailing tiles from which the diagonal block is factorised apart
 };
 

@@ -45,10 +45,10 @@ constexpr long long kSpinLimit = 1ll << 26;
 // (depth 512) while the trailing matrix has at least tune.lu_pair_tiles / lu_quad_tiles /
 // lu_oct_tiles 64-wide tiles: the saved passes over it (16 B per entry each) and the longer runs of
 // products per output tile then outweigh the narrow cross updates and diagonal factorisations the
-// group leaves exposed (~60 us each).  Measured on MI355X: N = 4000 (at most 62 tiles) is fastest
-// ungrouped; N = 32000 pairs (1.94 -> 1.53 s, four-product updates), groups in fours from 200 tiles
-// (700 -> 632 ms, three-product updates) and in eights from 250 (635 -> 616 ms; sixteens from 350
-// tiles gained nothing more).
+// group leaves exposed (~60 us each).  Measured on MI355X with the three-product update: twos
+// from 64 tiles, fours from 96, eights from 150 -- N = 4000 3.8 ms (ungrouped: at most 62 tiles),
+// N = 8000 18.2 ms ungrouped -> 15.9 ms, N = 12000 45.7 -> 42.5 ms (against twos from 100, fours
+// from 200, eights from 250), N = 32000 700 ms with twos only -> 613 ms; sixteens gained nothing.
 
 typedef float v4f __attribute__((ext_vector_type(4)));
 
