@@ -1,22 +1,27 @@
 // Mixed-precision dense complex solve: FP32 LU on the MFMA cores + FP64 iterative refinement.
 //
-// Factorisation (bem2d_lu32_factor): A32 = (complex64) A64 bordered by the identity to whole
-// 64-blocks, then a blocked right-looking LU with 64-wide panels and no row interchanges.  Step b:
-// the block column below and the block row right of diagonal block b are multiplied by the
-// inverses of its triangles (panel_trsm_kernel, MFMA), and the trailing matrix receives the
-// rank-64 update A22 -= L21 U12 as a complex GEMM on v_mfma_f32_16x16x4f32 with FP32 accumulation
-// (three real MFMA products per complex product, Gauss's form).  The workgroup of that update
-// which owns diagonal block b + 1 goes on to factorise it -- LU plus the inverses of both
-// triangles, blocked in 16-column panels inside one workgroup (diag_factor) -- while the rest of
-// the update runs, so the panel factorisation overlaps the GEMM.  Without interchanges
-// the panel has no sequential pivot search, the step that makes partial-pivoting panels latency
-// bound on a GPU.  A zero or non-finite pivot is flagged; refinement then reports failure and the
-// one-shot solver falls back to the FP64 LU.
+// Factorisation (bem2d_lu32_factor): A32 = (complex64) A64 in a static pivoting order (rows
+// claimed by the columns where they are largest), bordered by the identity to whole 64-blocks,
+// then a blocked right-looking LU with 64-wide panels and no row interchanges.  Step b: the block
+// column below and the block row right of diagonal block b are multiplied by the inverses of its
+// triangles (panel_trsm_kernel, MFMA), and the trailing matrix receives the update
+// A22 -= L21 U12 as a complex GEMM on v_mfma_f32_16x16x4f32 with FP32 accumulation (three real
+// MFMA products per complex product, Gauss's form).  On large trailing matrices panels go in
+// groups of two, four or eight: each step of a group updates only the next panel's block row and
+// column, and the group's last step applies all its panels to the rest at depth 128 to 512.  The
+// diagonal block b + 1 is factorised -- LU plus the inverses of both triangles, blocked in
+// 16-column panels inside one workgroup (diag_factor) -- by workgroup 0 of the update while the
+// rest of the update runs (small trailing matrices), or by diag_lu_kernel after it (large ones,
+// whose update then fits two workgroups per compute unit).  Without interchanges the panel has no
+// sequential pivot search, the step that makes partial-pivoting panels latency bound on a GPU.  A
+// zero or non-finite pivot, or element growth above 1e4, is flagged; refinement then reports
+// failure and the one-shot solver falls back to the FP64 LU.
 //
 // Refinement (bem2d_lu32_refine): x0 = 0, r = b - A64 x in FP64, d = U^{-1} L^{-1} r in FP32
 // (two persistent block-triangular sweeps with the inverted diagonal blocks; blocks hand their
-// solution on through tagged words), x += d in FP64, until the backward error
-// |b - A x|_inf / (|A|_inf |x|_inf + |b|_inf) <= tol for every right-hand side.
+// solution on through tagged words; rocBLAS ctrsm for 64 right-hand sides and more), x += d in
+// FP64, until the backward error |b - A x|_inf / (|A|_inf |x|_inf + |b|_inf) <= tol for every
+// right-hand side; GMRES-IR preconditioned by the FP32 factors takes over a stalled refinement.
 //
 // Every kernel takes a by-value batch of up to kMaxLuBatch problems, so that several problems of
 // different sizes (each at its own panel step, or its own refinement step) advance in one launch
