@@ -33,6 +33,23 @@ def ctx():
     c.close()
 
 
+@pytest.fixture
+def progress(request):
+    """A print that reaches the terminal while pytest captures the test's output: the tests that
+    spend minutes on the host (the complete oracle matrix of a large case) report each stage, so a
+    run is never silent for long."""
+    capman = request.config.pluginmanager.getplugin("capturemanager")
+
+    def say(msg):
+        if capman is None:
+            print(msg, flush=True)
+            return
+        with capman.global_and_fixture_disabled():
+            print(msg, flush=True)
+
+    return say
+
+
 def golden(name):
     return np.load(os.path.join(GOLDEN, name))
 

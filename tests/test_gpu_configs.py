@@ -48,16 +48,16 @@ def _sampled_columns(ctx, oracle, geom, k, A, cols, label):
     assert err <= max(TOL, 2 * floor), (label, err, floor)
 
 
-def _oracle_blocks(oracle, geom, k, variant):
-    """The complete oracle matrix, built in column blocks (progress printed per block)."""
+def _oracle_blocks(oracle, geom, k, variant, say):
+    """The complete oracle matrix, built in column blocks (progress reported per block)."""
     n = geom.n
     out = np.empty((n, n), dtype=np.complex128, order="F")
-    step = 500
+    step = 250
     for c0 in range(0, n, step):
         cols = np.arange(c0, min(n, c0 + step))
         out[:, cols] = oracle.assemble(geom, 16, 3, 0.15, oracle.K_COMBINED, k, k, 1.0, 2.0,
                                        cols=cols, variant=variant)
-        print("  oracle (%s) columns %d..%d" % (variant, c0, cols[-1]), flush=True)
+        say("  oracle (%s) columns %d..%d" % (variant, c0, cols[-1]))
     return out
 
 
@@ -86,13 +86,13 @@ def _c2_gpu(ctx):
     return out
 
 
-def test_c2_full_matrix_plain_oracle(ctx, oracle):
+def test_c2_full_matrix_plain_oracle(ctx, oracle, progress):
     """C2, first half: the GPU matrix and both solutions against the complete oracle matrix and
     the LAPACK zgesv solution of the oracle's system (the reference's SolveSystem)."""
     import scipy.linalg.lapack as lapack
     print("case C2 kite N=4000 k=100: full oracle matrix", flush=True)
     g = _c2_gpu(ctx)
-    Ao = _oracle_blocks(oracle, g["geom"], g["k"], "plain")
+    Ao = _oracle_blocks(oracle, g["geom"], g["k"], "plain", progress)
     bo = oracle.rhs(g["geom"], 16, oracle.F_COMBINED_PLANE, g["k"], g["k"], g["dirs"], 2.0)
     _, _, xo, info = lapack.zgesv(Ao.copy(order="F"), bo.copy())
     assert info == 0
@@ -108,7 +108,7 @@ def test_c2_full_matrix_plain_oracle(ctx, oracle):
     assert g["rep"].backward_error <= 1e-12
 
 
-def test_c2_against_noise_floor(ctx, oracle):
+def test_c2_against_noise_floor(ctx, oracle, progress):
     """C2, second half: the oracle with fused multiply-adds, whose distance from the plain oracle
     is how far rounding alone moves the reference's own matrix and solution; the GPU results
     within max(1e-12, 2 x floor), and the mixed far field within 1e-10 of the oracle's."""
@@ -117,7 +117,7 @@ def test_c2_against_noise_floor(ctx, oracle):
         pytest.skip("needs test_c2_full_matrix_plain_oracle in the same session")
     g = _C2
     print("case C2 noise floor: oracle with FMA", flush=True)
-    Af = _oracle_blocks(oracle, g["geom"], g["k"], "fma")
+    Af = _oracle_blocks(oracle, g["geom"], g["k"], "fma", progress)
     _, _, xf, info = lapack.zgesv(Af.copy(order="F"), g["bo"].copy())
     assert info == 0
     floor_a = rel_fro(Af, g["Ao"])
